@@ -1,0 +1,281 @@
+"""bs2e -- host-side mirror of the reference's two-electron R^k interfaces,
+running on the MI355X HIP library (libbs2e.so) through its C ABI.
+
+Names, argument meaning and error behaviour follow edvinolo/b-spline-two-e:
+
+    generate_grid(k, m, Z, h_max, r_max)                 grid_tools.f90:6-55
+    b_spline(k, t)                                       bspline_tools.f90:4-56
+    sparse_4d / sparse_6d  .init(max_k, b_splines)       sparse_array_tools.f90:9-31, 159-178
+    setup_Slater_integrals(b_splines, max_k, k_GL,
+                           r_k, r_m_k, r_d_k)            mat_els.f90:172-182
+    Nd_DOK, compute_R_K_map(r_d_k, r_k, r_m_k,
+                            b_splines, max_k, R)         sparse_array_tools.f90:452-555
+    ang_k_LS(k, l1, l2, L)                               wigner_tools.f90:126-138
+
+Everything numeric runs on the GPU; there is no CPU fallback.  A missing
+library or device raises Bs2eError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import capi
+from .capi import Bs2eError, check, ptr
+
+__all__ = ["Bs2eError", "Context", "generate_grid", "b_spline", "sparse_4d", "sparse_6d", "setup_Slater_integrals",
+           "Nd_DOK", "RkTensor", "compute_R_K_map", "ang_k_LS", "three_j", "six_j", "count_nnz"]
+
+
+class Context:
+    """One device and one stream (bs2e_create)."""
+
+    def __init__(self, device: int = 0):
+        L = capi.lib()
+        h = C.c_void_p()
+        check(L.bs2e_create(C.byref(h), device))
+        self.handle = h
+        self.device = device
+
+    def set_stream(self, stream_ptr: int | None):
+        check(capi.lib().bs2e_set_stream(self.handle, C.c_void_p(stream_ptr) if stream_ptr else None), self.handle)
+
+    def close(self):
+        if self.handle:
+            capi.lib().bs2e_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def generate_grid(k, m, Z, h_max, r_max) -> np.ndarray:
+    """grid_tools.f90:6-55: the knot vector (divided by Z)."""
+    L = capi.lib()
+    n = check(L.bs2e_generate_grid(k, m, Z, float(h_max), float(r_max), None, 0))
+    out = np.zeros(n)
+    check(L.bs2e_generate_grid(k, m, Z, float(h_max), float(r_max), ptr(out), n))
+    return out
+
+
+def count_nnz(k, t):
+    """(nnz4, nnz6, n_pairs): count_nnz_4d / count_nnz_6d / sqrt(count_nnz_R_k)."""
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+    check(capi.lib().bs2e_count_nnz(k, len(t), ptr(t), C.byref(a), C.byref(b), C.byref(c)))
+    return a.value, b.value, c.value
+
+
+class b_spline:
+    """bspline_tools.f90:4-56: order k, knots t; n = size(t)-k, n_b = n-2,
+    breakpoints = t(k:size(t)-k+1), support(iv, s) <=> iv <= s <= iv+k-1."""
+
+    def __init__(self, k, t):
+        self.k = int(k)
+        self.knots = np.ascontiguousarray(t, dtype=np.float64)
+        self.n = len(self.knots) - self.k
+        self.n_b = self.n - 2
+        self.breakpoints = self.knots[self.k - 1:len(self.knots) - self.k + 1].copy()
+        niv = len(self.breakpoints) - 1
+        iv = np.arange(1, niv + 1)[:, None]
+        s = np.arange(1, self.n + 1)[None, :]
+        self.support = (iv <= s) & (s <= iv + self.k - 1)   # (n_iv, n), 1-based semantics
+
+    init = __init__
+
+
+@dataclass
+class sparse_4d:
+    """sparse_array_tools.f90:9-18 (data(nnz, 0:max_k) column-major)."""
+    nnz: int = 0
+    data: np.ndarray = field(default_factory=lambda: np.zeros((0, 0), order="F"))
+    iv: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
+    i: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
+    j: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
+
+    def init(self, max_k, b_splines):
+        """init_4d (:159-167): count_nnz_4d, zeroed data and index arrays."""
+        self.nnz = count_nnz(b_splines.k, b_splines.knots)[0]
+        self.data = np.zeros((self.nnz, max_k + 1), order="F")
+        self.iv = np.zeros(self.nnz, np.int64)
+        self.i = np.zeros(self.nnz, np.int64)
+        self.j = np.zeros(self.nnz, np.int64)
+        return self
+
+
+@dataclass
+class sparse_6d:
+    """sparse_array_tools.f90:20-31."""
+    nnz: int = 0
+    data: np.ndarray = field(default_factory=lambda: np.zeros((0, 0), order="F"))
+    iv: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
+    i: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
+    j: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
+    i_p: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
+    j_p: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
+
+    def init(self, max_k, b_splines):
+        """init_6d (:169-178)."""
+        self.nnz = count_nnz(b_splines.k, b_splines.knots)[1]
+        self.data = np.zeros((self.nnz, max_k + 1), order="F")
+        for name in ("iv", "i", "j", "i_p", "j_p"):
+            setattr(self, name, np.zeros(self.nnz, np.int64))
+        return self
+
+
+def setup_Slater_integrals(b_splines, max_k, k_GL, r_k, r_m_k, r_d_k, ctx: Context | None = None):
+    """mat_els.f90:172-182 on the GPU: fills r_k, r_m_k (sparse_4d) and r_d_k
+    (sparse_6d), which the caller has init'ed, in the reference's order."""
+    ctx = ctx or default_context()
+    t = b_splines.knots
+    for c in (r_k, r_m_k, r_d_k):
+        if not (c.data.flags.f_contiguous and c.data.shape[1] == max_k + 1):
+            raise Bs2eError(capi.BS2E_EINVAL, "containers must be init'ed with the same max_k")
+    check(capi.lib().bs2e_slater_primitives(
+        ctx.handle, b_splines.k, len(t), ptr(t), k_GL, max_k,
+        r_k.nnz, ptr(r_k.data), ptr(r_m_k.data), ptr(r_k.iv, C.c_int64), ptr(r_k.i, C.c_int64),
+        ptr(r_k.j, C.c_int64),
+        r_d_k.nnz, ptr(r_d_k.data), ptr(r_d_k.iv, C.c_int64), ptr(r_d_k.i, C.c_int64), ptr(r_d_k.i_p, C.c_int64),
+        ptr(r_d_k.j, C.c_int64), ptr(r_d_k.j_p, C.c_int64)), ctx.handle)
+    r_m_k.iv[:] = r_k.iv
+    r_m_k.i[:] = r_k.i
+    r_m_k.j[:] = r_k.j
+
+
+class RkTensor:
+    """Device-resident banded R^k tensor R[pair(a,c), pair(b,d), k] of one basis."""
+
+    def __init__(self, handle, ctx: Context):
+        self.handle = handle
+        self.ctx = ctx
+        info = np.zeros(10, np.int64)
+        check(capi.lib().bs2e_rk_info(handle, ptr(info, C.c_int64), 10), ctx.handle)
+        (self.n_b, self.n_pairs, self.K, self.nnz4, self.nnz6, self.n_iv, self.k, self.k_GL, self.U,
+         self.n_knots) = (int(x) for x in info)
+        self.max_k = self.K - 1
+
+    @classmethod
+    def build(cls, k, t, k_GL, max_k, ctx: Context | None = None, compute=True):
+        """Plans + allocates, then (optionally) enqueues the full device build."""
+        ctx = ctx or default_context()
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        h = C.c_void_p()
+        check(capi.lib().bs2e_rk_create(ctx.handle, k, len(t), ptr(t), k_GL, max_k, capi.PREC_FP64, C.byref(h)),
+              ctx.handle)
+        obj = cls(h, ctx)
+        if compute:
+            obj.compute()
+        return obj
+
+    def compute(self):
+        check(capi.lib().bs2e_rk_compute(self.handle), self.ctx.handle)
+
+    def synchronize(self):
+        check(capi.lib().bs2e_rk_synchronize(self.handle), self.ctx.handle)
+
+    def device_ptr(self):
+        p = C.c_void_p()
+        n = C.c_int64()
+        check(capi.lib().bs2e_rk_device_ptr(self.handle, C.byref(p), C.byref(n)), self.ctx.handle)
+        return p.value, n.value
+
+    def pair_index(self, a, c):
+        return capi.lib().bs2e_rk_pair_index(self.handle, a, c)
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.n_pairs * self.n_pairs * self.K)
+        check(capi.lib().bs2e_rk_copy_to_host(self.handle, ptr(out), out.size), self.ctx.handle)
+        return out.reshape(self.n_pairs, self.n_pairs, self.K)
+
+    def mirror(self):
+        check(capi.lib().bs2e_rk_copy_to_host(self.handle, None, 0), self.ctx.handle)
+
+    def get_val(self, indices) -> np.ndarray:
+        idx = np.ascontiguousarray(indices, dtype=np.int64)
+        vals = np.zeros(self.K)
+        check(capi.lib().bs2e_rk_get_val(self.handle, ptr(idx, C.c_int64), ptr(vals), self.K), self.ctx.handle)
+        return vals
+
+    def set_profiling(self, on=True):
+        check(capi.lib().bs2e_rk_set_profiling(self.handle, int(on)), self.ctx.handle)
+
+    def stage_times(self):
+        ms = np.zeros(4)
+        check(capi.lib().bs2e_rk_stage_times(self.handle, ptr(ms), 4), self.ctx.handle)
+        return ms
+
+    def release(self):
+        if self.handle:
+            capi.lib().bs2e_rk_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class Nd_DOK:
+    """sparse_array_tools.f90:49-61 replacement: the banded R^k tensor held on
+    the device with arithmetic key -> offset instead of a hash map.  get_val
+    keeps the reference's signature (indices(4) -> values(0:max_k))."""
+
+    def __init__(self):
+        self.tensor: RkTensor | None = None
+
+    def get_val(self, indices, values=None):
+        if self.tensor is None:
+            raise Bs2eError(capi.BS2E_ESTATE, "Nd_DOK not built (call compute_R_K_map first)")
+        v = self.tensor.get_val(indices)
+        if values is not None:
+            values[:] = v
+        return v
+
+
+def compute_R_K_map(r_d_k, r_k, r_m_k, b_splines, max_k, R: Nd_DOK, ctx: Context | None = None):
+    """sparse_array_tools.f90:452-493: builds R from the given containers on the GPU."""
+    ctx = ctx or default_context()
+    t = b_splines.knots
+    rk = np.asfortranarray(r_k.data)
+    rmk = np.asfortranarray(r_m_k.data)
+    rdk = np.asfortranarray(r_d_k.data)
+    h = C.c_void_p()
+    I = C.c_int64
+    check(capi.lib().bs2e_rk_from_primitives(
+        ctx.handle, b_splines.k, len(t), ptr(t), max_k,
+        r_k.nnz, ptr(rk), ptr(rmk), ptr(r_k.iv, I), ptr(r_k.i, I), ptr(r_k.j, I),
+        r_d_k.nnz, ptr(rdk), ptr(r_d_k.iv, I), ptr(r_d_k.i, I), ptr(r_d_k.i_p, I), ptr(r_d_k.j, I),
+        ptr(r_d_k.j_p, I), C.byref(h)), ctx.handle)
+    R.tensor = RkTensor(h, ctx)
+    R.tensor.compute()
+    R.tensor.mirror()
+    return R
+
+
+def ang_k_LS(k, l1, l2, L):
+    """wigner_tools.f90:126-138 with l1 = confs(1)%l, l2 = confs(2)%l."""
+    return capi.lib().bs2e_ang_k_LS(k, l1[0], l1[1], l2[0], l2[1], L)
+
+
+def three_j(*a):
+    return capi.lib().bs2e_three_j(*[int(x) for x in a])
+
+
+def six_j(*a):
+    return capi.lib().bs2e_six_j(*[int(x) for x in a])

@@ -1,0 +1,111 @@
+// bs2e_internal.h -- shared declarations of the MI355X R^k builder.
+//
+// Index conventions (the reference's, edvinolo/b-spline-two-e):
+//   * spline index s is 1-based, s in [1, n]; basis index b in [1, n_b] is
+//     spline b+1 (bspline_tools.f90:33-34, mat_els.f90:208-210);
+//   * interval iv in [1, n_iv], breakpoints t(k .. size(t)-k+1)
+//     (bspline_tools.f90:36); spline s is supported on iv iff
+//     iv <= s <= iv+k-1 (bspline_tools.f90:42-56);
+//   * a banded pair (a,c), |a-c| < k, is numbered in the loop order of
+//     count_nnz_R_k (sparse_array_tools.f90:347-366): c outer, a inner.
+//
+// Device layouts (all FP64, row-major, 0-based):
+//   Bo [n_iv][k_gl][k_s]            spline values at the outer GL points
+//   Bi [n_iv][k_gl][k_gl][k_s]      spline values at the inner GL points of the
+//                                   triangle rule (mat_els.f90:469-475)
+//   ro [n_iv][k_gl]                 outer radii
+//   Pw, Qw, PP, QS [n_pairs][k_s][K]  per-pair window moments P^k, Q^k over the
+//                                   pair's support window, their prefix (P) and
+//                                   suffix (Q) sums over the window
+//   Ptot, Qtot [n_pairs][K]         window totals
+//   D  [n_iv][K][U][U]              diagonal-cell integrals, U = k_s(k_s+1)/2
+//                                   symmetric local pairs (outer, inner)
+//   R  [n_pairs][n_pairs][K]        the R^k tensor; key [a,b,c,d] lives at
+//                                   (pair(a,c)*n_pairs + pair(b,d))*K + k
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace bs2e {
+
+constexpr int kMaxOrder = 20;   // BVALUE_D work arrays are dimension(20) (bspline_tools.f90:163)
+constexpr int kMaxGL = 64;
+constexpr int kMaxK = 64;       // multipole orders 0..63
+
+struct PairInfo {               // one banded pair (a, c), 1-based basis indices
+    int32_t a, c;
+    int32_t v0;                 // first interval of the common support window (1-based)
+    int32_t m;                  // window length (number of intervals)
+};
+
+// Entry of the reference's sparse_6d order: pair J (inner) outer loop, pair I
+// (outer) inner loop, intervals v0 .. v0+len-1 (mat_els.f90:251-289).
+struct DiagEntry {
+    int32_t J, I;
+    int32_t v0, len;
+    int64_t ptr;                // 0-based first position in sparse_6d order
+};
+
+// Host-side description of one basis (knots + quadrature + multipoles).
+struct Plan {
+    int64_t k_s = 0, n_knots = 0, n = 0, n_b = 0, n_iv = 0;
+    int64_t k_gl = 0, max_k = 0, K = 0;
+    int64_t n_pairs = 0, nnz4 = 0, nnz6 = 0, U = 0;
+    std::vector<double> t, bp, x, w;
+    std::vector<PairInfo> pairs;
+    std::vector<int64_t> off4;          // per pair: first sparse_4d position
+    std::vector<int32_t> c_first;       // first pair index with column c (1-based c, size n_b+2)
+    std::vector<DiagEntry> diag;        // sparse_6d order (built on demand)
+};
+
+// Builds the plan; returns an error message or empty string.
+std::string make_plan(int64_t k_s, int64_t n_knots, const double* knots, int64_t k_gl, int64_t max_k, Plan& p);
+void build_diag_entries(Plan& p);
+
+// Host numerics shared by the C-ABI (setup, not the hot path).
+void gauss_legendre(int64_t N, double* x, double* w);
+int64_t generate_grid(int64_t k, int64_t m, int64_t Z, double h_max, double r_max, std::vector<double>& out);
+double wigner_3j(int64_t ja, int64_t jb, int64_t jc, int64_t ma, int64_t mb, int64_t mc);
+double wigner_6j(int64_t j1, int64_t j2, int64_t j3, int64_t j4, int64_t j5, int64_t j6);
+double ang_k_LS(int64_t k, int64_t l1a, int64_t l1b, int64_t l2a, int64_t l2b, int64_t L);
+
+// Device buffers of one tensor build.
+struct DevTables {
+    double *t = nullptr, *bp = nullptr, *x = nullptr, *w = nullptr;
+    PairInfo* pairs = nullptr;
+    double *Bo = nullptr, *Bi = nullptr, *ro = nullptr;
+    double *Pw = nullptr, *Qw = nullptr, *PP = nullptr, *QS = nullptr, *Ptot = nullptr, *Qtot = nullptr;
+    double* D = nullptr;
+};
+
+// Kernel launchers (each .hip file launches its own kernels).
+hipError_t launch_bspline_tables(const Plan& p, const DevTables& d, hipStream_t s);          // K1
+hipError_t launch_pair_moments(const Plan& p, const DevTables& d, hipStream_t s);            // K2
+hipError_t launch_diag_cells(const Plan& p, const DevTables& d, hipStream_t s);              // K4
+hipError_t launch_tensor_fill(const Plan& p, const DevTables& d, double* R, hipStream_t s);  // K3
+hipError_t launch_scatter_4d(const Plan& p, const DevTables& d, const int64_t* d_off4, double* rk, double* rmk,
+                             int64_t* iv, int64_t* i, int64_t* j, hipStream_t s);
+hipError_t launch_scatter_6d(const Plan& p, const DevTables& d, const DiagEntry* d_diag, int64_t n_diag, double* rdk,
+                             int64_t* iv, int64_t* i, int64_t* ip, int64_t* j, int64_t* jp, hipStream_t s);
+
+hipError_t launch_window_sums(const Plan& p, const DevTables& d, hipStream_t s);               // K2b
+hipError_t launch_fill(double* ptr, int64_t n, double v, hipStream_t s);
+hipError_t launch_gather_4d(const Plan& p, const DevTables& d, const int32_t* d_c_first, const double* rk,
+                            const double* rmk, const int64_t* iv, const int64_t* i, const int64_t* j, int* bad,
+                            hipStream_t s);
+hipError_t launch_gather_6d(const Plan& p, const DevTables& d, const double* rdk, const int64_t* iv, const int64_t* i,
+                            const int64_t* ip, const int64_t* j, const int64_t* jp, int* bad, hipStream_t s);
+size_t diag_cells_lds_bytes(const Plan& p);
+
+// Symmetric local pair index of local splines (ta, tc) on one interval.
+__host__ __device__ inline int sym_local(int ta, int tc, int ks)
+{
+    int lo = ta < tc ? ta : tc, hi = ta < tc ? tc : ta;
+    return lo * ks - (lo * (lo - 1)) / 2 + (hi - lo);
+}
+
+}  // namespace bs2e

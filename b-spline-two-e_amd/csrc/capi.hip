@@ -1,0 +1,584 @@
+// capi.hip -- the extern "C" boundary (include/bs2e.h) over the HIP kernels.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/bs2e.h"
+#include "bs2e_internal.h"
+
+using namespace bs2e;
+
+namespace bs2e {
+size_t diag_cells_lds_bytes(const Plan& p);
+}
+
+struct bs2e_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+};
+
+struct bs2e_rk {
+    bs2e_ctx* ctx = nullptr;
+    Plan plan;
+    DevTables d;
+    double* R = nullptr;
+    size_t R_elems = 0;
+    int64_t precision = BS2E_PREC_FP64;
+    std::vector<double> mirror;
+    bool mirror_valid = false;
+    bool profiling = false;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool computed = false;
+    bool from_primitives = false;   // tables gathered from caller containers: compute runs K3 only
+};
+
+static thread_local std::string g_err;
+
+static int64_t fail(bs2e_ctx* ctx, int64_t code, const std::string& msg)
+{
+    if (ctx) ctx->err = msg;
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                                   \
+    do {                                                                                                     \
+        hipError_t e_ = (expr);                                                                              \
+        if (e_ != hipSuccess)                                                                                \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? BS2E_ENOMEM : BS2E_EDEVICE,                       \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                                  \
+    } while (0)
+
+template <class T>
+static hipError_t dev_alloc(T** p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void**)p, n * sizeof(T));
+}
+
+template <class T>
+static void dev_free(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+extern "C" {
+
+int64_t bs2e_create(bs2e_ctx** out, int64_t device)
+{
+    if (!out) return fail(nullptr, BS2E_EINVAL, "bs2e_create: out is NULL");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return fail(nullptr, BS2E_EDEVICE, "bs2e_create: no HIP device available");
+    if (device < 0 || device >= n) return fail(nullptr, BS2E_EINVAL, "bs2e_create: device index out of range");
+    bs2e_ctx* c = new (std::nothrow) bs2e_ctx();
+    if (!c) return fail(nullptr, BS2E_ENOMEM, "bs2e_create: host allocation failed");
+    c->device = (int)device;
+    e = hipSetDevice(c->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(nullptr, BS2E_EDEVICE, std::string("bs2e_create: ") + hipGetErrorString(e));
+    }
+    c->stream = c->own;
+    *out = c;
+    return BS2E_OK;
+}
+
+int64_t bs2e_destroy(bs2e_ctx* ctx)
+{
+    if (!ctx) return BS2E_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->own) {
+        (void)hipStreamSynchronize(ctx->own);
+        (void)hipStreamDestroy(ctx->own);
+    }
+    delete ctx;
+    return BS2E_OK;
+}
+
+const char* bs2e_last_error(const bs2e_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int64_t bs2e_set_stream(bs2e_ctx* ctx, void* hip_stream)
+{
+    if (!ctx) return fail(nullptr, BS2E_EINVAL, "bs2e_set_stream: ctx is NULL");
+    ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own;
+    return BS2E_OK;
+}
+
+int64_t bs2e_generate_grid(int64_t k, int64_t m, int64_t Z, double h_max, double r_max, double* knots, int64_t cap)
+{
+    if (k < 1 || m < 0 || m > 60 || Z < 1 || !(h_max > 0.0) || !(r_max > 0.0))
+        return fail(nullptr, BS2E_EINVAL, "bs2e_generate_grid: need k >= 1, 0 <= m <= 60, Z >= 1, h_max > 0, r_max > 0");
+    std::vector<double> g;
+    int64_t n = generate_grid(k, m, Z, h_max, r_max, g);
+    if (knots) {
+        if (cap < n) return fail(nullptr, BS2E_EINVAL, "bs2e_generate_grid: cap smaller than the knot count");
+        std::memcpy(knots, g.data(), sizeof(double) * (size_t)n);
+    }
+    return n;
+}
+
+int64_t bs2e_count_nnz(int64_t k, int64_t n_knots, const double* knots, int64_t* nnz4, int64_t* nnz6,
+                       int64_t* n_pairs)
+{
+    Plan p;
+    std::string msg = make_plan(k, n_knots, knots, 1, 0, p);
+    if (!msg.empty()) return fail(nullptr, BS2E_EINVAL, "bs2e_count_nnz: " + msg);
+    if (nnz4) *nnz4 = p.nnz4;
+    if (nnz6) *nnz6 = p.nnz6;
+    if (n_pairs) *n_pairs = p.n_pairs;
+    return BS2E_OK;
+}
+
+double bs2e_ang_k_LS(int64_t k, int64_t l1a, int64_t l1b, int64_t l2a, int64_t l2b, int64_t L)
+{
+    return ang_k_LS(k, l1a, l1b, l2a, l2b, L);
+}
+
+double bs2e_three_j(int64_t ja, int64_t jb, int64_t jc, int64_t ma, int64_t mb, int64_t mc)
+{
+    return wigner_3j(ja, jb, jc, ma, mb, mc);
+}
+
+double bs2e_six_j(int64_t j1, int64_t j2, int64_t j3, int64_t j4, int64_t j5, int64_t j6)
+{
+    return wigner_6j(j1, j2, j3, j4, j5, j6);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// tensor handle
+// ---------------------------------------------------------------------------
+static void free_tables(bs2e_rk* rk)
+{
+    DevTables& d = rk->d;
+    dev_free(d.t);
+    dev_free(d.bp);
+    dev_free(d.x);
+    dev_free(d.w);
+    dev_free(d.pairs);
+    dev_free(d.Bo);
+    dev_free(d.Bi);
+    dev_free(d.ro);
+    dev_free(d.Pw);
+    dev_free(d.Qw);
+    dev_free(d.PP);
+    dev_free(d.QS);
+    dev_free(d.Ptot);
+    dev_free(d.Qtot);
+    dev_free(d.D);
+    dev_free(rk->R);
+    for (auto& e : rk->ev)
+        if (e) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+}
+
+static int64_t alloc_tables(bs2e_ctx* ctx, bs2e_rk* rk, bool with_tensor)
+{
+    const Plan& p = rk->plan;
+    DevTables& d = rk->d;
+    const size_t slots = (size_t)p.n_pairs * p.k_s * p.K;
+    HIP_TRY(ctx, dev_alloc(&d.t, p.n_knots));
+    HIP_TRY(ctx, dev_alloc(&d.bp, p.n_iv + 1));
+    HIP_TRY(ctx, dev_alloc(&d.x, p.k_gl));
+    HIP_TRY(ctx, dev_alloc(&d.w, p.k_gl));
+    HIP_TRY(ctx, dev_alloc(&d.pairs, p.n_pairs));
+    HIP_TRY(ctx, dev_alloc(&d.Bo, (size_t)p.n_iv * p.k_gl * p.k_s));
+    HIP_TRY(ctx, dev_alloc(&d.Bi, (size_t)p.n_iv * p.k_gl * p.k_gl * p.k_s));
+    HIP_TRY(ctx, dev_alloc(&d.ro, (size_t)p.n_iv * p.k_gl));
+    HIP_TRY(ctx, dev_alloc(&d.Pw, slots));
+    HIP_TRY(ctx, dev_alloc(&d.Qw, slots));
+    HIP_TRY(ctx, dev_alloc(&d.PP, slots));
+    HIP_TRY(ctx, dev_alloc(&d.QS, slots));
+    HIP_TRY(ctx, dev_alloc(&d.Ptot, (size_t)p.n_pairs * p.K));
+    HIP_TRY(ctx, dev_alloc(&d.Qtot, (size_t)p.n_pairs * p.K));
+    HIP_TRY(ctx, dev_alloc(&d.D, (size_t)p.n_iv * p.K * p.U * p.U));
+    if (with_tensor) {
+        rk->R_elems = (size_t)p.n_pairs * p.n_pairs * p.K;
+        HIP_TRY(ctx, dev_alloc(&rk->R, rk->R_elems));
+    }
+    hipStream_t s = ctx->stream;
+    HIP_TRY(ctx, hipMemcpyAsync(d.t, p.t.data(), sizeof(double) * p.n_knots, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(d.bp, p.bp.data(), sizeof(double) * (p.n_iv + 1), hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(d.x, p.x.data(), sizeof(double) * p.k_gl, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(d.w, p.w.data(), sizeof(double) * p.k_gl, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(d.pairs, p.pairs.data(), sizeof(PairInfo) * p.n_pairs, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    return BS2E_OK;
+}
+
+static int64_t check_plan_limits(bs2e_ctx* ctx, const Plan& p)
+{
+    if (diag_cells_lds_bytes(p) > 64 * 1024)
+        return fail(ctx, BS2E_EINVAL,
+                    "spline order / k_GL combination needs more than 64 KiB of LDS in the diagonal-cell kernel");
+    if (p.n_pairs * p.K >= ((int64_t)1 << 31)) return fail(ctx, BS2E_EINVAL, "tensor row too long (n_pairs*K >= 2^31)");
+    return BS2E_OK;
+}
+
+// K1, K2, K4 (the primitives) on the context stream.
+static int64_t run_primitives(bs2e_ctx* ctx, bs2e_rk* rk)
+{
+    hipStream_t s = ctx->stream;
+    if (rk->profiling) HIP_TRY(ctx, hipEventRecord(rk->ev[0], s));
+    HIP_TRY(ctx, launch_bspline_tables(rk->plan, rk->d, s));
+    if (rk->profiling) HIP_TRY(ctx, hipEventRecord(rk->ev[1], s));
+    HIP_TRY(ctx, launch_pair_moments(rk->plan, rk->d, s));
+    if (rk->profiling) HIP_TRY(ctx, hipEventRecord(rk->ev[2], s));
+    HIP_TRY(ctx, launch_diag_cells(rk->plan, rk->d, s));
+    if (rk->profiling) HIP_TRY(ctx, hipEventRecord(rk->ev[3], s));
+    return BS2E_OK;
+}
+
+extern "C" {
+
+int64_t bs2e_slater_primitives(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const double* knots, int64_t k_gl,
+                               int64_t max_k, int64_t nnz4, double* rk, double* rmk, int64_t* iv4, int64_t* i4,
+                               int64_t* j4, int64_t nnz6, double* rdk, int64_t* iv6, int64_t* i6, int64_t* ip6,
+                               int64_t* j6, int64_t* jp6)
+{
+    if (!ctx) return fail(nullptr, BS2E_EINVAL, "bs2e_slater_primitives: ctx is NULL");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    bs2e_rk h;
+    h.ctx = ctx;
+    std::string msg = make_plan(k, n_knots, knots, k_gl, max_k, h.plan);
+    if (!msg.empty()) return fail(ctx, BS2E_EINVAL, "bs2e_slater_primitives: " + msg);
+    const Plan& p = h.plan;
+    if (nnz4 != p.nnz4 || nnz6 != p.nnz6)
+        return fail(ctx, BS2E_EINVAL, "bs2e_slater_primitives: nnz4/nnz6 differ from count_nnz_4d/6d for this basis");
+    if (!rk || !rmk || !iv4 || !i4 || !j4 || !rdk || !iv6 || !i6 || !ip6 || !j6 || !jp6)
+        return fail(ctx, BS2E_EINVAL, "bs2e_slater_primitives: NULL output array");
+    int64_t rc = check_plan_limits(ctx, p);
+    if (rc) return rc;
+    build_diag_entries(h.plan);
+    struct Guard {
+        bs2e_rk* h;
+        std::vector<void*> extra;
+        ~Guard()
+        {
+            free_tables(h);
+            for (void* e : extra)
+                if (e) (void)hipFree(e);
+        }
+    } guard{&h, {}};
+    rc = alloc_tables(ctx, &h, false);
+    if (rc) return rc;
+    const int64_t K = p.K;
+    double *d_rk = nullptr, *d_rmk = nullptr, *d_rdk = nullptr;
+    int64_t *d_i4 = nullptr, *d_i6 = nullptr, *d_off4 = nullptr;
+    DiagEntry* d_diag = nullptr;
+    HIP_TRY(ctx, dev_alloc(&d_rk, (size_t)nnz4 * K));
+    guard.extra.push_back(d_rk);
+    HIP_TRY(ctx, dev_alloc(&d_rmk, (size_t)nnz4 * K));
+    guard.extra.push_back(d_rmk);
+    HIP_TRY(ctx, dev_alloc(&d_i4, (size_t)nnz4 * 3));
+    guard.extra.push_back(d_i4);
+    HIP_TRY(ctx, dev_alloc(&d_off4, (size_t)p.n_pairs));
+    guard.extra.push_back(d_off4);
+    HIP_TRY(ctx, dev_alloc(&d_rdk, (size_t)nnz6 * K));
+    guard.extra.push_back(d_rdk);
+    HIP_TRY(ctx, dev_alloc(&d_i6, (size_t)nnz6 * 5));
+    guard.extra.push_back(d_i6);
+    HIP_TRY(ctx, dev_alloc(&d_diag, p.diag.size()));
+    guard.extra.push_back(d_diag);
+    hipStream_t s = ctx->stream;
+    HIP_TRY(ctx, hipMemcpyAsync(d_off4, p.off4.data(), sizeof(int64_t) * p.n_pairs, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(d_diag, p.diag.data(), sizeof(DiagEntry) * p.diag.size(), hipMemcpyHostToDevice, s));
+    rc = run_primitives(ctx, &h);
+    if (rc) return rc;
+    HIP_TRY(ctx, launch_scatter_4d(p, h.d, d_off4, d_rk, d_rmk, d_i4, d_i4 + nnz4, d_i4 + 2 * nnz4, s));
+    HIP_TRY(ctx, launch_scatter_6d(p, h.d, d_diag, (int64_t)p.diag.size(), d_rdk, d_i6, d_i6 + nnz6,
+                                   d_i6 + 2 * nnz6, d_i6 + 3 * nnz6, d_i6 + 4 * nnz6, s));
+    HIP_TRY(ctx, hipMemcpyAsync(rk, d_rk, sizeof(double) * nnz4 * K, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(rmk, d_rmk, sizeof(double) * nnz4 * K, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(iv4, d_i4, sizeof(int64_t) * nnz4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(i4, d_i4 + nnz4, sizeof(int64_t) * nnz4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(j4, d_i4 + 2 * nnz4, sizeof(int64_t) * nnz4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(rdk, d_rdk, sizeof(double) * nnz6 * K, hipMemcpyDeviceToHost, s));
+    int64_t* outs6[5] = {iv6, i6, ip6, j6, jp6};
+    for (int a = 0; a < 5; ++a)
+        HIP_TRY(ctx, hipMemcpyAsync(outs6[a], d_i6 + a * nnz6, sizeof(int64_t) * nnz6, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    return BS2E_OK;
+}
+
+int64_t bs2e_rk_create(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const double* knots, int64_t k_gl, int64_t max_k,
+                       int64_t precision, bs2e_rk** out)
+{
+    if (!ctx) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_create: ctx is NULL");
+    if (!out) return fail(ctx, BS2E_EINVAL, "bs2e_rk_create: out is NULL");
+    *out = nullptr;
+    if (precision != BS2E_PREC_FP64) return fail(ctx, BS2E_EINVAL, "bs2e_rk_create: unknown precision mode");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    bs2e_rk* rk = new (std::nothrow) bs2e_rk();
+    if (!rk) return fail(ctx, BS2E_ENOMEM, "bs2e_rk_create: host allocation failed");
+    rk->ctx = ctx;
+    rk->precision = precision;
+    std::string msg = make_plan(k, n_knots, knots, k_gl, max_k, rk->plan);
+    int64_t rc = msg.empty() ? check_plan_limits(ctx, rk->plan) : fail(ctx, BS2E_EINVAL, "bs2e_rk_create: " + msg);
+    if (!rc) rc = alloc_tables(ctx, rk, true);
+    for (int i = 0; !rc && i < 5; ++i) {
+        hipError_t e = hipEventCreate(&rk->ev[i]);
+        if (e != hipSuccess) rc = fail(ctx, BS2E_EDEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e));
+    }
+    if (rc) {
+        free_tables(rk);
+        delete rk;
+        return rc;
+    }
+    *out = rk;
+    return BS2E_OK;
+}
+
+int64_t bs2e_rk_from_primitives(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const double* knots, int64_t max_k,
+                                int64_t nnz4, const double* rk4, const double* rmk, const int64_t* iv4,
+                                const int64_t* i4, const int64_t* j4, int64_t nnz6, const double* rdk,
+                                const int64_t* iv6, const int64_t* i6, const int64_t* ip6, const int64_t* j6,
+                                const int64_t* jp6, bs2e_rk** out)
+{
+    if (!ctx) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_from_primitives: ctx is NULL");
+    if (!out) return fail(ctx, BS2E_EINVAL, "bs2e_rk_from_primitives: out is NULL");
+    *out = nullptr;
+    if (!rk4 || !rmk || !iv4 || !i4 || !j4 || !rdk || !iv6 || !i6 || !ip6 || !j6 || !jp6)
+        return fail(ctx, BS2E_EINVAL, "bs2e_rk_from_primitives: NULL container array");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    bs2e_rk* rk = new (std::nothrow) bs2e_rk();
+    if (!rk) return fail(ctx, BS2E_ENOMEM, "bs2e_rk_from_primitives: host allocation failed");
+    rk->ctx = ctx;
+    rk->from_primitives = true;
+    // compute_R_K_map has no quadrature argument; the GL order only sizes work
+    // tables that this path never fills, so the smallest rule is planned.
+    std::string msg = make_plan(k, n_knots, knots, 1, max_k, rk->plan);
+    int64_t rc = msg.empty() ? check_plan_limits(ctx, rk->plan)
+                             : fail(ctx, BS2E_EINVAL, "bs2e_rk_from_primitives: " + msg);
+    if (!rc && (nnz4 != rk->plan.nnz4 || nnz6 != rk->plan.nnz6))
+        rc = fail(ctx, BS2E_EINVAL, "bs2e_rk_from_primitives: nnz4/nnz6 differ from count_nnz_4d/6d for this basis");
+    if (!rc) rc = alloc_tables(ctx, rk, true);
+    for (int i = 0; !rc && i < 5; ++i) {
+        hipError_t e = hipEventCreate(&rk->ev[i]);
+        if (e != hipSuccess) rc = fail(ctx, BS2E_EDEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e));
+    }
+    if (rc) {
+        free_tables(rk);
+        delete rk;
+        return rc;
+    }
+    const Plan& p = rk->plan;
+    const int64_t K = p.K;
+    std::vector<void*> tmp;
+    auto cleanup = [&]() {
+        for (void* e : tmp)
+            if (e) (void)hipFree(e);
+    };
+    auto bail = [&](int64_t code) {
+        cleanup();
+        free_tables(rk);
+        delete rk;
+        return code;
+    };
+#define STEP(expr)                                                                                           \
+    do {                                                                                                     \
+        hipError_t e_ = (expr);                                                                              \
+        if (e_ != hipSuccess)                                                                                \
+            return bail(fail(ctx, e_ == hipErrorOutOfMemory ? BS2E_ENOMEM : BS2E_EDEVICE,                    \
+                             std::string(#expr) + ": " + hipGetErrorString(e_)));                            \
+    } while (0)
+    double *d_rk = nullptr, *d_rmk = nullptr, *d_rdk = nullptr;
+    int64_t *d_i4 = nullptr, *d_i6 = nullptr;
+    int32_t* d_cf = nullptr;
+    int* d_bad = nullptr;
+    STEP(dev_alloc(&d_rk, (size_t)nnz4 * K));
+    tmp.push_back(d_rk);
+    STEP(dev_alloc(&d_rmk, (size_t)nnz4 * K));
+    tmp.push_back(d_rmk);
+    STEP(dev_alloc(&d_i4, (size_t)nnz4 * 3));
+    tmp.push_back(d_i4);
+    STEP(dev_alloc(&d_rdk, (size_t)nnz6 * K));
+    tmp.push_back(d_rdk);
+    STEP(dev_alloc(&d_i6, (size_t)nnz6 * 5));
+    tmp.push_back(d_i6);
+    STEP(dev_alloc(&d_cf, p.c_first.size()));
+    tmp.push_back(d_cf);
+    STEP(dev_alloc(&d_bad, 1));
+    tmp.push_back(d_bad);
+    hipStream_t s = ctx->stream;
+    STEP(hipMemcpyAsync(d_rk, rk4, sizeof(double) * nnz4 * K, hipMemcpyHostToDevice, s));
+    STEP(hipMemcpyAsync(d_rmk, rmk, sizeof(double) * nnz4 * K, hipMemcpyHostToDevice, s));
+    STEP(hipMemcpyAsync(d_i4, iv4, sizeof(int64_t) * nnz4, hipMemcpyHostToDevice, s));
+    STEP(hipMemcpyAsync(d_i4 + nnz4, i4, sizeof(int64_t) * nnz4, hipMemcpyHostToDevice, s));
+    STEP(hipMemcpyAsync(d_i4 + 2 * nnz4, j4, sizeof(int64_t) * nnz4, hipMemcpyHostToDevice, s));
+    STEP(hipMemcpyAsync(d_rdk, rdk, sizeof(double) * nnz6 * K, hipMemcpyHostToDevice, s));
+    const int64_t* in6[5] = {iv6, i6, ip6, j6, jp6};
+    for (int a = 0; a < 5; ++a)
+        STEP(hipMemcpyAsync(d_i6 + a * nnz6, in6[a], sizeof(int64_t) * nnz6, hipMemcpyHostToDevice, s));
+    STEP(hipMemcpyAsync(d_cf, p.c_first.data(), sizeof(int32_t) * p.c_first.size(), hipMemcpyHostToDevice, s));
+    const int zero = 0;
+    STEP(hipMemcpyAsync(d_bad, &zero, sizeof(int), hipMemcpyHostToDevice, s));
+    const size_t slots = (size_t)p.n_pairs * p.k_s * p.K;
+    STEP(launch_fill(rk->d.Pw, (int64_t)slots, 0.0, s));
+    STEP(launch_fill(rk->d.Qw, (int64_t)slots, 0.0, s));
+    STEP(launch_fill(rk->d.D, (int64_t)((size_t)p.n_iv * p.K * p.U * p.U), 0.0, s));
+    STEP(launch_gather_4d(p, rk->d, d_cf, d_rk, d_rmk, d_i4, d_i4 + nnz4, d_i4 + 2 * nnz4, d_bad, s));
+    STEP(launch_gather_6d(p, rk->d, d_rdk, d_i6, d_i6 + nnz6, d_i6 + 2 * nnz6, d_i6 + 3 * nnz6, d_i6 + 4 * nnz6,
+                          d_bad, s));
+    STEP(launch_window_sums(p, rk->d, s));
+    int bad = 0;
+    STEP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    STEP(hipStreamSynchronize(s));
+#undef STEP
+    cleanup();
+    if (bad != 0) {
+        free_tables(rk);
+        delete rk;
+        return fail(ctx, BS2E_ERANGE,
+                    "bs2e_rk_from_primitives: " + std::to_string(bad) +
+                        " container entries have indices outside the banded, supported set of this basis");
+    }
+    *out = rk;
+    return BS2E_OK;
+}
+
+int64_t bs2e_rk_compute(bs2e_rk* rk)
+{
+    if (!rk) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_compute: rk is NULL");
+    bs2e_ctx* ctx = rk->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    rk->mirror_valid = false;
+    if (rk->from_primitives) {
+        if (rk->profiling)
+            for (int i = 0; i < 4; ++i) HIP_TRY(ctx, hipEventRecord(rk->ev[i], ctx->stream));
+        HIP_TRY(ctx, launch_tensor_fill(rk->plan, rk->d, rk->R, ctx->stream));
+        if (rk->profiling) HIP_TRY(ctx, hipEventRecord(rk->ev[4], ctx->stream));
+        rk->computed = true;
+        return BS2E_OK;
+    }
+    int64_t rc = run_primitives(ctx, rk);
+    if (rc) return rc;
+    HIP_TRY(ctx, launch_tensor_fill(rk->plan, rk->d, rk->R, ctx->stream));
+    if (rk->profiling) HIP_TRY(ctx, hipEventRecord(rk->ev[4], ctx->stream));
+    rk->computed = true;
+    return BS2E_OK;
+}
+
+int64_t bs2e_rk_synchronize(bs2e_rk* rk)
+{
+    if (!rk) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_synchronize: rk is NULL");
+    HIP_TRY(rk->ctx, hipStreamSynchronize(rk->ctx->stream));
+    return BS2E_OK;
+}
+
+int64_t bs2e_rk_info(const bs2e_rk* rk, int64_t* info, int64_t n)
+{
+    if (!rk || !info) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_info: NULL argument");
+    const Plan& p = rk->plan;
+    const int64_t v[10] = {p.n_b, p.n_pairs, p.K, p.nnz4, p.nnz6, p.n_iv, p.k_s, p.k_gl, p.U, p.n_knots};
+    for (int64_t i = 0; i < n && i < 10; ++i) info[i] = v[i];
+    return BS2E_OK;
+}
+
+int64_t bs2e_rk_device_ptr(const bs2e_rk* rk, void** ptr, int64_t* n_elems)
+{
+    if (!rk) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_device_ptr: rk is NULL");
+    if (ptr) *ptr = rk->R;
+    if (n_elems) *n_elems = (int64_t)rk->R_elems;
+    return BS2E_OK;
+}
+
+int64_t bs2e_rk_pair_index(const bs2e_rk* rk, int64_t a, int64_t c)
+{
+    if (!rk) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_pair_index: rk is NULL");
+    const Plan& p = rk->plan;
+    if (a < 1 || c < 1 || a > p.n_b || c > p.n_b || std::llabs(a - c) >= p.k_s) return BS2E_ERANGE;
+    return p.c_first[c] + (a - std::max<int64_t>(1, c - p.k_s + 1));
+}
+
+int64_t bs2e_rk_copy_to_host(bs2e_rk* rk, double* dst, int64_t n_elems)
+{
+    if (!rk) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_copy_to_host: rk is NULL");
+    bs2e_ctx* ctx = rk->ctx;
+    if (!rk->computed) return fail(ctx, BS2E_ESTATE, "bs2e_rk_copy_to_host: tensor not computed");
+    if (dst && n_elems != (int64_t)rk->R_elems)
+        return fail(ctx, BS2E_EINVAL, "bs2e_rk_copy_to_host: n_elems differs from the tensor size");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    try {
+        rk->mirror.resize(rk->R_elems);
+    } catch (...) {
+        return fail(ctx, BS2E_ENOMEM, "bs2e_rk_copy_to_host: host mirror allocation failed");
+    }
+    HIP_TRY(ctx, hipMemcpy(rk->mirror.data(), rk->R, sizeof(double) * rk->R_elems, hipMemcpyDeviceToHost));
+    rk->mirror_valid = true;
+    if (dst) std::memcpy(dst, rk->mirror.data(), sizeof(double) * rk->R_elems);
+    return BS2E_OK;
+}
+
+int64_t bs2e_rk_get_val(const bs2e_rk* rk, const int64_t* idx, double* values, int64_t n_values)
+{
+    if (!rk || !idx || !values) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_get_val: NULL argument");
+    const Plan& p = rk->plan;
+    if (n_values < p.K) return fail(rk->ctx, BS2E_EINVAL, "bs2e_rk_get_val: values holds fewer than max_k+1 entries");
+    const int64_t pa = bs2e_rk_pair_index(rk, idx[0], idx[2]);
+    const int64_t pb = bs2e_rk_pair_index(rk, idx[1], idx[3]);
+    if (pa < 0 || pb < 0) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "bs2e_rk_get_val: key [%lld,%lld,%lld,%lld] is outside the banded key set",
+                      (long long)idx[0], (long long)idx[1], (long long)idx[2], (long long)idx[3]);
+        return fail(rk->ctx, BS2E_ERANGE, buf);
+    }
+    const size_t off = ((size_t)pa * p.n_pairs + pb) * p.K;
+    if (rk->mirror_valid) {
+        std::memcpy(values, rk->mirror.data() + off, sizeof(double) * p.K);
+        return BS2E_OK;
+    }
+    if (!rk->computed) return fail(rk->ctx, BS2E_ESTATE, "bs2e_rk_get_val: tensor not computed");
+    hipError_t e = hipSetDevice(rk->ctx->device);
+    if (e == hipSuccess) e = hipStreamSynchronize(rk->ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(values, rk->R + off, sizeof(double) * p.K, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail(rk->ctx, BS2E_EDEVICE, std::string("bs2e_rk_get_val: ") + hipGetErrorString(e));
+    return BS2E_OK;
+}
+
+int64_t bs2e_rk_set_profiling(bs2e_rk* rk, int64_t on)
+{
+    if (!rk) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_set_profiling: rk is NULL");
+    rk->profiling = on != 0;
+    return BS2E_OK;
+}
+
+int64_t bs2e_rk_stage_times(bs2e_rk* rk, double* ms, int64_t n)
+{
+    if (!rk || !ms) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_stage_times: NULL argument");
+    if (!rk->profiling || !rk->computed) return fail(rk->ctx, BS2E_ESTATE, "bs2e_rk_stage_times: profiling was off");
+    HIP_TRY(rk->ctx, hipEventSynchronize(rk->ev[4]));
+    for (int64_t i = 0; i < n && i < 4; ++i) {
+        float t = 0.f;
+        HIP_TRY(rk->ctx, hipEventElapsedTime(&t, rk->ev[i], rk->ev[i + 1]));
+        ms[i] = t;
+    }
+    return BS2E_OK;
+}
+
+int64_t bs2e_rk_release(bs2e_rk* rk)
+{
+    if (!rk) return BS2E_OK;
+    (void)hipSetDevice(rk->ctx->device);
+    (void)hipStreamSynchronize(rk->ctx->stream);
+    free_tables(rk);
+    delete rk;
+    return BS2E_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""bench.py -- R^k integrals/s of the device R^k build on one MI355X.
+
+One step = one complete build of the banded R^k tensor of a basis from its
+knot vector: de Boor tables, pair moments, diagonal cells and the tensor
+(the reference's setup_Slater_integrals + compute_R_K_map), with the knots
+and every work buffer already resident in HBM.  Integrals per step =
+n_pairs^2 * (max_k+1), the key count of the reference's Nd_DOK times its
+value length.
+
+    python bench.py [--steps K] [--warmup W] [--workload c2]
+
+Prints one JSON line.  The dominant kernel (the tensor fill) is timed with
+HIP events on the stream it runs on and priced against HBM (8 bytes stored
+per integral, 8 TB/s peak).  cpu_baseline times the oracle (the repository's
+C restatement of the reference path) on the host for the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "b-spline-two-e_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "R^k integrals/sec (mixed vs FP64 GPU) at <=1e-10 rel err; p99 build latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(w, t, R_gpu, nthreads):
+    """Oracle (C restatement, OpenMP over k like the reference) on the host:
+    primitives + dense tensor assembly for the same basis."""
+    import numpy as np
+
+    from oracle import oracle as O
+    K = w.max_k + 1
+    n_b = O.dims(w.k, t)[1]
+    t0 = time.perf_counter()
+    off = O.slater_off_diag(w.k, t, w.k_GL, w.max_k, nthreads=nthreads)
+    dg = O.slater_diag(w.k, t, w.k_GL, w.max_k, nthreads=nthreads)
+    R = O.rk_dense(w.k, n_b, w.max_k, off, dg)
+    dt = time.perf_counter() - t0
+    n_int = R.size
+    rel = float(np.max(np.abs(R_gpu - R) / R)) if R_gpu is not None else None
+    return dict(value=n_int / dt, unit="integrals/s", cores=int(min(nthreads, K)), kind="port",
+                sample=(f"full {w.name} build (primitives + tensor, {n_int} integrals) by the oracle: "
+                        f"OpenMP over the {K} multipoles as the reference's !$omp parallel do, "
+                        f"serial assembly; {dt:.2f} s"),
+                seconds=dt), rel
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "bench.py measures one GPU: the R^k build of one basis runs on one device "
+                                   "(multi-GPU sharding is a design note, not a run)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import bs2e
+    from bs2e.workloads import WORKLOADS
+
+    w = WORKLOADS[args.workload]
+    torch.cuda.set_device(0)
+    stream = torch.cuda.Stream(device=0)     # default priority; the build and its events share it
+    ctx = bs2e.Context(0)
+    ctx.set_stream(stream.cuda_stream)
+    t = bs2e.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+    rk = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx, compute=False)
+    n_int = rk.n_pairs * rk.n_pairs * rk.K
+    print(f"# bench {w.name}: n_b={rk.n_b} n_pairs={rk.n_pairs} K={rk.K} integrals/step={n_int}", file=sys.stderr,
+          flush=True)
+
+    for _ in range(args.warmup):
+        rk.compute()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        rk.compute()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    value = n_int * args.steps / wall
+
+    # per-stage device times with HIP events recorded on the build stream
+    rk.set_profiling(True)
+    stages = []
+    for _ in range(max(5, min(args.steps, 20))):
+        rk.compute()
+        stages.append(rk.stage_times())
+    rk.set_profiling(False)
+    stages = np.array(stages)
+    st_mean = stages.mean(axis=0)
+    t_fill = float(st_mean[3]) * 1e-3
+    traffic_alg = 8.0 * n_int
+    achieved = traffic_alg / t_fill / 1e9
+    R_gpu = rk.to_host() if not args.no_cpu_baseline else None
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "integrals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{w.name}: {w.description}", "n_b": rk.n_b, "n_pairs": rk.n_pairs, "K": rk.K,
+                   "k_GL": w.k_GL, "integrals_per_step": n_int, "precision": "plain FP64",
+                   "parallelism": "single GPU"},
+        "stage_ms": {"bspline_tables": float(st_mean[0]), "pair_moments": float(st_mean[1]),
+                     "diag_cells": float(st_mean[2]), "tensor_fill": float(st_mean[3])},
+        "event_ms_per_step": ev0.elapsed_time(ev1) / args.steps,
+        "roofline": {"bound": "hbm", "kernel": "k3_tensor_fill", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": traffic_alg},
+    }
+    if not args.no_cpu_baseline:
+        nthreads = min(16, os.cpu_count() or 1)
+        cb, rel = cpu_baseline(w, t, R_gpu, nthreads)
+        out["cpu_baseline"] = cb
+        out["max_rel_err_vs_oracle"] = rel
+        out["speedup_vs_cpu"] = value / cb["value"]
+    rk.release()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

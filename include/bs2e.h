@@ -1,0 +1,130 @@
+/*
+ * bs2e.h -- C ABI of the MI355X (gfx950) two-electron R^k builder.
+ *
+ * Drop-in boundary for the two-electron radial matrix-element path of
+ * edvinolo/b-spline-two-e (Fortran 2008, default integer kind 8).  Every
+ * integer crossing this ABI is int64_t (the reference builds with
+ * -fdefault-integer-8, CMakeLists.txt:44-57); Fortran logicals are passed as
+ * int64_t 0/1.  Index conventions are the reference's: spline/basis/interval
+ * indices are 1-based, containers are column-major where the reference's are.
+ *
+ * Errors: every entry point returns BS2E_OK (0) or a negative status; the
+ * message is available from bs2e_last_error(ctx) (or bs2e_last_error(NULL)
+ * for calls without a context).  Nothing aborts inside the library: the
+ * Fortran shim turns a negative status into `error stop`, as the reference
+ * stops on its own errors.
+ *
+ * Threading: any host thread may call.  A context serialises its device work
+ * on one HIP stream.  bs2e_rk_get_val is reentrant and lock-free once the
+ * host mirror exists (the reference wraps its hash-map lookup in
+ * `!$omp critical`, sparse_array_tools.f90:546-548).
+ */
+#ifndef BS2E_H
+#define BS2E_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BS2E_OK 0
+#define BS2E_EINVAL (-1)   /* invalid argument (sizes, orders, null pointers) */
+#define BS2E_EDEVICE (-2)  /* HIP runtime error */
+#define BS2E_ENOMEM (-3)   /* device or host allocation failed */
+#define BS2E_ERANGE (-4)   /* key outside the banded key set (reference: error stop) */
+#define BS2E_ESTATE (-5)   /* object not in a state that allows the call */
+
+#define BS2E_PREC_FP64 0   /* plain FP64 path */
+
+typedef struct bs2e_ctx bs2e_ctx; /* one device + one stream */
+typedef struct bs2e_rk bs2e_rk;   /* one basis and its device-resident R^k tensor */
+
+/* ---- lifecycle --------------------------------------------------------- */
+/* Creates a context on HIP device `device` with its own default-priority
+ * stream.  (No reference counterpart: the reference runs on the host.) */
+int64_t bs2e_create(bs2e_ctx **ctx, int64_t device);
+int64_t bs2e_destroy(bs2e_ctx *ctx);
+const char *bs2e_last_error(const bs2e_ctx *ctx);
+/* Runs the context's device work on `hip_stream` (a hipStream_t) instead of
+ * its own stream; NULL restores the context's stream. */
+int64_t bs2e_set_stream(bs2e_ctx *ctx, void *hip_stream);
+
+/* ---- basis set-up ------------------------------------------------------ */
+/* grid_tools.f90:6-55 generate_grid(k,m,Z,h_max,r_max,grid).  Returns the
+ * number of knots (writes them when knots != NULL and cap suffices), or a
+ * negative status. */
+int64_t bs2e_generate_grid(int64_t k, int64_t m, int64_t Z, double h_max, double r_max, double *knots, int64_t cap);
+
+/* Container sizes: sparse_array_tools.f90:276-296 count_nnz_4d,
+ * :298-324 count_nnz_6d, :347-366 count_nnz_R_k (= n_pairs^2). */
+int64_t bs2e_count_nnz(int64_t k, int64_t n_knots, const double *knots, int64_t *nnz4, int64_t *nnz6,
+                       int64_t *n_pairs);
+
+/* ---- primitives -------------------------------------------------------- */
+/* mat_els.f90:172-182 setup_Slater_integrals(b_splines, max_k, k_GL, r_k,
+ * r_m_k, r_d_k): fills the caller's containers (sparse_4d r_k / r_m_k and
+ * sparse_6d r_d_k, sparse_array_tools.f90:9-31) in the reference's order.
+ * rk, rmk: nnz4 x (max_k+1) column-major; rdk: nnz6 x (max_k+1).  The
+ * reference accumulates into containers that init_4d/init_6d zeroed
+ * (sparse_array_tools.f90:159-178); this call overwrites them. */
+int64_t bs2e_slater_primitives(bs2e_ctx *ctx, int64_t k, int64_t n_knots, const double *knots, int64_t k_gl,
+                               int64_t max_k, int64_t nnz4, double *rk, double *rmk, int64_t *iv4, int64_t *i4,
+                               int64_t *j4, int64_t nnz6, double *rdk, int64_t *iv6, int64_t *i6, int64_t *ip6,
+                               int64_t *j6, int64_t *jp6);
+
+/* ---- the R^k tensor ---------------------------------------------------- */
+/* sparse_array_tools.f90:452-493 compute_R_K_map(r_d_k, r_k, r_m_k,
+ * b_splines, max_k, R) and :495-507 Nd_DOK%init.  Plans the basis and
+ * allocates the device tensor R[pair(a,c)][pair(b,d)][k] (FP64) and its
+ * work tables; computes nothing yet. */
+int64_t bs2e_rk_create(bs2e_ctx *ctx, int64_t k, int64_t n_knots, const double *knots, int64_t k_gl, int64_t max_k,
+                       int64_t precision, bs2e_rk **out);
+/* sparse_array_tools.f90:452-493 compute_R_K_map(r_d_k, r_k, r_m_k,
+ * b_splines, max_k, R) from the caller's own containers (as filled by
+ * setup_Slater_integrals / bs2e_slater_primitives): uploads them, places each
+ * entry by its index arrays, and plans the tensor.  bs2e_rk_compute then runs
+ * only the tensor kernel.  BS2E_ERANGE if an entry's indices fall outside the
+ * banded, supported set of the basis. */
+int64_t bs2e_rk_from_primitives(bs2e_ctx *ctx, int64_t k, int64_t n_knots, const double *knots, int64_t max_k,
+                                int64_t nnz4, const double *rk, const double *rmk, const int64_t *iv4,
+                                const int64_t *i4, const int64_t *j4, int64_t nnz6, const double *rdk,
+                                const int64_t *iv6, const int64_t *i6, const int64_t *ip6, const int64_t *j6,
+                                const int64_t *jp6, bs2e_rk **out);
+/* Enqueues the whole build (de Boor tables, pair moments, diagonal cells,
+ * tensor) on the context's stream.  Allocates nothing. */
+int64_t bs2e_rk_compute(bs2e_rk *rk);
+int64_t bs2e_rk_synchronize(bs2e_rk *rk);
+/* info[0..9] = n_b, n_pairs, K, nnz4, nnz6, n_iv, k, k_GL, U, n_knots */
+int64_t bs2e_rk_info(const bs2e_rk *rk, int64_t *info, int64_t n);
+/* Device address and element count of the tensor (n_pairs^2 * K doubles). */
+int64_t bs2e_rk_device_ptr(const bs2e_rk *rk, void **ptr, int64_t *n_elems);
+/* 0-based position of the banded pair (a, c) in the tensor's row/column
+ * order (count_nnz_R_k loop order: c outer, a inner), or BS2E_ERANGE. */
+int64_t bs2e_rk_pair_index(const bs2e_rk *rk, int64_t a, int64_t c);
+/* Copies the tensor to the host: into dst (n_elems doubles) when dst != NULL,
+ * and into the handle's host mirror used by bs2e_rk_get_val. */
+int64_t bs2e_rk_copy_to_host(bs2e_rk *rk, double *dst, int64_t n_elems);
+/* sparse_array_tools.f90:536-555 Nd_DOK%get_val(indices(4), values(0:max_k)):
+ * values of key [a,b,c,d].  Reads the host mirror (lock-free); without a
+ * mirror it copies the K values from the device.  BS2E_ERANGE for a key
+ * outside |a-c| < k, |b-d| < k (the reference error-stops there). */
+int64_t bs2e_rk_get_val(const bs2e_rk *rk, const int64_t *indices, double *values, int64_t n_values);
+/* Per-stage device times (ms) of the most recent compute when profiling is
+ * on: [0] de Boor tables, [1] pair moments, [2] diagonal cells, [3] tensor. */
+int64_t bs2e_rk_set_profiling(bs2e_rk *rk, int64_t on);
+int64_t bs2e_rk_stage_times(bs2e_rk *rk, double *ms, int64_t n);
+int64_t bs2e_rk_release(bs2e_rk *rk);
+
+/* ---- angular algebra --------------------------------------------------- */
+/* wigner_tools.f90:126-138 ang_k_LS(k, confs, L) with confs(1)%l = (l1a,l1b),
+ * confs(2)%l = (l2a,l2b); wigner_tools.f90:30-60 three_j / six_j. */
+double bs2e_ang_k_LS(int64_t k, int64_t l1a, int64_t l1b, int64_t l2a, int64_t l2b, int64_t L);
+double bs2e_three_j(int64_t ja, int64_t jb, int64_t jc, int64_t ma, int64_t mb, int64_t mc);
+double bs2e_six_j(int64_t j1, int64_t j2, int64_t j3, int64_t j4, int64_t j5, int64_t j6);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BS2E_H */

@@ -1,0 +1,881 @@
+/*
+ * bs2e_oracle.c -- CPU restatement of the reference's two-electron R^k path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
+ * b-spline-two-e_amd/: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it.  Nothing in the product links or calls it.
+ *
+ * Every function restates one routine of the reference (edvinolo/b-spline-two-e,
+ * Fortran 2008) in plain C and cites the file:line it follows.  Index
+ * conventions follow the reference: spline indices s are 1-based, basis index
+ * b in [1,n_b] is spline b+1, knot vectors are passed 0-based (t[0] == t(1)).
+ * Arrays that the reference stores column-major (data(nnz,0:max_k)) are
+ * returned column-major: data[k*nnz + ptr].
+ *
+ * Arithmetic order follows the Fortran source expression by expression; the
+ * Makefile builds with -ffp-contract=off so that no multiply-add is fused.
+ *
+ * Parity pins (see tests/test_oracle_*.py):
+ *   or_generate_grid, or_bvalue         bit-exact vs the compiled reference
+ *                                       (oracle/_ref) + scipy BSpline
+ *   or_gauss_legendre                   numpy leggauss, mpmath, exactness
+ *   or_slater_off_diag (P)              exact polynomial integrals (fractions)
+ *   or_slater_off_diag (Q), or_slater_diag  independent numpy/scipy restatement
+ *                                       of the same rule + mpmath 2-D quadrature
+ *   or_rk_dense                         8-fold symmetry, hydrogenic closed forms
+ *   or_three_j, or_six_j                sympy exact values
+ *   or_count_configs                    bit-exact vs compiled reference
+ *   or_construct_block                  brute-force predicate restatement on
+ *                                       tiny bases + He eigenvalues
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef int64_t i64;
+
+/* ------------------------------------------------------------------------ */
+/* Integer power x**k as Fortran lowers it for an integer exponent: binary    */
+/* exponentiation (square-and-multiply), negative exponents by reciprocal.   */
+/* ------------------------------------------------------------------------ */
+static double ipow(double x, i64 m)
+{
+    uint64_t n = (uint64_t)(m < 0 ? -m : m);
+    double y = (n & 1) ? x : 1.0;
+    while (n >>= 1) {
+        x = x * x;
+        if (n & 1) y = y * x;
+    }
+    return m < 0 ? 1.0 / y : y;
+}
+
+/* ------------------------------------------------------------------------ */
+/* grid_tools.f90:6-55  generate_grid(k,m,Z,h_max,r_max,grid)                */
+/* Returns the number of knots; writes them to out[0..n) when out != NULL.  */
+/* Returns -1 if cap is too small.                                           */
+/* ------------------------------------------------------------------------ */
+i64 or_generate_grid(i64 k, i64 m, i64 Z, double h_max, double r_max, double *out, i64 cap)
+{
+    double h = ipow(2.0, -m);                                     /* :17 */
+    i64 n_alloc = 2 * k + m + (i64)ceil(r_max / h) + 8;           /* :20 (+slack) */
+    double *g = (double *)calloc((size_t)n_alloc + 1, sizeof(double)); /* 1-based */
+    i64 i;
+    double next;
+    for (i = 1; i <= k; ++i) g[i] = 0.0;                          /* :24 */
+    for (i = k + 1; i <= k + m; ++i) g[i] = g[i - 1] + h;         /* :27-29 */
+    i = k + m;                                                    /* :32 */
+    for (;;) {                                                    /* :33-38 */
+        next = g[i] * (1.0 + h);
+        if ((next - g[i]) >= h_max * (double)Z) break;
+        if (i + 1 > n_alloc) { free(g); return -1; }
+        g[i + 1] = next;
+        i = i + 1;
+    }
+    while (g[i] < (double)Z * r_max) {                            /* :41-44 */
+        if (i + 1 > n_alloc) { free(g); return -1; }
+        g[i + 1] = g[i] + (double)Z * h_max;
+        i = i + 1;
+    }
+    if (i + k - 1 > n_alloc) { free(g); return -1; }
+    for (i64 j = i + 1; j <= i + k - 1; ++j) g[j] = g[i];         /* :47 */
+    i64 n_grid = i + k - 1;                                       /* :49 */
+    if (out) {
+        if (n_grid > cap) { free(g); return -1; }
+        for (i64 j = 1; j <= n_grid; ++j) out[j - 1] = g[j] / (double)Z; /* :53 */
+    }
+    free(g);
+    return n_grid;
+}
+
+/* ------------------------------------------------------------------------ */
+/* bspline_tools.f90:151-224  BVALUE_D(t,a,n,k,x,i_deriv,iv)                 */
+/* Only the branch with iv present (the hot path always passes iv).         */
+/* t: knots (0-based storage of the 1-based t(1..n+k)); a: coefficients.    */
+/* ------------------------------------------------------------------------ */
+static double bvalue_core(const double *t, i64 k, const double *aw, double x, i64 i_deriv, i64 iv)
+{
+    /* aw[0..k) holds the coefficient window a(i-k+1 .. i), i = k-1+iv */
+    double a_j[66], d_p[66], d_m[66];
+#define T(ix) t[(ix) - 1]
+    i64 k_m_i_der = k - i_deriv;                                  /* :166 */
+    if (k_m_i_der <= 0) return 0.0;
+    if (k > 64) return NAN;
+    i64 i = k - 1 + iv;                                           /* :175 */
+    for (i64 j = 1; j <= k; ++j) a_j[j] = aw[j - 1];              /* :190-194 */
+    a_j[k + 1] = 0.0;
+    for (i64 j = 1; j <= i_deriv; ++j) {                          /* :196-203 */
+        i64 k_m_j = k - j;
+        for (i64 jj = 1; jj <= k_m_j; ++jj) {
+            i64 i_h_i = i + jj;
+            i64 i_h_m_k_m_j = i_h_i - k_m_j;
+            a_j[jj] = (a_j[jj + 1] - a_j[jj]) / (T(i_h_i) - T(i_h_m_k_m_j)) * (double)k_m_j;
+        }
+    }
+    for (i64 j = 1; j <= k_m_i_der; ++j) {                        /* :205-211 */
+        d_p[j] = T(i + j) - x;
+        d_m[j] = x - T(i + 1 - j);
+    }
+    for (i64 j = i_deriv + 1; j <= k - 1; ++j) {                  /* :213-220 */
+        i64 k_m_j = k - j;
+        i64 ii = k_m_j;
+        for (i64 jj = 1; jj <= k_m_j; ++jj) {
+            a_j[jj] = (a_j[jj + 1] * d_m[ii] + a_j[jj] * d_p[jj]) / (d_m[ii] + d_p[jj]);
+            ii = ii - 1;
+        }
+    }
+#undef T
+    return a_j[1];                                                /* :221 */
+}
+
+/* Full-coefficient form: a is the whole coefficient vector a(1..n). */
+double or_bvalue(const double *t, i64 n, i64 k, const double *a, double x, i64 i_deriv, i64 iv)
+{
+    (void)n;
+    i64 i = k - 1 + iv;
+    return bvalue_core(t, k, a + (i - k), x, i_deriv, iv);
+}
+
+/* BVALUE_D with the unit coefficient vector c = e_s (s is a 1-based spline
+ * index), as mat_els.f90:203-210 builds it for each basis function.        */
+double or_bvalue_unit(const double *t, i64 n, i64 k, i64 s, double x, i64 i_deriv, i64 iv)
+{
+    double aw[64];
+    (void)n;
+    if (k > 64) return NAN;
+    i64 i = k - 1 + iv;
+    for (i64 j = 1; j <= k; ++j) aw[j - 1] = (i - k + j == s) ? 1.0 : 0.0;
+    return bvalue_core(t, k, aw, x, i_deriv, iv);
+}
+
+/* ------------------------------------------------------------------------ */
+/* quad_tools.f90:14-27 setup_GL -> stdlib_quadrature gauss_legendre.       */
+/* fortran-lang/stdlib is not vendored in the reference (it is cloned at    */
+/* HEAD by utils/install_bs2e_aws_linux_2023.sh:72, so its version is       */
+/* unpinned).  Its published method: Newton iteration on P_N from the       */
+/* asymptotic initial guess, weights 2/((1-x^2) P_N'(x)^2).  Restated here   */
+/* in long double and rounded to double.                                     */
+/* ------------------------------------------------------------------------ */
+void or_gauss_legendre(i64 N, double *x, double *w)
+{
+    const long double PI = 3.141592653589793238462643383279502884L;
+    for (i64 i = 0; i < (N + 1) / 2; ++i) {
+        long double z = -cosl(PI * ((long double)i + 0.75L) / ((long double)N + 0.5L));
+        long double dp = 0.0L;
+        for (int it = 0; it < 100; ++it) {
+            long double p0 = 1.0L, p1 = z;
+            for (i64 l = 2; l <= N; ++l) {
+                long double p2 = ((2.0L * l - 1.0L) * z * p1 - (l - 1.0L) * p0) / (long double)l;
+                p0 = p1;
+                p1 = p2;
+            }
+            if (N == 1) { p1 = z; p0 = 1.0L; }
+            dp = (long double)N * (z * p1 - p0) / (z * z - 1.0L);
+            long double dz = p1 / dp;
+            z -= dz;
+            if (fabsl(dz) < 1e-19L) break;
+        }
+        {
+            long double p0 = 1.0L, p1 = z;
+            for (i64 l = 2; l <= N; ++l) {
+                long double p2 = ((2.0L * l - 1.0L) * z * p1 - (l - 1.0L) * p0) / (long double)l;
+                p0 = p1;
+                p1 = p2;
+            }
+            if (N == 1) { p1 = z; p0 = 1.0L; }
+            dp = (long double)N * (z * p1 - p0) / (z * z - 1.0L);
+        }
+        long double ww = 2.0L / ((1.0L - z * z) * dp * dp);
+        x[i] = (double)z;
+        x[N - 1 - i] = (double)(-z);
+        w[i] = (double)ww;
+        w[N - 1 - i] = (double)ww;
+    }
+    if (N % 2 == 1) x[N / 2] = 0.0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Spline bookkeeping: bspline_tools.f90:27-56                               */
+/*   n = size(t)-k, n_b = n-2, breakpoints = t(k : size(t)-k+1),             */
+/*   support(iv,s) <=> iv <= s <= iv+k-1.                                    */
+/* ------------------------------------------------------------------------ */
+static inline int supp(i64 k, i64 iv, i64 s) { return iv <= s && s <= iv + k - 1; }
+
+void or_spline_dims(i64 k, i64 n_knots, i64 *n, i64 *n_b, i64 *n_iv)
+{
+    *n = n_knots - k;
+    *n_b = *n - 2;
+    *n_iv = (n_knots - k + 1) - k + 1 - 1;   /* size(breakpoints)-1 */
+}
+
+/* sparse_array_tools.f90:276-296 count_nnz_4d */
+i64 or_count_nnz_4d(i64 k, i64 n_knots)
+{
+    i64 n, n_b, n_iv, nnz = 0;
+    or_spline_dims(k, n_knots, &n, &n_b, &n_iv);
+    for (i64 j = 1; j <= n_b; ++j)
+        for (i64 i = 1; i <= n_b; ++i) {
+            if (llabs(i - j) >= k) continue;
+            for (i64 iv = 1; iv <= n_iv; ++iv)
+                if (supp(k, iv, i + 1) && supp(k, iv, j + 1)) ++nnz;
+        }
+    return nnz;
+}
+
+/* sparse_array_tools.f90:298-324 count_nnz_6d */
+i64 or_count_nnz_6d(i64 k, i64 n_knots)
+{
+    i64 n, n_b, n_iv, nnz = 0;
+    or_spline_dims(k, n_knots, &n, &n_b, &n_iv);
+    for (i64 jp = 1; jp <= n_b; ++jp)
+        for (i64 j = 1; j <= n_b; ++j) {
+            if (llabs(j - jp) >= k) continue;
+            for (i64 ip = 1; ip <= n_b; ++ip)
+                for (i64 i = 1; i <= n_b; ++i) {
+                    if (llabs(i - ip) >= k) continue;
+                    for (i64 iv = 1; iv <= n_iv; ++iv) {
+                        int si = supp(k, iv, i + 1) && supp(k, iv, ip + 1);
+                        int sj = supp(k, iv, j + 1) && supp(k, iv, jp + 1);
+                        if (si && sj) ++nnz;
+                    }
+                }
+        }
+    return nnz;
+}
+
+/* sparse_array_tools.f90:347-366 count_nnz_R_k (number of banded pairs
+ * squared); also returns the number of banded (i,i') pairs.                */
+i64 or_count_pairs(i64 k, i64 n_b)
+{
+    i64 np = 0;
+    for (i64 c = 1; c <= n_b; ++c)
+        for (i64 a = 1; a <= n_b; ++a)
+            if (llabs(a - c) < k) ++np;
+    return np;
+}
+
+/* 0-based index of the banded pair (a,c) in the loop order of
+ * count_nnz_R_k / setup_Slater_off_diag: c (= j_b) outer, a (= i_b) inner.  */
+i64 or_pair_index(i64 k, i64 n_b, i64 a, i64 c)
+{
+    if (a < 1 || a > n_b || c < 1 || c > n_b || llabs(a - c) >= k) return -1;
+    i64 off = 0;
+    for (i64 cc = 1; cc < c; ++cc) {
+        i64 lo = cc - k + 1 < 1 ? 1 : cc - k + 1;
+        i64 hi = cc + k - 1 > n_b ? n_b : cc + k - 1;
+        off += hi - lo + 1;
+    }
+    i64 lo = c - k + 1 < 1 ? 1 : c - k + 1;
+    return off + (a - lo);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Per-interval B-spline table used to memoise BVALUE_D calls.  The value   */
+/* of BVALUE_D(t,e_s,n,k,x,0,iv) is a deterministic function of (x,s,iv),   */
+/* so caching it does not change a single bit of any result.                */
+/* ------------------------------------------------------------------------ */
+static double bval_spline(const double *t, i64 n, i64 k, i64 s, double x, i64 iv)
+{
+    return or_bvalue_unit(t, n, k, s, x, 0, iv);
+}
+
+/* ------------------------------------------------------------------------ */
+/* mat_els.f90:184-228 setup_Slater_off_diag + :392-439                      */
+/* compute_Slater_off_diag.  Fills rk/rmk (column-major nnz4 x (max_k+1))   */
+/* and the index arrays iv4/i4/j4 (1-based).  faithful=1 re-evaluates every */
+/* B-spline value inside the loops exactly as the reference does (used to   */
+/* time the reference's schedule); faithful=0 memoises them (same bits).     */
+/* Parallel over k like the reference's !$omp parallel do (:199).           */
+/* ------------------------------------------------------------------------ */
+void or_slater_off_diag(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i64 max_k, i64 nnz4,
+                        double *rk, double *rmk, i64 *iv4, i64 *i4, i64 *j4, i64 faithful, i64 nthreads)
+{
+    i64 n, n_b, n_iv;
+    or_spline_dims(k_s, n_knots, &n, &n_b, &n_iv);
+    const double *bp = t + (k_s - 1);               /* breakpoints(1) = t(k) */
+    double *x = (double *)malloc(sizeof(double) * (size_t)k_gl);
+    double *w = (double *)malloc(sizeof(double) * (size_t)k_gl);
+    or_gauss_legendre(k_gl, x, w);                  /* :196 setup_GL(k_GL,-1,1) */
+
+    /* memo table B[(iv-1)][q][s - iv] */
+    double *btab = NULL;
+    if (!faithful) {
+        btab = (double *)malloc(sizeof(double) * (size_t)(n_iv * k_gl * k_s));
+        for (i64 iv = 1; iv <= n_iv; ++iv) {
+            double scale = 0.5 * (bp[iv] - bp[iv - 1]);
+            double translate = 0.5 * (bp[iv] + bp[iv - 1]);
+            for (i64 q = 0; q < k_gl; ++q) {
+                double r = scale * x[q] + translate;
+                for (i64 ls = 0; ls < k_s; ++ls)
+                    btab[((iv - 1) * k_gl + q) * k_s + ls] = bval_spline(t, n, k_s, iv + ls, r, iv);
+            }
+        }
+    }
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads((int)nthreads);
+#pragma omp parallel for schedule(static)
+#else
+    (void)nthreads;
+#endif
+    for (i64 k = 0; k <= max_k; ++k) {
+        i64 ptr = 0;
+        double r[64], Bi[64], Bj[64];
+        for (i64 j_b = 1; j_b <= n_b; ++j_b)
+            for (i64 i_b = 1; i_b <= n_b; ++i_b) {
+                if (llabs(j_b - i_b) >= k_s) continue;            /* :212 */
+                for (i64 i_r = 1; i_r <= n_iv; ++i_r) {
+                    if (!(supp(k_s, i_r, i_b + 1) && supp(k_s, i_r, j_b + 1))) continue;
+                    ++ptr;
+                    double lim1 = bp[i_r - 1], lim2 = bp[i_r];
+                    double scale = 0.5 * (lim2 - lim1);           /* :413 */
+                    double translate = 0.5 * (lim2 + lim1);       /* :414 */
+                    for (i64 q = 0; q < k_gl; ++q) {              /* :415-419 */
+                        r[q] = scale * x[q] + translate;
+                        if (faithful) {
+                            Bi[q] = bval_spline(t, n, k_s, i_b + 1, r[q], i_r);
+                            Bj[q] = bval_spline(t, n, k_s, j_b + 1, r[q], i_r);
+                        } else {
+                            Bi[q] = btab[((i_r - 1) * k_gl + q) * k_s + (i_b + 1 - i_r)];
+                            Bj[q] = btab[((i_r - 1) * k_gl + q) * k_s + (j_b + 1 - i_r)];
+                        }
+                    }
+                    double accp = 0.0, accq = 0.0;                /* data zero-initialised (init_4d) */
+                    for (i64 q = 0; q < k_gl; ++q) {              /* :421-424 */
+                        accp = accp + w[q] * Bi[q] * Bj[q] * ipow(r[q], k);
+                        accq = accq + w[q] * Bi[q] * Bj[q] / ipow(r[q], k + 1);
+                    }
+                    rk[k * nnz4 + (ptr - 1)] = scale * accp;      /* :427 */
+                    rmk[k * nnz4 + (ptr - 1)] = scale * accq;     /* :428 */
+                    if (k == 0) {                                 /* :430-438 */
+                        iv4[ptr - 1] = i_r;
+                        i4[ptr - 1] = i_b;
+                        j4[ptr - 1] = j_b;
+                    }
+                }
+            }
+    }
+    free(btab);
+    free(x);
+    free(w);
+}
+
+/* ------------------------------------------------------------------------ */
+/* mat_els.f90:230-292 setup_Slater_diag + :441-491 compute_Slater_diag.     */
+/* Loop nest j_b_p, j_b (band), i_b_p, i_b (band), i_r (all four supported). */
+/* ------------------------------------------------------------------------ */
+void or_slater_diag(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i64 max_k, i64 nnz6,
+                    double *rdk, i64 *iv6, i64 *i6, i64 *ip6, i64 *j6, i64 *jp6, i64 faithful, i64 nthreads)
+{
+    i64 n, n_b, n_iv;
+    if (k_gl > 64 || k_s > 64) return;
+    or_spline_dims(k_s, n_knots, &n, &n_b, &n_iv);
+    const double *bp = t + (k_s - 1);
+    double *x = (double *)malloc(sizeof(double) * (size_t)k_gl);
+    double *w = (double *)malloc(sizeof(double) * (size_t)k_gl);
+    or_gauss_legendre(k_gl, x, w);                  /* :243 */
+
+    /* memo tables: outer points B[(iv-1)][q][ls], inner points
+     * B[(iv-1)][q][p][ls] at r_j = scale_j*x_p + translate_j (:473).        */
+    double *bo = NULL, *bi = NULL;
+    if (!faithful) {
+        bo = (double *)malloc(sizeof(double) * (size_t)(n_iv * k_gl * k_s));
+        bi = (double *)malloc(sizeof(double) * (size_t)(n_iv * k_gl * k_gl * k_s));
+        for (i64 iv = 1; iv <= n_iv; ++iv) {
+            double lim1 = bp[iv - 1], lim2 = bp[iv];
+            double scale_i = 0.5 * (lim2 - lim1);
+            double translate_i = 0.5 * (lim2 + lim1);
+            for (i64 q = 0; q < k_gl; ++q) {
+                double r = scale_i * x[q] + translate_i;
+                for (i64 ls = 0; ls < k_s; ++ls)
+                    bo[((iv - 1) * k_gl + q) * k_s + ls] = bval_spline(t, n, k_s, iv + ls, r, iv);
+                double scale_j = 0.5 * (r - lim1);
+                double translate_j = 0.5 * (r + lim1);
+                for (i64 p = 0; p < k_gl; ++p) {
+                    double rj = scale_j * x[p] + translate_j;
+                    for (i64 ls = 0; ls < k_s; ++ls)
+                        bi[(((iv - 1) * k_gl + q) * k_gl + p) * k_s + ls] = bval_spline(t, n, k_s, iv + ls, rj, iv);
+                }
+            }
+        }
+    }
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads((int)nthreads);
+#pragma omp parallel for schedule(static)
+#else
+    (void)nthreads;
+#endif
+    for (i64 k = 0; k <= max_k; ++k) {
+        i64 ptr = 0;
+        double r[64], Bi[64], Bip[64];
+        /* memo of r_j**k at the inner points and r**(k+1) at the outer points
+         * (deterministic functions of their arguments: same bits as inline). */
+        double *pin = NULL, *pout = NULL;
+        if (!faithful) {
+            pin = (double *)malloc(sizeof(double) * (size_t)(n_iv * k_gl * k_gl));
+            pout = (double *)malloc(sizeof(double) * (size_t)(n_iv * k_gl));
+            for (i64 iv = 1; iv <= n_iv; ++iv) {
+                double lim1 = bp[iv - 1], lim2 = bp[iv];
+                double scale_i = 0.5 * (lim2 - lim1);
+                double translate_i = 0.5 * (lim2 + lim1);
+                for (i64 q = 0; q < k_gl; ++q) {
+                    double rq = scale_i * x[q] + translate_i;
+                    pout[(iv - 1) * k_gl + q] = ipow(rq, k + 1);
+                    double scale_j = 0.5 * (rq - lim1);
+                    double translate_j = 0.5 * (rq + lim1);
+                    for (i64 p = 0; p < k_gl; ++p)
+                        pin[((iv - 1) * k_gl + q) * k_gl + p] = ipow(scale_j * x[p] + translate_j, k);
+                }
+            }
+        }
+        for (i64 jp = 1; jp <= n_b; ++jp)
+            for (i64 j = 1; j <= n_b; ++j) {
+                if (llabs(j - jp) >= k_s) continue;               /* :257 */
+                /* Non-faithful mode skips only (i_p, i_r) values for which the
+                 * support test of :277-279 is false, so ptr order is unchanged:
+                 * a common interval needs i_r in J's window and i_p within
+                 * k_s-2 of it. */
+                i64 vJ0 = (j > jp ? j : jp) + 2 - k_s, vJ1 = (j < jp ? j : jp) + 1;
+                if (vJ0 < 1) vJ0 = 1;
+                if (vJ1 > n_iv) vJ1 = n_iv;
+                i64 ip_lo = 1, ip_hi = n_b, r_lo = 1, r_hi = n_iv;
+                if (!faithful) {
+                    ip_lo = vJ0 - 1 < 1 ? 1 : vJ0 - 1;
+                    ip_hi = vJ1 + k_s - 2 > n_b ? n_b : vJ1 + k_s - 2;
+                    r_lo = vJ0;
+                    r_hi = vJ1;
+                }
+                for (i64 ip = ip_lo; ip <= ip_hi; ++ip)
+                    for (i64 i = 1; i <= n_b; ++i) {
+                        if (llabs(i - ip) >= k_s) continue;       /* :273 */
+                        for (i64 i_r = r_lo; i_r <= r_hi; ++i_r) {
+                            int si = supp(k_s, i_r, i + 1) && supp(k_s, i_r, ip + 1);
+                            int sj = supp(k_s, i_r, j + 1) && supp(k_s, i_r, jp + 1);
+                            if (!(si && sj)) continue;
+                            ++ptr;
+                            double lim1 = bp[i_r - 1], lim2 = bp[i_r];
+                            double scale_i = 0.5 * (lim2 - lim1);         /* :460 */
+                            double translate_i = 0.5 * (lim2 + lim1);     /* :461 */
+                            for (i64 q = 0; q < k_gl; ++q) {              /* :462-466 */
+                                r[q] = scale_i * x[q] + translate_i;
+                                if (faithful) {
+                                    Bi[q] = bval_spline(t, n, k_s, i + 1, r[q], i_r);
+                                    Bip[q] = bval_spline(t, n, k_s, ip + 1, r[q], i_r);
+                                } else {
+                                    Bi[q] = bo[((i_r - 1) * k_gl + q) * k_s + (i + 1 - i_r)];
+                                    Bip[q] = bo[((i_r - 1) * k_gl + q) * k_s + (ip + 1 - i_r)];
+                                }
+                            }
+                            double acc = 0.0;
+                            for (i64 q = 0; q < k_gl; ++q) {              /* :468-481 */
+                                double scale_j = 0.5 * (r[q] - lim1);
+                                double translate_j = 0.5 * (r[q] + lim1);
+                                double int_j = 0.0;
+                                for (i64 p = 0; p < k_gl; ++p) {
+                                    double r_j = scale_j * x[p] + translate_j;
+                                    double Bj, Bjp, rjk;
+                                    if (faithful) {
+                                        Bj = bval_spline(t, n, k_s, j + 1, r_j, i_r);
+                                        Bjp = bval_spline(t, n, k_s, jp + 1, r_j, i_r);
+                                        rjk = ipow(r_j, k);
+                                    } else {
+                                        const double *row = bi + (((i_r - 1) * k_gl + q) * k_gl + p) * k_s;
+                                        Bj = row[j + 1 - i_r];
+                                        Bjp = row[jp + 1 - i_r];
+                                        rjk = pin[((i_r - 1) * k_gl + q) * k_gl + p];
+                                    }
+                                    int_j = int_j + w[p] * Bj * Bjp * rjk;
+                                }
+                                int_j = scale_j * int_j;
+                                double rq1 = faithful ? ipow(r[q], k + 1) : pout[(i_r - 1) * k_gl + q];
+                                acc = acc + w[q] * Bi[q] * Bip[q] * int_j / rq1;
+                            }
+                            rdk[k * nnz6 + (ptr - 1)] = scale_i * acc;    /* :483 */
+                            if (k == 0) {                                 /* :484-490 */
+                                iv6[ptr - 1] = i_r;
+                                i6[ptr - 1] = i;
+                                ip6[ptr - 1] = ip;
+                                j6[ptr - 1] = j;
+                                jp6[ptr - 1] = jp;
+                            }
+                        }
+                    }
+            }
+        free(pin);
+        free(pout);
+    }
+    free(bo);
+    free(bi);
+    free(x);
+    free(w);
+}
+
+/* ------------------------------------------------------------------------ */
+/* sparse_array_tools.f90:452-493 compute_R_K_map, with the DOK replaced by   */
+/* a dense banded array R[(p*NP + q)*K + k], p = pair(a,c), q = pair(b,d)    */
+/* for key [a,b,c,d].  Accumulation follows the reference's insertion order: */
+/* the nnz4^2 (n,m) loop first (:472-482), then the diagonal cells, each     */
+/* added to [i,j,i_p,j_p] and to [j,i,j_p,i_p] (:484-488).  Every key of     */
+/* count_nnz_R_k is reached (checked by the caller through `touched`).      */
+/* ------------------------------------------------------------------------ */
+void or_rk_dense(i64 k_s, i64 n_b, i64 max_k,
+                 i64 nnz4, const double *rk, const double *rmk, const i64 *iv4, const i64 *i4, const i64 *j4,
+                 i64 nnz6, const double *rdk, const i64 *i6, const i64 *ip6, const i64 *j6, const i64 *jp6,
+                 double *R, unsigned char *touched)
+{
+    i64 K = max_k + 1;
+    i64 np = or_count_pairs(k_s, n_b);
+    /* pair index of each 4d entry */
+    i64 *pidx = (i64 *)malloc(sizeof(i64) * (size_t)(nnz4 > 0 ? nnz4 : 1));
+    for (i64 e = 0; e < nnz4; ++e) pidx[e] = or_pair_index(k_s, n_b, i4[e], j4[e]);
+    memset(R, 0, sizeof(double) * (size_t)(np * np * K));
+    if (touched) memset(touched, 0, (size_t)(np * np));
+    double *val = (double *)malloc(sizeof(double) * (size_t)K);
+    for (i64 nn = 0; nn < nnz4; ++nn)
+        for (i64 mm = 0; mm < nnz4; ++mm) {
+            int lt = iv4[nn] < iv4[mm], gt = iv4[nn] > iv4[mm];
+            if (!lt && !gt) continue;
+            /* key [i(n), i(m), j(n), j(m)] -> p = pair(i(n),j(n)), q = pair(i(m),j(m)) */
+            i64 cell = pidx[nn] * np + pidx[mm];
+            double *dst = R + cell * K;
+            for (i64 k = 0; k < K; ++k)
+                val[k] = lt ? rk[k * nnz4 + nn] * rmk[k * nnz4 + mm]
+                            : rk[k * nnz4 + mm] * rmk[k * nnz4 + nn];
+            for (i64 k = 0; k < K; ++k) dst[k] = dst[k] + val[k];
+            if (touched) touched[cell] = 1;
+        }
+    for (i64 e = 0; e < nnz6; ++e) {
+        /* key [i,j,i_p,j_p]: p = pair(i,i_p), q = pair(j,j_p) */
+        i64 p = or_pair_index(k_s, n_b, i6[e], ip6[e]);
+        i64 q = or_pair_index(k_s, n_b, j6[e], jp6[e]);
+        double *d1 = R + (p * np + q) * K;
+        for (i64 k = 0; k < K; ++k) d1[k] = d1[k] + rdk[k * nnz6 + e];
+        double *d2 = R + (q * np + p) * K;            /* key [j,i,j_p,i_p] */
+        for (i64 k = 0; k < K; ++k) d2[k] = d2[k] + rdk[k * nnz6 + e];
+        if (touched) { touched[p * np + q] = 1; touched[q * np + p] = 1; }
+    }
+    free(val);
+    free(pidx);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Angular algebra.  wigner_tools.f90:30-60 wraps GSL's gsl_sf_coupling_3j / */
+/* _6j (GSL is an external C library, absent here; version unpinned,        */
+/* installed as the system gsl-devel by utils/install...sh:44).  Restated    */
+/* with Racah's closed formulas in long double.                              */
+/* ------------------------------------------------------------------------ */
+static long double lfact(i64 n)
+{
+    static long double tab[301];
+    static int init = 0;
+    if (!init) {
+        tab[0] = 1.0L;
+        for (int i = 1; i <= 300; ++i) tab[i] = tab[i - 1] * (long double)i;
+        init = 1;
+    }
+    return (n < 0 || n > 300) ? NAN : tab[n];
+}
+
+static int tri_ok(i64 a, i64 b, i64 c)
+{
+    return a >= 0 && b >= 0 && c >= 0 && c >= llabs(a - b) && c <= a + b;
+}
+
+static long double tri_delta(i64 a, i64 b, i64 c)
+{
+    return lfact(a + b - c) * lfact(a - b + c) * lfact(-a + b + c) / lfact(a + b + c + 1);
+}
+
+/* Wigner 3j with integer angular momenta (the reference only passes
+ * integers: three_j doubles them before calling GSL).                     */
+double or_three_j(i64 ja, i64 jb, i64 jc, i64 ma, i64 mb, i64 mc)
+{
+    if (ma + mb + mc != 0) return 0.0;
+    if (!tri_ok(ja, jb, jc)) return 0.0;
+    if (llabs(ma) > ja || llabs(mb) > jb || llabs(mc) > jc) return 0.0;
+    i64 lo = 0;
+    if (-(jc - jb + ma) > lo) lo = -(jc - jb + ma);
+    if (-(jc - ja - mb) > lo) lo = -(jc - ja - mb);
+    i64 hi = ja + jb - jc;
+    if (ja - ma < hi) hi = ja - ma;
+    if (jb + mb < hi) hi = jb + mb;
+    long double sum = 0.0L;
+    for (i64 t = lo; t <= hi; ++t) {
+        long double den = lfact(t) * lfact(jc - jb + t + ma) * lfact(jc - ja + t - mb) * lfact(ja + jb - jc - t) *
+                          lfact(ja - t - ma) * lfact(jb - t + mb);
+        sum += ((t % 2) ? -1.0L : 1.0L) / den;
+    }
+    long double pre = sqrtl(tri_delta(ja, jb, jc) * lfact(ja + ma) * lfact(ja - ma) * lfact(jb + mb) *
+                            lfact(jb - mb) * lfact(jc + mc) * lfact(jc - mc));
+    i64 ph = ja - jb - mc;
+    long double sgn = (llabs(ph) % 2) ? -1.0L : 1.0L;
+    return (double)(sgn * pre * sum);
+}
+
+double or_six_j(i64 j1, i64 j2, i64 j3, i64 j4, i64 j5, i64 j6)
+{
+    if (!tri_ok(j1, j2, j3) || !tri_ok(j1, j5, j6) || !tri_ok(j4, j2, j6) || !tri_ok(j4, j5, j3)) return 0.0;
+    i64 a1 = j1 + j2 + j3, a2 = j1 + j5 + j6, a3 = j4 + j2 + j6, a4 = j4 + j5 + j3;
+    i64 b1 = j1 + j2 + j4 + j5, b2 = j2 + j3 + j5 + j6, b3 = j3 + j1 + j6 + j4;
+    i64 lo = a1;
+    if (a2 > lo) lo = a2;
+    if (a3 > lo) lo = a3;
+    if (a4 > lo) lo = a4;
+    i64 hi = b1;
+    if (b2 < hi) hi = b2;
+    if (b3 < hi) hi = b3;
+    long double sum = 0.0L;
+    for (i64 t = lo; t <= hi; ++t) {
+        long double num = lfact(t + 1);
+        long double den = lfact(t - a1) * lfact(t - a2) * lfact(t - a3) * lfact(t - a4) * lfact(b1 - t) *
+                          lfact(b2 - t) * lfact(b3 - t);
+        sum += ((t % 2) ? -1.0L : 1.0L) * num / den;
+    }
+    long double pre = sqrtl(tri_delta(j1, j2, j3) * tri_delta(j1, j5, j6) * tri_delta(j4, j2, j6) *
+                            tri_delta(j4, j5, j3));
+    return (double)(pre * sum);
+}
+
+static double isign(i64 e) { return (llabs(e) % 2) ? -1.0 : 1.0; }
+
+/* wigner_tools.f90:107-112 C_red_mat(k,a,b) */
+double or_C_red(i64 k, i64 a, i64 b)
+{
+    return isign(a) * sqrt((double)((2 * a + 1) * (2 * b + 1))) * or_three_j(a, k, b, 0, 0, 0);
+}
+
+/* wigner_tools.f90:126-138 ang_k_LS(k,confs,L) with confs(1)%l = (l1a,l1b),
+ * confs(2)%l = (l2a,l2b).                                                  */
+double or_ang_k_LS(i64 k, i64 l1a, i64 l1b, i64 l2a, i64 l2b, i64 L)
+{
+    if (((l1a + k + l2a) % 2) != 0 || ((l1b + k + l2b) % 2) != 0) return 0.0;
+    return isign(l1b + l2a + L) * or_six_j(l1a, l1b, L, l2b, l2a, k) * or_C_red(k, l1a, l2a) *
+           or_C_red(k, l1b, l2b);
+}
+
+/* ------------------------------------------------------------------------ */
+/* orbital_tools.f90:46-72 consistent + :119-216 count_configs.              */
+/* out: 5 int64 per config: n1, n2, l1, l2, eqv (0/1).  Returns the count,  */
+/* or -1 if cap is exceeded (count is still computed when out == NULL).     */
+/* term_pi: 1 for odd parity.                                                */
+/* ------------------------------------------------------------------------ */
+static int consistent(i64 l1, i64 l2, i64 L, i64 term_pi, int eqv)
+{
+    if (eqv && (L % 2) != 0) return 0;                         /* :54-57 */
+    if (!(llabs(l1 - l2) <= L && L <= l1 + l2)) return 0;      /* :59-63 */
+    int par = ((l1 % 2) != 0) ^ ((l2 % 2) != 0);               /* parity([pi1,pi2]) */
+    if (par != (term_pi != 0)) return 0;                       /* :65-69 */
+    return 1;
+}
+
+i64 or_count_configs(i64 L, i64 term_pi, i64 max_l_1p, i64 n_b, i64 k_spline, i64 max_n_b, i64 n_all_l,
+                     i64 l_2_max, i64 *out, i64 cap)
+{
+    i64 cnt = 0;
+    for (i64 i = 0; i <= max_l_1p; ++i)
+        for (i64 j = 0; j <= i; ++j) {
+            i64 ni0 = (i + 1 < k_spline - 1) ? i + 1 : k_spline - 1;
+            for (i64 n_i = ni0; n_i <= n_b; ++n_i) {
+                if (n_i > n_all_l && j > l_2_max) continue;     /* :165 */
+                i64 nj0 = (j + 1 < k_spline - 1) ? j + 1 : k_spline - 1;
+                i64 nj1;
+                if (j == i) nj1 = n_i < max_n_b ? n_i : max_n_b;   /* :167 */
+                else nj1 = n_b < max_n_b ? n_b : max_n_b;          /* :179 */
+                for (i64 n_j = nj0; n_j <= nj1; ++n_j) {
+                    int eqv = (i == j) && (n_i == n_j);
+                    if (!consistent(i, j, L, term_pi, eqv)) continue;
+                    if (out) {
+                        if (cnt >= cap) return -1;
+                        i64 *o = out + 5 * cnt;
+                        o[0] = n_i; o[1] = n_j; o[2] = i; o[3] = j; o[4] = eqv;
+                    }
+                    ++cnt;
+                }
+            }
+        }
+    return cnt;
+}
+
+/* ------------------------------------------------------------------------ */
+/* One-electron matrices (mat_els.f90:47-118, :294-346) used as inputs of    */
+/* the block assembly.  GL nodes per interval as quad_tools.f90:29-52 maps    */
+/* them: x = (b-a)/2*x0 + (a+b)/2, w = (b-a)/2*w0.  Matrices n_b x n_b,     */
+/* column-major, index (i_b, j_b).                                           */
+/* ------------------------------------------------------------------------ */
+void or_setup_S(i64 k_s, i64 n_knots, const double *t, i64 k_gl, double complex *S)
+{
+    i64 n, n_b, n_iv;
+    or_spline_dims(k_s, n_knots, &n, &n_b, &n_iv);
+    const double *bp = t + (k_s - 1);
+    double *x0 = malloc(sizeof(double) * (size_t)k_gl), *w0 = malloc(sizeof(double) * (size_t)k_gl);
+    or_gauss_legendre(k_gl, x0, w0);
+    for (i64 e = 0; e < n_b * n_b; ++e) S[e] = 0.0;
+    for (i64 j_b = 1; j_b <= n_b; ++j_b)
+        for (i64 i_b = 1; i_b <= n_b; ++i_b) {
+            if (llabs(j_b - i_b) >= k_s) continue;
+            for (i64 i_r = 1; i_r <= n_iv; ++i_r) {
+                if (!(supp(k_s, i_r, i_b + 1) && supp(k_s, i_r, j_b + 1))) continue;
+                double a = bp[i_r - 1], b = bp[i_r];
+                for (i64 q = 0; q < k_gl; ++q) {
+                    double r = (b - a) / 2.0 * x0[q] + (a + b) / 2.0;
+                    double wq = (b - a) / 2.0 * w0[q];
+                    double Bi = or_bvalue_unit(t, n, k_s, i_b + 1, r, 0, i_r);
+                    double Bj = or_bvalue_unit(t, n, k_s, j_b + 1, r, 0, i_r);
+                    S[(j_b - 1) * n_b + (i_b - 1)] += wq * Bi * Bj;
+                }
+            }
+        }
+    free(x0);
+    free(w0);
+}
+
+/* mat_els.f90:294-321 compute_H with potentials.f90:35-43 (hydrogenic) and
+ * CAP_tools.f90:24-33.                                                      */
+void or_setup_H1p(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i64 Z, i64 l, i64 cap_order, double cap_r0,
+                  double cap_eta_re, double cap_eta_im, double complex *H)
+{
+    i64 n, n_b, n_iv;
+    or_spline_dims(k_s, n_knots, &n, &n_b, &n_iv);
+    const double *bp = t + (k_s - 1);
+    double *x0 = malloc(sizeof(double) * (size_t)k_gl), *w0 = malloc(sizeof(double) * (size_t)k_gl);
+    or_gauss_legendre(k_gl, x0, w0);
+    double complex eta = cap_eta_re + I * cap_eta_im;
+    for (i64 e = 0; e < n_b * n_b; ++e) H[e] = 0.0;
+    for (i64 j_b = 1; j_b <= n_b; ++j_b)
+        for (i64 i_b = 1; i_b <= n_b; ++i_b) {
+            if (llabs(j_b - i_b) >= k_s) continue;
+            for (i64 i_r = 1; i_r <= n_iv; ++i_r) {
+                if (!(supp(k_s, i_r, i_b + 1) && supp(k_s, i_r, j_b + 1))) continue;
+                double a = bp[i_r - 1], b = bp[i_r];
+                for (i64 q = 0; q < k_gl; ++q) {
+                    double r = (b - a) / 2.0 * x0[q] + (a + b) / 2.0;
+                    double wq = (b - a) / 2.0 * w0[q];
+                    double Bi = or_bvalue_unit(t, n, k_s, i_b + 1, r, 0, i_r);
+                    double Bj = or_bvalue_unit(t, n, k_s, j_b + 1, r, 0, i_r);
+                    double DBj = or_bvalue_unit(t, n, k_s, j_b + 1, r, 2, i_r);
+                    double V = 0.5 * (double)l * (double)(l + 1) / (r * r) - (double)Z / r;
+                    double complex Vc = 0.0;
+                    if (r >= cap_r0) Vc = (0.0 - 1.0 * I) * eta * ipow(r - cap_r0, cap_order);
+                    H[(j_b - 1) * n_b + (i_b - 1)] += wq * (-0.5 * Bi * DBj + (V + Vc) * Bi * Bj);
+                }
+            }
+        }
+    free(x0);
+    free(w0);
+}
+
+/* ------------------------------------------------------------------------ */
+/* hamiltonian.f90:106-283 construct_block_tensor (+ mat_els.f90:552-571     */
+/* r_12_tens, :608-633 c_mat_neq_tens, :664-678 S_mat_neq, :697-715          */
+/* H_1p_neq) and :348-416 count_nnz.                                         */
+/*                                                                           */
+/* configs: 5 int64 per config (n1,n2,l1,l2,eqv).  R: dense banded tensor    */
+/* as written by or_rk_dense.  Hl: (max_l+1) matrices n_b x n_b col-major.  */
+/* S: n_b x n_b col-major.  CSR outputs are 1-based like the reference.     */
+/* When the output pointers are NULL only the counts are produced:           */
+/*   nnz[0], nnz[1]  = H and S entries created by the fill predicate;        */
+/*   ref_cnt[0], [1] = what the reference's count_nnz returns (it can be     */
+/*                     smaller than the fill; the reference then writes past */
+/*                     its allocation).                                      */
+/* ------------------------------------------------------------------------ */
+static double r12(i64 K, const double *vals, i64 L, i64 l1a, i64 l1b, i64 l2a, i64 l2b)
+{
+    double res = 0.0;
+    for (i64 k = 0; k < K; ++k) {                                  /* :566-570 */
+        double ang = or_ang_k_LS(k, l1a, l1b, l2a, l2b, L);
+        if (fabs(ang) < 5e-16) continue;
+        res = res + vals[k] * ang;
+    }
+    return res;
+}
+
+i64 or_construct_block(i64 k_s, i64 n_b, i64 L, i64 max_k, i64 full, i64 n_config, const i64 *cf, const double *R,
+                       const double complex *Hl, const double complex *S, i64 *ip_H, i64 *ix_H, double complex *dH,
+                       i64 *ip_S, i64 *ix_S, double complex *dS, i64 cap_H, i64 cap_S, i64 *nnz, i64 *ref_cnt)
+{
+    i64 K = max_k + 1;
+    i64 np = or_count_pairs(k_s, n_b);
+    i64 ptrH = 0, ptrS = 0;                  /* 0-based cursors */
+    i64 cntH = 0, cntS = 0;                  /* count_nnz (:348-416) */
+    int fill = (ip_H != NULL);
+    if (fill) { ip_H[0] = 1; ip_S[0] = 1; }
+#define GETV(a, b, c, d) (R + ((or_pair_index(k_s, n_b, (a), (c)) * np + or_pair_index(k_s, n_b, (b), (d))) * K))
+#define HL(l, r, c) Hl[(l) * n_b * n_b + ((c) - 1) * n_b + ((r) - 1)]
+#define SM(r, c) S[((c) - 1) * n_b + ((r) - 1)]
+    for (i64 i = 0; i < n_config; ++i) {
+        i64 n_a = cf[5 * i + 0], n_bb = cf[5 * i + 1], l_a = cf[5 * i + 2], l_b = cf[5 * i + 3];
+        i64 low = full ? 0 : i;
+        for (i64 j = low; j < n_config; ++j) {
+            i64 n_c = cf[5 * j + 0], n_d = cf[5 * j + 1], l_c = cf[5 * j + 2], l_d = cf[5 * j + 3];
+            int support = llabs(n_a - n_c) < k_s && llabs(n_bb - n_d) < k_s;
+            int support_ex = llabs(n_a - n_d) < k_s && llabs(n_bb - n_c) < k_s;
+            /* reference count_nnz predicate */
+            if (support) {
+                if (l_a == l_c && l_b == l_d) ++cntS;
+                for (i64 k = 0; k <= max_k; ++k)
+                    if (fabs(or_ang_k_LS(k, l_a, l_b, l_c, l_d, L)) > 5e-15) { ++cntH; break; }
+            } else if (support_ex) {
+                if (l_a == l_d && l_b == l_c) ++cntS;
+                for (i64 k = 0; k <= max_k; ++k)
+                    if (fabs(or_ang_k_LS(k, l_a, l_b, l_d, l_c, L)) > 5e-15) { ++cntH; break; }
+            }
+            if (!(support || support_ex)) continue;
+            int r12_allowed = 0;
+            for (i64 k = 0; k <= max_k; ++k) {                         /* :171-178 */
+                double ang = or_ang_k_LS(k, l_a, l_b, l_c, l_d, L);
+                double ang_ex = or_ang_k_LS(k, l_a, l_b, l_d, l_c, L);
+                if ((fabs(ang) > 5e-15 && support) || (fabs(ang_ex) > 5e-15 && support_ex)) {
+                    r12_allowed = 1;
+                    break;
+                }
+            }
+            int l_eq = (l_a == l_c) && (l_b == l_d);
+            int l_eq_ex = (l_a == l_d) && (l_b == l_c);
+            int one_el = (support && l_eq) || (support_ex && l_eq_ex);
+            int entry = r12_allowed || l_eq || l_eq_ex;
+            if (fill) {
+                double complex h = 0.0;
+                if (r12_allowed) {                                     /* c_mat_neq_tens */
+                    double res = 0.0;
+                    if (support) res = res + r12(K, GETV(n_a, n_bb, n_c, n_d), L, l_a, l_b, l_c, l_d);
+                    if (support_ex)
+                        res = res + isign(l_c + l_d + L) * r12(K, GETV(n_a, n_bb, n_d, n_c), L, l_a, l_b, l_d, l_c);
+                    h = h + res;
+                }
+                if (one_el) {
+                    double complex h1 = 0.0, s1 = 0.0;                 /* H_1p_neq, S_mat_neq */
+                    if (l_eq) {
+                        h1 = h1 + (HL(l_a, n_a, n_c) * SM(n_bb, n_d) + HL(l_b, n_bb, n_d) * SM(n_a, n_c));
+                        s1 = s1 + SM(n_a, n_c) * SM(n_bb, n_d);
+                    }
+                    if (l_eq_ex) {
+                        h1 = h1 + (HL(l_a, n_a, n_d) * SM(n_bb, n_c) + HL(l_b, n_bb, n_c) * SM(n_a, n_d)) *
+                                      isign(L + l_c + l_d);
+                        s1 = s1 + isign(L + l_c + l_d) * SM(n_a, n_d) * SM(n_bb, n_c);
+                    }
+                    h = h + h1;
+                    if (ptrS >= cap_S) return -2;
+                    dS[ptrS] = s1;
+                    ix_S[ptrS] = j + 1;
+                }
+                if (entry) {
+                    if (ptrH >= cap_H) return -1;
+                    dH[ptrH] = h;
+                    ix_H[ptrH] = j + 1;
+                }
+            }
+            if (one_el) ++ptrS;
+            if (entry) ++ptrH;
+        }
+        if (fill) { ip_H[i + 1] = ptrH + 1; ip_S[i + 1] = ptrS + 1; }
+    }
+#undef GETV
+#undef HL
+#undef SM
+    if (nnz) { nnz[0] = ptrH; nnz[1] = ptrS; }
+    if (ref_cnt) { ref_cnt[0] = cntH; ref_cnt[1] = cntS; }
+    return 0;
+}

@@ -1,0 +1,166 @@
+"""GPU parity of the R^k path (libbs2e.so through its C ABI) against the oracle.
+
+Bar: index arrays and key sets bit-exact; values within 1e-12 relative,
+element by element (every primitive and every R^k value is >= 0: B-splines
+are non-negative, GL weights positive, the kernel positive, so the relative
+error is well conditioned).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import bs2e  # noqa: E402
+from bs2e import capi  # noqa: E402
+from bs2e.workloads import WORKLOADS  # noqa: E402
+
+TOL = 1e-12
+
+
+def max_rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    assert a.shape == b.shape
+    d = np.abs(a - b)
+    s = np.abs(b)
+    nz = s > 0
+    rel = float((d[nz] / s[nz]).max()) if nz.any() else 0.0
+    zero_part = float(np.abs(a[~nz]).max()) if (~nz).any() else 0.0
+    return rel, zero_part
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return bs2e.Context(0)
+
+
+def knots(w):
+    return bs2e.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+
+
+def gpu_containers(w, t, ctx):
+    spl = bs2e.b_spline(w.k, t)
+    r_k = bs2e.sparse_4d().init(w.max_k, spl)
+    r_m_k = bs2e.sparse_4d().init(w.max_k, spl)
+    r_d_k = bs2e.sparse_6d().init(w.max_k, spl)
+    bs2e.setup_Slater_integrals(spl, w.max_k, w.k_GL, r_k, r_m_k, r_d_k, ctx=ctx)
+    return spl, r_k, r_m_k, r_d_k
+
+
+@pytest.mark.parametrize("name", ["tiny", "t0", "c1", "c2"])
+def test_setup_slater_integrals_matches_oracle(name, ctx, oracle):
+    w = WORKLOADS[name]
+    t = knots(w)
+    _, r_k, r_m_k, r_d_k = gpu_containers(w, t, ctx)
+    off = oracle.slater_off_diag(w.k, t, w.k_GL, w.max_k)
+    dg = oracle.slater_diag(w.k, t, w.k_GL, w.max_k)
+    assert r_k.nnz == off["nnz"] and r_d_k.nnz == dg["nnz"]
+    for a, b in ((r_k.iv, off["iv"]), (r_k.i, off["i"]), (r_k.j, off["j"]), (r_m_k.iv, off["iv"])):
+        assert np.array_equal(a, b)
+    for key in ("iv", "i", "ip", "j", "jp"):
+        mine = {"ip": r_d_k.i_p, "jp": r_d_k.j_p}.get(key, getattr(r_d_k, key, None))
+        assert np.array_equal(mine, dg[key]), key
+    for got, ref in ((r_k.data, off["rk"]), (r_m_k.data, off["rmk"]), (r_d_k.data, dg["rdk"])):
+        rel, zp = max_rel(got, ref)
+        assert rel <= TOL and zp == 0.0, (name, rel, zp)
+
+
+@pytest.mark.parametrize("name", ["tiny", "t0", "c1", "c2"])
+def test_rk_tensor_matches_oracle(name, ctx, oracle):
+    w = WORKLOADS[name]
+    t = knots(w)
+    rk = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx)
+    R = rk.to_host()
+    off = oracle.slater_off_diag(w.k, t, w.k_GL, w.max_k)
+    dg = oracle.slater_diag(w.k, t, w.k_GL, w.max_k)
+    n_b = oracle.dims(w.k, t)[1]
+    Rref, touched = oracle.rk_dense(w.k, n_b, w.max_k, off, dg, want_touched=True)
+    assert touched.all()          # every key of count_nnz_R_k is set by the reference
+    assert R.shape == Rref.shape == (oracle.n_pairs(w.k, n_b),) * 2 + (w.max_k + 1,)
+    rel, zp = max_rel(R, Rref)
+    assert rel <= TOL and zp == 0.0, (name, rel, zp)
+    assert (R > 0).all()
+    rk.release()
+
+
+@pytest.mark.parametrize("name", ["tiny", "c1"])
+def test_compute_R_K_map_from_containers(name, ctx, oracle):
+    """compute_R_K_map drop-in: the tensor is built from the caller's containers."""
+    w = WORKLOADS[name]
+    t = knots(w)
+    spl, r_k, r_m_k, r_d_k = gpu_containers(w, t, ctx)
+    R = bs2e.Nd_DOK()
+    bs2e.compute_R_K_map(r_d_k, r_k, r_m_k, spl, w.max_k, R, ctx=ctx)
+    got = R.tensor.to_host()
+    off = oracle.slater_off_diag(w.k, t, w.k_GL, w.max_k)
+    dg = oracle.slater_diag(w.k, t, w.k_GL, w.max_k)
+    Rref = oracle.rk_dense(w.k, spl.n_b, w.max_k, off, dg)
+    rel, zp = max_rel(got, Rref)
+    assert rel <= TOL and zp == 0.0
+    # and from the oracle's own containers (values the caller holds, any source)
+    r_k.data[:] = off["rk"]
+    r_m_k.data[:] = off["rmk"]
+    r_d_k.data[:] = dg["rdk"]
+    R2 = bs2e.compute_R_K_map(r_d_k, r_k, r_m_k, spl, w.max_k, bs2e.Nd_DOK(), ctx=ctx)
+    rel, zp = max_rel(R2.tensor.to_host(), Rref)
+    assert rel <= TOL and zp == 0.0
+
+
+def test_compute_R_K_map_rejects_foreign_indices(ctx):
+    w = WORKLOADS["tiny"]
+    t = knots(w)
+    spl, r_k, r_m_k, r_d_k = gpu_containers(w, t, ctx)
+    r_k.j[3] = r_k.i[3] + w.k + 2      # outside the band |i-j| < k
+    with pytest.raises(bs2e.Bs2eError) as e:
+        bs2e.compute_R_K_map(r_d_k, r_k, r_m_k, spl, w.max_k, bs2e.Nd_DOK(), ctx=ctx)
+    assert e.value.code == capi.BS2E_ERANGE
+
+
+def test_get_val_and_pair_index(ctx, oracle):
+    w = WORKLOADS["c1"]
+    t = knots(w)
+    rk = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx)
+    full = rk.to_host()
+    pairs = oracle.pair_table(w.k, rk.n_b)
+    for p, (a, c) in enumerate(pairs):
+        assert rk.pair_index(a, c) == p
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        p, q = rng.integers(0, rk.n_pairs, size=2)
+        (a, c), (b, d) = pairs[p], pairs[q]
+        np.testing.assert_array_equal(rk.get_val([a, b, c, d]), full[p, q])
+    with pytest.raises(bs2e.Bs2eError) as e:
+        rk.get_val([1, 1, 1 + w.k, 1])
+    assert e.value.code == capi.BS2E_ERANGE
+    # symmetric integrals: R(a,b,c,d) = R(b,a,d,c) = R(c,b,a,d) (exact math, ~1 ulp here)
+    perm = np.array([rk.pair_index(c, a) for a, c in pairs])
+    assert max_rel(full, full.transpose(1, 0, 2))[0] <= 1e-14
+    assert max_rel(full, full[perm])[0] <= 1e-14
+
+
+def test_stage_times_and_repeat_determinism(ctx):
+    w = WORKLOADS["c1"]
+    rk = bs2e.RkTensor.build(w.k, knots(w), w.k_GL, w.max_k, ctx=ctx, compute=False)
+    rk.set_profiling(True)
+    rk.compute()
+    ms = rk.stage_times()
+    assert (ms > 0).all() and ms.sum() < 1000.0
+    a = rk.to_host()
+    rk.compute()
+    b = rk.to_host()
+    assert np.array_equal(a, b)       # bit-identical across builds
+
+
+def test_invalid_arguments(ctx):
+    w = WORKLOADS["tiny"]
+    t = knots(w)
+    with pytest.raises(bs2e.Bs2eError) as e:
+        bs2e.RkTensor.build(21, t, w.k_GL, w.max_k, ctx=ctx)      # order > 20 (BVALUE_D limit)
+    assert e.value.code == capi.BS2E_EINVAL
+    with pytest.raises(bs2e.Bs2eError):
+        bs2e.RkTensor.build(w.k, t[::-1].copy(), w.k_GL, w.max_k, ctx=ctx)
+    h = C.c_void_p()
+    assert capi.lib().bs2e_rk_create(ctx.handle, w.k, len(t), capi.ptr(t), w.k_GL, w.max_k, 99, C.byref(h)) \
+        == capi.BS2E_EINVAL
