@@ -163,10 +163,10 @@ class RkTensor:
     def __init__(self, handle, ctx: Context):
         self.handle = handle
         self.ctx = ctx
-        info = np.zeros(10, np.int64)
-        check(capi.lib().bs2e_rk_info(handle, ptr(info, C.c_int64), 10), ctx.handle)
+        info = np.zeros(11, np.int64)
+        check(capi.lib().bs2e_rk_info(handle, ptr(info, C.c_int64), 11), ctx.handle)
         (self.n_b, self.n_pairs, self.K, self.nnz4, self.nnz6, self.n_iv, self.k, self.k_GL, self.U,
-         self.n_knots) = (int(x) for x in info)
+         self.n_knots, self.near_elems) = (int(x) for x in info)
         self.max_k = self.K - 1
 
     @classmethod
@@ -215,8 +215,10 @@ class RkTensor:
         check(capi.lib().bs2e_rk_set_profiling(self.handle, int(on)), self.ctx.handle)
 
     def stage_times(self):
-        ms = np.zeros(4)
-        check(capi.lib().bs2e_rk_stage_times(self.handle, ptr(ms), 4), self.ctx.handle)
+        """Timeline of the last compute, ms since its start: [moments + windows done,
+        rank-one fill done, diagonal cells done, near ranges done, end]."""
+        ms = np.zeros(5)
+        check(capi.lib().bs2e_rk_stage_times(self.handle, ptr(ms), 5), self.ctx.handle)
         return ms
 
     def release(self):

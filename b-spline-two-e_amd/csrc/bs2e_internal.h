@@ -10,16 +10,14 @@
 //     count_nnz_R_k (sparse_array_tools.f90:347-366): c outer, a inner.
 //
 // Device layouts (all FP64, row-major, 0-based):
-//   Bo [n_iv][k_gl][k_s]            spline values at the outer GL points
-//   Bi [n_iv][k_gl][k_gl][k_s]      spline values at the inner GL points of the
-//                                   triangle rule (mat_els.f90:469-475)
-//   ro [n_iv][k_gl]                 outer radii
+//   Pl, Ql [n_iv][U][K]             local pair moments P^k, Q^k of the U =
+//                                   k_s(k_s+1)/2 unordered local spline pairs
 //   Pw, Qw, PP, QS [n_pairs][k_s][K]  per-pair window moments P^k, Q^k over the
 //                                   pair's support window, their prefix (P) and
 //                                   suffix (Q) sums over the window
 //   Ptot, Qtot [n_pairs][K]         window totals
-//   D  [n_iv][K][U][U]              diagonal-cell integrals, U = k_s(k_s+1)/2
-//                                   symmetric local pairs (outer, inner)
+//   D  [n_iv][U][U][K]              diagonal-cell integrals (outer local pair,
+//                                   inner local pair, multipole)
 //   R  [n_pairs][n_pairs][K]        the R^k tensor; key [a,b,c,d] lives at
 //                                   (pair(a,c)*n_pairs + pair(b,d))*K + k
 #pragma once
@@ -50,6 +48,17 @@ struct DiagEntry {
     int64_t ptr;                // 0-based first position in sparse_6d order
 };
 
+// Per tensor row p: the column pairs q whose windows overlap W_p are exactly
+// the q in [qlo, qhi] that overlap; every q outside is a rank-one element.
+struct NearRange {
+    int32_t qlo, qhi;
+};
+
+// One near workgroup: elements [e0, e1) (e = q*K + k) of row p.
+struct NearItem {
+    int32_t p, e0, e1, pad;
+};
+
 // Host-side description of one basis (knots + quadrature + multipoles).
 struct Plan {
     int64_t k_s = 0, n_knots = 0, n = 0, n_b = 0, n_iv = 0;
@@ -60,11 +69,14 @@ struct Plan {
     std::vector<int64_t> off4;          // per pair: first sparse_4d position
     std::vector<int32_t> c_first;       // first pair index with column c (1-based c, size n_b+2)
     std::vector<DiagEntry> diag;        // sparse_6d order (built on demand)
+    std::vector<NearRange> near;        // per row (built with the tensor)
+    std::vector<NearItem> items;        // near workgroups of the tensor kernel
 };
 
 // Builds the plan; returns an error message or empty string.
 std::string make_plan(int64_t k_s, int64_t n_knots, const double* knots, int64_t k_gl, int64_t max_k, Plan& p);
 void build_diag_entries(Plan& p);
+void build_near_ranges(Plan& p);
 
 // Host numerics shared by the C-ABI (setup, not the hot path).
 void gauss_legendre(int64_t N, double* x, double* w);
@@ -77,29 +89,34 @@ double ang_k_LS(int64_t k, int64_t l1a, int64_t l1b, int64_t l2a, int64_t l2b, i
 struct DevTables {
     double *t = nullptr, *bp = nullptr, *x = nullptr, *w = nullptr;
     PairInfo* pairs = nullptr;
-    double *Bo = nullptr, *Bi = nullptr, *ro = nullptr;
+    double *Pl = nullptr, *Ql = nullptr;
     double *Pw = nullptr, *Qw = nullptr, *PP = nullptr, *QS = nullptr, *Ptot = nullptr, *Qtot = nullptr;
     double* D = nullptr;
+    NearRange* near = nullptr;
+    NearItem* items = nullptr;
 };
 
 // Kernel launchers (each .hip file launches its own kernels).
-hipError_t launch_bspline_tables(const Plan& p, const DevTables& d, hipStream_t s);          // K1
-hipError_t launch_pair_moments(const Plan& p, const DevTables& d, hipStream_t s);            // K2
-hipError_t launch_diag_cells(const Plan& p, const DevTables& d, hipStream_t s);              // K4
-hipError_t launch_tensor_fill(const Plan& p, const DevTables& d, double* R, hipStream_t s);  // K3
+hipError_t launch_outer_moments(const Plan& p, const DevTables& d, hipStream_t s);           // Pl, Ql
+hipError_t launch_diag_cells(const Plan& p, const DevTables& d, hipStream_t s);              // D
+hipError_t launch_pair_windows(const Plan& p, const DevTables& d, hipStream_t s);            // Pw/Qw + sums
+hipError_t launch_tensor_far(const Plan& p, const DevTables& d, double* R, hipStream_t s);   // rank-one part
+hipError_t launch_tensor_near(const Plan& p, const DevTables& d, double* R, hipStream_t s);  // near ranges
+size_t outer_lds_bytes(int64_t ks, int64_t kgl, int64_t K);
+size_t inner_lds_bytes(int64_t ks, int64_t kgl, int64_t kc);
+int inner_kc(int64_t ks, int64_t kgl, int64_t K);   // multipoles per diagonal-cell workgroup (0: unsupported)
 hipError_t launch_scatter_4d(const Plan& p, const DevTables& d, const int64_t* d_off4, double* rk, double* rmk,
                              int64_t* iv, int64_t* i, int64_t* j, hipStream_t s);
 hipError_t launch_scatter_6d(const Plan& p, const DevTables& d, const DiagEntry* d_diag, int64_t n_diag, double* rdk,
                              int64_t* iv, int64_t* i, int64_t* ip, int64_t* j, int64_t* jp, hipStream_t s);
 
-hipError_t launch_window_sums(const Plan& p, const DevTables& d, hipStream_t s);               // K2b
+hipError_t launch_window_sums(const Plan& p, const DevTables& d, hipStream_t s);             // sums of Pw/Qw
 hipError_t launch_fill(double* ptr, int64_t n, double v, hipStream_t s);
 hipError_t launch_gather_4d(const Plan& p, const DevTables& d, const int32_t* d_c_first, const double* rk,
                             const double* rmk, const int64_t* iv, const int64_t* i, const int64_t* j, int* bad,
                             hipStream_t s);
 hipError_t launch_gather_6d(const Plan& p, const DevTables& d, const double* rdk, const int64_t* iv, const int64_t* i,
                             const int64_t* ip, const int64_t* j, const int64_t* jp, int* bad, hipStream_t s);
-size_t diag_cells_lds_bytes(const Plan& p);
 
 // Symmetric local pair index of local splines (ta, tc) on one interval.
 __host__ __device__ inline int sym_local(int ta, int tc, int ks)

@@ -12,14 +12,11 @@
 
 using namespace bs2e;
 
-namespace bs2e {
-size_t diag_cells_lds_bytes(const Plan& p);
-}
-
 struct bs2e_ctx {
     int device = 0;
     hipStream_t own = nullptr;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // the build's stream (own, or the caller's)
+    hipStream_t side = nullptr;     // diagonal cells + near ranges, concurrent with the rank-one fill
     std::string err;
 };
 
@@ -33,7 +30,8 @@ struct bs2e_rk {
     std::vector<double> mirror;
     bool mirror_valid = false;
     bool profiling = false;
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   // profiling
+    hipEvent_t fork = nullptr, windows = nullptr, join = nullptr;                  // stream ordering
     bool computed = false;
     bool from_primitives = false;   // tables gathered from caller containers: compute runs K3 only
 };
@@ -84,8 +82,11 @@ int64_t bs2e_create(bs2e_ctx** out, int64_t device)
     if (!c) return fail(nullptr, BS2E_ENOMEM, "bs2e_create: host allocation failed");
     c->device = (int)device;
     e = hipSetDevice(c->device);
+    // both streams at the default priority
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     if (e != hipSuccess) {
+        if (c->own) (void)hipStreamDestroy(c->own);
         delete c;
         return fail(nullptr, BS2E_EDEVICE, std::string("bs2e_create: ") + hipGetErrorString(e));
     }
@@ -98,10 +99,11 @@ int64_t bs2e_destroy(bs2e_ctx* ctx)
 {
     if (!ctx) return BS2E_OK;
     (void)hipSetDevice(ctx->device);
-    if (ctx->own) {
-        (void)hipStreamSynchronize(ctx->own);
-        (void)hipStreamDestroy(ctx->own);
-    }
+    for (hipStream_t s : {ctx->own, ctx->side})
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
     delete ctx;
     return BS2E_OK;
 }
@@ -168,9 +170,8 @@ static void free_tables(bs2e_rk* rk)
     dev_free(d.x);
     dev_free(d.w);
     dev_free(d.pairs);
-    dev_free(d.Bo);
-    dev_free(d.Bi);
-    dev_free(d.ro);
+    dev_free(d.Pl);
+    dev_free(d.Ql);
     dev_free(d.Pw);
     dev_free(d.Qw);
     dev_free(d.PP);
@@ -178,16 +179,32 @@ static void free_tables(bs2e_rk* rk)
     dev_free(d.Ptot);
     dev_free(d.Qtot);
     dev_free(d.D);
+    dev_free(d.near);
+    dev_free(d.items);
     dev_free(rk->R);
     for (auto& e : rk->ev)
         if (e) {
             (void)hipEventDestroy(e);
             e = nullptr;
         }
+    for (hipEvent_t* e : {&rk->fork, &rk->windows, &rk->join})
+        if (*e) {
+            (void)hipEventDestroy(*e);
+            *e = nullptr;
+        }
+}
+
+static int64_t create_events(bs2e_ctx* ctx, bs2e_rk* rk)
+{
+    for (auto& e : rk->ev) HIP_TRY(ctx, hipEventCreate(&e));
+    for (hipEvent_t* e : {&rk->fork, &rk->windows, &rk->join})
+        HIP_TRY(ctx, hipEventCreateWithFlags(e, hipEventDisableTiming));
+    return BS2E_OK;
 }
 
 static int64_t alloc_tables(bs2e_ctx* ctx, bs2e_rk* rk, bool with_tensor)
 {
+    if (with_tensor) build_near_ranges(rk->plan);
     const Plan& p = rk->plan;
     DevTables& d = rk->d;
     const size_t slots = (size_t)p.n_pairs * p.k_s * p.K;
@@ -196,9 +213,8 @@ static int64_t alloc_tables(bs2e_ctx* ctx, bs2e_rk* rk, bool with_tensor)
     HIP_TRY(ctx, dev_alloc(&d.x, p.k_gl));
     HIP_TRY(ctx, dev_alloc(&d.w, p.k_gl));
     HIP_TRY(ctx, dev_alloc(&d.pairs, p.n_pairs));
-    HIP_TRY(ctx, dev_alloc(&d.Bo, (size_t)p.n_iv * p.k_gl * p.k_s));
-    HIP_TRY(ctx, dev_alloc(&d.Bi, (size_t)p.n_iv * p.k_gl * p.k_gl * p.k_s));
-    HIP_TRY(ctx, dev_alloc(&d.ro, (size_t)p.n_iv * p.k_gl));
+    HIP_TRY(ctx, dev_alloc(&d.Pl, (size_t)p.n_iv * p.U * p.K));
+    HIP_TRY(ctx, dev_alloc(&d.Ql, (size_t)p.n_iv * p.U * p.K));
     HIP_TRY(ctx, dev_alloc(&d.Pw, slots));
     HIP_TRY(ctx, dev_alloc(&d.Qw, slots));
     HIP_TRY(ctx, dev_alloc(&d.PP, slots));
@@ -209,8 +225,15 @@ static int64_t alloc_tables(bs2e_ctx* ctx, bs2e_rk* rk, bool with_tensor)
     if (with_tensor) {
         rk->R_elems = (size_t)p.n_pairs * p.n_pairs * p.K;
         HIP_TRY(ctx, dev_alloc(&rk->R, rk->R_elems));
+        HIP_TRY(ctx, dev_alloc(&d.near, p.near.size()));
+        HIP_TRY(ctx, dev_alloc(&d.items, p.items.size()));
     }
     hipStream_t s = ctx->stream;
+    if (with_tensor) {
+        HIP_TRY(ctx, hipMemcpyAsync(d.near, p.near.data(), sizeof(NearRange) * p.near.size(), hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx,
+                hipMemcpyAsync(d.items, p.items.data(), sizeof(NearItem) * p.items.size(), hipMemcpyHostToDevice, s));
+    }
     HIP_TRY(ctx, hipMemcpyAsync(d.t, p.t.data(), sizeof(double) * p.n_knots, hipMemcpyHostToDevice, s));
     HIP_TRY(ctx, hipMemcpyAsync(d.bp, p.bp.data(), sizeof(double) * (p.n_iv + 1), hipMemcpyHostToDevice, s));
     HIP_TRY(ctx, hipMemcpyAsync(d.x, p.x.data(), sizeof(double) * p.k_gl, hipMemcpyHostToDevice, s));
@@ -220,26 +243,58 @@ static int64_t alloc_tables(bs2e_ctx* ctx, bs2e_rk* rk, bool with_tensor)
     return BS2E_OK;
 }
 
-static int64_t check_plan_limits(bs2e_ctx* ctx, const Plan& p)
+static int64_t check_plan_limits(bs2e_ctx* ctx, const Plan& p, bool needs_primitives)
 {
-    if (diag_cells_lds_bytes(p) > 64 * 1024)
+    if (needs_primitives &&
+        (p.k_s > 16 || inner_kc(p.k_s, p.k_gl, p.K) == 0 || outer_lds_bytes(p.k_s, p.k_gl, p.K) > 64 * 1024))
         return fail(ctx, BS2E_EINVAL,
-                    "spline order / k_GL combination needs more than 64 KiB of LDS in the diagonal-cell kernel");
+                    "the per-interval primitive kernels support spline orders <= 16 whose tables fit 64 KiB of LDS "
+                    "(about k_GL^2 * k + k_GL * k^2 doubles; e.g. k <= 12 with k_GL = k+6)");
     if (p.n_pairs * p.K >= ((int64_t)1 << 31)) return fail(ctx, BS2E_EINVAL, "tensor row too long (n_pairs*K >= 2^31)");
     return BS2E_OK;
 }
 
-// K1, K2, K4 (the primitives) on the context stream.
-static int64_t run_primitives(bs2e_ctx* ctx, bs2e_rk* rk)
+// Primitives on one stream, in order (for callers that want the containers).
+static int64_t run_primitives_serial(bs2e_ctx* ctx, bs2e_rk* rk)
 {
     hipStream_t s = ctx->stream;
-    if (rk->profiling) HIP_TRY(ctx, hipEventRecord(rk->ev[0], s));
-    HIP_TRY(ctx, launch_bspline_tables(rk->plan, rk->d, s));
-    if (rk->profiling) HIP_TRY(ctx, hipEventRecord(rk->ev[1], s));
-    HIP_TRY(ctx, launch_pair_moments(rk->plan, rk->d, s));
-    if (rk->profiling) HIP_TRY(ctx, hipEventRecord(rk->ev[2], s));
+    HIP_TRY(ctx, launch_outer_moments(rk->plan, rk->d, s));
     HIP_TRY(ctx, launch_diag_cells(rk->plan, rk->d, s));
-    if (rk->profiling) HIP_TRY(ctx, hipEventRecord(rk->ev[3], s));
+    HIP_TRY(ctx, launch_pair_windows(rk->plan, rk->d, s));
+    return BS2E_OK;
+}
+
+// The full build.  Main stream: local moments -> pair windows -> rank-one
+// fill.  Side stream (forked at the start): diagonal cells -> (after the
+// pair windows) near ranges.  The main stream waits for the side stream at
+// the end, so the build is one unit of work on the caller's stream.
+// Profiling events, all relative to ev[0]: ev[1] windows done, ev[2] far
+// fill done, ev[3] diagonal cells done, ev[4] near ranges done, ev[5] end.
+static int64_t run_build(bs2e_ctx* ctx, bs2e_rk* rk)
+{
+    hipStream_t m = ctx->stream, s = ctx->side;
+    const bool prof = rk->profiling;
+    if (prof) HIP_TRY(ctx, hipEventRecord(rk->ev[0], m));
+    // The short critical-path kernel is enqueued before the fork so that its
+    // workgroups are dispatched ahead of the diagonal-cell workgroups.
+    if (!rk->from_primitives) HIP_TRY(ctx, launch_outer_moments(rk->plan, rk->d, m));
+    HIP_TRY(ctx, hipEventRecord(rk->fork, m));
+    HIP_TRY(ctx, hipStreamWaitEvent(s, rk->fork, 0));
+    if (!rk->from_primitives) {
+        HIP_TRY(ctx, launch_diag_cells(rk->plan, rk->d, s));
+        HIP_TRY(ctx, launch_pair_windows(rk->plan, rk->d, m));
+    }
+    if (prof) HIP_TRY(ctx, hipEventRecord(rk->ev[3], s));
+    if (prof) HIP_TRY(ctx, hipEventRecord(rk->ev[1], m));
+    HIP_TRY(ctx, hipEventRecord(rk->windows, m));
+    HIP_TRY(ctx, hipStreamWaitEvent(s, rk->windows, 0));
+    HIP_TRY(ctx, launch_tensor_near(rk->plan, rk->d, rk->R, s));
+    if (prof) HIP_TRY(ctx, hipEventRecord(rk->ev[4], s));
+    HIP_TRY(ctx, hipEventRecord(rk->join, s));
+    HIP_TRY(ctx, launch_tensor_far(rk->plan, rk->d, rk->R, m));
+    if (prof) HIP_TRY(ctx, hipEventRecord(rk->ev[2], m));
+    HIP_TRY(ctx, hipStreamWaitEvent(m, rk->join, 0));
+    if (prof) HIP_TRY(ctx, hipEventRecord(rk->ev[5], m));
     return BS2E_OK;
 }
 
@@ -261,7 +316,7 @@ int64_t bs2e_slater_primitives(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const 
         return fail(ctx, BS2E_EINVAL, "bs2e_slater_primitives: nnz4/nnz6 differ from count_nnz_4d/6d for this basis");
     if (!rk || !rmk || !iv4 || !i4 || !j4 || !rdk || !iv6 || !i6 || !ip6 || !j6 || !jp6)
         return fail(ctx, BS2E_EINVAL, "bs2e_slater_primitives: NULL output array");
-    int64_t rc = check_plan_limits(ctx, p);
+    int64_t rc = check_plan_limits(ctx, p, true);
     if (rc) return rc;
     build_diag_entries(h.plan);
     struct Guard {
@@ -297,7 +352,7 @@ int64_t bs2e_slater_primitives(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const 
     hipStream_t s = ctx->stream;
     HIP_TRY(ctx, hipMemcpyAsync(d_off4, p.off4.data(), sizeof(int64_t) * p.n_pairs, hipMemcpyHostToDevice, s));
     HIP_TRY(ctx, hipMemcpyAsync(d_diag, p.diag.data(), sizeof(DiagEntry) * p.diag.size(), hipMemcpyHostToDevice, s));
-    rc = run_primitives(ctx, &h);
+    rc = run_primitives_serial(ctx, &h);
     if (rc) return rc;
     HIP_TRY(ctx, launch_scatter_4d(p, h.d, d_off4, d_rk, d_rmk, d_i4, d_i4 + nnz4, d_i4 + 2 * nnz4, s));
     HIP_TRY(ctx, launch_scatter_6d(p, h.d, d_diag, (int64_t)p.diag.size(), d_rdk, d_i6, d_i6 + nnz6,
@@ -328,12 +383,10 @@ int64_t bs2e_rk_create(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const double* 
     rk->ctx = ctx;
     rk->precision = precision;
     std::string msg = make_plan(k, n_knots, knots, k_gl, max_k, rk->plan);
-    int64_t rc = msg.empty() ? check_plan_limits(ctx, rk->plan) : fail(ctx, BS2E_EINVAL, "bs2e_rk_create: " + msg);
+    int64_t rc =
+        msg.empty() ? check_plan_limits(ctx, rk->plan, true) : fail(ctx, BS2E_EINVAL, "bs2e_rk_create: " + msg);
     if (!rc) rc = alloc_tables(ctx, rk, true);
-    for (int i = 0; !rc && i < 5; ++i) {
-        hipError_t e = hipEventCreate(&rk->ev[i]);
-        if (e != hipSuccess) rc = fail(ctx, BS2E_EDEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e));
-    }
+    if (!rc) rc = create_events(ctx, rk);
     if (rc) {
         free_tables(rk);
         delete rk;
@@ -362,15 +415,12 @@ int64_t bs2e_rk_from_primitives(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const
     // compute_R_K_map has no quadrature argument; the GL order only sizes work
     // tables that this path never fills, so the smallest rule is planned.
     std::string msg = make_plan(k, n_knots, knots, 1, max_k, rk->plan);
-    int64_t rc = msg.empty() ? check_plan_limits(ctx, rk->plan)
+    int64_t rc = msg.empty() ? check_plan_limits(ctx, rk->plan, false)
                              : fail(ctx, BS2E_EINVAL, "bs2e_rk_from_primitives: " + msg);
     if (!rc && (nnz4 != rk->plan.nnz4 || nnz6 != rk->plan.nnz6))
         rc = fail(ctx, BS2E_EINVAL, "bs2e_rk_from_primitives: nnz4/nnz6 differ from count_nnz_4d/6d for this basis");
     if (!rc) rc = alloc_tables(ctx, rk, true);
-    for (int i = 0; !rc && i < 5; ++i) {
-        hipError_t e = hipEventCreate(&rk->ev[i]);
-        if (e != hipSuccess) rc = fail(ctx, BS2E_EDEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e));
-    }
+    if (!rc) rc = create_events(ctx, rk);
     if (rc) {
         free_tables(rk);
         delete rk;
@@ -457,18 +507,8 @@ int64_t bs2e_rk_compute(bs2e_rk* rk)
     bs2e_ctx* ctx = rk->ctx;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     rk->mirror_valid = false;
-    if (rk->from_primitives) {
-        if (rk->profiling)
-            for (int i = 0; i < 4; ++i) HIP_TRY(ctx, hipEventRecord(rk->ev[i], ctx->stream));
-        HIP_TRY(ctx, launch_tensor_fill(rk->plan, rk->d, rk->R, ctx->stream));
-        if (rk->profiling) HIP_TRY(ctx, hipEventRecord(rk->ev[4], ctx->stream));
-        rk->computed = true;
-        return BS2E_OK;
-    }
-    int64_t rc = run_primitives(ctx, rk);
+    int64_t rc = run_build(ctx, rk);
     if (rc) return rc;
-    HIP_TRY(ctx, launch_tensor_fill(rk->plan, rk->d, rk->R, ctx->stream));
-    if (rk->profiling) HIP_TRY(ctx, hipEventRecord(rk->ev[4], ctx->stream));
     rk->computed = true;
     return BS2E_OK;
 }
@@ -484,8 +524,10 @@ int64_t bs2e_rk_info(const bs2e_rk* rk, int64_t* info, int64_t n)
 {
     if (!rk || !info) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_info: NULL argument");
     const Plan& p = rk->plan;
-    const int64_t v[10] = {p.n_b, p.n_pairs, p.K, p.nnz4, p.nnz6, p.n_iv, p.k_s, p.k_gl, p.U, p.n_knots};
-    for (int64_t i = 0; i < n && i < 10; ++i) info[i] = v[i];
+    int64_t near_elems = 0;
+    for (const NearRange& r : p.near) near_elems += (int64_t)(r.qhi - r.qlo + 1) * p.K;
+    const int64_t v[11] = {p.n_b, p.n_pairs, p.K, p.nnz4, p.nnz6, p.n_iv, p.k_s, p.k_gl, p.U, p.n_knots, near_elems};
+    for (int64_t i = 0; i < n && i < 11; ++i) info[i] = v[i];
     return BS2E_OK;
 }
 
@@ -562,10 +604,10 @@ int64_t bs2e_rk_stage_times(bs2e_rk* rk, double* ms, int64_t n)
 {
     if (!rk || !ms) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_stage_times: NULL argument");
     if (!rk->profiling || !rk->computed) return fail(rk->ctx, BS2E_ESTATE, "bs2e_rk_stage_times: profiling was off");
-    HIP_TRY(rk->ctx, hipEventSynchronize(rk->ev[4]));
-    for (int64_t i = 0; i < n && i < 4; ++i) {
+    HIP_TRY(rk->ctx, hipEventSynchronize(rk->ev[5]));
+    for (int64_t i = 0; i < n && i < 5; ++i) {
         float t = 0.f;
-        HIP_TRY(rk->ctx, hipEventElapsedTime(&t, rk->ev[i], rk->ev[i + 1]));
+        HIP_TRY(rk->ctx, hipEventElapsedTime(&t, rk->ev[0], rk->ev[i + 1]));
         ms[i] = t;
     }
     return BS2E_OK;
@@ -576,6 +618,7 @@ int64_t bs2e_rk_release(bs2e_rk* rk)
     if (!rk) return BS2E_OK;
     (void)hipSetDevice(rk->ctx->device);
     (void)hipStreamSynchronize(rk->ctx->stream);
+    (void)hipStreamSynchronize(rk->ctx->side);
     free_tables(rk);
     delete rk;
     return BS2E_OK;

@@ -1,23 +1,32 @@
 // k_primitives.hip -- per-interval primitives of the R^k build.
 //
-//   K1 k1_bspline_tables : de Boor values of the k local splines of every
-//                          interval at the outer GL points and at the inner
-//                          points of the triangle rule (bspline_tools.f90:151-224,
-//                          mat_els.f90:413-419, :462-475).
-//   K2 k2_pair_moments   : P^k[v;a,c] = h_v sum_q w_q B_a B_c r^k and
-//                          Q^k = h_v sum_q w_q B_a B_c / r^(k+1) over each banded
-//                          pair's support window (mat_els.f90:413-428), with the
-//                          window prefix/suffix sums the tensor kernel contracts.
-//   K4 k4_diag_cells     : D^k[v; outer (i,i'), inner (j,j')] of mat_els.f90:460-483,
-//                          computed as the per-interval contraction
-//                          D = h_v * O^T J over the k_GL outer points, with
-//                          O[q][u] = w_q B_i B_i' / r_q^(k+1) and
-//                          J[q][u] = s_q sum_p w_p B_j B_j' rho_qp^k, staged in LDS.
+// B-spline values: the k splines that are non-zero on interval v are
+// evaluated together at a point by the Cox-de Boor triangle (de Boor,
+// A Practical Guide to Splines, BSPLVB), from the same differences
+// d_p(j) = t(i+j) - x and d_m(j) = x - t(i+1-j), i = k-1+iv, that BVALUE_D
+// forms (bspline_tools.f90:205-211); its per-spline recurrence (:213-220)
+// gives the same values to rounding.
+//
+// kp_outer  (one workgroup per interval): spline values at the k_GL outer GL
+//   points and the local pair moments of the U = k(k+1)/2 unordered local
+//   spline pairs u = (a,c)
+//     Pl[v][u][k] = h_v sum_q w_q B_a B_c r_q^k,
+//     Ql[v][u][k] = h_v sum_q w_q B_a B_c / r_q^(k+1)       (mat_els.f90:413-428)
+// kp_inner  (one workgroup per interval and chunk of kc multipoles): spline
+//   values at the k_GL^2 inner points of the triangle rule and the diagonal
+//   cells of mat_els.f90:460-483 as the contraction over the outer points
+//     D[v][uo][ui][k] = h_v sum_q (w_q B_a B_c)(r_q) * J'[k][q][ui],
+//     J'[k][q][u]     = s_q / r_q^(k+1) * sum_p w_p rho_qp^k B_b B_d(rho_qp),
+//   with s_q = (r_q - bp_v)/2 and rho_qp = s_q x_p + (r_q + bp_v)/2; each
+//   thread accumulates kc multipoles in registers, staging in LDS.
+// kw_pair_windows: per banded pair p and multipole k, the window moments
+//   Pw/Qw[p][j][k] (j-th interval of p's support window), their prefix (P)
+//   and suffix (Q) sums and totals, gathered from Pl/Ql.
 #include "bs2e_internal.h"
 
 namespace bs2e {
 
-__device__ inline double ipow_d(double x, int m)
+__device__ __forceinline__ double ipow_d(double x, int m)
 {
     // x**m for integer m >= 0 by square-and-multiply (how Fortran lowers an
     // integer power).
@@ -30,72 +39,206 @@ __device__ inline double ipow_d(double x, int m)
     return y;
 }
 
-// ---------------------------------------------------------------------------
-// K1: one thread per (interval, point); point < k_gl is an outer GL point,
-// otherwise inner point (q, p) of the triangle rule on [bp_v, r_q].
-// BVALUE_D with iv known uses left knot i = k-1+iv (bspline_tools.f90:175); the
-// unit coefficient e_s selects local spline s-iv.  Each local spline runs the
-// reference's convex-combination recurrence (:213-220).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k1_bspline_tables(int n_iv, int ks, int kgl, const double* __restrict__ t,
-                                                         const double* __restrict__ bp, const double* __restrict__ x,
-                                                         double* __restrict__ Bo, double* __restrict__ Bi,
-                                                         double* __restrict__ ro)
+// All KS splines non-zero on interval iv at x: out[ls] = B_{iv+ls}(x).
+// t is stored 0-based (t(n) == t[n-1]); i1 = k-1+iv is the left knot index.
+template <int KS>
+__device__ __forceinline__ void splines_at(const double* __restrict__ t, int i1, double x, double* out, int stride)
 {
-    const int pts = kgl * (1 + kgl);
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (int64_t)n_iv * pts) return;
-    const int v = (int)(gid / pts);          // 0-based interval
-    const int pt = (int)(gid - (int64_t)v * pts);
-    const double lim1 = bp[v], lim2 = bp[v + 1];
-    const double scale = 0.5 * (lim2 - lim1);
-    const double translate = 0.5 * (lim2 + lim1);
-    double r;
-    double* out;
-    if (pt < kgl) {
-        r = scale * x[pt] + translate;
-        ro[(size_t)v * kgl + pt] = r;
-        out = Bo + ((size_t)v * kgl + pt) * ks;
-    } else {
-        const int q = (pt - kgl) / kgl, p = (pt - kgl) - q * kgl;
-        const double rq = scale * x[q] + translate;
-        const double scale_j = 0.5 * (rq - lim1);
-        const double translate_j = 0.5 * (rq + lim1);
-        r = scale_j * x[p] + translate_j;
-        out = Bi + (((size_t)v * kgl + q) * kgl + p) * ks;
+    double dr[KS], dl[KS], b[KS + 1];
+#pragma unroll
+    for (int j = 1; j <= KS - 1; ++j) {
+        dr[j] = t[i1 + j - 1] - x;   // d_p(j)
+        dl[j] = x - t[i1 - j];       // d_m(j)
     }
-    // d_p(j) = t(i+j) - x, d_m(j) = x - t(i+1-j), i = k-1+iv, iv = v+1 (1-based);
-    // t is stored 0-based so t(n) == t[n-1].
-    const int i1 = ks - 1 + (v + 1);
-    double dp[kMaxOrder + 1], dm[kMaxOrder + 1];
-    for (int j = 1; j <= ks - 1; ++j) {
-        dp[j] = t[i1 + j - 1] - r;
-        dm[j] = r - t[i1 + 1 - j - 1];
-    }
-    for (int ls = 0; ls < ks; ++ls) {
-        double a[kMaxOrder + 2];
-        for (int j = 1; j <= ks; ++j) a[j] = (j == ls + 1) ? 1.0 : 0.0;
-        for (int j = 1; j <= ks - 1; ++j) {
-            const int kmj = ks - j;
-            int ii = kmj;
-            for (int jj = 1; jj <= kmj; ++jj) {
-                a[jj] = (a[jj + 1] * dm[ii] + a[jj] * dp[jj]) / (dm[ii] + dp[jj]);
-                --ii;
-            }
+    b[1] = 1.0;
+#pragma unroll
+    for (int j = 1; j <= KS - 1; ++j) {
+        double saved = 0.0;
+#pragma unroll
+        for (int r = 1; r <= j; ++r) {
+            const double term = b[r] / (dr[r] + dl[j + 1 - r]);
+            b[r] = saved + dr[r] * term;
+            saved = dl[j + 1 - r] * term;
         }
-        out[ls] = a[1];
+        b[j + 1] = saved;
+    }
+#pragma unroll
+    for (int ls = 0; ls < KS; ++ls) out[ls * stride] = b[ls + 1];
+}
+
+__device__ __forceinline__ void local_pair(int u, int ks, int& ta, int& tc)
+{
+    // inverse of sym_local: row-major upper triangle
+    int lo = 0, base = 0;
+    while (base + (ks - lo) <= u) {
+        base += ks - lo;
+        ++lo;
+    }
+    ta = lo;
+    tc = lo + (u - base);
+}
+
+constexpr int kPThreads = 256;
+
+size_t outer_lds_bytes(int64_t ks, int64_t kgl, int64_t K)
+{
+    return sizeof(double) * (size_t)(kgl * ks + kgl + 2 * kgl * K);
+}
+
+int inner_kc(int64_t ks, int64_t kgl, int64_t K)
+{
+    for (int kc : {4, 2, 1})
+        if (kc <= K && inner_lds_bytes(ks, kgl, kc) <= 64 * 1024) return kc;
+    return inner_lds_bytes(ks, kgl, 1) <= 64 * 1024 ? 1 : 0;
+}
+
+size_t inner_lds_bytes(int64_t ks, int64_t kgl, int64_t kc)
+{
+    const int64_t U = ks * (ks + 1) / 2;
+    return sizeof(double) *
+           (size_t)(kgl * ks + kgl * kgl * ks + kgl + kgl * kgl + kgl * U + kc * kgl * kgl + kc * kgl * U);
+}
+
+template <int KS>
+__global__ __launch_bounds__(kPThreads) void kp_outer(int K, int kgl, const double* __restrict__ t,
+                                                      const double* __restrict__ bp, const double* __restrict__ x,
+                                                      const double* __restrict__ w, double* __restrict__ Pl,
+                                                      double* __restrict__ Ql)
+{
+    constexpr int U = KS * (KS + 1) / 2;
+    extern __shared__ double lds[];
+    double* sB = lds;                 // [kgl][KS]
+    double* sR = sB + kgl * KS;       // [kgl]
+    double* sPow = sR + kgl;          // [kgl][K]  r^k
+    double* sInv = sPow + kgl * K;    // [kgl][K]  1 / r^(k+1)
+    const int v = blockIdx.x;
+    const int tid = threadIdx.x;
+    const double lim1 = bp[v], lim2 = bp[v + 1];
+    const double h = 0.5 * (lim2 - lim1);
+    const double c = 0.5 * (lim2 + lim1);
+    const int i1 = KS - 1 + (v + 1);
+    for (int q = tid; q < kgl; q += kPThreads) {
+        const double r = h * x[q] + c;
+        sR[q] = r;
+        splines_at<KS>(t, i1, r, sB + q * KS, 1);
+    }
+    __syncthreads();
+    for (int e = tid; e < kgl * K; e += kPThreads) {
+        const int q = e / K, k = e - q * K;
+        const double r = sR[q];
+        sPow[e] = ipow_d(r, k);
+        sInv[e] = 1.0 / ipow_d(r, k + 1);
+    }
+    __syncthreads();
+    for (int e = tid; e < U * K; e += kPThreads) {
+        const int u = e / K, k = e - u * K;
+        int ta, tc;
+        local_pair(u, KS, ta, tc);
+        double accp = 0.0, accq = 0.0;
+        for (int q = 0; q < kgl; ++q) {
+            const double bb = w[q] * sB[q * KS + ta] * sB[q * KS + tc];
+            accp = accp + bb * sPow[q * K + k];
+            accq = accq + bb * sInv[q * K + k];
+        }
+        Pl[((size_t)v * U + u) * K + k] = h * accp;
+        Ql[((size_t)v * U + u) * K + k] = h * accq;
     }
 }
 
-// ---------------------------------------------------------------------------
-// K2: one thread per (pair, k).  Writes the window moments Pw/Qw[p][j][k]
-// (zero-padded to k_s slots), the prefix sums PP[p][j][k] = sum_{i<=j} Pw,
-// the suffix sums QS[p][j][k] = sum_{i>=j} Qw and the totals.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k2_pair_moments(int n_pairs, int K, int ks, int kgl,
+template <int KS>
+__global__ __launch_bounds__(kPThreads) void kp_inner(int K, int kgl, int kc, int kchunks,
+                                                      const double* __restrict__ t, const double* __restrict__ bp,
+                                                      const double* __restrict__ x, const double* __restrict__ w,
+                                                      double* __restrict__ D)
+{
+    constexpr int U = KS * (KS + 1) / 2;
+    constexpr int KCMAX = 4;
+    extern __shared__ double lds[];
+    double* sBo = lds;                       // [kgl][KS]       outer points
+    double* sBi = sBo + kgl * KS;            // [kgl][kgl][KS]  inner points
+    double* sR = sBi + kgl * kgl * KS;       // [kgl]
+    double* sRho = sR + kgl;                 // [kgl][kgl]
+    double* sWB = sRho + kgl * kgl;          // [kgl][U]        w_q B_a B_c at r_q
+    double* sX = sWB + kgl * U;              // [kc][kgl][kgl]  s_q w_p rho^k / r_q^(k+1)
+    double* sJ = sX + kc * kgl * kgl;        // [kc][kgl][U]
+
+    const int v = blockIdx.x / kchunks;
+    const int k0 = (blockIdx.x - v * kchunks) * kc;
+    const int nk = (K - k0) < kc ? (K - k0) : kc;
+    const int tid = threadIdx.x;
+    const double lim1 = bp[v], lim2 = bp[v + 1];
+    const double h = 0.5 * (lim2 - lim1);
+    const double c = 0.5 * (lim2 + lim1);
+    const int i1 = KS - 1 + (v + 1);
+
+    for (int e = tid; e < kgl * (1 + kgl); e += kPThreads) {
+        if (e < kgl) {
+            const double r = h * x[e] + c;
+            sR[e] = r;
+            splines_at<KS>(t, i1, r, sBo + e * KS, 1);
+        } else {
+            const int q = (e - kgl) / kgl, p = (e - kgl) - q * kgl;
+            const double rq = h * x[q] + c;
+            const double rho = 0.5 * (rq - lim1) * x[p] + 0.5 * (rq + lim1);
+            sRho[q * kgl + p] = rho;
+            splines_at<KS>(t, i1, rho, sBi + (q * kgl + p) * KS, 1);
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < kgl * U; e += kPThreads) {
+        const int q = e / U, u = e - q * U;
+        int ta, tc;
+        local_pair(u, KS, ta, tc);
+        sWB[e] = w[q] * sBo[q * KS + ta] * sBo[q * KS + tc];
+    }
+    for (int e = tid; e < nk * kgl * kgl; e += kPThreads) {
+        const int kk = e / (kgl * kgl), qp = e - kk * kgl * kgl;
+        const int q = qp / kgl, p = qp - q * kgl;
+        const int k = k0 + kk;
+        const double rq = sR[q];
+        sX[e] = 0.5 * (rq - lim1) * w[p] * ipow_d(sRho[qp], k) / ipow_d(rq, k + 1);
+    }
+    __syncthreads();
+    // J'[kk][q][u] = sum_p X[kk][q][p] B_b(rho_qp) B_d(rho_qp)
+    for (int e = tid; e < kgl * U; e += kPThreads) {
+        const int q = e / U, u = e - q * U;
+        int ta, tc;
+        local_pair(u, KS, ta, tc);
+        double acc[KCMAX] = {0.0, 0.0, 0.0, 0.0};
+        const double* bq = sBi + q * kgl * KS;
+        for (int p = 0; p < kgl; ++p) {
+            const double bb = bq[p * KS + ta] * bq[p * KS + tc];
+#pragma unroll
+            for (int kk = 0; kk < KCMAX; ++kk)
+                if (kk < nk) acc[kk] += sX[(kk * kgl + q) * kgl + p] * bb;
+        }
+#pragma unroll
+        for (int kk = 0; kk < KCMAX; ++kk)
+            if (kk < nk) sJ[(kk * kgl + q) * U + u] = acc[kk];
+    }
+    __syncthreads();
+    // D[v][uo][ui][k0 .. k0+nk) = h * sum_q WB[q][uo] J'[kk][q][ui]
+    for (int e = tid; e < U * U; e += kPThreads) {
+        const int uo = e / U, ui = e - uo * U;
+        double acc[KCMAX] = {0.0, 0.0, 0.0, 0.0};
+        for (int q = 0; q < kgl; ++q) {
+            const double a = sWB[q * U + uo];
+#pragma unroll
+            for (int kk = 0; kk < KCMAX; ++kk)
+                if (kk < nk) acc[kk] += a * sJ[(kk * kgl + q) * U + ui];
+        }
+        double* dst = D + (((size_t)v * U + uo) * U + ui) * K + k0;
+#pragma unroll
+        for (int kk = 0; kk < KCMAX; ++kk)
+            if (kk < nk) dst[kk] = h * acc[kk];
+    }
+}
+
+// Per (pair, k): window moments from the local tables, prefix/suffix sums.
+template <int KSMAX>
+__global__ __launch_bounds__(256) void kw_pair_windows(int n_pairs, int K, int ks, int U,
                                                        const PairInfo* __restrict__ pairs,
-                                                       const double* __restrict__ bp, const double* __restrict__ w,
-                                                       const double* __restrict__ Bo, const double* __restrict__ ro,
+                                                       const double* __restrict__ Pl, const double* __restrict__ Ql,
                                                        double* __restrict__ Pw, double* __restrict__ Qw,
                                                        double* __restrict__ PP, double* __restrict__ QS,
                                                        double* __restrict__ Ptot, double* __restrict__ Qtot)
@@ -104,157 +247,47 @@ __global__ __launch_bounds__(256) void k2_pair_moments(int n_pairs, int K, int k
     if (gid >= (int64_t)n_pairs * K) return;
     const int p = (int)(gid / K), k = (int)(gid - (int64_t)p * K);
     const PairInfo pi = pairs[p];
-    double pw[kMaxOrder], qw[kMaxOrder];
-    for (int j = 0; j < ks; ++j) {
+    double pw[KSMAX], qw[KSMAX];
+#pragma unroll
+    for (int j = 0; j < KSMAX; ++j) {
         pw[j] = 0.0;
         qw[j] = 0.0;
-    }
-    for (int j = 0; j < pi.m; ++j) {
-        const int v = pi.v0 - 1 + j;                 // 0-based interval
-        const int ta = pi.a - v, tc = pi.c - v;      // local spline index s - iv
-        const double scale = 0.5 * (bp[v + 1] - bp[v]);
-        const double* b = Bo + (size_t)v * kgl * ks;
-        const double* rr = ro + (size_t)v * kgl;
-        double accp = 0.0, accq = 0.0;
-        for (int q = 0; q < kgl; ++q) {
-            const double bb = w[q] * b[q * ks + ta] * b[q * ks + tc];
-            const double r = rr[q];
-            accp = accp + bb * ipow_d(r, k);
-            accq = accq + bb / ipow_d(r, k + 1);
+        if (j < pi.m) {
+            const int v = pi.v0 - 1 + j;                         // 0-based interval
+            const int u = sym_local(pi.a - v, pi.c - v, ks);     // local splines s - iv
+            pw[j] = Pl[((size_t)v * U + u) * K + k];
+            qw[j] = Ql[((size_t)v * U + u) * K + k];
         }
-        pw[j] = scale * accp;
-        qw[j] = scale * accq;
     }
     const size_t base = (size_t)p * ks * K + k;
     double run = 0.0;
-    for (int j = 0; j < ks; ++j) {
-        run += pw[j];
-        Pw[base + (size_t)j * K] = pw[j];
-        Qw[base + (size_t)j * K] = qw[j];
-        PP[base + (size_t)j * K] = (j < pi.m) ? run : 0.0;
+#pragma unroll
+    for (int j = 0; j < KSMAX; ++j) {
+        if (j < ks) {
+            run += pw[j];
+            Pw[base + (size_t)j * K] = pw[j];
+            Qw[base + (size_t)j * K] = qw[j];
+            PP[base + (size_t)j * K] = (j < pi.m) ? run : 0.0;
+        }
     }
     Ptot[(size_t)p * K + k] = run;
     run = 0.0;
-    for (int j = ks - 1; j >= 0; --j) {
-        if (j < pi.m) run += qw[j];
-        QS[base + (size_t)j * K] = (j < pi.m) ? run : 0.0;
+#pragma unroll
+    for (int j = KSMAX - 1; j >= 0; --j) {
+        if (j < ks) {
+            run += qw[j];
+            QS[base + (size_t)j * K] = (j < pi.m) ? run : 0.0;
+        }
     }
     Qtot[(size_t)p * K + k] = run;
 }
 
-// ---------------------------------------------------------------------------
-// K4: one workgroup per (interval v, multipole k).
-//   rk_tab[q][p] = w_p rho_qp^k
-//   J[q][u]  = s_q * sum_p rk_tab[q][p] B_tb(rho_qp) B_td(rho_qp)   (inner pair u)
-//   O[q][u]  = w_q B_ta(r_q) B_tc(r_q) / r_q^(k+1)                    (outer pair u)
-//   D[v][k][uo][ui] = h_v * sum_q O[q][uo] J[q][ui]
-// Local pairs are the U = k(k+1)/2 unordered pairs of the interval's k splines
-// (B_a B_c is symmetric in (a, c)).
-// ---------------------------------------------------------------------------
-constexpr int kK4Threads = 256;
-
-__global__ __launch_bounds__(kK4Threads) void k4_diag_cells(int n_iv, int K, int ks, int kgl, int U,
-                                                            const double* __restrict__ bp,
-                                                            const double* __restrict__ x, const double* __restrict__ w,
-                                                            const double* __restrict__ Bo,
-                                                            const double* __restrict__ Bi,
-                                                            const double* __restrict__ ro, double* __restrict__ D)
-{
-    extern __shared__ double lds[];
-    const int v = blockIdx.x / K;
-    const int k = blockIdx.x - v * K;
-    const int tid = threadIdx.x;
-    double* sO = lds;                          // [kgl][U]
-    double* sJ = sO + kgl * U;                 // [kgl][U]
-    double* sRK = sJ + kgl * U;                // [kgl][kgl]
-    int* sTa = (int*)(sRK + kgl * kgl);        // [U]
-    int* sTc = sTa + U;                        // [U]
-
-    const double lim1 = bp[v], lim2 = bp[v + 1];
-    const double scale_i = 0.5 * (lim2 - lim1);
-    for (int u = tid; u < U; u += kK4Threads) {
-        // invert the symmetric local index (row-major upper triangle)
-        int lo = 0, base = 0;
-        while (base + (ks - lo) <= u) {
-            base += ks - lo;
-            ++lo;
-        }
-        sTa[u] = lo;
-        sTc[u] = lo + (u - base);
-    }
-    const double* rr = ro + (size_t)v * kgl;
-    for (int e = tid; e < kgl * kgl; e += kK4Threads) {
-        const int q = e / kgl, p = e - q * kgl;
-        const double rq = rr[q];
-        const double scale_j = 0.5 * (rq - lim1);
-        const double translate_j = 0.5 * (rq + lim1);
-        const double rho = scale_j * x[p] + translate_j;
-        sRK[e] = w[p] * ipow_d(rho, k);
-    }
-    __syncthreads();
-    const double* bo = Bo + (size_t)v * kgl * ks;
-    const double* bi = Bi + (size_t)v * kgl * kgl * ks;
-    for (int e = tid; e < kgl * U; e += kK4Threads) {
-        const int q = e / U, u = e - q * U;
-        const int ta = sTa[u], tc = sTc[u];
-        const double rq = rr[q];
-        sO[e] = w[q] * bo[q * ks + ta] * bo[q * ks + tc] / ipow_d(rq, k + 1);
-        const double scale_j = 0.5 * (rq - lim1);
-        const double* bq = bi + (size_t)q * kgl * ks;
-        double acc = 0.0;
-        for (int p = 0; p < kgl; ++p) acc = acc + sRK[q * kgl + p] * bq[p * ks + ta] * bq[p * ks + tc];
-        sJ[e] = scale_j * acc;
-    }
-    __syncthreads();
-    double* Dv = D + ((size_t)v * K + k) * U * U;
-    for (int e = tid; e < U * U; e += kK4Threads) {
-        const int uo = e / U, ui = e - uo * U;
-        double acc = 0.0;
-        for (int q = 0; q < kgl; ++q) acc = acc + sO[q * U + uo] * sJ[q * U + ui];
-        Dv[e] = scale_i * acc;
-    }
-}
-
-static inline unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
-
-hipError_t launch_bspline_tables(const Plan& p, const DevTables& d, hipStream_t s)
-{
-    const int64_t n = p.n_iv * p.k_gl * (1 + p.k_gl);
-    hipLaunchKernelGGL(k1_bspline_tables, dim3(blocks_for(n, 256)), dim3(256), 0, s, (int)p.n_iv, (int)p.k_s,
-                       (int)p.k_gl, d.t, d.bp, d.x, d.Bo, d.Bi, d.ro);
-    return hipGetLastError();
-}
-
-hipError_t launch_pair_moments(const Plan& p, const DevTables& d, hipStream_t s)
-{
-    const int64_t n = p.n_pairs * p.K;
-    hipLaunchKernelGGL(k2_pair_moments, dim3(blocks_for(n, 256)), dim3(256), 0, s, (int)p.n_pairs, (int)p.K,
-                       (int)p.k_s, (int)p.k_gl, d.pairs, d.bp, d.w, d.Bo, d.ro, d.Pw, d.Qw, d.PP, d.QS, d.Ptot,
-                       d.Qtot);
-    return hipGetLastError();
-}
-
-size_t diag_cells_lds_bytes(const Plan& p)
-{
-    return sizeof(double) * (size_t)(2 * p.k_gl * p.U + p.k_gl * p.k_gl) + sizeof(int) * (size_t)(2 * p.U);
-}
-
-hipError_t launch_diag_cells(const Plan& p, const DevTables& d, hipStream_t s)
-{
-    const size_t lds = diag_cells_lds_bytes(p);
-    hipLaunchKernelGGL(k4_diag_cells, dim3((unsigned)(p.n_iv * p.K)), dim3(kK4Threads), lds, s, (int)p.n_iv,
-                       (int)p.K, (int)p.k_s, (int)p.k_gl, (int)p.U, d.bp, d.x, d.w, d.Bo, d.Bi, d.ro, d.D);
-    return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
-// K2b: window prefix/suffix sums and totals from Pw/Qw already in place (used
-// when the moments are gathered from caller-provided containers).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k2b_window_sums(int n_pairs, int K, int ks, const PairInfo* __restrict__ pairs,
-                                                       const double* __restrict__ Pw, const double* __restrict__ Qw,
-                                                       double* __restrict__ PP, double* __restrict__ QS,
-                                                       double* __restrict__ Ptot, double* __restrict__ Qtot)
+// Window prefix/suffix sums from Pw/Qw already in place (the moments were
+// gathered from caller-provided containers).
+__global__ __launch_bounds__(256) void kw_window_sums(int n_pairs, int K, int ks, const PairInfo* __restrict__ pairs,
+                                                      const double* __restrict__ Pw, const double* __restrict__ Qw,
+                                                      double* __restrict__ PP, double* __restrict__ QS,
+                                                      double* __restrict__ Ptot, double* __restrict__ Qtot)
 {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= (int64_t)n_pairs * K) return;
@@ -275,10 +308,79 @@ __global__ __launch_bounds__(256) void k2b_window_sums(int n_pairs, int K, int k
     Qtot[(size_t)p * K + k] = run;
 }
 
+static inline unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+template <int KS>
+static void launch_outer(const Plan& p, const DevTables& d, hipStream_t s)
+{
+    hipLaunchKernelGGL(kp_outer<KS>, dim3((unsigned)p.n_iv), dim3(kPThreads), outer_lds_bytes(KS, p.k_gl, p.K), s,
+                       (int)p.K, (int)p.k_gl, d.t, d.bp, d.x, d.w, d.Pl, d.Ql);
+}
+
+template <int KS>
+static void launch_inner(const Plan& p, const DevTables& d, hipStream_t s)
+{
+    const int kc = inner_kc(KS, p.k_gl, p.K);
+    const int kchunks = (int)((p.K + kc - 1) / kc);
+    hipLaunchKernelGGL(kp_inner<KS>, dim3((unsigned)(p.n_iv * kchunks)), dim3(kPThreads),
+                       inner_lds_bytes(KS, p.k_gl, kc), s, (int)p.K, (int)p.k_gl, kc, kchunks, d.t, d.bp, d.x, d.w,
+                       d.D);
+}
+
+#define BS2E_ORDER_SWITCH(FN)                                                                                \
+    switch (p.k_s) {                                                                                         \
+        case 2: FN<2>(p, d, s); break;                                                                       \
+        case 3: FN<3>(p, d, s); break;                                                                       \
+        case 4: FN<4>(p, d, s); break;                                                                       \
+        case 5: FN<5>(p, d, s); break;                                                                       \
+        case 6: FN<6>(p, d, s); break;                                                                       \
+        case 7: FN<7>(p, d, s); break;                                                                       \
+        case 8: FN<8>(p, d, s); break;                                                                       \
+        case 9: FN<9>(p, d, s); break;                                                                       \
+        case 10: FN<10>(p, d, s); break;                                                                     \
+        case 11: FN<11>(p, d, s); break;                                                                     \
+        case 12: FN<12>(p, d, s); break;                                                                     \
+        case 13: FN<13>(p, d, s); break;                                                                     \
+        case 14: FN<14>(p, d, s); break;                                                                     \
+        case 15: FN<15>(p, d, s); break;                                                                     \
+        case 16: FN<16>(p, d, s); break;                                                                     \
+        default: return hipErrorInvalidValue;                                                                \
+    }
+
+hipError_t launch_outer_moments(const Plan& p, const DevTables& d, hipStream_t s)
+{
+    BS2E_ORDER_SWITCH(launch_outer)
+    return hipGetLastError();
+}
+
+hipError_t launch_diag_cells(const Plan& p, const DevTables& d, hipStream_t s)
+{
+    BS2E_ORDER_SWITCH(launch_inner)
+    return hipGetLastError();
+}
+
+template <int KSMAX>
+static void launch_kw(const Plan& p, const DevTables& d, hipStream_t s)
+{
+    const int64_t n = p.n_pairs * p.K;
+    hipLaunchKernelGGL(kw_pair_windows<KSMAX>, dim3(blocks_for(n, 256)), dim3(256), 0, s, (int)p.n_pairs, (int)p.K,
+                       (int)p.k_s, (int)p.U, d.pairs, d.Pl, d.Ql, d.Pw, d.Qw, d.PP, d.QS, d.Ptot, d.Qtot);
+}
+
+hipError_t launch_pair_windows(const Plan& p, const DevTables& d, hipStream_t s)
+{
+    if (p.k_s <= 6) launch_kw<6>(p, d, s);
+    else if (p.k_s <= 8) launch_kw<8>(p, d, s);
+    else if (p.k_s <= 10) launch_kw<10>(p, d, s);
+    else if (p.k_s <= 12) launch_kw<12>(p, d, s);
+    else launch_kw<16>(p, d, s);
+    return hipGetLastError();
+}
+
 hipError_t launch_window_sums(const Plan& p, const DevTables& d, hipStream_t s)
 {
     const int64_t n = p.n_pairs * p.K;
-    hipLaunchKernelGGL(k2b_window_sums, dim3(blocks_for(n, 256)), dim3(256), 0, s, (int)p.n_pairs, (int)p.K,
+    hipLaunchKernelGGL(kw_window_sums, dim3(blocks_for(n, 256)), dim3(256), 0, s, (int)p.n_pairs, (int)p.K,
                        (int)p.k_s, d.pairs, d.Pw, d.Qw, d.PP, d.QS, d.Ptot, d.Qtot);
     return hipGetLastError();
 }

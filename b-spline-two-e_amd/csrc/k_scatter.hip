@@ -50,7 +50,8 @@ __global__ __launch_bounds__(256) void k_scatter_6d(int64_t n_diag, int K, int k
     const PairInfo pI = pairs[de.I], pJ = pairs[de.J];
     const int uo = sym_local(pI.a - vz, pI.c - vz, ks);   // outer pair (i, i_p)
     const int ui = sym_local(pJ.a - vz, pJ.c - vz, ks);   // inner pair (j, j_p)
-    for (int k = 0; k < K; ++k) rdk[(size_t)k * nnz6 + ptr] = D[(((size_t)vz * K + k) * U + uo) * U + ui];
+    const double* Dv = D + (((size_t)vz * U + uo) * U + ui) * K;   // D[v][uo][ui][k]
+    for (int k = 0; k < K; ++k) rdk[(size_t)k * nnz6 + ptr] = Dv[k];
     iv[ptr] = v;
     i[ptr] = pI.a;
     ip[ptr] = pI.c;
@@ -144,7 +145,8 @@ __global__ __launch_bounds__(256) void k_gather_6d(int64_t nnz6, int K, int ks, 
     const int vz = (int)v - 1;
     const int uo = sym_local((int)b[0] - vz, (int)b[1] - vz, ks);
     const int ui = sym_local((int)b[2] - vz, (int)b[3] - vz, ks);
-    for (int k = 0; k < K; ++k) D[(((size_t)vz * K + k) * U + uo) * U + ui] = rdk[(size_t)k * nnz6 + ptr];
+    double* Dv = D + (((size_t)vz * U + uo) * U + ui) * K;         // D[v][uo][ui][k]
+    for (int k = 0; k < K; ++k) Dv[k] = rdk[(size_t)k * nnz6 + ptr];
 }
 
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s)

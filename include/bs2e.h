@@ -95,7 +95,8 @@ int64_t bs2e_rk_from_primitives(bs2e_ctx *ctx, int64_t k, int64_t n_knots, const
  * tensor) on the context's stream.  Allocates nothing. */
 int64_t bs2e_rk_compute(bs2e_rk *rk);
 int64_t bs2e_rk_synchronize(bs2e_rk *rk);
-/* info[0..9] = n_b, n_pairs, K, nnz4, nnz6, n_iv, k, k_GL, U, n_knots */
+/* info[0..10] = n_b, n_pairs, K, nnz4, nnz6, n_iv, k, k_GL, U, n_knots,
+ * near elements (tensor elements whose pairs share a support interval) */
 int64_t bs2e_rk_info(const bs2e_rk *rk, int64_t *info, int64_t n);
 /* Device address and element count of the tensor (n_pairs^2 * K doubles). */
 int64_t bs2e_rk_device_ptr(const bs2e_rk *rk, void **ptr, int64_t *n_elems);
@@ -110,8 +111,10 @@ int64_t bs2e_rk_copy_to_host(bs2e_rk *rk, double *dst, int64_t n_elems);
  * mirror it copies the K values from the device.  BS2E_ERANGE for a key
  * outside |a-c| < k, |b-d| < k (the reference error-stops there). */
 int64_t bs2e_rk_get_val(const bs2e_rk *rk, const int64_t *indices, double *values, int64_t n_values);
-/* Per-stage device times (ms) of the most recent compute when profiling is
- * on: [0] de Boor tables, [1] pair moments, [2] diagonal cells, [3] tensor. */
+/* Device timeline (ms since the start of the most recent compute, profiling
+ * on): [0] local moments + pair windows done, [1] rank-one fill done,
+ * [2] diagonal cells done (second stream), [3] near ranges done (second
+ * stream), [4] end of the build. */
 int64_t bs2e_rk_set_profiling(bs2e_rk *rk, int64_t on);
 int64_t bs2e_rk_stage_times(bs2e_rk *rk, double *ms, int64_t n);
 int64_t bs2e_rk_release(bs2e_rk *rk);
