@@ -17,7 +17,8 @@
 //                                   suffix (Q) sums over the window
 //   Ptot, Qtot [n_pairs][K]         window totals
 //   D  [n_iv][U][U][K]              diagonal-cell integrals (outer local pair,
-//                                   inner local pair, multipole)
+//                                   inner local pair, multipole); a tensor
+//                                   build keeps D + D^T (symmetric in the pairs)
 //   R  [n_pairs][n_pairs][K]        the R^k tensor; key [a,b,c,d] lives at
 //                                   (pair(a,c)*n_pairs + pair(b,d))*K + k
 #pragma once
@@ -98,7 +99,9 @@ struct DevTables {
 
 // Kernel launchers (each .hip file launches its own kernels).
 hipError_t launch_outer_moments(const Plan& p, const DevTables& d, hipStream_t s);           // Pl, Ql
-hipError_t launch_diag_cells(const Plan& p, const DevTables& d, hipStream_t s);              // D
+hipError_t launch_diag_cells(const Plan& p, const DevTables& d, hipStream_t s, bool symmetrised);  // D / Dsym
+hipError_t launch_symmetrize(const Plan& p, const DevTables& d, hipStream_t s);                    // D -> Dsym
+size_t near_lds_bytes(int64_t ks, int64_t K);
 hipError_t launch_pair_windows(const Plan& p, const DevTables& d, hipStream_t s);            // Pw/Qw + sums
 hipError_t launch_tensor_far(const Plan& p, const DevTables& d, double* R, hipStream_t s);   // rank-one part
 hipError_t launch_tensor_near(const Plan& p, const DevTables& d, double* R, hipStream_t s);  // near ranges

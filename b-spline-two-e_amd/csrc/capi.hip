@@ -251,6 +251,12 @@ static int64_t check_plan_limits(bs2e_ctx* ctx, const Plan& p, bool needs_primit
                     "the per-interval primitive kernels support spline orders <= 16 whose tables fit 64 KiB of LDS "
                     "(about k_GL^2 * k + k_GL * k^2 doubles; e.g. k <= 12 with k_GL = k+6)");
     if (p.n_pairs * p.K >= ((int64_t)1 << 31)) return fail(ctx, BS2E_EINVAL, "tensor row too long (n_pairs*K >= 2^31)");
+    if (p.n_iv * p.U * p.U * p.K >= ((int64_t)1 << 31))
+        return fail(ctx, BS2E_EINVAL, "diagonal-cell table too large (n_iv*U^2*K >= 2^31)");
+    if (near_lds_bytes(p.k_s, p.K) > 64 * 1024)
+        return fail(ctx, BS2E_EINVAL,
+                    "spline order x multipole count too large for the near-range kernel's LDS "
+                    "(8 rows x (2k+2) x (max_k+1) doubles must fit 64 KiB)");
     return BS2E_OK;
 }
 
@@ -259,7 +265,7 @@ static int64_t run_primitives_serial(bs2e_ctx* ctx, bs2e_rk* rk)
 {
     hipStream_t s = ctx->stream;
     HIP_TRY(ctx, launch_outer_moments(rk->plan, rk->d, s));
-    HIP_TRY(ctx, launch_diag_cells(rk->plan, rk->d, s));
+    HIP_TRY(ctx, launch_diag_cells(rk->plan, rk->d, s, false));
     HIP_TRY(ctx, launch_pair_windows(rk->plan, rk->d, s));
     return BS2E_OK;
 }
@@ -281,7 +287,7 @@ static int64_t run_build(bs2e_ctx* ctx, bs2e_rk* rk)
     HIP_TRY(ctx, hipEventRecord(rk->fork, m));
     HIP_TRY(ctx, hipStreamWaitEvent(s, rk->fork, 0));
     if (!rk->from_primitives) {
-        HIP_TRY(ctx, launch_diag_cells(rk->plan, rk->d, s));
+        HIP_TRY(ctx, launch_diag_cells(rk->plan, rk->d, s, true));
         HIP_TRY(ctx, launch_pair_windows(rk->plan, rk->d, m));
     }
     if (prof) HIP_TRY(ctx, hipEventRecord(rk->ev[3], s));
@@ -485,6 +491,7 @@ int64_t bs2e_rk_from_primitives(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const
     STEP(launch_gather_6d(p, rk->d, d_rdk, d_i6, d_i6 + nnz6, d_i6 + 2 * nnz6, d_i6 + 3 * nnz6, d_i6 + 4 * nnz6,
                           d_bad, s));
     STEP(launch_window_sums(p, rk->d, s));
+    STEP(launch_symmetrize(p, rk->d, s));
     int bad = 0;
     STEP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s));
     STEP(hipStreamSynchronize(s));

@@ -145,7 +145,9 @@ __global__ __launch_bounds__(kPThreads) void kp_outer(int K, int kgl, const doub
     }
 }
 
-template <int KS>
+// SYM: write Dsym[v][u1][u2][k] = D[v][u1][u2][k] + D[v][u2][u1][k] (what the
+// tensor consumes) instead of D (what the sparse_6d container holds).
+template <int KS, bool SYM>
 __global__ __launch_bounds__(kPThreads) void kp_inner(int K, int kgl, int kc, int kchunks,
                                                       const double* __restrict__ t, const double* __restrict__ bp,
                                                       const double* __restrict__ x, const double* __restrict__ w,
@@ -223,9 +225,13 @@ __global__ __launch_bounds__(kPThreads) void kp_inner(int K, int kgl, int kc, in
         double acc[KCMAX] = {0.0, 0.0, 0.0, 0.0};
         for (int q = 0; q < kgl; ++q) {
             const double a = sWB[q * U + uo];
+            const double b = SYM ? sWB[q * U + ui] : 0.0;
 #pragma unroll
             for (int kk = 0; kk < KCMAX; ++kk)
-                if (kk < nk) acc[kk] += a * sJ[(kk * kgl + q) * U + ui];
+                if (kk < nk) {
+                    acc[kk] += a * sJ[(kk * kgl + q) * U + ui];
+                    if (SYM) acc[kk] += b * sJ[(kk * kgl + q) * U + uo];
+                }
         }
         double* dst = D + (((size_t)v * U + uo) * U + ui) * K + k0;
 #pragma unroll
@@ -317,14 +323,26 @@ static void launch_outer(const Plan& p, const DevTables& d, hipStream_t s)
                        (int)p.K, (int)p.k_gl, d.t, d.bp, d.x, d.w, d.Pl, d.Ql);
 }
 
-template <int KS>
+template <int KS, bool SYM>
 static void launch_inner(const Plan& p, const DevTables& d, hipStream_t s)
 {
     const int kc = inner_kc(KS, p.k_gl, p.K);
     const int kchunks = (int)((p.K + kc - 1) / kc);
-    hipLaunchKernelGGL(kp_inner<KS>, dim3((unsigned)(p.n_iv * kchunks)), dim3(kPThreads),
+    hipLaunchKernelGGL((kp_inner<KS, SYM>), dim3((unsigned)(p.n_iv * kchunks)), dim3(kPThreads),
                        inner_lds_bytes(KS, p.k_gl, kc), s, (int)p.K, (int)p.k_gl, kc, kchunks, d.t, d.bp, d.x, d.w,
                        d.D);
+}
+
+template <int KS>
+static void launch_inner_sym(const Plan& p, const DevTables& d, hipStream_t s)
+{
+    launch_inner<KS, true>(p, d, s);
+}
+
+template <int KS>
+static void launch_inner_plain(const Plan& p, const DevTables& d, hipStream_t s)
+{
+    launch_inner<KS, false>(p, d, s);
 }
 
 #define BS2E_ORDER_SWITCH(FN)                                                                                \
@@ -353,9 +371,13 @@ hipError_t launch_outer_moments(const Plan& p, const DevTables& d, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_diag_cells(const Plan& p, const DevTables& d, hipStream_t s)
+hipError_t launch_diag_cells(const Plan& p, const DevTables& d, hipStream_t s, bool symmetrised)
 {
-    BS2E_ORDER_SWITCH(launch_inner)
+    if (symmetrised) {
+        BS2E_ORDER_SWITCH(launch_inner_sym)
+    } else {
+        BS2E_ORDER_SWITCH(launch_inner_plain)
+    }
     return hipGetLastError();
 }
 

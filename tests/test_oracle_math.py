@@ -1,0 +1,241 @@
+"""Pins of the oracle by what the mathematics fixes.
+
+* Gauss-Legendre: numpy's leggauss and exactness on polynomials.
+* B-splines: scipy's BSpline, partition of unity, non-negativity.
+* P^k: exact for polynomial integrands (degree 2(k-1)+k <= 2 k_GL - 1), so
+  it must equal the analytic integral of the piecewise polynomial.
+* Q^k, D^k: an independent numpy/scipy evaluation of the same rule (scipy
+  spline values, vectorised sums in a different order); the triangle rule's
+  diagonal cells summed both ways must equal an adaptive 2-D integral.
+* R^k: 8-fold symmetry, positivity, every key of count_nnz_R_k set, and the
+  textbook hydrogenic Slater integrals of Z=2 orbitals.
+* Angular algebra: sympy's exact 3j/6j.
+* One-electron Hamiltonian: hydrogenic levels -Z^2 / (2 n^2).
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+from scipy.interpolate import BSpline
+
+from bs2e.workloads import WORKLOADS
+
+
+def knots(oracle, name):
+    w = WORKLOADS[name]
+    return w, oracle.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+
+
+# ------------------------------------------------------------------ GL
+@pytest.mark.parametrize("N", [1, 2, 3, 6, 10, 12, 14, 16, 24, 33])
+def test_gauss_legendre(oracle, N):
+    from sympy.integrals.quadrature import gauss_legendre as sym_gl
+    x, w = oracle.gauss_legendre(N)
+    xs, ws = sym_gl(N, 40)                         # 40-digit nodes and weights
+    ex = np.array([float(v) for v in xs])
+    ew = np.array([float(v) for v in ws])
+    o = np.argsort(ex)
+    ex, ew = ex[o], ew[o]
+    assert np.all(np.abs(x - ex) <= np.spacing(np.abs(ex) + 1e-300))   # within 1 ulp
+    assert np.all(np.abs(w - ew) <= np.spacing(ew))
+    xr, wr = np.polynomial.legendre.leggauss(N)     # numpy's weights are only good to ~1e-13
+    assert np.allclose(x, xr, rtol=0, atol=4e-16) and np.allclose(w, wr, rtol=1e-12, atol=0)
+    for deg in range(2 * N):                        # exact up to degree 2N-1
+        exact = 0.0 if deg % 2 else 2.0 / (deg + 1)
+        assert abs(np.dot(w, x ** deg) - exact) < 2e-15 * max(1, N)
+
+
+# ------------------------------------------------------------------ splines
+def scipy_basis(t, k, s, x):
+    """B_s(x) (1-based s) from scipy: order k <-> degree k-1."""
+    c = np.zeros(len(t) - k)
+    c[s - 1] = 1.0
+    return BSpline(t, c, k - 1, extrapolate=False)(x)
+
+
+@pytest.mark.parametrize("name", ["tiny", "c1"])
+def test_bvalue_vs_scipy_and_partition_of_unity(oracle, name):
+    w, t = knots(oracle, name)
+    n, n_b, niv = oracle.dims(w.k, t)
+    bp = t[w.k - 1:w.k - 1 + niv + 1]
+    rng = np.random.default_rng(1)
+    for iv in range(1, niv + 1, max(1, niv // 15)):
+        for x in bp[iv - 1] + rng.random(3) * (bp[iv] - bp[iv - 1]):
+            vals = np.array([oracle.bvalue_unit(t, w.k, iv + ls, x, iv) for ls in range(w.k)])
+            ref = np.array([scipy_basis(t, w.k, iv + ls, x) for ls in range(w.k)])
+            assert np.allclose(vals, ref, rtol=1e-13, atol=1e-15)
+            assert (vals >= 0).all()
+            assert abs(vals.sum() - 1.0) < 1e-14
+
+
+# ------------------------------------------------------------------ primitives
+def spline_table(t, k, x):
+    """values[s-1, i] = B_s(x_i) for every spline s."""
+    n = len(t) - k
+    return np.array([scipy_basis(t, k, s, x) for s in range(1, n + 1)])
+
+
+def test_P_is_the_exact_integral(oracle):
+    """P^k[v;a,c] = int_v B_a B_c r^k dr exactly when the rule is exact."""
+    w, t = knots(oracle, "tiny")
+    k_gl, max_k = 8, 4                                  # 2(k-1)+max_k = 10 <= 2*8-1
+    off = oracle.slater_off_diag(w.k, t, k_gl, max_k)
+    bp = t[w.k - 1:]
+    # exact integral of the piecewise polynomial with a 40-point rule per interval
+    xg, wg = np.polynomial.legendre.leggauss(40)
+    for e in range(0, off["nnz"], 7):
+        iv, a, c = off["iv"][e], off["i"][e], off["j"][e]
+        lo, hi = bp[iv - 1], bp[iv]
+        r = 0.5 * (hi - lo) * xg + 0.5 * (hi + lo)
+        ba, bc = scipy_basis(t, w.k, a + 1, r), scipy_basis(t, w.k, c + 1, r)
+        for k in range(max_k + 1):
+            exact = 0.5 * (hi - lo) * np.sum(wg * ba * bc * r ** k)   # itself rounded to ~1e-15
+            assert abs(off["rk"][e, k] - exact) <= 1e-13 * abs(exact) + 1e-300
+
+
+@pytest.mark.parametrize("name", ["tiny", "c1"])
+def test_Q_and_D_independent_restatement(oracle, name):
+    """Same rule, independent code: scipy spline values, vectorised numpy."""
+    w, t = knots(oracle, name)
+    k_gl, max_k = w.k_GL, w.max_k
+    off = oracle.slater_off_diag(w.k, t, k_gl, max_k)
+    dg = oracle.slater_diag(w.k, t, k_gl, max_k)
+    xq, wq = np.polynomial.legendre.leggauss(k_gl)
+    bp = t[w.k - 1:]
+    # off-diagonal Q
+    for e in range(0, off["nnz"], 3):
+        iv, a, c = off["iv"][e], off["i"][e], off["j"][e]
+        lo, hi = bp[iv - 1], bp[iv]
+        h = 0.5 * (hi - lo)
+        r = h * xq + 0.5 * (hi + lo)
+        bb = wq * scipy_basis(t, w.k, a + 1, r) * scipy_basis(t, w.k, c + 1, r)
+        q = h * np.array([np.sum(bb / r ** (k + 1)) for k in range(max_k + 1)])
+        assert np.allclose(off["rmk"][e], q, rtol=2e-14, atol=0)
+    # diagonal cells (mat_els.f90:460-483): outer pair (i,i') at r, inner (j,j') on [bp_v, r]
+    step = max(1, dg["nnz"] // 400)
+    for e in range(0, dg["nnz"], step):
+        iv = dg["iv"][e]
+        lo, hi = bp[iv - 1], bp[iv]
+        h = 0.5 * (hi - lo)
+        r = h * xq + 0.5 * (hi + lo)
+        bo = wq * scipy_basis(t, w.k, dg["i"][e] + 1, r) * scipy_basis(t, w.k, dg["ip"][e] + 1, r)
+        s = 0.5 * (r - lo)
+        rho = s[:, None] * xq[None, :] + 0.5 * (r + lo)[:, None]
+        bi = wq[None, :] * scipy_basis(t, w.k, dg["j"][e] + 1, rho) * scipy_basis(t, w.k, dg["jp"][e] + 1, rho)
+        for k in range(max_k + 1):
+            inner = s * np.sum(bi * rho ** k, axis=1)
+            ref = h * np.sum(bo * inner / r ** (k + 1))
+            assert abs(dg["rdk"][e, k] - ref) <= 1e-13 * abs(ref)
+
+
+def test_diagonal_cell_triangles_sum_to_the_cell_integral(oracle):
+    """D[v;p,q] + D[v;q,p] = int int_{cell^2} B_p(r1) B_q(r2) r<^k / r>^(k+1)."""
+    from scipy.integrate import dblquad
+    w, t = knots(oracle, "tiny")
+    dg = oracle.slater_diag(w.k, t, 12, 2)
+    key = {}
+    for e in range(dg["nnz"]):
+        key[(dg["iv"][e], dg["i"][e], dg["ip"][e], dg["j"][e], dg["jp"][e])] = e
+    bp = t[w.k - 1:]
+    rng = np.random.default_rng(5)
+    picks = rng.choice(dg["nnz"], 6, replace=False)
+    for e in picks:
+        iv, i, ip, j, jp = (dg[n][e] for n in ("iv", "i", "ip", "j", "jp"))
+        swapped = key[(iv, j, jp, i, ip)]
+        lo, hi = bp[iv - 1], bp[iv]
+        for k in range(3):
+            f = lambda r2, r1: (scipy_basis(t, w.k, i + 1, r1) * scipy_basis(t, w.k, ip + 1, r1) *  # noqa: E731
+                                scipy_basis(t, w.k, j + 1, r2) * scipy_basis(t, w.k, jp + 1, r2) *
+                                min(r1, r2) ** k / max(r1, r2) ** (k + 1))
+            # split along the diagonal so each half is smooth
+            a1, _ = dblquad(f, lo, hi, lambda r1: lo, lambda r1: r1, epsabs=0, epsrel=1e-11)
+            a2, _ = dblquad(f, lo, hi, lambda r1: r1, lambda r1: hi, epsabs=0, epsrel=1e-11)
+            both = dg["rdk"][e, k] + dg["rdk"][swapped, k]
+            assert abs(both - (a1 + a2)) <= 1e-9 * abs(a1 + a2), (e, k)
+
+
+# ------------------------------------------------------------------ tensor
+@pytest.fixture(scope="module")
+def c1_tensor(oracle):
+    w, t = knots(oracle, "c1")
+    n, n_b, _ = oracle.dims(w.k, t)
+    off = oracle.slater_off_diag(w.k, t, w.k_GL, w.max_k)
+    dg = oracle.slater_diag(w.k, t, w.k_GL, w.max_k)
+    R, touched = oracle.rk_dense(w.k, n_b, w.max_k, off, dg, want_touched=True)
+    return w, t, n_b, R, touched
+
+
+def test_tensor_keys_symmetry_positivity(oracle, c1_tensor):
+    w, t, n_b, R, touched = c1_tensor
+    assert touched.all() and R.shape[0] == oracle.n_pairs(w.k, n_b)
+    assert R.shape[0] == n_b * (2 * w.k - 1) - w.k * (w.k - 1)
+    assert (R > 0).all()
+    pairs = oracle.pair_table(w.k, n_b)
+    idx = {tuple(p): i for i, p in enumerate(pairs)}
+    swap = np.array([idx[(c, a)] for a, c in pairs])
+    scale = np.abs(R).max()
+    assert np.abs(R - R.transpose(1, 0, 2)).max() <= 1e-15 * scale * 8      # R(a,b,c,d) = R(b,a,d,c)
+    assert np.abs(R - R[swap]).max() <= 1e-15 * scale * 8                   # R(a,b,c,d) = R(c,b,a,d)
+    assert np.abs(R - R[:, swap]).max() <= 1e-15 * scale * 8                # R(a,b,c,d) = R(a,d,c,b)
+
+
+def test_hydrogenic_slater_integrals(oracle, c1_tensor):
+    """Textbook hydrogenic Slater integrals for Z=2 (CAP off)."""
+    w, t, n_b, R, _ = c1_tensor
+    Z = 2
+    S = oracle.setup_S(w.k, t, w.k_GL).real
+    orb = {}
+    for l in (0, 1):
+        H = oracle.setup_H1p(w.k, t, w.k_GL, Z, l, cap_eta=0.0).real
+        E, V = sl.eigh(H, S)
+        for nprinc in (1, 2):
+            i = nprinc - 1 - l
+            if i >= 0:
+                assert abs(E[i] + Z * Z / (2.0 * nprinc ** 2)) < 1e-9
+                orb[(nprinc, l)] = V[:, i]
+    pairs = oracle.pair_table(w.k, n_b)
+
+    def slater(k, a, b, c, d):
+        A = np.array([a[x - 1] * c[y - 1] for x, y in pairs])
+        B = np.array([b[x - 1] * d[y - 1] for x, y in pairs])
+        return A @ R[:, :, k] @ B
+
+    s1, s2, p2 = orb[(1, 0)], orb[(2, 0)], orb[(2, 1)]
+    cases = [(slater(0, s1, s1, s1, s1), 5 * Z / 8), (slater(0, s1, s2, s1, s2), 17 * Z / 81),
+             (slater(0, s1, s2, s2, s1), 16 * Z / 729), (slater(0, s2, s2, s2, s2), 77 * Z / 512),
+             (slater(0, s1, p2, s1, p2), 59 * Z / 243), (slater(1, s1, p2, p2, s1), 112 * Z / 2187),
+             (slater(0, p2, p2, p2, p2), 93 * Z / 512), (slater(2, p2, p2, p2, p2), 45 * Z / 512)]
+    for got, exact in cases:
+        assert abs(got - exact) < 1e-9 * exact, (got, exact)
+
+
+# ------------------------------------------------------------------ angular
+def test_wigner_vs_sympy(oracle):
+    from sympy.physics.wigner import wigner_3j, wigner_6j
+    rng = np.random.default_rng(2)
+    for _ in range(300):
+        a, b = rng.integers(0, 13, 2)
+        c = int(rng.integers(abs(a - b), a + b + 1))
+        ref = float(wigner_3j(int(a), int(b), c, 0, 0, 0))
+        assert abs(oracle.three_j(a, b, c, 0, 0, 0) - ref) < 1e-14
+    for _ in range(300):
+        j = [int(x) for x in rng.integers(0, 13, 6)]
+        ref = float(wigner_6j(*j))
+        assert abs(oracle.six_j(*j) - ref) < 1e-13
+    # large arguments used by lmax = 12, k <= 24
+    for args in [(12, 12, 4, 12, 12, 24), (11, 12, 3, 10, 9, 20), (12, 7, 5, 9, 10, 17)]:
+        assert abs(oracle.six_j(*args) - float(wigner_6j(*args))) < 1e-13
+
+
+def test_ang_k_LS_closed_form(oracle):
+    """ang_k_LS for (s s) -> (s s), L=0, k=0 is 1; parity forbids odd k between s and p."""
+    assert abs(oracle.ang_k_LS(0, (0, 0), (0, 0), 0) - 1.0) < 1e-15
+    assert oracle.ang_k_LS(1, (0, 0), (0, 0), 0) == 0.0
+    # direct Coulomb between p^2 1S and p^2 1S: k=0 -> 1, k=2 -> 2/5 * sqrt? check with sympy
+    from sympy import sqrt, Rational  # noqa: F401
+    from sympy.physics.wigner import wigner_3j, wigner_6j
+    for k in (0, 2):
+        la, lb, lc, ld, L = 1, 1, 1, 1, 0
+        ref = ((-1) ** (lb + lc + L) * float(wigner_6j(la, lb, L, ld, lc, k)) *
+               (-1) ** la * np.sqrt((2 * la + 1) * (2 * lc + 1)) * float(wigner_3j(la, k, lc, 0, 0, 0)) *
+               (-1) ** lb * np.sqrt((2 * lb + 1) * (2 * ld + 1)) * float(wigner_3j(lb, k, ld, 0, 0, 0)))
+        assert abs(oracle.ang_k_LS(k, (la, lb), (lc, ld), L) - ref) < 1e-14
