@@ -19,7 +19,7 @@ FX = np.load(os.path.join(ROOT, "tests", "golden", "ref_fixtures.npz"))
 def header_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(bs2e_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(bs2e_[A-Za-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
