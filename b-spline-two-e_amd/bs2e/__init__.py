@@ -11,6 +11,10 @@ Names, argument meaning and error behaviour follow edvinolo/b-spline-two-e:
     Nd_DOK, compute_R_K_map(r_d_k, r_k, r_m_k,
                             b_splines, max_k, R)         sparse_array_tools.f90:452-555
     ang_k_LS(k, l1, l2, L)                               wigner_tools.f90:126-138
+    sym, count_configs(term, max_l_1p, n_b, k_spline,
+                       max_n_b, n_all_l, l_2_max)        orbital_tools.f90:10-19, 119-216
+    CSR_matrix, construct_block_tensor(H, S, b_splines,
+                       term, max_k, R_k, full)           hamiltonian.f90:106-283, 348-416
 
 Everything numeric runs on the GPU; there is no CPU fallback.  A missing
 library or device raises Bs2eError.
@@ -26,7 +30,8 @@ from . import capi
 from .capi import Bs2eError, check, ptr
 
 __all__ = ["Bs2eError", "Context", "generate_grid", "b_spline", "sparse_4d", "sparse_6d", "setup_Slater_integrals",
-           "Nd_DOK", "RkTensor", "compute_R_K_map", "ang_k_LS", "three_j", "six_j", "count_nnz"]
+           "Nd_DOK", "RkTensor", "compute_R_K_map", "ang_k_LS", "three_j", "six_j", "count_nnz", "sym",
+           "count_configs", "CSR_matrix", "Block", "construct_block_tensor"]
 
 
 class Context:
@@ -281,3 +286,118 @@ def three_j(*a):
 
 def six_j(*a):
     return capi.lib().bs2e_six_j(*[int(x) for x in a])
+
+
+@dataclass
+class sym:
+    """orbital_tools.f90:10-19: a two-electron symmetry (total L, M, parity pi)
+    and its configurations, one row (n1, n2, l1, l2, eqv) per config%."""
+    l: int
+    m: int = 0
+    pi: bool = False
+    configs: np.ndarray | None = None
+
+    @property
+    def n_config(self) -> int:
+        return 0 if self.configs is None else int(self.configs.shape[0])
+
+
+def count_configs(term: sym, max_l_1p, n_b, k_spline, max_n_b, n_all_l, l_2_max, eigs=None) -> np.ndarray:
+    """orbital_tools.f90:119-216: the configurations of `term` as an (n, 5)
+    int64 array (n1, n2, l1, l2, eqv).  `eigs` is accepted for signature
+    parity; the reference computes an energy from it that no filter uses."""
+    L = capi.lib()
+    args = [int(term.l), int(bool(term.pi)), max_l_1p, n_b, k_spline, max_n_b, n_all_l, l_2_max]
+    n = check(L.bs2e_count_configs(*args, None, 0))
+    out = np.zeros((max(n, 1), 5), np.int64)
+    check(L.bs2e_count_configs(*args, ptr(out, C.c_int64), n))
+    return out[:n]
+
+
+@dataclass
+class CSR_matrix:
+    """sparse_array_tools.f90:127-135: 1-based CSR (index_ptr, indices) with
+    double complex data."""
+    shape: tuple
+    index_ptr: np.ndarray
+    indices: np.ndarray
+    data: np.ndarray
+
+    @property
+    def nnz(self) -> int:
+        return int(self.indices.size)
+
+    def get_dense(self) -> np.ndarray:
+        out = np.zeros(self.shape, np.complex128)
+        for r in range(self.shape[0]):
+            lo, hi = self.index_ptr[r] - 1, self.index_ptr[r + 1] - 1
+            out[r, self.indices[lo:hi] - 1] += self.data[lo:hi]
+        return out
+
+
+class Block:
+    """One symmetry block on the device: configurations, CSR pattern counts
+    (hamiltonian.f90:348-416) and the fill of H_sp / S_sp."""
+
+    def __init__(self, tensor: RkTensor, term: sym, full=True):
+        cf = np.ascontiguousarray(term.configs, dtype=np.int64)
+        self.tensor = tensor
+        self.ctx = tensor.ctx
+        self.n_config = int(cf.shape[0])
+        h = C.c_void_p()
+        check(capi.lib().bs2e_block_create(tensor.handle, int(term.l), self.n_config, ptr(cf, C.c_int64),
+                                           int(bool(full)), C.byref(h)), self.ctx.handle)
+        self.handle = h
+        sz = np.zeros(5, np.int64)
+        check(capi.lib().bs2e_block_sizes(h, ptr(sz, C.c_int64), 5), self.ctx.handle)
+        self.nnz_H, self.nnz_S, self.ref_nnz_H, self.ref_nnz_S = (int(x) for x in sz[:4])
+
+    def fill(self, H, S):
+        """H: one-electron blocks per l (n_b x n_b each, l = 0 first); S: overlap."""
+        n_b = self.tensor.n_b
+        Hl = np.ascontiguousarray(np.stack([np.asarray(h, np.complex128).ravel(order="F") for h in H]))
+        Sf = np.ascontiguousarray(np.asarray(S, np.complex128).ravel(order="F"))
+        if Hl.shape[1] != n_b * n_b or Sf.size != n_b * n_b:
+            raise Bs2eError(capi.BS2E_EINVAL, f"one-electron matrices must be {n_b} x {n_b}")
+        n = self.n_config
+        ipH = np.zeros(n + 1, np.int64)
+        ipS = np.zeros(n + 1, np.int64)
+        ixH = np.zeros(max(self.nnz_H, 1), np.int64)
+        ixS = np.zeros(max(self.nnz_S, 1), np.int64)
+        dH = np.zeros(max(self.nnz_H, 1), np.complex128)
+        dS = np.zeros(max(self.nnz_S, 1), np.complex128)
+        I = C.c_int64
+        D = C.POINTER(C.c_double)
+        check(capi.lib().bs2e_block_fill(self.handle, len(H) - 1, Hl.ctypes.data_as(D), Sf.ctypes.data_as(D),
+                                         ptr(ipH, I), ptr(ixH, I), dH.ctypes.data_as(D), ptr(ipS, I), ptr(ixS, I),
+                                         dS.ctypes.data_as(D)), self.ctx.handle)
+        H_sp = CSR_matrix((n, n), ipH, ixH[:self.nnz_H], dH[:self.nnz_H])
+        S_sp = CSR_matrix((n, n), ipS, ixS[:self.nnz_S], dS[:self.nnz_S])
+        return H_sp, S_sp
+
+    def release(self):
+        if self.handle:
+            capi.lib().bs2e_block_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def construct_block_tensor(H, S, b_splines, term: sym, max_k, R_k, full=True):
+    """hamiltonian.f90:106-283: returns (H_sp, S_sp) for `term` from the
+    device tensor held by R_k (an Nd_DOK from compute_R_K_map, or an RkTensor).
+    H: one-electron blocks H(l)%data for l = 0.. (n_b x n_b complex each)."""
+    tensor = R_k.tensor if isinstance(R_k, Nd_DOK) else R_k
+    if tensor is None:
+        raise Bs2eError(capi.BS2E_ESTATE, "R_k holds no tensor (call compute_R_K_map first)")
+    if tensor.K != max_k + 1 or tensor.k != b_splines.k:
+        raise Bs2eError(capi.BS2E_EINVAL, "max_k / b_splines do not match the tensor")
+    blk = Block(tensor, term, full)
+    try:
+        return blk.fill(H, S)
+    finally:
+        blk.release()

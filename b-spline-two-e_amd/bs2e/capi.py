@@ -53,6 +53,11 @@ SIGNATURES = [
     ("bs2e_ang_k_LS", dbl, [i64] * 6),
     ("bs2e_three_j", dbl, [i64] * 6),
     ("bs2e_six_j", dbl, [i64] * 6),
+    ("bs2e_count_configs", i64, [i64] * 8 + [P(i64), i64]),
+    ("bs2e_block_create", i64, [vp, i64, i64, P(i64), i64, P(vp)]),
+    ("bs2e_block_sizes", i64, [vp, P(i64), i64]),
+    ("bs2e_block_fill", i64, [vp, i64, P(dbl), P(dbl), P(i64), P(i64), P(dbl), P(i64), P(i64), P(dbl)]),
+    ("bs2e_block_release", i64, [vp]),
 ]
 
 

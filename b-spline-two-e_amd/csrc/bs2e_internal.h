@@ -95,7 +95,52 @@ struct DevTables {
     double* D = nullptr;
     NearRange* near = nullptr;
     NearItem* items = nullptr;
+    int32_t* c_first = nullptr;
 };
+
+// ---- symmetry blocks (angular-channel summation) --------------------------
+struct BlockChan {            // one channel (l1, l2) of a block's configurations
+    int32_t l1, l2;
+    int32_t start, n;         // first configuration (0-based) and count
+};
+
+struct BlockSeg {             // configurations of one channel with a given n1
+    int32_t off;              // offset of the first one within the channel
+    int32_t lo, hi;           // n2 range (hi < lo: none)
+    int32_t pad;
+};
+
+struct BlockArgs {
+    int n_config, n_chan, full, L, ks, n_b, n_pairs, K;
+    const int64_t* cfg;          // 5 per config: n1, n2, l1, l2, eqv
+    const int32_t* cfg_chan;     // channel of each config
+    const BlockChan* chan;       // per channel
+    const BlockSeg* seg;         // [n_chan][n_b + 1]: per n1
+    const double* coef;          // [n_chan][n_chan][2][K]: direct, exchange (with its sign)
+    const int32_t* flags;        // [n_chan][n_chan]: bit0 direct allowed, bit1 exchange allowed
+    const int32_t* c_first;      // pair index of the first pair with column c
+    const double* R;             // [n_pairs][n_pairs][K]
+    const double* Hl;            // complex, per l an n_b x n_b column-major matrix
+    const double* S;             // complex, n_b x n_b column-major
+    int64_t* cntH;               // per row (count pass)
+    int64_t* cntS;
+    int64_t* refH;               // per row: the reference's count_nnz predicate
+    int64_t* refS;
+    const int64_t* ptrH;         // 0-based row starts (fill pass)
+    const int64_t* ptrS;
+    int64_t* ixH;
+    double* dH;                  // complex
+    int64_t* ixS;
+    double* dS;                  // complex
+};
+
+hipError_t launch_block_count(const BlockArgs& a, hipStream_t s);
+hipError_t launch_block_fill(const BlockArgs& a, hipStream_t s);
+
+// orbital_tools.f90:119-216 count_configs: 5 int64 per config (n1, n2, l1, l2,
+// eqv); returns the count (writes at most cap configs when out != nullptr).
+int64_t count_configs(int64_t L, int64_t term_pi, int64_t max_l_1p, int64_t n_b, int64_t k_spline, int64_t max_n_b,
+                      int64_t n_all_l, int64_t l_2_max, int64_t* out, int64_t cap);
 
 // Kernel launchers (each .hip file launches its own kernels).
 hipError_t launch_outer_moments(const Plan& p, const DevTables& d, hipStream_t s);           // Pl, Ql

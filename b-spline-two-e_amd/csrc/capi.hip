@@ -7,65 +7,17 @@
 #include <string>
 #include <vector>
 
-#include "../../include/bs2e.h"
-#include "bs2e_internal.h"
+#include "capi_internal.h"
 
 using namespace bs2e;
 
-struct bs2e_ctx {
-    int device = 0;
-    hipStream_t own = nullptr;
-    hipStream_t stream = nullptr;   // the build's stream (own, or the caller's)
-    hipStream_t side = nullptr;     // diagonal cells + near ranges, concurrent with the rank-one fill
-    std::string err;
-};
-
-struct bs2e_rk {
-    bs2e_ctx* ctx = nullptr;
-    Plan plan;
-    DevTables d;
-    double* R = nullptr;
-    size_t R_elems = 0;
-    int64_t precision = BS2E_PREC_FP64;
-    std::vector<double> mirror;
-    bool mirror_valid = false;
-    bool profiling = false;
-    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   // profiling
-    hipEvent_t fork = nullptr, windows = nullptr, join = nullptr;                  // stream ordering
-    bool computed = false;
-    bool from_primitives = false;   // tables gathered from caller containers: compute runs K3 only
-};
-
 static thread_local std::string g_err;
 
-static int64_t fail(bs2e_ctx* ctx, int64_t code, const std::string& msg)
+int64_t bs2e::fail(bs2e_ctx* ctx, int64_t code, const std::string& msg)
 {
     if (ctx) ctx->err = msg;
     g_err = msg;
     return code;
-}
-
-#define HIP_TRY(ctx, expr)                                                                                   \
-    do {                                                                                                     \
-        hipError_t e_ = (expr);                                                                              \
-        if (e_ != hipSuccess)                                                                                \
-            return fail((ctx), e_ == hipErrorOutOfMemory ? BS2E_ENOMEM : BS2E_EDEVICE,                       \
-                        std::string(#expr) + ": " + hipGetErrorString(e_));                                  \
-    } while (0)
-
-template <class T>
-static hipError_t dev_alloc(T** p, size_t n)
-{
-    *p = nullptr;
-    if (n == 0) n = 1;
-    return hipMalloc((void**)p, n * sizeof(T));
-}
-
-template <class T>
-static void dev_free(T*& p)
-{
-    if (p) (void)hipFree(p);
-    p = nullptr;
 }
 
 extern "C" {
@@ -181,6 +133,7 @@ static void free_tables(bs2e_rk* rk)
     dev_free(d.D);
     dev_free(d.near);
     dev_free(d.items);
+    dev_free(d.c_first);
     dev_free(rk->R);
     for (auto& e : rk->ev)
         if (e) {
@@ -228,7 +181,10 @@ static int64_t alloc_tables(bs2e_ctx* ctx, bs2e_rk* rk, bool with_tensor)
         HIP_TRY(ctx, dev_alloc(&d.near, p.near.size()));
         HIP_TRY(ctx, dev_alloc(&d.items, p.items.size()));
     }
+    HIP_TRY(ctx, dev_alloc(&d.c_first, p.c_first.size()));
     hipStream_t s = ctx->stream;
+    HIP_TRY(ctx, hipMemcpyAsync(d.c_first, p.c_first.data(), sizeof(int32_t) * p.c_first.size(), hipMemcpyHostToDevice,
+                                s));
     if (with_tensor) {
         HIP_TRY(ctx, hipMemcpyAsync(d.near, p.near.data(), sizeof(NearRange) * p.near.size(), hipMemcpyHostToDevice, s));
         HIP_TRY(ctx,

@@ -1,0 +1,239 @@
+// k_channel.hip -- angular-channel summation of R^k into the CSR blocks of one
+// symmetry (L, parity): hamiltonian.f90:106-283 construct_block_tensor with
+// mat_els.f90:552-571 r_12_tens, :608-633 c_mat_neq_tens, :664-678 S_mat_neq,
+// :697-715 H_1p_neq, and the sizes of :348-416 count_nnz.
+//
+// Configurations (n1, n2, l1, l2) of a block come in the reference's order
+// (orbital_tools.f90:144-193): channel (l1, l2), then n1, then n2.  Row i =
+// (n_a, n_b, l_a, l_b), column j = (n_c, n_d, l_c, l_d).  An entry can only
+// exist where the radial pairs share support:
+//   support    = |n_a-n_c| < k  and |n_b-n_d| < k
+//   support_ex = |n_a-n_d| < k  and |n_b-n_c| < k,
+// so one wavefront per row walks, channel by channel, only the columns with
+// n_c within k of n_a or n_b, and for each such n_c the n_d within k of n_b
+// (direct) or n_a (exchange), in ascending column order; lanes evaluate the
+// reference's predicate
+//   H entry <=> (support or support_ex) and (r12_allowed or l_eq or l_eq_ex)
+//   S entry <=> (support and l_eq) or (support_ex and l_eq_ex)
+// and ballots give each entry its CSR position.  The angular factors are per
+// (row channel, column channel) coefficient vectors:
+//   cdir[k] = ang_k_LS(k, confs, L)               if |ang| >= 5e-16 else 0
+//   cex[k]  = (-1)^(l_c+l_d+L) ang_k_LS(k, confs_ex, L)  (same threshold)
+// and r12_allowed uses the 5e-15 threshold of :171-178.
+#include "bs2e_internal.h"
+
+namespace bs2e {
+
+constexpr int kCThreads = 256;
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int iabs_d(int x) { return x < 0 ? -x : x; }
+
+struct Cplx {
+    double re, im;
+};
+
+// complex product and sum without fused multiply-adds, as Fortran evaluates
+// double complex arithmetic: (a+bi)(c+di) = (ac-bd) + (ad+bc)i
+__device__ __forceinline__ Cplx cmul(Cplx a, Cplx b)
+{
+    return Cplx{__dsub_rn(__dmul_rn(a.re, b.re), __dmul_rn(a.im, b.im)),
+                __dadd_rn(__dmul_rn(a.re, b.im), __dmul_rn(a.im, b.re))};
+}
+__device__ __forceinline__ Cplx cadd(Cplx a, Cplx b) { return Cplx{__dadd_rn(a.re, b.re), __dadd_rn(a.im, b.im)}; }
+__device__ __forceinline__ Cplx cscale(double s, Cplx a) { return Cplx{__dmul_rn(s, a.re), __dmul_rn(s, a.im)}; }
+
+__device__ __forceinline__ int pair_of(const int32_t* c_first, int ks, int a, int c)
+{
+    const int lo = (c - ks + 1) < 1 ? 1 : (c - ks + 1);
+    return c_first[c] + (a - lo);
+}
+
+__device__ __forceinline__ Cplx mat(const double* M, int n_b, int r, int c)
+{
+    const size_t o = 2 * ((size_t)(c - 1) * n_b + (r - 1));
+    return Cplx{M[o], M[o + 1]};
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(kCThreads) void kc_block(BlockArgs A)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int i = (int)(((size_t)blockIdx.x * kCThreads + threadIdx.x) / kWave);
+    if (i >= A.n_config) return;   // whole wave exits together
+    const int ks = A.ks, K = A.K;
+    const int64_t* ci = A.cfg + 5 * (size_t)i;
+    const int n_a = (int)ci[0], n_bb = (int)ci[1], l_a = (int)ci[2], l_b = (int)ci[3];
+    const int ch_i = A.cfg_chan[i];
+    const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int64_t nH = 0, nS = 0, rH = 0, rS = 0;
+    const int64_t baseH = FILL ? A.ptrH[i] : 0, baseS = FILL ? A.ptrS[i] : 0;
+
+    for (int ch = 0; ch < A.n_chan; ++ch) {
+        const BlockChan cj = A.chan[ch];
+        const int l_c = cj.l1, l_d = cj.l2;
+        const int fl = A.flags[ch_i * A.n_chan + ch];
+        const bool dir_any = fl & 1, ex_any = (fl >> 1) & 1;
+        const bool l_eq = (l_a == l_c) && (l_b == l_d);
+        const bool l_eq_ex = (l_a == l_d) && (l_b == l_c);
+        if (!dir_any && !ex_any && !l_eq && !l_eq_ex) continue;   // no entry can exist in this channel
+        const double* cd = A.coef + ((size_t)ch_i * A.n_chan + ch) * 2 * K;
+        const double* ce = cd + K;
+        const BlockSeg* sg = A.seg + (size_t)ch * (A.n_b + 1);
+        // n_c ranges: within k of n_a (direct) or of n_b (exchange), ascending
+        int r1lo = n_a - ks + 1, r1hi = n_a + ks - 1, r2lo = n_bb - ks + 1, r2hi = n_bb + ks - 1;
+        if (r2lo < r1lo) {
+            int t0 = r1lo, t1 = r1hi;
+            r1lo = r2lo;
+            r1hi = r2hi;
+            r2lo = t0;
+            r2hi = t1;
+        }
+        if (r2lo <= r1hi + 1) {       // overlapping or adjacent: one range
+            r1hi = r2hi > r1hi ? r2hi : r1hi;
+            r2lo = 1;
+            r2hi = 0;
+        }
+        for (int part = 0; part < 2; ++part) {
+            int clo = part ? r2lo : r1lo, chi = part ? r2hi : r1hi;
+            if (clo < 1) clo = 1;
+            if (chi > A.n_b) chi = A.n_b;
+            for (int n_c = clo; n_c <= chi; ++n_c) {
+                const BlockSeg s = sg[n_c];
+                if (s.hi < s.lo) continue;
+                const bool in_dir_c = iabs_d(n_c - n_a) < ks;
+                const bool in_ex_c = iabs_d(n_c - n_bb) < ks;
+                // n_d segments: direct needs |n_b-n_d| < k, exchange |n_a-n_d| < k
+                int s1lo = 1, s1hi = 0, s2lo = 1, s2hi = 0;
+                if (in_dir_c) {
+                    s1lo = n_bb - ks + 1;
+                    s1hi = n_bb + ks - 1;
+                }
+                if (in_ex_c) {
+                    s2lo = n_a - ks + 1;
+                    s2hi = n_a + ks - 1;
+                }
+                if (!in_dir_c) {
+                    s1lo = s2lo;
+                    s1hi = s2hi;
+                    s2lo = 1;
+                    s2hi = 0;
+                } else if (in_ex_c) {
+                    if (s2lo < s1lo) {
+                        int t0 = s1lo, t1 = s1hi;
+                        s1lo = s2lo;
+                        s1hi = s2hi;
+                        s2lo = t0;
+                        s2hi = t1;
+                    }
+                    if (s2lo <= s1hi + 1) {
+                        s1hi = s2hi > s1hi ? s2hi : s1hi;
+                        s2lo = 1;
+                        s2hi = 0;
+                    }
+                }
+                for (int dseg = 0; dseg < 2; ++dseg) {
+                    int dlo = dseg ? s2lo : s1lo, dhi = dseg ? s2hi : s1hi;
+                    if (dlo < s.lo) dlo = s.lo;
+                    if (dhi > s.hi) dhi = s.hi;
+                    for (int base = dlo; base <= dhi; base += kWave) {
+                        const int n_d = base + lane;
+                        bool active = n_d <= dhi;
+                        const int j = cj.start + s.off + (n_d - s.lo);
+                        if (!A.full && j < i) active = false;
+                        const bool support = in_dir_c && iabs_d(n_bb - n_d) < ks;
+                        const bool support_ex = in_ex_c && iabs_d(n_a - n_d) < ks;
+                        const bool r12 = (dir_any && support) || (ex_any && support_ex);
+                        const bool entry = active && (support || support_ex) && (r12 || l_eq || l_eq_ex);
+                        const bool one_el = active && ((support && l_eq) || (support_ex && l_eq_ex));
+                        const unsigned long long mH = __ballot(entry);
+                        const unsigned long long mS = __ballot(one_el);
+                        if (FILL) {
+                            if (entry) {
+                                Cplx h{0.0, 0.0};
+                                if (r12) {
+                                    double res = 0.0;
+                                    if (support) {
+                                        const double* v = A.R + ((size_t)pair_of(A.c_first, ks, n_a, n_c) * A.n_pairs +
+                                                                 pair_of(A.c_first, ks, n_bb, n_d)) * K;
+                                        double t = 0.0;
+                                        for (int k = 0; k < K; ++k) t = __dadd_rn(t, __dmul_rn(v[k], cd[k]));
+                                        res = __dadd_rn(res, t);
+                                    }
+                                    if (support_ex) {
+                                        const double* v = A.R + ((size_t)pair_of(A.c_first, ks, n_a, n_d) * A.n_pairs +
+                                                                 pair_of(A.c_first, ks, n_bb, n_c)) * K;
+                                        double t = 0.0;
+                                        for (int k = 0; k < K; ++k) t = __dadd_rn(t, __dmul_rn(v[k], ce[k]));
+                                        res = __dadd_rn(res, t);
+                                    }
+                                    h.re = res;
+                                }
+                                if (one_el) {
+                                    const size_t nn = (size_t)A.n_b * A.n_b;
+                                    const double* Ha = A.Hl + 2 * nn * l_a;
+                                    const double* Hb = A.Hl + 2 * nn * l_b;
+                                    Cplx h1{0.0, 0.0}, s1{0.0, 0.0};
+                                    const double sgn = ((l_c + l_d + A.L) & 1) ? -1.0 : 1.0;
+                                    if (l_eq) {
+                                        h1 = cadd(cadd(h1, cmul(mat(Ha, A.n_b, n_a, n_c), mat(A.S, A.n_b, n_bb, n_d))),
+                                                  cmul(mat(Hb, A.n_b, n_bb, n_d), mat(A.S, A.n_b, n_a, n_c)));
+                                        s1 = cadd(s1, cmul(mat(A.S, A.n_b, n_a, n_c), mat(A.S, A.n_b, n_bb, n_d)));
+                                    }
+                                    if (l_eq_ex) {
+                                        const Cplx x = cadd(cmul(mat(Ha, A.n_b, n_a, n_d), mat(A.S, A.n_b, n_bb, n_c)),
+                                                            cmul(mat(Hb, A.n_b, n_bb, n_c), mat(A.S, A.n_b, n_a, n_d)));
+                                        h1 = cadd(h1, cscale(sgn, x));
+                                        s1 = cadd(s1, cmul(cscale(sgn, mat(A.S, A.n_b, n_a, n_d)),
+                                                           mat(A.S, A.n_b, n_bb, n_c)));
+                                    }
+                                    h = cadd(h, h1);
+                                    const int64_t ps = baseS + nS + __popcll(mS & lt_mask);
+                                    A.ixS[ps] = j + 1;
+                                    A.dS[2 * ps] = s1.re;
+                                    A.dS[2 * ps + 1] = s1.im;
+                                }
+                                const int64_t ph = baseH + nH + __popcll(mH & lt_mask);
+                                A.ixH[ph] = j + 1;
+                                A.dH[2 * ph] = h.re;
+                                A.dH[2 * ph + 1] = h.im;
+                            }
+                        } else {
+                            // the reference's count_nnz predicate (:386-413), for diagnostics
+                            const bool rh = active && (support ? dir_any : (support_ex && ex_any));
+                            const bool rs = active && (support ? l_eq : (support_ex && l_eq_ex));
+                            rH += __popcll(__ballot(rh));
+                            rS += __popcll(__ballot(rs));
+                        }
+                        nH += __popcll(mH);
+                        nS += __popcll(mS);
+                    }
+                }
+            }
+        }
+    }
+    if (!FILL && lane == 0) {
+        A.cntH[i] = nH;
+        A.cntS[i] = nS;
+        A.refH[i] = rH;
+        A.refS[i] = rS;
+    }
+}
+
+hipError_t launch_block_count(const BlockArgs& a, hipStream_t s)
+{
+    const int64_t threads = (int64_t)a.n_config * kWave;
+    hipLaunchKernelGGL(kc_block<false>, dim3((unsigned)((threads + kCThreads - 1) / kCThreads)), dim3(kCThreads), 0, s,
+                       a);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_fill(const BlockArgs& a, hipStream_t s)
+{
+    const int64_t threads = (int64_t)a.n_config * kWave;
+    hipLaunchKernelGGL(kc_block<true>, dim3((unsigned)((threads + kCThreads - 1) / kCThreads)), dim3(kCThreads), 0, s,
+                       a);
+    return hipGetLastError();
+}
+
+}  // namespace bs2e

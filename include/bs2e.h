@@ -126,6 +126,42 @@ double bs2e_ang_k_LS(int64_t k, int64_t l1a, int64_t l1b, int64_t l2a, int64_t l
 double bs2e_three_j(int64_t ja, int64_t jb, int64_t jc, int64_t ma, int64_t mb, int64_t mc);
 double bs2e_six_j(int64_t j1, int64_t j2, int64_t j3, int64_t j4, int64_t j5, int64_t j6);
 
+/* ---- symmetry blocks --------------------------------------------------- */
+typedef struct bs2e_block bs2e_block; /* the configurations of one symmetry and its CSR pattern */
+
+/* orbital_tools.f90:119-216 count_configs(term, max_l_1p, n_b, k_spline,
+ * max_n_b, n_all_l, l_2_max) with :46-72 consistent: enumerates the
+ * two-electron configurations of symmetry (L, term_pi) (term_pi 1 = odd) in
+ * the reference's order (l1, l2 <= l1, n1, n2).  configs receives 5 int64 per
+ * configuration: n1, n2, l1, l2, eqv (0/1), when non-NULL.  Returns the
+ * count, or BS2E_EINVAL when cap is smaller. */
+int64_t bs2e_count_configs(int64_t L, int64_t term_pi, int64_t max_l_1p, int64_t n_b, int64_t k_spline,
+                           int64_t max_n_b, int64_t n_all_l, int64_t l_2_max, int64_t *configs, int64_t cap);
+
+/* hamiltonian.f90:106-283 construct_block_tensor(H, S, b_splines, term,
+ * max_k, R_k, H_sp, S_sp, full), first half: takes the block's
+ * configurations (term%configs, as bs2e_count_configs writes them: each
+ * (l1,l2) channel contiguous, n1 ascending, n2 consecutive) and counts, on
+ * the device, the CSR entries of the Hamiltonian and overlap blocks against
+ * the computed tensor of `rk` (full = 0: upper triangle j >= i only).  The
+ * tensor must stay alive while the block is used. */
+int64_t bs2e_block_create(bs2e_rk *rk, int64_t L, int64_t n_config, const int64_t *configs, int64_t full,
+                          bs2e_block **out);
+/* sizes[0..4] = nnz of H, nnz of S (the entries the fill writes), the
+ * reference's count_nnz(b_splines, term, max_k, full) for H and S
+ * (hamiltonian.f90:348-416; it can be smaller than the fill, in which case
+ * the reference writes past its allocation), n_config. */
+int64_t bs2e_block_sizes(const bs2e_block *b, int64_t *sizes, int64_t n);
+/* hamiltonian.f90:106-283 second half: fills the 1-based CSR arrays of H_sp
+ * and S_sp (index_ptr: n_config+1, indices/data: nnz; data double complex as
+ * interleaved re/im) with mat_els.f90:608-633 c_mat_neq_tens (angular sums
+ * of R^k, :552-571 r_12_tens), :697-715 H_1p_neq and :664-678 S_mat_neq.
+ * Hl: max_l_1p+1 one-electron blocks H(l)%data, each n_b x n_b complex
+ * column-major (l = 0 first); S: n_b x n_b complex column-major. */
+int64_t bs2e_block_fill(bs2e_block *b, int64_t max_l_1p, const double *Hl, const double *S, int64_t *ip_H,
+                        int64_t *ix_H, double *d_H, int64_t *ip_S, int64_t *ix_S, double *d_S);
+int64_t bs2e_block_release(bs2e_block *b);
+
 #ifdef __cplusplus
 }
 #endif

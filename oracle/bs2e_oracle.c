@@ -781,11 +781,13 @@ void or_setup_H1p(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i64 Z, i64 l,
 /*                     smaller than the fill; the reference then writes past */
 /*                     its allocation).                                      */
 /* ------------------------------------------------------------------------ */
-static double r12(i64 K, const double *vals, i64 L, i64 l1a, i64 l1b, i64 l2a, i64 l2b)
+/* ang_k_LS is a pure function of (k, l's, L): the block assembly reads it
+ * from a table filled once per block with the same routine.                */
+static double r12(i64 K, const double *vals, const double *ang_row)
 {
     double res = 0.0;
     for (i64 k = 0; k < K; ++k) {                                  /* :566-570 */
-        double ang = or_ang_k_LS(k, l1a, l1b, l2a, l2b, L);
+        double ang = ang_row[k];
         if (fabs(ang) < 5e-16) continue;
         res = res + vals[k] * ang;
     }
@@ -802,6 +804,20 @@ i64 or_construct_block(i64 k_s, i64 n_b, i64 L, i64 max_k, i64 full, i64 n_confi
     i64 cntH = 0, cntS = 0;                  /* count_nnz (:348-416) */
     int fill = (ip_H != NULL);
     if (fill) { ip_H[0] = 1; ip_S[0] = 1; }
+    i64 ml = 0;
+    for (i64 i = 0; i < n_config; ++i) {
+        if (cf[5 * i + 2] > ml) ml = cf[5 * i + 2];
+        if (cf[5 * i + 3] > ml) ml = cf[5 * i + 3];
+    }
+    const i64 nl = ml + 1;
+    double *angt = malloc(sizeof(double) * (size_t)(nl * nl * nl * nl * K));
+    for (i64 la = 0; la < nl; ++la)
+        for (i64 lb = 0; lb < nl; ++lb)
+            for (i64 lc = 0; lc < nl; ++lc)
+                for (i64 ld = 0; ld < nl; ++ld)
+                    for (i64 k = 0; k < K; ++k)
+                        angt[(((la * nl + lb) * nl + lc) * nl + ld) * K + k] = or_ang_k_LS(k, la, lb, lc, ld, L);
+#define ANGROW(a, b, c, d) (angt + ((((a) * nl + (b)) * nl + (c)) * nl + (d)) * K)
 #define GETV(a, b, c, d) (R + ((or_pair_index(k_s, n_b, (a), (c)) * np + or_pair_index(k_s, n_b, (b), (d))) * K))
 #define HL(l, r, c) Hl[(l) * n_b * n_b + ((c) - 1) * n_b + ((r) - 1)]
 #define SM(r, c) S[((c) - 1) * n_b + ((r) - 1)]
@@ -813,20 +829,21 @@ i64 or_construct_block(i64 k_s, i64 n_b, i64 L, i64 max_k, i64 full, i64 n_confi
             int support = llabs(n_a - n_c) < k_s && llabs(n_bb - n_d) < k_s;
             int support_ex = llabs(n_a - n_d) < k_s && llabs(n_bb - n_c) < k_s;
             /* reference count_nnz predicate */
+            const double *ad = ANGROW(l_a, l_b, l_c, l_d), *ae = ANGROW(l_a, l_b, l_d, l_c);
             if (support) {
                 if (l_a == l_c && l_b == l_d) ++cntS;
                 for (i64 k = 0; k <= max_k; ++k)
-                    if (fabs(or_ang_k_LS(k, l_a, l_b, l_c, l_d, L)) > 5e-15) { ++cntH; break; }
+                    if (fabs(ad[k]) > 5e-15) { ++cntH; break; }
             } else if (support_ex) {
                 if (l_a == l_d && l_b == l_c) ++cntS;
                 for (i64 k = 0; k <= max_k; ++k)
-                    if (fabs(or_ang_k_LS(k, l_a, l_b, l_d, l_c, L)) > 5e-15) { ++cntH; break; }
+                    if (fabs(ae[k]) > 5e-15) { ++cntH; break; }
             }
             if (!(support || support_ex)) continue;
             int r12_allowed = 0;
             for (i64 k = 0; k <= max_k; ++k) {                         /* :171-178 */
-                double ang = or_ang_k_LS(k, l_a, l_b, l_c, l_d, L);
-                double ang_ex = or_ang_k_LS(k, l_a, l_b, l_d, l_c, L);
+                double ang = ad[k];
+                double ang_ex = ae[k];
                 if ((fabs(ang) > 5e-15 && support) || (fabs(ang_ex) > 5e-15 && support_ex)) {
                     r12_allowed = 1;
                     break;
@@ -840,9 +857,8 @@ i64 or_construct_block(i64 k_s, i64 n_b, i64 L, i64 max_k, i64 full, i64 n_confi
                 double complex h = 0.0;
                 if (r12_allowed) {                                     /* c_mat_neq_tens */
                     double res = 0.0;
-                    if (support) res = res + r12(K, GETV(n_a, n_bb, n_c, n_d), L, l_a, l_b, l_c, l_d);
-                    if (support_ex)
-                        res = res + isign(l_c + l_d + L) * r12(K, GETV(n_a, n_bb, n_d, n_c), L, l_a, l_b, l_d, l_c);
+                    if (support) res = res + r12(K, GETV(n_a, n_bb, n_c, n_d), ad);
+                    if (support_ex) res = res + isign(l_c + l_d + L) * r12(K, GETV(n_a, n_bb, n_d, n_c), ae);
                     h = h + res;
                 }
                 if (one_el) {
@@ -857,12 +873,12 @@ i64 or_construct_block(i64 k_s, i64 n_b, i64 L, i64 max_k, i64 full, i64 n_confi
                         s1 = s1 + isign(L + l_c + l_d) * SM(n_a, n_d) * SM(n_bb, n_c);
                     }
                     h = h + h1;
-                    if (ptrS >= cap_S) return -2;
+                    if (ptrS >= cap_S) { free(angt); return -2; }
                     dS[ptrS] = s1;
                     ix_S[ptrS] = j + 1;
                 }
                 if (entry) {
-                    if (ptrH >= cap_H) return -1;
+                    if (ptrH >= cap_H) { free(angt); return -1; }
                     dH[ptrH] = h;
                     ix_H[ptrH] = j + 1;
                 }
@@ -875,6 +891,8 @@ i64 or_construct_block(i64 k_s, i64 n_b, i64 L, i64 max_k, i64 full, i64 n_confi
 #undef GETV
 #undef HL
 #undef SM
+#undef ANGROW
+    free(angt);
     if (nnz) { nnz[0] = ptrH; nnz[1] = ptrS; }
     if (ref_cnt) { ref_cnt[0] = cntH; ref_cnt[1] = cntS; }
     return 0;

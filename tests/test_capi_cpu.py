@@ -67,6 +67,26 @@ def test_angular_matches_oracle(oracle):
         assert abs(bs2e.three_j(a, b, c, 0, 0, 0) - oracle.three_j(a, b, c, 0, 0, 0)) < 1e-14
 
 
+@pytest.mark.parametrize("L", [0, 1, 2])
+def test_count_configs_bit_exact(oracle, L):
+    """orbital_tools.f90:119-216 through the C ABI (host code): the compiled
+    reference's configurations, with and without the truncations."""
+    import bs2e
+    t = FX["knots_c1"]
+    _, n_b, _ = oracle.dims(7, t)
+    term = bs2e.sym(l=L, pi=bool(L % 2))
+    assert np.array_equal(bs2e.count_configs(term, 2, n_b, 7, n_b, n_b, 2), FX[f"cfg_c1_L{L}"])
+    n2, nall = (int(v) for v in FX["c1_find_max_n_b"])
+    assert np.array_equal(bs2e.count_configs(term, 3, n_b, 7, n2, nall, 1), FX[f"cfg_c1_trunc_L{L}"])
+    # odd parity, and a cap that is too small
+    odd = bs2e.sym(l=L, pi=not bool(L % 2))
+    assert np.array_equal(bs2e.count_configs(odd, 2, n_b, 7, n_b, n_b, 2),
+                          oracle.count_configs(L, 1 - L % 2, 2, n_b, 7, n_b, n_b, 2))
+    buf = np.zeros(5 * 3, np.int64)
+    assert capi.lib().bs2e_count_configs(L, L % 2, 2, n_b, 7, n_b, n_b, 2, capi.ptr(buf, C.c_int64), 3) \
+        == capi.BS2E_EINVAL
+
+
 def test_error_paths_without_device():
     L = capi.lib()
     h = C.c_void_p()

@@ -1,0 +1,120 @@
+"""GPU parity of the symmetry-block assembly (construct_block_tensor on the
+device tensor) against the oracle's restatement of hamiltonian.f90:106-283.
+
+Bar: index_ptr and indices bit-exact; S data bit-exact (the same products of
+the same inputs in the same order, no fused multiply-adds); H data within
+1e-12 of the block's largest entry (the R^k values under it differ from the
+oracle's by ~1e-15 relative, and direct and exchange terms can cancel, so an
+entry-wise relative bound would be ill-conditioned).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import bs2e  # noqa: E402
+from bs2e import capi  # noqa: E402
+from bs2e.workloads import WORKLOADS  # noqa: E402
+
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return bs2e.Context(0)
+
+
+@pytest.fixture(scope="module")
+def c1(ctx, oracle):
+    w = WORKLOADS["c1"]
+    t = bs2e.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+    n_b = oracle.dims(w.k, t)[1]
+    rk = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx)
+    off = oracle.slater_off_diag(w.k, t, w.k_GL, w.max_k)
+    dg = oracle.slater_diag(w.k, t, w.k_GL, w.max_k)
+    Rref = oracle.rk_dense(w.k, n_b, w.max_k, off, dg)
+    S = oracle.setup_S(w.k, t, w.k_GL)
+    Hl = [oracle.setup_H1p(w.k, t, w.k_GL, 2, l, cap_eta=0.0) for l in range(3)]
+    return w, t, n_b, rk, Rref, S, Hl
+
+
+def compare(got_H, got_S, ref):
+    ipH, ixH, dH = ref["H"]
+    ipS, ixS, dS = ref["S"]
+    assert np.array_equal(got_H.index_ptr, ipH)
+    assert np.array_equal(got_H.indices, ixH)
+    assert np.array_equal(got_S.index_ptr, ipS)
+    assert np.array_equal(got_S.indices, ixS)
+    assert np.array_equal(got_S.data, dS)
+    err = np.abs(got_H.data - dH).max() if dH.size else 0.0
+    assert err <= TOL * np.abs(dH).max(), err
+
+
+@pytest.mark.parametrize("L", [0, 1, 2])
+@pytest.mark.parametrize("full", [True, False])
+def test_block_matches_oracle(c1, oracle, L, full):
+    w, t, n_b, rk, Rref, S, Hl = c1
+    term = bs2e.sym(l=L, pi=bool(L % 2))
+    term.configs = bs2e.count_configs(term, 2, n_b, w.k, n_b, n_b, 2)
+    print(f"block c1 L={L} full={full} n_config={term.n_config}", flush=True)
+    blk = bs2e.Block(rk, term, full)
+    ref = oracle.construct_block(w.k, n_b, L, w.max_k, full, term.configs, Rref, Hl, S)
+    assert (blk.nnz_H, blk.nnz_S) == ref["nnz"]
+    assert (blk.ref_nnz_H, blk.ref_nnz_S) == ref["ref_count"]
+    H_sp, S_sp = blk.fill(Hl, S)
+    blk.release()
+    compare(H_sp, S_sp, ref)
+
+
+def test_block_truncated_configs_and_cap(c1, oracle):
+    """Truncated configuration lists (max_n_b, n_all_l, l_2_max) and a complex
+    absorbing potential in the one-electron blocks."""
+    w, t, n_b, rk, Rref, S, _ = c1
+    Hc = [oracle.setup_H1p(w.k, t, w.k_GL, 2, l, cap_order=2, cap_r0=30.0, cap_eta=0.02 + 0.005j) for l in range(4)]
+    term = bs2e.sym(l=1, pi=True)
+    term.configs = bs2e.count_configs(term, 3, n_b, w.k, 40, 30, 1)
+    print(f"block c1 truncated n_config={term.n_config}", flush=True)
+    H_sp, S_sp = bs2e.construct_block_tensor(Hc, S, bs2e.b_spline(w.k, t), term, w.max_k, rk, full=True)
+    ref = oracle.construct_block(w.k, n_b, 1, w.max_k, True, term.configs, Rref, Hc, S)
+    compare(H_sp, S_sp, ref)
+    assert np.abs(H_sp.data.imag).max() > 0.0
+
+
+def test_helium_ground_state_on_device_blocks(c1):
+    """The device tensor and block give He 1s^2 1S in the spd limit
+    (-2.902767, Carroll, Silverstone and Metzger 1979)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    w, t, n_b, rk, _, S, Hl = c1
+    term = bs2e.sym(l=0, pi=False)
+    term.configs = bs2e.count_configs(term, 2, n_b, w.k, n_b, n_b, 2)
+    H_sp, S_sp = bs2e.construct_block_tensor(Hl, S, bs2e.b_spline(w.k, t), term, w.max_k, rk)
+    n = term.n_config
+    H = sp.csr_matrix((H_sp.data.real, H_sp.indices - 1, H_sp.index_ptr - 1), shape=(n, n))
+    M = sp.csr_matrix((S_sp.data.real, S_sp.indices - 1, S_sp.index_ptr - 1), shape=(n, n))
+    E = sla.eigsh(H.tocsc(), k=1, M=M.tocsc(), sigma=-3.0, which="LM", return_eigenvectors=False)
+    assert abs(E[0] + 2.902767) < 2e-6
+
+
+def test_block_errors(c1, ctx):
+    w, t, n_b, rk, _, S, Hl = c1
+    term = bs2e.sym(l=0, pi=False)
+    cf = bs2e.count_configs(term, 2, n_b, w.k, n_b, n_b, 2)
+    # channels must be contiguous runs and n2 consecutive (count_configs order)
+    bad = cf.copy()
+    bad[[0, -1]] = bad[[-1, 0]]
+    term.configs = bad
+    with pytest.raises(capi.Bs2eError) as e:
+        bs2e.Block(rk, term)
+    assert e.value.code == capi.BS2E_EINVAL
+    term.configs = cf
+    blk = bs2e.Block(rk, term)
+    with pytest.raises(capi.Bs2eError):
+        blk.fill(Hl[:2], S)        # l = 2 channel without its one-electron block
+    blk.release()
+    # a tensor that has not been computed
+    fresh = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx, compute=False)
+    with pytest.raises(capi.Bs2eError) as e:
+        bs2e.Block(fresh, term)
+    assert e.value.code == capi.BS2E_ESTATE
+    fresh.release()

@@ -11,6 +11,9 @@
   textbook hydrogenic Slater integrals of Z=2 orbitals.
 * Angular algebra: sympy's exact 3j/6j.
 * One-electron Hamiltonian: hydrogenic levels -Z^2 / (2 n^2).
+* Symmetry blocks (construct_block_tensor): the helium ground state in the
+  s-, sp- and spd-partial-wave limits and the 1s2p 1P / 1s3d 1D levels, the
+  symmetry of H and S, and the upper-triangle (full = .false.) layout.
 """
 import numpy as np
 import pytest
@@ -239,3 +242,72 @@ def test_ang_k_LS_closed_form(oracle):
                (-1) ** la * np.sqrt((2 * la + 1) * (2 * lc + 1)) * float(wigner_3j(la, k, lc, 0, 0, 0)) *
                (-1) ** lb * np.sqrt((2 * lb + 1) * (2 * ld + 1)) * float(wigner_3j(lb, k, ld, 0, 0, 0)))
         assert abs(oracle.ang_k_LS(k, (la, lb), (lc, ld), L) - ref) < 1e-14
+
+
+# ------------------------------------------------------------------ blocks
+def _csr(b, key, n):
+    import scipy.sparse as sp
+    ip, ix, d = b[key]
+    return sp.csr_matrix((d, ix - 1, ip - 1), shape=(n, n))
+
+
+def _lowest(H, S, m=1):
+    import scipy.sparse.linalg as sla
+    E = sla.eigsh(H.real.tocsc(), k=m, M=S.real.tocsc(), sigma=-3.0, which="LM", return_eigenvectors=False)
+    return np.sort(E)
+
+
+@pytest.fixture(scope="module")
+def c1_one_electron(oracle, c1_tensor):
+    w, t, n_b, R, _ = c1_tensor
+    S = oracle.setup_S(w.k, t, w.k_GL)
+    Hl = [oracle.setup_H1p(w.k, t, w.k_GL, 2, l, cap_eta=0.0) for l in range(3)]
+    return S, Hl
+
+
+@pytest.mark.parametrize("max_l,exact,tol", [(0, -2.8790287673, 2e-8), (1, -2.900516, 2e-6), (2, -2.902767, 2e-6)])
+def test_helium_partial_wave_limits(oracle, c1_tensor, c1_one_electron, max_l, exact, tol):
+    """He 1s^2 1S in the l <= max_l partial-wave limits (Carroll, Silverstone
+    and Metzger, J. Chem. Phys. 71, 4142 (1979): -2.8790287673, -2.900516,
+    -2.902767).  A wrong exchange sign, a dropped direct or exchange term or a
+    transposed index moves these by more than 1e-3."""
+    w, t, n_b, R, _ = c1_tensor
+    S, Hl = c1_one_electron
+    cf = oracle.count_configs(0, 0, max_l, n_b, w.k, n_b, n_b, max_l)
+    b = oracle.construct_block(w.k, n_b, 0, w.max_k, True, cf, R, Hl, S)
+    H, SS = _csr(b, "H", len(cf)), _csr(b, "S", len(cf))
+    assert abs(_lowest(H, SS)[0] - exact) < tol
+
+
+@pytest.mark.parametrize("L,pi,exact", [(1, 1, -2.1238431), (2, 0, -2.0556207)])
+def test_helium_singly_excited(oracle, c1_tensor, c1_one_electron, L, pi, exact):
+    """1s2p 1P^o and 1s3d 1D^e of helium (Drake's non-relativistic values);
+    the l <= 2 basis leaves a few 1e-4 Ha."""
+    w, t, n_b, R, _ = c1_tensor
+    S, Hl = c1_one_electron
+    cf = oracle.count_configs(L, pi, 2, n_b, w.k, n_b, n_b, 2)
+    b = oracle.construct_block(w.k, n_b, L, w.max_k, True, cf, R, Hl, S)
+    E = _lowest(_csr(b, "H", len(cf)), _csr(b, "S", len(cf)))[0]
+    assert abs(E - exact) < 5e-4
+
+
+@pytest.mark.parametrize("L", [0, 1])
+def test_block_symmetry_and_upper_triangle(oracle, c1_tensor, c1_one_electron, L):
+    """H and S are real symmetric without a CAP; full=.false. keeps exactly
+    the j >= i part of the full block; count_nnz never exceeds the fill."""
+    w, t, n_b, R, _ = c1_tensor
+    S, Hl = c1_one_electron
+    cf = oracle.count_configs(L, L % 2, 2, n_b, w.k, n_b, n_b, 2)
+    n = len(cf)
+    full = oracle.construct_block(w.k, n_b, L, w.max_k, True, cf, R, Hl, S)
+    up = oracle.construct_block(w.k, n_b, L, w.max_k, False, cf, R, Hl, S)
+    for key in ("H", "S"):
+        A = _csr(full, key, n)
+        assert abs(A - A.T).max() <= 1e-14 * abs(A).max()
+        U = _csr(up, key, n)
+        import scipy.sparse as sp
+        assert (sp.triu(A) != U).nnz == 0
+        ip, ix, _ = up[key]
+        rows = np.repeat(np.arange(1, n + 1), np.diff(ip))
+        assert (ix >= rows).all()
+    assert full["ref_count"][0] <= full["nnz"][0] and full["ref_count"][1] <= full["nnz"][1]
