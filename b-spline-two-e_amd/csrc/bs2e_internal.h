@@ -49,17 +49,6 @@ struct DiagEntry {
     int64_t ptr;                // 0-based first position in sparse_6d order
 };
 
-// Per tensor row p: the column pairs q whose windows overlap W_p are exactly
-// the q in [qlo, qhi] that overlap; every q outside is a rank-one element.
-struct NearRange {
-    int32_t qlo, qhi;
-};
-
-// One near workgroup: elements [e0, e1) (e = q*K + k) of row p.
-struct NearItem {
-    int32_t p, e0, e1, pad;
-};
-
 // Host-side description of one basis (knots + quadrature + multipoles).
 struct Plan {
     int64_t k_s = 0, n_knots = 0, n = 0, n_b = 0, n_iv = 0;
@@ -70,14 +59,14 @@ struct Plan {
     std::vector<int64_t> off4;          // per pair: first sparse_4d position
     std::vector<int32_t> c_first;       // first pair index with column c (1-based c, size n_b+2)
     std::vector<DiagEntry> diag;        // sparse_6d order (built on demand)
-    std::vector<NearRange> near;        // per row (built with the tensor)
-    std::vector<NearItem> items;        // near workgroups of the tensor kernel
+    std::vector<int32_t> rows;          // near-kernel rows: pairs (a, c) with a <= c (built with the tensor)
+    int64_t near_elems = 0;             // tensor elements whose pairs share a support interval
 };
 
 // Builds the plan; returns an error message or empty string.
 std::string make_plan(int64_t k_s, int64_t n_knots, const double* knots, int64_t k_gl, int64_t max_k, Plan& p);
 void build_diag_entries(Plan& p);
-void build_near_ranges(Plan& p);
+void build_near_rows(Plan& p);
 
 // Host numerics shared by the C-ABI (setup, not the hot path).
 void gauss_legendre(int64_t N, double* x, double* w);
@@ -93,8 +82,7 @@ struct DevTables {
     double *Pl = nullptr, *Ql = nullptr;
     double *Pw = nullptr, *Qw = nullptr, *PP = nullptr, *QS = nullptr, *Ptot = nullptr, *Qtot = nullptr;
     double* D = nullptr;
-    NearRange* near = nullptr;
-    NearItem* items = nullptr;
+    int32_t* rows = nullptr;
     int32_t* c_first = nullptr;
 };
 
@@ -147,9 +135,10 @@ hipError_t launch_outer_moments(const Plan& p, const DevTables& d, hipStream_t s
 hipError_t launch_diag_cells(const Plan& p, const DevTables& d, hipStream_t s, bool symmetrised);  // D / Dsym
 hipError_t launch_symmetrize(const Plan& p, const DevTables& d, hipStream_t s);                    // D -> Dsym
 size_t near_lds_bytes(int64_t ks, int64_t K);
+constexpr int kNearMaxSeg = 64;   // column segments of one near row (<= 2k - 1 <= 39 for k <= 20)
 hipError_t launch_pair_windows(const Plan& p, const DevTables& d, hipStream_t s);            // Pw/Qw + sums
 hipError_t launch_tensor_far(const Plan& p, const DevTables& d, double* R, hipStream_t s);   // rank-one part
-hipError_t launch_tensor_near(const Plan& p, const DevTables& d, double* R, hipStream_t s);  // near ranges
+hipError_t launch_tensor_near(const Plan& p, const DevTables& d, double* R, hipStream_t s);  // overlapping pairs
 size_t outer_lds_bytes(int64_t ks, int64_t kgl, int64_t K);
 size_t inner_lds_bytes(int64_t ks, int64_t kgl, int64_t kc);
 int inner_kc(int64_t ks, int64_t kgl, int64_t K);   // multipoles per diagonal-cell workgroup (0: unsupported)

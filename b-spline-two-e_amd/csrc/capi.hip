@@ -131,8 +131,7 @@ static void free_tables(bs2e_rk* rk)
     dev_free(d.Ptot);
     dev_free(d.Qtot);
     dev_free(d.D);
-    dev_free(d.near);
-    dev_free(d.items);
+    dev_free(d.rows);
     dev_free(d.c_first);
     dev_free(rk->R);
     for (auto& e : rk->ev)
@@ -157,7 +156,7 @@ static int64_t create_events(bs2e_ctx* ctx, bs2e_rk* rk)
 
 static int64_t alloc_tables(bs2e_ctx* ctx, bs2e_rk* rk, bool with_tensor)
 {
-    if (with_tensor) build_near_ranges(rk->plan);
+    if (with_tensor) build_near_rows(rk->plan);
     const Plan& p = rk->plan;
     DevTables& d = rk->d;
     const size_t slots = (size_t)p.n_pairs * p.k_s * p.K;
@@ -178,17 +177,14 @@ static int64_t alloc_tables(bs2e_ctx* ctx, bs2e_rk* rk, bool with_tensor)
     if (with_tensor) {
         rk->R_elems = (size_t)p.n_pairs * p.n_pairs * p.K;
         HIP_TRY(ctx, dev_alloc(&rk->R, rk->R_elems));
-        HIP_TRY(ctx, dev_alloc(&d.near, p.near.size()));
-        HIP_TRY(ctx, dev_alloc(&d.items, p.items.size()));
+        HIP_TRY(ctx, dev_alloc(&d.rows, p.rows.size()));
     }
     HIP_TRY(ctx, dev_alloc(&d.c_first, p.c_first.size()));
     hipStream_t s = ctx->stream;
     HIP_TRY(ctx, hipMemcpyAsync(d.c_first, p.c_first.data(), sizeof(int32_t) * p.c_first.size(), hipMemcpyHostToDevice,
                                 s));
     if (with_tensor) {
-        HIP_TRY(ctx, hipMemcpyAsync(d.near, p.near.data(), sizeof(NearRange) * p.near.size(), hipMemcpyHostToDevice, s));
-        HIP_TRY(ctx,
-                hipMemcpyAsync(d.items, p.items.data(), sizeof(NearItem) * p.items.size(), hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(d.rows, p.rows.data(), sizeof(int32_t) * p.rows.size(), hipMemcpyHostToDevice, s));
     }
     HIP_TRY(ctx, hipMemcpyAsync(d.t, p.t.data(), sizeof(double) * p.n_knots, hipMemcpyHostToDevice, s));
     HIP_TRY(ctx, hipMemcpyAsync(d.bp, p.bp.data(), sizeof(double) * (p.n_iv + 1), hipMemcpyHostToDevice, s));
@@ -211,8 +207,8 @@ static int64_t check_plan_limits(bs2e_ctx* ctx, const Plan& p, bool needs_primit
         return fail(ctx, BS2E_EINVAL, "diagonal-cell table too large (n_iv*U^2*K >= 2^31)");
     if (near_lds_bytes(p.k_s, p.K) > 64 * 1024)
         return fail(ctx, BS2E_EINVAL,
-                    "spline order x multipole count too large for the near-range kernel's LDS "
-                    "(8 rows x (2k+2) x (max_k+1) doubles must fit 64 KiB)");
+                    "spline order x multipole count too large for the near kernel's LDS "
+                    "((2k+2) x (max_k+1) doubles must fit 64 KiB)");
     return BS2E_OK;
 }
 
@@ -487,9 +483,7 @@ int64_t bs2e_rk_info(const bs2e_rk* rk, int64_t* info, int64_t n)
 {
     if (!rk || !info) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_info: NULL argument");
     const Plan& p = rk->plan;
-    int64_t near_elems = 0;
-    for (const NearRange& r : p.near) near_elems += (int64_t)(r.qhi - r.qlo + 1) * p.K;
-    const int64_t v[11] = {p.n_b, p.n_pairs, p.K, p.nnz4, p.nnz6, p.n_iv, p.k_s, p.k_gl, p.U, p.n_knots, near_elems};
+    const int64_t v[11] = {p.n_b, p.n_pairs, p.K, p.nnz4, p.nnz6, p.n_iv, p.k_s, p.k_gl, p.U, p.n_knots, p.near_elems};
     for (int64_t i = 0; i < n && i < 11; ++i) info[i] = v[i];
     return BS2E_OK;
 }

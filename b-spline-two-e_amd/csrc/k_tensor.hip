@@ -13,140 +13,137 @@
 // with PP_p / QS_p the prefix / suffix sums of P / Q over W_p and
 // Dsym[v] = D[v] + D[v]^T the symmetrised diagonal cell.
 //
-// Rows are grouped in blocks of kNearRows consecutive pairs.  For each block
-// the column pairs whose windows overlap the window of any of its rows form
-// the index range [qlo, qhi] (the block's "near" range, a few percent of the
-// row).  Two kernels write disjoint elements, each element exactly once:
-//   * k3_far : kRows rows x one column chunk of rank-one products, streamed
-//     with non-temporal stores, skipping the block's near range.  Bound by
+// Two kernels write disjoint elements, each element exactly once:
+//   * k3_far : kRows rows x one column chunk of rank-one products (the pairs'
+//     windows do not overlap), streamed with non-temporal stores.  Bound by
 //     the FP64 store of R: 8 bytes per integral.
-//   * k3_near: one block x one chunk of its near range, general formula.
-//     Each lane loads its column pair's window once into registers and reuses
-//     it for all rows of the block, whose prefix/suffix sums sit in LDS; the
-//     diagonal cells are one load per common interval.  It runs on a second
-//     stream, concurrently with k3_far.
+//   * k3_near: the elements whose pairs share a support interval (a few
+//     percent of each row).  Rows p = (a,c) and (c,a) are identical
+//     (R(a,b,c,d) = R(c,b,a,d)), so one workgroup evaluates row (a,c), a <= c,
+//     and stores it into both rows.  With W_p = [lo, hi], for column q:
+//       R = Qtot_p * sum_{w in W_q, w < lo} P_q(w)
+//         + Ptot_p * sum_{w in W_q, w > hi} Q_q(w)
+//         + sum_{w in W_p and W_q} Q_q(w) PP_p(w-1) + P_q(w) QS_p(w+1) + Dsym[w; p, q]
+//     (the general formula with PP_p = 0 left of W_p and Ptot_p right of it,
+//     QS_p = Qtot_p left of W_p and 0 right of it); the first two sums are
+//     q's own prefix / suffix sums.  It runs on a second stream, concurrently
+//     with k3_far.
 #include "bs2e_internal.h"
 
 namespace bs2e {
 
 constexpr int kThreads = 256;
-constexpr int kRows = 4;          // far workgroup: tensor rows (divides kNearRows)
+constexpr int kRows = 4;          // far workgroup: tensor rows
 constexpr int kFarIters = 8;      // far workgroup: elements per thread per row
-constexpr int kNearRows = 8;      // near block: tensor rows
-constexpr int kNearIters = 2;     // near workgroup: elements per thread
 
-static_assert(kNearRows % kRows == 0, "far row groups must not straddle near blocks");
-
-// LDS of a near workgroup, per row r of the block:
-//   PPx[r][i][k], i in [0, m]: 0, PP[0], ..., PP[m-1] (= Ptot)
-//   QSx[r][i][k], i in [0, m]: QS[0] (= Qtot), ..., QS[m-1], 0
-// so that, with i clamped to [0, m], PP_p(x) = PPx[x - v0p + 1] and
-// QS_p(x) = QSx[x - v0p] for every interval x; and the local pair index of
-// the row's pair on each interval of its window.
+// LDS of a near workgroup (row p, window length m):
+//   PPx[i][k], i in [0, m]: 0, PP[0], ..., PP[m-1] (= Ptot)
+//   QSx[i][k], i in [0, m]: QS[0] (= Qtot), ..., QS[m-1], 0
+// so that PP_p(w-1) = PPx[w - lo] and QS_p(w+1) = QSx[w - lo + 1] for w in
+// W_p; the local pair index of p on each interval of W_p; and the row's
+// column segments (one run of consecutive pairs per column spline d).
 size_t near_lds_bytes(int64_t ks, int64_t K)
 {
-    return sizeof(double) * (size_t)(kNearRows * 2 * (ks + 1) * K) + sizeof(int) * (size_t)(kNearRows * kMaxOrder) +
-           sizeof(PairInfo) * kNearRows;
+    return sizeof(double) * (size_t)(2 * (ks + 1) * K) + sizeof(int) * (size_t)(kMaxOrder + 2 * kNearMaxSeg + 4);
 }
 
-template <int KSMAX>
-__global__ __launch_bounds__(kThreads) void k3_near(int n_pairs, int K, int ks, int U,
+__device__ __forceinline__ int pair_at(const int32_t* __restrict__ c_first, int ks, int a, int c)
+{
+    const int lo = (c - ks + 1) < 1 ? 1 : (c - ks + 1);
+    return c_first[c] + (a - lo);
+}
+
+__global__ __launch_bounds__(kThreads) void k3_near(int n_pairs, int n_b, int K, int ks, int U,
                                                     const PairInfo* __restrict__ pairs,
-                                                    const NearItem* __restrict__ items,
+                                                    const int32_t* __restrict__ c_first,
+                                                    const int32_t* __restrict__ rows,
                                                     const double* __restrict__ Pw, const double* __restrict__ Qw,
                                                     const double* __restrict__ PP, const double* __restrict__ QS,
-                                                    const double* __restrict__ Ptot, const double* __restrict__ Qtot,
                                                     const double* __restrict__ Dsym, double* __restrict__ R)
 {
     extern __shared__ double lds[];
-    const int rowdata = 2 * (ks + 1) * K;
-    int* sUp = (int*)(lds + kNearRows * rowdata);              // [kNearRows][kMaxOrder]
-    PairInfo* sPair = (PairInfo*)(sUp + kNearRows * kMaxOrder);
-    const int64_t rowlen = (int64_t)n_pairs * K;
+    double* sPPx = lds;                                    // [(ks+1)][K]
+    double* sQSx = lds + (ks + 1) * K;                     // [(ks+1)][K]
+    int* sUp = (int*)(lds + 2 * (ks + 1) * K);             // [kMaxOrder]
+    int* sQ0 = sUp + kMaxOrder;                            // [kNearMaxSeg]: first pair of each segment
+    int* sJ0 = sQ0 + kNearMaxSeg;                          // [kNearMaxSeg + 1]: segment starts in row order
+    int* sNseg = sJ0 + kNearMaxSeg + 1;
     const int tid = threadIdx.x;
-    const NearItem it = items[blockIdx.x];
-    const int p0 = it.p;
-    const int nrows = (n_pairs - p0) < kNearRows ? (n_pairs - p0) : kNearRows;
-    for (int e = tid; e < nrows * rowdata; e += kThreads) {
-        const int r = e / rowdata, o = e - r * rowdata;
-        const int half = (ks + 1) * K;
-        const int which = o >= half, oi = o - which * half;
-        const int i = oi / K, k = oi - i * K;
-        const PairInfo pr = pairs[p0 + r];
-        const size_t pb = (size_t)(p0 + r) * ks * K + k;
-        double val;
-        if (!which) val = (i == 0) ? 0.0 : PP[pb + (size_t)(i - 1) * K];
-        else val = (i >= pr.m) ? 0.0 : QS[pb + (size_t)i * K];
-        lds[e] = val;
+    const int p = rows[blockIdx.x];
+    const PairInfo pp = pairs[p];
+    const int lo = pp.v0, m = pp.m, hi = pp.v0 + pp.m - 1;
+    const int pt = (pp.a == pp.c) ? -1 : pair_at(c_first, ks, pp.c, pp.a);   // the identical row (c, a)
+    for (int e = tid; e < (ks + 1) * K; e += kThreads) {
+        const int i = e / K, k = e - i * K;
+        const size_t pb = (size_t)p * ks * K + k;
+        sPPx[e] = (i == 0) ? 0.0 : PP[pb + (size_t)(i - 1) * K];
+        sQSx[e] = (i >= m) ? 0.0 : QS[pb + (size_t)i * K];
     }
-    for (int e = tid; e < nrows * kMaxOrder; e += kThreads) {
-        const int r = e / kMaxOrder, j = e - r * kMaxOrder;
-        const PairInfo pr = pairs[p0 + r];
-        const int vz = pr.v0 - 1 + j;
-        sUp[e] = (j < pr.m) ? sym_local(pr.a - vz, pr.c - vz, ks) : 0;
+    if (tid < m) {
+        const int vz = lo - 1 + tid;
+        sUp[tid] = sym_local(pp.a - vz, pp.c - vz, ks);
     }
-    if (tid < nrows) sPair[tid] = pairs[p0 + tid];
+    if (tid == 0) {
+        // column pairs (b, d) share an interval with W_p iff
+        // max(b,d) + 2 - k <= hi and min(b,d) + 1 >= lo
+        const int dlo = max(1, lo - 1), dhi = min(n_b, hi + ks - 2);
+        int ns = 0, J = 0;
+        for (int d = dlo; d <= dhi; ++d) {
+            const int b0 = max(1, d - ks + 1);
+            const int blo = max(b0, lo - 1), bhi = min(min(d + ks - 1, n_b), hi + ks - 2);
+            if (blo > bhi) continue;
+            sQ0[ns] = c_first[d] + (blo - b0);
+            sJ0[ns] = J;
+            J += bhi - blo + 1;
+            ++ns;
+        }
+        sJ0[ns] = J;
+        *sNseg = ns;
+    }
     __syncthreads();
-
+    const int ns = *sNseg;
+    const int total = sJ0[ns] * K;
+    const int64_t rowlen = (int64_t)n_pairs * K;
+    double* Rp = R + (size_t)p * rowlen;
+    double* Rt = R + (size_t)(pt < 0 ? p : pt) * rowlen;
     const unsigned UK = (unsigned)(U * K);   // Dsym has n_iv*U*U*K < 2^31 entries (checked on the host)
-    int64_t e = it.e0 + tid;
-    int q = (int)(e / K);
-    int k = (int)(e - (int64_t)q * K);
-    const int dq = kThreads / K, dk = kThreads - (kThreads / K) * K;
-#pragma unroll 1
-    for (int i = 0; i < kNearIters; ++i) {
-        if (e >= it.e1) break;
+    for (int e = tid; e < total; e += kThreads) {
+        const int j = e / K, k = e - j * K;
+        int s = 0;                       // last segment starting at or before j
+        for (int step = kNearMaxSeg / 2; step > 0; step >>= 1)
+            if (s + step < ns && sJ0[s + step] <= j) s += step;
+        const int q = sQ0[s] + (j - sJ0[s]);
         const PairInfo pq = pairs[q];
-        const size_t qbase = (size_t)q * ks * K + k;
-        double pw[KSMAX], qw[KSMAX];
-        unsigned dbase[KSMAX];   // Dsym offset of (interval v0q+j, pair q) without the row's pair term
-#pragma unroll
-        for (int j = 0; j < KSMAX; ++j) {
-            const bool in = j < pq.m;
-            pw[j] = in ? Pw[qbase + (size_t)j * K] : 0.0;
-            qw[j] = in ? Qw[qbase + (size_t)j * K] : 0.0;
-            const int vz = pq.v0 - 1 + j;
-            dbase[j] = in ? ((unsigned)vz * U * UK + (unsigned)sym_local(pq.a - vz, pq.c - vz, ks) * K + k) : 0u;
+        const int v0q = pq.v0, endq = pq.v0 + pq.m - 1;
+        const size_t qb = (size_t)q * ks * K + k;
+        double val = 0.0;
+        if (v0q < lo) val = sQSx[k] * PP[qb + (size_t)(lo - 1 - v0q) * K];
+        if (endq > hi) val += sPPx[m * K + k] * QS[qb + (size_t)(hi + 1 - v0q) * K];
+        const int wlo = max(lo, v0q), whi = min(hi, endq);
+        double acc = 0.0, dacc = 0.0;
+#pragma unroll 2
+        for (int w = wlo; w <= whi; ++w) {
+            const int jq = w - v0q, jp = w - lo, vz = w - 1;
+            acc += Qw[qb + (size_t)jq * K] * sPPx[jp * K + k] + Pw[qb + (size_t)jq * K] * sQSx[(jp + 1) * K + k];
+            const unsigned uq = (unsigned)sym_local(pq.a - vz, pq.c - vz, ks);
+            dacc += Dsym[(unsigned)vz * U * UK + (unsigned)sUp[jp] * UK + uq * (unsigned)K + (unsigned)k];
         }
-#pragma unroll 1
-        for (int r = 0; r < nrows; ++r) {
-            const PairInfo pp = sPair[r];
-            const double* sPPx = lds + r * rowdata;
-            const double* sQSx = sPPx + (ks + 1) * K;
-            const int* up = sUp + r * kMaxOrder;
-            const int o = pq.v0 - pp.v0;   // q's window start relative to p's
-            double acc = 0.0, dacc = 0.0;
-#pragma unroll
-            for (int j = 0; j < KSMAX; ++j) {
-                // sum_{v<w} P_p(v) Q_q(w) + sum_{v>w} Q_p(v) P_q(w) at w = v0q + j
-                const int ip = min(max(o + j, 0), pp.m);        // PP_p(w-1)
-                const int is = min(max(o + j + 1, 0), pp.m);    // QS_p(w+1)
-                acc += qw[j] * sPPx[ip * K + k] + pw[j] * sQSx[is * K + k];
-                // diagonal cell of interval w if it lies in both windows
-                if (j < pq.m && (unsigned)(o + j) < (unsigned)pp.m) dacc += Dsym[dbase[j] + (unsigned)up[o + j] * UK];
-            }
-            __builtin_nontemporal_store(acc + dacc, R + (size_t)(p0 + r) * rowlen + e);
-        }
-        e += kThreads;
-        q += dq;
-        k += dk;
-        if (k >= K) {
-            k -= K;
-            q += 1;
-        }
+        val += acc + dacc;
+        const size_t o = (size_t)q * K + k;
+        Rp[o] = val;
+        if (pt >= 0) Rt[o] = val;
     }
 }
 
 // Far workgroups: kRows rows x one column chunk of rank-one products; the
-// elements of each row's (block's) near range are left to k3_near.
+// elements whose pairs' windows overlap are left to k3_near.
 __global__ __launch_bounds__(kThreads) void k3_far(int n_pairs, int K, int far_cols, const PairInfo* __restrict__ pairs,
-                                                   const NearRange* __restrict__ near,
                                                    const double* __restrict__ Ptot,
                                                    const double* __restrict__ Qtot, double* __restrict__ R)
 {
     __shared__ double sA[kRows * kMaxK];   // Ptot of the rows
     __shared__ double sB[kRows * kMaxK];   // Qtot of the rows
-    __shared__ int sI[4 * kRows];
+    __shared__ int sLo[kRows], sHi[kRows];
     const int64_t rowlen = (int64_t)n_pairs * K;
     const int tid = threadIdx.x;
     const int ry = (int)blockIdx.x / far_cols, cx = (int)blockIdx.x - ry * far_cols;
@@ -159,10 +156,8 @@ __global__ __launch_bounds__(kThreads) void k3_far(int n_pairs, int K, int far_c
     }
     if (tid < nrows) {
         const PairInfo pi = pairs[p0 + tid];
-        const NearRange nr = near[p0 + tid];
-        sI[4 * tid + 0] = pi.v0 + pi.m - 1;   // last interval of the row's window
-        sI[4 * tid + 1] = nr.qlo;
-        sI[4 * tid + 2] = nr.qhi;
+        sLo[tid] = pi.v0;
+        sHi[tid] = pi.v0 + pi.m - 1;   // last interval of the row's window
     }
     __syncthreads();
     int64_t e = (int64_t)cx * kThreads * kFarIters + tid;
@@ -174,10 +169,12 @@ __global__ __launch_bounds__(kThreads) void k3_far(int n_pairs, int K, int far_c
         if (e >= rowlen) break;
         const double pt = Ptot[e];
         const double qt = Qtot[e];
-        const int v0q = pairs[q].v0;
+        const PairInfo pq = pairs[q];
+        const int v1q = pq.v0 + pq.m - 1;
         for (int r = 0; r < nrows; ++r) {
-            if (q >= sI[4 * r + 1] && q <= sI[4 * r + 2]) continue;   // near range: written by k3_near
-            const double val = (sI[4 * r + 0] < v0q) ? sA[r * kMaxK + k] * qt : sB[r * kMaxK + k] * pt;
+            const bool left = sHi[r] < pq.v0, right = sLo[r] > v1q;
+            if (!left && !right) continue;   // windows overlap: written by k3_near
+            const double val = left ? sA[r * kMaxK + k] * qt : sB[r * kMaxK + k] * pt;
             __builtin_nontemporal_store(val, Rrow + (size_t)r * rowlen + e);
         }
         e += kThreads;
@@ -210,57 +207,32 @@ __global__ __launch_bounds__(256) void k_symmetrize(int n_iv, int U, int K, doub
     *b = s;
 }
 
-// Host: per-row window overlap ranges, merged per block of kNearRows rows, and
-// the near work items (chunks of kThreads*kNearIters elements of a block's
-// near range).
-void build_near_ranges(Plan& p)
+// Host: the near kernel's rows (pairs (a, c) with a <= c) and the number of
+// tensor elements whose pairs' windows overlap.
+void build_near_rows(Plan& p)
 {
-    p.near.assign((size_t)p.n_pairs, NearRange{0, -1});
-    p.items.clear();
+    p.rows.clear();
+    p.near_elems = 0;
     const int64_t ks = p.k_s, nb = p.n_b;
-    const int64_t chunk = (int64_t)kThreads * kNearIters;
-    for (int64_t B0 = 0; B0 < p.n_pairs; B0 += kNearRows) {
-        const int64_t B1 = std::min<int64_t>(p.n_pairs, B0 + kNearRows);
-        int32_t lo = INT32_MAX, hi = -1;
-        for (int64_t P = B0; P < B1; ++P) {
-            const PairInfo& pp = p.pairs[P];
-            const int64_t v1p = pp.v0 + pp.m - 1;
-            // a column pair (b, d) can only share an interval with W_p if
-            // d lies in [v0p - 1, v1p + k - 2]
-            const int64_t cmin = std::max<int64_t>(1, pp.v0 - 1);
-            const int64_t cmax = std::min<int64_t>(nb, v1p + ks - 2);
-            for (int64_t Q = p.c_first[cmin]; Q < p.c_first[cmax + 1]; ++Q) {
-                const PairInfo& pq = p.pairs[Q];
-                if (pq.v0 <= v1p && pp.v0 <= pq.v0 + pq.m - 1) {
-                    lo = std::min(lo, (int32_t)Q);
-                    hi = std::max(hi, (int32_t)Q);
-                }
-            }
+    for (int64_t P = 0; P < p.n_pairs; ++P) {
+        const PairInfo& pp = p.pairs[P];
+        if (pp.a <= pp.c) p.rows.push_back((int32_t)P);
+        const int64_t v1p = pp.v0 + pp.m - 1;
+        const int64_t cmin = std::max<int64_t>(1, pp.v0 - 1);
+        const int64_t cmax = std::min<int64_t>(nb, v1p + ks - 2);
+        for (int64_t Q = p.c_first[cmin]; Q < p.c_first[cmax + 1]; ++Q) {
+            const PairInfo& pq = p.pairs[Q];
+            if (pq.v0 <= v1p && pp.v0 <= pq.v0 + pq.m - 1) p.near_elems += p.K;
         }
-        for (int64_t P = B0; P < B1; ++P) p.near[P] = NearRange{lo, hi};
-        const int64_t e0 = (int64_t)lo * p.K, e1 = (int64_t)(hi + 1) * p.K;
-        for (int64_t e = e0; e < e1; e += chunk)
-            p.items.push_back(NearItem{(int32_t)B0, (int32_t)e, (int32_t)std::min(e1, e + chunk), 0});
     }
-}
-
-template <int KSMAX>
-static void launch_near(const Plan& p, const DevTables& d, double* R, hipStream_t s)
-{
-    if (p.items.empty()) return;
-    hipLaunchKernelGGL(k3_near<KSMAX>, dim3((unsigned)p.items.size()), dim3(kThreads), near_lds_bytes(p.k_s, p.K), s,
-                       (int)p.n_pairs, (int)p.K, (int)p.k_s, (int)p.U, d.pairs, d.items, d.Pw, d.Qw, d.PP, d.QS,
-                       d.Ptot, d.Qtot, d.D, R);
 }
 
 hipError_t launch_tensor_near(const Plan& p, const DevTables& d, double* R, hipStream_t s)
 {
-    if (p.k_s <= 6) launch_near<6>(p, d, R, s);
-    else if (p.k_s <= 8) launch_near<8>(p, d, R, s);
-    else if (p.k_s <= 10) launch_near<10>(p, d, R, s);
-    else if (p.k_s <= 12) launch_near<12>(p, d, R, s);
-    else if (p.k_s <= 16) launch_near<16>(p, d, R, s);
-    else launch_near<20>(p, d, R, s);
+    if (p.rows.empty()) return hipSuccess;
+    hipLaunchKernelGGL(k3_near, dim3((unsigned)p.rows.size()), dim3(kThreads), near_lds_bytes(p.k_s, p.K), s,
+                       (int)p.n_pairs, (int)p.n_b, (int)p.K, (int)p.k_s, (int)p.U, d.pairs, d.c_first, d.rows, d.Pw,
+                       d.Qw, d.PP, d.QS, d.D, R);
     return hipGetLastError();
 }
 
@@ -270,7 +242,7 @@ hipError_t launch_tensor_far(const Plan& p, const DevTables& d, double* R, hipSt
     const int far_cols = (int)((rowlen + kThreads * kFarIters - 1) / (kThreads * kFarIters));
     const int far_rows = (int)((p.n_pairs + kRows - 1) / kRows);
     hipLaunchKernelGGL(k3_far, dim3((unsigned)(far_cols * far_rows)), dim3(kThreads), 0, s, (int)p.n_pairs, (int)p.K,
-                       far_cols, d.pairs, d.near, d.Ptot, d.Qtot, R);
+                       far_cols, d.pairs, d.Ptot, d.Qtot, R);
     return hipGetLastError();
 }
 

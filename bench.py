@@ -61,6 +61,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--latency-samples", type=int, default=50,
+                    help="synchronised single builds timed for p50/p99 latency")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the run, copy a buffer of this size 3 times on the build stream "
+                         "(reference kernel for HBM counter scaling under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "bench.py measures one GPU: the R^k build of one basis runs on one device "
@@ -113,6 +118,24 @@ def main():
     far_elems = n_int - rk.near_elems
     traffic_alg = 8.0 * far_elems            # algorithmic bytes of one k3_far launch
     achieved = traffic_alg / t_far / 1e9
+    # build latency: one build submitted to an idle device, host clock to completion
+    lat = []
+    for _ in range(args.latency_samples):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        rk.compute()
+        rk.synchronize()
+        lat.append((time.perf_counter() - a) * 1e3)
+    lat = np.array(lat) if lat else np.zeros(1)
+    if args.copy_calibration_mib > 0:
+        n = args.copy_calibration_mib * (1 << 20) // 8
+        with torch.cuda.stream(stream):
+            src = torch.ones(n, dtype=torch.float64, device="cuda")
+            dst = torch.empty_like(src)
+            for _ in range(3):
+                dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     R_gpu = rk.to_host() if not args.no_cpu_baseline else None
 
     out = {
@@ -136,6 +159,8 @@ def main():
                         "build_done": float(st[4])},
         "tensor_stage_GBs": 8.0 * n_int / (float(st[4] - st[0]) * 1e-3) / 1e9,
         "event_ms_per_step": ev0.elapsed_time(ev1) / args.steps,
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "samples": int(args.latency_samples)},
         "roofline": {"bound": "hbm", "kernel": "k3_far (rank-one tensor fill)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes": traffic_alg, "elements": far_elems,
