@@ -220,8 +220,10 @@ class RkTensor:
         check(capi.lib().bs2e_rk_set_profiling(self.handle, int(on)), self.ctx.handle)
 
     def stage_times(self):
-        """Timeline of the last compute, ms since its start: [moments + windows done,
-        rank-one fill done, diagonal cells done, near ranges done, end]."""
+        """Device timeline averaged over the builds computed since profiling was
+        switched on (or since the previous call), ms since each build's start:
+        [moments + windows done, rank-one fill done, diagonal cells done,
+        near elements done, end]."""
         ms = np.zeros(5)
         check(capi.lib().bs2e_rk_stage_times(self.handle, ptr(ms), 5), self.ctx.handle)
         return ms

@@ -66,13 +66,14 @@ def build_fortran(verbose: bool = False) -> str | None:
     srcs = [os.path.join(FSRC, f) for f in ("bs2e_capi.f90", "bs2e_mat_els.f90", "bs2e_basis_setup.f90")]
     if not all(os.path.exists(s) for s in srcs) or shutil.which(FC) is None:
         return None
-    mod = os.path.join(LIB, "fmod")
+    mod = os.path.join(OBJ, "fmod")
     os.makedirs(mod, exist_ok=True)
     flags = ["-fdefault-integer-8", "-O2", "-fPIC", "-module-dir", mod, "-I", mod]
+    rpath = "-Wl,-rpath,$ORIGIN"          # the .so and the driver sit beside libbs2e.so
     so = os.path.join(LIB, "libbs2e_fortran.so")
-    _run([FC, *flags, "-shared", srcs[0], srcs[1], "-o", so, f"-L{LIB}", "-lbs2e", f"-Wl,-rpath,{LIB}"])
+    _run([FC, *flags, "-shared", srcs[0], srcs[1], "-o", so, f"-L{LIB}", "-lbs2e", rpath])
     exe = os.path.join(LIB, "bs2e_basis_setup")
-    _run([FC, *flags, srcs[2], "-o", exe, f"-L{LIB}", "-lbs2e_fortran", "-lbs2e", f"-Wl,-rpath,{LIB}"])
+    _run([FC, *flags, srcs[2], "-o", exe, f"-L{LIB}", "-lbs2e_fortran", "-lbs2e", rpath])
     return so
 
 

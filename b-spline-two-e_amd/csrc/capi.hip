@@ -134,12 +134,12 @@ static void free_tables(bs2e_rk* rk)
     dev_free(d.rows);
     dev_free(d.c_first);
     dev_free(rk->R);
-    for (auto& e : rk->ev)
-        if (e) {
-            (void)hipEventDestroy(e);
-            e = nullptr;
-        }
-    for (hipEvent_t* e : {&rk->fork, &rk->windows, &rk->join})
+    for (auto& set : rk->evpool)
+        for (hipEvent_t e : set)
+            if (e) (void)hipEventDestroy(e);
+    rk->evpool.clear();
+    rk->evused = 0;
+    for (hipEvent_t* e : {&rk->windows, &rk->join})
         if (*e) {
             (void)hipEventDestroy(*e);
             *e = nullptr;
@@ -148,8 +148,7 @@ static void free_tables(bs2e_rk* rk)
 
 static int64_t create_events(bs2e_ctx* ctx, bs2e_rk* rk)
 {
-    for (auto& e : rk->ev) HIP_TRY(ctx, hipEventCreate(&e));
-    for (hipEvent_t* e : {&rk->fork, &rk->windows, &rk->join})
+    for (hipEvent_t* e : {&rk->windows, &rk->join})
         HIP_TRY(ctx, hipEventCreateWithFlags(e, hipEventDisableTiming));
     return BS2E_OK;
 }
@@ -223,36 +222,51 @@ static int64_t run_primitives_serial(bs2e_ctx* ctx, bs2e_rk* rk)
 }
 
 // The full build.  Main stream: local moments -> pair windows -> rank-one
-// fill.  Side stream (forked at the start): diagonal cells -> (after the
-// pair windows) near ranges.  The main stream waits for the side stream at
-// the end, so the build is one unit of work on the caller's stream.
+// fill.  Side stream (forked once the pair windows exist, so that those two
+// short critical-path kernels run alone): diagonal cells -> near elements,
+// concurrently with the rank-one fill.  The main stream waits for the side
+// stream at the end, so the build is one unit of work on the caller's stream.
 // Profiling events, all relative to ev[0]: ev[1] windows done, ev[2] far
-// fill done, ev[3] diagonal cells done, ev[4] near ranges done, ev[5] end.
+// fill done, ev[3] diagonal cells done, ev[4] near elements done, ev[5] end.
+static constexpr size_t kMaxProfiledBuilds = 1 << 16;
+
 static int64_t run_build(bs2e_ctx* ctx, bs2e_rk* rk)
 {
     hipStream_t m = ctx->stream, s = ctx->side;
-    const bool prof = rk->profiling;
-    if (prof) HIP_TRY(ctx, hipEventRecord(rk->ev[0], m));
-    // The short critical-path kernel is enqueued before the fork so that its
-    // workgroups are dispatched ahead of the diagonal-cell workgroups.
-    if (!rk->from_primitives) HIP_TRY(ctx, launch_outer_moments(rk->plan, rk->d, m));
-    HIP_TRY(ctx, hipEventRecord(rk->fork, m));
-    HIP_TRY(ctx, hipStreamWaitEvent(s, rk->fork, 0));
+    const bool prof = rk->profiling && rk->evused < kMaxProfiledBuilds;
+    hipEvent_t* ev = nullptr;
+    if (prof) {
+        if (rk->evused == rk->evpool.size()) {
+            std::array<hipEvent_t, 6> set{};
+            for (auto& e : set) {
+                hipError_t err = hipEventCreate(&e);
+                if (err != hipSuccess) {
+                    for (hipEvent_t x : set)
+                        if (x) (void)hipEventDestroy(x);
+                    return fail(ctx, BS2E_EDEVICE, std::string("hipEventCreate: ") + hipGetErrorString(err));
+                }
+            }
+            rk->evpool.push_back(set);
+        }
+        ev = rk->evpool[rk->evused++].data();
+    }
+    if (prof) HIP_TRY(ctx, hipEventRecord(ev[0], m));
     if (!rk->from_primitives) {
-        HIP_TRY(ctx, launch_diag_cells(rk->plan, rk->d, s, true));
+        HIP_TRY(ctx, launch_outer_moments(rk->plan, rk->d, m));
         HIP_TRY(ctx, launch_pair_windows(rk->plan, rk->d, m));
     }
-    if (prof) HIP_TRY(ctx, hipEventRecord(rk->ev[3], s));
-    if (prof) HIP_TRY(ctx, hipEventRecord(rk->ev[1], m));
+    if (prof) HIP_TRY(ctx, hipEventRecord(ev[1], m));
     HIP_TRY(ctx, hipEventRecord(rk->windows, m));
     HIP_TRY(ctx, hipStreamWaitEvent(s, rk->windows, 0));
+    if (!rk->from_primitives) HIP_TRY(ctx, launch_diag_cells(rk->plan, rk->d, s, true));
+    if (prof) HIP_TRY(ctx, hipEventRecord(ev[3], s));
     HIP_TRY(ctx, launch_tensor_near(rk->plan, rk->d, rk->R, s));
-    if (prof) HIP_TRY(ctx, hipEventRecord(rk->ev[4], s));
+    if (prof) HIP_TRY(ctx, hipEventRecord(ev[4], s));
     HIP_TRY(ctx, hipEventRecord(rk->join, s));
     HIP_TRY(ctx, launch_tensor_far(rk->plan, rk->d, rk->R, m));
-    if (prof) HIP_TRY(ctx, hipEventRecord(rk->ev[2], m));
+    if (prof) HIP_TRY(ctx, hipEventRecord(ev[2], m));
     HIP_TRY(ctx, hipStreamWaitEvent(m, rk->join, 0));
-    if (prof) HIP_TRY(ctx, hipEventRecord(rk->ev[5], m));
+    if (prof) HIP_TRY(ctx, hipEventRecord(ev[5], m));
     return BS2E_OK;
 }
 
@@ -554,19 +568,25 @@ int64_t bs2e_rk_set_profiling(bs2e_rk* rk, int64_t on)
 {
     if (!rk) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_set_profiling: rk is NULL");
     rk->profiling = on != 0;
+    rk->evused = 0;
     return BS2E_OK;
 }
 
 int64_t bs2e_rk_stage_times(bs2e_rk* rk, double* ms, int64_t n)
 {
     if (!rk || !ms) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_stage_times: NULL argument");
-    if (!rk->profiling || !rk->computed) return fail(rk->ctx, BS2E_ESTATE, "bs2e_rk_stage_times: profiling was off");
-    HIP_TRY(rk->ctx, hipEventSynchronize(rk->ev[5]));
-    for (int64_t i = 0; i < n && i < 5; ++i) {
-        float t = 0.f;
-        HIP_TRY(rk->ctx, hipEventElapsedTime(&t, rk->ev[0], rk->ev[i + 1]));
-        ms[i] = t;
-    }
+    if (rk->evused == 0) return fail(rk->ctx, BS2E_ESTATE, "bs2e_rk_stage_times: no build was profiled");
+    const size_t nb = rk->evused;
+    HIP_TRY(rk->ctx, hipEventSynchronize(rk->evpool[nb - 1][5]));
+    double sum[5] = {0, 0, 0, 0, 0};
+    for (size_t b = 0; b < nb; ++b)
+        for (int i = 0; i < 5; ++i) {
+            float t = 0.f;
+            HIP_TRY(rk->ctx, hipEventElapsedTime(&t, rk->evpool[b][0], rk->evpool[b][i + 1]));
+            sum[i] += t;
+        }
+    for (int64_t i = 0; i < n && i < 5; ++i) ms[i] = sum[i] / (double)nb;
+    rk->evused = 0;
     return BS2E_OK;
 }
 

@@ -1,6 +1,7 @@
 // capi_internal.h -- handle types and helpers shared by the C ABI sources.
 #pragma once
 
+#include <array>
 #include <string>
 #include <vector>
 
@@ -25,8 +26,11 @@ struct bs2e_rk {
     std::vector<double> mirror;
     bool mirror_valid = false;
     bool profiling = false;
-    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   // profiling
-    hipEvent_t fork = nullptr, windows = nullptr, join = nullptr;                  // stream ordering
+    // profiling: one set of 6 timing events per build since profiling was
+    // switched on (the pool is reused; see run_build for their meaning)
+    std::vector<std::array<hipEvent_t, 6>> evpool;
+    size_t evused = 0;
+    hipEvent_t windows = nullptr, join = nullptr;                                  // stream ordering
     bool computed = false;
     bool from_primitives = false;   // tables gathered from caller containers: compute runs the fill only
 };

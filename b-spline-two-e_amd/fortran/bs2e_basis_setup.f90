@@ -1,0 +1,130 @@
+! bs2e_basis_setup.f90 -- the two-electron branch of the reference's
+! basis_setup (main_basis_setup.f90:47-118) written against the drop-in
+! modules: knots, B-splines, primitive containers, the R^k tensor (from the
+! containers and, a second time, straight from the knots) and the H/S blocks
+! of one symmetry.
+!
+! usage: bs2e_basis_setup k m Z h_max r_max k_GL max_k max_l L pi full key_stride hs_file out_file
+!
+! hs_file (stream): S then H(l) for l = 0..max_l, each n_b x n_b complex(dp)
+! column-major (the one-electron matrices, mat_els.f90:85-118 and :294-321).
+! out_file (stream, 8-byte integers and doubles) receives every result:
+!   n_knots, knots
+!   nnz4, K, r_k%data, r_m_k%data, r_k%iv, r_k%i, r_k%j
+!   nnz6, r_d_k%data, r_d_k%iv, %i, %i_p, %j, %j_p
+!   n_keys, keys(4, n_keys), values(K, n_keys) from compute_R_k_map,
+!   values(K, n_keys) from compute_R_k_direct
+!   n_config, configs(5, n_config)            (n1, n2, l1, l2, eqv)
+!   H_sp: nnz, index_ptr, indices, data;  S_sp: the same
+program bs2e_basis_setup
+    use bs2e_mat_els
+    implicit none
+    integer, parameter :: dp = kind(1.d0)
+    integer :: k, m, Z, k_GL, max_k, max_l, L, ipi, ifull, stride
+    real(dp) :: h_max, r_max
+    character(len=4096) :: hs_file, out_file, arg
+    real(dp), allocatable :: grid(:), vals(:,:), vals_direct(:,:)
+    type(b_spline) :: splines
+    type(sparse_4d) :: r_k, r_m_k
+    type(sparse_6d) :: r_d_k
+    class(Nd_DOK), allocatable :: R_p
+    type(Nd_DOK) :: R_direct
+    type(block), allocatable :: H_vec(:)
+    complex(dp), allocatable :: S(:,:)
+    type(sym) :: term
+    type(CSR_matrix) :: H_sp, S_sp
+    integer, allocatable :: keys(:,:)
+    integer :: u, i, a, b, c, d, n_keys, p, q, n_pairs
+
+    call get_command_argument(1, arg); read(arg, *) k
+    call get_command_argument(2, arg); read(arg, *) m
+    call get_command_argument(3, arg); read(arg, *) Z
+    call get_command_argument(4, arg); read(arg, *) h_max
+    call get_command_argument(5, arg); read(arg, *) r_max
+    call get_command_argument(6, arg); read(arg, *) k_GL
+    call get_command_argument(7, arg); read(arg, *) max_k
+    call get_command_argument(8, arg); read(arg, *) max_l
+    call get_command_argument(9, arg); read(arg, *) L
+    call get_command_argument(10, arg); read(arg, *) ipi
+    call get_command_argument(11, arg); read(arg, *) ifull
+    call get_command_argument(12, arg); read(arg, *) stride
+    call get_command_argument(13, hs_file)
+    call get_command_argument(14, out_file)
+
+    call generate_grid(k, m, Z, h_max, r_max, grid)
+    call splines%init(k, grid)
+    write(6, *) "n_b: ", splines%n_b
+
+    call r_k%init(max_k, splines)
+    call r_m_k%init(max_k, splines)
+    call r_d_k%init(max_k, splines)
+    call setup_Slater_integrals(splines, max_k, k_GL, r_k, r_m_k, r_d_k)
+    allocate(Nd_DOK :: R_p)
+    call compute_R_k_map(r_d_k, r_k, r_m_k, splines, max_k, R_p)
+    call compute_R_k_direct(splines, max_k, k_GL, R_direct)
+
+    ! a strided sample of the banded keys [a,b,c,d] (|a-c| < k, |b-d| < k)
+    n_pairs = 0
+    do c = 1, splines%n_b
+        do a = max(1, c - k + 1), min(splines%n_b, c + k - 1)
+            n_pairs = n_pairs + 1
+        end do
+    end do
+    n_keys = ((n_pairs + stride - 1) / stride) ** 2
+    allocate(keys(4, n_keys), vals(0:max_k, n_keys), vals_direct(0:max_k, n_keys))
+    n_keys = 0
+    p = 0
+    do c = 1, splines%n_b
+        do a = max(1, c - k + 1), min(splines%n_b, c + k - 1)
+            p = p + 1
+            if (mod(p - 1, stride) /= 0) cycle
+            q = 0
+            do d = 1, splines%n_b
+                do b = max(1, d - k + 1), min(splines%n_b, d + k - 1)
+                    q = q + 1
+                    if (mod(q - 1, stride) /= 0) cycle
+                    n_keys = n_keys + 1
+                    keys(:, n_keys) = [a, b, c, d]
+                end do
+            end do
+        end do
+    end do
+    do i = 1, n_keys
+        call R_p%get_val(keys(:, i), vals(:, i))
+        call R_direct%get_val(keys(:, i), vals_direct(:, i))
+    end do
+
+    allocate(S(splines%n_b, splines%n_b), H_vec(0:max_l))
+    open(newunit=u, file=trim(hs_file), access='stream', form='unformatted', status='old', action='read')
+    read(u) S
+    do i = 0, max_l
+        call H_vec(i)%init(splines%n_b, splines%n_b)
+        read(u) H_vec(i)%data
+    end do
+    close(u)
+
+    term%l = L
+    term%m = 0
+    term%pi = ipi /= 0
+    term%configs = count_configs(term, max_l, splines%n_b, splines%k, splines%n_b, splines%n_b, max_l)
+    term%n_config = size(term%configs)
+    call construct_block_tensor(H_vec, S, splines, term, max_k, R_p, H_sp, S_sp, ifull /= 0)
+    write(6, '(a,i0,a,i0,a,i0)') "block L=", L, ": n_config ", term%n_config, ", nnz(H) ", H_sp%nnz
+
+    open(newunit=u, file=trim(out_file), access='stream', form='unformatted', status='replace', action='write')
+    write(u) size(grid), grid
+    write(u) r_k%nnz, max_k + 1, r_k%data, r_m_k%data, r_k%iv, r_k%i, r_k%j
+    write(u) r_d_k%nnz, r_d_k%data, r_d_k%iv, r_d_k%i, r_d_k%i_p, r_d_k%j, r_d_k%j_p
+    write(u) n_keys, keys(:, 1:n_keys), vals(:, 1:n_keys), vals_direct(:, 1:n_keys)
+    write(u) term%n_config
+    do i = 1, term%n_config
+        write(u) term%configs(i)%n, term%configs(i)%l, merge(1, 0, term%configs(i)%eqv)
+    end do
+    write(u) H_sp%nnz, H_sp%index_ptr, H_sp%indices, H_sp%data
+    write(u) S_sp%nnz, S_sp%index_ptr, S_sp%indices, S_sp%data
+    close(u)
+
+    call R_p%release()
+    call R_direct%release()
+    call bs2e_shutdown()
+end program bs2e_basis_setup

@@ -1,0 +1,417 @@
+! bs2e_mat_els.f90 -- Fortran drop-in for the two-electron radial path of
+! edvinolo/b-spline-two-e, running on the MI355X through the C ABI.
+!
+! The public names and argument lists are the reference's, so that a caller
+! written against them (main_basis_setup.f90:75-118) compiles unchanged:
+!
+!   generate_grid(k, m, Z, h_max, r_max, grid)           grid_tools.f90:6-55
+!   b_spline%init(k, t), %find_max_n_b(x)                bspline_tools.f90:27-39, 364-373
+!   sparse_4d / sparse_6d %init(max_k, b_splines)        sparse_array_tools.f90:9-31, 159-178
+!   setup_Slater_integrals(b_splines, max_k, k_GL,
+!                          r_k, r_m_k, r_d_k)            mat_els.f90:172-182
+!   Nd_DOK (%init, %set_val, %get_val)                   sparse_array_tools.f90:49-61, 495-555
+!   compute_R_k_map(r_d_k, r_k, r_m_k, b_splines,
+!                   max_k, R)                            sparse_array_tools.f90:452-493
+!   config, sym, count_configs(term, max_l_1p, n_b,
+!       k_spline, max_n_b, n_all_l, l_2_max, eigs)       orbital_tools.f90:10-19, 119-216
+!   block, CSR_matrix, construct_block_tensor(H, S,
+!       b_splines, term, max_k, R_k, H_sp, S_sp, full)   hamiltonian.f90:106-283
+!
+! plus compute_R_k_direct(b_splines, max_k, k_GL, R), which builds the tensor
+! on the device straight from the knots without the host containers.
+!
+! Errors: a negative status from the library becomes `error stop` with the
+! library's message, as the reference stops on its own errors.  The device
+! context is created on first use (device 0, or bs2e_use_device).
+module bs2e_mat_els
+    use, intrinsic :: iso_c_binding
+    use bs2e_capi
+    implicit none
+    private
+
+    integer, parameter :: dp = kind(1.d0)
+
+    public :: b_spline, sparse_4d, sparse_6d, Nd_DOK, config, sym, block, CSR_matrix
+    public :: generate_grid, setup_Slater_integrals, compute_R_k_map, compute_R_k_direct
+    public :: count_configs, construct_block_tensor, ang_k_LS, bs2e_use_device, bs2e_shutdown
+
+    type(c_ptr), save :: g_ctx = c_null_ptr
+    integer, save :: g_device = 0
+
+    type :: b_spline
+        integer :: k = 0
+        integer :: n = 0
+        integer :: n_b = 0
+        real(dp), allocatable :: knots(:)
+        real(dp), allocatable :: breakpoints(:)
+    contains
+        procedure :: init => b_spline_init
+        procedure :: find_max_n_b => b_spline_find_max_n_b
+    end type b_spline
+
+    type :: sparse_4d
+        integer :: nnz = 0
+        real(dp), allocatable :: data(:,:)
+        integer, allocatable :: iv(:), i(:), j(:)
+    contains
+        procedure :: init => init_4d
+    end type sparse_4d
+
+    type :: sparse_6d
+        integer :: nnz = 0
+        real(dp), allocatable :: data(:,:)
+        integer, allocatable :: iv(:), i(:), j(:), i_p(:), j_p(:)
+    contains
+        procedure :: init => init_6d
+    end type sparse_6d
+
+    ! The banded R^k tensor, device-resident with a host mirror; the key
+    ! [a,b,c,d] is located by arithmetic instead of a hash map.
+    type :: Nd_DOK
+        integer :: N = 4
+        integer :: N_val = 0
+        integer :: N_elements = 0
+        type(c_ptr) :: handle = c_null_ptr
+    contains
+        procedure :: init => init_Nd_DOK
+        procedure :: set_val => set_val_Nd_DOK
+        procedure :: get_val => get_val_Nd_DOK
+        procedure :: release => release_Nd_DOK
+    end type Nd_DOK
+
+    type :: config
+        integer :: n(2) = 0
+        integer :: l(2) = 0
+        logical :: eqv = .false.
+    end type config
+
+    type :: sym
+        integer :: l = 0
+        integer :: m = 0
+        logical :: pi = .false.
+        integer :: n_config = 0
+        type(config), allocatable :: configs(:)
+    end type sym
+
+    type :: block
+        complex(dp), allocatable :: data(:,:)
+    contains
+        procedure :: init => block_init
+    end type block
+
+    type :: CSR_matrix
+        integer :: shape(2) = 0
+        integer :: nnz = 0
+        integer, allocatable :: index_ptr(:)
+        integer, allocatable :: indices(:)
+        complex(dp), allocatable :: data(:)
+    end type CSR_matrix
+
+contains
+
+    ! ---------------------------------------------------------------- context
+    subroutine bs2e_use_device(device)
+        integer, intent(in) :: device
+        if (c_associated(g_ctx)) call bs2e_shutdown()
+        g_device = device
+    end subroutine bs2e_use_device
+
+    subroutine bs2e_shutdown()
+        integer(c_int64_t) :: rc
+        if (c_associated(g_ctx)) rc = bs2e_destroy(g_ctx)
+        g_ctx = c_null_ptr
+    end subroutine bs2e_shutdown
+
+    function ctx() result(c)
+        type(c_ptr) :: c
+        if (.not. c_associated(g_ctx)) call check(bs2e_create(g_ctx, int(g_device, c_int64_t)), "bs2e_create")
+        c = g_ctx
+    end function ctx
+
+    subroutine check(rc, where)
+        integer(c_int64_t), intent(in) :: rc
+        character(len=*), intent(in) :: where
+        if (rc < 0) then
+            write(0, '(a,a,i0,a,a)') where, ": bs2e status ", rc, ": ", c_string(bs2e_last_error(c_null_ptr))
+            error stop "bs2e: device library call failed"
+        end if
+    end subroutine check
+
+    ! ---------------------------------------------------------------- basis
+    subroutine generate_grid(k, m, Z, h_max, r_max, grid)
+        integer, intent(in) :: k, m, Z
+        real(dp), intent(in) :: h_max, r_max
+        real(dp), allocatable, target, intent(out) :: grid(:)
+        integer(c_int64_t) :: n
+        n = bs2e_generate_grid(int(k, c_int64_t), int(m, c_int64_t), int(Z, c_int64_t), h_max, r_max, c_null_ptr, &
+                               0_c_int64_t)
+        call check(n, "generate_grid")
+        allocate(grid(n))
+        call check(bs2e_generate_grid(int(k, c_int64_t), int(m, c_int64_t), int(Z, c_int64_t), h_max, r_max, &
+                                      c_loc(grid), n), "generate_grid")
+    end subroutine generate_grid
+
+    subroutine b_spline_init(this, k, t)
+        class(b_spline), intent(inout) :: this
+        integer, intent(in) :: k
+        real(dp), intent(in) :: t(:)
+        this%k = k
+        this%n = size(t) - k
+        this%n_b = this%n - 2
+        this%knots = t
+        this%breakpoints = t(k:size(t) - k + 1)
+    end subroutine b_spline_init
+
+    ! index of the last interval whose left breakpoint is below x (bspline_tools.f90:364-373)
+    function b_spline_find_max_n_b(this, x) result(res)
+        class(b_spline), intent(in) :: this
+        real(dp), intent(in) :: x
+        integer :: res, i
+        res = -1
+        do i = 1, size(this%breakpoints)
+            if (this%breakpoints(i) >= x) then
+                res = i - 1
+                return
+            end if
+        end do
+    end function b_spline_find_max_n_b
+
+    subroutine block_init(this, n, m)
+        class(block), intent(inout) :: this
+        integer, intent(in) :: n, m
+        if (allocated(this%data)) deallocate(this%data)
+        allocate(this%data(n, m))
+        this%data = (0.0_dp, 0.0_dp)
+    end subroutine block_init
+
+    ! ---------------------------------------------------------------- containers
+    subroutine sizes_of(b_splines, nnz4, nnz6, n_pairs)
+        type(b_spline), intent(in) :: b_splines
+        integer(c_int64_t), intent(out) :: nnz4, nnz6, n_pairs
+        call check(bs2e_count_nnz(int(b_splines%k, c_int64_t), int(size(b_splines%knots), c_int64_t), b_splines%knots, &
+                                  nnz4, nnz6, n_pairs), "count_nnz")
+    end subroutine sizes_of
+
+    subroutine init_4d(this, max_k, b_splines)
+        class(sparse_4d), intent(inout) :: this
+        integer, intent(in) :: max_k
+        type(b_spline), intent(in) :: b_splines
+        integer(c_int64_t) :: nnz4, nnz6, n_pairs
+        call sizes_of(b_splines, nnz4, nnz6, n_pairs)
+        this%nnz = int(nnz4)
+        if (allocated(this%data)) deallocate(this%data, this%iv, this%i, this%j)
+        allocate(this%data(this%nnz, 0:max_k), this%iv(this%nnz), this%i(this%nnz), this%j(this%nnz))
+        this%data = 0.0_dp
+        this%iv = 0
+        this%i = 0
+        this%j = 0
+    end subroutine init_4d
+
+    subroutine init_6d(this, max_k, b_splines)
+        class(sparse_6d), intent(inout) :: this
+        integer, intent(in) :: max_k
+        type(b_spline), intent(in) :: b_splines
+        integer(c_int64_t) :: nnz4, nnz6, n_pairs
+        call sizes_of(b_splines, nnz4, nnz6, n_pairs)
+        this%nnz = int(nnz6)
+        if (allocated(this%data)) deallocate(this%data, this%iv, this%i, this%j, this%i_p, this%j_p)
+        allocate(this%data(this%nnz, 0:max_k), this%iv(this%nnz), this%i(this%nnz), this%j(this%nnz), &
+                 this%i_p(this%nnz), this%j_p(this%nnz))
+        this%data = 0.0_dp
+        this%iv = 0
+        this%i = 0
+        this%j = 0
+        this%i_p = 0
+        this%j_p = 0
+    end subroutine init_6d
+
+    ! mat_els.f90:172-182: fills the primitive containers (initialised by the
+    ! caller) on the device, in the reference's order.
+    subroutine setup_Slater_integrals(b_splines, max_k, k_GL, r_k, r_m_k, r_d_k)
+        type(b_spline), intent(in) :: b_splines
+        integer, intent(in) :: max_k, k_GL
+        type(sparse_4d), intent(inout) :: r_k, r_m_k
+        type(sparse_6d), intent(inout) :: r_d_k
+        integer(c_int64_t), allocatable :: iv4(:), i4(:), j4(:), iv6(:), i6(:), ip6(:), j6(:), jp6(:)
+        allocate(iv4(r_k%nnz), i4(r_k%nnz), j4(r_k%nnz))
+        allocate(iv6(r_d_k%nnz), i6(r_d_k%nnz), ip6(r_d_k%nnz), j6(r_d_k%nnz), jp6(r_d_k%nnz))
+        call check(bs2e_slater_primitives(ctx(), int(b_splines%k, c_int64_t), int(size(b_splines%knots), c_int64_t), &
+                   b_splines%knots, int(k_GL, c_int64_t), int(max_k, c_int64_t), int(r_k%nnz, c_int64_t), &
+                   r_k%data, r_m_k%data, iv4, i4, j4, int(r_d_k%nnz, c_int64_t), r_d_k%data, &
+                   iv6, i6, ip6, j6, jp6), "setup_Slater_integrals")
+        r_k%iv = int(iv4)
+        r_k%i = int(i4)
+        r_k%j = int(j4)
+        r_m_k%iv = int(iv4)
+        r_m_k%i = int(i4)
+        r_m_k%j = int(j4)
+        r_d_k%iv = int(iv6)
+        r_d_k%i = int(i6)
+        r_d_k%i_p = int(ip6)
+        r_d_k%j = int(j6)
+        r_d_k%j_p = int(jp6)
+    end subroutine setup_Slater_integrals
+
+    ! ---------------------------------------------------------------- Nd_DOK
+    subroutine init_Nd_DOK(this, N, N_val, nnz)
+        class(Nd_DOK), intent(inout) :: this
+        integer, intent(in) :: N, N_val, nnz
+        call this%release()
+        this%N = N
+        this%N_val = N_val
+        this%N_elements = nnz
+    end subroutine init_Nd_DOK
+
+    subroutine set_val_Nd_DOK(this, indices, val)
+        class(Nd_DOK), intent(inout) :: this
+        integer, intent(in) :: indices(:)
+        real(dp), intent(in) :: val(:)
+        error stop "Nd_DOK%set_val: the tensor is built on the device by compute_R_k_map; it is read-only"
+    end subroutine set_val_Nd_DOK
+
+    ! sparse_array_tools.f90:536-555; reentrant (reads the host mirror).
+    subroutine get_val_Nd_DOK(this, indices, values)
+        class(Nd_DOK), intent(inout) :: this
+        integer, intent(in) :: indices(:)
+        real(dp), intent(out) :: values(:)
+        integer(c_int64_t) :: idx(4)
+        if (.not. c_associated(this%handle)) error stop "Nd_DOK%get_val: no tensor (call compute_R_k_map first)"
+        idx = int(indices(1:4), c_int64_t)
+        call check(bs2e_rk_get_val(this%handle, idx, values, int(size(values), c_int64_t)), "Nd_DOK%get_val")
+    end subroutine get_val_Nd_DOK
+
+    subroutine release_Nd_DOK(this)
+        class(Nd_DOK), intent(inout) :: this
+        integer(c_int64_t) :: rc
+        if (c_associated(this%handle)) rc = bs2e_rk_release(this%handle)
+        this%handle = c_null_ptr
+    end subroutine release_Nd_DOK
+
+    subroutine finish_tensor(R, max_k)
+        class(Nd_DOK), intent(inout) :: R
+        integer, intent(in) :: max_k
+        integer(c_int64_t) :: info(2)
+        call check(bs2e_rk_compute(R%handle), "compute_R_k_map")
+        call check(bs2e_rk_copy_to_host(R%handle, c_null_ptr, 0_c_int64_t), "compute_R_k_map")
+        call check(bs2e_rk_info(R%handle, info, 2_c_int64_t), "compute_R_k_map")
+        R%N = 4
+        R%N_val = max_k + 1
+        R%N_elements = int(info(2) * info(2))
+    end subroutine finish_tensor
+
+    ! sparse_array_tools.f90:452-493: the tensor from the caller's containers.
+    subroutine compute_R_k_map(r_d_k, r_k, r_m_k, b_splines, max_k, R)
+        type(sparse_6d), intent(in) :: r_d_k
+        type(sparse_4d), intent(in) :: r_k, r_m_k
+        type(b_spline), intent(in) :: b_splines
+        integer, intent(in) :: max_k
+        class(Nd_DOK), intent(inout) :: R
+        call R%release()
+        call check(bs2e_rk_from_primitives(ctx(), int(b_splines%k, c_int64_t), int(size(b_splines%knots), c_int64_t), &
+                   b_splines%knots, int(max_k, c_int64_t), int(r_k%nnz, c_int64_t), r_k%data, r_m_k%data, &
+                   int(r_k%iv, c_int64_t), int(r_k%i, c_int64_t), int(r_k%j, c_int64_t), int(r_d_k%nnz, c_int64_t), &
+                   r_d_k%data, int(r_d_k%iv, c_int64_t), int(r_d_k%i, c_int64_t), int(r_d_k%i_p, c_int64_t), &
+                   int(r_d_k%j, c_int64_t), int(r_d_k%j_p, c_int64_t), R%handle), "compute_R_k_map")
+        call finish_tensor(R, max_k)
+    end subroutine compute_R_k_map
+
+    ! The whole build on the device from the knots (no host containers).
+    subroutine compute_R_k_direct(b_splines, max_k, k_GL, R)
+        type(b_spline), intent(in) :: b_splines
+        integer, intent(in) :: max_k, k_GL
+        class(Nd_DOK), intent(inout) :: R
+        call R%release()
+        call check(bs2e_rk_create(ctx(), int(b_splines%k, c_int64_t), int(size(b_splines%knots), c_int64_t), &
+                   b_splines%knots, int(k_GL, c_int64_t), int(max_k, c_int64_t), BS2E_PREC_FP64, R%handle), &
+                   "compute_R_k_direct")
+        call finish_tensor(R, max_k)
+    end subroutine compute_R_k_direct
+
+    ! ---------------------------------------------------------------- blocks
+    function ang_k_LS(k, confs, L) result(res)
+        integer, intent(in) :: k, L
+        type(config), intent(in) :: confs(2)
+        real(dp) :: res
+        res = bs2e_ang_k_LS(int(k, c_int64_t), int(confs(1)%l(1), c_int64_t), int(confs(1)%l(2), c_int64_t), &
+                            int(confs(2)%l(1), c_int64_t), int(confs(2)%l(2), c_int64_t), int(L, c_int64_t))
+    end function ang_k_LS
+
+    ! orbital_tools.f90:119-216.  eigs is accepted for the signature; the
+    ! reference only forms an energy from it that no filter uses.
+    function count_configs(term, max_l_1p, n_b, k_spline, max_n_b, n_all_l, l_2_max, eigs) result(res)
+        type(sym), intent(in) :: term
+        integer, intent(in) :: max_l_1p, n_b, k_spline, max_n_b, n_all_l, l_2_max
+        complex(dp), intent(in), optional :: eigs(:,:)
+        type(config), allocatable :: res(:)
+        integer(c_int64_t), allocatable, target :: buf(:)
+        integer(c_int64_t) :: n, pi, args(8)
+        integer :: c
+        pi = merge(1_c_int64_t, 0_c_int64_t, term%pi)
+        args = [int(term%l, c_int64_t), pi, int(max_l_1p, c_int64_t), int(n_b, c_int64_t), &
+                int(k_spline, c_int64_t), int(max_n_b, c_int64_t), int(n_all_l, c_int64_t), int(l_2_max, c_int64_t)]
+        n = bs2e_count_configs(args(1), args(2), args(3), args(4), args(5), args(6), args(7), args(8), c_null_ptr, &
+                               0_c_int64_t)
+        call check(n, "count_configs")
+        allocate(buf(5 * max(n, 1_c_int64_t)), res(n))
+        call check(bs2e_count_configs(args(1), args(2), args(3), args(4), args(5), args(6), args(7), args(8), &
+                                      c_loc(buf), n), "count_configs")
+        do c = 1, int(n)
+            res(c)%n = int(buf(5 * (c - 1) + 1:5 * (c - 1) + 2))
+            res(c)%l = int(buf(5 * (c - 1) + 3:5 * (c - 1) + 4))
+            res(c)%eqv = buf(5 * c) /= 0
+        end do
+    end function count_configs
+
+    ! hamiltonian.f90:106-283: CSR H and S of one symmetry block, with the
+    ! R^k-dependent values summed over angular channels on the device.
+    ! H(l)%data, l = 0..max_l_1p, are the one-electron blocks.
+    subroutine construct_block_tensor(H, S, b_splines, term, max_k, R_k, H_sp, S_sp, full)
+        type(block), allocatable, intent(in) :: H(:)
+        complex(dp), intent(in) :: S(:,:)
+        type(b_spline), intent(in) :: b_splines
+        type(sym), intent(in) :: term
+        integer, intent(in) :: max_k
+        class(Nd_DOK), intent(inout) :: R_k
+        type(CSR_matrix), intent(out) :: H_sp, S_sp
+        logical, intent(in) :: full
+        integer(c_int64_t), allocatable :: cf(:), ipH(:), ixH(:), ipS(:), ixS(:)
+        complex(dp), allocatable :: Hl(:,:,:), dH(:), dS(:)
+        integer(c_int64_t) :: sizes(5)
+        type(c_ptr) :: blk
+        integer :: c, l, n, nb
+        if (.not. c_associated(R_k%handle)) error stop "construct_block_tensor: R_k holds no tensor"
+        if (R_k%N_val /= max_k + 1) error stop "construct_block_tensor: max_k differs from the tensor's"
+        if (lbound(H, 1) /= 0) error stop "construct_block_tensor: H must be indexed from l = 0"
+        n = term%n_config
+        nb = b_splines%n_b
+        allocate(cf(5 * max(n, 1)))
+        do c = 1, n
+            cf(5 * (c - 1) + 1:5 * (c - 1) + 2) = int(term%configs(c)%n, c_int64_t)
+            cf(5 * (c - 1) + 3:5 * (c - 1) + 4) = int(term%configs(c)%l, c_int64_t)
+            cf(5 * c) = merge(1_c_int64_t, 0_c_int64_t, term%configs(c)%eqv)
+        end do
+        allocate(Hl(nb, nb, 0:ubound(H, 1)))
+        do l = 0, ubound(H, 1)
+            Hl(:, :, l) = H(l)%data
+        end do
+        call check(bs2e_block_create(R_k%handle, int(term%l, c_int64_t), int(n, c_int64_t), cf, &
+                                     merge(1_c_int64_t, 0_c_int64_t, full), blk), "construct_block_tensor")
+        call check(bs2e_block_sizes(blk, sizes, 5_c_int64_t), "construct_block_tensor")
+        allocate(ipH(n + 1), ipS(n + 1), ixH(max(sizes(1), 1_c_int64_t)), ixS(max(sizes(2), 1_c_int64_t)))
+        allocate(dH(max(sizes(1), 1_c_int64_t)), dS(max(sizes(2), 1_c_int64_t)))
+        call check(bs2e_block_fill(blk, int(ubound(H, 1), c_int64_t), Hl, S, ipH, ixH, dH, ipS, ixS, dS), &
+                   "construct_block_tensor")
+        call check(bs2e_block_release(blk), "construct_block_tensor")
+        H_sp%shape = [n, n]
+        S_sp%shape = [n, n]
+        H_sp%nnz = int(sizes(1))
+        S_sp%nnz = int(sizes(2))
+        H_sp%index_ptr = int(ipH)
+        S_sp%index_ptr = int(ipS)
+        H_sp%indices = int(ixH(1:sizes(1)))
+        S_sp%indices = int(ixS(1:sizes(2)))
+        H_sp%data = dH(1:sizes(1))
+        S_sp%data = dS(1:sizes(2))
+    end subroutine construct_block_tensor
+
+end module bs2e_mat_els

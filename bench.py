@@ -30,6 +30,10 @@ for _p in (ROOT, os.path.join(ROOT, "b-spline-two-e_amd")):
 
 METRIC = "R^k integrals/sec (mixed vs FP64 GPU) at <=1e-10 rel err; p99 build latency"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k3_far launch, FETCH_SIZE + WRITE_SIZE from rocprofv3 --pmc,
+# each scaled by what the counter reads for a 1 GiB device copy
+# (profiles/rk_build_c2.md); None for workloads not measured that way.
+FAR_TRAFFIC_BYTES = {"c2": 739.94e6}
 
 
 def cpu_baseline(w, t, R_gpu, nthreads):
@@ -57,8 +61,8 @@ def cpu_baseline(w, t, R_gpu, nthreads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--latency-samples", type=int, default=50,
@@ -105,15 +109,18 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     value = n_int * args.steps / wall
 
-    # per-stage device times with HIP events recorded on the build stream
+    # The same K steps again with the library's HIP events recorded on both
+    # build streams around every kernel stage: per-stage device times averaged
+    # over the steps (and the profiled rate, to show the events cost nothing).
     rk.set_profiling(True)
-    stages = []
-    for _ in range(max(5, min(args.steps, 20))):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
         rk.compute()
-        stages.append(rk.stage_times())
+    torch.cuda.synchronize()
+    wall_prof = time.perf_counter() - t0
+    st = rk.stage_times()        # ms since start: windows, far end, diag end, near end, total
     rk.set_profiling(False)
-    stages = np.array(stages)
-    st = stages.mean(axis=0)     # ms since start: windows, far end, diag end, near end, total
     t_far = float(st[1] - st[0]) * 1e-3
     far_elems = n_int - rk.near_elems
     traffic_alg = 8.0 * far_elems            # algorithmic bytes of one k3_far launch
@@ -159,10 +166,12 @@ def main():
                         "build_done": float(st[4])},
         "tensor_stage_GBs": 8.0 * n_int / (float(st[4] - st[0]) * 1e-3) / 1e9,
         "event_ms_per_step": ev0.elapsed_time(ev1) / args.steps,
+        "profiled_ms_per_step": wall_prof * 1e3 / args.steps,
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "samples": int(args.latency_samples)},
         "roofline": {"bound": "hbm", "kernel": "k3_far (rank-one tensor fill)", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": FAR_TRAFFIC_BYTES.get(w.name),
                      "algorithmic_bytes": traffic_alg, "elements": far_elems,
                      "avg_launch_ms": t_far * 1e3},
     }

@@ -111,10 +111,12 @@ int64_t bs2e_rk_copy_to_host(bs2e_rk *rk, double *dst, int64_t n_elems);
  * mirror it copies the K values from the device.  BS2E_ERANGE for a key
  * outside |a-c| < k, |b-d| < k (the reference error-stops there). */
 int64_t bs2e_rk_get_val(const bs2e_rk *rk, const int64_t *indices, double *values, int64_t n_values);
-/* Device timeline (ms since the start of the most recent compute, profiling
- * on): [0] local moments + pair windows done, [1] rank-one fill done,
- * [2] diagonal cells done (second stream), [3] near ranges done (second
- * stream), [4] end of the build. */
+/* Device timeline of the builds computed while profiling is on (HIP events
+ * on the build's two streams), in ms since each build's start, averaged over
+ * the builds since profiling was switched on or since the previous
+ * bs2e_rk_stage_times call: [0] local moments + pair windows done,
+ * [1] rank-one fill done, [2] diagonal cells done (second stream), [3] near
+ * elements done (second stream), [4] end of the build. */
 int64_t bs2e_rk_set_profiling(bs2e_rk *rk, int64_t on);
 int64_t bs2e_rk_stage_times(bs2e_rk *rk, double *ms, int64_t n);
 int64_t bs2e_rk_release(bs2e_rk *rk);
