@@ -31,7 +31,7 @@ from .capi import Bs2eError, check, ptr
 
 __all__ = ["Bs2eError", "Context", "generate_grid", "b_spline", "sparse_4d", "sparse_6d", "setup_Slater_integrals",
            "Nd_DOK", "RkTensor", "compute_R_K_map", "ang_k_LS", "three_j", "six_j", "count_nnz", "sym",
-           "count_configs", "CSR_matrix", "Block", "construct_block_tensor"]
+           "count_configs", "CSR_matrix", "Block", "construct_block_tensor", "Server", "footprint"]
 
 
 class Context:
@@ -162,12 +162,18 @@ def setup_Slater_integrals(b_splines, max_k, k_GL, r_k, r_m_k, r_d_k, ctx: Conte
     r_m_k.j[:] = r_k.j
 
 
+class _ThreadErrors:
+    """Stands in for a context: errors are read from the calling thread's last message."""
+    handle = None
+
+
 class RkTensor:
     """Device-resident banded R^k tensor R[pair(a,c), pair(b,d), k] of one basis."""
 
-    def __init__(self, handle, ctx: Context):
+    def __init__(self, handle, ctx, owned=True):
         self.handle = handle
         self.ctx = ctx
+        self.owned = owned
         info = np.zeros(11, np.int64)
         check(capi.lib().bs2e_rk_info(handle, ptr(info, C.c_int64), 11), ctx.handle)
         (self.n_b, self.n_pairs, self.K, self.nnz4, self.nnz6, self.n_iv, self.k, self.k_GL, self.U,
@@ -228,14 +234,88 @@ class RkTensor:
         check(capi.lib().bs2e_rk_stage_times(self.handle, ptr(ms), 5), self.ctx.handle)
         return ms
 
+    def set_graph(self, on=True):
+        """Capture the build into a hipGraph at the next compute and replay it afterwards."""
+        check(capi.lib().bs2e_rk_set_graph(self.handle, int(on)), self.ctx.handle)
+
     def release(self):
         if self.handle:
-            capi.lib().bs2e_rk_release(self.handle)
+            if self.owned:
+                capi.lib().bs2e_rk_release(self.handle)
             self.handle = None
 
     def __del__(self):
         try:
             self.release()
+        except Exception:
+            pass
+
+
+def footprint(k, t, k_GL, max_k) -> int:
+    """Device bytes of one build (tables + tensor) of this basis."""
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    b = C.c_int64()
+    check(capi.lib().bs2e_footprint(k, len(t), ptr(t), k_GL, max_k, C.byref(b)))
+    return b.value
+
+
+class Server:
+    """A build service on one device (bs2e_server_*): jobs are admitted while a
+    stream pair is free and their footprint fits the HBM budget; admitted
+    builds overlap on the device."""
+
+    QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
+
+    def __init__(self, hbm_bytes, slots=4, device=0):
+        h = C.c_void_p()
+        check(capi.lib().bs2e_server_create(device, int(hbm_bytes), slots, C.byref(h)))
+        self.handle = h
+
+    def _check(self, rc):
+        if rc < 0:
+            msg = capi.lib().bs2e_server_last_error(self.handle)
+            raise Bs2eError(int(rc), msg.decode() if msg else "")
+        return rc
+
+    def submit(self, k, t, k_GL, max_k) -> int:
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        return self._check(capi.lib().bs2e_server_submit(self.handle, k, len(t), ptr(t), k_GL, max_k))
+
+    def progress(self) -> int:
+        return self._check(capi.lib().bs2e_server_progress(self.handle))
+
+    def status(self, ticket):
+        """(state, ms from submission to admission, ms from submission to completion)"""
+        st = C.c_int64()
+        tm = np.zeros(2)
+        self._check(capi.lib().bs2e_server_status(self.handle, ticket, C.byref(st), ptr(tm)))
+        return st.value, float(tm[0]), float(tm[1])
+
+    def wait(self, ticket):
+        self._check(capi.lib().bs2e_server_wait(self.handle, ticket))
+
+    def tensor(self, ticket) -> RkTensor:
+        h = C.c_void_p()
+        self._check(capi.lib().bs2e_server_tensor(self.handle, ticket, C.byref(h)))
+        return RkTensor(h, _ThreadErrors(), owned=False)
+
+    def retire(self, ticket):
+        self._check(capi.lib().bs2e_server_retire(self.handle, ticket))
+
+    def memory(self):
+        """(budget bytes, bytes held by admitted or finished jobs)"""
+        m = np.zeros(2, np.int64)
+        self._check(capi.lib().bs2e_server_memory(self.handle, ptr(m, C.c_int64)))
+        return int(m[0]), int(m[1])
+
+    def close(self):
+        if self.handle:
+            capi.lib().bs2e_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 

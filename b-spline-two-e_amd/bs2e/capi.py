@@ -58,6 +58,18 @@ SIGNATURES = [
     ("bs2e_block_sizes", i64, [vp, P(i64), i64]),
     ("bs2e_block_fill", i64, [vp, i64, P(dbl), P(dbl), P(i64), P(i64), P(dbl), P(i64), P(i64), P(dbl)]),
     ("bs2e_block_release", i64, [vp]),
+    ("bs2e_rk_set_graph", i64, [vp, i64]),
+    ("bs2e_server_create", i64, [i64, i64, i64, P(vp)]),
+    ("bs2e_server_destroy", i64, [vp]),
+    ("bs2e_server_last_error", C.c_char_p, [vp]),
+    ("bs2e_footprint", i64, [i64, i64, P(dbl), i64, i64, P(i64)]),
+    ("bs2e_server_submit", i64, [vp, i64, i64, P(dbl), i64, i64]),
+    ("bs2e_server_progress", i64, [vp]),
+    ("bs2e_server_status", i64, [vp, i64, P(i64), P(dbl)]),
+    ("bs2e_server_wait", i64, [vp, i64]),
+    ("bs2e_server_tensor", i64, [vp, i64, P(vp)]),
+    ("bs2e_server_retire", i64, [vp, i64]),
+    ("bs2e_server_memory", i64, [vp, P(i64)]),
 ]
 
 

@@ -112,28 +112,105 @@ double bs2e_six_j(int64_t j1, int64_t j2, int64_t j3, int64_t j4, int64_t j5, in
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
+// device arena
+// ---------------------------------------------------------------------------
+hipError_t bs2e::Arena::init(size_t bytes)
+{
+    bytes = (bytes + 255) / 256 * 256;
+    hipError_t e = hipMalloc((void**)&base_, bytes);
+    if (e != hipSuccess) {
+        base_ = nullptr;
+        return e;
+    }
+    cap_ = bytes;
+    used_ = 0;
+    free_.clear();
+    taken_.clear();
+    free_[0] = bytes;
+    return hipSuccess;
+}
+
+void bs2e::Arena::destroy()
+{
+    if (base_) (void)hipFree(base_);
+    base_ = nullptr;
+    cap_ = used_ = 0;
+    free_.clear();
+    taken_.clear();
+}
+
+void* bs2e::Arena::alloc(size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    bytes = (bytes + 255) / 256 * 256;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second < bytes) continue;
+        const size_t off = it->first, rest = it->second - bytes;
+        free_.erase(it);
+        if (rest) free_[off + bytes] = rest;
+        taken_[off] = bytes;
+        used_ += bytes;
+        return base_ + off;
+    }
+    return nullptr;
+}
+
+void bs2e::Arena::release(void* p)
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    const size_t off = (size_t)((char*)p - base_);
+    auto t = taken_.find(off);
+    if (t == taken_.end()) return;
+    size_t start = off, size = t->second;
+    used_ -= size;
+    taken_.erase(t);
+    auto next = free_.lower_bound(start);
+    if (next != free_.end() && next->first == start + size) {   // merge with the following range
+        size += next->second;
+        next = free_.erase(next);
+    }
+    if (next != free_.begin()) {                                 // and with the preceding one
+        auto prev = std::prev(next);
+        if (prev->first + prev->second == start) {
+            start = prev->first;
+            size += prev->second;
+            free_.erase(prev);
+        }
+    }
+    free_[start] = size;
+}
+
+size_t bs2e::Arena::largest_free() const
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    size_t m = 0;
+    for (const auto& f : free_) m = std::max(m, f.second);
+    return m;
+}
+
+// ---------------------------------------------------------------------------
 // tensor handle
 // ---------------------------------------------------------------------------
 static void free_tables(bs2e_rk* rk)
 {
     DevTables& d = rk->d;
-    dev_free(d.t);
-    dev_free(d.bp);
-    dev_free(d.x);
-    dev_free(d.w);
-    dev_free(d.pairs);
-    dev_free(d.Pl);
-    dev_free(d.Ql);
-    dev_free(d.Pw);
-    dev_free(d.Qw);
-    dev_free(d.PP);
-    dev_free(d.QS);
-    dev_free(d.Ptot);
-    dev_free(d.Qtot);
-    dev_free(d.D);
-    dev_free(d.rows);
-    dev_free(d.c_first);
-    dev_free(rk->R);
+    ctx_free(rk->ctx, d.t);
+    ctx_free(rk->ctx, d.bp);
+    ctx_free(rk->ctx, d.x);
+    ctx_free(rk->ctx, d.w);
+    ctx_free(rk->ctx, d.pairs);
+    ctx_free(rk->ctx, d.Pl);
+    ctx_free(rk->ctx, d.Ql);
+    ctx_free(rk->ctx, d.Pw);
+    ctx_free(rk->ctx, d.Qw);
+    ctx_free(rk->ctx, d.PP);
+    ctx_free(rk->ctx, d.QS);
+    ctx_free(rk->ctx, d.Ptot);
+    ctx_free(rk->ctx, d.Qtot);
+    ctx_free(rk->ctx, d.D);
+    ctx_free(rk->ctx, d.rows);
+    ctx_free(rk->ctx, d.c_first);
+    ctx_free(rk->ctx, rk->R);
     for (auto& set : rk->evpool)
         for (hipEvent_t e : set)
             if (e) (void)hipEventDestroy(e);
@@ -144,6 +221,10 @@ static void free_tables(bs2e_rk* rk)
             (void)hipEventDestroy(*e);
             *e = nullptr;
         }
+    if (rk->graph_exec) (void)hipGraphExecDestroy(rk->graph_exec);
+    if (rk->graph) (void)hipGraphDestroy(rk->graph);
+    rk->graph_exec = nullptr;
+    rk->graph = nullptr;
 }
 
 static int64_t create_events(bs2e_ctx* ctx, bs2e_rk* rk)
@@ -153,32 +234,44 @@ static int64_t create_events(bs2e_ctx* ctx, bs2e_rk* rk)
     return BS2E_OK;
 }
 
+// The allocations of alloc_tables(with_tensor = true), rounded as the arena rounds.
+size_t bs2e::rk_footprint(const Plan& p)
+{
+    auto r = [](size_t n, size_t elem) { return ((n ? n : 1) * elem + 255) / 256 * 256; };
+    const size_t slots = (size_t)p.n_pairs * p.k_s * p.K;
+    size_t b = r(p.n_knots, 8) + r(p.n_iv + 1, 8) + 2 * r(p.k_gl, 8) + r(p.n_pairs, sizeof(PairInfo));
+    b += 2 * r((size_t)p.n_iv * p.U * p.K, 8) + 4 * r(slots, 8) + 2 * r((size_t)p.n_pairs * p.K, 8);
+    b += r((size_t)p.n_iv * p.K * p.U * p.U, 8);
+    b += r((size_t)p.n_pairs * p.n_pairs * p.K, 8) + r(p.rows.size(), 4) + r(p.c_first.size(), 4);
+    return b;
+}
+
 static int64_t alloc_tables(bs2e_ctx* ctx, bs2e_rk* rk, bool with_tensor)
 {
-    if (with_tensor) build_near_rows(rk->plan);
+    if (with_tensor && rk->plan.rows.empty()) build_near_rows(rk->plan);
     const Plan& p = rk->plan;
     DevTables& d = rk->d;
     const size_t slots = (size_t)p.n_pairs * p.k_s * p.K;
-    HIP_TRY(ctx, dev_alloc(&d.t, p.n_knots));
-    HIP_TRY(ctx, dev_alloc(&d.bp, p.n_iv + 1));
-    HIP_TRY(ctx, dev_alloc(&d.x, p.k_gl));
-    HIP_TRY(ctx, dev_alloc(&d.w, p.k_gl));
-    HIP_TRY(ctx, dev_alloc(&d.pairs, p.n_pairs));
-    HIP_TRY(ctx, dev_alloc(&d.Pl, (size_t)p.n_iv * p.U * p.K));
-    HIP_TRY(ctx, dev_alloc(&d.Ql, (size_t)p.n_iv * p.U * p.K));
-    HIP_TRY(ctx, dev_alloc(&d.Pw, slots));
-    HIP_TRY(ctx, dev_alloc(&d.Qw, slots));
-    HIP_TRY(ctx, dev_alloc(&d.PP, slots));
-    HIP_TRY(ctx, dev_alloc(&d.QS, slots));
-    HIP_TRY(ctx, dev_alloc(&d.Ptot, (size_t)p.n_pairs * p.K));
-    HIP_TRY(ctx, dev_alloc(&d.Qtot, (size_t)p.n_pairs * p.K));
-    HIP_TRY(ctx, dev_alloc(&d.D, (size_t)p.n_iv * p.K * p.U * p.U));
+    HIP_TRY(ctx, ctx_alloc(ctx, &d.t, p.n_knots));
+    HIP_TRY(ctx, ctx_alloc(ctx, &d.bp, p.n_iv + 1));
+    HIP_TRY(ctx, ctx_alloc(ctx, &d.x, p.k_gl));
+    HIP_TRY(ctx, ctx_alloc(ctx, &d.w, p.k_gl));
+    HIP_TRY(ctx, ctx_alloc(ctx, &d.pairs, p.n_pairs));
+    HIP_TRY(ctx, ctx_alloc(ctx, &d.Pl, (size_t)p.n_iv * p.U * p.K));
+    HIP_TRY(ctx, ctx_alloc(ctx, &d.Ql, (size_t)p.n_iv * p.U * p.K));
+    HIP_TRY(ctx, ctx_alloc(ctx, &d.Pw, slots));
+    HIP_TRY(ctx, ctx_alloc(ctx, &d.Qw, slots));
+    HIP_TRY(ctx, ctx_alloc(ctx, &d.PP, slots));
+    HIP_TRY(ctx, ctx_alloc(ctx, &d.QS, slots));
+    HIP_TRY(ctx, ctx_alloc(ctx, &d.Ptot, (size_t)p.n_pairs * p.K));
+    HIP_TRY(ctx, ctx_alloc(ctx, &d.Qtot, (size_t)p.n_pairs * p.K));
+    HIP_TRY(ctx, ctx_alloc(ctx, &d.D, (size_t)p.n_iv * p.K * p.U * p.U));
     if (with_tensor) {
         rk->R_elems = (size_t)p.n_pairs * p.n_pairs * p.K;
-        HIP_TRY(ctx, dev_alloc(&rk->R, rk->R_elems));
-        HIP_TRY(ctx, dev_alloc(&d.rows, p.rows.size()));
+        HIP_TRY(ctx, ctx_alloc(ctx, &rk->R, rk->R_elems));
+        HIP_TRY(ctx, ctx_alloc(ctx, &d.rows, p.rows.size()));
     }
-    HIP_TRY(ctx, dev_alloc(&d.c_first, p.c_first.size()));
+    HIP_TRY(ctx, ctx_alloc(ctx, &d.c_first, p.c_first.size()));
     hipStream_t s = ctx->stream;
     HIP_TRY(ctx, hipMemcpyAsync(d.c_first, p.c_first.data(), sizeof(int32_t) * p.c_first.size(), hipMemcpyHostToDevice,
                                 s));
@@ -342,21 +435,20 @@ int64_t bs2e_slater_primitives(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const 
     return BS2E_OK;
 }
 
-int64_t bs2e_rk_create(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const double* knots, int64_t k_gl, int64_t max_k,
-                       int64_t precision, bs2e_rk** out)
+}  // extern "C"
+
+// A tensor handle for an existing plan (whose near rows are built): checks
+// the limits, allocates (in the context's arena when it has one), uploads.
+int64_t bs2e::rk_from_plan(bs2e_ctx* ctx, Plan&& plan, bs2e_rk** out)
 {
-    if (!ctx) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_create: ctx is NULL");
-    if (!out) return fail(ctx, BS2E_EINVAL, "bs2e_rk_create: out is NULL");
     *out = nullptr;
-    if (precision != BS2E_PREC_FP64) return fail(ctx, BS2E_EINVAL, "bs2e_rk_create: unknown precision mode");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     bs2e_rk* rk = new (std::nothrow) bs2e_rk();
     if (!rk) return fail(ctx, BS2E_ENOMEM, "bs2e_rk_create: host allocation failed");
     rk->ctx = ctx;
-    rk->precision = precision;
-    std::string msg = make_plan(k, n_knots, knots, k_gl, max_k, rk->plan);
-    int64_t rc =
-        msg.empty() ? check_plan_limits(ctx, rk->plan, true) : fail(ctx, BS2E_EINVAL, "bs2e_rk_create: " + msg);
+    rk->precision = BS2E_PREC_FP64;
+    rk->plan = std::move(plan);
+    int64_t rc = check_plan_limits(ctx, rk->plan, true);
     if (!rc) rc = alloc_tables(ctx, rk, true);
     if (!rc) rc = create_events(ctx, rk);
     if (rc) {
@@ -366,6 +458,22 @@ int64_t bs2e_rk_create(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const double* 
     }
     *out = rk;
     return BS2E_OK;
+}
+
+extern "C" {
+
+int64_t bs2e_rk_create(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const double* knots, int64_t k_gl, int64_t max_k,
+                       int64_t precision, bs2e_rk** out)
+{
+    if (!ctx) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_create: ctx is NULL");
+    if (!out) return fail(ctx, BS2E_EINVAL, "bs2e_rk_create: out is NULL");
+    *out = nullptr;
+    if (precision != BS2E_PREC_FP64) return fail(ctx, BS2E_EINVAL, "bs2e_rk_create: unknown precision mode");
+    Plan plan;
+    std::string msg = make_plan(k, n_knots, knots, k_gl, max_k, plan);
+    if (!msg.empty()) return fail(ctx, BS2E_EINVAL, "bs2e_rk_create: " + msg);
+    build_near_rows(plan);
+    return rk_from_plan(ctx, std::move(plan), out);
 }
 
 int64_t bs2e_rk_from_primitives(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const double* knots, int64_t max_k,
@@ -480,9 +588,43 @@ int64_t bs2e_rk_compute(bs2e_rk* rk)
     bs2e_ctx* ctx = rk->ctx;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     rk->mirror_valid = false;
-    int64_t rc = run_build(ctx, rk);
-    if (rc) return rc;
+    if (rk->use_graph) {
+        if (!rk->graph_exec) {
+            // capture the two-stream build once; its kernels and event
+            // dependencies become the graph's nodes (no memset nodes)
+            HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            int64_t rc = run_build(ctx, rk);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            HIP_TRY(ctx, e);
+            rk->graph = g;
+            HIP_TRY(ctx, hipGraphInstantiateWithFlags(&rk->graph_exec, rk->graph, 0));
+        }
+        HIP_TRY(ctx, hipGraphLaunch(rk->graph_exec, ctx->stream));
+    } else {
+        int64_t rc = run_build(ctx, rk);
+        if (rc) return rc;
+    }
     rk->computed = true;
+    return BS2E_OK;
+}
+
+int64_t bs2e_rk_set_graph(bs2e_rk* rk, int64_t on)
+{
+    if (!rk) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_set_graph: rk is NULL");
+    if (on && rk->profiling)
+        return fail(rk->ctx, BS2E_ESTATE, "bs2e_rk_set_graph: switch profiling off first (events are not captured)");
+    if (!on || rk->use_graph != (on != 0)) {
+        if (rk->graph_exec) (void)hipGraphExecDestroy(rk->graph_exec);
+        if (rk->graph) (void)hipGraphDestroy(rk->graph);
+        rk->graph_exec = nullptr;
+        rk->graph = nullptr;
+    }
+    rk->use_graph = on != 0;
     return BS2E_OK;
 }
 
@@ -567,6 +709,8 @@ int64_t bs2e_rk_get_val(const bs2e_rk* rk, const int64_t* idx, double* values, i
 int64_t bs2e_rk_set_profiling(bs2e_rk* rk, int64_t on)
 {
     if (!rk) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_set_profiling: rk is NULL");
+    if (on && rk->use_graph)
+        return fail(rk->ctx, BS2E_ESTATE, "bs2e_rk_set_profiling: the build is a captured graph; switch it off first");
     rk->profiling = on != 0;
     rk->evused = 0;
     return BS2E_OK;
@@ -594,8 +738,12 @@ int64_t bs2e_rk_release(bs2e_rk* rk)
 {
     if (!rk) return BS2E_OK;
     (void)hipSetDevice(rk->ctx->device);
-    (void)hipStreamSynchronize(rk->ctx->stream);
-    (void)hipStreamSynchronize(rk->ctx->side);
+    // A build service's contexts (arena set) carry other jobs' work; the
+    // service releases a tensor only after its build has completed.
+    if (!rk->ctx->arena) {
+        (void)hipStreamSynchronize(rk->ctx->stream);
+        (void)hipStreamSynchronize(rk->ctx->side);
+    }
     free_tables(rk);
     delete rk;
     return BS2E_OK;

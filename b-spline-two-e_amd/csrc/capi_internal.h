@@ -2,14 +2,42 @@
 #pragma once
 
 #include <array>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/bs2e.h"
 #include "bs2e_internal.h"
 
+namespace bs2e {
+
+// First-fit sub-allocator over one device allocation: the memory of a build
+// service, so that admitting and retiring jobs never calls hipMalloc/hipFree
+// (hipFree waits for the whole device).  Host-side bookkeeping only.
+class Arena {
+public:
+    hipError_t init(size_t bytes);
+    void destroy();
+    void* alloc(size_t bytes);   // 256-byte aligned; nullptr when no free range fits
+    void release(void* p);
+    size_t capacity() const { return cap_; }
+    size_t in_use() const { return used_; }
+    size_t largest_free() const;
+
+private:
+    char* base_ = nullptr;
+    size_t cap_ = 0, used_ = 0;
+    std::map<size_t, size_t> free_;    // offset -> size
+    std::map<size_t, size_t> taken_;   // offset -> size
+    mutable std::mutex mu_;
+};
+
+}  // namespace bs2e
+
 struct bs2e_ctx {
     int device = 0;
+    bs2e::Arena* arena = nullptr;   // when set, the tensors of this context live in it
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;   // the build's stream (own, or the caller's)
     hipStream_t side = nullptr;     // diagonal cells + near ranges, concurrent with the rank-one fill
@@ -31,6 +59,9 @@ struct bs2e_rk {
     std::vector<std::array<hipEvent_t, 6>> evpool;
     size_t evused = 0;
     hipEvent_t windows = nullptr, join = nullptr;                                  // stream ordering
+    bool use_graph = false;         // compute launches a captured hipGraph of the build
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
     bool computed = false;
     bool from_primitives = false;   // tables gathered from caller containers: compute runs the fill only
 };
@@ -53,6 +84,32 @@ inline void dev_free(T*& p)
     if (p) (void)hipFree(p);
     p = nullptr;
 }
+
+// Allocation of a context's tensors: its arena when it has one.
+template <class T>
+inline hipError_t ctx_alloc(bs2e_ctx* c, T** p, size_t n)
+{
+    if (c && c->arena) {
+        *p = (T*)c->arena->alloc((n ? n : 1) * sizeof(T));
+        return *p ? hipSuccess : hipErrorOutOfMemory;
+    }
+    return dev_alloc(p, n);
+}
+
+template <class T>
+inline void ctx_free(bs2e_ctx* c, T*& p)
+{
+    if (p && c && c->arena) c->arena->release(p);
+    else if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// Device bytes a tensor build of this plan allocates (tables and tensor,
+// with the arena's alignment); the plan's near rows must be built.
+size_t rk_footprint(const Plan& p);
+
+// bs2e_rk_create for an existing plan with its near rows built.
+int64_t rk_from_plan(bs2e_ctx* ctx, Plan&& plan, bs2e_rk** out);
 
 }  // namespace bs2e
 

@@ -119,6 +119,11 @@ int64_t bs2e_rk_get_val(const bs2e_rk *rk, const int64_t *indices, double *value
  * elements done (second stream), [4] end of the build. */
 int64_t bs2e_rk_set_profiling(bs2e_rk *rk, int64_t on);
 int64_t bs2e_rk_stage_times(bs2e_rk *rk, double *ms, int64_t n);
+/* on != 0: the next bs2e_rk_compute captures the build (both streams) into a
+ * hipGraph and every compute then launches that graph, with one launch call
+ * instead of five kernels and the stream events; 0 drops the graph.  Not
+ * combinable with profiling.  (No reference counterpart.) */
+int64_t bs2e_rk_set_graph(bs2e_rk *rk, int64_t on);
 int64_t bs2e_rk_release(bs2e_rk *rk);
 
 /* ---- angular algebra --------------------------------------------------- */
@@ -163,6 +168,44 @@ int64_t bs2e_block_sizes(const bs2e_block *b, int64_t *sizes, int64_t n);
 int64_t bs2e_block_fill(bs2e_block *b, int64_t max_l_1p, const double *Hl, const double *S, int64_t *ip_H,
                         int64_t *ix_H, double *d_H, int64_t *ip_S, int64_t *ix_S, double *d_S);
 int64_t bs2e_block_release(bs2e_block *b);
+
+/* ---- build service ----------------------------------------------------- */
+/* No reference counterpart: the reference builds one basis per process
+ * (main_basis_setup.f90:75-86).  A service owns an HBM budget (one device
+ * allocation, sub-allocated without hipMalloc/hipFree) and `slots` pairs of
+ * default-priority streams.  Jobs are R^k builds of differing bases (orders,
+ * knot vectors, GL orders, multipole counts); a queued job is admitted as
+ * soon as a slot is free and its footprint fits the free budget (in arrival
+ * order, smaller jobs passing one that does not fit yet), so builds of
+ * several jobs overlap on the device.  Each job's tensor is computed by the
+ * same kernels as bs2e_rk_compute, so its values do not depend on what else
+ * runs.  Calls on one service must come from one thread at a time. */
+typedef struct bs2e_server bs2e_server;
+int64_t bs2e_server_create(int64_t device, int64_t hbm_bytes, int64_t slots, bs2e_server **out);
+int64_t bs2e_server_destroy(bs2e_server *srv);
+const char *bs2e_server_last_error(const bs2e_server *srv);
+/* Device bytes one build of this basis allocates (tables + tensor). */
+int64_t bs2e_footprint(int64_t k, int64_t n_knots, const double *knots, int64_t k_gl, int64_t max_k, int64_t *bytes);
+/* Queues a job and admits what fits; returns its ticket (>= 0) or a
+ * negative status (BS2E_ENOMEM when the job alone exceeds the budget). */
+int64_t bs2e_server_submit(bs2e_server *srv, int64_t k, int64_t n_knots, const double *knots, int64_t k_gl,
+                           int64_t max_k);
+/* Retires finished builds and admits queued jobs; returns the number of
+ * jobs still queued or running. */
+int64_t bs2e_server_progress(bs2e_server *srv);
+/* state: 0 queued, 1 running, 2 done, 3 failed; times[0] = admission and
+ * times[1] = completion, ms after submission (-1 when not yet). */
+int64_t bs2e_server_status(bs2e_server *srv, int64_t ticket, int64_t *state, double *times);
+/* Blocks until the job is done, admitting others meanwhile. */
+int64_t bs2e_server_wait(bs2e_server *srv, int64_t ticket);
+/* The finished job's tensor (read it with bs2e_rk_copy_to_host, _get_val,
+ * _device_ptr or the block calls); it stays valid until the job is retired
+ * and belongs to the service (do not bs2e_rk_release it). */
+int64_t bs2e_server_tensor(bs2e_server *srv, int64_t ticket, bs2e_rk **rk);
+/* Returns a finished job's memory to the budget. */
+int64_t bs2e_server_retire(bs2e_server *srv, int64_t ticket);
+/* mem[0] = budget bytes, mem[1] = bytes held by admitted or finished jobs. */
+int64_t bs2e_server_memory(const bs2e_server *srv, int64_t *mem);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,126 @@
+"""Graph replay of a build and the build service (bs2e_server_*).
+
+* A captured hipGraph of the two-stream build gives the eager build's tensor
+  bit for bit, on every replay, with other eager builds and synchronisations
+  between the replays.
+* Jobs of differing bases admitted concurrently by the service give, each,
+  the tensor the same basis gives when built alone (bit for bit: the same
+  kernels, no atomics, nothing shared between jobs), and the service never
+  holds more device memory than its budget.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import bs2e  # noqa: E402
+from bs2e import capi  # noqa: E402
+from bs2e.workloads import WORKLOADS  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return bs2e.Context(0)
+
+
+def knots_of(name):
+    w = WORKLOADS[name]
+    return w, bs2e.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+
+
+def test_graph_replays_match_eager(ctx):
+    w, t = knots_of("c1")
+    eager = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx)
+    ref = eager.to_host()
+    g = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx, compute=False)
+    g.set_graph(True)
+    w0, t0 = knots_of("t0")
+    other = bs2e.RkTensor.build(w0.k, t0, w0.k_GL, w0.max_k, ctx=ctx, compute=False)
+    for rep in range(4):
+        print(f"graph replay {rep}", flush=True)
+        g.compute()
+        assert np.array_equal(g.to_host(), ref), rep
+        other.compute()               # eager work between replays
+        other.synchronize()
+    with pytest.raises(capi.Bs2eError) as e:
+        g.set_profiling(True)
+    assert e.value.code == capi.BS2E_ESTATE
+    g.set_graph(False)
+    g.compute()
+    assert np.array_equal(g.to_host(), ref)
+    for x in (g, other, eager):
+        x.release()
+
+
+def jobs():
+    out = []
+    for name in ("tiny", "t0", "c1"):
+        w, t = knots_of(name)
+        out.append((w.k, t, w.k_GL, w.max_k))
+    rng = np.random.default_rng(5)
+    for _ in range(7):
+        k = int(rng.integers(5, 10))
+        t = bs2e.generate_grid(k, 3, 2, float(rng.uniform(1.0, 3.0)), float(rng.uniform(20.0, 90.0)))
+        out.append((k, t, k + 4, int(rng.integers(2, 9))))
+    return out
+
+
+def test_service_results_equal_alone(ctx):
+    js = jobs()
+    alone = []
+    for k, t, kgl, mk in js:
+        r = bs2e.RkTensor.build(k, t, kgl, mk, ctx=ctx)
+        alone.append(r.to_host())
+        r.release()
+    fp = [bs2e.footprint(k, t, kgl, mk) for k, t, kgl, mk in js]
+    budget = int(2.5 * max(fp))
+    srv = bs2e.Server(budget, slots=3)
+    tickets = []
+    for n, (k, t, kgl, mk) in enumerate(js):
+        tickets.append(srv.submit(k, t, kgl, mk))
+        cap, used = srv.memory()
+        assert cap >= budget and used <= cap
+        if n == 2:
+            # three slots and room for them: the first three jobs were admitted
+            # at submission, none waited for another to finish
+            states = [srv.status(x)[0] for x in tickets]
+            assert all(st in (bs2e.Server.RUNNING, bs2e.Server.DONE) for st in states), states
+    print(f"service: {len(js)} jobs, budget {budget / 1e6:.1f} MB", flush=True)
+    for i, tk in enumerate(tickets):
+        srv.wait(tk)
+        st, t_admit, t_done = srv.status(tk)
+        assert st == bs2e.Server.DONE and 0.0 <= t_admit <= t_done
+        got = srv.tensor(tk).to_host()
+        assert np.array_equal(got, alone[i]), i
+        srv.retire(tk)
+    assert srv.memory()[1] == 0
+    # a job larger than the whole budget is refused at submission
+    small = bs2e.Server(max(fp) // 4, slots=1)
+    k, t, kgl, mk = js[int(np.argmax(fp))]
+    with pytest.raises(capi.Bs2eError) as e:
+        small.submit(k, t, kgl, mk)
+    assert e.value.code == capi.BS2E_ENOMEM
+    small.close()
+    srv.close()
+
+
+def test_service_waits_for_memory(ctx):
+    """With room for one job at a time, later jobs start only after earlier
+    ones are retired, and wait() reports a job that cannot be admitted."""
+    js = jobs()[:3][::-1]          # c1, t0, tiny: the first takes nearly the whole budget
+    fp = [bs2e.footprint(k, t, kgl, mk) for k, t, kgl, mk in js]
+    assert fp[0] == max(fp) and min(fp) > 4096
+    srv = bs2e.Server(max(fp) + 4096, slots=2)
+    tk = [srv.submit(*j) for j in js]
+    assert srv.status(tk[0])[0] in (bs2e.Server.RUNNING, bs2e.Server.DONE)
+    assert srv.status(tk[1])[0] == bs2e.Server.QUEUED
+    srv.wait(tk[0])
+    with pytest.raises(capi.Bs2eError) as e:      # job 1 waits for job 0's memory, which is not retired
+        srv.wait(tk[1])
+    assert e.value.code == capi.BS2E_ESTATE
+    srv.retire(tk[0])
+    for x in tk[1:]:
+        srv.wait(x)
+        srv.retire(x)
+    assert srv.memory()[1] == 0
+    srv.close()
