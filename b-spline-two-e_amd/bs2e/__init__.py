@@ -299,6 +299,13 @@ class Server:
         self._check(capi.lib().bs2e_server_tensor(self.handle, ticket, C.byref(h)))
         return RkTensor(h, _ThreadErrors(), owned=False)
 
+    def finished(self, cap=64):
+        """Progresses the service; [(ticket, latency ms)] of the jobs done since the previous call."""
+        tk = np.zeros(cap, np.int64)
+        lat = np.zeros(cap)
+        n = self._check(capi.lib().bs2e_server_finished(self.handle, ptr(tk, C.c_int64), ptr(lat), cap))
+        return [(int(tk[i]), float(lat[i])) for i in range(n)]
+
     def retire(self, ticket):
         self._check(capi.lib().bs2e_server_retire(self.handle, ticket))
 

@@ -69,6 +69,7 @@ SIGNATURES = [
     ("bs2e_server_wait", i64, [vp, i64]),
     ("bs2e_server_tensor", i64, [vp, i64, P(vp)]),
     ("bs2e_server_retire", i64, [vp, i64]),
+    ("bs2e_server_finished", i64, [vp, P(i64), P(dbl), i64]),
     ("bs2e_server_memory", i64, [vp, P(i64)]),
 ]
 

@@ -60,13 +60,14 @@ struct Plan {
     std::vector<int32_t> c_first;       // first pair index with column c (1-based c, size n_b+2)
     std::vector<DiagEntry> diag;        // sparse_6d order (built on demand)
     std::vector<int32_t> rows;          // near-kernel rows: pairs (a, c) with a <= c (built with the tensor)
-    int64_t near_elems = 0;             // tensor elements whose pairs share a support interval
+    int64_t near_elems = -1;            // tensor elements whose pairs share a support interval (-1: not counted)
 };
 
 // Builds the plan; returns an error message or empty string.
 std::string make_plan(int64_t k_s, int64_t n_knots, const double* knots, int64_t k_gl, int64_t max_k, Plan& p);
 void build_diag_entries(Plan& p);
 void build_near_rows(Plan& p);
+int64_t count_near_elements(const Plan& p);
 
 // Host numerics shared by the C-ABI (setup, not the hot path).
 void gauss_legendre(int64_t N, double* x, double* w);

@@ -194,11 +194,10 @@ size_t bs2e::Arena::largest_free() const
 static void free_tables(bs2e_rk* rk)
 {
     DevTables& d = rk->d;
-    ctx_free(rk->ctx, d.t);
-    ctx_free(rk->ctx, d.bp);
-    ctx_free(rk->ctx, d.x);
-    ctx_free(rk->ctx, d.w);
-    ctx_free(rk->ctx, d.pairs);
+    ctx_free(rk->ctx, rk->small);   // knots, breakpoints, GL rule, pairs, near rows, c_first
+    d.t = d.bp = d.x = d.w = nullptr;
+    d.pairs = nullptr;
+    d.rows = d.c_first = nullptr;
     ctx_free(rk->ctx, d.Pl);
     ctx_free(rk->ctx, d.Ql);
     ctx_free(rk->ctx, d.Pw);
@@ -208,8 +207,6 @@ static void free_tables(bs2e_rk* rk)
     ctx_free(rk->ctx, d.Ptot);
     ctx_free(rk->ctx, d.Qtot);
     ctx_free(rk->ctx, d.D);
-    ctx_free(rk->ctx, d.rows);
-    ctx_free(rk->ctx, d.c_first);
     ctx_free(rk->ctx, rk->R);
     for (auto& set : rk->evpool)
         for (hipEvent_t e : set)
@@ -234,29 +231,69 @@ static int64_t create_events(bs2e_ctx* ctx, bs2e_rk* rk)
     return BS2E_OK;
 }
 
+// The small tables of a build, packed into one device block (256-byte aligned
+// pieces): knots, breakpoints, GL nodes and weights, pairs, near rows, c_first.
+namespace {
+struct SmallLayout {
+    size_t t, bp, x, w, pairs, rows, c_first, total;
+};
+SmallLayout small_layout(const Plan& p, bool with_tensor)
+{
+    auto r = [](size_t bytes) { return (std::max<size_t>(bytes, 1) + 255) / 256 * 256; };
+    SmallLayout L{};
+    size_t o = 0;
+    L.t = o, o += r(8 * p.n_knots);
+    L.bp = o, o += r(8 * (p.n_iv + 1));
+    L.x = o, o += r(8 * p.k_gl);
+    L.w = o, o += r(8 * p.k_gl);
+    L.pairs = o, o += r(sizeof(PairInfo) * p.n_pairs);
+    L.rows = o, o += with_tensor ? r(4 * p.rows.size()) : 0;
+    L.c_first = o, o += r(4 * p.c_first.size());
+    L.total = o;
+    return L;
+}
+}  // namespace
+
 // The allocations of alloc_tables(with_tensor = true), rounded as the arena rounds.
 size_t bs2e::rk_footprint(const Plan& p)
 {
     auto r = [](size_t n, size_t elem) { return ((n ? n : 1) * elem + 255) / 256 * 256; };
     const size_t slots = (size_t)p.n_pairs * p.k_s * p.K;
-    size_t b = r(p.n_knots, 8) + r(p.n_iv + 1, 8) + 2 * r(p.k_gl, 8) + r(p.n_pairs, sizeof(PairInfo));
+    size_t b = small_layout(p, true).total;
     b += 2 * r((size_t)p.n_iv * p.U * p.K, 8) + 4 * r(slots, 8) + 2 * r((size_t)p.n_pairs * p.K, 8);
     b += r((size_t)p.n_iv * p.K * p.U * p.U, 8);
-    b += r((size_t)p.n_pairs * p.n_pairs * p.K, 8) + r(p.rows.size(), 4) + r(p.c_first.size(), 4);
+    b += r((size_t)p.n_pairs * p.n_pairs * p.K, 8);
     return b;
 }
 
+// Allocates the build's device tables (in the context's arena when it has
+// one) and enqueues the upload of the small ones on the context's stream as
+// one copy from the handle's packed host image (which lives as long as the
+// handle); the kernels that read them follow on the same stream.
 static int64_t alloc_tables(bs2e_ctx* ctx, bs2e_rk* rk, bool with_tensor)
 {
     if (with_tensor && rk->plan.rows.empty()) build_near_rows(rk->plan);
     const Plan& p = rk->plan;
     DevTables& d = rk->d;
     const size_t slots = (size_t)p.n_pairs * p.k_s * p.K;
-    HIP_TRY(ctx, ctx_alloc(ctx, &d.t, p.n_knots));
-    HIP_TRY(ctx, ctx_alloc(ctx, &d.bp, p.n_iv + 1));
-    HIP_TRY(ctx, ctx_alloc(ctx, &d.x, p.k_gl));
-    HIP_TRY(ctx, ctx_alloc(ctx, &d.w, p.k_gl));
-    HIP_TRY(ctx, ctx_alloc(ctx, &d.pairs, p.n_pairs));
+    const SmallLayout L = small_layout(p, with_tensor);
+    HIP_TRY(ctx, ctx_alloc(ctx, &rk->small, L.total));
+    rk->small_host.assign(L.total, 0);
+    char* h = rk->small_host.data();
+    std::memcpy(h + L.t, p.t.data(), 8 * p.n_knots);
+    std::memcpy(h + L.bp, p.bp.data(), 8 * (p.n_iv + 1));
+    std::memcpy(h + L.x, p.x.data(), 8 * p.k_gl);
+    std::memcpy(h + L.w, p.w.data(), 8 * p.k_gl);
+    std::memcpy(h + L.pairs, p.pairs.data(), sizeof(PairInfo) * p.n_pairs);
+    if (with_tensor) std::memcpy(h + L.rows, p.rows.data(), 4 * p.rows.size());
+    std::memcpy(h + L.c_first, p.c_first.data(), 4 * p.c_first.size());
+    d.t = (double*)(rk->small + L.t);
+    d.bp = (double*)(rk->small + L.bp);
+    d.x = (double*)(rk->small + L.x);
+    d.w = (double*)(rk->small + L.w);
+    d.pairs = (PairInfo*)(rk->small + L.pairs);
+    d.rows = with_tensor ? (int32_t*)(rk->small + L.rows) : nullptr;
+    d.c_first = (int32_t*)(rk->small + L.c_first);
     HIP_TRY(ctx, ctx_alloc(ctx, &d.Pl, (size_t)p.n_iv * p.U * p.K));
     HIP_TRY(ctx, ctx_alloc(ctx, &d.Ql, (size_t)p.n_iv * p.U * p.K));
     HIP_TRY(ctx, ctx_alloc(ctx, &d.Pw, slots));
@@ -269,21 +306,8 @@ static int64_t alloc_tables(bs2e_ctx* ctx, bs2e_rk* rk, bool with_tensor)
     if (with_tensor) {
         rk->R_elems = (size_t)p.n_pairs * p.n_pairs * p.K;
         HIP_TRY(ctx, ctx_alloc(ctx, &rk->R, rk->R_elems));
-        HIP_TRY(ctx, ctx_alloc(ctx, &d.rows, p.rows.size()));
     }
-    HIP_TRY(ctx, ctx_alloc(ctx, &d.c_first, p.c_first.size()));
-    hipStream_t s = ctx->stream;
-    HIP_TRY(ctx, hipMemcpyAsync(d.c_first, p.c_first.data(), sizeof(int32_t) * p.c_first.size(), hipMemcpyHostToDevice,
-                                s));
-    if (with_tensor) {
-        HIP_TRY(ctx, hipMemcpyAsync(d.rows, p.rows.data(), sizeof(int32_t) * p.rows.size(), hipMemcpyHostToDevice, s));
-    }
-    HIP_TRY(ctx, hipMemcpyAsync(d.t, p.t.data(), sizeof(double) * p.n_knots, hipMemcpyHostToDevice, s));
-    HIP_TRY(ctx, hipMemcpyAsync(d.bp, p.bp.data(), sizeof(double) * (p.n_iv + 1), hipMemcpyHostToDevice, s));
-    HIP_TRY(ctx, hipMemcpyAsync(d.x, p.x.data(), sizeof(double) * p.k_gl, hipMemcpyHostToDevice, s));
-    HIP_TRY(ctx, hipMemcpyAsync(d.w, p.w.data(), sizeof(double) * p.k_gl, hipMemcpyHostToDevice, s));
-    HIP_TRY(ctx, hipMemcpyAsync(d.pairs, p.pairs.data(), sizeof(PairInfo) * p.n_pairs, hipMemcpyHostToDevice, s));
-    HIP_TRY(ctx, hipStreamSynchronize(s));
+    HIP_TRY(ctx, hipMemcpyAsync(rk->small, h, L.total, hipMemcpyHostToDevice, ctx->stream));
     return BS2E_OK;
 }
 
@@ -639,6 +663,7 @@ int64_t bs2e_rk_info(const bs2e_rk* rk, int64_t* info, int64_t n)
 {
     if (!rk || !info) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_info: NULL argument");
     const Plan& p = rk->plan;
+    if (p.near_elems < 0) const_cast<Plan&>(p).near_elems = count_near_elements(p);   // counted once, on demand
     const int64_t v[11] = {p.n_b, p.n_pairs, p.K, p.nnz4, p.nnz6, p.n_iv, p.k_s, p.k_gl, p.U, p.n_knots, p.near_elems};
     for (int64_t i = 0; i < n && i < 11; ++i) info[i] = v[i];
     return BS2E_OK;

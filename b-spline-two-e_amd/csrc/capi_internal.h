@@ -48,6 +48,8 @@ struct bs2e_rk {
     bs2e_ctx* ctx = nullptr;
     bs2e::Plan plan;
     bs2e::DevTables d;
+    char* small = nullptr;          // one device block holding the small tables (knots .. c_first)
+    std::vector<char> small_host;   // their packed host image, uploaded with one copy
     double* R = nullptr;
     size_t R_elems = 0;
     int64_t precision = BS2E_PREC_FP64;

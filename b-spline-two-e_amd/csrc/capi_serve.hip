@@ -40,6 +40,7 @@ struct bs2e_server {
     std::vector<int64_t> slot_job;   // ticket running on each slot, or -1
     std::deque<int64_t> queue;
     std::map<int64_t, Job> jobs;
+    std::deque<int64_t> finished;    // done since the last bs2e_server_finished
     int64_t next = 0;
     bs2e_ctx* err = nullptr;         // error messages of the service
 };
@@ -70,6 +71,7 @@ static int64_t progress(bs2e_server* s)
         j.state = 2;
         j.t_done = now_ms();
         s->slot_job[i] = -1;
+        s->finished.push_back(t);
     }
     for (auto it = s->queue.begin(); it != s->queue.end();) {
         int slot = -1;
@@ -284,6 +286,24 @@ int64_t bs2e_server_retire(bs2e_server* s, int64_t ticket)
     s->jobs.erase(it);
     const int64_t rc = progress(s);
     return rc < 0 ? rc : BS2E_OK;
+}
+
+int64_t bs2e_server_finished(bs2e_server* s, int64_t* tickets, double* latency_ms, int64_t cap)
+{
+    if (!s || (cap > 0 && !tickets)) return fail(nullptr, BS2E_EINVAL, "bs2e_server_finished: NULL");
+    const int64_t rc = progress(s);
+    if (rc < 0) return rc;
+    int64_t n = 0;
+    while (n < cap && !s->finished.empty()) {
+        const int64_t t = s->finished.front();
+        s->finished.pop_front();
+        auto it = s->jobs.find(t);
+        if (it == s->jobs.end()) continue;   // retired already
+        tickets[n] = t;
+        if (latency_ms) latency_ms[n] = it->second.t_done - it->second.t_submit;
+        ++n;
+    }
+    return n;
 }
 
 int64_t bs2e_server_memory(const bs2e_server* s, int64_t* mem)

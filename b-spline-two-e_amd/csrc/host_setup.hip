@@ -120,17 +120,17 @@ std::string make_plan(int64_t k_s, int64_t n_knots, const double* knots, int64_t
     p.nnz4 = nnz4;
     p.U = ks * (ks + 1) / 2;
 
-    // nnz6 = sum over (J, I) of the overlap of their windows (sparse_array_tools.f90:298-324).
-    int64_t nnz6 = 0;
-    for (int64_t J = 0; J < p.n_pairs; ++J) {
-        const PairInfo& pj = p.pairs[J];
-        int64_t cmin = std::max<int64_t>(1, pj.v0 - 1);
-        int64_t cmax = std::min<int64_t>(nb, pj.v0 + pj.m - 1 + ks - 2);
-        for (int64_t I = p.c_first[cmin]; I < p.c_first[cmax + 1]; ++I) {
-            const PairInfo& pi = p.pairs[I];
-            int64_t lo = std::max(pi.v0, pj.v0), hi = std::min(pi.v0 + pi.m, pj.v0 + pj.m) - 1;
-            if (hi >= lo) nnz6 += hi - lo + 1;
-        }
+    // nnz6 = sum over (J, I) of the overlap of their windows (sparse_array_tools.f90:298-324)
+    //      = sum over intervals v of (number of pairs whose window holds v)^2.
+    std::vector<int64_t> cover((size_t)niv + 2, 0);
+    for (const PairInfo& pi : p.pairs) {
+        cover[pi.v0] += 1;
+        cover[pi.v0 + pi.m] -= 1;
+    }
+    int64_t nnz6 = 0, run = 0;
+    for (int64_t v = 1; v <= niv; ++v) {
+        run += cover[v];
+        nnz6 += run * run;
     }
     p.nnz6 = nnz6;
     return std::string();

@@ -221,24 +221,30 @@ __global__ __launch_bounds__(256) void k_symmetrize(int n_iv, int U, int K, doub
     *b = s;
 }
 
-// Host: the near kernel's rows (pairs (a, c) with a <= c) and the number of
-// tensor elements whose pairs' windows overlap.
+// Host: the near kernel's rows (pairs (a, c) with a <= c).
 void build_near_rows(Plan& p)
 {
     p.rows.clear();
-    p.near_elems = 0;
+    for (int64_t P = 0; P < p.n_pairs; ++P)
+        if (p.pairs[P].a <= p.pairs[P].c) p.rows.push_back((int32_t)P);
+}
+
+// Host: the number of tensor elements whose pairs' windows overlap.
+int64_t count_near_elements(const Plan& p)
+{
+    int64_t n = 0;
     const int64_t ks = p.k_s, nb = p.n_b;
     for (int64_t P = 0; P < p.n_pairs; ++P) {
         const PairInfo& pp = p.pairs[P];
-        if (pp.a <= pp.c) p.rows.push_back((int32_t)P);
         const int64_t v1p = pp.v0 + pp.m - 1;
         const int64_t cmin = std::max<int64_t>(1, pp.v0 - 1);
         const int64_t cmax = std::min<int64_t>(nb, v1p + ks - 2);
         for (int64_t Q = p.c_first[cmin]; Q < p.c_first[cmax + 1]; ++Q) {
             const PairInfo& pq = p.pairs[Q];
-            if (pq.v0 <= v1p && pp.v0 <= pq.v0 + pq.m - 1) p.near_elems += p.K;
+            if (pq.v0 <= v1p && pp.v0 <= pq.v0 + pq.m - 1) n += p.K;
         }
     }
+    return n;
 }
 
 hipError_t launch_tensor_near(const Plan& p, const DevTables& d, double* R, hipStream_t s)

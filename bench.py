@@ -58,6 +58,93 @@ def cpu_baseline(w, t, R_gpu, nthreads):
                 seconds=dt), rel
 
 
+def solve_h_max(k, m, Z, r_max, n_b):
+    """h_max whose grid (grid_tools.f90:6-55 rule) gives exactly n_b B-splines, or None."""
+    import bs2e
+    nb = lambda h: len(bs2e.generate_grid(k, m, Z, h, r_max)) - k - 2   # noqa: E731
+    lo, hi = r_max / 4000.0, r_max
+    if nb(lo) < n_b or nb(hi) > n_b:
+        return None
+    for _ in range(80):
+        mid = 0.5 * (lo + hi)
+        v = nb(mid)
+        if v == n_b:
+            return mid
+        if v > n_b:
+            lo = mid
+        else:
+            hi = mid
+    return None
+
+
+def serving_jobs(n, seed):
+    """Serving stream jobs: B-spline count uniform in [40,400], order in {6,7,8,9},
+    r_max uniform in [50,500], lmax uniform in [2,8], k <= 2 lmax, k_GL = order+6."""
+    import numpy as np
+
+    import bs2e
+    rng = np.random.default_rng(seed)
+    jobs = []
+    while len(jobs) < n:
+        n_b = int(rng.integers(40, 401))
+        k = int(rng.choice([6, 7, 8, 9]))
+        r_max = float(rng.uniform(50.0, 500.0))
+        lmax = int(rng.integers(2, 9))
+        h = solve_h_max(k, 3, 2, r_max, n_b)
+        if h is None:
+            continue
+        jobs.append((k, bs2e.generate_grid(k, 3, 2, h, r_max), k + 6, 2 * lmax))
+    return jobs
+
+
+def run_stream(srv, jobs, arrivals):
+    """Submits job i at arrivals[i] (s after the start), retires finished jobs;
+    returns (latency ms per job, wall s)."""
+    n = len(jobs)
+    lat = [None] * n
+    tickets = {}
+    nxt = 0
+    t0 = time.perf_counter()
+    while nxt < n or tickets:
+        now = time.perf_counter() - t0
+        while nxt < n and arrivals[nxt] <= now:
+            tickets[srv.submit(*jobs[nxt])] = nxt
+            nxt += 1
+        for tk, ms in srv.finished():
+            lat[tickets.pop(tk)] = ms
+            srv.retire(tk)
+    return lat, time.perf_counter() - t0
+
+
+def serving_benchmark(n_jobs, seed=20261202, slots=4, budget_gib=64, load=0.8):
+    """Continuous batching of a seeded stream of differing builds through the
+    build service: closed-loop capacity, then Poisson arrivals at `load` x
+    capacity with p50/p99 latency (submission to completion)."""
+    import numpy as np
+
+    import bs2e
+    jobs = serving_jobs(n_jobs, 7)
+    integrals = []
+    for k, t, kgl, mk in jobs:
+        _, _, n_pairs = bs2e.count_nnz(k, t)
+        integrals.append(n_pairs * n_pairs * (mk + 1))
+    srv = bs2e.Server(budget_gib << 30, slots=slots)
+    run_stream(srv, jobs[:8], [0.0] * 8)                     # warm-up
+    _, wall_cap = run_stream(srv, jobs, [0.0] * n_jobs)       # all at once: capacity
+    cap = n_jobs / wall_cap
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / (load * cap), n_jobs))
+    lat, wall = run_stream(srv, jobs, arrivals)
+    srv.close()
+    lat = np.array(lat)
+    return {"jobs": n_jobs, "slots": slots, "budget_GiB": budget_gib, "job_seed": 7, "arrival_seed": seed,
+            "integrals_total": int(sum(integrals)),
+            "capacity_jobs_per_s": cap, "capacity_integrals_per_s": sum(integrals) / wall_cap,
+            "offered_load": load, "stream_integrals_per_s": sum(integrals) / wall,
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                           "mean": float(lat.mean()), "max": float(lat.max())}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +154,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--latency-samples", type=int, default=50,
                     help="synchronised single builds timed for p50/p99 latency")
+    ap.add_argument("--serve-jobs", type=int, default=256,
+                    help="jobs of the serving stream (differing bases through the build service); 0 skips it")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size 3 times on the build stream "
                          "(reference kernel for HBM counter scaling under rocprofv3 --pmc)")
@@ -121,6 +210,16 @@ def main():
     wall_prof = time.perf_counter() - t0
     st = rk.stage_times()        # ms since start: windows, far end, diag end, near end, total
     rk.set_profiling(False)
+    # the same K steps replayed from a captured hipGraph of the build
+    rk.set_graph(True)
+    rk.compute()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rk.compute()
+    torch.cuda.synchronize()
+    wall_graph = time.perf_counter() - t0
+    rk.set_graph(False)
     t_far = float(st[1] - st[0]) * 1e-3
     far_elems = n_int - rk.near_elems
     traffic_alg = 8.0 * far_elems            # algorithmic bytes of one k3_far launch
@@ -167,6 +266,7 @@ def main():
         "tensor_stage_GBs": 8.0 * n_int / (float(st[4] - st[0]) * 1e-3) / 1e9,
         "event_ms_per_step": ev0.elapsed_time(ev1) / args.steps,
         "profiled_ms_per_step": wall_prof * 1e3 / args.steps,
+        "graph_ms_per_step": wall_graph * 1e3 / args.steps,
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "samples": int(args.latency_samples)},
         "roofline": {"bound": "hbm", "kernel": "k3_far (rank-one tensor fill)", "achieved": achieved,
@@ -175,6 +275,8 @@ def main():
                      "algorithmic_bytes": traffic_alg, "elements": far_elems,
                      "avg_launch_ms": t_far * 1e3},
     }
+    if args.serve_jobs > 0:
+        out["serving"] = serving_benchmark(args.serve_jobs)
     if not args.no_cpu_baseline:
         nthreads = min(16, os.cpu_count() or 1)
         cb, rel = cpu_baseline(w, t, R_gpu, nthreads)

@@ -202,6 +202,10 @@ int64_t bs2e_server_wait(bs2e_server *srv, int64_t ticket);
  * _device_ptr or the block calls); it stays valid until the job is retired
  * and belongs to the service (do not bs2e_rk_release it). */
 int64_t bs2e_server_tensor(bs2e_server *srv, int64_t ticket, bs2e_rk **rk);
+/* Progresses the service and hands out the jobs that finished since the
+ * previous call (at most cap): their tickets and their latency (ms from
+ * submission to completion).  Returns the count. */
+int64_t bs2e_server_finished(bs2e_server *srv, int64_t *tickets, double *latency_ms, int64_t cap);
 /* Returns a finished job's memory to the budget. */
 int64_t bs2e_server_retire(bs2e_server *srv, int64_t ticket);
 /* mem[0] = budget bytes, mem[1] = bytes held by admitted or finished jobs. */

@@ -115,6 +115,9 @@ def test_service_waits_for_memory(ctx):
     assert srv.status(tk[0])[0] in (bs2e.Server.RUNNING, bs2e.Server.DONE)
     assert srv.status(tk[1])[0] == bs2e.Server.QUEUED
     srv.wait(tk[0])
+    done = srv.finished()
+    assert [t for t, _ in done] == [tk[0]] and done[0][1] > 0.0
+    assert srv.finished() == []
     with pytest.raises(capi.Bs2eError) as e:      # job 1 waits for job 0's memory, which is not retired
         srv.wait(tk[1])
     assert e.value.code == capi.BS2E_ESTATE
