@@ -47,9 +47,8 @@ int64_t bs2e_create(bs2e_ctx** out, int64_t device)
     return BS2E_OK;
 }
 
-int64_t bs2e_destroy(bs2e_ctx* ctx)
+static void ctx_free_now(bs2e_ctx* ctx)
 {
-    if (!ctx) return BS2E_OK;
     (void)hipSetDevice(ctx->device);
     for (hipStream_t s : {ctx->own, ctx->side})
         if (s) {
@@ -57,6 +56,16 @@ int64_t bs2e_destroy(bs2e_ctx* ctx)
             (void)hipStreamDestroy(s);
         }
     delete ctx;
+}
+
+int64_t bs2e_destroy(bs2e_ctx* ctx)
+{
+    if (!ctx) return BS2E_OK;
+    if (ctx->refs > 0) {      // tensors still use its streams: freed with the last of them
+        ctx->closing = true;
+        return BS2E_OK;
+    }
+    ctx_free_now(ctx);
     return BS2E_OK;
 }
 
@@ -331,6 +340,7 @@ static int64_t check_plan_limits(bs2e_ctx* ctx, const Plan& p, bool needs_primit
 // Primitives on one stream, in order (for callers that want the containers).
 static int64_t run_primitives_serial(bs2e_ctx* ctx, bs2e_rk* rk)
 {
+    clear_stale_error();
     hipStream_t s = ctx->stream;
     HIP_TRY(ctx, launch_outer_moments(rk->plan, rk->d, s));
     HIP_TRY(ctx, launch_diag_cells(rk->plan, rk->d, s, false));
@@ -349,6 +359,7 @@ static constexpr size_t kMaxProfiledBuilds = 1 << 16;
 
 static int64_t run_build(bs2e_ctx* ctx, bs2e_rk* rk)
 {
+    clear_stale_error();
     hipStream_t m = ctx->stream, s = ctx->side;
     const bool prof = rk->profiling && rk->evused < kMaxProfiledBuilds;
     hipEvent_t* ev = nullptr;
@@ -469,7 +480,7 @@ int64_t bs2e::rk_from_plan(bs2e_ctx* ctx, Plan&& plan, bs2e_rk** out)
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     bs2e_rk* rk = new (std::nothrow) bs2e_rk();
     if (!rk) return fail(ctx, BS2E_ENOMEM, "bs2e_rk_create: host allocation failed");
-    rk->ctx = ctx;
+    rk_attach(ctx, rk);
     rk->precision = BS2E_PREC_FP64;
     rk->plan = std::move(plan);
     int64_t rc = check_plan_limits(ctx, rk->plan, true);
@@ -477,7 +488,7 @@ int64_t bs2e::rk_from_plan(bs2e_ctx* ctx, Plan&& plan, bs2e_rk** out)
     if (!rc) rc = create_events(ctx, rk);
     if (rc) {
         free_tables(rk);
-        delete rk;
+        rk_delete(rk);
         return rc;
     }
     *out = rk;
@@ -514,7 +525,7 @@ int64_t bs2e_rk_from_primitives(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     bs2e_rk* rk = new (std::nothrow) bs2e_rk();
     if (!rk) return fail(ctx, BS2E_ENOMEM, "bs2e_rk_from_primitives: host allocation failed");
-    rk->ctx = ctx;
+    rk_attach(ctx, rk);
     rk->from_primitives = true;
     // compute_R_K_map has no quadrature argument; the GL order only sizes work
     // tables that this path never fills, so the smallest rule is planned.
@@ -527,7 +538,7 @@ int64_t bs2e_rk_from_primitives(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const
     if (!rc) rc = create_events(ctx, rk);
     if (rc) {
         free_tables(rk);
-        delete rk;
+        rk_delete(rk);
         return rc;
     }
     const Plan& p = rk->plan;
@@ -540,7 +551,7 @@ int64_t bs2e_rk_from_primitives(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const
     auto bail = [&](int64_t code) {
         cleanup();
         free_tables(rk);
-        delete rk;
+        rk_delete(rk);
         return code;
     };
 #define STEP(expr)                                                                                           \
@@ -582,6 +593,7 @@ int64_t bs2e_rk_from_primitives(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const
     const int zero = 0;
     STEP(hipMemcpyAsync(d_bad, &zero, sizeof(int), hipMemcpyHostToDevice, s));
     const size_t slots = (size_t)p.n_pairs * p.k_s * p.K;
+    clear_stale_error();
     STEP(launch_fill(rk->d.Pw, (int64_t)slots, 0.0, s));
     STEP(launch_fill(rk->d.Qw, (int64_t)slots, 0.0, s));
     STEP(launch_fill(rk->d.D, (int64_t)((size_t)p.n_iv * p.K * p.U * p.U), 0.0, s));
@@ -597,7 +609,7 @@ int64_t bs2e_rk_from_primitives(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const
     cleanup();
     if (bad != 0) {
         free_tables(rk);
-        delete rk;
+        rk_delete(rk);
         return fail(ctx, BS2E_ERANGE,
                     "bs2e_rk_from_primitives: " + std::to_string(bad) +
                         " container entries have indices outside the banded, supported set of this basis");
@@ -762,6 +774,30 @@ int64_t bs2e_rk_stage_times(bs2e_rk* rk, double* ms, int64_t n)
 int64_t bs2e_rk_release(bs2e_rk* rk)
 {
     if (!rk) return BS2E_OK;
+    if (rk->blocks > 0) {     // blocks still read the tensor: freed with the last of them
+        rk->closing = true;
+        return BS2E_OK;
+    }
+    return rk_release_now(rk);
+}
+
+}  // extern "C"
+
+void bs2e::rk_attach(bs2e_ctx* ctx, bs2e_rk* rk)
+{
+    rk->ctx = ctx;
+    ctx->refs += 1;
+}
+
+void bs2e::rk_delete(bs2e_rk* rk)
+{
+    bs2e_ctx* ctx = rk->ctx;
+    delete rk;
+    if (ctx && --ctx->refs == 0 && ctx->closing) ctx_free_now(ctx);
+}
+
+int64_t bs2e::rk_release_now(bs2e_rk* rk)
+{
     (void)hipSetDevice(rk->ctx->device);
     // A build service's contexts (arena set) carry other jobs' work; the
     // service releases a tensor only after its build has completed.
@@ -770,8 +806,6 @@ int64_t bs2e_rk_release(bs2e_rk* rk)
         (void)hipStreamSynchronize(rk->ctx->side);
     }
     free_tables(rk);
-    delete rk;
+    rk_delete(rk);
     return BS2E_OK;
 }
-
-}  // extern "C"

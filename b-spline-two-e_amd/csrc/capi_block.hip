@@ -226,6 +226,7 @@ int64_t bs2e_block_create(bs2e_rk* rk, int64_t L, int64_t n_config, const int64_
                                     hipMemcpyHostToDevice, s));
         HIP_TRY(ctx, hipMemcpyAsync(b->d_flags, b->flags.data(), sizeof(int32_t) * b->flags.size(),
                                     hipMemcpyHostToDevice, s));
+        clear_stale_error();
         BlockArgs a = block_args(b);
         a.cntH = b->d_cnt;
         a.cntS = b->d_cnt + n_config;
@@ -259,6 +260,7 @@ int64_t bs2e_block_create(bs2e_rk* rk, int64_t L, int64_t n_config, const int64_
         delete b;
         return rc;
     }
+    rk->blocks += 1;
     *out = b;
     return BS2E_OK;
 }
@@ -302,6 +304,7 @@ int64_t bs2e_block_fill(bs2e_block* b, int64_t max_l_1p, const double* Hl, const
         hipStream_t s = ctx->stream;
         HIP_TRY(ctx, hipMemcpyAsync(dHl, Hl, sizeof(double) * 2 * nn * (max_l_1p + 1), hipMemcpyHostToDevice, s));
         HIP_TRY(ctx, hipMemcpyAsync(dS, S, sizeof(double) * 2 * nn, hipMemcpyHostToDevice, s));
+        clear_stale_error();
         BlockArgs a = block_args(b);
         a.Hl = dHl;
         a.S = dS;
@@ -335,9 +338,11 @@ int64_t bs2e_block_fill(bs2e_block* b, int64_t max_l_1p, const double* Hl, const
 int64_t bs2e_block_release(bs2e_block* b)
 {
     if (!b) return BS2E_OK;
-    (void)hipSetDevice(b->rk->ctx->device);
+    bs2e_rk* rk = b->rk;
+    (void)hipSetDevice(rk->ctx->device);
     block_free(b);
     delete b;
+    if (--rk->blocks == 0 && rk->closing) return rk_release_now(rk);   // the tensor's release was deferred
     return BS2E_OK;
 }
 

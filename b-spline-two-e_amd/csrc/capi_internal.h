@@ -37,6 +37,8 @@ private:
 
 struct bs2e_ctx {
     int device = 0;
+    int refs = 0;                   // live tensor handles created on this context
+    bool closing = false;           // bs2e_destroy was called; freed with the last tensor
     bs2e::Arena* arena = nullptr;   // when set, the tensors of this context live in it
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;   // the build's stream (own, or the caller's)
@@ -65,12 +67,27 @@ struct bs2e_rk {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool computed = false;
+    int blocks = 0;                 // live block handles reading this tensor
+    bool closing = false;           // bs2e_rk_release was called; freed with the last block
     bool from_primitives = false;   // tables gathered from caller containers: compute runs the fill only
 };
 
 namespace bs2e {
 
 int64_t fail(bs2e_ctx* ctx, int64_t code, const std::string& msg);
+
+// Kernel launches report hipGetLastError(); an unrelated earlier call of the
+// process that failed without being checked would otherwise be reported by
+// the next launch, so a launch sequence starts by dropping it.
+inline void clear_stale_error() { (void)hipGetLastError(); }
+
+// Handle lifetimes: a tensor keeps its context, a block keeps its tensor; a
+// destroy/release of the owner while dependents live is deferred to the
+// release of the last dependent (so that no release order frees memory a
+// live handle still reads).
+void rk_attach(bs2e_ctx* ctx, bs2e_rk* rk);
+void rk_delete(bs2e_rk* rk);      // deletes the handle object, then drops its context reference
+int64_t rk_release_now(bs2e_rk* rk);
 
 template <class T>
 inline hipError_t dev_alloc(T** p, size_t n)

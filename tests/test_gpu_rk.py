@@ -164,3 +164,32 @@ def test_invalid_arguments(ctx):
     h = C.c_void_p()
     assert capi.lib().bs2e_rk_create(ctx.handle, w.k, len(t), capi.ptr(t), w.k_GL, w.max_k, 99, C.byref(h)) \
         == capi.BS2E_EINVAL
+
+
+def test_release_order_does_not_matter(oracle):
+    """A context destroyed before its tensor, and a tensor released before its
+    block, stay usable until the last dependent handle goes; afterwards new
+    builds work normally."""
+    w = WORKLOADS["tiny"]
+    t = knots(w)
+    n_b = oracle.dims(w.k, t)[1]
+    c = bs2e.Context(0)
+    rk = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=c)
+    term = bs2e.sym(l=0, pi=False)
+    term.configs = bs2e.count_configs(term, 1, n_b, w.k, n_b, n_b, 1)
+    blk = bs2e.Block(rk, term)
+    c.close()
+    rk.release()
+    S = oracle.setup_S(w.k, t, w.k_GL)
+    Hl = [oracle.setup_H1p(w.k, t, w.k_GL, 2, l, cap_eta=0.0) for l in range(2)]
+    H_sp, S_sp = blk.fill(Hl, S)
+    assert H_sp.nnz > 0 and S_sp.nnz > 0
+    blk.release()
+    c2 = bs2e.Context(0)
+    rk2 = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=c2)
+    ref = oracle.rk_dense(w.k, n_b, w.max_k, oracle.slater_off_diag(w.k, t, w.k_GL, w.max_k),
+                          oracle.slater_diag(w.k, t, w.k_GL, w.max_k))
+    rel, _ = max_rel(rk2.to_host(), ref)
+    assert rel <= TOL
+    rk2.release()
+    c2.close()
