@@ -234,6 +234,16 @@ class RkTensor:
         check(capi.lib().bs2e_rk_stage_times(self.handle, ptr(ms), 5), self.ctx.handle)
         return ms
 
+    def autotune(self, force=False):
+        """Time the fill shapes and diagonal-cell chunks on this basis; keep the fastest."""
+        check(capi.lib().bs2e_rk_autotune(self.handle, int(force)), self.ctx.handle)
+
+    def tuning(self):
+        """(rank-one fill shape, multipoles per diagonal-cell workgroup) in use."""
+        out = np.zeros(2, np.int64)
+        check(capi.lib().bs2e_rk_tuning(self.handle, ptr(out, C.c_int64), 2), self.ctx.handle)
+        return int(out[0]), int(out[1])
+
     def set_graph(self, on=True):
         """Capture the build into a hipGraph at the next compute and replay it afterwards."""
         check(capi.lib().bs2e_rk_set_graph(self.handle, int(on)), self.ctx.handle)

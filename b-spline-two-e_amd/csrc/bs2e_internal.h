@@ -59,6 +59,8 @@ struct Plan {
     std::vector<int64_t> off4;          // per pair: first sparse_4d position
     std::vector<int32_t> c_first;       // first pair index with column c (1-based c, size n_b+2)
     std::vector<DiagEntry> diag;        // sparse_6d order (built on demand)
+    int far_variant = -1;               // rank-one fill shape (index into the far variants; -1: default)
+    int inner_kc = 0;                   // multipoles per diagonal-cell workgroup (0: the largest that fits)
     std::vector<int32_t> rows;          // near-kernel rows: pairs (a, c) with a <= c (built with the tensor)
     int64_t near_elems = -1;            // tensor elements whose pairs share a support interval (-1: not counted)
 };
@@ -139,6 +141,9 @@ size_t near_lds_bytes(int64_t ks, int64_t K);
 constexpr int kNearMaxSeg = 64;   // column segments of one near row (<= 2k - 1 <= 39 for k <= 20)
 hipError_t launch_pair_windows(const Plan& p, const DevTables& d, hipStream_t s);            // Pw/Qw + sums
 hipError_t launch_tensor_far(const Plan& p, const DevTables& d, double* R, hipStream_t s);   // rank-one part
+int far_variant_count();
+bool far_variant_valid(const Plan& p, int v);      // 16-byte variants need rows of even length
+int far_variant_default(const Plan& p);
 hipError_t launch_tensor_near(const Plan& p, const DevTables& d, double* R, hipStream_t s);  // overlapping pairs
 size_t outer_lds_bytes(int64_t ks, int64_t kgl, int64_t K);
 size_t inner_lds_bytes(int64_t ks, int64_t kgl, int64_t kc);

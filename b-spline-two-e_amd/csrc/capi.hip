@@ -1,6 +1,9 @@
 // capi.hip -- the extern "C" boundary (include/bs2e.h) over the HIP kernels.
 #include <algorithm>
 #include <cstdio>
+#include <functional>
+#include <map>
+#include <tuple>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -646,6 +649,109 @@ int64_t bs2e_rk_compute(bs2e_rk* rk)
         if (rc) return rc;
     }
     rk->computed = true;
+    return BS2E_OK;
+}
+
+// Autotuner: per (order, GL order, multipoles, pairs, intervals) the fastest
+// rank-one fill shape and diagonal-cell multipole chunk, timed on the
+// handle's own tables with HIP events, kept for the process.
+namespace {
+struct TuneKey {
+    int64_t ks, kgl, K, n_pairs, n_iv;
+    bool operator<(const TuneKey& o) const
+    {
+        return std::tie(ks, kgl, K, n_pairs, n_iv) < std::tie(o.ks, o.kgl, o.K, o.n_pairs, o.n_iv);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::pair<int, int>> g_tuned;
+}  // namespace
+
+static int64_t time_ms(bs2e_ctx* ctx, const std::function<hipError_t()>& launch, float* ms)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(ctx, hipEventCreate(&a));
+    HIP_TRY(ctx, hipEventCreate(&b));
+    hipError_t e = launch();   // warm-up
+    for (int r = 0; r < 3 && e == hipSuccess; ++r) {
+        if (r == 0) e = hipEventRecord(a, ctx->stream);
+        if (e == hipSuccess) e = launch();
+    }
+    if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    HIP_TRY(ctx, e);
+    return BS2E_OK;
+}
+
+int64_t bs2e_rk_autotune(bs2e_rk* rk, int64_t force)
+{
+    if (!rk) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_autotune: rk is NULL");
+    bs2e_ctx* ctx = rk->ctx;
+    if (rk->from_primitives) return fail(ctx, BS2E_ESTATE, "bs2e_rk_autotune: tensors built from containers are not tuned");
+    if (rk->use_graph) return fail(ctx, BS2E_ESTATE, "bs2e_rk_autotune: switch the graph off first");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    Plan& p = rk->plan;
+    const TuneKey key{p.k_s, p.k_gl, p.K, p.n_pairs, p.n_iv};
+    if (!force) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            p.far_variant = it->second.first;
+            p.inner_kc = it->second.second;
+            return BS2E_OK;
+        }
+    }
+    const bool prof = rk->profiling;
+    rk->profiling = false;
+    int64_t rc = run_build(ctx, rk);   // tables in place (every variant computes the same values)
+    if (!rc) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    float best = 1e30f;
+    int best_far = far_variant_default(p);
+    for (int v = 0; v < far_variant_count() && !rc; ++v) {
+        if (!far_variant_valid(p, v)) continue;
+        p.far_variant = v;
+        float ms = 0.f;
+        rc = time_ms(ctx, [&]() { return launch_tensor_far(p, rk->d, rk->R, ctx->stream); }, &ms);
+        if (!rc && ms < best) {
+            best = ms;
+            best_far = v;
+        }
+    }
+    p.far_variant = best_far;
+    best = 1e30f;
+    int best_kc = 0;
+    const int kmax = inner_kc(p.k_s, p.k_gl, p.K);
+    for (int kc : {1, 2, 4}) {
+        if (rc || kc > kmax) continue;
+        p.inner_kc = kc;
+        float ms = 0.f;
+        rc = time_ms(ctx, [&]() { return launch_diag_cells(p, rk->d, ctx->stream, true); }, &ms);
+        if (!rc && ms < best) {
+            best = ms;
+            best_kc = kc;
+        }
+    }
+    p.inner_kc = best_kc;
+    rk->profiling = prof;
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[key] = {p.far_variant, p.inner_kc};
+    }
+    rk->mirror_valid = false;   // the timing launches rewrote R and D with the same values
+    return BS2E_OK;
+}
+
+int64_t bs2e_rk_tuning(const bs2e_rk* rk, int64_t* out, int64_t n)
+{
+    if (!rk || !out) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_tuning: NULL argument");
+    const Plan& p = rk->plan;
+    const int64_t v[2] = {far_variant_valid(p, p.far_variant) ? p.far_variant : far_variant_default(p),
+                          (p.inner_kc >= 1) ? p.inner_kc : inner_kc(p.k_s, p.k_gl, p.K)};
+    for (int64_t i = 0; i < n && i < 2; ++i) out[i] = v[i];
     return BS2E_OK;
 }
 

@@ -326,7 +326,8 @@ static void launch_outer(const Plan& p, const DevTables& d, hipStream_t s)
 template <int KS, bool SYM>
 static void launch_inner(const Plan& p, const DevTables& d, hipStream_t s)
 {
-    const int kc = inner_kc(KS, p.k_gl, p.K);
+    const int kmax = inner_kc(KS, p.k_gl, p.K);
+    const int kc = (p.inner_kc >= 1 && p.inner_kc <= kmax) ? p.inner_kc : kmax;
     const int kchunks = (int)((p.K + kc - 1) / kc);
     hipLaunchKernelGGL((kp_inner<KS, SYM>), dim3((unsigned)(p.n_iv * kchunks)), dim3(kPThreads),
                        inner_lds_bytes(KS, p.k_gl, kc), s, (int)p.K, (int)p.k_gl, kc, kchunks, d.t, d.bp, d.x, d.w,

@@ -256,23 +256,45 @@ hipError_t launch_tensor_near(const Plan& p, const DevTables& d, double* R, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_tensor_far(const Plan& p, const DevTables& d, double* R, hipStream_t s)
+// Shapes of the rank-one fill the autotuner chooses from: elements per
+// store (V), rows and element groups per workgroup.
+struct FarShape {
+    int V, rows, iters;
+};
+static constexpr FarShape kFarShapes[] = {{2, 2, 2}, {2, 4, 4}, {2, 2, 8}, {2, 1, 4}, {1, 4, 8}, {1, 2, 4}};
+
+int far_variant_count() { return (int)(sizeof(kFarShapes) / sizeof(kFarShapes[0])); }
+
+bool far_variant_valid(const Plan& p, int v)
+{
+    return v >= 0 && v < far_variant_count() && (kFarShapes[v].V == 1 || (p.n_pairs * p.K) % 2 == 0);
+}
+
+// 2 rows x 1024 elements, 16-byte stores, measured fastest on its own at
+// 0.7-20 GB tensors; rows of odd length fall back to 8-byte stores.
+int far_variant_default(const Plan& p) { return (p.n_pairs * p.K) % 2 == 0 ? 0 : 4; }
+
+template <int V, int ROWS, int ITERS>
+static void launch_far_shape(const Plan& p, const DevTables& d, double* R, hipStream_t s)
 {
     const int64_t rowlen = p.n_pairs * p.K;
-    if (rowlen % 2 == 0) {
-        // 2 rows x 1024 elements per workgroup, 16-byte stores (measured
-        // fastest at the 0.7-20 GB tensor sizes of interest)
-        constexpr int rows = 2, iters = 2, per = 2 * kThreads * iters;
-        const int far_cols = (int)((rowlen + per - 1) / per);
-        const int far_rows = (int)((p.n_pairs + rows - 1) / rows);
-        hipLaunchKernelGGL((k3_far<2, rows, iters>), dim3((unsigned)(far_cols * far_rows)), dim3(kThreads), 0, s,
-                           (int)p.n_pairs, (int)p.K, far_cols, d.pairs, d.Ptot, d.Qtot, R);
-    } else {
-        constexpr int rows = 4, iters = 8, per = kThreads * iters;
-        const int far_cols = (int)((rowlen + per - 1) / per);
-        const int far_rows = (int)((p.n_pairs + rows - 1) / rows);
-        hipLaunchKernelGGL((k3_far<1, rows, iters>), dim3((unsigned)(far_cols * far_rows)), dim3(kThreads), 0, s,
-                           (int)p.n_pairs, (int)p.K, far_cols, d.pairs, d.Ptot, d.Qtot, R);
+    const int64_t per = (int64_t)V * kThreads * ITERS;
+    const int far_cols = (int)((rowlen + per - 1) / per);
+    const int far_rows = (int)((p.n_pairs + ROWS - 1) / ROWS);
+    hipLaunchKernelGGL((k3_far<V, ROWS, ITERS>), dim3((unsigned)(far_cols * far_rows)), dim3(kThreads), 0, s,
+                       (int)p.n_pairs, (int)p.K, far_cols, d.pairs, d.Ptot, d.Qtot, R);
+}
+
+hipError_t launch_tensor_far(const Plan& p, const DevTables& d, double* R, hipStream_t s)
+{
+    const int v = far_variant_valid(p, p.far_variant) ? p.far_variant : far_variant_default(p);
+    switch (v) {
+    case 0: launch_far_shape<2, 2, 2>(p, d, R, s); break;
+    case 1: launch_far_shape<2, 4, 4>(p, d, R, s); break;
+    case 2: launch_far_shape<2, 2, 8>(p, d, R, s); break;
+    case 3: launch_far_shape<2, 1, 4>(p, d, R, s); break;
+    case 4: launch_far_shape<1, 4, 8>(p, d, R, s); break;
+    default: launch_far_shape<1, 2, 4>(p, d, R, s); break;
     }
     return hipGetLastError();
 }

@@ -182,6 +182,8 @@ def main():
     print(f"# bench {w.name}: n_b={rk.n_b} n_pairs={rk.n_pairs} K={rk.K} integrals/step={n_int}", file=sys.stderr,
           flush=True)
 
+    rk.autotune()                 # fill shape and multipole chunk timed on this basis
+    tuning = rk.tuning()
     for _ in range(args.warmup):
         rk.compute()
     torch.cuda.synchronize()
@@ -267,6 +269,7 @@ def main():
         "event_ms_per_step": ev0.elapsed_time(ev1) / args.steps,
         "profiled_ms_per_step": wall_prof * 1e3 / args.steps,
         "graph_ms_per_step": wall_graph * 1e3 / args.steps,
+        "tuning": {"far_fill_shape": tuning[0], "diag_multipole_chunk": tuning[1]},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "samples": int(args.latency_samples)},
         "roofline": {"bound": "hbm", "kernel": "k3_far (rank-one tensor fill)", "achieved": achieved,

@@ -124,6 +124,17 @@ int64_t bs2e_rk_stage_times(bs2e_rk *rk, double *ms, int64_t n);
  * instead of five kernels and the stream events; 0 drops the graph.  Not
  * combinable with profiling.  (No reference counterpart.) */
 int64_t bs2e_rk_set_graph(bs2e_rk *rk, int64_t on);
+/* Autotuner: times, on this handle's own tables, each shape of the rank-one
+ * fill (elements per store, rows and elements per workgroup) and each
+ * multipole chunk of the diagonal-cell kernel, and keeps the fastest for
+ * this handle and, for the process, for every basis of the same order, GL
+ * order, multipole count, pair and interval counts (force != 0 re-times a
+ * known one).  All choices compute the same values.  (No reference
+ * counterpart.) */
+int64_t bs2e_rk_autotune(bs2e_rk *rk, int64_t force);
+/* out[0] = rank-one fill shape in use, out[1] = multipoles per
+ * diagonal-cell workgroup. */
+int64_t bs2e_rk_tuning(const bs2e_rk *rk, int64_t *out, int64_t n);
 int64_t bs2e_rk_release(bs2e_rk *rk);
 
 /* ---- angular algebra --------------------------------------------------- */

@@ -127,3 +127,28 @@ def test_service_waits_for_memory(ctx):
         srv.retire(x)
     assert srv.memory()[1] == 0
     srv.close()
+
+
+@pytest.mark.parametrize("name", ["c1", "c2"])
+def test_autotune_keeps_values(ctx, name):
+    """Every fill shape and multipole chunk the autotuner picks from computes
+    the same tensor, bit for bit; a second basis of the same shape reuses the
+    choice."""
+    w, t = knots_of(name)
+    ref_rk = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx)
+    ref = ref_rk.to_host()
+    ref_rk.release()
+    rk = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx, compute=False)
+    rk.autotune(force=True)
+    shape, kc = rk.tuning()
+    print(f"{name}: fill shape {shape}, multipole chunk {kc}", flush=True)
+    assert 0 <= shape < 6 and kc in (1, 2, 4)
+    rk.compute()
+    assert np.array_equal(rk.to_host(), ref)
+    again = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx, compute=False)
+    again.autotune()
+    assert again.tuning() == (shape, kc)
+    again.compute()
+    assert np.array_equal(again.to_host(), ref)
+    rk.release()
+    again.release()
