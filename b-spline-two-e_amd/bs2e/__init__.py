@@ -31,7 +31,8 @@ from .capi import Bs2eError, check, ptr
 
 __all__ = ["Bs2eError", "Context", "generate_grid", "b_spline", "sparse_4d", "sparse_6d", "setup_Slater_integrals",
            "Nd_DOK", "RkTensor", "compute_R_K_map", "ang_k_LS", "three_j", "six_j", "count_nnz", "sym",
-           "count_configs", "CSR_matrix", "Block", "construct_block_tensor", "Server", "footprint"]
+           "count_configs", "CSR_matrix", "Block", "construct_block_tensor", "Server", "footprint", "setup_S",
+           "setup_H_one_particle"]
 
 
 class Context:
@@ -372,6 +373,36 @@ def compute_R_K_map(r_d_k, r_k, r_m_k, b_splines, max_k, R: Nd_DOK, ctx: Context
     R.tensor.compute()
     R.tensor.mirror()
     return R
+
+
+def setup_S(b_splines, k_GL, ctx: Context | None = None) -> np.ndarray:
+    """mat_els.f90:85-118: the overlap matrix (n_b x n_b complex)."""
+    ctx = ctx or default_context()
+    t = b_splines.knots
+    n_b = b_splines.n_b
+    S = np.zeros(n_b * n_b, np.complex128)
+    check(capi.lib().bs2e_setup_S(ctx.handle, b_splines.k, len(t), ptr(t), k_GL,
+                                  S.ctypes.data_as(C.POINTER(C.c_double))), ctx.handle)
+    return S.reshape(n_b, n_b, order="F")
+
+
+def setup_H_one_particle(Z, CAP_c, l, b_splines, k_GL, ctx: Context | None = None) -> np.ndarray:
+    """mat_els.f90:47-83 with potentials.f90 hydrogenic(Z) and CAP_c = (order, r_0, eta)
+    (CAP_tools.f90); l an int or a range of l's (then an (n_l, n_b, n_b) array)."""
+    ctx = ctx or default_context()
+    t = b_splines.knots
+    n_b = b_splines.n_b
+    ls = [l] if np.isscalar(l) else list(l)
+    l_min, l_max = min(ls), max(ls)
+    order, r0, eta = CAP_c
+    eta = complex(eta)
+    H = np.zeros((l_max - l_min + 1) * n_b * n_b, np.complex128)
+    check(capi.lib().bs2e_setup_H_one_particle(ctx.handle, b_splines.k, len(t), ptr(t), k_GL, int(Z), l_min, l_max,
+                                               int(order), float(r0), eta.real, eta.imag,
+                                               H.ctypes.data_as(C.POINTER(C.c_double))), ctx.handle)
+    H = H.reshape(l_max - l_min + 1, n_b, n_b).transpose(0, 2, 1)   # each block column-major
+    out = np.stack([H[x - l_min] for x in ls])
+    return out[0] if np.isscalar(l) else out
 
 
 def ang_k_LS(k, l1, l2, L):

@@ -125,6 +125,13 @@ struct BlockArgs {
     double* dS;                  // complex
 };
 
+// One-electron matrices: spline value / second-derivative tables at the GL
+// points (B, B2: [n_iv][k_gl][k]; rw: [n_iv][k_gl][2] r and w) and the
+// assembly of S (optional) and H for l = l_min .. l_min+n_l-1.
+hipError_t launch_one_body(const Plan& p, const DevTables& d, double* B, double* B2, double* rw, int l_min, int n_l,
+                           int Z, int cap_order, double cap_r0, double eta_re, double eta_im, double* S, double* H,
+                           hipStream_t s);
+
 hipError_t launch_block_count(const BlockArgs& a, hipStream_t s);
 hipError_t launch_block_fill(const BlockArgs& a, hipStream_t s);
 

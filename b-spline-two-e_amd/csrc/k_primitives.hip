@@ -23,48 +23,9 @@
 //   Pw/Qw[p][j][k] (j-th interval of p's support window), their prefix (P)
 //   and suffix (Q) sums and totals, gathered from Pl/Ql.
 #include "bs2e_internal.h"
+#include "bspline_device.h"
 
 namespace bs2e {
-
-__device__ __forceinline__ double ipow_d(double x, int m)
-{
-    // x**m for integer m >= 0 by square-and-multiply (how Fortran lowers an
-    // integer power).
-    unsigned n = (unsigned)m;
-    double y = (n & 1u) ? x : 1.0;
-    while (n >>= 1) {
-        x = x * x;
-        if (n & 1u) y = y * x;
-    }
-    return y;
-}
-
-// All KS splines non-zero on interval iv at x: out[ls] = B_{iv+ls}(x).
-// t is stored 0-based (t(n) == t[n-1]); i1 = k-1+iv is the left knot index.
-template <int KS>
-__device__ __forceinline__ void splines_at(const double* __restrict__ t, int i1, double x, double* out, int stride)
-{
-    double dr[KS], dl[KS], b[KS + 1];
-#pragma unroll
-    for (int j = 1; j <= KS - 1; ++j) {
-        dr[j] = t[i1 + j - 1] - x;   // d_p(j)
-        dl[j] = x - t[i1 - j];       // d_m(j)
-    }
-    b[1] = 1.0;
-#pragma unroll
-    for (int j = 1; j <= KS - 1; ++j) {
-        double saved = 0.0;
-#pragma unroll
-        for (int r = 1; r <= j; ++r) {
-            const double term = b[r] / (dr[r] + dl[j + 1 - r]);
-            b[r] = saved + dr[r] * term;
-            saved = dl[j + 1 - r] * term;
-        }
-        b[j + 1] = saved;
-    }
-#pragma unroll
-    for (int ls = 0; ls < KS; ++ls) out[ls * stride] = b[ls + 1];
-}
 
 __device__ __forceinline__ void local_pair(int u, int ks, int& ta, int& tc)
 {

@@ -7,7 +7,9 @@
 ! usage: bs2e_basis_setup k m Z h_max r_max k_GL max_k max_l L pi full key_stride hs_file out_file
 !
 ! hs_file (stream): S then H(l) for l = 0..max_l, each n_b x n_b complex(dp)
-! column-major (the one-electron matrices, mat_els.f90:85-118 and :294-321).
+! column-major (the one-electron matrices, mat_els.f90:85-118 and :294-321);
+! hs_file "device": they are computed by setup_S / setup_H_one_particle on
+! the device for the hydrogenic potential of charge Z (no CAP).
 ! out_file (stream, 8-byte integers and doubles) receives every result:
 !   n_knots, knots
 !   nnz4, K, r_k%data, r_m_k%data, r_k%iv, r_k%i, r_k%j
@@ -33,6 +35,8 @@ program bs2e_basis_setup
     complex(dp), allocatable :: S(:,:)
     type(sym) :: term
     type(CSR_matrix) :: H_sp, S_sp
+    type(hydrogenic) :: pot
+    type(CAP) :: CAP_c
     integer, allocatable :: keys(:,:)
     integer :: u, i, a, b, c, d, n_keys, p, q, n_pairs
 
@@ -95,13 +99,23 @@ program bs2e_basis_setup
     end do
 
     allocate(S(splines%n_b, splines%n_b), H_vec(0:max_l))
-    open(newunit=u, file=trim(hs_file), access='stream', form='unformatted', status='old', action='read')
-    read(u) S
-    do i = 0, max_l
-        call H_vec(i)%init(splines%n_b, splines%n_b)
-        read(u) H_vec(i)%data
-    end do
-    close(u)
+    if (trim(hs_file) == "device") then
+        call pot%init(Z)
+        call CAP_c%init(2, 2.0d0 * r_max, (0.0d0, 0.0d0))
+        call setup_S(splines, k_GL, S)
+        do i = 0, max_l
+            call H_vec(i)%init(splines%n_b, splines%n_b)
+            call setup_H_one_particle(pot, CAP_c, i, splines, k_GL, H_vec(i)%data)
+        end do
+    else
+        open(newunit=u, file=trim(hs_file), access='stream', form='unformatted', status='old', action='read')
+        read(u) S
+        do i = 0, max_l
+            call H_vec(i)%init(splines%n_b, splines%n_b)
+            read(u) H_vec(i)%data
+        end do
+        close(u)
+    end if
 
     term%l = L
     term%m = 0

@@ -15,7 +15,7 @@ module bs2e_capi
     public :: bs2e_slater_primitives, bs2e_rk_create, bs2e_rk_from_primitives, bs2e_rk_compute
     public :: bs2e_rk_synchronize, bs2e_rk_info, bs2e_rk_copy_to_host, bs2e_rk_get_val, bs2e_rk_release
     public :: bs2e_ang_k_LS, bs2e_count_configs, bs2e_block_create, bs2e_block_sizes, bs2e_block_fill
-    public :: bs2e_block_release, c_string
+    public :: bs2e_block_release, c_string, bs2e_setup_S, bs2e_setup_H_one_particle
 
     interface
         function bs2e_create(ctx, device) bind(C, name="bs2e_create") result(rc)
@@ -63,6 +63,26 @@ module bs2e_capi
             real(c_double), intent(in) :: knots(*)
             real(c_double), intent(out) :: rk(*), rmk(*), rdk(*)
             integer(c_int64_t), intent(out) :: iv4(*), i4(*), j4(*), iv6(*), i6(*), ip6(*), j6(*), jp6(*)
+            integer(c_int64_t) :: rc
+        end function
+
+        function bs2e_setup_S(ctx, k, n_knots, knots, k_gl, S) bind(C, name="bs2e_setup_S") result(rc)
+            import :: c_ptr, c_int64_t, c_double, c_double_complex
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), value :: k, n_knots, k_gl
+            real(c_double), intent(in) :: knots(*)
+            complex(c_double_complex), intent(out) :: S(*)
+            integer(c_int64_t) :: rc
+        end function
+
+        function bs2e_setup_H_one_particle(ctx, k, n_knots, knots, k_gl, Z, l_min, l_max, cap_order, cap_r0, &
+                                           eta_re, eta_im, H) bind(C, name="bs2e_setup_H_one_particle") result(rc)
+            import :: c_ptr, c_int64_t, c_double, c_double_complex
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), value :: k, n_knots, k_gl, Z, l_min, l_max, cap_order
+            real(c_double), value :: cap_r0, eta_re, eta_im
+            real(c_double), intent(in) :: knots(*)
+            complex(c_double_complex), intent(out) :: H(*)
             integer(c_int64_t) :: rc
         end function
 

@@ -16,6 +16,10 @@
 !       k_spline, max_n_b, n_all_l, l_2_max, eigs)       orbital_tools.f90:10-19, 119-216
 !   block, CSR_matrix, construct_block_tensor(H, S,
 !       b_splines, term, max_k, R_k, H_sp, S_sp, full)   hamiltonian.f90:106-283
+!   hydrogenic (%init(Z)), CAP (%init(order, r_0, eta))  potentials.f90, CAP_tools.f90
+!   setup_S(b_splines, k_GL, S)                          mat_els.f90:85-118
+!   setup_H_one_particle(potential, CAP_c, l,
+!                        b_splines, k_GL, H)             mat_els.f90:47-83
 !
 ! plus compute_R_k_direct(b_splines, max_k, k_GL, R), which builds the tensor
 ! on the device straight from the knots without the host containers.
@@ -34,6 +38,7 @@ module bs2e_mat_els
     public :: b_spline, sparse_4d, sparse_6d, Nd_DOK, config, sym, block, CSR_matrix
     public :: generate_grid, setup_Slater_integrals, compute_R_k_map, compute_R_k_direct
     public :: count_configs, construct_block_tensor, ang_k_LS, bs2e_use_device, bs2e_shutdown
+    public :: hydrogenic, CAP, setup_S, setup_H_one_particle
 
     type(c_ptr), save :: g_ctx = c_null_ptr
     integer, save :: g_device = 0
@@ -98,6 +103,20 @@ module bs2e_mat_els
     contains
         procedure :: init => block_init
     end type block
+
+    type :: hydrogenic
+        integer :: Z = 1
+    contains
+        procedure :: init => init_hydrogenic
+    end type hydrogenic
+
+    type :: CAP
+        integer :: order = 2
+        real(dp) :: r_0 = 0.0_dp
+        complex(dp) :: eta = (0.0_dp, 0.0_dp)
+    contains
+        procedure :: init => init_CAP
+    end type CAP
 
     type :: CSR_matrix
         integer :: shape(2) = 0
@@ -183,6 +202,46 @@ contains
         allocate(this%data(n, m))
         this%data = (0.0_dp, 0.0_dp)
     end subroutine block_init
+
+    ! ---------------------------------------------------------------- one-electron
+    subroutine init_hydrogenic(this, Z)
+        class(hydrogenic), intent(inout) :: this
+        integer, intent(in) :: Z
+        this%Z = Z
+    end subroutine init_hydrogenic
+
+    subroutine init_CAP(this, order, r_0, eta)
+        class(CAP), intent(inout) :: this
+        integer, intent(in) :: order
+        real(dp), intent(in) :: r_0
+        complex(dp), intent(in) :: eta
+        this%order = order
+        this%r_0 = r_0
+        this%eta = eta
+    end subroutine init_CAP
+
+    ! mat_els.f90:85-118 on the device
+    subroutine setup_S(b_splines, k_GL, S)
+        type(b_spline), intent(in) :: b_splines
+        integer, intent(in) :: k_GL
+        complex(dp), intent(out) :: S(:,:)
+        call check(bs2e_setup_S(ctx(), int(b_splines%k, c_int64_t), int(size(b_splines%knots), c_int64_t), &
+                                b_splines%knots, int(k_GL, c_int64_t), S), "setup_S")
+    end subroutine setup_S
+
+    ! mat_els.f90:47-83 on the device (hydrogenic potential, CAP)
+    subroutine setup_H_one_particle(potential, CAP_c, l, b_splines, k_GL, H)
+        type(hydrogenic), intent(in) :: potential
+        type(CAP), intent(in) :: CAP_c
+        integer, intent(in) :: l
+        type(b_spline), intent(in) :: b_splines
+        integer, intent(in) :: k_GL
+        complex(dp), intent(out) :: H(:,:)
+        call check(bs2e_setup_H_one_particle(ctx(), int(b_splines%k, c_int64_t), int(size(b_splines%knots), c_int64_t), &
+                   b_splines%knots, int(k_GL, c_int64_t), int(potential%Z, c_int64_t), int(l, c_int64_t), &
+                   int(l, c_int64_t), int(CAP_c%order, c_int64_t), CAP_c%r_0, real(CAP_c%eta, dp), aimag(CAP_c%eta), H), &
+                   "setup_H_one_particle")
+    end subroutine setup_H_one_particle
 
     ! ---------------------------------------------------------------- containers
     subroutine sizes_of(b_splines, nnz4, nnz6, n_pairs)

@@ -73,6 +73,21 @@ int64_t bs2e_slater_primitives(bs2e_ctx *ctx, int64_t k, int64_t n_knots, const 
                                int64_t *j4, int64_t nnz6, double *rdk, int64_t *iv6, int64_t *i6, int64_t *ip6,
                                int64_t *j6, int64_t *jp6);
 
+/* ---- one-electron matrices -------------------------------------------- */
+/* mat_els.f90:85-118 setup_S(b_splines, k_GL, S): S(i,j) = sum over the
+ * intervals holding both splines and their GL points of w B_i B_j; S is
+ * n_b x n_b double complex, column-major (interleaved re/im), zero outside
+ * the band. */
+int64_t bs2e_setup_S(bs2e_ctx *ctx, int64_t k, int64_t n_knots, const double *knots, int64_t k_gl, double *S);
+/* mat_els.f90:47-83 setup_H_one_particle(potential, CAP_c, l, b_splines,
+ * k_GL, H) for the hydrogenic potential (potentials.f90 V_hydrogenic(Z):
+ * 0.5 l(l+1)/r^2 - Z/r) and the absorbing potential of CAP_tools.f90:24-34
+ * (-i eta (r - r_0)^order for r >= r_0), for every l in [l_min, l_max]:
+ * H holds l_max-l_min+1 matrices of n_b x n_b double complex, column-major. */
+int64_t bs2e_setup_H_one_particle(bs2e_ctx *ctx, int64_t k, int64_t n_knots, const double *knots, int64_t k_gl,
+                                  int64_t Z, int64_t l_min, int64_t l_max, int64_t cap_order, double cap_r0,
+                                  double cap_eta_re, double cap_eta_im, double *H);
+
 /* ---- the R^k tensor ---------------------------------------------------- */
 /* sparse_array_tools.f90:452-493 compute_R_K_map(r_d_k, r_k, r_m_k,
  * b_splines, max_k, R) and :495-507 Nd_DOK%init.  Plans the basis and

@@ -36,8 +36,8 @@ def max_rel(a, b):
     return float(np.max(np.abs(a - b) / np.abs(b)))
 
 
-@pytest.mark.parametrize("L,pi,full", [(0, 0, 1), (1, 1, 0)])
-def test_fortran_basis_setup_matches_oracle(oracle, tmp_path, L, pi, full):
+@pytest.mark.parametrize("L,pi,full,hs_on_device", [(0, 0, 1, False), (1, 1, 0, False), (0, 0, 1, True)])
+def test_fortran_basis_setup_matches_oracle(oracle, tmp_path, L, pi, full, hs_on_device):
     if not os.path.exists(EXE):
         pytest.fail(f"{EXE} is missing: build with `python b-spline-two-e_amd/build.py` (needs amdflang)")
     w = WORKLOADS["c1"]
@@ -52,7 +52,7 @@ def test_fortran_basis_setup_matches_oracle(oracle, tmp_path, L, pi, full):
             f.write(np.asfortranarray(M, dtype=np.complex128).tobytes(order="F"))
     out = tmp_path / "out.bin"
     cmd = [EXE, str(w.k), str(w.m), str(w.Z), repr(w.h_max), repr(w.r_max), str(w.k_GL), str(w.max_k), str(max_l),
-           str(L), str(pi), str(full), str(stride), str(hs), str(out)]
+           str(L), str(pi), str(full), str(stride), "device" if hs_on_device else str(hs), str(out)]
     print("run:", " ".join(cmd), flush=True)
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
     print(r.stdout)
@@ -102,5 +102,5 @@ def test_fortran_basis_setup_matches_oracle(oracle, tmp_path, L, pi, full):
         d = s.take(2 * nnz, np.float64).view(np.complex128)
         rip, rix, rd = blk[name]
         assert np.array_equal(ip, rip) and np.array_equal(ix, rix), name
-        assert np.abs(d - rd).max() <= TOL * np.abs(rd).max(), name
+        assert np.abs(d - rd).max() <= TOL * np.abs(rd).max(), name   # one-electron inputs from the device too
     assert s.pos == len(s.raw)
