@@ -5,6 +5,12 @@
 ! of one symmetry.
 !
 ! usage: bs2e_basis_setup k m Z h_max r_max k_GL max_k max_l L pi full key_stride hs_file out_file
+!                         [max_L res_dir]
+!
+! With max_L and res_dir, it also builds the blocks of every symmetry of the
+! z-polarised two-electron basis (L = 0..max_L) and stores what the
+! reference's basis_setup stores for them (main_basis_setup.f90:154-161):
+! res_dir/H_diag.dat, S_diag.dat, basis.dat, basis_info.txt, splines.dat.
 !
 ! hs_file (stream): S then H(l) for l = 0..max_l, each n_b x n_b complex(dp)
 ! column-major (the one-electron matrices, mat_els.f90:85-118 and :294-321);
@@ -38,7 +44,10 @@ program bs2e_basis_setup
     type(hydrogenic) :: pot
     type(CAP) :: CAP_c
     integer, allocatable :: keys(:,:)
-    integer :: u, i, a, b, c, d, n_keys, p, q, n_pairs
+    integer :: u, i, a, b, c, d, n_keys, p, q, n_pairs, max_L_all
+    character(len=4096) :: res_dir
+    type(basis) :: bas
+    type(block_diag_CS) :: H_diag, S_diag
 
     call get_command_argument(1, arg); read(arg, *) k
     call get_command_argument(2, arg); read(arg, *) m
@@ -137,6 +146,23 @@ program bs2e_basis_setup
     write(u) H_sp%nnz, H_sp%index_ptr, H_sp%indices, H_sp%data
     write(u) S_sp%nnz, S_sp%index_ptr, S_sp%indices, S_sp%data
     close(u)
+
+    if (command_argument_count() >= 16) then
+        call get_command_argument(15, arg); read(arg, *) max_L_all
+        call get_command_argument(16, res_dir)
+        call bas%init(max_L_all, max_l, splines%n_b, splines%k, splines%n_b, splines%n_b, max_l, .true., .true.)
+        call H_diag%init(bas%n_sym)
+        call S_diag%init(bas%n_sym)
+        do i = 1, bas%n_sym
+            call construct_block_tensor(H_vec, S, splines, bas%syms(i), max_k, R_p, H_diag%blocks(i), &
+                                        S_diag%blocks(i), ifull /= 0)
+        end do
+        call H_diag%store(trim(res_dir) // "H_diag.dat")
+        call S_diag%store(trim(res_dir) // "S_diag.dat")
+        call bas%store(trim(res_dir))
+        call splines%store(trim(res_dir))
+        write(6, '(a,i0,a)') "stored ", bas%n_sym, " symmetry blocks in " // trim(res_dir)
+    end if
 
     call R_p%release()
     call R_direct%release()

@@ -20,6 +20,11 @@
 !   setup_S(b_splines, k_GL, S)                          mat_els.f90:85-118
 !   setup_H_one_particle(potential, CAP_c, l,
 !                        b_splines, k_GL, H)             mat_els.f90:47-83
+!   basis (%init, %compute_size, %store)                 orbital_tools.f90:21-34, 263-397
+!   block_diag_CS (%init, %compute_shape, %store,
+!                  %load), CSR_matrix%store              block_tools.f90:41-151, 458-524,
+!                                                        sparse_array_tools.f90:695-742
+!   b_spline%store(dir)                                  bspline_tools.f90:375-387
 !
 ! plus compute_R_k_direct(b_splines, max_k, k_GL, R), which builds the tensor
 ! on the device straight from the knots without the host containers.
@@ -38,7 +43,7 @@ module bs2e_mat_els
     public :: b_spline, sparse_4d, sparse_6d, Nd_DOK, config, sym, block, CSR_matrix
     public :: generate_grid, setup_Slater_integrals, compute_R_k_map, compute_R_k_direct
     public :: count_configs, construct_block_tensor, ang_k_LS, bs2e_use_device, bs2e_shutdown
-    public :: hydrogenic, CAP, setup_S, setup_H_one_particle
+    public :: hydrogenic, CAP, setup_S, setup_H_one_particle, basis, block_diag_CS
 
     type(c_ptr), save :: g_ctx = c_null_ptr
     integer, save :: g_device = 0
@@ -52,6 +57,7 @@ module bs2e_mat_els
     contains
         procedure :: init => b_spline_init
         procedure :: find_max_n_b => b_spline_find_max_n_b
+        procedure :: store => b_spline_store
     end type b_spline
 
     type :: sparse_4d
@@ -124,7 +130,34 @@ module bs2e_mat_els
         integer, allocatable :: index_ptr(:)
         integer, allocatable :: indices(:)
         complex(dp), allocatable :: data(:)
+    contains
+        procedure :: store => CSR_store
     end type CSR_matrix
+
+    type :: block_diag_CS
+        integer :: block_shape(2) = 0
+        integer :: shape(2) = 0
+        type(CSR_matrix), allocatable :: blocks(:)
+    contains
+        procedure :: init => init_block_diag_CS
+        procedure :: compute_shape => compute_shape_block_diag_CS
+        procedure :: store => CS_block_diag_store
+        procedure :: load => CS_block_diag_load
+    end type block_diag_CS
+
+    type :: basis
+        integer :: max_l_1p = 0
+        integer :: max_L = 0
+        logical :: two_el = .true.
+        integer :: n_sym = 0
+        integer :: n_states = 0
+        type(sym), allocatable :: syms(:)
+        integer, allocatable :: sym_ptr(:)
+    contains
+        procedure :: init => init_basis
+        procedure :: compute_size => compute_basis_size
+        procedure :: store => store_basis
+    end type basis
 
 contains
 
@@ -202,6 +235,190 @@ contains
         allocate(this%data(n, m))
         this%data = (0.0_dp, 0.0_dp)
     end subroutine block_init
+
+    ! ---------------------------------------------------------------- files
+    ! Fortran unformatted sequential records, as the reference writes them.
+    subroutine b_spline_store(this, dir)
+        class(b_spline), intent(in) :: this
+        character(len=*), intent(in) :: dir
+        integer :: u
+        open(file=dir // "splines.dat", newunit=u, action='write', form='unformatted')
+        write(u) this%k
+        write(u) size(this%knots)
+        write(u) this%knots
+        close(u)
+    end subroutine b_spline_store
+
+    subroutine CSR_store(this, loc)
+        class(CSR_matrix), intent(in) :: this
+        character(len=*), intent(in) :: loc
+        integer :: u
+        open(file=loc, newunit=u, action='write', form='unformatted')
+        write(u) 'CSR'
+        write(u) this%shape
+        write(u) this%nnz
+        write(u) this%index_ptr
+        write(u) this%indices
+        write(u) this%data
+        close(u)
+    end subroutine CSR_store
+
+    subroutine init_block_diag_CS(this, n)
+        class(block_diag_CS), intent(inout) :: this
+        integer, intent(in) :: n
+        this%block_shape = [n, n]
+        if (allocated(this%blocks)) deallocate(this%blocks)
+        allocate(this%blocks(n))
+    end subroutine init_block_diag_CS
+
+    subroutine compute_shape_block_diag_CS(this)
+        class(block_diag_CS), intent(inout) :: this
+        integer :: i
+        this%shape = 0
+        do i = 1, this%block_shape(1)
+            this%shape = this%shape + this%blocks(i)%shape
+        end do
+    end subroutine compute_shape_block_diag_CS
+
+    subroutine CS_block_diag_store(this, loc)
+        class(block_diag_CS), intent(inout) :: this
+        character(len=*), intent(in) :: loc
+        integer :: i, u
+        call this%compute_shape()
+        open(file=loc, newunit=u, action='write', form='unformatted')
+        write(u) 'CSR'
+        write(u) this%block_shape
+        write(u) this%shape
+        do i = 1, this%block_shape(1)
+            write(u) this%blocks(i)%shape
+            write(u) this%blocks(i)%nnz
+            if (this%blocks(i)%nnz > 0) then
+                write(u) this%blocks(i)%index_ptr
+                write(u) this%blocks(i)%indices
+                write(u) this%blocks(i)%data
+            end if
+        end do
+        close(u)
+    end subroutine CS_block_diag_store
+
+    subroutine CS_block_diag_load(this, loc)
+        class(block_diag_CS), intent(inout) :: this
+        character(len=*), intent(in) :: loc
+        integer :: i, u, nnz, shape(2), block_shape(2)
+        character(len=3) :: kind
+        open(file=loc, newunit=u, action='read', form='unformatted', status='old')
+        read(u) kind
+        if (kind /= 'CSR') error stop "block_diag_CS%load: blocks must be CSR"
+        read(u) block_shape
+        call this%init(block_shape(1))
+        read(u) this%shape
+        do i = 1, block_shape(1)
+            read(u) shape
+            read(u) nnz
+            this%blocks(i)%shape = shape
+            this%blocks(i)%nnz = nnz
+            allocate(this%blocks(i)%index_ptr(shape(1) + 1), this%blocks(i)%indices(nnz), this%blocks(i)%data(nnz))
+            this%blocks(i)%index_ptr = 1
+            if (nnz > 0) then
+                read(u) this%blocks(i)%index_ptr
+                read(u) this%blocks(i)%indices
+                read(u) this%blocks(i)%data
+            end if
+        end do
+        close(u)
+    end subroutine CS_block_diag_load
+
+    ! orbital_tools.f90:263-343 init_basis, two-electron symmetries: (0,0,even),
+    ! then per L either (L, 0, parity of L) (z_pol) or every (L, M, parity)
+    ! with |M| even for even parity and odd for odd parity.
+    subroutine init_basis(this, max_L, max_l_1p, n_b, k_spline, max_n_b, n_all_l, l_2_max, z_pol, two_el)
+        class(basis), intent(inout) :: this
+        integer, intent(in) :: max_L, max_l_1p, n_b, k_spline, max_n_b, n_all_l, l_2_max
+        logical, intent(in) :: z_pol, two_el
+        integer :: l, m, p, ptr
+        if (.not. two_el) error stop "basis%init: only the two-electron basis is built here"
+        this%max_l_1p = max_l_1p
+        this%max_L = max_L
+        this%two_el = two_el
+        if (z_pol) then
+            this%n_sym = max_L + 1
+        else
+            this%n_sym = (max_L + 1)**2
+        end if
+        if (allocated(this%syms)) deallocate(this%syms)
+        allocate(this%syms(this%n_sym))
+        this%syms(1)%l = 0
+        this%syms(1)%m = 0
+        this%syms(1)%pi = .false.
+        ptr = 2
+        do l = 1, max_L
+            if (z_pol) then
+                this%syms(ptr)%l = l
+                this%syms(ptr)%m = 0
+                this%syms(ptr)%pi = mod(l, 2) /= 0
+                ptr = ptr + 1
+            else
+                do p = 0, 1
+                    do m = -l, l
+                        if (abs(mod(m, 2)) /= p) cycle
+                        this%syms(ptr)%l = l
+                        this%syms(ptr)%m = m
+                        this%syms(ptr)%pi = p == 1
+                        ptr = ptr + 1
+                    end do
+                end do
+            end if
+        end do
+        do ptr = 1, this%n_sym
+            this%syms(ptr)%configs = count_configs(this%syms(ptr), max_l_1p, n_b, k_spline, max_n_b, n_all_l, l_2_max)
+            this%syms(ptr)%n_config = size(this%syms(ptr)%configs)
+        end do
+    end subroutine init_basis
+
+    subroutine compute_basis_size(this)
+        class(basis), intent(inout) :: this
+        integer :: i, ptr
+        this%n_states = 0
+        do i = 1, this%n_sym
+            this%n_states = this%n_states + this%syms(i)%n_config
+        end do
+        if (allocated(this%sym_ptr)) deallocate(this%sym_ptr)
+        allocate(this%sym_ptr(this%n_sym + 1))
+        ptr = 1
+        do i = 1, this%n_sym
+            this%sym_ptr(i) = ptr
+            ptr = ptr + this%syms(i)%n_config
+        end do
+        this%sym_ptr(this%n_sym + 1) = ptr
+    end subroutine compute_basis_size
+
+    subroutine store_basis(this, loc)
+        class(basis), intent(inout) :: this
+        character(len=*), intent(in) :: loc
+        integer :: i, u
+        call this%compute_size()
+        open(file=loc // "basis.dat", newunit=u, action='write', form='unformatted')
+        write(u) this%max_l_1p
+        write(u) this%max_L
+        write(u) this%two_el
+        write(u) this%n_sym
+        write(u) this%n_states
+        write(u) this%sym_ptr
+        do i = 1, this%n_sym
+            write(u) this%syms(i)%l
+            write(u) this%syms(i)%m
+            write(u) this%syms(i)%pi
+            write(u) this%syms(i)%n_config
+            write(u) this%syms(i)%configs
+        end do
+        close(u)
+        open(file=loc // "basis_info.txt", newunit=u, action='write')
+        write(u, "(a5,a5,a5,a5,a9)") "# Sym", "L", "M", "Pi", "states"
+        do i = 1, this%n_sym
+            write(u, "(i5,i5,i5,l5,i9)") i, this%syms(i)%l, this%syms(i)%m, this%syms(i)%pi, this%syms(i)%n_config
+        end do
+        close(u)
+    end subroutine store_basis
 
     ! ---------------------------------------------------------------- one-electron
     subroutine init_hydrogenic(this, Z)

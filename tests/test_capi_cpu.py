@@ -104,3 +104,34 @@ def test_error_paths_without_device():
     with pytest.raises(capi.Bs2eError):
         import bs2e
         bs2e.Context(0)
+
+
+def test_fortran_records_round_trip(tmp_path, monkeypatch):
+    """Unformatted sequential records, including records split into
+    subrecords (forced here by a small subrecord limit)."""
+    from bs2e import CSR_matrix
+    from bs2e import fortran_io as fio
+    rng = np.random.default_rng(3)
+    blocks = []
+    for n in (3, 5, 0, 4):
+        nnz = n * 2
+        ip = np.concatenate([[1], 1 + np.sort(rng.integers(0, nnz + 1, n - 1)), [nnz + 1]]) if n else np.ones(1, int)
+        if n:
+            ip[-1] = nnz + 1
+        blocks.append(CSR_matrix((n, n), np.asarray(ip, np.int64), rng.integers(1, n + 1, nnz).astype(np.int64) if n
+                                 else np.zeros(0, np.int64), rng.normal(size=nnz) + 1j * rng.normal(size=nnz)))
+    for limit in (fio.MAX_SUB, 24):
+        monkeypatch.setattr(fio, "MAX_SUB", limit)
+        p = tmp_path / f"b{limit}.dat"
+        fio.store_block_diag(p, blocks)
+        got = fio.load_block_diag(p)
+        assert len(got) == len(blocks)
+        for a, b in zip(got, blocks):
+            assert a.shape == b.shape and a.nnz == b.nnz
+            if b.nnz:
+                assert np.array_equal(a.index_ptr, b.index_ptr) and np.array_equal(a.indices, b.indices)
+                assert np.array_equal(a.data, b.data)
+    raw = (tmp_path / "b24.dat").read_bytes()
+    # first record 'CSR': 3 bytes in one subrecord; a 5x2 complex data record (160 bytes) is split
+    assert raw[:4] == (3).to_bytes(4, "little") and raw[7:11] == (3).to_bytes(4, "little")
+    assert (-24).to_bytes(4, "little", signed=True) in raw

@@ -104,3 +104,55 @@ def test_fortran_basis_setup_matches_oracle(oracle, tmp_path, L, pi, full, hs_on
         assert np.array_equal(ip, rip) and np.array_equal(ix, rix), name
         assert np.abs(d - rd).max() <= TOL * np.abs(rd).max(), name   # one-electron inputs from the device too
     assert s.pos == len(s.raw)
+
+
+def test_fortran_basis_files_match_reference_format(oracle, tmp_path):
+    """The driver stores H_diag.dat, S_diag.dat, basis.dat, basis_info.txt and
+    splines.dat for every symmetry L = 0..2 (as main_basis_setup.f90:154-161
+    does); their contents equal the oracle's blocks, and the Python writer of
+    the same format reproduces the amdflang-written files byte for byte."""
+    from bs2e import fortran_io as fio
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} is missing")
+    w = WORKLOADS["c1"]
+    max_l, max_L = 2, 2
+    t = oracle.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+    _, n_b, _ = oracle.dims(w.k, t)
+    res = str(tmp_path) + "/"
+    cmd = [EXE, str(w.k), str(w.m), str(w.Z), repr(w.h_max), repr(w.r_max), str(w.k_GL), str(w.max_k), str(max_l),
+           "0", "0", "1", "50", "device", str(tmp_path / "out.bin"), str(max_L), res]
+    print("run:", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=180)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout
+    k, knots = fio.load_splines(res)
+    assert k == w.k and np.array_equal(knots, t)
+    max_l_1p, mL, two_el, syms = fio.load_basis(res)
+    assert (max_l_1p, mL, two_el, len(syms)) == (max_l, max_L, True, max_L + 1)
+    H = fio.load_block_diag(res + "H_diag.dat")
+    S = fio.load_block_diag(res + "S_diag.dat")
+    off = oracle.slater_off_diag(w.k, t, w.k_GL, w.max_k)
+    dg = oracle.slater_diag(w.k, t, w.k_GL, w.max_k)
+    R = oracle.rk_dense(w.k, n_b, w.max_k, off, dg)
+    Sm = oracle.setup_S(w.k, t, w.k_GL)
+    Hl = [oracle.setup_H1p(w.k, t, w.k_GL, w.Z, l, cap_eta=0.0) for l in range(max_l + 1)]
+    for L in range(max_L + 1):
+        s = syms[L]
+        assert (s.l, s.m, s.pi) == (L, 0, bool(L % 2))
+        cf = oracle.count_configs(L, L % 2, max_l, n_b, w.k, n_b, n_b, max_l)
+        assert np.array_equal(s.configs, cf)
+        ref = oracle.construct_block(w.k, n_b, L, w.max_k, True, cf, R, Hl, Sm)
+        for got, key in ((H[L], "H"), (S[L], "S")):
+            rip, rix, rd = ref[key]
+            assert got.shape == (len(cf), len(cf))
+            assert np.array_equal(got.index_ptr, rip) and np.array_equal(got.indices, rix), (L, key)
+            assert np.abs(got.data - rd).max() <= TOL * np.abs(rd).max(), (L, key)
+    # the Python writer reproduces the files byte for byte
+    py = tmp_path / "py"
+    py.mkdir()
+    fio.store_block_diag(py / "H_diag.dat", H)
+    fio.store_block_diag(py / "S_diag.dat", S)
+    fio.store_basis(str(py), max_l_1p, mL, two_el, syms)
+    fio.store_splines(str(py), k, knots)
+    for name in ("H_diag.dat", "S_diag.dat", "basis.dat", "basis_info.txt", "splines.dat"):
+        assert (py / name).read_bytes() == (tmp_path / name).read_bytes(), name
