@@ -505,6 +505,12 @@ class Block:
         S_sp = CSR_matrix((n, n), ipS, ixS[:self.nnz_S], dS[:self.nnz_S])
         return H_sp, S_sp
 
+    def kernel_ms(self):
+        """(count kernel, fill kernel) device ms, HIP events."""
+        ms = np.zeros(2)
+        check(capi.lib().bs2e_block_times(self.handle, ptr(ms), 2), self.ctx.handle)
+        return float(ms[0]), float(ms[1])
+
     def release(self):
         if self.handle:
             capi.lib().bs2e_block_release(self.handle)

@@ -58,6 +58,7 @@ SIGNATURES = [
     ("bs2e_block_sizes", i64, [vp, P(i64), i64]),
     ("bs2e_block_fill", i64, [vp, i64, P(dbl), P(dbl), P(i64), P(i64), P(dbl), P(i64), P(i64), P(dbl)]),
     ("bs2e_block_release", i64, [vp]),
+    ("bs2e_block_times", i64, [vp, P(dbl), i64]),
     ("bs2e_rk_set_graph", i64, [vp, i64]),
     ("bs2e_setup_S", i64, [vp, i64, i64, P(dbl), i64, P(dbl)]),
     ("bs2e_setup_H_one_particle", i64, [vp, i64, i64, P(dbl), i64, i64, i64, i64, i64, dbl, dbl, dbl, P(dbl)]),

@@ -54,6 +54,7 @@ int64_t count_configs(int64_t L, int64_t term_pi, int64_t max_l_1p, int64_t n_b,
 struct bs2e_block {
     bs2e_rk* rk = nullptr;
     int64_t L = 0, n_config = 0, full = 1, max_l = 0;
+    float ms_count = 0.f, ms_fill = 0.f;   // device time of the count and fill kernels (HIP events)
     std::vector<BlockChan> chan;
     std::vector<BlockSeg> seg;
     std::vector<int32_t> cfg_chan, flags;
@@ -232,10 +233,20 @@ int64_t bs2e_block_create(bs2e_rk* rk, int64_t L, int64_t n_config, const int64_
         a.cntS = b->d_cnt + n_config;
         a.refH = b->d_cnt + 2 * n_config;
         a.refS = b->d_cnt + 3 * n_config;
-        HIP_TRY(ctx, launch_block_count(a, s));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIP_TRY(ctx, hipEventCreate(&e0));
+        HIP_TRY(ctx, hipEventCreate(&e1));
+        hipError_t le = hipEventRecord(e0, s);
+        if (le == hipSuccess) le = launch_block_count(a, s);
+        if (le == hipSuccess) le = hipEventRecord(e1, s);
         std::vector<int64_t> cnt((size_t)4 * n_config);
-        HIP_TRY(ctx, hipMemcpyAsync(cnt.data(), b->d_cnt, sizeof(int64_t) * 4 * n_config, hipMemcpyDeviceToHost, s));
-        HIP_TRY(ctx, hipStreamSynchronize(s));
+        if (le == hipSuccess)
+            le = hipMemcpyAsync(cnt.data(), b->d_cnt, sizeof(int64_t) * 4 * n_config, hipMemcpyDeviceToHost, s);
+        if (le == hipSuccess) le = hipStreamSynchronize(s);
+        if (le == hipSuccess) le = hipEventElapsedTime(&b->ms_count, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        HIP_TRY(ctx, le);
         b->cntH.assign(cnt.begin(), cnt.begin() + n_config);
         b->cntS.assign(cnt.begin() + n_config, cnt.begin() + 2 * n_config);
         b->ptrH.resize(n_config);
@@ -262,6 +273,14 @@ int64_t bs2e_block_create(bs2e_rk* rk, int64_t L, int64_t n_config, const int64_
     }
     rk->blocks += 1;
     *out = b;
+    return BS2E_OK;
+}
+
+int64_t bs2e_block_times(const bs2e_block* b, double* ms, int64_t n)
+{
+    if (!b || !ms) return fail(nullptr, BS2E_EINVAL, "bs2e_block_times: NULL argument");
+    const double v[2] = {b->ms_count, b->ms_fill};
+    for (int64_t i = 0; i < n && i < 2; ++i) ms[i] = v[i];
     return BS2E_OK;
 }
 
@@ -314,7 +333,17 @@ int64_t bs2e_block_fill(bs2e_block* b, int64_t max_l_1p, const double* Hl, const
         a.dH = ddH;
         a.ixS = dixS;
         a.dS = ddS;
-        HIP_TRY(ctx, launch_block_fill(a, s));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIP_TRY(ctx, hipEventCreate(&e0));
+        HIP_TRY(ctx, hipEventCreate(&e1));
+        hipError_t le = hipEventRecord(e0, s);
+        if (le == hipSuccess) le = launch_block_fill(a, s);
+        if (le == hipSuccess) le = hipEventRecord(e1, s);
+        if (le == hipSuccess) le = hipEventSynchronize(e1);
+        if (le == hipSuccess) le = hipEventElapsedTime(&b->ms_fill, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        HIP_TRY(ctx, le);
         HIP_TRY(ctx, hipMemcpyAsync(ix_H, dixH, sizeof(int64_t) * b->nnzH, hipMemcpyDeviceToHost, s));
         HIP_TRY(ctx, hipMemcpyAsync(d_H, ddH, sizeof(double) * 2 * b->nnzH, hipMemcpyDeviceToHost, s));
         HIP_TRY(ctx, hipMemcpyAsync(ix_S, dixS, sizeof(int64_t) * b->nnzS, hipMemcpyDeviceToHost, s));

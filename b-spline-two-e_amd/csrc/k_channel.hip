@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kCThreads) void kc_block(BlockArgs A)
         const double* cd = A.coef + ((size_t)ch_i * A.n_chan + ch) * 2 * K;
         const double* ce = cd + K;
         const BlockSeg* sg = A.seg + (size_t)ch * (A.n_b + 1);
-        // n_c ranges: within k of n_a (direct) or of n_b (exchange), ascending
+        // n_c candidates: within k of n_a (direct) or of n_b (exchange), ascending
         int r1lo = n_a - ks + 1, r1hi = n_a + ks - 1, r2lo = n_bb - ks + 1, r2hi = n_bb + ks - 1;
         if (r2lo < r1lo) {
             int t0 = r1lo, t1 = r1hi;
@@ -94,16 +94,22 @@ __global__ __launch_bounds__(kCThreads) void kc_block(BlockArgs A)
             r2lo = 1;
             r2hi = 0;
         }
-        for (int part = 0; part < 2; ++part) {
-            int clo = part ? r2lo : r1lo, chi = part ? r2hi : r1hi;
-            if (clo < 1) clo = 1;
-            if (chi > A.n_b) chi = A.n_b;
-            for (int n_c = clo; n_c <= chi; ++n_c) {
-                const BlockSeg s = sg[n_c];
-                if (s.hi < s.lo) continue;
-                const bool in_dir_c = iabs_d(n_c - n_a) < ks;
-                const bool in_ex_c = iabs_d(n_c - n_bb) < ks;
-                // n_d segments: direct needs |n_b-n_d| < k, exchange |n_a-n_d| < k
+        r1lo = r1lo < 1 ? 1 : r1lo;
+        r1hi = r1hi > A.n_b ? A.n_b : r1hi;
+        r2lo = r2lo < 1 ? 1 : r2lo;
+        r2hi = r2hi > A.n_b ? A.n_b : r2hi;
+        const int n1 = r1hi >= r1lo ? r1hi - r1lo + 1 : 0, n2 = r2hi >= r2lo ? r2hi - r2lo + 1 : 0;
+        const int ncand = n1 + n2;    // <= 2 (2k - 1) <= 62 for k <= 16: one per lane
+        // lane l takes the l-th candidate n_c and its n_d runs: direct needs
+        // |n_b - n_d| < k, exchange |n_a - n_d| < k, both clipped to the
+        // channel's n2 range for that n1 = n_c
+        int my_c = 0, d1lo = 1, d1hi = 0, d2lo = 1, d2hi = 0, my_soff = 0, my_slo = 0;
+        if (lane < ncand) {
+            my_c = lane < n1 ? r1lo + lane : r2lo + (lane - n1);
+            const BlockSeg sgc = sg[my_c];
+            if (sgc.hi >= sgc.lo) {
+                const bool in_dir_c = iabs_d(my_c - n_a) < ks;
+                const bool in_ex_c = iabs_d(my_c - n_bb) < ks;
                 int s1lo = 1, s1hi = 0, s2lo = 1, s2hi = 0;
                 if (in_dir_c) {
                     s1lo = n_bb - ks + 1;
@@ -132,14 +138,45 @@ __global__ __launch_bounds__(kCThreads) void kc_block(BlockArgs A)
                         s2hi = 0;
                     }
                 }
-                for (int dseg = 0; dseg < 2; ++dseg) {
-                    int dlo = dseg ? s2lo : s1lo, dhi = dseg ? s2hi : s1hi;
-                    if (dlo < s.lo) dlo = s.lo;
-                    if (dhi > s.hi) dhi = s.hi;
-                    for (int base = dlo; base <= dhi; base += kWave) {
-                        const int n_d = base + lane;
-                        bool active = n_d <= dhi;
-                        const int j = cj.start + s.off + (n_d - s.lo);
+                d1lo = s1lo < sgc.lo ? sgc.lo : s1lo;
+                d1hi = s1hi > sgc.hi ? sgc.hi : s1hi;
+                d2lo = s2lo < sgc.lo ? sgc.lo : s2lo;
+                d2hi = s2hi > sgc.hi ? sgc.hi : s2hi;
+                my_soff = sgc.off;
+                my_slo = sgc.lo;
+            }
+        }
+        const int len1 = d1hi >= d1lo ? d1hi - d1lo + 1 : 0, len2 = d2hi >= d2lo ? d2hi - d2lo + 1 : 0;
+        // exclusive prefix over the lanes of the candidates' run lengths
+        int incl = len1 + len2;
+        for (int off = 1; off < kWave; off <<= 1) {
+            const int y = __shfl_up(incl, off, kWave);
+            if (lane >= off) incl += y;
+        }
+        const int total = __shfl(incl, kWave - 1, kWave);
+        const int excl = incl - len1 - len2;
+        // candidates flattened in (n_c, n_d) order, which is ascending column order
+        for (int f0 = 0; f0 < total; f0 += kWave) {
+            const int f = f0 + lane;
+            int pos = 0;   // last candidate lane whose run starts at or before f
+            for (int step = kWave / 2; step > 0; step >>= 1) {
+                const int cand = pos + step;
+                const int e = __shfl(excl, cand < kWave ? cand : kWave - 1, kWave);
+                if (cand < ncand && e <= f) pos = cand;
+            }
+            const int n_c = __shfl(my_c, pos, kWave);
+            const int o_lo1 = __shfl(d1lo, pos, kWave), o_len1 = __shfl(len1, pos, kWave);
+            const int o_lo2 = __shfl(d2lo, pos, kWave), o_ex = __shfl(excl, pos, kWave);
+            const int s_off = __shfl(my_soff, pos, kWave), s_lo = __shfl(my_slo, pos, kWave);
+            const int local = f - o_ex;
+            const int n_d = local < o_len1 ? o_lo1 + local : o_lo2 + (local - o_len1);
+            const bool in_dir_c = iabs_d(n_c - n_a) < ks;
+            const bool in_ex_c = iabs_d(n_c - n_bb) < ks;
+            {
+                {
+                    {
+                        bool active = f < total;
+                        const int j = cj.start + s_off + (n_d - s_lo);
                         if (!A.full && j < i) active = false;
                         const bool support = in_dir_c && iabs_d(n_bb - n_d) < ks;
                         const bool support_ex = in_ex_c && iabs_d(n_a - n_d) < ks;
@@ -153,16 +190,20 @@ __global__ __launch_bounds__(kCThreads) void kc_block(BlockArgs A)
                                 Cplx h{0.0, 0.0};
                                 if (r12) {
                                     double res = 0.0;
+                                    // R(a,b,c,d) = R(a,d,c,b): the key [n_a, n_d, n_c, n_b] puts the
+                                    // wavefront's consecutive n_d on consecutive columns of one row
                                     if (support) {
                                         const double* v = A.R + ((size_t)pair_of(A.c_first, ks, n_a, n_c) * A.n_pairs +
-                                                                 pair_of(A.c_first, ks, n_bb, n_d)) * K;
+                                                                 pair_of(A.c_first, ks, n_d, n_bb)) * K;
                                         double t = 0.0;
                                         for (int k = 0; k < K; ++k) t = __dadd_rn(t, __dmul_rn(v[k], cd[k]));
                                         res = __dadd_rn(res, t);
                                     }
+                                    // exchange R(a,b,d,c) = R(b,a,c,d) = R(b,d,c,a): row pair(n_b, n_c),
+                                    // consecutive n_d on consecutive columns
                                     if (support_ex) {
-                                        const double* v = A.R + ((size_t)pair_of(A.c_first, ks, n_a, n_d) * A.n_pairs +
-                                                                 pair_of(A.c_first, ks, n_bb, n_c)) * K;
+                                        const double* v = A.R + ((size_t)pair_of(A.c_first, ks, n_bb, n_c) * A.n_pairs +
+                                                                 pair_of(A.c_first, ks, n_d, n_a)) * K;
                                         double t = 0.0;
                                         for (int k = 0; k < K; ++k) t = __dadd_rn(t, __dmul_rn(v[k], ce[k]));
                                         res = __dadd_rn(res, t);

@@ -145,6 +145,30 @@ def serving_benchmark(n_jobs, seed=20261202, slots=4, budget_gib=64, load=0.8):
                            "mean": float(lat.mean()), "max": float(lat.max())}}
 
 
+def blocks_benchmark(rk, w, t, max_l=2, L=0):
+    """Channel summation into the CSR H/S block of one symmetry (L, even
+    parity, l1, l2 <= max_l) from the built tensor; kernel times from the
+    library's HIP events, wall time including the host transfers."""
+    import bs2e
+    spl = bs2e.b_spline(w.k, t)
+    S = bs2e.setup_S(spl, w.k_GL, ctx=rk.ctx)
+    Hl = bs2e.setup_H_one_particle(w.Z, (2, 0.8 * w.r_max, 0.0), range(max_l + 1), spl, w.k_GL, ctx=rk.ctx)
+    term = bs2e.sym(l=L, pi=bool(L % 2))
+    term.configs = bs2e.count_configs(term, max_l, spl.n_b, w.k, spl.n_b, spl.n_b, max_l)
+    bs2e.Block(rk, term).release()                 # warm-up
+    t0 = time.perf_counter()
+    blk = bs2e.Block(rk, term)
+    H_sp, S_sp = blk.fill(list(Hl), S)
+    wall = time.perf_counter() - t0
+    ms_count, ms_fill = blk.kernel_ms()
+    blk.release()
+    written = 24.0 * (H_sp.nnz + S_sp.nnz)          # 1-based column index + complex value per entry
+    return {"workload": f"{w.name} basis, L={L} block, l1,l2 <= {max_l}", "n_config": term.n_config,
+            "nnz_H": H_sp.nnz, "nnz_S": S_sp.nnz, "count_kernel_ms": ms_count, "fill_kernel_ms": ms_fill,
+            "H_entries_per_s": H_sp.nnz / (ms_fill * 1e-3), "fill_GBs": written / (ms_fill * 1e-3) / 1e9,
+            "wall_ms_with_transfers": wall * 1e3}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +178,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--latency-samples", type=int, default=50,
                     help="synchronised single builds timed for p50/p99 latency")
+    ap.add_argument("--blocks", type=int, default=1, help="also time the L=0 symmetry block (0 skips it)")
     ap.add_argument("--serve-jobs", type=int, default=256,
                     help="jobs of the serving stream (differing bases through the build service); 0 skips it")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -278,6 +303,8 @@ def main():
                      "algorithmic_bytes": traffic_alg, "elements": far_elems,
                      "avg_launch_ms": t_far * 1e3},
     }
+    if args.blocks:
+        out["blocks"] = blocks_benchmark(rk, w, t)
     if args.serve_jobs > 0:
         out["serving"] = serving_benchmark(args.serve_jobs)
     if not args.no_cpu_baseline:

@@ -194,6 +194,9 @@ int64_t bs2e_block_sizes(const bs2e_block *b, int64_t *sizes, int64_t n);
 int64_t bs2e_block_fill(bs2e_block *b, int64_t max_l_1p, const double *Hl, const double *S, int64_t *ip_H,
                         int64_t *ix_H, double *d_H, int64_t *ip_S, int64_t *ix_S, double *d_S);
 int64_t bs2e_block_release(bs2e_block *b);
+/* Device time (ms, HIP events on the context's stream) of the block's count
+ * kernel (ms[0], at create) and fill kernel (ms[1], at the last fill). */
+int64_t bs2e_block_times(const bs2e_block *b, double *ms, int64_t n);
 
 /* ---- build service ----------------------------------------------------- */
 /* No reference counterpart: the reference builds one basis per process
