@@ -32,7 +32,7 @@ from .capi import Bs2eError, check, ptr
 __all__ = ["Bs2eError", "Context", "generate_grid", "b_spline", "sparse_4d", "sparse_6d", "setup_Slater_integrals",
            "Nd_DOK", "RkTensor", "compute_R_K_map", "ang_k_LS", "three_j", "six_j", "count_nnz", "sym",
            "count_configs", "CSR_matrix", "Block", "construct_block_tensor", "Server", "footprint", "setup_S",
-           "setup_H_one_particle"]
+           "setup_H_one_particle", "radial_dipole", "setup_radial_dip", "DipBlock", "construct_dip_block_tensor"]
 
 
 class Context:
@@ -535,5 +535,116 @@ def construct_block_tensor(H, S, b_splines, term: sym, max_k, R_k, full=True):
     blk = Block(tensor, term, full)
     try:
         return blk.fill(H, S)
+    finally:
+        blk.release()
+
+
+# --------------------------------------------------------------------- dipoles
+@dataclass
+class radial_dipole:
+    """mat_els.f90:14-19: the radial dipole matrices of one gauge ('l': r_mat;
+    'v': dr_mat and r_inv_mat), n_b x n_b complex."""
+    gauge: str = "l"
+    r_mat: np.ndarray | None = None
+    dr_mat: np.ndarray | None = None
+    r_inv_mat: np.ndarray | None = None
+
+
+def _gauge_code(gauge) -> int:
+    if gauge not in ("l", "v"):
+        raise Bs2eError(capi.BS2E_EINVAL, f"Unrecognized gauge option for radial dipole: {gauge!r}")
+    return 0 if gauge == "l" else 1
+
+
+def setup_radial_dip(b_splines, k_GL, gauge="l", ctx: Context | None = None) -> radial_dipole:
+    """mat_els.f90:120-170 setup_radial_dip(b_splines, k_GL, radial_dip, gauge)
+    on the device (:348-390 compute_radial_dip_len / _vel)."""
+    ctx = ctx or default_context()
+    g = _gauge_code(gauge)
+    t = b_splines.knots
+    n_b = b_splines.n_b
+    m1 = np.zeros(n_b * n_b, np.complex128)
+    m2 = np.zeros(n_b * n_b if g else 1, np.complex128)
+    D = C.POINTER(C.c_double)
+    check(capi.lib().bs2e_setup_radial_dip(ctx.handle, b_splines.k, len(t), ptr(t), k_GL, g, m1.ctypes.data_as(D),
+                                           m2.ctypes.data_as(D)), ctx.handle)
+    m1 = m1.reshape(n_b, n_b, order="F")
+    if g == 0:
+        return radial_dipole("l", r_mat=m1)
+    return radial_dipole("v", dr_mat=m1, r_inv_mat=m2.reshape(n_b, n_b, order="F"))
+
+
+class DipBlock:
+    """One dipole block between two symmetries on the device (dipole.f90:95-150
+    init_dip_block counts; fill evaluates dip_mat_neq for a gauge)."""
+
+    def __init__(self, syms, q, b_splines, compute=True, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        s1, s2 = syms
+        cf1 = np.ascontiguousarray(s1.configs, dtype=np.int64)
+        cf2 = np.ascontiguousarray(s2.configs, dtype=np.int64)
+        self.shape = (int(cf1.shape[0]), int(cf2.shape[0]))
+        self.n_b = int(b_splines.n_b)
+        h = C.c_void_p()
+        I = C.c_int64
+        check(capi.lib().bs2e_dip_block_create(
+            self.ctx.handle, int(b_splines.k), self.n_b, int(s1.l), int(s1.m), int(bool(s1.pi)), self.shape[0],
+            ptr(cf1, I), int(s2.l), int(s2.m), int(bool(s2.pi)), self.shape[1], ptr(cf2, I), int(q),
+            int(bool(compute)), C.byref(h)), self.ctx.handle)
+        self.handle = h
+        sz = np.zeros(3, np.int64)
+        check(capi.lib().bs2e_dip_block_sizes(h, ptr(sz, I), 3), self.ctx.handle)
+        self.nnz = int(sz[0])
+
+    def fill(self, S, radial_dip: radial_dipole) -> CSR_matrix:
+        n_b = self.n_b
+        g = _gauge_code(radial_dip.gauge)
+        mats = [S, radial_dip.r_mat] if g == 0 else [S, radial_dip.dr_mat, radial_dip.r_inv_mat]
+        flat = []
+        for M in mats:
+            M = np.asarray(M, np.complex128)
+            if M.shape != (n_b, n_b):
+                raise Bs2eError(capi.BS2E_EINVAL, f"S and the radial dipole matrices must be {n_b} x {n_b}")
+            flat.append(np.ascontiguousarray(M.ravel(order="F")))
+        if g == 0:
+            flat.append(flat[1])
+        n1 = self.shape[0]
+        ip = np.zeros(n1 + 1, np.int64)
+        ix = np.zeros(max(self.nnz, 1), np.int64)
+        d = np.zeros(max(self.nnz, 1), np.complex128)
+        D = C.POINTER(C.c_double)
+        I = C.c_int64
+        check(capi.lib().bs2e_dip_block_fill(self.handle, g, flat[0].ctypes.data_as(D), flat[1].ctypes.data_as(D),
+                                             flat[2].ctypes.data_as(D), ptr(ip, I), ptr(ix, I), d.ctypes.data_as(D)),
+              self.ctx.handle)
+        return CSR_matrix(self.shape, ip, ix[:self.nnz], d[:self.nnz])
+
+    def kernel_ms(self):
+        """(count kernel, fill kernel) device ms, HIP events."""
+        ms = np.zeros(2)
+        check(capi.lib().bs2e_dip_block_times(self.handle, ptr(ms), 2), self.ctx.handle)
+        return float(ms[0]), float(ms[1])
+
+    def release(self):
+        if self.handle:
+            capi.lib().bs2e_dip_block_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def construct_dip_block_tensor(syms, q, b_splines, S, radial_dip: radial_dipole, compute=True,
+                               ctx: Context | None = None) -> CSR_matrix:
+    """dipole.f90:8-47 construct_dip_block_tensor(syms, q, b_splines, S,
+    radial_dip, dip_block, compute): the CSR dipole block between the
+    configurations of syms[0] (rows) and syms[1] (columns) for spherical
+    component q, on the device."""
+    blk = DipBlock(syms, q, b_splines, compute, ctx)
+    try:
+        return blk.fill(S, radial_dip)
     finally:
         blk.release()

@@ -76,6 +76,7 @@ void gauss_legendre(int64_t N, double* x, double* w);
 int64_t generate_grid(int64_t k, int64_t m, int64_t Z, double h_max, double r_max, std::vector<double>& out);
 double wigner_3j(int64_t ja, int64_t jb, int64_t jc, int64_t ma, int64_t mb, int64_t mc);
 double wigner_6j(int64_t j1, int64_t j2, int64_t j3, int64_t j4, int64_t j5, int64_t j6);
+double c_red(int64_t k, int64_t a, int64_t b);   // wigner_tools.f90:107-112 C_red_mat
 double ang_k_LS(int64_t k, int64_t l1a, int64_t l1b, int64_t l2a, int64_t l2b, int64_t L);
 
 // Device buffers of one tensor build.
@@ -131,6 +132,41 @@ struct BlockArgs {
 hipError_t launch_one_body(const Plan& p, const DevTables& d, double* B, double* B2, double* rw, int l_min, int n_l,
                            int Z, int cap_order, double cap_r0, double eta_re, double eta_im, double* S, double* H,
                            hipStream_t s);
+
+// Dipole blocks: per (row channel, column channel) the terms of dip_mat_neq
+// (0: direct electron 1, 1: direct electron 2, 2 and 3: exchange).
+struct DipCoef {
+    int32_t nz;                 // ang_dip_red > 5e-16: the channel pair carries entries
+    int32_t cond[4];            // the term's l-equality holds
+    int32_t vkind[4];           // velocity gauge: 0 |l - l'| != 1, 1 l > l', 2 l < l'
+    int32_t pad;
+    double pf[4];               // (-1)^(..) sqrt((2L1+1)(2L2+1)) 6j(..)
+    double cred[4];             // length gauge: C_red(1, l, l')
+    double vsq[4];              // velocity gauge: sqrt(l) (l > l') or sqrt(l') (l < l')
+    double vm[4];               // velocity gauge: l'+1 (l > l') or l' (l < l')
+    double sgn_ex;              // (-1)^(L2 + l_c + l_d)
+};
+
+struct DipArgs {
+    int n1, n_chan2, ks, n_b, gauge;
+    double ang;                 // (-1)^(L1-M1) 3j(L1 1 L2; -M1 q M2)
+    const int64_t* cfg1;        // rows: 5 per config
+    const int32_t* cfg_chan1;   // channel of each row
+    const BlockChan* chan2;     // column channels
+    const BlockSeg* seg2;       // [n_chan2][n_b + 1]
+    const DipCoef* coef;        // [n_chan1][n_chan2]
+    const double *S, *rad1, *rad2;   // complex n_b x n_b column-major (rad: r_mat, or dr_mat and r_inv_mat)
+    int64_t* cnt;               // count pass
+    const int64_t* ptr;         // fill pass: 0-based row starts
+    int64_t* ix;
+    double* data;               // complex
+};
+
+hipError_t launch_dip_count(const DipArgs& a, hipStream_t s);
+hipError_t launch_dip_fill(const DipArgs& a, hipStream_t s);
+// radial dipole matrices: gauge 0 -> M1 = r_mat; 1 -> M1 = dr_mat, M2 = r_inv_mat
+hipError_t launch_radial_dipole(const Plan& p, const DevTables& d, double* B, double* B1, double* rw, int gauge,
+                                double* M1, double* M2, hipStream_t s);
 
 hipError_t launch_block_count(const BlockArgs& a, hipStream_t s);
 hipError_t launch_block_fill(const BlockArgs& a, hipStream_t s);

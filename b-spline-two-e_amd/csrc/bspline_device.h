@@ -54,6 +54,33 @@ __device__ __forceinline__ void splines_at(const double* __restrict__ t, int i1,
     for (int ls = 0; ls < KS; ++ls) out[ls * stride] = b[ls + 1];
 }
 
+// d1[ls] = B'_{s}(x) for the KS splines s = i1-KS+1+ls non-zero on the
+// interval with left knot index i1: the recurrence once, on order KS-1 values.
+template <int KS>
+__device__ __forceinline__ void splines_d1_at(const double* __restrict__ t, int i1, double x, double* d1)
+{
+    if constexpr (KS < 2) {
+        d1[0] = 0.0;
+    } else {
+        double low[KS - 1];   // order KS-1: splines i1-KS+2 .. i1
+        splines_at<KS - 1>(t, i1, x, low, 1);
+        const int s0 = i1 - KS + 1;
+        auto bm1 = [&](int s) {
+            const int ls = s - (i1 - KS + 2);
+            return (ls >= 0 && ls < KS - 1) ? low[ls] : 0.0;
+        };
+        auto coef = [&](int s) {   // (KS-1) / (t(s+KS-1) - t(s))
+            const double den = t[s + KS - 2] - t[s - 1];
+            return den > 0.0 ? (double)(KS - 1) / den : 0.0;
+        };
+#pragma unroll
+        for (int ls = 0; ls < KS; ++ls) {
+            const int s = s0 + ls;
+            d1[ls] = coef(s) * bm1(s) - coef(s + 1) * bm1(s + 1);
+        }
+    }
+}
+
 // d2[ls] = B''_{s}(x) for the KS splines s = i1-KS+1+ls non-zero on the
 // interval with left knot index i1 (t stored 0-based, t(n) == t[n-1]); a term
 // whose knot difference vanishes is zero.

@@ -895,6 +895,13 @@ void bs2e::rk_attach(bs2e_ctx* ctx, bs2e_rk* rk)
     ctx->refs += 1;
 }
 
+void bs2e::ctx_hold(bs2e_ctx* ctx) { ctx->refs += 1; }
+
+void bs2e::ctx_drop(bs2e_ctx* ctx)
+{
+    if (ctx && --ctx->refs == 0 && ctx->closing) ctx_free_now(ctx);
+}
+
 void bs2e::rk_delete(bs2e_rk* rk)
 {
     bs2e_ctx* ctx = rk->ctx;

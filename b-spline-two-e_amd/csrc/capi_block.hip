@@ -51,6 +51,52 @@ int64_t count_configs(int64_t L, int64_t term_pi, int64_t max_l_1p, int64_t n_b,
 
 }  // namespace bs2e
 
+// Channel runs of a configuration list in count_configs order (each (l1,l2)
+// contiguous, n1 ascending within it, n2 consecutive for each n1): per
+// channel its first configuration and count; per (channel, n1) the offset of
+// its first configuration within the channel and its n2 range; per
+// configuration its channel.  Returns an error message or "".
+std::string bs2e::channel_tables(const int64_t* configs, int64_t n_config, int64_t nb, std::vector<BlockChan>& chan,
+                                 std::vector<BlockSeg>& seg, std::vector<int32_t>& cfg_chan, int64_t& max_l)
+{
+    chan.clear();
+    seg.clear();
+    cfg_chan.assign((size_t)n_config, 0);
+    max_l = 0;
+    int64_t prev_n1 = -1, prev_n2 = -1;
+    for (int64_t i = 0; i < n_config; ++i) {
+        const int64_t* c = configs + 5 * i;
+        const int64_t n1 = c[0], n2 = c[1], l1 = c[2], l2 = c[3];
+        if (n1 < 1 || n1 > nb || n2 < 1 || n2 > nb || l1 < 0 || l2 < 0 || l1 > 63 || l2 > 63)
+            return "configuration " + std::to_string(i + 1) + " has indices outside the basis";
+        max_l = std::max({max_l, l1, l2});
+        if (chan.empty() || chan.back().l1 != l1 || chan.back().l2 != l2) {
+            for (const BlockChan& ch : chan)
+                if (ch.l1 == l1 && ch.l2 == l2) return "channel (l1,l2) appears in two separate runs";
+            chan.push_back(BlockChan{(int32_t)l1, (int32_t)l2, (int32_t)i, 0});
+            seg.resize(chan.size() * (size_t)(nb + 1), BlockSeg{0, 1, 0, 0});
+            prev_n1 = -1;
+            prev_n2 = -1;
+        }
+        BlockChan& ch = chan.back();
+        BlockSeg& sg = seg[(chan.size() - 1) * (size_t)(nb + 1) + n1];
+        if (n1 != prev_n1) {
+            if (n1 < prev_n1 || sg.hi >= sg.lo) return "configurations are not ordered by n1 within a channel";
+            sg.off = (int32_t)(i - ch.start);
+            sg.lo = (int32_t)n2;
+            sg.hi = (int32_t)n2;
+        } else {
+            if (n2 != prev_n2 + 1) return "n2 values of a channel's n1 are not consecutive";
+            sg.hi = (int32_t)n2;
+        }
+        ch.n += 1;
+        cfg_chan[i] = (int32_t)(chan.size() - 1);
+        prev_n1 = n1;
+        prev_n2 = n2;
+    }
+    return std::string();
+}
+
 struct bs2e_block {
     bs2e_rk* rk = nullptr;
     int64_t L = 0, n_config = 0, full = 1, max_l = 0;
@@ -138,47 +184,7 @@ int64_t bs2e_block_create(bs2e_rk* rk, int64_t L, int64_t n_config, const int64_
     b->full = full ? 1 : 0;
     // channel tables; the configurations must be in count_configs order
     const int64_t nb = p.n_b;
-    std::string bad;
-    b->cfg_chan.resize(n_config);
-    int64_t prev_n1 = -1, prev_n2 = -1;
-    for (int64_t i = 0; i < n_config && bad.empty(); ++i) {
-        const int64_t* c = configs + 5 * i;
-        const int64_t n1 = c[0], n2 = c[1], l1 = c[2], l2 = c[3];
-        if (n1 < 1 || n1 > nb || n2 < 1 || n2 > nb || l1 < 0 || l2 < 0 || l1 > 63 || l2 > 63) {
-            bad = "configuration " + std::to_string(i + 1) + " has indices outside the basis";
-            break;
-        }
-        b->max_l = std::max({b->max_l, l1, l2});
-        if (b->chan.empty() || b->chan.back().l1 != l1 || b->chan.back().l2 != l2) {
-            for (const BlockChan& ch : b->chan)
-                if (ch.l1 == l1 && ch.l2 == l2) bad = "channel (l1,l2) appears in two separate runs";
-            b->chan.push_back(BlockChan{(int32_t)l1, (int32_t)l2, (int32_t)i, 0});
-            b->seg.resize(b->chan.size() * (size_t)(nb + 1), BlockSeg{0, 1, 0, 0});
-            prev_n1 = -1;
-            prev_n2 = -1;
-        }
-        BlockChan& ch = b->chan.back();
-        BlockSeg& s = b->seg[(b->chan.size() - 1) * (size_t)(nb + 1) + n1];
-        if (n1 != prev_n1) {
-            if (n1 < prev_n1 || s.hi >= s.lo) {
-                bad = "configurations are not ordered by n1 within a channel";
-                break;
-            }
-            s.off = (int32_t)(i - ch.start);
-            s.lo = (int32_t)n2;
-            s.hi = (int32_t)n2;
-        } else {
-            if (n2 != prev_n2 + 1) {
-                bad = "n2 values of a channel's n1 are not consecutive";
-                break;
-            }
-            s.hi = (int32_t)n2;
-        }
-        ch.n += 1;
-        b->cfg_chan[i] = (int32_t)(b->chan.size() - 1);
-        prev_n1 = n1;
-        prev_n2 = n2;
-    }
+    std::string bad = channel_tables(configs, n_config, nb, b->chan, b->seg, b->cfg_chan, b->max_l);
     if (!bad.empty()) {
         delete b;
         return fail(ctx, BS2E_EINVAL, "bs2e_block_create: " + bad + " (count_configs order required)");

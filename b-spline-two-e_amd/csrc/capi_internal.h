@@ -85,6 +85,10 @@ inline void clear_stale_error() { (void)hipGetLastError(); }
 // destroy/release of the owner while dependents live is deferred to the
 // release of the last dependent (so that no release order frees memory a
 // live handle still reads).
+// A handle other than a tensor that uses the context's streams holds it
+// alive the same way (bs2e_destroy defers while any is held).
+void ctx_hold(bs2e_ctx* ctx);
+void ctx_drop(bs2e_ctx* ctx);
 void rk_attach(bs2e_ctx* ctx, bs2e_rk* rk);
 void rk_delete(bs2e_rk* rk);      // deletes the handle object, then drops its context reference
 int64_t rk_release_now(bs2e_rk* rk);
@@ -126,6 +130,10 @@ inline void ctx_free(bs2e_ctx* c, T*& p)
 // Device bytes a tensor build of this plan allocates (tables and tensor,
 // with the arena's alignment); the plan's near rows must be built.
 size_t rk_footprint(const Plan& p);
+
+// Channel runs of a configuration list in count_configs order (capi_block.hip).
+std::string channel_tables(const int64_t* configs, int64_t n_config, int64_t nb, std::vector<BlockChan>& chan,
+                           std::vector<BlockSeg>& seg, std::vector<int32_t>& cfg_chan, int64_t& max_l);
 
 // bs2e_rk_create for an existing plan with its near rows built.
 int64_t rk_from_plan(bs2e_ctx* ctx, Plan&& plan, bs2e_rk** out);

@@ -223,7 +223,7 @@ double wigner_6j(int64_t j1, int64_t j2, int64_t j3, int64_t j4, int64_t j5, int
 static double neg1pow(int64_t e) { return (std::llabs(e) & 1) ? -1.0 : 1.0; }
 
 // wigner_tools.f90:107-112 C_red_mat(k, a, b) and :126-138 ang_k_LS.
-static double c_red(int64_t k, int64_t a, int64_t b)
+double c_red(int64_t k, int64_t a, int64_t b)
 {
     return neg1pow(a) * std::sqrt((double)((2 * a + 1) * (2 * b + 1))) * wigner_3j(a, k, b, 0, 0, 0);
 }

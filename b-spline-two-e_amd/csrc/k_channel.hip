@@ -21,13 +21,12 @@
 //   cex[k]  = (-1)^(l_c+l_d+L) ang_k_LS(k, confs_ex, L)  (same threshold)
 // and r12_allowed uses the 5e-15 threshold of :171-178.
 #include "bs2e_internal.h"
+#include "channel_walk.h"
 
 namespace bs2e {
 
 constexpr int kCThreads = 256;
 constexpr int kWave = 64;
-
-__device__ __forceinline__ int iabs_d(int x) { return x < 0 ? -x : x; }
 
 struct Cplx {
     double re, im;
@@ -80,103 +79,12 @@ __global__ __launch_bounds__(kCThreads) void kc_block(BlockArgs A)
         const double* cd = A.coef + ((size_t)ch_i * A.n_chan + ch) * 2 * K;
         const double* ce = cd + K;
         const BlockSeg* sg = A.seg + (size_t)ch * (A.n_b + 1);
-        // n_c candidates: within k of n_a (direct) or of n_b (exchange), ascending
-        int r1lo = n_a - ks + 1, r1hi = n_a + ks - 1, r2lo = n_bb - ks + 1, r2hi = n_bb + ks - 1;
-        if (r2lo < r1lo) {
-            int t0 = r1lo, t1 = r1hi;
-            r1lo = r2lo;
-            r1hi = r2hi;
-            r2lo = t0;
-            r2hi = t1;
-        }
-        if (r2lo <= r1hi + 1) {       // overlapping or adjacent: one range
-            r1hi = r2hi > r1hi ? r2hi : r1hi;
-            r2lo = 1;
-            r2hi = 0;
-        }
-        r1lo = r1lo < 1 ? 1 : r1lo;
-        r1hi = r1hi > A.n_b ? A.n_b : r1hi;
-        r2lo = r2lo < 1 ? 1 : r2lo;
-        r2hi = r2hi > A.n_b ? A.n_b : r2hi;
-        const int n1 = r1hi >= r1lo ? r1hi - r1lo + 1 : 0, n2 = r2hi >= r2lo ? r2hi - r2lo + 1 : 0;
-        const int ncand = n1 + n2;    // <= 2 (2k - 1) <= 62 for k <= 16: one per lane
-        // lane l takes the l-th candidate n_c and its n_d runs: direct needs
-        // |n_b - n_d| < k, exchange |n_a - n_d| < k, both clipped to the
-        // channel's n2 range for that n1 = n_c
-        int my_c = 0, d1lo = 1, d1hi = 0, d2lo = 1, d2hi = 0, my_soff = 0, my_slo = 0;
-        if (lane < ncand) {
-            my_c = lane < n1 ? r1lo + lane : r2lo + (lane - n1);
-            const BlockSeg sgc = sg[my_c];
-            if (sgc.hi >= sgc.lo) {
-                const bool in_dir_c = iabs_d(my_c - n_a) < ks;
-                const bool in_ex_c = iabs_d(my_c - n_bb) < ks;
-                int s1lo = 1, s1hi = 0, s2lo = 1, s2hi = 0;
-                if (in_dir_c) {
-                    s1lo = n_bb - ks + 1;
-                    s1hi = n_bb + ks - 1;
-                }
-                if (in_ex_c) {
-                    s2lo = n_a - ks + 1;
-                    s2hi = n_a + ks - 1;
-                }
-                if (!in_dir_c) {
-                    s1lo = s2lo;
-                    s1hi = s2hi;
-                    s2lo = 1;
-                    s2hi = 0;
-                } else if (in_ex_c) {
-                    if (s2lo < s1lo) {
-                        int t0 = s1lo, t1 = s1hi;
-                        s1lo = s2lo;
-                        s1hi = s2hi;
-                        s2lo = t0;
-                        s2hi = t1;
-                    }
-                    if (s2lo <= s1hi + 1) {
-                        s1hi = s2hi > s1hi ? s2hi : s1hi;
-                        s2lo = 1;
-                        s2hi = 0;
-                    }
-                }
-                d1lo = s1lo < sgc.lo ? sgc.lo : s1lo;
-                d1hi = s1hi > sgc.hi ? sgc.hi : s1hi;
-                d2lo = s2lo < sgc.lo ? sgc.lo : s2lo;
-                d2hi = s2hi > sgc.hi ? sgc.hi : s2hi;
-                my_soff = sgc.off;
-                my_slo = sgc.lo;
-            }
-        }
-        const int len1 = d1hi >= d1lo ? d1hi - d1lo + 1 : 0, len2 = d2hi >= d2lo ? d2hi - d2lo + 1 : 0;
-        // exclusive prefix over the lanes of the candidates' run lengths
-        int incl = len1 + len2;
-        for (int off = 1; off < kWave; off <<= 1) {
-            const int y = __shfl_up(incl, off, kWave);
-            if (lane >= off) incl += y;
-        }
-        const int total = __shfl(incl, kWave - 1, kWave);
-        const int excl = incl - len1 - len2;
-        // candidates flattened in (n_c, n_d) order, which is ascending column order
-        for (int f0 = 0; f0 < total; f0 += kWave) {
-            const int f = f0 + lane;
-            int pos = 0;   // last candidate lane whose run starts at or before f
-            for (int step = kWave / 2; step > 0; step >>= 1) {
-                const int cand = pos + step;
-                const int e = __shfl(excl, cand < kWave ? cand : kWave - 1, kWave);
-                if (cand < ncand && e <= f) pos = cand;
-            }
-            const int n_c = __shfl(my_c, pos, kWave);
-            const int o_lo1 = __shfl(d1lo, pos, kWave), o_len1 = __shfl(len1, pos, kWave);
-            const int o_lo2 = __shfl(d2lo, pos, kWave), o_ex = __shfl(excl, pos, kWave);
-            const int s_off = __shfl(my_soff, pos, kWave), s_lo = __shfl(my_slo, pos, kWave);
-            const int local = f - o_ex;
-            const int n_d = local < o_len1 ? o_lo1 + local : o_lo2 + (local - o_len1);
-            const bool in_dir_c = iabs_d(n_c - n_a) < ks;
-            const bool in_ex_c = iabs_d(n_c - n_bb) < ks;
+        walk_channel(n_a, n_bb, ks, A.n_b, sg, lane, [&](bool active, int n_c, int n_d, int col, bool in_dir_c,
+                                                          bool in_ex_c) {
             {
                 {
                     {
-                        bool active = f < total;
-                        const int j = cj.start + s_off + (n_d - s_lo);
+                        const int j = cj.start + col;
                         if (!A.full && j < i) active = false;
                         const bool support = in_dir_c && iabs_d(n_bb - n_d) < ks;
                         const bool support_ex = in_ex_c && iabs_d(n_a - n_d) < ks;
@@ -251,7 +159,7 @@ __global__ __launch_bounds__(kCThreads) void kc_block(BlockArgs A)
                     }
                 }
             }
-        }
+        });
     }
     if (!FILL && lane == 0) {
         A.cntH[i] = nH;

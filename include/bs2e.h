@@ -198,6 +198,43 @@ int64_t bs2e_block_release(bs2e_block *b);
  * kernel (ms[0], at create) and fill kernel (ms[1], at the last fill). */
 int64_t bs2e_block_times(const bs2e_block *b, double *ms, int64_t n);
 
+/* ---- dipole operators -------------------------------------------------- */
+/* mat_els.f90:123-170 setup_radial_dip(b_splines, k_GL, gauge, radial_dip)
+ * with :348-370 compute_radial_dip_len / :372-390 compute_radial_dip_vel:
+ * gauge 0 (length) writes m1 = r_mat(i,j) = sum w r B_i B_j; gauge 1
+ * (velocity) writes m1 = dr_mat = sum (0,-1) w B_i B_j' and m2 = r_inv_mat =
+ * sum (0,-1) w B_i B_j / r, the sums over the intervals holding both splines
+ * and their GL points.  n_b x n_b double complex, column-major. */
+int64_t bs2e_setup_radial_dip(bs2e_ctx *ctx, int64_t k, int64_t n_knots, const double *knots, int64_t k_gl,
+                              int64_t gauge, double *m1, double *m2);
+
+typedef struct bs2e_dip bs2e_dip; /* the CSR pattern of one dipole block */
+
+/* dipole.f90:8-47 construct_dip_block_tensor(syms, q, b_splines, S,
+ * radial_dip, dip_block, compute), first half (with :95-150
+ * init_dip_block): rows are the configurations of syms(1) = (L1, M1, pi1),
+ * columns those of syms(2), both as bs2e_count_configs writes them; q is the
+ * spherical component (-1, 0, 1).  Counts on the device the entries whose
+ * radial pairs share support (direct or exchange) and whose angular factor
+ * (mat_els.f90:811-828 ang_dip_red) exceeds 5e-16.  A block without a parity
+ * change, with a vanishing Wigner-Eckart factor or with compute = 0 has no
+ * entries.  k: the spline order (<= 16); n_b: the radial basis size. */
+int64_t bs2e_dip_block_create(bs2e_ctx *ctx, int64_t k, int64_t n_b, int64_t L1, int64_t M1, int64_t pi1,
+                              int64_t n1, const int64_t *cf1, int64_t L2, int64_t M2, int64_t pi2, int64_t n2,
+                              const int64_t *cf2, int64_t q, int64_t compute, bs2e_dip **out);
+/* sizes[0..2] = nnz, rows (syms(1)%n_config), columns (syms(2)%n_config). */
+int64_t bs2e_dip_block_sizes(const bs2e_dip *d, int64_t *sizes, int64_t n);
+/* dipole.f90:33-46, second half: fills the 1-based CSR arrays (ip: rows+1;
+ * ix, data: nnz; data double complex interleaved) with ang * dip_mat_neq
+ * (mat_els.f90:717-737 with :739-770 dip_red_mat and :772-809
+ * dip_red_1p_len / _vel).  gauge 0: rad1 = r_mat; gauge 1: rad1 = dr_mat,
+ * rad2 = r_inv_mat (as bs2e_setup_radial_dip writes them); S: the overlap. */
+int64_t bs2e_dip_block_fill(bs2e_dip *d, int64_t gauge, const double *S, const double *rad1, const double *rad2,
+                            int64_t *ip, int64_t *ix, double *data);
+/* Device time (ms) of the count kernel (ms[0]) and the last fill (ms[1]). */
+int64_t bs2e_dip_block_times(const bs2e_dip *d, double *ms, int64_t n);
+int64_t bs2e_dip_block_release(bs2e_dip *d);
+
 /* ---- build service ----------------------------------------------------- */
 /* No reference counterpart: the reference builds one basis per process
  * (main_basis_setup.f90:75-86).  A service owns an HBM budget (one device

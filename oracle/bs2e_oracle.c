@@ -897,3 +897,130 @@ i64 or_construct_block(i64 k_s, i64 n_b, i64 L, i64 max_k, i64 full, i64 n_confi
     if (ref_cnt) { ref_cnt[0] = cntH; ref_cnt[1] = cntS; }
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* mat_els.f90:123-170 setup_radial_dip + :348-390 compute_radial_dip_len /  */
+/* _vel: gauge 0 (length): r_mat(i,j) = sum w r B_i B_j; gauge 1 (velocity): */
+/* dr_mat = sum (0,-1) w B_i B_j', r_inv_mat = sum (0,-1) w B_i B_j / r.     */
+/* n_b x n_b complex, column-major; unused outputs may be NULL.              */
+/* ------------------------------------------------------------------------ */
+void or_setup_radial_dip(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i64 gauge, double complex *r_mat,
+                         double complex *dr_mat, double complex *r_inv_mat)
+{
+    i64 n, n_b, n_iv;
+    or_spline_dims(k_s, n_knots, &n, &n_b, &n_iv);
+    const double *bp = t + (k_s - 1);
+    double *x0 = malloc(sizeof(double) * (size_t)k_gl), *w0 = malloc(sizeof(double) * (size_t)k_gl);
+    or_gauss_legendre(k_gl, x0, w0);
+    for (i64 e = 0; e < n_b * n_b; ++e) {
+        if (r_mat) r_mat[e] = 0.0;
+        if (dr_mat) dr_mat[e] = 0.0;
+        if (r_inv_mat) r_inv_mat[e] = 0.0;
+    }
+    const double complex mi = 0.0 - 1.0 * I;
+    for (i64 j_b = 1; j_b <= n_b; ++j_b)
+        for (i64 i_b = 1; i_b <= n_b; ++i_b) {
+            if (llabs(j_b - i_b) >= k_s) continue;
+            for (i64 i_r = 1; i_r <= n_iv; ++i_r) {
+                if (!(supp(k_s, i_r, i_b + 1) && supp(k_s, i_r, j_b + 1))) continue;
+                double a = bp[i_r - 1], b = bp[i_r];
+                for (i64 q = 0; q < k_gl; ++q) {
+                    double r = (b - a) / 2.0 * x0[q] + (a + b) / 2.0;
+                    double wq = (b - a) / 2.0 * w0[q];
+                    double Bi = or_bvalue_unit(t, n, k_s, i_b + 1, r, 0, i_r);
+                    double Bj = or_bvalue_unit(t, n, k_s, j_b + 1, r, 0, i_r);
+                    const i64 e = (j_b - 1) * n_b + (i_b - 1);
+                    if (gauge == 0) {
+                        r_mat[e] += wq * r * Bi * Bj;
+                    } else {
+                        double DBj = or_bvalue_unit(t, n, k_s, j_b + 1, r, 1, i_r);
+                        dr_mat[e] += mi * wq * Bi * DBj;
+                        r_inv_mat[e] += mi * wq * Bi * Bj / r;
+                    }
+                }
+            }
+        }
+    free(x0);
+    free(w0);
+}
+
+/* mat_els.f90:772-809 dip_red_1p_len / _vel for electron idx (0 or 1).      */
+static double complex dip_red_1p(const i64 *c1, const i64 *c2, int idx, i64 n_b, i64 gauge,
+                                 const double complex *r_mat, const double complex *dr_mat,
+                                 const double complex *r_inv_mat)
+{
+    i64 n = c1[idx], n_p = c2[idx], l = c1[2 + idx], l_p = c2[2 + idx];
+    const i64 e = (n_p - 1) * n_b + (n - 1);
+    if (gauge == 0) return r_mat[e] * or_C_red(1, l, l_p);
+    if (llabs(l - l_p) != 1) return 0.0;
+    if (l > l_p) return sqrt((double)l) * (dr_mat[e] - (double)(l_p + 1) * r_inv_mat[e]);
+    return -(sqrt((double)l_p) * (dr_mat[e] + (double)l_p * r_inv_mat[e]));
+}
+
+/* mat_els.f90:739-770 dip_red_mat (Lindgren and Morrison Eq. 4.23).         */
+static double complex dip_red_mat(i64 L1, i64 L2, const i64 *c1, const i64 *c2, i64 n_b, i64 gauge,
+                                  const double complex *S, const double complex *r_mat,
+                                  const double complex *dr_mat, const double complex *r_inv_mat)
+{
+    double complex res_1 = 0.0, res_2 = 0.0;
+    const double sq = sqrt((double)((2 * L1 + 1) * (2 * L2 + 1)));
+    if (c1[3] == c2[3])
+        res_1 = isign(c1[2] + c1[3] + L2 + 1) * sq * or_six_j(L1, 1, L2, c2[2], c1[3], c1[2]) *
+                dip_red_1p(c1, c2, 0, n_b, gauge, r_mat, dr_mat, r_inv_mat) * S[(c2[1] - 1) * n_b + (c1[1] - 1)];
+    if (c1[2] == c2[2])
+        res_2 = isign(c1[2] + c2[3] + L1 + 1) * sq * or_six_j(L1, 1, L2, c2[3], c1[2], c1[3]) *
+                dip_red_1p(c1, c2, 1, n_b, gauge, r_mat, dr_mat, r_inv_mat) * S[(c2[0] - 1) * n_b + (c1[0] - 1)];
+    return res_1 + res_2;
+}
+
+/* mat_els.f90:811-828 ang_dip_red                                          */
+static double ang_dip_red(i64 L1, i64 L2, const i64 *c1, const i64 *c2)
+{
+    double res = 0.0;
+    if (c1[3] == c2[3]) res = res + fabs(or_six_j(L1, 1, L2, c2[2], c1[3], c1[2]) * or_C_red(1, c1[2], c2[2]));
+    if (c1[2] == c2[2]) res = res + fabs(or_six_j(L1, 1, L2, c2[3], c1[2], c1[3]) * or_C_red(1, c1[3], c2[3]));
+    return res;
+}
+
+/* ------------------------------------------------------------------------ */
+/* dipole.f90:8-47 construct_dip_block_tensor with :95-150 init_dip_block   */
+/* and mat_els.f90:717-737 dip_mat_neq.  cf1/cf2: 5 int64 per config        */
+/* (n1, n2, l1, l2, eqv); symmetry (L, M, parity) per side; q in {-1,0,1}.  */
+/* Returns the nnz (CSR 1-based, written when ix != NULL and cap suffices); */
+/* -1 when cap is too small.  An empty block (parity, angular factor or     */
+/* compute = 0) has nnz 0 and index_ptr all 1.                              */
+/* ------------------------------------------------------------------------ */
+i64 or_construct_dip_block(i64 k_s, i64 n_b, i64 L1, i64 M1, i64 pi1, i64 n1, const i64 *cf1, i64 L2, i64 M2,
+                           i64 pi2, i64 n2, const i64 *cf2, i64 q, i64 compute, i64 gauge,
+                           const double complex *S, const double complex *r_mat, const double complex *dr_mat,
+                           const double complex *r_inv_mat, i64 *ip, i64 *ix, double complex *data, i64 cap)
+{
+    const int parity_allowed = (pi1 != 0) != (pi2 != 0);
+    const double ang = isign(L1 - M1) * or_three_j(L1, 1, L2, -M1, q, M2);
+    if (ip)
+        for (i64 i = 0; i <= n1; ++i) ip[i] = 1;
+    if (fabs(ang) < 5e-16 || !parity_allowed || !compute) return 0;
+    i64 ptr = 0;
+    for (i64 i = 0; i < n1; ++i) {
+        const i64 *c1 = cf1 + 5 * i;
+        for (i64 j = 0; j < n2; ++j) {
+            const i64 *c2 = cf2 + 5 * j;
+            int support = llabs(c1[0] - c2[0]) < k_s && llabs(c1[1] - c2[1]) < k_s;
+            int support_ex = llabs(c1[0] - c2[1]) < k_s && llabs(c1[1] - c2[0]) < k_s;
+            int nz = (support || support_ex) && ang_dip_red(L1, L2, c1, c2) > 5e-16;
+            if (!nz) continue;
+            if (ix) {
+                if (ptr >= cap) return -1;
+                const i64 c2x[5] = {c2[1], c2[0], c2[3], c2[2], c2[4]};
+                double complex red = dip_red_mat(L1, L2, c1, c2, n_b, gauge, S, r_mat, dr_mat, r_inv_mat);
+                double complex red_ex =
+                    isign(L2 + c2[2] + c2[3]) * dip_red_mat(L1, L2, c1, c2x, n_b, gauge, S, r_mat, dr_mat, r_inv_mat);
+                ix[ptr] = j + 1;
+                data[ptr] = ang * (red + red_ex);
+            }
+            ++ptr;
+        }
+        if (ip) ip[i + 1] = ptr + 1;
+    }
+    return ptr;
+}

@@ -92,6 +92,11 @@ def lib():
         L.or_construct_block.argtypes = [i64, i64, i64, i64, i64, i64, P(i64), P(dbl), P(dbl), P(dbl),
                                          P(i64), P(i64), P(dbl), P(i64), P(i64), P(dbl), i64, i64, P(i64),
                                          P(i64)]
+        L.or_setup_radial_dip.restype = None
+        L.or_setup_radial_dip.argtypes = [i64, i64, P(dbl), i64, i64, P(dbl), P(dbl), P(dbl)]
+        L.or_construct_dip_block.restype = i64
+        L.or_construct_dip_block.argtypes = [i64, i64, i64, i64, i64, i64, P(i64), i64, i64, i64, i64, P(i64), i64,
+                                             i64, i64, P(dbl), P(dbl), P(dbl), P(dbl), P(i64), P(i64), P(dbl), i64]
         _lib = L
     return _lib
 
@@ -367,3 +372,40 @@ def ref_count_configs(L, term_pi, max_l_1p, n_b, k_spline, max_n_b, n_all_l, l_2
     n = ref().ref_count_configs(L, int(term_pi), max_l_1p, n_b, k_spline, max_n_b, n_all_l, l_2_max,
                                 _ptr(out, i64), cap)
     return out[:5 * n].reshape(n, 5)
+
+
+def setup_radial_dip(k, t, k_gl, gauge="l"):
+    """mat_els.f90:123-170: gauge 'l' -> r_mat; 'v' -> (dr_mat, r_inv_mat); complex (n_b, n_b)."""
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    _, n_b, _ = dims(k, t)
+    g = 0 if gauge == "l" else 1
+    mats = [np.zeros(n_b * n_b, np.complex128) for _ in range(3)]
+    P_ = lambda a: a.ctypes.data_as(P(dbl))   # noqa: E731
+    lib().or_setup_radial_dip(k, len(t), _ptr(t, dbl), k_gl, g, P_(mats[0]), P_(mats[1]), P_(mats[2]))
+    out = [m.reshape(n_b, n_b, order="F") for m in mats]
+    return out[0] if g == 0 else (out[1], out[2])
+
+
+def construct_dip_block(k, n_b, sym1, cf1, sym2, cf2, q, compute, gauge, S, rad):
+    """dipole.f90:8-47 (+ init_dip_block :95-150): sym = (L, M, pi); rad = r_mat
+    (gauge 'l') or (dr_mat, r_inv_mat) ('v').  Returns (index_ptr, indices, data)."""
+    Lb = lib()
+    c1 = np.ascontiguousarray(cf1, dtype=np.int64).reshape(-1, 5)
+    c2 = np.ascontiguousarray(cf2, dtype=np.int64).reshape(-1, 5)
+    g = 0 if gauge == "l" else 1
+    F = lambda m: np.ascontiguousarray(np.asfortranarray(m).ravel(order="F"), dtype=np.complex128)   # noqa: E731
+    Sf = F(S)
+    if g == 0:
+        r, dr, ri = F(rad), None, None
+    else:
+        r, dr, ri = None, F(rad[0]), F(rad[1])
+    cp = lambda a: a.ctypes.data_as(P(dbl)) if a is not None else None   # noqa: E731
+    args = [k, n_b, sym1[0], sym1[1], int(sym1[2]), len(c1), _ptr(c1, i64), sym2[0], sym2[1], int(sym2[2]), len(c2),
+            _ptr(c2, i64), q, int(compute), g, cp(Sf), cp(r), cp(dr), cp(ri)]
+    nnz = Lb.or_construct_dip_block(*args, None, None, None, 0)
+    ip = np.ones(len(c1) + 1, np.int64)
+    ix = np.zeros(max(nnz, 1), np.int64)
+    d = np.zeros(max(nnz, 1), np.complex128)
+    n2 = Lb.or_construct_dip_block(*args, _ptr(ip, i64), _ptr(ix, i64), d.ctypes.data_as(P(dbl)), max(nnz, 1))
+    assert n2 == nnz
+    return ip, ix[:nnz], d[:nnz]

@@ -311,3 +311,107 @@ def test_block_symmetry_and_upper_triangle(oracle, c1_tensor, c1_one_electron, L
         rows = np.repeat(np.arange(1, n + 1), np.diff(ip))
         assert (ix >= rows).all()
     assert full["ref_count"][0] <= full["nnz"][0] and full["ref_count"][1] <= full["nnz"][1]
+
+
+# ------------------------------------------------------------------ dipoles
+def test_radial_dipole_hydrogenic(oracle):
+    """Radial dipole matrices (mat_els.f90:120-170, :348-390) between the
+    hydrogenic 1s and 2p orbitals of Z = 2 in the c1 basis: the length form
+    gives the closed form <1s|r|2p> = 2^7 sqrt(6) / (3^5 Z); the velocity
+    form, -(dr + r_inv) for l = 0 -> l' = 1 (mat_els.f90:796-808), equals
+    i (E_1s - E_2p) times the length form's reduced element r C_red(1, 0, 1)
+    (commutator [H, r] = -i p; C_red(1, 0, 1) = -1); r_mat and r_inv_mat are
+    symmetric and dr_mat antisymmetric (the splines vanish at both ends)."""
+    import scipy.linalg as sl
+    from bs2e.workloads import WORKLOADS
+    w = WORKLOADS["c1"]
+    t = oracle.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+    S = oracle.setup_S(w.k, t, w.k_GL).real
+    rl = oracle.setup_radial_dip(w.k, t, w.k_GL, "l")
+    dr, ri = oracle.setup_radial_dip(w.k, t, w.k_GL, "v")
+    assert abs(rl - rl.T).max() <= 1e-14 * abs(rl).max() and abs(rl.imag).max() == 0
+    assert abs(ri - ri.T).max() <= 1e-14 * abs(ri).max() and abs(ri.real).max() == 0
+    assert abs(dr + dr.T).max() <= 1e-13 * abs(dr).max() and abs(dr.real).max() == 0
+    orb, E = {}, {}
+    for l in (0, 1):
+        e, V = sl.eigh(oracle.setup_H1p(w.k, t, w.k_GL, 2, l, cap_eta=0.0).real, S)
+        orb[l], E[l] = V[:, 0], e[0]
+    assert abs(E[0] + 2.0) < 1e-10 and abs(E[1] + 0.5) < 1e-10
+    ln = orb[0] @ rl.real @ orb[1]
+    exact = 2**7 * np.sqrt(6) / 3**5 / 2
+    assert abs(abs(ln) - exact) < 1e-11 * exact
+    assert abs(oracle.C_red(1, 0, 1) + 1.0) < 1e-15
+    vel = orb[0] @ (-(dr + ri)) @ orb[1]
+    assert abs(vel - 1j * (E[0] - E[1]) * (ln * oracle.C_red(1, 0, 1))) < 1e-11 * abs(ln)
+
+
+@pytest.mark.parametrize("L1,p1,L2,p2", [(0, 0, 1, 1), (1, 1, 2, 0), (1, 1, 1, 0), (2, 0, 3, 1)])
+def test_dipole_block_adjoint(oracle, L1, p1, L2, p2):
+    """d_q^dagger = (-1)^q d_-q: the block from syms (1, 2) with component q
+    is (-1)^q times the transpose of the block from (2, 1) with -q in the
+    length gauge (real) and -(-1)^q times it in the velocity gauge (imaginary:
+    p is Hermitian); the patterns are transposes.  The two blocks reach the
+    6j symbols, exchange terms and support tests from opposite sides, so a
+    swapped argument, a wrong exchange sign or a transposed radial index
+    breaks the relation."""
+    import scipy.sparse as sp
+    from bs2e.workloads import WORKLOADS
+    w = WORKLOADS["tiny"]
+    t = oracle.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+    _, n_b, _ = oracle.dims(w.k, t)
+    S = oracle.setup_S(w.k, t, w.k_GL)
+    rad = {"l": oracle.setup_radial_dip(w.k, t, w.k_GL, "l"), "v": oracle.setup_radial_dip(w.k, t, w.k_GL, "v")}
+    cf1 = oracle.count_configs(L1, p1, 2, n_b, w.k, n_b, n_b, 2)
+    cf2 = oracle.count_configs(L2, p2, 2, n_b, w.k, n_b, n_b, 2)
+    csr = lambda b, n, m: sp.csr_matrix((b[2], b[1] - 1, b[0] - 1), shape=(n, m))   # noqa: E731
+    seen = 0
+    for q in (-1, 0, 1):
+        M2 = 0
+        M1 = q + M2
+        if abs(M1) > L1:
+            continue
+        for g, sgn in (("l", (-1) ** q), ("v", -(-1) ** q)):
+            A = csr(oracle.construct_dip_block(w.k, n_b, (L1, M1, p1), cf1, (L2, M2, p2), cf2, q, 1, g, S, rad[g]),
+                    len(cf1), len(cf2))
+            B = csr(oracle.construct_dip_block(w.k, n_b, (L2, M2, p2), cf2, (L1, M1, p1), cf1, -q, 1, g, S, rad[g]),
+                    len(cf2), len(cf1))
+            assert A.nnz == B.nnz
+            if A.nnz == 0:   # 3j(1 1 1; 0 0 0) = 0
+                continue
+            pa = sp.csr_matrix((np.ones(A.nnz), A.indices, A.indptr), shape=A.shape)
+            pb = sp.csr_matrix((np.ones(B.nnz), B.indices, B.indptr), shape=B.shape)
+            assert (pa != pb.T).nnz == 0
+            assert abs(A - sgn * B.T).max() <= 1e-13 * abs(A).max()
+            seen += 1
+    assert seen >= 2
+
+
+def test_helium_oscillator_strength(oracle, c1_tensor, c1_one_electron):
+    """He 1s^2 1S -> 1s2p 1P^o oscillator strength from the dipole block
+    (dipole.f90:8-47) between the lowest states of the L = 0 and L = 1 blocks,
+    as the reference's test does (test_mat_els.f90:464-490: 2 dE |<0|z|1>|^2
+    in the length gauge, 2 |<0|z|1>|^2 / dE in the velocity gauge).  The l <= 2
+    basis gives 0.2747 / 0.2742 against the converged 0.27616 (Drake); the two
+    gauges agree to 0.2 %."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    w, t, n_b, R, _ = c1_tensor
+    S, Hl = c1_one_electron
+    st = {}
+    for L, pi in ((0, 0), (1, 1)):
+        cf = oracle.count_configs(L, pi, 2, n_b, w.k, n_b, n_b, 2)
+        b = oracle.construct_block(w.k, n_b, L, w.max_k, True, cf, R, Hl, S)
+        H, M = _csr(b, "H", len(cf)).real.tocsc(), _csr(b, "S", len(cf)).real.tocsc()
+        e, v = sla.eigsh(H, k=1, M=M, sigma=-3.0, which="LM")
+        st[L] = (cf, e[0], v[:, 0] / np.sqrt(v[:, 0] @ (M @ v[:, 0])))
+    (cf0, E0, c0), (cf1, E1, c1) = st[0], st[1]
+    dE = E1 - E0
+    f = {}
+    for g in ("l", "v"):
+        rad = oracle.setup_radial_dip(w.k, t, w.k_GL, g)
+        ip, ix, d = oracle.construct_dip_block(w.k, n_b, (0, 0, 0), cf0, (1, 0, 1), cf1, 0, 1, g, S, rad)
+        D = sp.csr_matrix((d, ix - 1, ip - 1), shape=(len(cf0), len(cf1)))
+        m = c0 @ (D @ c1)
+        f[g] = 2 * dE * abs(m) ** 2 if g == "l" else 2 * abs(m) ** 2 / dE
+    assert abs(f["l"] - 0.27616) < 0.01 * 0.27616 and abs(f["v"] - 0.27616) < 0.01 * 0.27616
+    assert abs(f["l"] - f["v"]) < 0.002 * f["l"]
