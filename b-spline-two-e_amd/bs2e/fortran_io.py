@@ -3,6 +3,7 @@ written from Python, so that blocks built here can be handed to its `diag`
 and its files read back:
 
     store_block_diag / load_block_diag   block_tools.f90:458-524  (H_diag.dat, S_diag.dat)
+    store_block_CS / load_block_CS       block_tools.f90:386-456  (D_-1.dat, D_0.dat, D_1.dat)
     store_CS / load_CS                    sparse_array_tools.f90:695-742
     store_splines / load_splines          bspline_tools.f90:375-400 (splines.dat)
     store_basis / load_basis              orbital_tools.f90:364-397 (basis.dat, basis_info.txt)
@@ -160,6 +161,55 @@ def load_block_diag(path) -> list[CSR_matrix]:
             ix = np.zeros(0, np.int64)
             d = np.zeros(0, np.complex128)
         out.append(CSR_matrix(shape, ip, ix, d))
+    r.close()
+    return out
+
+
+def store_block_CS(path, blocks: list[list[CSR_matrix]]):
+    """block_tools.f90:386-415 CS_block_store: 'CSR', block_shape, shape (the
+    row counts of the diagonal blocks, then their column counts), then per
+    block, column-block j outer and row-block i inner, its shape, nnz and,
+    when nnz > 0, index_ptr, indices and data.  blocks[i][j]."""
+    n1 = len(blocks)
+    n2 = len(blocks[0]) if n1 else 0
+    shape = [sum(blocks[i][i].shape[0] for i in range(n1)), sum(blocks[i][i].shape[1] for i in range(n2))]
+    w = _Writer(path)
+    w.rec(b"CSR")
+    w.rec(_i8(n1, n2))
+    w.rec(_i8(*shape))
+    for j in range(n2):
+        for i in range(n1):
+            b = blocks[i][j]
+            w.rec(_i8(*b.shape))
+            w.rec(_i8(b.nnz))
+            if b.nnz > 0:
+                _put_csr_arrays(w, b)
+    w.close()
+
+
+def load_block_CS(path) -> list[list[CSR_matrix]]:
+    """block_tools.f90:417-456 CS_block_load; returns blocks[i][j] (an empty
+    block has index_ptr all 1)."""
+    r = _Reader(path)
+    kind = r.raw().decode()
+    if kind != "CSR":
+        raise ValueError(f"{path}: only CSR blocks are implemented, found {kind}")
+    n1, n2 = (int(x) for x in r.ints())
+    r.ints()
+    out = [[None] * n2 for _ in range(n1)]
+    for j in range(n2):
+        for i in range(n1):
+            shape = tuple(int(x) for x in r.ints())
+            nnz = r.int()
+            if nnz > 0:
+                ip = r.ints()
+                ix = r.ints()
+                d = np.frombuffer(r.raw(), dtype="<c16").copy()
+            else:
+                ip = np.ones(shape[0] + 1, np.int64)
+                ix = np.zeros(0, np.int64)
+                d = np.zeros(0, np.complex128)
+            out[i][j] = CSR_matrix(shape, ip, ix, d)
     r.close()
     return out
 

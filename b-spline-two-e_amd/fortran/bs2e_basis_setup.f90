@@ -5,12 +5,15 @@
 ! of one symmetry.
 !
 ! usage: bs2e_basis_setup k m Z h_max r_max k_GL max_k max_l L pi full key_stride hs_file out_file
-!                         [max_L res_dir]
+!                         [max_L res_dir [gauge]]
 !
 ! With max_L and res_dir, it also builds the blocks of every symmetry of the
-! z-polarised two-electron basis (L = 0..max_L) and stores what the
-! reference's basis_setup stores for them (main_basis_setup.f90:154-161):
-! res_dir/H_diag.dat, S_diag.dat, basis.dat, basis_info.txt, splines.dat.
+! z-polarised two-electron basis (L = 0..max_L) and the dipole blocks
+! between every pair of them for q = -1, 0, 1 in the given gauge ('l' by
+! default; main_basis_setup.f90:122-145), and stores what the reference's
+! basis_setup stores for them (main_basis_setup.f90:154-161):
+! res_dir/H_diag.dat, S_diag.dat, D_-1.dat, D_0.dat, D_1.dat, basis.dat,
+! basis_info.txt, splines.dat.
 !
 ! hs_file (stream): S then H(l) for l = 0..max_l, each n_b x n_b complex(dp)
 ! column-major (the one-electron matrices, mat_els.f90:85-118 and :294-321);
@@ -48,6 +51,12 @@ program bs2e_basis_setup
     character(len=4096) :: res_dir
     type(basis) :: bas
     type(block_diag_CS) :: H_diag, S_diag
+    type(block_CS) :: dipoles(-1:1)
+    type(radial_dipole) :: radial_dip
+    type(sym) :: dsyms(2)
+    character :: gauge
+    integer :: iq, j
+    character(len=2) :: qs
 
     call get_command_argument(1, arg); read(arg, *) k
     call get_command_argument(2, arg); read(arg, *) m
@@ -157,8 +166,29 @@ program bs2e_basis_setup
             call construct_block_tensor(H_vec, S, splines, bas%syms(i), max_k, R_p, H_diag%blocks(i), &
                                         S_diag%blocks(i), ifull /= 0)
         end do
+        gauge = 'l'
+        if (command_argument_count() >= 17) then
+            call get_command_argument(17, arg)
+            gauge = arg(1:1)
+        end if
+        call setup_radial_dip(splines, k_GL, radial_dip, gauge)
+        do iq = -1, 1
+            call dipoles(iq)%init([bas%n_sym, bas%n_sym])
+            do j = 1, bas%n_sym
+                dsyms(2) = bas%syms(j)
+                do i = 1, bas%n_sym
+                    dsyms(1) = bas%syms(i)
+                    call construct_dip_block_tensor(dsyms, iq, splines, S, radial_dip, dipoles(iq)%blocks(i, j), &
+                                                    ifull /= 0 .or. i <= j)
+                end do
+            end do
+        end do
         call H_diag%store(trim(res_dir) // "H_diag.dat")
         call S_diag%store(trim(res_dir) // "S_diag.dat")
+        do iq = -1, 1
+            write(qs, '(i0)') iq
+            call dipoles(iq)%store(trim(res_dir) // "D_" // trim(qs) // ".dat")
+        end do
         call bas%store(trim(res_dir))
         call splines%store(trim(res_dir))
         write(6, '(a,i0,a)') "stored ", bas%n_sym, " symmetry blocks in " // trim(res_dir)

@@ -16,6 +16,8 @@ module bs2e_capi
     public :: bs2e_rk_synchronize, bs2e_rk_info, bs2e_rk_copy_to_host, bs2e_rk_get_val, bs2e_rk_release
     public :: bs2e_ang_k_LS, bs2e_count_configs, bs2e_block_create, bs2e_block_sizes, bs2e_block_fill
     public :: bs2e_block_release, c_string, bs2e_setup_S, bs2e_setup_H_one_particle
+    public :: bs2e_setup_radial_dip, bs2e_dip_block_create, bs2e_dip_block_sizes, bs2e_dip_block_fill
+    public :: bs2e_dip_block_release
 
     interface
         function bs2e_create(ctx, device) bind(C, name="bs2e_create") result(rc)
@@ -196,6 +198,49 @@ module bs2e_capi
         function bs2e_block_release(blk) bind(C, name="bs2e_block_release") result(rc)
             import :: c_ptr, c_int64_t
             type(c_ptr), value :: blk
+            integer(c_int64_t) :: rc
+        end function
+
+        function bs2e_setup_radial_dip(ctx, k, n_knots, knots, k_gl, gauge, m1, m2) &
+                bind(C, name="bs2e_setup_radial_dip") result(rc)
+            import :: c_ptr, c_int64_t, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), value :: k, n_knots, k_gl, gauge
+            real(c_double), intent(in) :: knots(*)
+            type(c_ptr), value :: m1, m2
+            integer(c_int64_t) :: rc
+        end function
+
+        function bs2e_dip_block_create(ctx, k, n_b, L1, M1, pi1, n1, cf1, L2, M2, pi2, n2, cf2, q, compute, out) &
+                bind(C, name="bs2e_dip_block_create") result(rc)
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), value :: k, n_b, L1, M1, pi1, n1, L2, M2, pi2, n2, q, compute
+            integer(c_int64_t), intent(in) :: cf1(*), cf2(*)
+            type(c_ptr), intent(out) :: out
+            integer(c_int64_t) :: rc
+        end function
+
+        function bs2e_dip_block_sizes(d, sizes, n) bind(C, name="bs2e_dip_block_sizes") result(rc)
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: d
+            integer(c_int64_t), intent(out) :: sizes(*)
+            integer(c_int64_t), value :: n
+            integer(c_int64_t) :: rc
+        end function
+
+        function bs2e_dip_block_fill(d, gauge, S, rad1, rad2, ip, ix, data) bind(C, name="bs2e_dip_block_fill") &
+                result(rc)
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: d
+            integer(c_int64_t), value :: gauge
+            type(c_ptr), value :: S, rad1, rad2, ip, ix, data
+            integer(c_int64_t) :: rc
+        end function
+
+        function bs2e_dip_block_release(d) bind(C, name="bs2e_dip_block_release") result(rc)
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: d
             integer(c_int64_t) :: rc
         end function
 

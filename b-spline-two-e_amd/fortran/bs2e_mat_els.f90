@@ -44,6 +44,7 @@ module bs2e_mat_els
     public :: generate_grid, setup_Slater_integrals, compute_R_k_map, compute_R_k_direct
     public :: count_configs, construct_block_tensor, ang_k_LS, bs2e_use_device, bs2e_shutdown
     public :: hydrogenic, CAP, setup_S, setup_H_one_particle, basis, block_diag_CS
+    public :: block_CS, radial_dipole, setup_radial_dip, construct_dip_block_tensor
 
     type(c_ptr), save :: g_ctx = c_null_ptr
     integer, save :: g_device = 0
@@ -144,6 +145,26 @@ module bs2e_mat_els
         procedure :: store => CS_block_diag_store
         procedure :: load => CS_block_diag_load
     end type block_diag_CS
+
+    ! block_tools.f90:24-39: a matrix of CSR blocks (the dipole operator
+    ! between every pair of symmetries)
+    type :: block_CS
+        integer :: block_shape(2) = 0
+        integer :: shape(2) = 0
+        type(CSR_matrix), allocatable :: blocks(:,:)
+    contains
+        procedure :: init => init_block_CS
+        procedure :: compute_shape => compute_shape_block_CS
+        procedure :: store => CS_block_store
+        procedure :: load => CS_block_load
+    end type block_CS
+
+    ! mat_els.f90:14-19: radial dipole matrices of one gauge
+    type :: radial_dipole
+        complex(dp), allocatable :: r_mat(:,:)
+        complex(dp), allocatable :: dr_mat(:,:)
+        complex(dp), allocatable :: r_inv_mat(:,:)
+    end type radial_dipole
 
     type :: basis
         integer :: max_l_1p = 0
@@ -436,6 +457,169 @@ contains
         this%r_0 = r_0
         this%eta = eta
     end subroutine init_CAP
+
+    ! block_tools.f90:97-104
+    subroutine init_block_CS(this, shape_in)
+        class(block_CS), intent(inout) :: this
+        integer, intent(in) :: shape_in(2)
+        this%block_shape = shape_in
+        if (allocated(this%blocks)) deallocate(this%blocks)
+        allocate(this%blocks(shape_in(1), shape_in(2)))
+    end subroutine init_block_CS
+
+    ! block_tools.f90:127-140: the shape from the diagonal blocks
+    subroutine compute_shape_block_CS(this)
+        class(block_CS), intent(inout) :: this
+        integer :: i
+        this%shape = 0
+        do i = 1, this%block_shape(1)
+            this%shape(1) = this%shape(1) + this%blocks(i, i)%shape(1)
+        end do
+        do i = 1, this%block_shape(2)
+            this%shape(2) = this%shape(2) + this%blocks(i, i)%shape(2)
+        end do
+    end subroutine compute_shape_block_CS
+
+    ! block_tools.f90:386-415: 'CSR', block_shape, shape, then per block
+    ! (j outer, i inner) its shape, nnz and, when nnz > 0, its arrays
+    subroutine CS_block_store(this, loc)
+        class(block_CS), intent(inout) :: this
+        character(len=*), intent(in) :: loc
+        integer :: i, j, u
+        call this%compute_shape()
+        open(file=loc, newunit=u, action='write', form='unformatted')
+        write(u) 'CSR'
+        write(u) this%block_shape
+        write(u) this%shape
+        do j = 1, this%block_shape(2)
+            do i = 1, this%block_shape(1)
+                write(u) this%blocks(i, j)%shape
+                write(u) this%blocks(i, j)%nnz
+                if (this%blocks(i, j)%nnz > 0) then
+                    write(u) this%blocks(i, j)%index_ptr
+                    write(u) this%blocks(i, j)%indices
+                    write(u) this%blocks(i, j)%data
+                end if
+            end do
+        end do
+        close(u)
+    end subroutine CS_block_store
+
+    ! block_tools.f90:417-456
+    subroutine CS_block_load(this, loc)
+        class(block_CS), intent(inout) :: this
+        character(len=*), intent(in) :: loc
+        integer :: i, j, u, nnz, shape(2), block_shape(2)
+        character(len=3) :: kind
+        open(file=loc, newunit=u, action='read', form='unformatted', status='old')
+        read(u) kind
+        if (kind /= 'CSR') error stop "block_CS%load: only CSR blocks are implemented"
+        read(u) block_shape
+        call this%init(block_shape)
+        read(u) this%shape
+        do j = 1, block_shape(2)
+            do i = 1, block_shape(1)
+                read(u) shape
+                read(u) nnz
+                this%blocks(i, j)%shape = shape
+                this%blocks(i, j)%nnz = nnz
+                if (nnz > 0) then
+                    allocate(this%blocks(i, j)%index_ptr(shape(1) + 1), this%blocks(i, j)%indices(nnz), &
+                             this%blocks(i, j)%data(nnz))
+                    read(u) this%blocks(i, j)%index_ptr
+                    read(u) this%blocks(i, j)%indices
+                    read(u) this%blocks(i, j)%data
+                end if
+            end do
+        end do
+        close(u)
+    end subroutine CS_block_load
+
+    ! mat_els.f90:120-170 setup_radial_dip on the device: gauge 'l' fills
+    ! r_mat, 'v' dr_mat and r_inv_mat (:348-390)
+    subroutine setup_radial_dip(b_splines, k_GL, radial_dip, gauge)
+        type(b_spline), intent(in) :: b_splines
+        integer, intent(in) :: k_GL
+        type(radial_dipole), intent(out), target :: radial_dip
+        character, intent(in) :: gauge
+        integer :: nb
+        nb = b_splines%n_b
+        if (gauge == 'l') then
+            allocate(radial_dip%r_mat(nb, nb))
+            call check(bs2e_setup_radial_dip(ctx(), int(b_splines%k, c_int64_t), int(size(b_splines%knots), c_int64_t), &
+                       b_splines%knots, int(k_GL, c_int64_t), 0_c_int64_t, c_loc(radial_dip%r_mat), c_null_ptr), &
+                       "setup_radial_dip")
+        else if (gauge == 'v') then
+            allocate(radial_dip%dr_mat(nb, nb), radial_dip%r_inv_mat(nb, nb))
+            call check(bs2e_setup_radial_dip(ctx(), int(b_splines%k, c_int64_t), int(size(b_splines%knots), c_int64_t), &
+                       b_splines%knots, int(k_GL, c_int64_t), 1_c_int64_t, c_loc(radial_dip%dr_mat), &
+                       c_loc(radial_dip%r_inv_mat)), "setup_radial_dip")
+        else
+            write(6, *) "Error! Unrecognized gauge option in for radial dipole: ", gauge
+            stop
+        end if
+    end subroutine setup_radial_dip
+
+    ! dipole.f90:8-47 construct_dip_block_tensor: the pattern (init_dip_block,
+    ! :95-150) and the values ang * dip_mat_neq on the device.  The gauge is
+    ! the one radial_dip was set up with.
+    subroutine construct_dip_block_tensor(syms, q, b_splines, S, radial_dip, dip_block, compute)
+        type(sym), intent(in) :: syms(2)
+        integer, intent(in) :: q
+        type(b_spline), intent(in) :: b_splines
+        complex(dp), intent(in), target, contiguous :: S(:,:)
+        type(radial_dipole), intent(in), target :: radial_dip
+        type(CSR_matrix), intent(out) :: dip_block
+        logical, intent(in) :: compute
+        integer(c_int64_t), allocatable :: cf1(:), cf2(:)
+        integer(c_int64_t), allocatable, target :: ip(:), ix(:)
+        complex(dp), allocatable, target :: d(:)
+        integer(c_int64_t) :: sizes(3), gauge
+        type(c_ptr) :: blk, r1, r2
+        if (allocated(radial_dip%r_mat)) then
+            gauge = 0
+            r1 = c_loc(radial_dip%r_mat)
+            r2 = c_null_ptr
+        else if (allocated(radial_dip%dr_mat) .and. allocated(radial_dip%r_inv_mat)) then
+            gauge = 1
+            r1 = c_loc(radial_dip%dr_mat)
+            r2 = c_loc(radial_dip%r_inv_mat)
+        else
+            error stop "construct_dip_block_tensor: radial_dip has not been set up"
+        end if
+        call pack_configs(syms(1), cf1)
+        call pack_configs(syms(2), cf2)
+        call check(bs2e_dip_block_create(ctx(), int(b_splines%k, c_int64_t), int(b_splines%n_b, c_int64_t), &
+                   int(syms(1)%l, c_int64_t), int(syms(1)%m, c_int64_t), merge(1_c_int64_t, 0_c_int64_t, syms(1)%pi), &
+                   int(syms(1)%n_config, c_int64_t), cf1, int(syms(2)%l, c_int64_t), int(syms(2)%m, c_int64_t), &
+                   merge(1_c_int64_t, 0_c_int64_t, syms(2)%pi), int(syms(2)%n_config, c_int64_t), cf2, &
+                   int(q, c_int64_t), merge(1_c_int64_t, 0_c_int64_t, compute), blk), "construct_dip_block_tensor")
+        call check(bs2e_dip_block_sizes(blk, sizes, 3_c_int64_t), "construct_dip_block_tensor")
+        dip_block%shape = [syms(1)%n_config, syms(2)%n_config]
+        dip_block%nnz = int(sizes(1))
+        if (sizes(1) > 0) then
+            allocate(ip(syms(1)%n_config + 1), ix(sizes(1)), d(sizes(1)))
+            call check(bs2e_dip_block_fill(blk, gauge, c_loc(S), r1, r2, c_loc(ip), c_loc(ix), c_loc(d)), &
+                       "construct_dip_block_tensor")
+            dip_block%index_ptr = int(ip)
+            dip_block%indices = int(ix)
+            dip_block%data = d
+        end if
+        call check(bs2e_dip_block_release(blk), "construct_dip_block_tensor")
+    end subroutine construct_dip_block_tensor
+
+    subroutine pack_configs(term, cf)
+        type(sym), intent(in) :: term
+        integer(c_int64_t), allocatable, intent(out) :: cf(:)
+        integer :: c
+        allocate(cf(5 * max(term%n_config, 1)))
+        cf = 0
+        do c = 1, term%n_config
+            cf(5 * (c - 1) + 1:5 * (c - 1) + 2) = int(term%configs(c)%n, c_int64_t)
+            cf(5 * (c - 1) + 3:5 * (c - 1) + 4) = int(term%configs(c)%l, c_int64_t)
+            cf(5 * c) = merge(1_c_int64_t, 0_c_int64_t, term%configs(c)%eqv)
+        end do
+    end subroutine pack_configs
 
     ! mat_els.f90:85-118 on the device
     subroutine setup_S(b_splines, k_GL, S)

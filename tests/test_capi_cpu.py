@@ -135,3 +135,42 @@ def test_fortran_records_round_trip(tmp_path, monkeypatch):
     # first record 'CSR': 3 bytes in one subrecord; a 5x2 complex data record (160 bytes) is split
     assert raw[:4] == (3).to_bytes(4, "little") and raw[7:11] == (3).to_bytes(4, "little")
     assert (-24).to_bytes(4, "little", signed=True) in raw
+
+
+def test_block_CS_round_trip(tmp_path):
+    """block_tools.f90:386-456: a 3 x 3 matrix of CSR blocks (j outer, i
+    inner), empty blocks carrying only shape and nnz; the stored shape is the
+    sum over the diagonal blocks (compute_shape, :127-140, reads blocks(i,i)
+    for both dimensions, so the block matrix is square)."""
+    from bs2e import CSR_matrix
+    from bs2e import fortran_io as fio
+    rng = np.random.default_rng(5)
+    rows, cols = (3, 4, 2), (3, 5, 2)
+    blocks = []
+    for i, n in enumerate(rows):
+        row = []
+        for j, m in enumerate(cols):
+            if (i + j) % 2 == 0:
+                row.append(CSR_matrix((n, m), np.ones(n + 1, np.int64), np.zeros(0, np.int64),
+                                      np.zeros(0, np.complex128)))
+                continue
+            cnt = rng.integers(0, m + 1, n)
+            ip = np.concatenate([[1], 1 + np.cumsum(cnt)]).astype(np.int64)
+            ix = np.concatenate([np.sort(rng.choice(m, c, replace=False)) + 1 for c in cnt]).astype(np.int64)
+            d = rng.normal(size=ix.size) + 1j * rng.normal(size=ix.size)
+            row.append(CSR_matrix((n, m), ip, ix, d))
+        blocks.append(row)
+    p = tmp_path / "D_0.dat"
+    fio.store_block_CS(p, blocks)
+    got = fio.load_block_CS(p)
+    for i in range(3):
+        for j in range(3):
+            a, b = got[i][j], blocks[i][j]
+            assert a.shape == b.shape and a.nnz == b.nnz
+            assert np.array_equal(a.index_ptr, b.index_ptr) and np.array_equal(a.indices, b.indices)
+            assert np.array_equal(a.data, b.data)
+    raw = p.read_bytes()
+    # header records: 'CSR', block_shape (3, 3), shape (3+4+2, 3+5+2)
+    assert raw[:11] == (3).to_bytes(4, "little") + b"CSR" + (3).to_bytes(4, "little")
+    hdr = np.frombuffer(raw[15:31], "<i8")
+    assert hdr.tolist() == [3, 3] and np.frombuffer(raw[39:55], "<i8").tolist() == [9, 10]

@@ -106,21 +106,23 @@ def test_fortran_basis_setup_matches_oracle(oracle, tmp_path, L, pi, full, hs_on
     assert s.pos == len(s.raw)
 
 
-def test_fortran_basis_files_match_reference_format(oracle, tmp_path):
-    """The driver stores H_diag.dat, S_diag.dat, basis.dat, basis_info.txt and
-    splines.dat for every symmetry L = 0..2 (as main_basis_setup.f90:154-161
-    does); their contents equal the oracle's blocks, and the Python writer of
-    the same format reproduces the amdflang-written files byte for byte."""
+@pytest.mark.parametrize("gauge,full", [("l", 1), ("v", 0)])
+def test_fortran_basis_files_match_reference_format(oracle, tmp_path, gauge, full):
+    """The driver stores H_diag.dat, S_diag.dat, D_-1.dat, D_0.dat, D_1.dat,
+    basis.dat, basis_info.txt and splines.dat for every symmetry L = 0..2 (as
+    main_basis_setup.f90:122-161 does); their contents equal the oracle's
+    blocks, and the Python writer of the same format reproduces the
+    amdflang-written files byte for byte."""
     from bs2e import fortran_io as fio
     if not os.path.exists(EXE):
         pytest.fail(f"{EXE} is missing")
-    w = WORKLOADS["c1"]
+    w = WORKLOADS["t0"]
     max_l, max_L = 2, 2
     t = oracle.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
     _, n_b, _ = oracle.dims(w.k, t)
     res = str(tmp_path) + "/"
     cmd = [EXE, str(w.k), str(w.m), str(w.Z), repr(w.h_max), repr(w.r_max), str(w.k_GL), str(w.max_k), str(max_l),
-           "0", "0", "1", "50", "device", str(tmp_path / "out.bin"), str(max_L), res]
+           "0", "0", str(full), "50", "device", str(tmp_path / "out.bin"), str(max_L), res, gauge]
     print("run:", " ".join(cmd), flush=True)
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=180)
     print(r.stdout)
@@ -141,18 +143,42 @@ def test_fortran_basis_files_match_reference_format(oracle, tmp_path):
         assert (s.l, s.m, s.pi) == (L, 0, bool(L % 2))
         cf = oracle.count_configs(L, L % 2, max_l, n_b, w.k, n_b, n_b, max_l)
         assert np.array_equal(s.configs, cf)
-        ref = oracle.construct_block(w.k, n_b, L, w.max_k, True, cf, R, Hl, Sm)
+        ref = oracle.construct_block(w.k, n_b, L, w.max_k, bool(full), cf, R, Hl, Sm)
         for got, key in ((H[L], "H"), (S[L], "S")):
             rip, rix, rd = ref[key]
             assert got.shape == (len(cf), len(cf))
             assert np.array_equal(got.index_ptr, rip) and np.array_equal(got.indices, rix), (L, key)
             assert np.abs(got.data - rd).max() <= TOL * np.abs(rd).max(), (L, key)
+    # dipoles: every (i, j) pair for q = -1, 0, 1 (z-polarised: only q = 0
+    # reaches M = 0 -> M = 0); upper triangle only when full = 0
+    rad = oracle.setup_radial_dip(w.k, t, w.k_GL, gauge)
+    D = {q: fio.load_block_CS(res + f"D_{q}.dat") for q in (-1, 0, 1)}
+    n_nonempty = 0
+    for q in (-1, 0, 1):
+        assert len(D[q]) == max_L + 1 and all(len(r) == max_L + 1 for r in D[q])
+        for i in range(max_L + 1):
+            for j in range(max_L + 1):
+                got = D[q][i][j]
+                s1, s2 = syms[i], syms[j]
+                assert got.shape == (s1.n_config, s2.n_config)
+                ip, ix, d = oracle.construct_dip_block(w.k, n_b, (s1.l, s1.m, s1.pi), s1.configs,
+                                                       (s2.l, s2.m, s2.pi), s2.configs, q, bool(full) or i <= j,
+                                                       gauge, Sm, rad)
+                assert got.nnz == len(ix), (q, i, j)
+                if got.nnz:
+                    n_nonempty += 1
+                    assert np.array_equal(got.index_ptr, ip) and np.array_equal(got.indices, ix), (q, i, j)
+                    assert np.abs(got.data - d).max() <= TOL * np.abs(d).max(), (q, i, j)
+    assert n_nonempty == (4 if full else 2)
     # the Python writer reproduces the files byte for byte
     py = tmp_path / "py"
     py.mkdir()
     fio.store_block_diag(py / "H_diag.dat", H)
     fio.store_block_diag(py / "S_diag.dat", S)
+    for q in (-1, 0, 1):
+        fio.store_block_CS(py / f"D_{q}.dat", D[q])
     fio.store_basis(str(py), max_l_1p, mL, two_el, syms)
     fio.store_splines(str(py), k, knots)
-    for name in ("H_diag.dat", "S_diag.dat", "basis.dat", "basis_info.txt", "splines.dat"):
+    for name in ("H_diag.dat", "S_diag.dat", "D_-1.dat", "D_0.dat", "D_1.dat", "basis.dat", "basis_info.txt",
+                 "splines.dat"):
         assert (py / name).read_bytes() == (tmp_path / name).read_bytes(), name
