@@ -169,6 +169,34 @@ def blocks_benchmark(rk, w, t, max_l=2, L=0):
             "wall_ms_with_transfers": wall * 1e3}
 
 
+def dipoles_benchmark(ctx, w, t, max_l=2, L1=0, gauge="l"):
+    """Dipole block between symmetries (L1, even) and (L1+1, odd), q = 0
+    (dipole.f90:8-47): radial dipole matrices, pattern count and fill on the
+    device; kernel times from the library's HIP events, wall time including
+    the host transfers."""
+    import bs2e
+    spl = bs2e.b_spline(w.k, t)
+    S = bs2e.setup_S(spl, w.k_GL, ctx=ctx)
+    rad = bs2e.setup_radial_dip(spl, w.k_GL, gauge, ctx=ctx)
+    syms = []
+    for L in (L1, L1 + 1):
+        s = bs2e.sym(l=L, pi=bool(L % 2))
+        s.configs = bs2e.count_configs(s, max_l, spl.n_b, w.k, spl.n_b, spl.n_b, max_l)
+        syms.append(s)
+    bs2e.DipBlock(syms, 0, spl, ctx=ctx).release()   # warm-up
+    t0 = time.perf_counter()
+    blk = bs2e.DipBlock(syms, 0, spl, ctx=ctx)
+    D = blk.fill(S, rad)
+    wall = time.perf_counter() - t0
+    ms_count, ms_fill = blk.kernel_ms()
+    blk.release()
+    written = 24.0 * D.nnz                            # 1-based column index + complex value per entry
+    return {"workload": f"{w.name} basis, L={L1} -> L={L1 + 1} dipole block, q=0, gauge {gauge}, l1,l2 <= {max_l}",
+            "shape": list(D.shape), "nnz": D.nnz, "count_kernel_ms": ms_count, "fill_kernel_ms": ms_fill,
+            "entries_per_s": D.nnz / (ms_fill * 1e-3), "fill_GBs": written / (ms_fill * 1e-3) / 1e9,
+            "wall_ms_with_transfers": wall * 1e3}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +207,7 @@ def main():
     ap.add_argument("--latency-samples", type=int, default=50,
                     help="synchronised single builds timed for p50/p99 latency")
     ap.add_argument("--blocks", type=int, default=1, help="also time the L=0 symmetry block (0 skips it)")
+    ap.add_argument("--dipoles", type=int, default=1, help="also time the L=0 -> L=1 dipole block (0 skips it)")
     ap.add_argument("--serve-jobs", type=int, default=256,
                     help="jobs of the serving stream (differing bases through the build service); 0 skips it")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -305,6 +334,8 @@ def main():
     }
     if args.blocks:
         out["blocks"] = blocks_benchmark(rk, w, t)
+    if args.dipoles:
+        out["dipoles"] = dipoles_benchmark(rk.ctx, w, t)
     if args.serve_jobs > 0:
         out["serving"] = serving_benchmark(args.serve_jobs)
     if not args.no_cpu_baseline:
