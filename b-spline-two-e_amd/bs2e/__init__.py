@@ -32,7 +32,7 @@ from .capi import Bs2eError, check, ptr
 __all__ = ["Bs2eError", "Context", "generate_grid", "b_spline", "sparse_4d", "sparse_6d", "setup_Slater_integrals",
            "Nd_DOK", "RkTensor", "compute_R_K_map", "ang_k_LS", "three_j", "six_j", "count_nnz", "sym",
            "count_configs", "CSR_matrix", "Block", "construct_block_tensor", "Server", "footprint", "setup_S",
-           "setup_H_one_particle", "radial_dipole", "setup_radial_dip", "DipBlock", "construct_dip_block_tensor"]
+           "setup_H_one_particle", "radial_dipole", "setup_radial_dip", "DipBlock", "construct_dip_block_tensor", "setup_r_k_int"]
 
 
 class Context:
@@ -537,6 +537,19 @@ def construct_block_tensor(H, S, b_splines, term: sym, max_k, R_k, full=True):
         return blk.fill(H, S)
     finally:
         blk.release()
+
+
+def setup_r_k_int(b_splines, k_GL, max_k, r_0, ctx: Context | None = None) -> np.ndarray:
+    """RKH_hamiltonian.f90:91-159 setup_r_k_int on the device: the array
+    r_k_int[k, i, j] (k = 0..max_k; real) for the multipole expansion about
+    r_0 (adiabatic_potentials.f90:24-34 r_k)."""
+    ctx = ctx or default_context()
+    t = b_splines.knots
+    n_b = b_splines.n_b
+    out = np.zeros((max_k + 1) * n_b * n_b, np.float64)
+    check(capi.lib().bs2e_setup_r_k_int(ctx.handle, b_splines.k, len(t), ptr(t), k_GL, int(max_k), float(r_0),
+                                        ptr(out)), ctx.handle)
+    return out.reshape(max_k + 1, n_b, n_b, order="F")
 
 
 # --------------------------------------------------------------------- dipoles

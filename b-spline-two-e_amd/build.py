@@ -6,6 +6,7 @@ Produces, in-tree (git-ignored, shipped with the tree to the GPU host):
     b-spline-two-e_amd/lib/libbs2e.so           HIP kernels + C ABI (gfx950)
     b-spline-two-e_amd/lib/libbs2e_fortran.so   Fortran drop-in modules (amdflang)
     b-spline-two-e_amd/lib/bs2e_basis_setup     Fortran driver over the drop-in
+    b-spline-two-e_amd/lib/bs2e_rkh_multipoles  Fortran RKH multipole step over the drop-in
 """
 from __future__ import annotations
 
@@ -74,6 +75,9 @@ def build_fortran(verbose: bool = False) -> str | None:
     _run([FC, *flags, "-shared", srcs[0], srcs[1], "-o", so, f"-L{LIB}", "-lbs2e", rpath])
     exe = os.path.join(LIB, "bs2e_basis_setup")
     _run([FC, *flags, srcs[2], "-o", exe, f"-L{LIB}", "-lbs2e_fortran", "-lbs2e", rpath])
+    rkh = os.path.join(FSRC, "bs2e_rkh_multipoles.f90")
+    _run([FC, *flags, rkh, "-o", os.path.join(LIB, "bs2e_rkh_multipoles"), f"-L{LIB}", "-lbs2e_fortran", "-lbs2e",
+          rpath])
     return so
 
 

@@ -129,6 +129,9 @@ struct BlockArgs {
 // One-electron matrices: spline value / second-derivative tables at the GL
 // points (B, B2: [n_iv][k_gl][k]; rw: [n_iv][k_gl][2] r and w) and the
 // assembly of S (optional) and H for l = l_min .. l_min+n_l-1.
+// RKH one-electron multipoles r_k_int(0:max_k, n_b, n_b), real, k fastest
+hipError_t launch_rk_int(const Plan& p, const DevTables& d, double* B, double* B2, double* rw, int max_k, double r0,
+                         double* out, hipStream_t s);
 hipError_t launch_one_body(const Plan& p, const DevTables& d, double* B, double* B2, double* rw, int l_min, int n_l,
                            int Z, int cap_order, double cap_r0, double eta_re, double eta_im, double* S, double* H,
                            hipStream_t s);

@@ -14,6 +14,13 @@
 //       H(i,j) += w (-0.5 B_i B_j'' + (V_l(r) + V_CAP(r)) B_i B_j)
 //     with V_l(r) = 0.5 l(l+1)/r^2 - Z/r and V_CAP(r) = -i eta (r - r_0)^order
 //     for r >= r_0 (0 below).  Elements with |i_b - j_b| >= k are zero.
+//
+//   ko_rk_int: one thread per (k, i_b, j_b) of the RKH one-electron
+//     multipoles (RKH_hamiltonian.f90:91-159 setup_r_k_int with
+//     adiabatic_potentials.f90:24-34 r_k): per interval holding both splines,
+//     in ascending order, the GL sum sum_q w r_k(r_q) B_i B_j, added to the
+//     running total; r_k(r) = r_0^k / r^(k+1) for r > r_0 and
+//     r^k / r_0^(k+1) otherwise, integer powers by squaring.
 #include "bs2e_internal.h"
 #include "bspline_device.h"
 
@@ -99,6 +106,38 @@ __global__ __launch_bounds__(256) void ko_assemble(OneBodyArgs A)
     }
 }
 
+__global__ __launch_bounds__(256) void ko_rk_int(int n_b, int n_iv, int ks, int kgl, int K, double r0,
+                                                 const double* __restrict__ B, const double* __restrict__ rw,
+                                                 double* __restrict__ out)
+{
+    const int64_t n = (int64_t)K * n_b * n_b;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int k = (int)(e % K);
+    const int64_t ij = e / K;
+    const int jb = (int)(ij / n_b) + 1, ib = (int)(ij - (int64_t)(jb - 1) * n_b) + 1;
+    double acc = 0.0;
+    const int d = ib > jb ? ib - jb : jb - ib;
+    if (d < ks) {
+        const int mx = ib > jb ? ib : jb, mn = ib < jb ? ib : jb;
+        const int lo = (mx + 2 - ks) > 1 ? (mx + 2 - ks) : 1;
+        const int hiv = (mn + 1) < n_iv ? (mn + 1) : n_iv;
+        const double r0k = ipow_d(r0, k), r0k1 = ipow_d(r0, k + 1);
+        for (int iv = lo; iv <= hiv; ++iv) {
+            const int ta = ib + 1 - iv, tc = jb + 1 - iv;
+            double res = 0.0;
+            for (int q = 0; q < kgl; ++q) {
+                const size_t o = (size_t)(iv - 1) * kgl + q;
+                const double r = rw[2 * o], wq = rw[2 * o + 1];
+                const double rk = r > r0 ? r0k / ipow_d(r, k + 1) : ipow_d(r, k) / r0k1;
+                res = __dadd_rn(res, __dmul_rn(__dmul_rn(__dmul_rn(wq, rk), B[o * ks + ta]), B[o * ks + tc]));
+            }
+            acc = __dadd_rn(acc, res);
+        }
+    }
+    out[e] = acc;
+}
+
 template <int KS>
 static void launch_tables(const Plan& p, const DevTables& d, double* B, double* B2, double* rw, hipStream_t s)
 {
@@ -134,6 +173,35 @@ hipError_t launch_one_body(const Plan& p, const DevTables& d, double* B, double*
                   B, B2, rw, S, H};
     const int64_t n = (int64_t)p.n_b * p.n_b * n_l;
     hipLaunchKernelGGL(ko_assemble, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_rk_int(const Plan& p, const DevTables& d, double* B, double* B2, double* rw, int max_k, double r0,
+                         double* out, hipStream_t s)
+{
+    switch (p.k_s) {
+        case 2: launch_tables<2>(p, d, B, B2, rw, s); break;
+        case 3: launch_tables<3>(p, d, B, B2, rw, s); break;
+        case 4: launch_tables<4>(p, d, B, B2, rw, s); break;
+        case 5: launch_tables<5>(p, d, B, B2, rw, s); break;
+        case 6: launch_tables<6>(p, d, B, B2, rw, s); break;
+        case 7: launch_tables<7>(p, d, B, B2, rw, s); break;
+        case 8: launch_tables<8>(p, d, B, B2, rw, s); break;
+        case 9: launch_tables<9>(p, d, B, B2, rw, s); break;
+        case 10: launch_tables<10>(p, d, B, B2, rw, s); break;
+        case 11: launch_tables<11>(p, d, B, B2, rw, s); break;
+        case 12: launch_tables<12>(p, d, B, B2, rw, s); break;
+        case 13: launch_tables<13>(p, d, B, B2, rw, s); break;
+        case 14: launch_tables<14>(p, d, B, B2, rw, s); break;
+        case 15: launch_tables<15>(p, d, B, B2, rw, s); break;
+        case 16: launch_tables<16>(p, d, B, B2, rw, s); break;
+        default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int64_t n = (int64_t)(max_k + 1) * p.n_b * p.n_b;
+    hipLaunchKernelGGL(ko_rk_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int)p.n_b, (int)p.n_iv,
+                       (int)p.k_s, (int)p.k_gl, max_k + 1, r0, B, rw, out);
     return hipGetLastError();
 }
 

@@ -17,7 +17,7 @@ module bs2e_capi
     public :: bs2e_ang_k_LS, bs2e_count_configs, bs2e_block_create, bs2e_block_sizes, bs2e_block_fill
     public :: bs2e_block_release, c_string, bs2e_setup_S, bs2e_setup_H_one_particle
     public :: bs2e_setup_radial_dip, bs2e_dip_block_create, bs2e_dip_block_sizes, bs2e_dip_block_fill
-    public :: bs2e_dip_block_release
+    public :: bs2e_dip_block_release, bs2e_setup_r_k_int
 
     interface
         function bs2e_create(ctx, device) bind(C, name="bs2e_create") result(rc)
@@ -198,6 +198,17 @@ module bs2e_capi
         function bs2e_block_release(blk) bind(C, name="bs2e_block_release") result(rc)
             import :: c_ptr, c_int64_t
             type(c_ptr), value :: blk
+            integer(c_int64_t) :: rc
+        end function
+
+        function bs2e_setup_r_k_int(ctx, k, n_knots, knots, k_gl, max_k, r_0, out) &
+                bind(C, name="bs2e_setup_r_k_int") result(rc)
+            import :: c_ptr, c_int64_t, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), value :: k, n_knots, k_gl, max_k
+            real(c_double), value :: r_0
+            real(c_double), intent(in) :: knots(*)
+            real(c_double), intent(out) :: out(*)
             integer(c_int64_t) :: rc
         end function
 

@@ -44,7 +44,7 @@ module bs2e_mat_els
     public :: generate_grid, setup_Slater_integrals, compute_R_k_map, compute_R_k_direct
     public :: count_configs, construct_block_tensor, ang_k_LS, bs2e_use_device, bs2e_shutdown
     public :: hydrogenic, CAP, setup_S, setup_H_one_particle, basis, block_diag_CS
-    public :: block_CS, radial_dipole, setup_radial_dip, construct_dip_block_tensor
+    public :: block_CS, radial_dipole, setup_radial_dip, construct_dip_block_tensor, rkh_multipoles
 
     type(c_ptr), save :: g_ctx = c_null_ptr
     integer, save :: g_device = 0
@@ -621,6 +621,19 @@ contains
         end do
     end subroutine pack_configs
 
+    ! RKH_hamiltonian.f90:91-159 setup_r_k_int on the device, with the
+    ! reference's module variables (k_GL of init_g_l, max_k, r_0) as arguments
+    subroutine rkh_multipoles(b_splines, k_GL, max_k, r_0, r_k_int)
+        type(b_spline), intent(in) :: b_splines
+        integer, intent(in) :: k_GL, max_k
+        real(dp), intent(in) :: r_0
+        real(dp), intent(out) :: r_k_int(0:, :, :)
+        if (size(r_k_int, 1) /= max_k + 1 .or. size(r_k_int, 2) /= b_splines%n_b .or. &
+            size(r_k_int, 3) /= b_splines%n_b) error stop "rkh_multipoles: r_k_int must be (0:max_k, n_b, n_b)"
+        call check(bs2e_setup_r_k_int(ctx(), int(b_splines%k, c_int64_t), int(size(b_splines%knots), c_int64_t), &
+                   b_splines%knots, int(k_GL, c_int64_t), int(max_k, c_int64_t), r_0, r_k_int), "setup_r_k_int")
+    end subroutine rkh_multipoles
+
     ! mat_els.f90:85-118 on the device
     subroutine setup_S(b_splines, k_GL, S)
         type(b_spline), intent(in) :: b_splines
@@ -875,3 +888,39 @@ contains
     end subroutine construct_block_tensor
 
 end module bs2e_mat_els
+
+
+! The RKH frame's one-electron multipoles (RKH_hamiltonian.f90:14-28, 91-159)
+! with the module state the reference keeps: r_0 and max_k (set by
+! adiabatic_potentials.f90:127,138 setup_U_r), the GL order of init_g_l and
+! the result r_k_int(0:max_k, n_b, n_b).
+module bs2e_rkh
+    use bs2e_mat_els, only: b_spline, rkh_multipoles
+    implicit none
+    private
+    integer, parameter :: dp = kind(1.d0)
+    public :: r_0, max_k, r_k_int, init_g_l, setup_r_k_int
+
+    real(dp) :: r_0 = 0.0_dp
+    integer :: max_k = 0
+    real(dp), allocatable :: r_k_int(:,:,:)
+    integer :: k_GL_rkh = 0
+
+contains
+
+    subroutine init_g_l(b_splines, k_GL)
+        type(b_spline), intent(in) :: b_splines
+        integer, intent(in) :: k_GL
+        if (b_splines%n_b < 1) error stop "init_g_l: b_splines is not initialised"
+        k_GL_rkh = k_GL
+    end subroutine init_g_l
+
+    subroutine setup_r_k_int(b_splines)
+        type(b_spline), intent(in) :: b_splines
+        if (k_GL_rkh < 1) error stop "setup_r_k_int: call init_g_l first"
+        if (allocated(r_k_int)) deallocate(r_k_int)
+        allocate(r_k_int(0:max_k, b_splines%n_b, b_splines%n_b))
+        call rkh_multipoles(b_splines, k_GL_rkh, max_k, r_0, r_k_int)
+    end subroutine setup_r_k_int
+
+end module bs2e_rkh

@@ -198,6 +198,17 @@ int64_t bs2e_block_release(bs2e_block *b);
  * kernel (ms[0], at create) and fill kernel (ms[1], at the last fill). */
 int64_t bs2e_block_times(const bs2e_block *b, double *ms, int64_t n);
 
+/* RKH_hamiltonian.f90:91-159 setup_r_k_int(b_splines) (with init_g_l,
+ * :23-28, for k_GL, and adiabatic_potentials.f90:24-34 r_k): the one-electron
+ * multipoles of the Kramers-Henneberger frame, r_k_int(k, i, j) = sum over
+ * the intervals holding both splines and their GL points of
+ * w r_k(r; r_0) B_i B_j with r_k = r_0^k / r^(k+1) for r > r_0 and
+ * r^k / r_0^(k+1) otherwise (the reference's module variables r_0 =
+ * E_0 / omega^2 and max_k = 2 max_l_1p become arguments).  r_k_int is real,
+ * (max_k+1) x n_b x n_b with k fastest, as r_k_int(0:max_k, n_b, n_b). */
+int64_t bs2e_setup_r_k_int(bs2e_ctx *ctx, int64_t k, int64_t n_knots, const double *knots, int64_t k_gl,
+                           int64_t max_k, double r_0, double *r_k_int);
+
 /* ---- dipole operators -------------------------------------------------- */
 /* mat_els.f90:123-170 setup_radial_dip(b_splines, k_GL, gauge, radial_dip)
  * with :348-370 compute_radial_dip_len / :372-390 compute_radial_dip_vel:

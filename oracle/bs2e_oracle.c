@@ -27,6 +27,14 @@
  *   or_count_configs                    bit-exact vs compiled reference
  *   or_construct_block                  brute-force predicate restatement on
  *                                       tiny bases + He eigenvalues
+ *   or_setup_S, or_setup_H1p            hydrogenic eigenvalues -Z^2/(2n^2)
+ *   or_setup_radial_dip                 hydrogenic <1s|r|2p> closed form,
+ *                                       commutator relation of the two gauges
+ *   or_construct_dip_block              adjoint relation between (1,2,q) and
+ *                                       (2,1,-q) blocks + He oscillator strength
+ *   or_setup_r_k_int                    r_0 beyond / below the knots: the P^k
+ *                                       and Q^k moments; <1s|1/max(r,r_0)|1s>
+ *                                       closed form
  */
 #include <complex.h>
 #include <math.h>
@@ -1023,4 +1031,46 @@ i64 or_construct_dip_block(i64 k_s, i64 n_b, i64 L1, i64 M1, i64 pi1, i64 n1, co
         if (ip) ip[i + 1] = ptr + 1;
     }
     return ptr;
+}
+
+/* ------------------------------------------------------------------------ */
+/* RKH_hamiltonian.f90:91-159 setup_r_k_int / compute_r_k_int /             */
+/* r_k_integral with adiabatic_potentials.f90:24-34 r_k(r, r_0, k):          */
+/*   r_k_int(k,i,j) = sum over the intervals holding both splines of         */
+/*                    ( sum_q w r_k(x_q) B_i B_j )                            */
+/*   r_k(r) = r_0^k / r^(k+1) for r > r_0, r^k / r_0^(k+1) otherwise.       */
+/* out: (max_k+1) x n_b x n_b, k fastest (r_k_int(0:max_k, n_b, n_b)).       */
+/* ------------------------------------------------------------------------ */
+void or_setup_r_k_int(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i64 max_k, double r_0, double *out)
+{
+    i64 n, n_b, n_iv;
+    or_spline_dims(k_s, n_knots, &n, &n_b, &n_iv);
+    const double *bp = t + (k_s - 1);
+    const i64 K = max_k + 1;
+    double *x0 = malloc(sizeof(double) * (size_t)k_gl), *w0 = malloc(sizeof(double) * (size_t)k_gl);
+    or_gauss_legendre(k_gl, x0, w0);
+    for (i64 e = 0; e < K * n_b * n_b; ++e) out[e] = 0.0;
+    for (i64 j_b = 1; j_b <= n_b; ++j_b)
+        for (i64 i_b = 1; i_b <= n_b; ++i_b) {
+            if (llabs(j_b - i_b) >= k_s) continue;
+            double *o = out + K * ((j_b - 1) * n_b + (i_b - 1));
+            for (i64 i_r = 1; i_r <= n_iv; ++i_r) {
+                if (!(supp(k_s, i_r, i_b + 1) && supp(k_s, i_r, j_b + 1))) continue;
+                double a = bp[i_r - 1], b = bp[i_r];
+                for (i64 k = 0; k <= max_k; ++k) {
+                    double res = 0.0;
+                    for (i64 q = 0; q < k_gl; ++q) {
+                        double r = (b - a) / 2.0 * x0[q] + (a + b) / 2.0;
+                        double wq = (b - a) / 2.0 * w0[q];
+                        double Bi = or_bvalue_unit(t, n, k_s, i_b + 1, r, 0, i_r);
+                        double Bj = or_bvalue_unit(t, n, k_s, j_b + 1, r, 0, i_r);
+                        double rk = r > r_0 ? ipow(r_0, k) / ipow(r, k + 1) : ipow(r, k) / ipow(r_0, k + 1);
+                        res = res + wq * rk * Bi * Bj;
+                    }
+                    o[k] = o[k] + (0.0 + res);
+                }
+            }
+        }
+    free(x0);
+    free(w0);
 }

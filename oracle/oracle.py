@@ -94,6 +94,8 @@ def lib():
                                          P(i64)]
         L.or_setup_radial_dip.restype = None
         L.or_setup_radial_dip.argtypes = [i64, i64, P(dbl), i64, i64, P(dbl), P(dbl), P(dbl)]
+        L.or_setup_r_k_int.restype = None
+        L.or_setup_r_k_int.argtypes = [i64, i64, P(dbl), i64, i64, dbl, P(dbl)]
         L.or_construct_dip_block.restype = i64
         L.or_construct_dip_block.argtypes = [i64, i64, i64, i64, i64, i64, P(i64), i64, i64, i64, i64, P(i64), i64,
                                              i64, i64, P(dbl), P(dbl), P(dbl), P(dbl), P(i64), P(i64), P(dbl), i64]
@@ -409,3 +411,13 @@ def construct_dip_block(k, n_b, sym1, cf1, sym2, cf2, q, compute, gauge, S, rad)
     n2 = Lb.or_construct_dip_block(*args, _ptr(ip, i64), _ptr(ix, i64), d.ctypes.data_as(P(dbl)), max(nnz, 1))
     assert n2 == nnz
     return ip, ix[:nnz], d[:nnz]
+
+
+def setup_r_k_int(k, t, k_gl, max_k, r_0):
+    """RKH_hamiltonian.f90:91-159: r_k_int(0:max_k, n_b, n_b) as an array
+    [k, i, j] (Fortran order: k fastest)."""
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    _, n_b, _ = dims(k, t)
+    out = np.zeros((max_k + 1) * n_b * n_b, np.float64)
+    lib().or_setup_r_k_int(k, len(t), _ptr(t, dbl), k_gl, max_k, float(r_0), _ptr(out, dbl))
+    return out.reshape(max_k + 1, n_b, n_b, order="F")

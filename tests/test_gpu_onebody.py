@@ -51,3 +51,59 @@ def test_hydrogenic_levels_from_device_matrices():
         E = sl.eigh(H, S, eigvals_only=True)
         for i, n in enumerate(range(l + 1, l + 3)):   # levels well inside the r <= 50 box
             assert abs(E[i] + 4.0 / (2 * n * n)) < 1e-9, (l, n, E[i])
+
+
+@pytest.mark.parametrize("name,max_k,r0_of", [("tiny", 2, "mid"), ("t0", 4, "knot"), ("c1", 4, "mid"),
+                                               ("c2", 10, "mid"), ("c1", 8, "far"), ("c1", 4, "near")])
+def test_r_k_int_matches_oracle(oracle, name, max_k, r0_of):
+    """RKH_hamiltonian.f90:91-159 setup_r_k_int on the device against the
+    oracle, per multipole k within 1e-12 of that k's largest element; r_0
+    inside the grid (between or on breakpoints), beyond it and below its
+    first non-zero knot."""
+    w = WORKLOADS[name]
+    t = bs2e.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+    spl = bs2e.b_spline(w.k, t)
+    r0 = {"mid": 0.37 * t[-1], "knot": t[w.k + 9], "far": 3.0 * t[-1], "near": 1e-3 * t[w.k]}[r0_of]
+    print(f"r_k_int {name} max_k={max_k} r_0={r0}", flush=True)
+    got = bs2e.setup_r_k_int(spl, w.k_GL, max_k, r0)
+    ref = oracle.setup_r_k_int(w.k, t, w.k_GL, max_k, r0)
+    assert got.shape == ref.shape == (max_k + 1, spl.n_b, spl.n_b)
+    for k in range(max_k + 1):
+        assert normwise(got[k], ref[k]) <= TOL, k
+    i, j = np.indices((spl.n_b, spl.n_b))
+    assert np.all(got[:, np.abs(i - j) >= w.k] == 0)
+
+
+def test_r_k_int_rejects_bad_input():
+    w = WORKLOADS["tiny"]
+    spl = bs2e.b_spline(w.k, bs2e.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max))
+    for max_k, r0 in ((-1, 1.0), (2, 0.0), (2, float("nan")), (2, -3.0)):
+        with pytest.raises(bs2e.Bs2eError):
+            bs2e.setup_r_k_int(spl, w.k_GL, max_k, r0)
+
+
+def test_fortran_rkh_multipoles(oracle, tmp_path):
+    """The amdflang program over the drop-in module bs2e_rkh (init_g_l,
+    setup_r_k_int with the module variables r_0 and max_k) writes the same
+    multipoles as the oracle."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(bs2e.__file__), "..", "lib", "bs2e_rkh_multipoles")
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} is missing: build with `python b-spline-two-e_amd/build.py` (needs amdflang)")
+    w = WORKLOADS["c1"]
+    r0 = 7.25
+    out = tmp_path / "rk.bin"
+    cmd = [exe, str(w.k), str(w.m), str(w.Z), repr(w.h_max), repr(w.r_max), str(w.k_GL), "6", repr(r0), str(out)]
+    print("run:", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout
+    raw = out.read_bytes()
+    n_b, max_k = np.frombuffer(raw[:16], np.int64)
+    got = np.frombuffer(raw[16:], np.float64).reshape(max_k + 1, n_b, n_b, order="F")
+    t = oracle.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+    ref = oracle.setup_r_k_int(w.k, t, w.k_GL, 6, r0)
+    assert max_k == 6 and got.shape == ref.shape
+    for k in range(7):
+        assert normwise(got[k], ref[k]) <= TOL, k

@@ -415,3 +415,47 @@ def test_helium_oscillator_strength(oracle, c1_tensor, c1_one_electron):
         f[g] = 2 * dE * abs(m) ** 2 if g == "l" else 2 * abs(m) ** 2 / dE
     assert abs(f["l"] - 0.27616) < 0.01 * 0.27616 and abs(f["v"] - 0.27616) < 0.01 * 0.27616
     assert abs(f["l"] - f["v"]) < 0.002 * f["l"]
+
+
+# ------------------------------------------------------------------ RKH multipoles
+def test_r_k_int_limits_and_closed_form(oracle):
+    """RKH_hamiltonian.f90:91-159 setup_r_k_int.  With r_0 beyond the last
+    knot every point has r < r_0, so r_k_int(k) is the interval sum of the
+    P^k moments over r_0^(k+1); with r_0 below the first non-zero knot it is
+    the Q^k moments times r_0^k (both from the independently pinned
+    setup_Slater_off_diag restatement).  With r_0 on a breakpoint, the
+    hydrogenic 1s orbital of Z = 2 gives the closed form
+    <1s| 1/max(r, r_0) |1s> = [1 - e^-x (1 + x + x^2/2)] / r_0 + Z e^-x (1 + x),
+    x = 2 Z r_0."""
+    import scipy.linalg as sl
+    from bs2e.workloads import WORKLOADS
+    w = WORKLOADS["c1"]
+    t = oracle.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+    _, n_b, _ = oracle.dims(w.k, t)
+    K = w.max_k + 1
+    off = oracle.slater_off_diag(w.k, t, w.k_GL, w.max_k)
+    P = np.zeros((K, n_b, n_b))
+    Q = np.zeros((K, n_b, n_b))
+    for e in range(off["nnz"]):
+        P[:, off["i"][e] - 1, off["j"][e] - 1] += off["rk"][e]
+        Q[:, off["i"][e] - 1, off["j"][e] - 1] += off["rmk"][e]
+    r0 = 2 * t[-1]
+    A = oracle.setup_r_k_int(w.k, t, w.k_GL, w.max_k, r0)
+    for k in range(K):
+        assert np.abs(A[k] - P[k] / r0 ** (k + 1)).max() <= 1e-14 * np.abs(P[k]).max() / r0 ** (k + 1)
+    r0 = t[w.k] * 1e-3
+    A = oracle.setup_r_k_int(w.k, t, w.k_GL, w.max_k, r0)
+    for k in range(K):
+        assert np.abs(A[k] - Q[k] * r0 ** k).max() <= 1e-14 * np.abs(Q[k]).max() * r0 ** k
+    # banded: zero outside |i - j| < k
+    i, j = np.indices((n_b, n_b))
+    assert np.all(A[:, np.abs(i - j) >= w.k] == 0)
+    S = oracle.setup_S(w.k, t, w.k_GL).real
+    _, V = sl.eigh(oracle.setup_H1p(w.k, t, w.k_GL, 2, 0, cap_eta=0.0).real, S)
+    c = V[:, 0]
+    Z = 2.0
+    for r0 in (t[w.k + 5], t[w.k + 12]):   # breakpoints: the kink of r_k sits on an interval edge
+        x = 2 * Z * r0
+        exact = (1 - np.exp(-x) * (1 + x + x * x / 2)) / r0 + Z * np.exp(-x) * (1 + x)
+        got = c @ oracle.setup_r_k_int(w.k, t, w.k_GL, 0, r0)[0] @ c
+        assert abs(got - exact) < 1e-9 * exact
