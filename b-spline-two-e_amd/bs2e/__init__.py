@@ -31,8 +31,9 @@ from .capi import Bs2eError, check, ptr
 
 __all__ = ["Bs2eError", "Context", "generate_grid", "b_spline", "sparse_4d", "sparse_6d", "setup_Slater_integrals",
            "Nd_DOK", "RkTensor", "compute_R_K_map", "ang_k_LS", "three_j", "six_j", "count_nnz", "sym",
-           "count_configs", "CSR_matrix", "Block", "construct_block_tensor", "Server", "footprint", "setup_S",
-           "setup_H_one_particle", "radial_dipole", "setup_radial_dip", "DipBlock", "construct_dip_block_tensor", "setup_r_k_int"]
+           "count_configs", "CSR_matrix", "Block", "construct_block_tensor", "BlockSet", "construct_blocks_tensor",
+           "Server", "footprint", "setup_S", "setup_H_one_particle", "radial_dipole", "setup_radial_dip", "DipBlock",
+           "construct_dip_block_tensor", "setup_r_k_int"]
 
 
 class Context:
@@ -521,6 +522,97 @@ class Block:
             self.release()
         except Exception:
             pass
+
+
+def _one_electron_arrays(H, S, n_b):
+    Hl = np.ascontiguousarray(np.stack([np.asarray(h, np.complex128).ravel(order="F") for h in H]))
+    Sf = np.ascontiguousarray(np.asarray(S, np.complex128).ravel(order="F"))
+    if Hl.shape[1] != n_b * n_b or Sf.size != n_b * n_b:
+        raise Bs2eError(capi.BS2E_EINVAL, f"one-electron matrices must be {n_b} x {n_b}")
+    return Hl, Sf
+
+
+class BlockSet:
+    """The H/S blocks of several symmetries against one device tensor, counted
+    in one launch and filled in one launch over the configuration rows of all
+    of them (main_basis_setup.f90:104-115 runs construct_block_tensor per
+    symmetry)."""
+
+    def __init__(self, tensor: RkTensor, terms, full=True):
+        self.tensor = tensor
+        self.ctx = tensor.ctx
+        self.terms = list(terms)
+        cfs = [np.ascontiguousarray(t.configs, dtype=np.int64) for t in self.terms]
+        self.n_config = [int(c.shape[0]) for c in cfs]
+        Ls = np.array([int(t.l) for t in self.terms], np.int64)
+        ns = np.array(self.n_config, np.int64)
+        allc = np.ascontiguousarray(np.concatenate(cfs, axis=0)) if cfs else np.zeros((0, 5), np.int64)
+        h = C.c_void_p()
+        I = C.c_int64
+        check(capi.lib().bs2e_blockset_create(tensor.handle, len(self.terms), ptr(Ls, I), ptr(ns, I), ptr(allc, I),
+                                              int(bool(full)), C.byref(h)), self.ctx.handle)
+        self.handle = h
+        sz = np.zeros(5 * len(self.terms), np.int64)
+        check(capi.lib().bs2e_blockset_sizes(h, ptr(sz, I), sz.size), self.ctx.handle)
+        self.sizes = sz.reshape(-1, 5)
+
+    def fill(self, H, S):
+        """One launch over every row; results stay on the device (get())."""
+        Hl, Sf = _one_electron_arrays(H, S, self.tensor.n_b)
+        D = C.POINTER(C.c_double)
+        check(capi.lib().bs2e_blockset_fill(self.handle, len(H) - 1, Hl.ctypes.data_as(D), Sf.ctypes.data_as(D)),
+              self.ctx.handle)
+
+    def get(self, i):
+        """(H_sp, S_sp) of symmetry i (0-based position in `terms`)."""
+        if not 0 <= i < len(self.terms):
+            raise Bs2eError(capi.BS2E_ERANGE, f"symmetry index {i} outside 0..{len(self.terms) - 1}")
+        n = self.n_config[i]
+        nH, nS = int(self.sizes[i, 0]), int(self.sizes[i, 1])
+        ipH = np.zeros(n + 1, np.int64)
+        ipS = np.zeros(n + 1, np.int64)
+        ixH = np.zeros(max(nH, 1), np.int64)
+        ixS = np.zeros(max(nS, 1), np.int64)
+        dH = np.zeros(max(nH, 1), np.complex128)
+        dS = np.zeros(max(nS, 1), np.complex128)
+        I = C.c_int64
+        D = C.POINTER(C.c_double)
+        check(capi.lib().bs2e_blockset_get(self.handle, i + 1, ptr(ipH, I), ptr(ixH, I), dH.ctypes.data_as(D),
+                                           ptr(ipS, I), ptr(ixS, I), dS.ctypes.data_as(D)), self.ctx.handle)
+        return (CSR_matrix((n, n), ipH, ixH[:nH], dH[:nH]), CSR_matrix((n, n), ipS, ixS[:nS], dS[:nS]))
+
+    def kernel_ms(self):
+        """(count launch, fill launch) device ms, HIP events."""
+        ms = np.zeros(2)
+        check(capi.lib().bs2e_blockset_times(self.handle, ptr(ms), 2), self.ctx.handle)
+        return float(ms[0]), float(ms[1])
+
+    def release(self):
+        if self.handle:
+            capi.lib().bs2e_blockset_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def construct_blocks_tensor(H, S, b_splines, terms, max_k, R_k, full=True):
+    """construct_block_tensor (hamiltonian.f90:106-283) for every symmetry of
+    `terms` in one count and one fill launch: a list of (H_sp, S_sp)."""
+    tensor = R_k.tensor if isinstance(R_k, Nd_DOK) else R_k
+    if tensor is None:
+        raise Bs2eError(capi.BS2E_ESTATE, "R_k holds no tensor (call compute_R_K_map first)")
+    if tensor.K != max_k + 1 or tensor.k != b_splines.k:
+        raise Bs2eError(capi.BS2E_EINVAL, "max_k / b_splines do not match the tensor")
+    bset = BlockSet(tensor, terms, full)
+    try:
+        bset.fill(H, S)
+        return [bset.get(i) for i in range(len(bset.terms))]
+    finally:
+        bset.release()
 
 
 def construct_block_tensor(H, S, b_splines, term: sym, max_k, R_k, full=True):

@@ -129,12 +129,12 @@ struct BlockArgs {
 // One-electron matrices: spline value / second-derivative tables at the GL
 // points (B, B2: [n_iv][k_gl][k]; rw: [n_iv][k_gl][2] r and w) and the
 // assembly of S (optional) and H for l = l_min .. l_min+n_l-1.
-// RKH one-electron multipoles r_k_int(0:max_k, n_b, n_b), real, k fastest
-hipError_t launch_rk_int(const Plan& p, const DevTables& d, double* B, double* B2, double* rw, int max_k, double r0,
-                         double* out, hipStream_t s);
 hipError_t launch_one_body(const Plan& p, const DevTables& d, double* B, double* B2, double* rw, int l_min, int n_l,
                            int Z, int cap_order, double cap_r0, double eta_re, double eta_im, double* S, double* H,
                            hipStream_t s);
+// RKH one-electron multipoles r_k_int(0:max_k, n_b, n_b), real, k fastest
+hipError_t launch_rk_int(const Plan& p, const DevTables& d, double* B, double* B2, double* rw, int max_k, double r0,
+                         double* out, hipStream_t s);
 
 // Dipole blocks: per (row channel, column channel) the terms of dip_mat_neq
 // (0: direct electron 1, 1: direct electron 2, 2 and 3: exchange).
@@ -171,8 +171,10 @@ hipError_t launch_dip_fill(const DipArgs& a, hipStream_t s);
 hipError_t launch_radial_dipole(const Plan& p, const DevTables& d, double* B, double* B1, double* rw, int gauge,
                                 double* M1, double* M2, hipStream_t s);
 
-hipError_t launch_block_count(const BlockArgs& a, hipStream_t s);
-hipError_t launch_block_fill(const BlockArgs& a, hipStream_t s);
+// Count / fill launches over the rows of n_blocks blocks (device table tab,
+// device row offsets row0[n_blocks + 1], n_rows = row0[n_blocks]).
+hipError_t launch_block_count(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, hipStream_t s);
+hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, hipStream_t s);
 
 // orbital_tools.f90:119-216 count_configs: 5 int64 per config (n1, n2, l1, l2,
 // eqv); returns the count (writes at most cap configs when out != nullptr).

@@ -97,16 +97,28 @@ std::string bs2e::channel_tables(const int64_t* configs, int64_t n_config, int64
     return std::string();
 }
 
-struct bs2e_block {
+// The blocks of one or more symmetries against one computed tensor.  Their
+// configuration rows are laid end to end (symmetry s owns global rows
+// [row0[s], row0[s+1])); one count launch and one fill launch cover them
+// all, each wavefront locating its block in the descriptor table.  The fill
+// keeps the CSR arrays of every symmetry on the device, concatenated in
+// symmetry order; bs2e_blockset_get copies one symmetry's arrays out.
+struct bs2e_blockset {
     bs2e_rk* rk = nullptr;
-    int64_t L = 0, n_config = 0, full = 1, max_l = 0;
+    int64_t n_sym = 0, full = 1, max_l = 0, n_rows = 0;
     float ms_count = 0.f, ms_fill = 0.f;   // device time of the count and fill kernels (HIP events)
+    std::vector<int64_t> L, row0;
+    std::vector<int64_t> chan_off, seg_off, coef_off, flag_off, n_chan;
     std::vector<BlockChan> chan;
     std::vector<BlockSeg> seg;
     std::vector<int32_t> cfg_chan, flags;
     std::vector<double> coef;
-    std::vector<int64_t> cntH, cntS, ptrH, ptrS;
-    int64_t nnzH = 0, nnzS = 0, refH = 0, refS = 0;
+    std::vector<int64_t> cntH, cntS;               // per global row
+    std::vector<int64_t> nnzH, nnzS, refH, refS;   // per symmetry
+    std::vector<int64_t> offH, offS;               // first entry of each symmetry in the concatenated output
+    int64_t totH = 0, totS = 0;
+    std::vector<BlockArgs> tab;                    // host copy of the descriptor table
+    bool filled = false;
     // device
     int64_t* d_cfg = nullptr;
     int32_t* d_cfg_chan = nullptr;
@@ -114,12 +126,32 @@ struct bs2e_block {
     BlockSeg* d_seg = nullptr;
     double* d_coef = nullptr;
     int32_t* d_flags = nullptr;
-    int64_t* d_cnt = nullptr;   // 4 x n_config
-    int64_t* d_ptr = nullptr;   // 2 x n_config
+    int64_t* d_cnt = nullptr;   // 4 x n_rows: H, S, reference H, reference S
+    int64_t* d_ptr = nullptr;   // 2 x n_rows: H, S (0-based offsets into the concatenated output)
+    BlockArgs* d_tab = nullptr;
+    int64_t* d_row0 = nullptr;
+    double *d_Hl = nullptr, *d_S = nullptr;
+    int64_t *d_ixH = nullptr, *d_ixS = nullptr;
+    double *d_dH = nullptr, *d_dS = nullptr;
 };
 
-static void block_free(bs2e_block* b)
+// A single symmetry's block is a set of one.
+struct bs2e_block : bs2e_blockset {};
+
+static void set_drop_outputs(bs2e_blockset* b)
 {
+    dev_free(b->d_Hl);
+    dev_free(b->d_S);
+    dev_free(b->d_ixH);
+    dev_free(b->d_dH);
+    dev_free(b->d_ixS);
+    dev_free(b->d_dS);
+    b->filled = false;
+}
+
+static void set_free(bs2e_blockset* b)
+{
+    set_drop_outputs(b);
     dev_free(b->d_cfg);
     dev_free(b->d_cfg_chan);
     dev_free(b->d_chan);
@@ -128,29 +160,253 @@ static void block_free(bs2e_block* b)
     dev_free(b->d_flags);
     dev_free(b->d_cnt);
     dev_free(b->d_ptr);
+    dev_free(b->d_tab);
+    dev_free(b->d_row0);
 }
 
-static BlockArgs block_args(const bs2e_block* b)
+// descriptor of symmetry s for the count (fill = false) or fill pass
+static BlockArgs set_args(const bs2e_blockset* b, int64_t s, bool fill)
 {
     const Plan& p = b->rk->plan;
+    const int64_t r0 = b->row0[s], N = b->n_rows;
     BlockArgs a{};
-    a.n_config = (int)b->n_config;
-    a.n_chan = (int)b->chan.size();
+    a.n_config = (int)(b->row0[s + 1] - r0);
+    a.n_chan = (int)b->n_chan[s];
     a.full = (int)b->full;
-    a.L = (int)b->L;
+    a.L = (int)b->L[s];
     a.ks = (int)p.k_s;
     a.n_b = (int)p.n_b;
     a.n_pairs = (int)p.n_pairs;
     a.K = (int)p.K;
-    a.cfg = b->d_cfg;
-    a.cfg_chan = b->d_cfg_chan;
-    a.chan = b->d_chan;
-    a.seg = b->d_seg;
-    a.coef = b->d_coef;
-    a.flags = b->d_flags;
+    a.cfg = b->d_cfg + 5 * r0;
+    a.cfg_chan = b->d_cfg_chan + r0;
+    a.chan = b->d_chan + b->chan_off[s];
+    a.seg = b->d_seg + b->seg_off[s];
+    a.coef = b->d_coef + b->coef_off[s];
+    a.flags = b->d_flags + b->flag_off[s];
     a.c_first = b->rk->d.c_first;
     a.R = b->rk->R;
+    if (!fill) {
+        a.cntH = b->d_cnt + r0;
+        a.cntS = b->d_cnt + N + r0;
+        a.refH = b->d_cnt + 2 * N + r0;
+        a.refS = b->d_cnt + 3 * N + r0;
+    } else {
+        a.Hl = b->d_Hl;
+        a.S = b->d_S;
+        a.ptrH = b->d_ptr + r0;
+        a.ptrS = b->d_ptr + N + r0;
+        a.ixH = b->d_ixH;
+        a.dH = b->d_dH;
+        a.ixS = b->d_ixS;
+        a.dS = b->d_dS;
+    }
     return a;
+}
+
+// Host tables, device upload and the count launch of a set whose rk, n_sym,
+// full, L and row0 are given.  Returns BS2E_OK or the error code (set on ctx).
+static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* who)
+{
+    bs2e_ctx* ctx = b->rk->ctx;
+    const Plan& p = b->rk->plan;
+    const int64_t nb = p.n_b, K = p.K, N = b->n_rows;
+    b->chan_off.assign(b->n_sym, 0);
+    b->seg_off.assign(b->n_sym, 0);
+    b->coef_off.assign(b->n_sym, 0);
+    b->flag_off.assign(b->n_sym, 0);
+    b->n_chan.assign(b->n_sym, 0);
+    b->cfg_chan.assign((size_t)N, 0);
+    for (int64_t s = 0; s < b->n_sym; ++s) {
+        std::vector<BlockChan> ch;
+        std::vector<BlockSeg> sg;
+        std::vector<int32_t> cc;
+        int64_t ml = 0;
+        const int64_t r0 = b->row0[s], n = b->row0[s + 1] - r0;
+        std::string bad = channel_tables(configs + 5 * r0, n, nb, ch, sg, cc, ml);
+        if (!bad.empty())
+            return fail(ctx, BS2E_EINVAL, std::string(who) + ": symmetry " + std::to_string(s + 1) + ": " + bad +
+                                              " (count_configs order required)");
+        b->max_l = std::max(b->max_l, ml);
+        b->chan_off[s] = (int64_t)b->chan.size();
+        b->seg_off[s] = (int64_t)b->seg.size();
+        b->coef_off[s] = (int64_t)b->coef.size();
+        b->flag_off[s] = (int64_t)b->flags.size();
+        b->n_chan[s] = (int64_t)ch.size();
+        std::copy(cc.begin(), cc.end(), b->cfg_chan.begin() + r0);
+        b->chan.insert(b->chan.end(), ch.begin(), ch.end());
+        b->seg.insert(b->seg.end(), sg.begin(), sg.end());
+        // angular coefficients per (row channel, column channel): wigner_tools.f90:126-138
+        const int64_t nch = (int64_t)ch.size(), L = b->L[s];
+        const size_t c0 = b->coef.size(), f0 = b->flags.size();
+        b->coef.resize(c0 + (size_t)(nch * nch * 2 * K), 0.0);
+        b->flags.resize(f0 + (size_t)(nch * nch), 0);
+        for (int64_t ci = 0; ci < nch; ++ci)
+            for (int64_t cj = 0; cj < nch; ++cj) {
+                const int64_t la = ch[ci].l1, lb = ch[ci].l2, lc = ch[cj].l1, ld = ch[cj].l2;
+                const double sgn = ((lc + ld + L) & 1) ? -1.0 : 1.0;
+                double* cd = b->coef.data() + c0 + (ci * nch + cj) * 2 * K;
+                double* ce = cd + K;
+                int32_t fl = 0;
+                for (int64_t k = 0; k < K; ++k) {
+                    const double ang = ang_k_LS(k, la, lb, lc, ld, L);
+                    const double ang_ex = ang_k_LS(k, la, lb, ld, lc, L);
+                    if (std::fabs(ang) >= 5e-16) cd[k] = ang;              // r_12_tens, mat_els.f90:566-570
+                    if (std::fabs(ang_ex) >= 5e-16) ce[k] = sgn * ang_ex;
+                    if (std::fabs(ang) > 5e-15) fl |= 1;                    // hamiltonian.f90:171-178
+                    if (std::fabs(ang_ex) > 5e-15) fl |= 2;
+                }
+                b->flags[f0 + ci * nch + cj] = fl;
+            }
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, dev_alloc(&b->d_cfg, (size_t)5 * N));
+    HIP_TRY(ctx, dev_alloc(&b->d_cfg_chan, (size_t)N));
+    HIP_TRY(ctx, dev_alloc(&b->d_chan, b->chan.size()));
+    HIP_TRY(ctx, dev_alloc(&b->d_seg, b->seg.size()));
+    HIP_TRY(ctx, dev_alloc(&b->d_coef, std::max<size_t>(b->coef.size(), 1)));
+    HIP_TRY(ctx, dev_alloc(&b->d_flags, std::max<size_t>(b->flags.size(), 1)));
+    HIP_TRY(ctx, dev_alloc(&b->d_cnt, (size_t)4 * N));
+    HIP_TRY(ctx, dev_alloc(&b->d_ptr, (size_t)2 * N));
+    HIP_TRY(ctx, dev_alloc(&b->d_tab, (size_t)b->n_sym));
+    HIP_TRY(ctx, dev_alloc(&b->d_row0, (size_t)b->n_sym + 1));
+    hipStream_t st = ctx->stream;
+    HIP_TRY(ctx, hipMemcpyAsync(b->d_cfg, configs, sizeof(int64_t) * 5 * N, hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(b->d_cfg_chan, b->cfg_chan.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(b->d_chan, b->chan.data(), sizeof(BlockChan) * b->chan.size(), hipMemcpyHostToDevice,
+                                st));
+    HIP_TRY(ctx, hipMemcpyAsync(b->d_seg, b->seg.data(), sizeof(BlockSeg) * b->seg.size(), hipMemcpyHostToDevice, st));
+    if (!b->coef.empty())
+        HIP_TRY(ctx, hipMemcpyAsync(b->d_coef, b->coef.data(), sizeof(double) * b->coef.size(), hipMemcpyHostToDevice,
+                                    st));
+    if (!b->flags.empty())
+        HIP_TRY(ctx, hipMemcpyAsync(b->d_flags, b->flags.data(), sizeof(int32_t) * b->flags.size(),
+                                    hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(b->d_row0, b->row0.data(), sizeof(int64_t) * (b->n_sym + 1), hipMemcpyHostToDevice,
+                                st));
+    b->tab.resize((size_t)b->n_sym);
+    for (int64_t s = 0; s < b->n_sym; ++s) b->tab[s] = set_args(b, s, false);
+    HIP_TRY(ctx, hipMemcpyAsync(b->d_tab, b->tab.data(), sizeof(BlockArgs) * b->n_sym, hipMemcpyHostToDevice, st));
+    clear_stale_error();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(ctx, hipEventCreate(&e0));
+    HIP_TRY(ctx, hipEventCreate(&e1));
+    std::vector<int64_t> cnt((size_t)4 * N);
+    hipError_t le = hipEventRecord(e0, st);
+    if (le == hipSuccess) le = launch_block_count(b->d_tab, b->d_row0, (int)b->n_sym, N, st);
+    if (le == hipSuccess) le = hipEventRecord(e1, st);
+    if (le == hipSuccess)
+        le = hipMemcpyAsync(cnt.data(), b->d_cnt, sizeof(int64_t) * 4 * N, hipMemcpyDeviceToHost, st);
+    if (le == hipSuccess) le = hipStreamSynchronize(st);
+    if (le == hipSuccess) le = hipEventElapsedTime(&b->ms_count, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIP_TRY(ctx, le);
+    b->cntH.assign(cnt.begin(), cnt.begin() + N);
+    b->cntS.assign(cnt.begin() + N, cnt.begin() + 2 * N);
+    std::vector<int64_t> ptr((size_t)2 * N);
+    b->nnzH.assign(b->n_sym, 0);
+    b->nnzS.assign(b->n_sym, 0);
+    b->refH.assign(b->n_sym, 0);
+    b->refS.assign(b->n_sym, 0);
+    b->offH.assign(b->n_sym, 0);
+    b->offS.assign(b->n_sym, 0);
+    b->totH = b->totS = 0;
+    for (int64_t s = 0; s < b->n_sym; ++s) {
+        b->offH[s] = b->totH;
+        b->offS[s] = b->totS;
+        for (int64_t g = b->row0[s]; g < b->row0[s + 1]; ++g) {
+            ptr[g] = b->totH;
+            ptr[N + g] = b->totS;
+            b->totH += b->cntH[g];
+            b->totS += b->cntS[g];
+            b->refH[s] += cnt[2 * N + g];
+            b->refS[s] += cnt[3 * N + g];
+        }
+        b->nnzH[s] = b->totH - b->offH[s];
+        b->nnzS[s] = b->totS - b->offS[s];
+    }
+    HIP_TRY(ctx, hipMemcpyAsync(b->d_ptr, ptr.data(), sizeof(int64_t) * 2 * N, hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    return BS2E_OK;
+}
+
+// Uploads the one-electron matrices, allocates the outputs and runs the fill
+// launch over every row of the set; results stay on the device.
+static int64_t set_fill(bs2e_blockset* b, int64_t max_l_1p, const double* Hl, const double* S, const char* who)
+{
+    bs2e_ctx* ctx = b->rk->ctx;
+    if (!Hl || !S) return fail(ctx, BS2E_EINVAL, std::string(who) + ": NULL argument");
+    if (max_l_1p < b->max_l)
+        return fail(ctx, BS2E_EINVAL, std::string(who) + ": one-electron matrices do not cover every l of the blocks");
+    const Plan& p = b->rk->plan;
+    const size_t nn = (size_t)p.n_b * p.n_b;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    set_drop_outputs(b);
+    HIP_TRY(ctx, dev_alloc(&b->d_Hl, 2 * nn * (size_t)(max_l_1p + 1)));
+    HIP_TRY(ctx, dev_alloc(&b->d_S, 2 * nn));
+    HIP_TRY(ctx, dev_alloc(&b->d_ixH, (size_t)std::max<int64_t>(b->totH, 1)));
+    HIP_TRY(ctx, dev_alloc(&b->d_dH, 2 * (size_t)std::max<int64_t>(b->totH, 1)));
+    HIP_TRY(ctx, dev_alloc(&b->d_ixS, (size_t)std::max<int64_t>(b->totS, 1)));
+    HIP_TRY(ctx, dev_alloc(&b->d_dS, 2 * (size_t)std::max<int64_t>(b->totS, 1)));
+    hipStream_t st = ctx->stream;
+    HIP_TRY(ctx, hipMemcpyAsync(b->d_Hl, Hl, sizeof(double) * 2 * nn * (max_l_1p + 1), hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(b->d_S, S, sizeof(double) * 2 * nn, hipMemcpyHostToDevice, st));
+    for (int64_t s = 0; s < b->n_sym; ++s) b->tab[s] = set_args(b, s, true);
+    HIP_TRY(ctx, hipMemcpyAsync(b->d_tab, b->tab.data(), sizeof(BlockArgs) * b->n_sym, hipMemcpyHostToDevice, st));
+    clear_stale_error();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(ctx, hipEventCreate(&e0));
+    HIP_TRY(ctx, hipEventCreate(&e1));
+    hipError_t le = hipEventRecord(e0, st);
+    if (le == hipSuccess) le = launch_block_fill(b->d_tab, b->d_row0, (int)b->n_sym, b->n_rows, st);
+    if (le == hipSuccess) le = hipEventRecord(e1, st);
+    if (le == hipSuccess) le = hipEventSynchronize(e1);
+    if (le == hipSuccess) le = hipEventElapsedTime(&b->ms_fill, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIP_TRY(ctx, le);
+    b->filled = true;
+    return BS2E_OK;
+}
+
+// Copies symmetry s's CSR arrays to the host (index_ptr 1-based as the
+// reference's, hamiltonian.f90:142-143, 203-204).
+static int64_t set_get(const bs2e_blockset* b, int64_t s, int64_t* ip_H, int64_t* ix_H, double* d_H, int64_t* ip_S,
+                       int64_t* ix_S, double* d_S, const char* who)
+{
+    bs2e_ctx* ctx = b->rk->ctx;
+    if (!ip_H || !ix_H || !d_H || !ip_S || !ix_S || !d_S)
+        return fail(ctx, BS2E_EINVAL, std::string(who) + ": NULL argument");
+    if (!b->filled) return fail(ctx, BS2E_ESTATE, std::string(who) + ": the blocks have not been filled");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int64_t nH = b->nnzH[s], nS = b->nnzS[s];
+    if (nH > 0) {
+        HIP_TRY(ctx, hipMemcpyAsync(ix_H, b->d_ixH + b->offH[s], sizeof(int64_t) * nH, hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipMemcpyAsync(d_H, b->d_dH + 2 * b->offH[s], sizeof(double) * 2 * nH, hipMemcpyDeviceToHost, st));
+    }
+    if (nS > 0) {
+        HIP_TRY(ctx, hipMemcpyAsync(ix_S, b->d_ixS + b->offS[s], sizeof(int64_t) * nS, hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipMemcpyAsync(d_S, b->d_dS + 2 * b->offS[s], sizeof(double) * 2 * nS, hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    ip_H[0] = 1;
+    ip_S[0] = 1;
+    const int64_t r0 = b->row0[s], n = b->row0[s + 1] - r0;
+    for (int64_t i = 0; i < n; ++i) {
+        ip_H[i + 1] = ip_H[i] + b->cntH[r0 + i];
+        ip_S[i + 1] = ip_S[i] + b->cntS[r0 + i];
+    }
+    return BS2E_OK;
+}
+
+// After a set's device memory is freed and the object deleted: drops its
+// hold on the tensor.
+static int64_t rk_unblock(bs2e_rk* rk)
+{
+    if (--rk->blocks == 0 && rk->closing) return rk_release_now(rk);   // the tensor's release was deferred
+    return BS2E_OK;
 }
 
 extern "C" {
@@ -175,105 +431,17 @@ int64_t bs2e_block_create(bs2e_rk* rk, int64_t L, int64_t n_config, const int64_
     if (!rk->computed) return fail(ctx, BS2E_ESTATE, "bs2e_block_create: the R^k tensor has not been computed");
     if (L < 0 || n_config < 1 || n_config >= ((int64_t)1 << 31))
         return fail(ctx, BS2E_EINVAL, "bs2e_block_create: need L >= 0 and 1 <= n_config < 2^31");
-    const Plan& p = rk->plan;
     bs2e_block* b = new (std::nothrow) bs2e_block();
     if (!b) return fail(ctx, BS2E_ENOMEM, "bs2e_block_create: host allocation failed");
     b->rk = rk;
-    b->L = L;
-    b->n_config = n_config;
+    b->n_sym = 1;
     b->full = full ? 1 : 0;
-    // channel tables; the configurations must be in count_configs order
-    const int64_t nb = p.n_b;
-    std::string bad = channel_tables(configs, n_config, nb, b->chan, b->seg, b->cfg_chan, b->max_l);
-    if (!bad.empty()) {
-        delete b;
-        return fail(ctx, BS2E_EINVAL, "bs2e_block_create: " + bad + " (count_configs order required)");
-    }
-    // angular coefficients per (row channel, column channel): wigner_tools.f90:126-138
-    const int64_t nch = (int64_t)b->chan.size(), K = p.K;
-    b->coef.assign((size_t)(nch * nch * 2 * K), 0.0);
-    b->flags.assign((size_t)(nch * nch), 0);
-    for (int64_t ci = 0; ci < nch; ++ci)
-        for (int64_t cj = 0; cj < nch; ++cj) {
-            const int64_t la = b->chan[ci].l1, lb = b->chan[ci].l2, lc = b->chan[cj].l1, ld = b->chan[cj].l2;
-            const double sgn = ((lc + ld + L) & 1) ? -1.0 : 1.0;
-            double* cd = b->coef.data() + (ci * nch + cj) * 2 * K;
-            double* ce = cd + K;
-            int32_t fl = 0;
-            for (int64_t k = 0; k < K; ++k) {
-                const double ang = ang_k_LS(k, la, lb, lc, ld, L);
-                const double ang_ex = ang_k_LS(k, la, lb, ld, lc, L);
-                if (std::fabs(ang) >= 5e-16) cd[k] = ang;              // r_12_tens, mat_els.f90:566-570
-                if (std::fabs(ang_ex) >= 5e-16) ce[k] = sgn * ang_ex;
-                if (std::fabs(ang) > 5e-15) fl |= 1;                    // hamiltonian.f90:171-178
-                if (std::fabs(ang_ex) > 5e-15) fl |= 2;
-            }
-            b->flags[ci * nch + cj] = fl;
-        }
-    int64_t rc = BS2E_OK;
-    auto run = [&]() -> int64_t {
-        HIP_TRY(ctx, hipSetDevice(ctx->device));
-        HIP_TRY(ctx, dev_alloc(&b->d_cfg, (size_t)5 * n_config));
-        HIP_TRY(ctx, dev_alloc(&b->d_cfg_chan, (size_t)n_config));
-        HIP_TRY(ctx, dev_alloc(&b->d_chan, b->chan.size()));
-        HIP_TRY(ctx, dev_alloc(&b->d_seg, b->seg.size()));
-        HIP_TRY(ctx, dev_alloc(&b->d_coef, b->coef.size()));
-        HIP_TRY(ctx, dev_alloc(&b->d_flags, b->flags.size()));
-        HIP_TRY(ctx, dev_alloc(&b->d_cnt, (size_t)4 * n_config));
-        HIP_TRY(ctx, dev_alloc(&b->d_ptr, (size_t)2 * n_config));
-        hipStream_t s = ctx->stream;
-        HIP_TRY(ctx, hipMemcpyAsync(b->d_cfg, configs, sizeof(int64_t) * 5 * n_config, hipMemcpyHostToDevice, s));
-        HIP_TRY(ctx, hipMemcpyAsync(b->d_cfg_chan, b->cfg_chan.data(), sizeof(int32_t) * n_config,
-                                    hipMemcpyHostToDevice, s));
-        HIP_TRY(ctx, hipMemcpyAsync(b->d_chan, b->chan.data(), sizeof(BlockChan) * b->chan.size(),
-                                    hipMemcpyHostToDevice, s));
-        HIP_TRY(ctx, hipMemcpyAsync(b->d_seg, b->seg.data(), sizeof(BlockSeg) * b->seg.size(), hipMemcpyHostToDevice,
-                                    s));
-        HIP_TRY(ctx, hipMemcpyAsync(b->d_coef, b->coef.data(), sizeof(double) * b->coef.size(),
-                                    hipMemcpyHostToDevice, s));
-        HIP_TRY(ctx, hipMemcpyAsync(b->d_flags, b->flags.data(), sizeof(int32_t) * b->flags.size(),
-                                    hipMemcpyHostToDevice, s));
-        clear_stale_error();
-        BlockArgs a = block_args(b);
-        a.cntH = b->d_cnt;
-        a.cntS = b->d_cnt + n_config;
-        a.refH = b->d_cnt + 2 * n_config;
-        a.refS = b->d_cnt + 3 * n_config;
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        HIP_TRY(ctx, hipEventCreate(&e0));
-        HIP_TRY(ctx, hipEventCreate(&e1));
-        hipError_t le = hipEventRecord(e0, s);
-        if (le == hipSuccess) le = launch_block_count(a, s);
-        if (le == hipSuccess) le = hipEventRecord(e1, s);
-        std::vector<int64_t> cnt((size_t)4 * n_config);
-        if (le == hipSuccess)
-            le = hipMemcpyAsync(cnt.data(), b->d_cnt, sizeof(int64_t) * 4 * n_config, hipMemcpyDeviceToHost, s);
-        if (le == hipSuccess) le = hipStreamSynchronize(s);
-        if (le == hipSuccess) le = hipEventElapsedTime(&b->ms_count, e0, e1);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        HIP_TRY(ctx, le);
-        b->cntH.assign(cnt.begin(), cnt.begin() + n_config);
-        b->cntS.assign(cnt.begin() + n_config, cnt.begin() + 2 * n_config);
-        b->ptrH.resize(n_config);
-        b->ptrS.resize(n_config);
-        for (int64_t i = 0; i < n_config; ++i) {
-            b->ptrH[i] = b->nnzH;
-            b->ptrS[i] = b->nnzS;
-            b->nnzH += b->cntH[i];
-            b->nnzS += b->cntS[i];
-            b->refH += cnt[2 * n_config + i];
-            b->refS += cnt[3 * n_config + i];
-        }
-        HIP_TRY(ctx, hipMemcpyAsync(b->d_ptr, b->ptrH.data(), sizeof(int64_t) * n_config, hipMemcpyHostToDevice, s));
-        HIP_TRY(ctx, hipMemcpyAsync(b->d_ptr + n_config, b->ptrS.data(), sizeof(int64_t) * n_config,
-                                    hipMemcpyHostToDevice, s));
-        HIP_TRY(ctx, hipStreamSynchronize(s));
-        return BS2E_OK;
-    };
-    rc = run();
+    b->L = {L};
+    b->row0 = {0, n_config};
+    b->n_rows = n_config;
+    const int64_t rc = set_create(b, configs, "bs2e_block_create");
     if (rc) {
-        block_free(b);
+        set_free(b);
         delete b;
         return rc;
     }
@@ -293,7 +461,7 @@ int64_t bs2e_block_times(const bs2e_block* b, double* ms, int64_t n)
 int64_t bs2e_block_sizes(const bs2e_block* b, int64_t* sizes, int64_t n)
 {
     if (!b || !sizes) return fail(nullptr, BS2E_EINVAL, "bs2e_block_sizes: NULL argument");
-    const int64_t v[5] = {b->nnzH, b->nnzS, b->refH, b->refS, b->n_config};
+    const int64_t v[5] = {b->nnzH[0], b->nnzS[0], b->refH[0], b->refS[0], b->n_rows};
     for (int64_t i = 0; i < n && i < 5; ++i) sizes[i] = v[i];
     return BS2E_OK;
 }
@@ -303,71 +471,10 @@ int64_t bs2e_block_fill(bs2e_block* b, int64_t max_l_1p, const double* Hl, const
 {
     if (!b || !Hl || !S || !ip_H || !ix_H || !d_H || !ip_S || !ix_S || !d_S)
         return fail(b ? b->rk->ctx : nullptr, BS2E_EINVAL, "bs2e_block_fill: NULL argument");
-    bs2e_ctx* ctx = b->rk->ctx;
-    if (max_l_1p < b->max_l)
-        return fail(ctx, BS2E_EINVAL, "bs2e_block_fill: one-electron matrices do not cover every l of the block");
-    const Plan& p = b->rk->plan;
-    const size_t nn = (size_t)p.n_b * p.n_b;
-    double *dHl = nullptr, *dS = nullptr, *ddH = nullptr, *ddS = nullptr;
-    int64_t *dixH = nullptr, *dixS = nullptr;
-    auto cleanup = [&]() {
-        dev_free(dHl);
-        dev_free(dS);
-        dev_free(ddH);
-        dev_free(ddS);
-        dev_free(dixH);
-        dev_free(dixS);
-    };
-    auto run = [&]() -> int64_t {
-        HIP_TRY(ctx, hipSetDevice(ctx->device));
-        HIP_TRY(ctx, dev_alloc(&dHl, 2 * nn * (size_t)(max_l_1p + 1)));
-        HIP_TRY(ctx, dev_alloc(&dS, 2 * nn));
-        HIP_TRY(ctx, dev_alloc(&ddH, 2 * (size_t)b->nnzH));
-        HIP_TRY(ctx, dev_alloc(&ddS, 2 * (size_t)b->nnzS));
-        HIP_TRY(ctx, dev_alloc(&dixH, (size_t)b->nnzH));
-        HIP_TRY(ctx, dev_alloc(&dixS, (size_t)b->nnzS));
-        hipStream_t s = ctx->stream;
-        HIP_TRY(ctx, hipMemcpyAsync(dHl, Hl, sizeof(double) * 2 * nn * (max_l_1p + 1), hipMemcpyHostToDevice, s));
-        HIP_TRY(ctx, hipMemcpyAsync(dS, S, sizeof(double) * 2 * nn, hipMemcpyHostToDevice, s));
-        clear_stale_error();
-        BlockArgs a = block_args(b);
-        a.Hl = dHl;
-        a.S = dS;
-        a.ptrH = b->d_ptr;
-        a.ptrS = b->d_ptr + b->n_config;
-        a.ixH = dixH;
-        a.dH = ddH;
-        a.ixS = dixS;
-        a.dS = ddS;
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        HIP_TRY(ctx, hipEventCreate(&e0));
-        HIP_TRY(ctx, hipEventCreate(&e1));
-        hipError_t le = hipEventRecord(e0, s);
-        if (le == hipSuccess) le = launch_block_fill(a, s);
-        if (le == hipSuccess) le = hipEventRecord(e1, s);
-        if (le == hipSuccess) le = hipEventSynchronize(e1);
-        if (le == hipSuccess) le = hipEventElapsedTime(&b->ms_fill, e0, e1);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        HIP_TRY(ctx, le);
-        HIP_TRY(ctx, hipMemcpyAsync(ix_H, dixH, sizeof(int64_t) * b->nnzH, hipMemcpyDeviceToHost, s));
-        HIP_TRY(ctx, hipMemcpyAsync(d_H, ddH, sizeof(double) * 2 * b->nnzH, hipMemcpyDeviceToHost, s));
-        HIP_TRY(ctx, hipMemcpyAsync(ix_S, dixS, sizeof(int64_t) * b->nnzS, hipMemcpyDeviceToHost, s));
-        HIP_TRY(ctx, hipMemcpyAsync(d_S, ddS, sizeof(double) * 2 * b->nnzS, hipMemcpyDeviceToHost, s));
-        HIP_TRY(ctx, hipStreamSynchronize(s));
-        return BS2E_OK;
-    };
-    const int64_t rc = run();
-    cleanup();
-    if (rc) return rc;
-    // CSR row pointers, 1-based as the reference's (hamiltonian.f90:142-143, 203-204)
-    ip_H[0] = 1;
-    ip_S[0] = 1;
-    for (int64_t i = 0; i < b->n_config; ++i) {
-        ip_H[i + 1] = ip_H[i] + b->cntH[i];
-        ip_S[i + 1] = ip_S[i] + b->cntS[i];
-    }
-    return BS2E_OK;
+    int64_t rc = set_fill(b, max_l_1p, Hl, S, "bs2e_block_fill");
+    if (rc == BS2E_OK) rc = set_get(b, 0, ip_H, ix_H, d_H, ip_S, ix_S, d_S, "bs2e_block_fill");
+    set_drop_outputs(b);   // a single block hands its arrays over and keeps no device copy
+    return rc;
 }
 
 int64_t bs2e_block_release(bs2e_block* b)
@@ -375,10 +482,88 @@ int64_t bs2e_block_release(bs2e_block* b)
     if (!b) return BS2E_OK;
     bs2e_rk* rk = b->rk;
     (void)hipSetDevice(rk->ctx->device);
-    block_free(b);
+    set_free(b);
     delete b;
-    if (--rk->blocks == 0 && rk->closing) return rk_release_now(rk);   // the tensor's release was deferred
+    return rk_unblock(rk);
+}
+
+int64_t bs2e_blockset_create(bs2e_rk* rk, int64_t n_sym, const int64_t* L, const int64_t* n_config,
+                             const int64_t* configs, int64_t full, bs2e_blockset** out)
+{
+    if (!rk || !out || !L || !n_config || !configs)
+        return fail(rk ? rk->ctx : nullptr, BS2E_EINVAL, "bs2e_blockset_create: NULL argument");
+    bs2e_ctx* ctx = rk->ctx;
+    *out = nullptr;
+    if (!rk->computed) return fail(ctx, BS2E_ESTATE, "bs2e_blockset_create: the R^k tensor has not been computed");
+    if (n_sym < 1 || n_sym > 100000) return fail(ctx, BS2E_EINVAL, "bs2e_blockset_create: need 1 <= n_sym <= 100000");
+    int64_t total = 0;
+    for (int64_t s = 0; s < n_sym; ++s) {
+        if (L[s] < 0 || n_config[s] < 1 || n_config[s] >= ((int64_t)1 << 31))
+            return fail(ctx, BS2E_EINVAL, "bs2e_blockset_create: need L >= 0 and 1 <= n_config < 2^31 per symmetry");
+        total += n_config[s];
+    }
+    if (total >= ((int64_t)1 << 40)) return fail(ctx, BS2E_EINVAL, "bs2e_blockset_create: too many rows");
+    bs2e_blockset* b = new (std::nothrow) bs2e_blockset();
+    if (!b) return fail(ctx, BS2E_ENOMEM, "bs2e_blockset_create: host allocation failed");
+    b->rk = rk;
+    b->n_sym = n_sym;
+    b->full = full ? 1 : 0;
+    b->L.assign(L, L + n_sym);
+    b->row0.assign((size_t)n_sym + 1, 0);
+    for (int64_t s = 0; s < n_sym; ++s) b->row0[s + 1] = b->row0[s] + n_config[s];
+    b->n_rows = total;
+    const int64_t rc = set_create(b, configs, "bs2e_blockset_create");
+    if (rc) {
+        set_free(b);
+        delete b;
+        return rc;
+    }
+    rk->blocks += 1;
+    *out = b;
     return BS2E_OK;
+}
+
+int64_t bs2e_blockset_sizes(const bs2e_blockset* b, int64_t* sizes, int64_t n)
+{
+    if (!b || !sizes) return fail(nullptr, BS2E_EINVAL, "bs2e_blockset_sizes: NULL argument");
+    for (int64_t s = 0; s < b->n_sym; ++s) {
+        const int64_t v[5] = {b->nnzH[s], b->nnzS[s], b->refH[s], b->refS[s], b->row0[s + 1] - b->row0[s]};
+        for (int64_t i = 0; i < 5 && 5 * s + i < n; ++i) sizes[5 * s + i] = v[i];
+    }
+    return BS2E_OK;
+}
+
+int64_t bs2e_blockset_fill(bs2e_blockset* b, int64_t max_l_1p, const double* Hl, const double* S)
+{
+    if (!b) return fail(nullptr, BS2E_EINVAL, "bs2e_blockset_fill: NULL argument");
+    return set_fill(b, max_l_1p, Hl, S, "bs2e_blockset_fill");
+}
+
+int64_t bs2e_blockset_get(const bs2e_blockset* b, int64_t sym, int64_t* ip_H, int64_t* ix_H, double* d_H,
+                          int64_t* ip_S, int64_t* ix_S, double* d_S)
+{
+    if (!b) return fail(nullptr, BS2E_EINVAL, "bs2e_blockset_get: NULL argument");
+    if (sym < 1 || sym > b->n_sym)
+        return fail(b->rk->ctx, BS2E_ERANGE, "bs2e_blockset_get: symmetry index outside 1..n_sym");
+    return set_get(b, sym - 1, ip_H, ix_H, d_H, ip_S, ix_S, d_S, "bs2e_blockset_get");
+}
+
+int64_t bs2e_blockset_times(const bs2e_blockset* b, double* ms, int64_t n)
+{
+    if (!b || !ms) return fail(nullptr, BS2E_EINVAL, "bs2e_blockset_times: NULL argument");
+    const double v[2] = {b->ms_count, b->ms_fill};
+    for (int64_t i = 0; i < n && i < 2; ++i) ms[i] = v[i];
+    return BS2E_OK;
+}
+
+int64_t bs2e_blockset_release(bs2e_blockset* b)
+{
+    if (!b) return BS2E_OK;
+    bs2e_rk* rk = b->rk;
+    (void)hipSetDevice(rk->ctx->device);
+    set_free(b);
+    delete b;
+    return rk_unblock(rk);
 }
 
 }  // extern "C"

@@ -54,12 +54,26 @@ __device__ __forceinline__ Cplx mat(const double* M, int n_b, int r, int c)
     return Cplx{M[o], M[o + 1]};
 }
 
+// One wavefront per configuration row of any of n_blocks symmetry blocks:
+// global row g belongs to the last block b with row0[b] <= g and is row
+// g - row0[b] of it; each block's tables, inputs and outputs are in tab[b].
+// Five waves per SIMD keep the fill at 94 VGPRs without spilling.
 template <bool FILL>
-__global__ __launch_bounds__(kCThreads) void kc_block(BlockArgs A)
+__global__ __launch_bounds__(kCThreads) __attribute__((amdgpu_waves_per_eu(5))) void kc_block(const BlockArgs* __restrict__ tab,
+                                                      const int64_t* __restrict__ row0, int n_blocks, int64_t n_rows)
 {
     const int lane = threadIdx.x & (kWave - 1);
-    const int i = (int)(((size_t)blockIdx.x * kCThreads + threadIdx.x) / kWave);
-    if (i >= A.n_config) return;   // whole wave exits together
+    const int64_t g = ((int64_t)blockIdx.x * kCThreads + threadIdx.x) / kWave;
+    if (g >= n_rows) return;   // whole wave exits together
+    int blo = 0, bhi = n_blocks - 1;
+    while (blo < bhi) {
+        const int mid = (blo + bhi + 1) >> 1;
+        if (row0[mid] <= g) blo = mid;
+        else bhi = mid - 1;
+    }
+    const int b = __builtin_amdgcn_readfirstlane(blo);
+    const BlockArgs A = tab[b];
+    const int i = (int)(g - row0[b]);
     const int ks = A.ks, K = A.K;
     const int64_t* ci = A.cfg + 5 * (size_t)i;
     const int n_a = (int)ci[0], n_bb = (int)ci[1], l_a = (int)ci[2], l_b = (int)ci[3];
@@ -81,84 +95,78 @@ __global__ __launch_bounds__(kCThreads) void kc_block(BlockArgs A)
         const BlockSeg* sg = A.seg + (size_t)ch * (A.n_b + 1);
         walk_channel(n_a, n_bb, ks, A.n_b, sg, lane, [&](bool active, int n_c, int n_d, int col, bool in_dir_c,
                                                           bool in_ex_c) {
-            {
-                {
-                    {
-                        const int j = cj.start + col;
-                        if (!A.full && j < i) active = false;
-                        const bool support = in_dir_c && iabs_d(n_bb - n_d) < ks;
-                        const bool support_ex = in_ex_c && iabs_d(n_a - n_d) < ks;
-                        const bool r12 = (dir_any && support) || (ex_any && support_ex);
-                        const bool entry = active && (support || support_ex) && (r12 || l_eq || l_eq_ex);
-                        const bool one_el = active && ((support && l_eq) || (support_ex && l_eq_ex));
-                        const unsigned long long mH = __ballot(entry);
-                        const unsigned long long mS = __ballot(one_el);
-                        if (FILL) {
-                            if (entry) {
-                                Cplx h{0.0, 0.0};
-                                if (r12) {
-                                    double res = 0.0;
-                                    // R(a,b,c,d) = R(a,d,c,b): the key [n_a, n_d, n_c, n_b] puts the
-                                    // wavefront's consecutive n_d on consecutive columns of one row
-                                    if (support) {
-                                        const double* v = A.R + ((size_t)pair_of(A.c_first, ks, n_a, n_c) * A.n_pairs +
-                                                                 pair_of(A.c_first, ks, n_d, n_bb)) * K;
-                                        double t = 0.0;
-                                        for (int k = 0; k < K; ++k) t = __dadd_rn(t, __dmul_rn(v[k], cd[k]));
-                                        res = __dadd_rn(res, t);
-                                    }
-                                    // exchange R(a,b,d,c) = R(b,a,c,d) = R(b,d,c,a): row pair(n_b, n_c),
-                                    // consecutive n_d on consecutive columns
-                                    if (support_ex) {
-                                        const double* v = A.R + ((size_t)pair_of(A.c_first, ks, n_bb, n_c) * A.n_pairs +
-                                                                 pair_of(A.c_first, ks, n_d, n_a)) * K;
-                                        double t = 0.0;
-                                        for (int k = 0; k < K; ++k) t = __dadd_rn(t, __dmul_rn(v[k], ce[k]));
-                                        res = __dadd_rn(res, t);
-                                    }
-                                    h.re = res;
-                                }
-                                if (one_el) {
-                                    const size_t nn = (size_t)A.n_b * A.n_b;
-                                    const double* Ha = A.Hl + 2 * nn * l_a;
-                                    const double* Hb = A.Hl + 2 * nn * l_b;
-                                    Cplx h1{0.0, 0.0}, s1{0.0, 0.0};
-                                    const double sgn = ((l_c + l_d + A.L) & 1) ? -1.0 : 1.0;
-                                    if (l_eq) {
-                                        h1 = cadd(cadd(h1, cmul(mat(Ha, A.n_b, n_a, n_c), mat(A.S, A.n_b, n_bb, n_d))),
-                                                  cmul(mat(Hb, A.n_b, n_bb, n_d), mat(A.S, A.n_b, n_a, n_c)));
-                                        s1 = cadd(s1, cmul(mat(A.S, A.n_b, n_a, n_c), mat(A.S, A.n_b, n_bb, n_d)));
-                                    }
-                                    if (l_eq_ex) {
-                                        const Cplx x = cadd(cmul(mat(Ha, A.n_b, n_a, n_d), mat(A.S, A.n_b, n_bb, n_c)),
-                                                            cmul(mat(Hb, A.n_b, n_bb, n_c), mat(A.S, A.n_b, n_a, n_d)));
-                                        h1 = cadd(h1, cscale(sgn, x));
-                                        s1 = cadd(s1, cmul(cscale(sgn, mat(A.S, A.n_b, n_a, n_d)),
-                                                           mat(A.S, A.n_b, n_bb, n_c)));
-                                    }
-                                    h = cadd(h, h1);
-                                    const int64_t ps = baseS + nS + __popcll(mS & lt_mask);
-                                    A.ixS[ps] = j + 1;
-                                    A.dS[2 * ps] = s1.re;
-                                    A.dS[2 * ps + 1] = s1.im;
-                                }
-                                const int64_t ph = baseH + nH + __popcll(mH & lt_mask);
-                                A.ixH[ph] = j + 1;
-                                A.dH[2 * ph] = h.re;
-                                A.dH[2 * ph + 1] = h.im;
-                            }
-                        } else {
-                            // the reference's count_nnz predicate (:386-413), for diagnostics
-                            const bool rh = active && (support ? dir_any : (support_ex && ex_any));
-                            const bool rs = active && (support ? l_eq : (support_ex && l_eq_ex));
-                            rH += __popcll(__ballot(rh));
-                            rS += __popcll(__ballot(rs));
+            const int j = cj.start + col;
+            if (!A.full && j < i) active = false;
+            const bool support = in_dir_c && iabs_d(n_bb - n_d) < ks;
+            const bool support_ex = in_ex_c && iabs_d(n_a - n_d) < ks;
+            const bool r12 = (dir_any && support) || (ex_any && support_ex);
+            const bool entry = active && (support || support_ex) && (r12 || l_eq || l_eq_ex);
+            const bool one_el = active && ((support && l_eq) || (support_ex && l_eq_ex));
+            const unsigned long long mH = __ballot(entry);
+            const unsigned long long mS = __ballot(one_el);
+            if (FILL) {
+                if (entry) {
+                    Cplx h{0.0, 0.0};
+                    if (r12) {
+                        double res = 0.0;
+                        // R(a,b,c,d) = R(a,d,c,b): the key [n_a, n_d, n_c, n_b] puts the
+                        // wavefront's consecutive n_d on consecutive columns of one row
+                        if (support) {
+                            const double* v = A.R + ((size_t)pair_of(A.c_first, ks, n_a, n_c) * A.n_pairs +
+                                                     pair_of(A.c_first, ks, n_d, n_bb)) * K;
+                            double t = 0.0;
+                            for (int k = 0; k < K; ++k) t = __dadd_rn(t, __dmul_rn(v[k], cd[k]));
+                            res = __dadd_rn(res, t);
                         }
-                        nH += __popcll(mH);
-                        nS += __popcll(mS);
+                        // exchange R(a,b,d,c) = R(b,a,c,d) = R(b,d,c,a): row pair(n_b, n_c),
+                        // consecutive n_d on consecutive columns
+                        if (support_ex) {
+                            const double* v = A.R + ((size_t)pair_of(A.c_first, ks, n_bb, n_c) * A.n_pairs +
+                                                     pair_of(A.c_first, ks, n_d, n_a)) * K;
+                            double t = 0.0;
+                            for (int k = 0; k < K; ++k) t = __dadd_rn(t, __dmul_rn(v[k], ce[k]));
+                            res = __dadd_rn(res, t);
+                        }
+                        h.re = res;
                     }
+                    if (one_el) {
+                        const size_t nn = (size_t)A.n_b * A.n_b;
+                        const double* Ha = A.Hl + 2 * nn * l_a;
+                        const double* Hb = A.Hl + 2 * nn * l_b;
+                        Cplx h1{0.0, 0.0}, s1{0.0, 0.0};
+                        const double sgn = ((l_c + l_d + A.L) & 1) ? -1.0 : 1.0;
+                        if (l_eq) {
+                            h1 = cadd(cadd(h1, cmul(mat(Ha, A.n_b, n_a, n_c), mat(A.S, A.n_b, n_bb, n_d))),
+                                      cmul(mat(Hb, A.n_b, n_bb, n_d), mat(A.S, A.n_b, n_a, n_c)));
+                            s1 = cadd(s1, cmul(mat(A.S, A.n_b, n_a, n_c), mat(A.S, A.n_b, n_bb, n_d)));
+                        }
+                        if (l_eq_ex) {
+                            const Cplx x = cadd(cmul(mat(Ha, A.n_b, n_a, n_d), mat(A.S, A.n_b, n_bb, n_c)),
+                                                cmul(mat(Hb, A.n_b, n_bb, n_c), mat(A.S, A.n_b, n_a, n_d)));
+                            h1 = cadd(h1, cscale(sgn, x));
+                            s1 = cadd(s1, cmul(cscale(sgn, mat(A.S, A.n_b, n_a, n_d)),
+                                               mat(A.S, A.n_b, n_bb, n_c)));
+                        }
+                        h = cadd(h, h1);
+                        const int64_t ps = baseS + nS + __popcll(mS & lt_mask);
+                        A.ixS[ps] = j + 1;
+                        A.dS[2 * ps] = s1.re;
+                        A.dS[2 * ps + 1] = s1.im;
+                    }
+                    const int64_t ph = baseH + nH + __popcll(mH & lt_mask);
+                    A.ixH[ph] = j + 1;
+                    A.dH[2 * ph] = h.re;
+                    A.dH[2 * ph + 1] = h.im;
                 }
+            } else {
+                // the reference's count_nnz predicate (:386-413), for diagnostics
+                const bool rh = active && (support ? dir_any : (support_ex && ex_any));
+                const bool rs = active && (support ? l_eq : (support_ex && l_eq_ex));
+                rH += __popcll(__ballot(rh));
+                rS += __popcll(__ballot(rs));
             }
+            nH += __popcll(mH);
+            nS += __popcll(mS);
         });
     }
     if (!FILL && lane == 0) {
@@ -169,19 +177,19 @@ __global__ __launch_bounds__(kCThreads) void kc_block(BlockArgs A)
     }
 }
 
-hipError_t launch_block_count(const BlockArgs& a, hipStream_t s)
+hipError_t launch_block_count(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, hipStream_t s)
 {
-    const int64_t threads = (int64_t)a.n_config * kWave;
+    const int64_t threads = n_rows * kWave;
     hipLaunchKernelGGL(kc_block<false>, dim3((unsigned)((threads + kCThreads - 1) / kCThreads)), dim3(kCThreads), 0, s,
-                       a);
+                       tab, row0, n_blocks, n_rows);
     return hipGetLastError();
 }
 
-hipError_t launch_block_fill(const BlockArgs& a, hipStream_t s)
+hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, hipStream_t s)
 {
-    const int64_t threads = (int64_t)a.n_config * kWave;
+    const int64_t threads = n_rows * kWave;
     hipLaunchKernelGGL(kc_block<true>, dim3((unsigned)((threads + kCThreads - 1) / kCThreads)), dim3(kCThreads), 0, s,
-                       a);
+                       tab, row0, n_blocks, n_rows);
     return hipGetLastError();
 }
 

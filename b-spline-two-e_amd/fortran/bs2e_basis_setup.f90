@@ -162,10 +162,7 @@ program bs2e_basis_setup
         call bas%init(max_L_all, max_l, splines%n_b, splines%k, splines%n_b, splines%n_b, max_l, .true., .true.)
         call H_diag%init(bas%n_sym)
         call S_diag%init(bas%n_sym)
-        do i = 1, bas%n_sym
-            call construct_block_tensor(H_vec, S, splines, bas%syms(i), max_k, R_p, H_diag%blocks(i), &
-                                        S_diag%blocks(i), ifull /= 0)
-        end do
+        call construct_blocks_tensor(H_vec, S, splines, bas%syms, max_k, R_p, H_diag, S_diag, ifull /= 0)
         gauge = 'l'
         if (command_argument_count() >= 17) then
             call get_command_argument(17, arg)

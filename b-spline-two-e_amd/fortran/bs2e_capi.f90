@@ -18,6 +18,7 @@ module bs2e_capi
     public :: bs2e_block_release, c_string, bs2e_setup_S, bs2e_setup_H_one_particle
     public :: bs2e_setup_radial_dip, bs2e_dip_block_create, bs2e_dip_block_sizes, bs2e_dip_block_fill
     public :: bs2e_dip_block_release, bs2e_setup_r_k_int
+    public :: bs2e_blockset_create, bs2e_blockset_sizes, bs2e_blockset_fill, bs2e_blockset_get, bs2e_blockset_release
 
     interface
         function bs2e_create(ctx, device) bind(C, name="bs2e_create") result(rc)
@@ -198,6 +199,46 @@ module bs2e_capi
         function bs2e_block_release(blk) bind(C, name="bs2e_block_release") result(rc)
             import :: c_ptr, c_int64_t
             type(c_ptr), value :: blk
+            integer(c_int64_t) :: rc
+        end function
+
+        function bs2e_blockset_create(rk, n_sym, L, n_config, configs, full, out) &
+                bind(C, name="bs2e_blockset_create") result(rc)
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: rk
+            integer(c_int64_t), value :: n_sym, full
+            integer(c_int64_t), intent(in) :: L(*), n_config(*), configs(*)
+            type(c_ptr), intent(out) :: out
+            integer(c_int64_t) :: rc
+        end function
+
+        function bs2e_blockset_sizes(set, sizes, n) bind(C, name="bs2e_blockset_sizes") result(rc)
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: set
+            integer(c_int64_t), intent(out) :: sizes(*)
+            integer(c_int64_t), value :: n
+            integer(c_int64_t) :: rc
+        end function
+
+        function bs2e_blockset_fill(set, max_l_1p, Hl, S) bind(C, name="bs2e_blockset_fill") result(rc)
+            import :: c_ptr, c_int64_t, c_double_complex
+            type(c_ptr), value :: set
+            integer(c_int64_t), value :: max_l_1p
+            complex(c_double_complex), intent(in) :: Hl(*), S(*)
+            integer(c_int64_t) :: rc
+        end function
+
+        function bs2e_blockset_get(set, sym, ipH, ixH, dH, ipS, ixS, dS) bind(C, name="bs2e_blockset_get") result(rc)
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: set
+            integer(c_int64_t), value :: sym
+            type(c_ptr), value :: ipH, ixH, dH, ipS, ixS, dS
+            integer(c_int64_t) :: rc
+        end function
+
+        function bs2e_blockset_release(set) bind(C, name="bs2e_blockset_release") result(rc)
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: set
             integer(c_int64_t) :: rc
         end function
 

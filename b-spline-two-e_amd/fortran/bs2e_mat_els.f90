@@ -45,6 +45,7 @@ module bs2e_mat_els
     public :: count_configs, construct_block_tensor, ang_k_LS, bs2e_use_device, bs2e_shutdown
     public :: hydrogenic, CAP, setup_S, setup_H_one_particle, basis, block_diag_CS
     public :: block_CS, radial_dipole, setup_radial_dip, construct_dip_block_tensor, rkh_multipoles
+    public :: construct_blocks_tensor
 
     type(c_ptr), save :: g_ctx = c_null_ptr
     integer, save :: g_device = 0
@@ -886,6 +887,74 @@ contains
         H_sp%data = dH(1:sizes(1))
         S_sp%data = dS(1:sizes(2))
     end subroutine construct_block_tensor
+
+    ! main_basis_setup.f90:101-118: construct_block_tensor for every
+    ! symmetry of syms, with one count and one fill launch over the
+    ! configuration rows of all of them; H_diag%blocks(i), S_diag%blocks(i)
+    ! receive symmetry i.  H_diag and S_diag must be init'ed with size(syms).
+    subroutine construct_blocks_tensor(H, S, b_splines, syms, max_k, R_k, H_diag, S_diag, full)
+        type(block), allocatable, intent(in) :: H(:)
+        complex(dp), intent(in) :: S(:,:)
+        type(b_spline), intent(in) :: b_splines
+        type(sym), intent(in) :: syms(:)
+        integer, intent(in) :: max_k
+        class(Nd_DOK), intent(inout) :: R_k
+        type(block_diag_CS), intent(inout) :: H_diag, S_diag
+        logical, intent(in) :: full
+        integer(c_int64_t), allocatable :: cf(:), Ls(:), ns(:), sizes(:)
+        integer(c_int64_t), allocatable, target :: ipH(:), ixH(:), ipS(:), ixS(:)
+        complex(dp), allocatable, target :: dH(:), dS(:)
+        complex(dp), allocatable :: Hl(:,:,:)
+        type(c_ptr) :: set
+        integer :: i, c, l, n, nb, ns_tot, off
+        if (.not. c_associated(R_k%handle)) error stop "construct_blocks_tensor: R_k holds no tensor"
+        if (R_k%N_val /= max_k + 1) error stop "construct_blocks_tensor: max_k differs from the tensor's"
+        if (lbound(H, 1) /= 0) error stop "construct_blocks_tensor: H must be indexed from l = 0"
+        if (size(H_diag%blocks) /= size(syms) .or. size(S_diag%blocks) /= size(syms)) &
+            error stop "construct_blocks_tensor: H_diag and S_diag must hold one block per symmetry"
+        nb = b_splines%n_b
+        ns_tot = sum(syms(:)%n_config)
+        allocate(cf(5 * max(ns_tot, 1)), Ls(size(syms)), ns(size(syms)), sizes(5 * size(syms)))
+        off = 0
+        do i = 1, size(syms)
+            Ls(i) = syms(i)%l
+            ns(i) = syms(i)%n_config
+            do c = 1, syms(i)%n_config
+                cf(off + 5 * (c - 1) + 1:off + 5 * (c - 1) + 2) = int(syms(i)%configs(c)%n, c_int64_t)
+                cf(off + 5 * (c - 1) + 3:off + 5 * (c - 1) + 4) = int(syms(i)%configs(c)%l, c_int64_t)
+                cf(off + 5 * c) = merge(1_c_int64_t, 0_c_int64_t, syms(i)%configs(c)%eqv)
+            end do
+            off = off + 5 * syms(i)%n_config
+        end do
+        allocate(Hl(nb, nb, 0:ubound(H, 1)))
+        do l = 0, ubound(H, 1)
+            Hl(:, :, l) = H(l)%data
+        end do
+        call check(bs2e_blockset_create(R_k%handle, int(size(syms), c_int64_t), Ls, ns, cf, &
+                                        merge(1_c_int64_t, 0_c_int64_t, full), set), "construct_blocks_tensor")
+        call check(bs2e_blockset_sizes(set, sizes, int(size(sizes), c_int64_t)), "construct_blocks_tensor")
+        call check(bs2e_blockset_fill(set, int(ubound(H, 1), c_int64_t), Hl, S), "construct_blocks_tensor")
+        do i = 1, size(syms)
+            n = syms(i)%n_config
+            allocate(ipH(n + 1), ipS(n + 1), ixH(max(sizes(5 * i - 4), 1_c_int64_t)), &
+                     ixS(max(sizes(5 * i - 3), 1_c_int64_t)))
+            allocate(dH(max(sizes(5 * i - 4), 1_c_int64_t)), dS(max(sizes(5 * i - 3), 1_c_int64_t)))
+            call check(bs2e_blockset_get(set, int(i, c_int64_t), c_loc(ipH), c_loc(ixH), c_loc(dH), c_loc(ipS), &
+                                         c_loc(ixS), c_loc(dS)), "construct_blocks_tensor")
+            H_diag%blocks(i)%shape = [n, n]
+            S_diag%blocks(i)%shape = [n, n]
+            H_diag%blocks(i)%nnz = int(sizes(5 * i - 4))
+            S_diag%blocks(i)%nnz = int(sizes(5 * i - 3))
+            H_diag%blocks(i)%index_ptr = int(ipH)
+            S_diag%blocks(i)%index_ptr = int(ipS)
+            H_diag%blocks(i)%indices = int(ixH(1:sizes(5 * i - 4)))
+            S_diag%blocks(i)%indices = int(ixS(1:sizes(5 * i - 3)))
+            H_diag%blocks(i)%data = dH(1:sizes(5 * i - 4))
+            S_diag%blocks(i)%data = dS(1:sizes(5 * i - 3))
+            deallocate(ipH, ipS, ixH, ixS, dH, dS)
+        end do
+        call check(bs2e_blockset_release(set), "construct_blocks_tensor")
+    end subroutine construct_blocks_tensor
 
 end module bs2e_mat_els
 

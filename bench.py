@@ -169,6 +169,48 @@ def blocks_benchmark(rk, w, t, max_l=2, L=0):
             "wall_ms_with_transfers": wall * 1e3}
 
 
+def blockset_benchmark(ctx, name="c3", max_l=4, max_L=4):
+    """Channel summation of every symmetry L = 0..max_L (natural parity) of
+    the named basis with l1, l2 <= max_l in one count and one fill launch
+    (uneven channel counts per symmetry), from the basis's own tensor (k up to
+    its max_k); kernel times from the library's HIP events.  The CSR arrays
+    stay on the device (nothing is copied back)."""
+    import bs2e
+    from bs2e.workloads import WORKLOADS
+    w = WORKLOADS[name]
+    t = bs2e.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+    spl = bs2e.b_spline(w.k, t)
+    rk = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx)
+    S = bs2e.setup_S(spl, w.k_GL, ctx=ctx)
+    Hl = bs2e.setup_H_one_particle(w.Z, (2, 0.8 * w.r_max, 0.0), range(max_l + 1), spl, w.k_GL, ctx=ctx)
+    terms = []
+    for L in range(max_L + 1):
+        term = bs2e.sym(l=L, pi=bool(L % 2))
+        term.configs = bs2e.count_configs(term, max_l, spl.n_b, w.k, spl.n_b, spl.n_b, max_l)
+        terms.append(term)
+    out = {"workload": f"{w.name} basis (order {w.k}, {spl.n_b} B-splines, k <= {w.max_k}), L = 0..{max_L}, "
+                       f"l1,l2 <= {max_l}, all channels, one count + one fill launch",
+           "rows": int(sum(t_.n_config for t_ in terms))}
+    try:
+        t0 = time.perf_counter()
+        bset = bs2e.BlockSet(rk, terms)
+        bset.fill(list(Hl), S)
+        wall = time.perf_counter() - t0
+    except bs2e.Bs2eError as e:
+        rk.release()
+        out["error"] = str(e)
+        return out
+    ms_count, ms_fill = bset.kernel_ms()
+    nH, nS = int(bset.sizes[:, 0].sum()), int(bset.sizes[:, 1].sum())
+    bset.release()
+    rk.release()
+    out.update({"channels_per_symmetry": [int(len({(int(a), int(b)) for a, b in t_.configs[:, 2:4]})) for t_ in terms],
+                "nnz_H": nH, "nnz_S": nS, "count_kernel_ms": ms_count, "fill_kernel_ms": ms_fill,
+                "H_entries_per_s": nH / (ms_fill * 1e-3), "fill_GBs": 24.0 * (nH + nS) / (ms_fill * 1e-3) / 1e9,
+                "wall_ms_create_and_fill": wall * 1e3})
+    return out
+
+
 def dipoles_benchmark(ctx, w, t, max_l=2, L1=0, gauge="l"):
     """Dipole block between symmetries (L1, even) and (L1+1, odd), q = 0
     (dipole.f90:8-47): radial dipole matrices, pattern count and fill on the
@@ -207,6 +249,8 @@ def main():
     ap.add_argument("--latency-samples", type=int, default=50,
                     help="synchronised single builds timed for p50/p99 latency")
     ap.add_argument("--blocks", type=int, default=1, help="also time the L=0 symmetry block (0 skips it)")
+    ap.add_argument("--blockset", type=int, default=1,
+                    help="also time all symmetries L=0..4 (l <= 4) of the c3 basis in one launch (0 skips it)")
     ap.add_argument("--dipoles", type=int, default=1, help="also time the L=0 -> L=1 dipole block (0 skips it)")
     ap.add_argument("--serve-jobs", type=int, default=256,
                     help="jobs of the serving stream (differing bases through the build service); 0 skips it")
@@ -336,6 +380,8 @@ def main():
         out["blocks"] = blocks_benchmark(rk, w, t)
     if args.dipoles:
         out["dipoles"] = dipoles_benchmark(rk.ctx, w, t)
+    if args.blockset:
+        out["blockset"] = blockset_benchmark(rk.ctx)
     if args.serve_jobs > 0:
         out["serving"] = serving_benchmark(args.serve_jobs)
     if not args.no_cpu_baseline:

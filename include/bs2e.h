@@ -209,6 +209,31 @@ int64_t bs2e_block_times(const bs2e_block *b, double *ms, int64_t n);
 int64_t bs2e_setup_r_k_int(bs2e_ctx *ctx, int64_t k, int64_t n_knots, const double *knots, int64_t k_gl,
                            int64_t max_k, double r_0, double *r_k_int);
 
+/* ---- several symmetry blocks in one launch ------------------------------ */
+typedef struct bs2e_blockset bs2e_blockset; /* the blocks of several symmetries against one tensor */
+
+/* hamiltonian.f90:106-283 construct_block_tensor for n_sym symmetries at
+ * once, as main_basis_setup.f90:104-115 loops over bas%syms: symmetry s
+ * (1-based) has total L[s-1] and n_config[s-1] configurations, stored one
+ * symmetry after the other in configs (5 int64 each, count_configs order).
+ * One count launch over the configuration rows of every symmetry (uneven
+ * channel counts and sizes share the chip through one wavefront per row). */
+int64_t bs2e_blockset_create(bs2e_rk *rk, int64_t n_sym, const int64_t *L, const int64_t *n_config,
+                             const int64_t *configs, int64_t full, bs2e_blockset **out);
+/* sizes[5(s-1) .. 5(s-1)+4] = the bs2e_block_sizes values of symmetry s. */
+int64_t bs2e_blockset_sizes(const bs2e_blockset *set, int64_t *sizes, int64_t n);
+/* One fill launch over every row of every symmetry (inputs as
+ * bs2e_block_fill); the CSR arrays stay on the device until the next fill or
+ * the release. */
+int64_t bs2e_blockset_fill(bs2e_blockset *set, int64_t max_l_1p, const double *Hl, const double *S);
+/* Copies symmetry sym's (1-based) H_sp and S_sp to the host, as
+ * bs2e_block_fill writes them (sizes from bs2e_blockset_sizes). */
+int64_t bs2e_blockset_get(const bs2e_blockset *set, int64_t sym, int64_t *ip_H, int64_t *ix_H, double *d_H,
+                          int64_t *ip_S, int64_t *ix_S, double *d_S);
+/* Device time (ms) of the count launch (ms[0]) and the last fill (ms[1]). */
+int64_t bs2e_blockset_times(const bs2e_blockset *set, double *ms, int64_t n);
+int64_t bs2e_blockset_release(bs2e_blockset *set);
+
 /* ---- dipole operators -------------------------------------------------- */
 /* mat_els.f90:123-170 setup_radial_dip(b_splines, k_GL, gauge, radial_dip)
  * with :348-370 compute_radial_dip_len / :372-390 compute_radial_dip_vel:

@@ -7,6 +7,8 @@ the same inputs in the same order, no fused multiply-adds); H data within
 oracle's by ~1e-15 relative, and direct and exchange terms can cancel, so an
 entry-wise relative bound would be ill-conditioned).
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -118,3 +120,74 @@ def test_block_errors(c1, ctx):
         bs2e.Block(fresh, term)
     assert e.value.code == capi.BS2E_ESTATE
     fresh.release()
+
+
+@pytest.mark.parametrize("full", [True, False])
+def test_blockset_equals_single_blocks_and_oracle(c1, oracle, full):
+    """The symmetries L = 0..2 (and a truncated L = 3 list) in one count and
+    one fill launch: every symmetry's CSR arrays equal the single-block path
+    bit for bit and the oracle to the block bar; sizes report per symmetry."""
+    w, t, n_b, rk, Rref, S, Hl = c1
+    terms = []
+    for L in range(3):
+        term = bs2e.sym(l=L, pi=bool(L % 2))
+        term.configs = bs2e.count_configs(term, 2, n_b, w.k, n_b, n_b, 2)
+        terms.append(term)
+    t3 = bs2e.sym(l=3, pi=True)
+    t3.configs = bs2e.count_configs(t3, 2, n_b, w.k, 40, 30, 1)
+    terms.append(t3)
+    print("blockset c1 L=0..3 full", full, [t.n_config for t in terms], flush=True)
+    bset = bs2e.BlockSet(rk, terms, full)
+    bset.fill(Hl, S)
+    ms = bset.kernel_ms()
+    assert ms[0] > 0 and ms[1] > 0
+    for i, term in enumerate(terms):
+        H_sp, S_sp = bset.get(i)
+        blk = bs2e.Block(rk, term, full)
+        assert (blk.nnz_H, blk.nnz_S, blk.ref_nnz_H, blk.ref_nnz_S) == tuple(int(x) for x in bset.sizes[i, :4])
+        H1, S1 = blk.fill(Hl, S)
+        blk.release()
+        for a, b in ((H_sp, H1), (S_sp, S1)):
+            assert np.array_equal(a.index_ptr, b.index_ptr) and np.array_equal(a.indices, b.indices)
+            assert np.array_equal(a.data, b.data)
+        if i in (0, 3):
+            ref = oracle.construct_block(w.k, n_b, term.l, w.max_k, full, term.configs, Rref, Hl, S)
+            compare(H_sp, S_sp, ref)
+    with pytest.raises(capi.Bs2eError) as e:
+        bset.get(len(terms))
+    assert e.value.code == capi.BS2E_ERANGE
+    one = np.zeros(1, np.int64)
+    cd = np.zeros(2)
+    I, D = C.POINTER(C.c_int64), C.POINTER(C.c_double)
+    args = [one.ctypes.data_as(I), one.ctypes.data_as(I), cd.ctypes.data_as(D)] * 2
+    for bad in (0, len(terms) + 1):   # the C ABI counts symmetries from 1
+        assert capi.lib().bs2e_blockset_get(bset.handle, bad, *args) == capi.BS2E_ERANGE
+    bset.release()
+
+
+def test_blockset_uneven_channels_t0(oracle, ctx):
+    """The reference test's basis (order 8, r <= 15) with l <= 3 and every
+    symmetry L = 0..4 of both parities in one launch: ten blocks of very
+    different channel counts, each equal to the oracle's."""
+    w = WORKLOADS["t0"]
+    t = bs2e.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+    n_b = oracle.dims(w.k, t)[1]
+    max_l = 3
+    rk = bs2e.RkTensor.build(w.k, t, w.k_GL, 2 * max_l, ctx=ctx)
+    Rref = oracle.rk_dense(w.k, n_b, 2 * max_l, oracle.slater_off_diag(w.k, t, w.k_GL, 2 * max_l),
+                           oracle.slater_diag(w.k, t, w.k_GL, 2 * max_l))
+    S = oracle.setup_S(w.k, t, w.k_GL)
+    Hl = [oracle.setup_H1p(w.k, t, w.k_GL, 2, l, cap_eta=0.0) for l in range(max_l + 1)]
+    terms = []
+    for L in range(5):
+        for pi in (False, True):
+            term = bs2e.sym(l=L, pi=pi)
+            term.configs = bs2e.count_configs(term, max_l, n_b, w.k, n_b, n_b, max_l)
+            if term.n_config:
+                terms.append(term)
+    print("blockset t0 terms", [(t_.l, t_.pi, t_.n_config) for t_ in terms], flush=True)
+    blocks = bs2e.construct_blocks_tensor(Hl, S, bs2e.b_spline(w.k, t), terms, 2 * max_l, rk)
+    for term, (H_sp, S_sp) in zip(terms, blocks):
+        ref = oracle.construct_block(w.k, n_b, term.l, 2 * max_l, True, term.configs, Rref, Hl, S)
+        compare(H_sp, S_sp, ref)
+    rk.release()
