@@ -102,6 +102,18 @@ struct BlockSeg {             // configurations of one channel with a given n1
     int32_t pad;
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
+// with its own L2.  xcd_block maps the launch's workgroup b to the logical
+// workgroup index that gives XCD (b mod 8) one contiguous range of the n
+// logical workgroups, so neighbours that gather the same tables share an L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t n)
+{
+    const int64_t x = b % kXcds, j = b / kXcds;
+    const int64_t per = n / kXcds, extra = n % kXcds;
+    return x * per + (x < extra ? x : extra) + j;
+}
+
 struct BlockArgs {
     int n_config, n_chan, full, L, ks, n_b, n_pairs, K;
     const int64_t* cfg;          // 5 per config: n1, n2, l1, l2, eqv

@@ -673,7 +673,7 @@ static int64_t time_ms(bs2e_ctx* ctx, const std::function<hipError_t()>& launch,
     HIP_TRY(ctx, hipEventCreate(&a));
     HIP_TRY(ctx, hipEventCreate(&b));
     hipError_t e = launch();   // warm-up
-    for (int r = 0; r < 3 && e == hipSuccess; ++r) {
+    for (int r = 0; r < 8 && e == hipSuccess; ++r) {
         if (r == 0) e = hipEventRecord(a, ctx->stream);
         if (e == hipSuccess) e = launch();
     }
@@ -706,15 +706,21 @@ int64_t bs2e_rk_autotune(bs2e_rk* rk, int64_t force)
     }
     const bool prof = rk->profiling;
     rk->profiling = false;
-    int64_t rc = run_build(ctx, rk);   // tables in place (every variant computes the same values)
-    if (!rc) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    // Each candidate is timed over whole two-stream builds: the fill shares
+    // the chip with the diagonal cells and the near ranges, and the fastest
+    // shape on its own is not always the fastest beside them.
+    int64_t rc = BS2E_OK;
+    auto build = [&]() -> hipError_t {
+        const int64_t r = run_build(ctx, rk);
+        return r ? hipErrorLaunchFailure : hipSuccess;
+    };
     float best = 1e30f;
     int best_far = far_variant_default(p);
     for (int v = 0; v < far_variant_count() && !rc; ++v) {
         if (!far_variant_valid(p, v)) continue;
         p.far_variant = v;
         float ms = 0.f;
-        rc = time_ms(ctx, [&]() { return launch_tensor_far(p, rk->d, rk->R, ctx->stream); }, &ms);
+        rc = time_ms(ctx, build, &ms);
         if (!rc && ms < best) {
             best = ms;
             best_far = v;
@@ -728,7 +734,7 @@ int64_t bs2e_rk_autotune(bs2e_rk* rk, int64_t force)
         if (rc || kc > kmax) continue;
         p.inner_kc = kc;
         float ms = 0.f;
-        rc = time_ms(ctx, [&]() { return launch_diag_cells(p, rk->d, ctx->stream, true); }, &ms);
+        rc = time_ms(ctx, build, &ms);
         if (!rc && ms < best) {
             best = ms;
             best_kc = kc;

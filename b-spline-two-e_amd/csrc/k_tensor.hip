@@ -67,7 +67,7 @@ __global__ __launch_bounds__(kThreads) void k3_near(int n_pairs, int n_b, int K,
     int* sJ0 = sQ0 + kNearMaxSeg;                          // [kNearMaxSeg + 1]: segment starts in row order
     int* sNseg = sJ0 + kNearMaxSeg + 1;
     const int tid = threadIdx.x;
-    const int p = rows[blockIdx.x];
+    const int p = rows[xcd_block(blockIdx.x, gridDim.x)];   // neighbouring rows gather the same Dsym cells
     const PairInfo pp = pairs[p];
     const int lo = pp.v0, m = pp.m, hi = pp.v0 + pp.m - 1;
     const int pt = (pp.a == pp.c) ? -1 : pair_at(c_first, ks, pp.c, pp.a);   // the identical row (c, a)
@@ -164,10 +164,11 @@ __global__ __launch_bounds__(kThreads) void k3_far(int n_pairs, int K, int far_c
     }
     __syncthreads();
     double* Rrow = R + (size_t)p0 * rowlen;
+    const unsigned uK = (unsigned)K, urow = (unsigned)rowlen;   // rowlen = n_pairs * K < 2^31 (checked on the host)
     for (int i = 0; i < ITERS; ++i) {
-        const int64_t e = V * ((int64_t)cx * kThreads * ITERS + (int64_t)i * kThreads + tid);
-        if (e >= rowlen) break;
-        const int q0 = (int)(e / K), k0 = (int)(e - (int64_t)q0 * K);
+        const unsigned e = (unsigned)V * ((unsigned)cx * kThreads * ITERS + (unsigned)i * kThreads + (unsigned)tid);
+        if (e >= urow) break;
+        const int q0 = (int)(e / uK), k0 = (int)(e - (unsigned)q0 * uK);
         const PairInfo a0 = pairs[q0];
         const int v1a = a0.v0 + a0.m - 1;
         if (V == 1) {
@@ -261,7 +262,7 @@ hipError_t launch_tensor_near(const Plan& p, const DevTables& d, double* R, hipS
 struct FarShape {
     int V, rows, iters;
 };
-static constexpr FarShape kFarShapes[] = {{2, 2, 2}, {2, 4, 4}, {2, 2, 8}, {2, 1, 4}, {1, 4, 8}, {1, 2, 4}};
+static constexpr FarShape kFarShapes[] = {{2, 2, 1}, {2, 2, 2}, {2, 4, 4}, {2, 2, 8}, {2, 1, 4}, {1, 4, 8}, {1, 2, 4}};
 
 int far_variant_count() { return (int)(sizeof(kFarShapes) / sizeof(kFarShapes[0])); }
 
@@ -270,9 +271,11 @@ bool far_variant_valid(const Plan& p, int v)
     return v >= 0 && v < far_variant_count() && (kFarShapes[v].V == 1 || (p.n_pairs * p.K) % 2 == 0);
 }
 
-// 2 rows x 1024 elements, 16-byte stores, measured fastest on its own at
-// 0.7-20 GB tensors; rows of odd length fall back to 8-byte stores.
-int far_variant_default(const Plan& p) { return (p.n_pairs * p.K) % 2 == 0 ? 0 : 4; }
+// 2 rows x 512 elements, one 16-byte store per thread and row: a short
+// workgroup life keeps more stores in flight than longer loops (5.7 TB/s
+// against 5.5 for 2 x 1024 at 0.76 GB); rows of odd length fall back to
+// 8-byte stores.
+int far_variant_default(const Plan& p) { return (p.n_pairs * p.K) % 2 == 0 ? 0 : 5; }
 
 template <int V, int ROWS, int ITERS>
 static void launch_far_shape(const Plan& p, const DevTables& d, double* R, hipStream_t s)
@@ -289,11 +292,12 @@ hipError_t launch_tensor_far(const Plan& p, const DevTables& d, double* R, hipSt
 {
     const int v = far_variant_valid(p, p.far_variant) ? p.far_variant : far_variant_default(p);
     switch (v) {
-    case 0: launch_far_shape<2, 2, 2>(p, d, R, s); break;
-    case 1: launch_far_shape<2, 4, 4>(p, d, R, s); break;
-    case 2: launch_far_shape<2, 2, 8>(p, d, R, s); break;
-    case 3: launch_far_shape<2, 1, 4>(p, d, R, s); break;
-    case 4: launch_far_shape<1, 4, 8>(p, d, R, s); break;
+    case 0: launch_far_shape<2, 2, 1>(p, d, R, s); break;
+    case 1: launch_far_shape<2, 2, 2>(p, d, R, s); break;
+    case 2: launch_far_shape<2, 4, 4>(p, d, R, s); break;
+    case 3: launch_far_shape<2, 2, 8>(p, d, R, s); break;
+    case 4: launch_far_shape<2, 1, 4>(p, d, R, s); break;
+    case 5: launch_far_shape<1, 4, 8>(p, d, R, s); break;
     default: launch_far_shape<1, 2, 4>(p, d, R, s); break;
     }
     return hipGetLastError();
