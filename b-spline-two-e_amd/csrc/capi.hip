@@ -730,8 +730,11 @@ int64_t bs2e_rk_autotune(bs2e_rk* rk, int64_t force)
     best = 1e30f;
     int best_kc = 0;
     const int kmax = inner_kc(p.k_s, p.k_gl, p.K);
-    for (int kc : {1, 2, 4}) {
-        if (rc || kc > kmax) continue;
+    std::vector<int> kcs;
+    for (int kc : {1, 2, 4, 8, kmax})
+        if (kc >= 1 && kc <= kmax && std::find(kcs.begin(), kcs.end(), kc) == kcs.end()) kcs.push_back(kc);
+    for (int kc : kcs) {
+        if (rc) continue;
         p.inner_kc = kc;
         float ms = 0.f;
         rc = time_ms(ctx, build, &ms);

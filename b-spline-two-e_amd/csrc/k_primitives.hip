@@ -22,6 +22,8 @@
 // kw_pair_windows: per banded pair p and multipole k, the window moments
 //   Pw/Qw[p][j][k] (j-th interval of p's support window), their prefix (P)
 //   and suffix (Q) sums and totals, gathered from Pl/Ql.
+#include <algorithm>
+
 #include "bs2e_internal.h"
 #include "bspline_device.h"
 
@@ -46,18 +48,32 @@ size_t outer_lds_bytes(int64_t ks, int64_t kgl, int64_t K)
     return sizeof(double) * (size_t)(kgl * ks + kgl + 2 * kgl * K);
 }
 
-int inner_kc(int64_t ks, int64_t kgl, int64_t K)
-{
-    for (int kc : {4, 2, 1})
-        if (kc <= K && inner_lds_bytes(ks, kgl, kc) <= 64 * 1024) return kc;
-    return inner_lds_bytes(ks, kgl, 1) <= 64 * 1024 ? 1 : 0;
-}
+// Orders up to kMfOrder (U <= 45 local pairs, three 16-wide tiles) with at
+// most 16 GL points contract the diagonal cells on the FP64 matrix cores
+// (kp_inner_mf); other bases use the vector-ALU kernel kp_inner.
+constexpr int kMfOrder = 9;
+constexpr int kMfMaxKc = 16;  // multipoles per matrix-core workgroup (register accumulators)
+constexpr int kMfSteps = 4;   // 4-point steps of the contraction: the matrix-core path takes k_GL <= 16
 
 size_t inner_lds_bytes(int64_t ks, int64_t kgl, int64_t kc)
 {
     const int64_t U = ks * (ks + 1) / 2;
+    if (ks <= kMfOrder && kgl <= 4 * kMfSteps)   // sBo, sBi, sR, sWB, sX0, sXr, sJ[kc]
+        return sizeof(double) * (size_t)(kgl * ks + kgl * kgl * ks + kgl + kgl * U + 2 * kgl * kgl + kc * kgl * U);
     return sizeof(double) *
            (size_t)(kgl * ks + kgl * kgl * ks + kgl + kgl * kgl + kgl * U + kc * kgl * kgl + kc * kgl * U);
+}
+
+int inner_kc(int64_t ks, int64_t kgl, int64_t K)
+{
+    if (ks <= kMfOrder && kgl <= 4 * kMfSteps) {   // all multipoles of an interval in one workgroup when they fit
+        for (int64_t kc = std::min<int64_t>(K, kMfMaxKc); kc >= 1; --kc)
+            if (inner_lds_bytes(ks, kgl, kc) <= 64 * 1024) return (int)kc;
+        return 0;
+    }
+    for (int kc : {4, 2, 1})
+        if (kc <= K && inner_lds_bytes(ks, kgl, kc) <= 64 * 1024) return kc;
+    return inner_lds_bytes(ks, kgl, 1) <= 64 * 1024 ? 1 : 0;
 }
 
 template <int KS>
@@ -201,6 +217,163 @@ __global__ __launch_bounds__(kPThreads) void kp_inner(int K, int kgl, int kc, in
     }
 }
 
+// Matrix-core variant of kp_inner for orders <= kMfOrder.  Per workgroup:
+// interval v and multipoles k0 .. k0+nk-1 (the chunks of one interval sit on
+// one XCD, so the K-contiguous records of D are completed in one L2).
+//   stage 1: splines at the outer and inner points; X_k0[q][p] =
+//            s_q w_p rho^k0 / r_q^(k0+1) (from X_0 by k0 multiplications with
+//            the ratio rho / r_q, so that every chunking gives the same bits);
+//   stage 2: WB[q][u] = w_q B_a B_c (r_q);
+//   stage 3: J'[kk][q][u] = sum_p X_k[q][p] B_b B_d (rho_qp), the powers for
+//            k > k0 by repeated multiplication with the ratio;
+//   stage 4: per 16 x 16 tile of (uo, ui) and multipole, on v_mfma_f64_16x16x4:
+//            D = WB^T J' (SYM: Dsym = WB^T J' + J'^T WB, the two products in
+//            separate accumulators), zero-padded to whole tiles and to 16
+//            GL points; two multipoles per pass.
+template <int KS, bool SYM>
+__global__ __launch_bounds__(kPThreads) void kp_inner_mf(int K, int kgl, int kc, int kchunks,
+                                                         const double* __restrict__ t, const double* __restrict__ bp,
+                                                         const double* __restrict__ x, const double* __restrict__ w,
+                                                         double* __restrict__ D)
+{
+    constexpr int U = KS * (KS + 1) / 2;
+    constexpr int TU = (U + 15) / 16;
+    typedef double d4_t __attribute__((ext_vector_type(4)));
+    extern __shared__ double lds[];
+    double* sBo = lds;                       // [kgl][KS]       outer points
+    double* sBi = sBo + kgl * KS;            // [kgl][kgl][KS]  inner points
+    double* sR = sBi + kgl * kgl * KS;       // [kgl]
+    double* sWB = sR + kgl;                  // [kgl][U]        w_q B_a B_c at r_q
+    double* sX0 = sWB + kgl * U;             // [kgl][kgl]      X_k0
+    double* sXr = sX0 + kgl * kgl;           // [kgl][kgl]      rho / r_q
+    double* sJ = sXr + kgl * kgl;            // [kc][kgl][U]
+
+    const int64_t lg = xcd_block(blockIdx.x, gridDim.x);
+    const int v = (int)(lg / kchunks);
+    const int k0 = (int)(lg - (int64_t)v * kchunks) * kc;
+    const int nk = (K - k0) < kc ? (K - k0) : kc;
+    const int tid = threadIdx.x;
+    const double lim1 = bp[v], lim2 = bp[v + 1];
+    const double h = 0.5 * (lim2 - lim1);
+    const double c = 0.5 * (lim2 + lim1);
+    const int i1 = KS - 1 + (v + 1);
+
+    for (int e = tid; e < kgl * (1 + kgl); e += kPThreads) {
+        if (e < kgl) {
+            const double r = h * x[e] + c;
+            sR[e] = r;
+            splines_at<KS>(t, i1, r, sBo + e * KS, 1);
+        } else {
+            const int q = (e - kgl) / kgl, pp = (e - kgl) - q * kgl;
+            const double rq = h * x[q] + c;
+            const double rho = 0.5 * (rq - lim1) * x[pp] + 0.5 * (rq + lim1);
+            splines_at<KS>(t, i1, rho, sBi + (q * kgl + pp) * KS, 1);
+            // X_k = s_q w_p rho^k / r_q^(k+1) = X_0 (rho / r_q)^k, the same
+            // multiplication sequence from k = 0 for every chunking
+            const double ratio = rho / rq;
+            double xk = 0.5 * (rq - lim1) * w[pp] / rq;
+            for (int j = 0; j < k0; ++j) xk *= ratio;
+            sX0[q * kgl + pp] = xk;
+            sXr[q * kgl + pp] = ratio;
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < kgl * U; e += kPThreads) {
+        const int q = e / U, u = e - q * U;
+        int ta, tc;
+        local_pair(u, KS, ta, tc);
+        sWB[e] = w[q] * sBo[q * KS + ta] * sBo[q * KS + tc];
+    }
+    for (int e = tid; e < kgl * U; e += kPThreads) {
+        const int q = e / U, u = e - q * U;
+        int ta, tc;
+        local_pair(u, KS, ta, tc);
+        double acc[kMfMaxKc];
+#pragma unroll
+        for (int kk = 0; kk < kMfMaxKc; ++kk) acc[kk] = 0.0;
+        const double* bq = sBi + q * kgl * KS;
+        for (int pp = 0; pp < kgl; ++pp) {
+            const double bb = bq[pp * KS + ta] * bq[pp * KS + tc];
+            double xk = sX0[q * kgl + pp];
+            const double ratio = sXr[q * kgl + pp];
+#pragma unroll
+            for (int kk = 0; kk < kMfMaxKc; ++kk)
+                if (kk < nk) {
+                    acc[kk] += xk * bb;
+                    xk *= ratio;
+                }
+        }
+#pragma unroll
+        for (int kk = 0; kk < kMfMaxKc; ++kk)
+            if (kk < nk) sJ[(kk * kgl + q) * U + u] = acc[kk];
+    }
+    __syncthreads();
+    const int wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    // Operands are read from clamped addresses and zeroed outside the U x kgl
+    // range (no lane-predicated loads), so the loads of a pass issue back to
+    // back ahead of its matrix instructions.
+    for (int tile = wave; tile < TU * TU; tile += kPThreads / 64) {
+        const int ta = tile / TU, tb = tile - ta * TU;
+        const int uo_a = ta * 16 + li, ui_b = tb * 16 + li;   // operand row / column of this lane
+        const bool va = uo_a < U, vb = ui_b < U;
+        const int ua = va ? uo_a : U - 1, ub = vb ? ui_b : U - 1;
+        int qo[kMfSteps];
+        bool qv[kMfSteps];
+#pragma unroll
+        for (int st = 0; st < kMfSteps; ++st) {
+            const int q = st * 4 + lk;
+            qv[st] = q < kgl;
+            qo[st] = (qv[st] ? q : kgl - 1) * U;
+        }
+        // WB operands do not depend on the multipole: loaded once per tile
+        double a1[kMfSteps], b2[kMfSteps];
+#pragma unroll
+        for (int st = 0; st < kMfSteps; ++st) {
+            const double x1 = sWB[qo[st] + ua], x2 = sWB[qo[st] + ub];
+            a1[st] = (qv[st] && va) ? x1 : 0.0;
+            b2[st] = (qv[st] && vb) ? x2 : 0.0;
+        }
+        for (int kk = 0; kk < nk; kk += 2) {
+            const bool two = kk + 1 < nk;   // two multipoles per pass: four independent chains
+            const double* J0 = sJ + (size_t)kk * kgl * U;
+            const double* J1 = J0 + (two ? kgl * U : 0);
+            double b10[kMfSteps], b11[kMfSteps], a20[kMfSteps], a21[kMfSteps];
+#pragma unroll
+            for (int st = 0; st < kMfSteps; ++st) {
+                const double y0 = J0[qo[st] + ub], y1 = J1[qo[st] + ub];
+                b10[st] = (qv[st] && vb) ? y0 : 0.0;
+                b11[st] = (qv[st] && vb) ? y1 : 0.0;
+                if (SYM) {
+                    const double z0 = J0[qo[st] + ua], z1 = J1[qo[st] + ua];
+                    a20[st] = (qv[st] && va) ? z0 : 0.0;
+                    a21[st] = (qv[st] && va) ? z1 : 0.0;
+                }
+            }
+            d4_t c0 = {0.0, 0.0, 0.0, 0.0}, c1 = c0, e0 = c0, e1 = c0;
+#pragma unroll
+            for (int st = 0; st < kMfSteps; ++st) {
+                c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[st], b10[st], c0, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[st], b11[st], c1, 0, 0, 0);
+                if (SYM) {
+                    e0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a20[st], b2[st], e0, 0, 0, 0);
+                    e1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a21[st], b2[st], e1, 0, 0, 0);
+                }
+            }
+            // result row (lane >> 4) + 4 r, column lane & 15 of the tile
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int uo = ta * 16 + lk + 4 * r, ui = tb * 16 + li;
+                if (uo < U && ui < U) {
+                    double* dst = D + (((size_t)v * U + uo) * U + ui) * K + k0 + kk;
+                    dst[0] = h * (SYM ? c0[r] + e0[r] : c0[r]);
+                    if (two) dst[1] = h * (SYM ? c1[r] + e1[r] : c1[r]);
+                }
+            }
+        }
+    }
+}
+
 // Per (pair, k): window moments from the local tables, prefix/suffix sums.
 template <int KSMAX>
 __global__ __launch_bounds__(256) void kw_pair_windows(int n_pairs, int K, int ks, int U,
@@ -290,9 +463,14 @@ static void launch_inner(const Plan& p, const DevTables& d, hipStream_t s)
     const int kmax = inner_kc(KS, p.k_gl, p.K);
     const int kc = (p.inner_kc >= 1 && p.inner_kc <= kmax) ? p.inner_kc : kmax;
     const int kchunks = (int)((p.K + kc - 1) / kc);
-    hipLaunchKernelGGL((kp_inner<KS, SYM>), dim3((unsigned)(p.n_iv * kchunks)), dim3(kPThreads),
-                       inner_lds_bytes(KS, p.k_gl, kc), s, (int)p.K, (int)p.k_gl, kc, kchunks, d.t, d.bp, d.x, d.w,
-                       d.D);
+    if (KS <= kMfOrder && p.k_gl <= 4 * kMfSteps)
+        hipLaunchKernelGGL((kp_inner_mf<KS <= kMfOrder ? KS : 2, SYM>), dim3((unsigned)(p.n_iv * kchunks)),
+                           dim3(kPThreads), inner_lds_bytes(KS, p.k_gl, kc), s, (int)p.K, (int)p.k_gl, kc, kchunks, d.t,
+                           d.bp, d.x, d.w, d.D);
+    else
+        hipLaunchKernelGGL((kp_inner<KS, SYM>), dim3((unsigned)(p.n_iv * kchunks)), dim3(kPThreads),
+                           inner_lds_bytes(KS, p.k_gl, kc), s, (int)p.K, (int)p.k_gl, kc, kchunks, d.t, d.bp, d.x,
+                           d.w, d.D);
 }
 
 template <int KS>
