@@ -32,8 +32,10 @@ METRIC = "R^k integrals/sec (mixed vs FP64 GPU) at <=1e-10 rel err; p99 build la
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per k3_far launch, FETCH_SIZE + WRITE_SIZE from rocprofv3 --pmc,
 # each scaled by what the counter reads for a 1 GiB device copy
-# (profiles/rk_build_c2.md); None for workloads not measured that way.
-FAR_TRAFFIC_BYTES = {"c2": 739.94e6}
+# (profiles/rk_build_c2.md), per (workload, fill shape); None for pairs not
+# measured that way.
+FAR_TRAFFIC_BYTES = {("c2", 0): 739.33e6, ("c2", 1): 739.94e6, ("c2", 2): 740.30e6, ("c2", 3): 739.94e6,
+                     ("c2", 4): 739.83e6, ("c2", 5): 739.68e6, ("c2", 6): 739.05e6}
 
 
 def cpu_baseline(w, t, R_gpu, nthreads):
@@ -372,7 +374,7 @@ def main():
                        "samples": int(args.latency_samples)},
         "roofline": {"bound": "hbm", "kernel": "k3_far (rank-one tensor fill)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": FAR_TRAFFIC_BYTES.get(w.name),
+                     "traffic": FAR_TRAFFIC_BYTES.get((w.name, int(tuning[0]))),
                      "algorithmic_bytes": traffic_alg, "elements": far_elems,
                      "avg_launch_ms": t_far * 1e3},
     }
