@@ -193,3 +193,63 @@ def test_release_order_does_not_matter(oracle):
     assert rel <= TOL
     rk2.release()
     c2.close()
+
+
+@pytest.mark.parametrize("name", ["c3", "c5"])
+def test_full_size_tensor_properties(name, ctx, oracle):
+    """The two largest single-GPU configurations (c3: 2.69 GB, c5: 20.5 GB),
+    checked through size-independent properties instead of the element-wise
+    oracle: every banded integral is positive (non-negative B-splines,
+    positive weights and kernel); rows of the pairs (a, c) and (c, a) are
+    equal bit for bit (R(a,b,c,d) = R(c,b,a,d)), as are the columns of
+    (b, d) and (d, b) (R(a,b,c,d) = R(a,d,c,b)); and contracting the whole
+    tensor with the hydrogenic orbitals of Z = 2 (from the oracle's one-
+    electron matrices) gives the textbook Slater integrals F^0(1s,1s) = 5Z/8
+    ... F^2(2p,2p) = 45Z/512 to 1e-9."""
+    import scipy.linalg as sl
+    w = WORKLOADS[name]
+    t = knots(w)
+    print(f"full-size properties {name}", flush=True)
+    rk = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx)
+    R = rk.to_host()
+    n_pairs, K = rk.n_pairs, rk.K
+    rk.release()
+    n_b = oracle.dims(w.k, t)[1]
+    pairs = oracle.pair_table(w.k, n_b)
+    assert R.shape == (len(pairs), len(pairs), w.max_k + 1) == (n_pairs, n_pairs, K)
+    assert bool((R > 0).all())
+    index = {(int(a), int(c)): i for i, (a, c) in enumerate(pairs)}
+    swap = np.array([index[(int(c), int(a))] for a, c in pairs])
+    rng = np.random.default_rng(11)
+    rows = rng.choice(n_pairs, size=64, replace=False)
+    assert np.array_equal(R[rows], R[swap[rows]])
+    assert np.array_equal(R[rows][:, swap], R[rows])
+    Z = 2
+    S = oracle.setup_S(w.k, t, w.k_GL).real
+    orb = {}
+    for l in (0, 1):
+        E, V = sl.eigh(oracle.setup_H1p(w.k, t, w.k_GL, Z, l, cap_eta=0.0).real, S)
+        for nprinc in (1, 2):
+            i = nprinc - 1 - l
+            if i >= 0:
+                assert abs(E[i] + Z * Z / (2.0 * nprinc ** 2)) < 1e-9
+                orb[(nprinc, l)] = V[:, i]
+    flat = R.reshape(n_pairs, n_pairs * K)
+    s1, s2, p2 = orb[(1, 0)], orb[(2, 0)], orb[(2, 1)]
+    names = {id(s1): "1s", id(s2): "2s", id(p2): "2p"}
+    rows_of = {}   # one pass over the tensor per electron-1 pair density
+
+    def slater(k, a, b, c, d):
+        key = (names[id(a)], names[id(c)])
+        if key not in rows_of:
+            A = a[pairs[:, 0] - 1] * c[pairs[:, 1] - 1]
+            rows_of[key] = (A @ flat).reshape(n_pairs, K)
+        B = b[pairs[:, 0] - 1] * d[pairs[:, 1] - 1]
+        return float(rows_of[key][:, k] @ B)
+
+    cases = [(slater(0, s1, s1, s1, s1), 5 * Z / 8), (slater(0, s1, s2, s1, s2), 17 * Z / 81),
+             (slater(0, s1, s2, s2, s1), 16 * Z / 729), (slater(0, s2, s2, s2, s2), 77 * Z / 512),
+             (slater(0, s1, p2, s1, p2), 59 * Z / 243), (slater(1, s1, p2, p2, s1), 112 * Z / 2187),
+             (slater(0, p2, p2, p2, p2), 93 * Z / 512), (slater(2, p2, p2, p2, p2), 45 * Z / 512)]
+    for got, exact in cases:
+        assert abs(got - exact) < 1e-9 * exact, (got, exact)
