@@ -124,6 +124,8 @@ struct BlockArgs {
     const int32_t* flags;        // [n_chan][n_chan]: bit0 direct allowed, bit1 exchange allowed
     const int32_t* c_first;      // pair index of the first pair with column c
     const double* R;             // [n_pairs][n_pairs][K]
+    const PairInfo* pairs;       // windows of the pairs
+    const double *Ptot, *Qtot;   // [n_pairs][K] window totals: the rank-one factors of R
     const double* Hl;            // complex, per l an n_b x n_b column-major matrix
     const double* S;             // complex, n_b x n_b column-major
     int64_t* cntH;               // per row (count pass)

@@ -186,6 +186,9 @@ static BlockArgs set_args(const bs2e_blockset* b, int64_t s, bool fill)
     a.flags = b->d_flags + b->flag_off[s];
     a.c_first = b->rk->d.c_first;
     a.R = b->rk->R;
+    a.pairs = b->rk->d.pairs;
+    a.Ptot = b->rk->d.Ptot;
+    a.Qtot = b->rk->d.Qtot;
     if (!fill) {
         a.cntH = b->d_cnt + r0;
         a.cntS = b->d_cnt + N + r0;

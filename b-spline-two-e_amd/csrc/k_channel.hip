@@ -48,6 +48,32 @@ __device__ __forceinline__ int pair_of(const int32_t* c_first, int ks, int a, in
     return c_first[c] + (a - lo);
 }
 
+// sum_k coef[k] R^k[p][q].  When the windows of p and q do not overlap,
+// R^k[p][q] is the rank-one product the fill kernel stored (Ptot_p Qtot_q if
+// p's window lies left of q's, Qtot_p Ptot_q otherwise: the same single
+// multiplication, so the same bits), read from the small factor tables,
+// which stay in L2, instead of the tensor; overlapping pairs read the tensor.
+__device__ __forceinline__ double r12_row(const BlockArgs& A, int p, int q, const double* __restrict__ coef)
+{
+    const int K = A.K;
+    const double *x, *y;
+    const PairInfo pp = A.pairs[p], pq = A.pairs[q];
+    double t = 0.0;
+    if (pp.v0 + pp.m - 1 < pq.v0) {
+        x = A.Ptot + (size_t)p * K;
+        y = A.Qtot + (size_t)q * K;
+    } else if (pp.v0 > pq.v0 + pq.m - 1) {
+        x = A.Qtot + (size_t)p * K;
+        y = A.Ptot + (size_t)q * K;
+    } else {
+        const double* v = A.R + ((size_t)p * A.n_pairs + q) * K;
+        for (int k = 0; k < K; ++k) t = __dadd_rn(t, __dmul_rn(v[k], coef[k]));
+        return t;
+    }
+    for (int k = 0; k < K; ++k) t = __dadd_rn(t, __dmul_rn(__dmul_rn(x[k], y[k]), coef[k]));
+    return t;
+}
+
 __device__ __forceinline__ Cplx mat(const double* M, int n_b, int r, int c)
 {
     const size_t o = 2 * ((size_t)(c - 1) * n_b + (r - 1));
@@ -111,22 +137,14 @@ __global__ __launch_bounds__(kCThreads) __attribute__((amdgpu_waves_per_eu(5))) 
                         double res = 0.0;
                         // R(a,b,c,d) = R(a,d,c,b): the key [n_a, n_d, n_c, n_b] puts the
                         // wavefront's consecutive n_d on consecutive columns of one row
-                        if (support) {
-                            const double* v = A.R + ((size_t)pair_of(A.c_first, ks, n_a, n_c) * A.n_pairs +
-                                                     pair_of(A.c_first, ks, n_d, n_bb)) * K;
-                            double t = 0.0;
-                            for (int k = 0; k < K; ++k) t = __dadd_rn(t, __dmul_rn(v[k], cd[k]));
-                            res = __dadd_rn(res, t);
-                        }
+                        if (support)
+                            res = __dadd_rn(res, r12_row(A, pair_of(A.c_first, ks, n_a, n_c),
+                                                         pair_of(A.c_first, ks, n_d, n_bb), cd));
                         // exchange R(a,b,d,c) = R(b,a,c,d) = R(b,d,c,a): row pair(n_b, n_c),
                         // consecutive n_d on consecutive columns
-                        if (support_ex) {
-                            const double* v = A.R + ((size_t)pair_of(A.c_first, ks, n_bb, n_c) * A.n_pairs +
-                                                     pair_of(A.c_first, ks, n_d, n_a)) * K;
-                            double t = 0.0;
-                            for (int k = 0; k < K; ++k) t = __dadd_rn(t, __dmul_rn(v[k], ce[k]));
-                            res = __dadd_rn(res, t);
-                        }
+                        if (support_ex)
+                            res = __dadd_rn(res, r12_row(A, pair_of(A.c_first, ks, n_bb, n_c),
+                                                         pair_of(A.c_first, ks, n_d, n_a), ce));
                         h.re = res;
                     }
                     if (one_el) {
