@@ -38,9 +38,11 @@ FAR_TRAFFIC_BYTES = {("c2", 0): 739.33e6, ("c2", 1): 739.94e6, ("c2", 2): 740.30
                      ("c2", 4): 739.83e6, ("c2", 5): 739.68e6, ("c2", 6): 739.05e6}
 
 
-def cpu_baseline(w, t, R_gpu, nthreads):
+def cpu_baseline(w, t, R_gpu, nthreads, block_max_l=None):
     """Oracle (C restatement, OpenMP over k like the reference) on the host:
-    primitives + dense tensor assembly for the same basis."""
+    primitives + dense tensor assembly for the same basis; with block_max_l,
+    also the L = 0 block of bench's `blocks` line (count and fill, one
+    thread: the reference builds each symmetry on one thread)."""
     import numpy as np
 
     from oracle import oracle as O
@@ -55,11 +57,24 @@ def cpu_baseline(w, t, R_gpu, nthreads):
     rel = None
     if R_gpu is not None:   # largest relative error per multipole k (every R^k value is positive)
         rel = [float(x) for x in (np.abs(R_gpu - R) / R).reshape(-1, K).max(axis=0)]
+    blocks = None
+    if block_max_l is not None:
+        S = O.setup_S(w.k, t, w.k_GL)
+        Hl = [O.setup_H1p(w.k, t, w.k_GL, w.Z, l, cap_order=2, cap_r0=0.8 * w.r_max, cap_eta=0.0)
+              for l in range(block_max_l + 1)]
+        cf = O.count_configs(0, 0, block_max_l, n_b, w.k, n_b, n_b, block_max_l)
+        tb = time.perf_counter()
+        blk = O.construct_block(w.k, n_b, 0, w.max_k, True, cf, R, Hl, S)
+        db = time.perf_counter() - tb
+        blocks = dict(value=blk["nnz"][0] / db, unit="H entries/s", cores=1, kind="port",
+                      sample=(f"{w.name} basis, L=0 block, l1,l2 <= {block_max_l} ({len(cf)} configurations, "
+                              f"{blk['nnz'][0]} H entries): the oracle's count and fill passes; {db:.2f} s"),
+                      seconds=db)
     return dict(value=n_int / dt, unit="integrals/s", cores=int(min(nthreads, K)), kind="port",
                 sample=(f"full {w.name} build (primitives + tensor, {n_int} integrals) by the oracle: "
                         f"OpenMP over the {K} multipoles as the reference's !$omp parallel do, "
                         f"serial assembly; {dt:.2f} s"),
-                seconds=dt), rel
+                seconds=dt), rel, blocks
 
 
 def solve_h_max(k, m, Z, r_max, n_b):
@@ -390,8 +405,11 @@ def main():
         out["serving"] = serving_benchmark(args.serve_jobs)
     if not args.no_cpu_baseline:
         nthreads = min(16, os.cpu_count() or 1)
-        cb, rel = cpu_baseline(w, t, R_gpu, nthreads)
+        cb, rel, cblocks = cpu_baseline(w, t, R_gpu, nthreads, block_max_l=2 if args.blocks else None)
         out["cpu_baseline"] = cb
+        if cblocks is not None:
+            out["blocks"]["cpu_baseline"] = cblocks
+            out["blocks"]["speedup_vs_cpu"] = out["blocks"]["H_entries_per_s"] / cblocks["value"]
         out["max_rel_err_vs_oracle"] = max(rel)
         out["max_rel_err_vs_oracle_per_k"] = rel
         out["speedup_vs_cpu"] = value / cb["value"]
