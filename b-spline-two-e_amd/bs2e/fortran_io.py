@@ -7,6 +7,7 @@ and its files read back:
     store_CS / load_CS                    sparse_array_tools.f90:695-742
     store_splines / load_splines          bspline_tools.f90:375-400 (splines.dat)
     store_basis / load_basis              orbital_tools.f90:364-397 (basis.dat, basis_info.txt)
+    read_namelist                         input_tools.f90:765-773 (basis_input.dat, a namelist)
 
 Records follow the gfortran convention (which amdflang's runtime writes as
 well): a 4-byte little-endian length before and after each record; a record
@@ -278,3 +279,41 @@ def load_basis(directory):
         syms.append(sym(l=l, m=m, pi=pi, configs=cf))
     r.close()
     return max_l_1p, max_L, two_el, syms
+
+
+# --------------------------------------------------------------------- namelists
+def read_namelist(path) -> tuple[str, dict]:
+    """A Fortran namelist file such as basis_input.dat (written by
+    `write(unit, nml=basis_input)`, input_tools.f90:765-773): the group name
+    and its variables (lower-case names; integers, reals, complex numbers,
+    logicals and strings converted)."""
+    import re
+    text = open(path).read()
+    m = re.search(r"&(\w+)(.*?)(?:^\s*/|/\s*$)", text, re.S | re.M)
+    if not m:
+        raise ValueError(f"{path}: no namelist group")
+    group, body = m.group(1).lower(), m.group(2)
+    item = re.compile(r"(\w+)\s*=\s*('(?:[^']|'')*'|\"(?:[^\"]|\"\")*\"|\([^)]*\)|[^,\n]+)")
+
+    def value(v):
+        v = v.strip()
+        if v[0] in "'\"":   # a character constant may continue over records: the record ends add nothing
+            return v[1:-1].replace("\r", "").replace("\n", "").replace(v[0] * 2, v[0])
+        if v.startswith("("):
+            re_, im_ = (float(x.replace("d", "e").replace("D", "e")) for x in v[1:-1].split(","))
+            return complex(re_, im_)
+        u = v.upper().strip(".")
+        if u in ("T", "TRUE"):
+            return True
+        if u in ("F", "FALSE"):
+            return False
+        try:
+            return int(v)
+        except ValueError:
+            pass
+        try:
+            return float(v.replace("d", "e").replace("D", "e"))
+        except ValueError:
+            return v   # a string written without delimiters
+
+    return group, {k.lower(): value(v) for k, v in item.findall(body)}

@@ -13,7 +13,7 @@
 ! default; main_basis_setup.f90:122-145), and stores what the reference's
 ! basis_setup stores for them (main_basis_setup.f90:154-161):
 ! res_dir/H_diag.dat, S_diag.dat, D_-1.dat, D_0.dat, D_1.dat, basis.dat,
-! basis_info.txt, splines.dat.
+! basis_info.txt, splines.dat, basis_input.dat.
 !
 ! hs_file (stream): S then H(l) for l = 0..max_l, each n_b x n_b complex(dp)
 ! column-major (the one-electron matrices, mat_els.f90:85-118 and :294-321);
@@ -29,6 +29,11 @@
 !   H_sp: nnz, index_ptr, indices, data;  S_sp: the same
 program bs2e_basis_setup
     use bs2e_mat_els
+    use bs2e_basis_input, only: set_basis_defaults, write_basis_input, get_basis_input, in_dir => basis_output_dir, &
+        in_k => k, in_m => m, in_Z => Z, in_h_max => h_max, in_r_max => r_max, in_k_GL => k_GL, in_max_k => max_k, &
+        in_max_L => max_L, in_max_l_1p => max_l_1p, in_max_l2 => max_l2, in_cap_order => CAP_order, &
+        in_cap_r_0 => CAP_r_0, in_cap_eta => CAP_eta, in_gauge => gauge, in_full => full, in_z_pol => z_pol, &
+        in_two_el => two_el
     implicit none
     integer, parameter :: dp = kind(1.d0)
     integer :: k, m, Z, k_GL, max_k, max_l, L, ipi, ifull, stride
@@ -188,6 +193,31 @@ program bs2e_basis_setup
         end do
         call bas%store(trim(res_dir))
         call splines%store(trim(res_dir))
+        ! the input record of this run (input_tools.f90:765-773); the CAP is
+        ! the one the device one-electron matrices use (order 2 at 2 r_max,
+        ! eta = 0)
+        call set_basis_defaults()
+        in_dir = trim(res_dir)
+        in_k = k
+        in_m = m
+        in_Z = Z
+        in_h_max = h_max
+        in_r_max = r_max
+        in_k_GL = k_GL
+        in_max_k = max_k
+        in_max_L = max_L_all
+        in_max_l_1p = max_l
+        in_max_l2 = max_l
+        in_cap_order = 2
+        in_cap_r_0 = 2.0d0 * r_max
+        in_cap_eta = (0.0d0, 0.0d0)
+        in_gauge = gauge
+        in_full = ifull /= 0
+        in_z_pol = .true.
+        in_two_el = .true.
+        call write_basis_input(trim(res_dir))
+        call get_basis_input(trim(res_dir) // "basis_input.dat")   ! what the other programs read back
+        if (in_k /= k .or. in_max_L /= max_L_all .or. in_gauge /= gauge) error stop "basis_input.dat does not read back"
         write(6, '(a,i0,a)') "stored ", bas%n_sym, " symmetry blocks in " // trim(res_dir)
     end if
 

@@ -993,3 +993,73 @@ contains
     end subroutine setup_r_k_int
 
 end module bs2e_rkh
+
+
+! The basis_setup input record the other programs read back from the result
+! directory (input_tools.f90:7-43 variables and namelist /basis_input/,
+! :254-327 get_basis_input, :765-773 write_basis_input, :825-844
+! set_basis_defaults): same group name, variable names and types.
+module bs2e_basis_input
+    implicit none
+    private
+    integer, parameter :: dp = kind(1.d0)
+    public :: basis_output_dir, k, m, Z, max_k, k_GL, max_L, max_l_1p, max_l2, h_max, r_max, r_2_max, r_all_l
+    public :: CAP_order, CAP_r_0, CAP_eta, gauge, z_pol, full, two_el
+    public :: set_basis_defaults, write_basis_input, get_basis_input
+
+    character(len=64) :: basis_output_dir = ''
+    integer :: k, m, Z, max_k, k_GL, max_L, max_l_1p, max_l2
+    real(dp) :: h_max, r_max, r_2_max, r_all_l
+    integer :: CAP_order
+    real(dp) :: CAP_r_0
+    complex(dp) :: CAP_eta
+    character :: gauge = 'l'
+    logical :: z_pol, full, two_el
+
+    namelist /basis_input/ basis_output_dir, k, m, Z, h_max, r_max, r_2_max, r_all_l, k_GL, CAP_order, CAP_r_0, &
+        CAP_eta, max_L, max_l_1p, max_l2, max_k, gauge, z_pol, full, two_el
+
+contains
+
+    subroutine set_basis_defaults()
+        k = 6
+        m = 3
+        Z = 2
+        h_max = 0.5_dp
+        r_max = 15.0_dp
+        r_2_max = -1.0_dp
+        r_all_l = -1.0_dp
+        k_GL = k + 6
+        CAP_order = 2
+        CAP_r_0 = 10.0_dp
+        CAP_eta = cmplx(1.d-3, 0.d0, kind=dp)
+        max_L = 2
+        max_l_1p = 5
+        max_l2 = 5
+        max_k = 4
+        z_pol = .true.
+        full = .true.
+        two_el = .true.
+    end subroutine set_basis_defaults
+
+    ! Strings are written between double quotes (delim='quote', what
+    ! gfortran's namelist output uses), so that any compiler's namelist
+    ! input reads them back.
+    subroutine write_basis_input(res_dir)
+        character(len=*), intent(in) :: res_dir
+        integer :: unit
+        open(file=res_dir // 'basis_input.dat', newunit=unit, action='write', delim='quote')
+        write(unit, nml=basis_input)
+        close(unit)
+    end subroutine write_basis_input
+
+    subroutine get_basis_input(input_file)
+        character(len=*), intent(in) :: input_file
+        integer :: unit
+        open(file=input_file, newunit=unit, action='read', status='old')
+        read(unit, nml=basis_input)
+        close(unit)
+        if ((gauge /= 'l') .and. (gauge /= 'v')) error stop "get_basis_input: gauge must be l (length) or v (velocity)"
+    end subroutine get_basis_input
+
+end module bs2e_basis_input

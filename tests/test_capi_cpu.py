@@ -174,3 +174,23 @@ def test_block_CS_round_trip(tmp_path):
     assert raw[:11] == (3).to_bytes(4, "little") + b"CSR" + (3).to_bytes(4, "little")
     hdr = np.frombuffer(raw[15:31], "<i8")
     assert hdr.tolist() == [3, 3] and np.frombuffer(raw[39:55], "<i8").tolist() == [9, 10]
+
+
+def test_read_namelist_formats(tmp_path):
+    """basis_input.dat as gfortran and as flang write namelists (input_tools.f90:765-773)."""
+    from bs2e import fortran_io as fio
+    gfortran = ("&BASIS_INPUT\n BASIS_OUTPUT_DIR=\"/data/run one/                                     \",\n K=8,\n M=3,\n"
+                " Z=2,\n H_MAX=  1.5000000000000000     ,\n R_MAX=  15.000000000000000     ,\n"
+                " CAP_ETA=(1.0000000000000000E-003,0.0000000000000000),\n GAUGE=\"v\",\n Z_POL=T,\n FULL=F,\n"
+                " TWO_EL=T,\n /\n")
+    flang = (" &BASIS_INPUT BASIS_OUTPUT_DIR=\"/data/run\n one/\", K=8, M=3, Z=2, H_MAX=1.5, R_MAX=15.,\n"
+             " CAP_ETA=(1.E-3,0.), GAUGE='v', Z_POL=.TRUE., FULL=.FALSE., TWO_EL=.TRUE./\n")
+    for name, text in (("g", gfortran), ("f", flang)):
+        p = tmp_path / f"{name}.dat"
+        p.write_text(text)
+        group, v = fio.read_namelist(p)
+        assert group == "basis_input"
+        assert v["basis_output_dir"].strip() == "/data/run one/"
+        assert (v["k"], v["m"], v["z"]) == (8, 3, 2)
+        assert v["h_max"] == 1.5 and v["r_max"] == 15.0 and v["cap_eta"] == 1e-3 + 0j
+        assert v["gauge"] == "v" and v["z_pol"] is True and v["full"] is False and v["two_el"] is True

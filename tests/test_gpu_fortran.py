@@ -170,6 +170,15 @@ def test_fortran_basis_files_match_reference_format(oracle, tmp_path, gauge, ful
                     assert np.array_equal(got.index_ptr, ip) and np.array_equal(got.indices, ix), (q, i, j)
                     assert np.abs(got.data - d).max() <= TOL * np.abs(d).max(), (q, i, j)
     assert n_nonempty == (4 if full else 2)
+    # basis_input.dat: the run's parameters as the namelist the other programs read
+    group, nml = fio.read_namelist(res + "basis_input.dat")
+    assert group == "basis_input"
+    expect = dict(k=w.k, m=w.m, z=w.Z, k_gl=w.k_GL, max_k=w.max_k, max_l=max_L, max_l_1p=max_l, max_l2=max_l,
+                  gauge=gauge, full=bool(full), z_pol=True, two_el=True, cap_order=2, cap_eta=0j)
+    for key, v in expect.items():
+        assert nml[key] == v, (key, nml[key], v)
+    assert abs(nml["h_max"] - w.h_max) < 1e-12 and abs(nml["r_max"] - w.r_max) < 1e-12
+    assert nml["basis_output_dir"].strip() == res.strip()[:64].strip()
     # the Python writer reproduces the files byte for byte
     py = tmp_path / "py"
     py.mkdir()
