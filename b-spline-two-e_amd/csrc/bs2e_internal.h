@@ -34,6 +34,10 @@ namespace bs2e {
 constexpr int kMaxOrder = 20;   // BVALUE_D work arrays are dimension(20) (bspline_tools.f90:163)
 constexpr int kMaxGL = 64;
 constexpr int kMaxK = 64;       // multipole orders 0..63
+// LDS one workgroup may allocate on gfx950 (the whole 160 KiB of a CU).
+constexpr size_t kLdsPerWorkgroup = 160 * 1024;
+// Lets kernel fn take `bytes` of dynamic LDS (needed beyond 64 KiB).
+hipError_t allow_lds(const void* fn, size_t bytes);
 
 struct PairInfo {               // one banded pair (a, c), 1-based basis indices
     int32_t a, c;

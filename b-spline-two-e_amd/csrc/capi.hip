@@ -326,17 +326,17 @@ static int64_t alloc_tables(bs2e_ctx* ctx, bs2e_rk* rk, bool with_tensor)
 static int64_t check_plan_limits(bs2e_ctx* ctx, const Plan& p, bool needs_primitives)
 {
     if (needs_primitives &&
-        (p.k_s > 16 || inner_kc(p.k_s, p.k_gl, p.K) == 0 || outer_lds_bytes(p.k_s, p.k_gl, p.K) > 64 * 1024))
+        (p.k_s > 16 || inner_kc(p.k_s, p.k_gl, p.K) == 0 || outer_lds_bytes(p.k_s, p.k_gl, p.K) > kLdsPerWorkgroup))
         return fail(ctx, BS2E_EINVAL,
-                    "the per-interval primitive kernels support spline orders <= 16 whose tables fit 64 KiB of LDS "
-                    "(about k_GL^2 * k + k_GL * k^2 doubles; e.g. k <= 12 with k_GL = k+6)");
+                    "the per-interval primitive kernels support spline orders <= 16 whose tables fit 160 KiB of LDS "
+                    "(about k_GL^2 * k + k_GL * k^2 doubles; e.g. k = 16 with k_GL <= 26)");
     if (p.n_pairs * p.K >= ((int64_t)1 << 31)) return fail(ctx, BS2E_EINVAL, "tensor row too long (n_pairs*K >= 2^31)");
     if (p.n_iv * p.U * p.U * p.K >= ((int64_t)1 << 31))
         return fail(ctx, BS2E_EINVAL, "diagonal-cell table too large (n_iv*U^2*K >= 2^31)");
-    if (near_lds_bytes(p.k_s, p.K) > 64 * 1024)
+    if (near_lds_bytes(p.k_s, p.K) > kLdsPerWorkgroup)
         return fail(ctx, BS2E_EINVAL,
                     "spline order x multipole count too large for the near kernel's LDS "
-                    "((2k+2) x (max_k+1) doubles must fit 64 KiB)");
+                    "((2k+2) x (max_k+1) doubles must fit 160 KiB)");
     return BS2E_OK;
 }
 

@@ -43,6 +43,12 @@ __device__ __forceinline__ void local_pair(int u, int ks, int& ta, int& tc)
 
 constexpr int kPThreads = 256;
 
+hipError_t allow_lds(const void* fn, size_t bytes)
+{
+    if (bytes <= 64 * 1024) return hipSuccess;
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
 size_t outer_lds_bytes(int64_t ks, int64_t kgl, int64_t K)
 {
     return sizeof(double) * (size_t)(kgl * ks + kgl + 2 * kgl * K);
@@ -68,12 +74,12 @@ int inner_kc(int64_t ks, int64_t kgl, int64_t K)
 {
     if (ks <= kMfOrder && kgl <= 4 * kMfSteps) {   // all multipoles of an interval in one workgroup when they fit
         for (int64_t kc = std::min<int64_t>(K, kMfMaxKc); kc >= 1; --kc)
-            if (inner_lds_bytes(ks, kgl, kc) <= 64 * 1024) return (int)kc;
+            if (inner_lds_bytes(ks, kgl, kc) <= kLdsPerWorkgroup) return (int)kc;
         return 0;
     }
     for (int kc : {4, 2, 1})
-        if (kc <= K && inner_lds_bytes(ks, kgl, kc) <= 64 * 1024) return kc;
-    return inner_lds_bytes(ks, kgl, 1) <= 64 * 1024 ? 1 : 0;
+        if (kc <= K && inner_lds_bytes(ks, kgl, kc) <= kLdsPerWorkgroup) return kc;
+    return inner_lds_bytes(ks, kgl, 1) <= kLdsPerWorkgroup ? 1 : 0;
 }
 
 template <int KS>
@@ -453,8 +459,10 @@ static inline unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n
 template <int KS>
 static void launch_outer(const Plan& p, const DevTables& d, hipStream_t s)
 {
-    hipLaunchKernelGGL(kp_outer<KS>, dim3((unsigned)p.n_iv), dim3(kPThreads), outer_lds_bytes(KS, p.k_gl, p.K), s,
-                       (int)p.K, (int)p.k_gl, d.t, d.bp, d.x, d.w, d.Pl, d.Ql);
+    const size_t lds = outer_lds_bytes(KS, p.k_gl, p.K);
+    if (allow_lds((const void*)kp_outer<KS>, lds) != hipSuccess) return;   // the error stays for hipGetLastError
+    hipLaunchKernelGGL(kp_outer<KS>, dim3((unsigned)p.n_iv), dim3(kPThreads), lds, s, (int)p.K, (int)p.k_gl, d.t, d.bp,
+                       d.x, d.w, d.Pl, d.Ql);
 }
 
 template <int KS, bool SYM>
@@ -463,14 +471,18 @@ static void launch_inner(const Plan& p, const DevTables& d, hipStream_t s)
     const int kmax = inner_kc(KS, p.k_gl, p.K);
     const int kc = (p.inner_kc >= 1 && p.inner_kc <= kmax) ? p.inner_kc : kmax;
     const int kchunks = (int)((p.K + kc - 1) / kc);
-    if (KS <= kMfOrder && p.k_gl <= 4 * kMfSteps)
-        hipLaunchKernelGGL((kp_inner_mf<KS <= kMfOrder ? KS : 2, SYM>), dim3((unsigned)(p.n_iv * kchunks)),
-                           dim3(kPThreads), inner_lds_bytes(KS, p.k_gl, kc), s, (int)p.K, (int)p.k_gl, kc, kchunks, d.t,
-                           d.bp, d.x, d.w, d.D);
-    else
-        hipLaunchKernelGGL((kp_inner<KS, SYM>), dim3((unsigned)(p.n_iv * kchunks)), dim3(kPThreads),
-                           inner_lds_bytes(KS, p.k_gl, kc), s, (int)p.K, (int)p.k_gl, kc, kchunks, d.t, d.bp, d.x,
-                           d.w, d.D);
+    const size_t lds = inner_lds_bytes(KS, p.k_gl, kc);
+    if (KS <= kMfOrder && p.k_gl <= 4 * kMfSteps) {
+        const auto fn = kp_inner_mf<KS <= kMfOrder ? KS : 2, SYM>;
+        if (allow_lds((const void*)fn, lds) != hipSuccess) return;   // the error stays for hipGetLastError
+        hipLaunchKernelGGL(fn, dim3((unsigned)(p.n_iv * kchunks)), dim3(kPThreads), lds, s, (int)p.K, (int)p.k_gl, kc,
+                           kchunks, d.t, d.bp, d.x, d.w, d.D);
+    } else {
+        const auto fn = kp_inner<KS, SYM>;
+        if (allow_lds((const void*)fn, lds) != hipSuccess) return;
+        hipLaunchKernelGGL(fn, dim3((unsigned)(p.n_iv * kchunks)), dim3(kPThreads), lds, s, (int)p.K, (int)p.k_gl, kc,
+                           kchunks, d.t, d.bp, d.x, d.w, d.D);
+    }
 }
 
 template <int KS>

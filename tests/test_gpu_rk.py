@@ -253,3 +253,34 @@ def test_full_size_tensor_properties(name, ctx, oracle):
              (slater(0, p2, p2, p2, p2), 93 * Z / 512), (slater(2, p2, p2, p2, p2), 45 * Z / 512)]
     for got, exact in cases:
         assert abs(got - exact) < 1e-9 * exact, (got, exact)
+
+
+# orders 2..12 and GL orders on both sides of 16: the matrix-core diagonal
+# cells (order <= 9, k_GL <= 16) and the vector-ALU kernel (the rest)
+# orders 13..16 take more than 64 KiB of LDS per workgroup (up to 160 KiB on gfx950)
+SWEEP = [(2, 4, 0), (3, 5, 2), (5, 11, 4), (6, 17, 3), (9, 15, 6), (9, 18, 2), (10, 16, 5), (11, 17, 4),
+         (12, 18, 3), (14, 20, 3), (16, 22, 2)]
+
+
+@pytest.mark.parametrize("k,k_gl,max_k", SWEEP)
+def test_order_and_gl_sweep_matches_oracle(k, k_gl, max_k, ctx, oracle):
+    """Small bases (r <= 6, about 20-40 B-splines; r <= 3 from order 13) of
+    every order the primitive kernels take, element by element against the
+    oracle: primitives (P, Q, D containers) and the tensor."""
+    t = bs2e.generate_grid(k, 3, 2, 0.5, 6.0 if k <= 12 else 3.0)
+    n_b = oracle.dims(k, t)[1]
+    print(f"sweep k={k} k_GL={k_gl} max_k={max_k} n_b={n_b}", flush=True)
+    spl = bs2e.b_spline(k, t)
+    r_k = bs2e.sparse_4d().init(max_k, spl)
+    r_m_k = bs2e.sparse_4d().init(max_k, spl)
+    r_d_k = bs2e.sparse_6d().init(max_k, spl)
+    bs2e.setup_Slater_integrals(spl, max_k, k_gl, r_k, r_m_k, r_d_k, ctx=ctx)
+    off = oracle.slater_off_diag(k, t, k_gl, max_k)
+    dg = oracle.slater_diag(k, t, k_gl, max_k)
+    for got, ref in ((r_k.data, off["rk"]), (r_m_k.data, off["rmk"]), (r_d_k.data, dg["rdk"])):
+        rel, zp = max_rel(got, ref)
+        assert rel <= TOL and zp == 0.0, (k, k_gl, rel, zp)
+    rk = bs2e.RkTensor.build(k, t, k_gl, max_k, ctx=ctx)
+    rel, zp = max_rel(rk.to_host(), oracle.rk_dense(k, n_b, max_k, off, dg))
+    assert rel <= TOL and zp == 0.0, (k, k_gl, rel, zp)
+    rk.release()
