@@ -87,6 +87,10 @@ int64_t bs2e_generate_grid(int64_t k, int64_t m, int64_t Z, double h_max, double
         return fail(nullptr, BS2E_EINVAL, "bs2e_generate_grid: need k >= 1, 0 <= m <= 60, Z >= 1, h_max > 0, r_max > 0");
     std::vector<double> g;
     int64_t n = generate_grid(k, m, Z, h_max, r_max, g);
+    if (n < 0)
+        return fail(nullptr, BS2E_EINVAL,
+                    "bs2e_generate_grid: the grid does not fit the reference's 2k + m + ceiling(r_max / 2^-m) knots "
+                    "(grid_tools.f90:20; r_max too small for the geometric ramp to Z h_max)");
     if (knots) {
         if (cap < n) return fail(nullptr, BS2E_EINVAL, "bs2e_generate_grid: cap smaller than the knot count");
         std::memcpy(knots, g.data(), sizeof(double) * (size_t)n);

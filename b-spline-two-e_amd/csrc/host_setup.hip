@@ -10,22 +10,33 @@ namespace bs2e {
 
 // grid_tools.f90:6-55 generate_grid: k zeros, m steps of h = 2^-m, geometric
 // growth x(1+h) until the step reaches Z*h_max, linear steps of Z*h_max up to
-// Z*r_max, k-1 repeats of the last knot, everything divided by Z.
+// Z*r_max, k-1 repeats of the last knot, everything divided by Z.  The
+// reference fills an array of 2k + m + ceiling(r_max/h) knots (:20); a grid
+// that would not fit (r_max too small for the geometric ramp) overruns it
+// there, and is refused here: returns -1.
 int64_t generate_grid(int64_t k, int64_t m, int64_t Z, double h_max, double r_max, std::vector<double>& out)
 {
     const double h = std::ldexp(1.0, (int)-m);
     const double step_cap = h_max * (double)Z;
     const double r_end = (double)Z * r_max;
+    const double cap_d = 2.0 * (double)k + (double)m + std::ceil(r_max / h);
+    if (!(cap_d < 1e9)) return -1;
+    const size_t cap = (size_t)cap_d;
     std::vector<double> g((size_t)k, 0.0);
-    g.reserve((size_t)(2 * k + m + std::ceil(r_max / h) + 8));
+    g.reserve(cap);
     for (int64_t i = 0; i < m; ++i) g.push_back(g.back() + h);
     for (;;) {
         double cur = g.back();
         double next = cur * (1.0 + h);
         if ((next - cur) >= step_cap) break;
+        if (g.size() + 1 > cap) return -1;
         g.push_back(next);
     }
-    while (g.back() < r_end) g.push_back(g.back() + (double)Z * h_max);
+    while (g.back() < r_end) {
+        if (g.size() + 1 > cap) return -1;
+        g.push_back(g.back() + (double)Z * h_max);
+    }
+    if (g.size() + (size_t)(k - 1) > cap) return -1;
     double last = g.back();
     for (int64_t i = 0; i < k - 1; ++i) g.push_back(last);
     out.resize(g.size());

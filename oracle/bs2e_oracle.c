@@ -70,7 +70,7 @@ static double ipow(double x, i64 m)
 i64 or_generate_grid(i64 k, i64 m, i64 Z, double h_max, double r_max, double *out, i64 cap)
 {
     double h = ipow(2.0, -m);                                     /* :17 */
-    i64 n_alloc = 2 * k + m + (i64)ceil(r_max / h) + 8;           /* :20 (+slack) */
+    i64 n_alloc = 2 * k + m + (i64)ceil(r_max / h);               /* :20; beyond it the reference overruns */
     double *g = (double *)calloc((size_t)n_alloc + 1, sizeof(double)); /* 1-based */
     i64 i;
     double next;

@@ -110,6 +110,8 @@ def generate_grid(k, m, Z, h_max, r_max):
     """grid_tools.f90:6-55"""
     L = lib()
     n = L.or_generate_grid(k, m, Z, h_max, r_max, None, 0)
+    if n < 0:
+        raise ValueError("the grid overruns the reference's 2k + m + ceiling(r_max / 2^-m) knots (grid_tools.f90:20)")
     out = np.zeros(n)
     L.or_generate_grid(k, m, Z, h_max, r_max, _ptr(out, dbl), n)
     return out

@@ -39,6 +39,36 @@ def test_generate_grid_bit_exact(name):
     assert np.array_equal(bs2e.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max), FX[f"knots_{name}"])
 
 
+def test_generate_grid_capacity_boundary(oracle):
+    """The reference fills 2k + m + ceiling(r_max / 2^-m) knots
+    (grid_tools.f90:20); a small r_max leaves the geometric ramp no room and the
+    reference writes past that array.  The library refuses exactly the inputs
+    the oracle's bounded restatement refuses, and agrees bit for bit on the
+    rest."""
+    import bs2e
+    n_ok = n_bad = 0
+    for k in (2, 4, 8, 12, 16):
+        for m in (0, 3, 5):
+            for Z in (1, 2, 5):
+                for h_max in (0.25, 1.0):
+                    for r_max in (0.05, 0.3, 1.0, 5.0, 40.0):
+                        try:
+                            ref = oracle.generate_grid(k, m, Z, h_max, r_max)
+                        except ValueError:
+                            ref = None
+                        if ref is None:
+                            n_bad += 1
+                            with pytest.raises(bs2e.Bs2eError, match="2k \\+ m"):
+                                bs2e.generate_grid(k, m, Z, h_max, r_max)
+                        else:
+                            n_ok += 1
+                            assert np.array_equal(bs2e.generate_grid(k, m, Z, h_max, r_max), ref), (k, m, Z, h_max,
+                                                                                                    r_max)
+    assert n_ok >= 50 and n_bad >= 50   # m = 0 never leaves 0 and always overruns
+    with pytest.raises(bs2e.Bs2eError):
+        bs2e.generate_grid(4, 3, 2, 0.5, 0.3)
+
+
 @pytest.mark.parametrize("name", ["tiny", "t0", "c1", "c2"])
 def test_count_nnz(oracle, name):
     import bs2e
