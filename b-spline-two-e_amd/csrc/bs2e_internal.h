@@ -119,7 +119,7 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t n)
 }
 
 struct BlockArgs {
-    int n_config, n_chan, full, L, ks, n_b, n_pairs, K;
+    int n_config, n_chan, full, L, ks, n_b, n_pairs, K, n_iv;
     const int64_t* cfg;          // 5 per config: n1, n2, l1, l2, eqv
     const int32_t* cfg_chan;     // channel of each config
     const BlockChan* chan;       // per channel
@@ -128,10 +128,9 @@ struct BlockArgs {
     const int32_t* flags;        // [n_chan][n_chan]: bit0 direct allowed, bit1 exchange allowed
     const int32_t* c_first;      // pair index of the first pair with column c
     const double* R;             // [n_pairs][n_pairs][K]
-    const PairInfo* pairs;       // windows of the pairs
-    const double *Ptot, *Qtot;   // [n_pairs][K] window totals: the rank-one factors of R
-    const double* Hl;            // complex, per l an n_b x n_b column-major matrix
-    const double* S;             // complex, n_b x n_b column-major
+    const double *PtotT, *QtotT; // [K][n_pairs] window totals (multipole-major): the rank-one factors of R
+    const double* Hl;            // complex, per l an n_b x n_b row-major matrix (the transpose of the caller's)
+    const double* S;             // complex, n_b x n_b row-major
     int64_t* cntH;               // per row (count pass)
     int64_t* cntS;
     int64_t* refH;               // per row: the reference's count_nnz predicate
@@ -193,6 +192,12 @@ hipError_t launch_radial_dipole(const Plan& p, const DevTables& d, double* B, do
 // device row offsets row0[n_blocks + 1], n_rows = row0[n_blocks]).
 hipError_t launch_block_count(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, hipStream_t s);
 hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, hipStream_t s);
+// The fill's operand layouts: the caller's column-major one-electron matrices
+// (n_l of them, then S) to row-major, and the window totals [n_pairs][K] to
+// [K][n_pairs].
+hipError_t launch_block_prepare(const double* Hl, const double* S, int n_l, int n_b, const double* Ptot,
+                                const double* Qtot, int n_pairs, int K, double* HlT, double* ST, double* PtotT,
+                                double* QtotT, hipStream_t s);
 
 // orbital_tools.f90:119-216 count_configs: 5 int64 per config (n1, n2, l1, l2,
 // eqv); returns the count (writes at most cap configs when out != nullptr).

@@ -105,7 +105,7 @@ std::string bs2e::channel_tables(const int64_t* configs, int64_t n_config, int64
 // symmetry order; bs2e_blockset_get copies one symmetry's arrays out.
 struct bs2e_blockset {
     bs2e_rk* rk = nullptr;
-    int64_t n_sym = 0, full = 1, max_l = 0, n_rows = 0;
+    int64_t n_sym = 0, full = 1, max_l = 0, n_rows = 0, n_l = 0;   // n_l: one-electron matrices of the last fill
     float ms_count = 0.f, ms_fill = 0.f;   // device time of the count and fill kernels (HIP events)
     std::vector<int64_t> L, row0;
     std::vector<int64_t> chan_off, seg_off, coef_off, flag_off, n_chan;
@@ -130,7 +130,8 @@ struct bs2e_blockset {
     int64_t* d_ptr = nullptr;   // 2 x n_rows: H, S (0-based offsets into the concatenated output)
     BlockArgs* d_tab = nullptr;
     int64_t* d_row0 = nullptr;
-    double *d_Hl = nullptr, *d_S = nullptr;
+    double *d_Hl = nullptr, *d_S = nullptr;   // the caller's column-major one-electron matrices
+    double* d_T = nullptr;                     // the fill's layouts: Hl, S row-major; Ptot, Qtot [K][n_pairs]
     int64_t *d_ixH = nullptr, *d_ixS = nullptr;
     double *d_dH = nullptr, *d_dS = nullptr;
 };
@@ -142,6 +143,7 @@ static void set_drop_outputs(bs2e_blockset* b)
 {
     dev_free(b->d_Hl);
     dev_free(b->d_S);
+    dev_free(b->d_T);
     dev_free(b->d_ixH);
     dev_free(b->d_dH);
     dev_free(b->d_ixS);
@@ -178,6 +180,7 @@ static BlockArgs set_args(const bs2e_blockset* b, int64_t s, bool fill)
     a.n_b = (int)p.n_b;
     a.n_pairs = (int)p.n_pairs;
     a.K = (int)p.K;
+    a.n_iv = (int)p.n_iv;
     a.cfg = b->d_cfg + 5 * r0;
     a.cfg_chan = b->d_cfg_chan + r0;
     a.chan = b->d_chan + b->chan_off[s];
@@ -186,17 +189,17 @@ static BlockArgs set_args(const bs2e_blockset* b, int64_t s, bool fill)
     a.flags = b->d_flags + b->flag_off[s];
     a.c_first = b->rk->d.c_first;
     a.R = b->rk->R;
-    a.pairs = b->rk->d.pairs;
-    a.Ptot = b->rk->d.Ptot;
-    a.Qtot = b->rk->d.Qtot;
     if (!fill) {
         a.cntH = b->d_cnt + r0;
         a.cntS = b->d_cnt + N + r0;
         a.refH = b->d_cnt + 2 * N + r0;
         a.refS = b->d_cnt + 3 * N + r0;
     } else {
-        a.Hl = b->d_Hl;
-        a.S = b->d_S;
+        const size_t nn = (size_t)p.n_b * p.n_b, nf = (size_t)p.n_pairs * p.K;
+        a.Hl = b->d_T;
+        a.S = b->d_T + 2 * nn * (size_t)b->n_l;
+        a.PtotT = a.S + 2 * nn;
+        a.QtotT = a.PtotT + nf;
         a.ptrH = b->d_ptr + r0;
         a.ptrS = b->d_ptr + N + r0;
         a.ixH = b->d_ixH;
@@ -348,6 +351,9 @@ static int64_t set_fill(bs2e_blockset* b, int64_t max_l_1p, const double* Hl, co
     set_drop_outputs(b);
     HIP_TRY(ctx, dev_alloc(&b->d_Hl, 2 * nn * (size_t)(max_l_1p + 1)));
     HIP_TRY(ctx, dev_alloc(&b->d_S, 2 * nn));
+    b->n_l = max_l_1p + 1;
+    const size_t nf = (size_t)p.n_pairs * p.K;
+    HIP_TRY(ctx, dev_alloc(&b->d_T, 2 * nn * (size_t)(b->n_l + 1) + 2 * nf));
     HIP_TRY(ctx, dev_alloc(&b->d_ixH, (size_t)std::max<int64_t>(b->totH, 1)));
     HIP_TRY(ctx, dev_alloc(&b->d_dH, 2 * (size_t)std::max<int64_t>(b->totH, 1)));
     HIP_TRY(ctx, dev_alloc(&b->d_ixS, (size_t)std::max<int64_t>(b->totS, 1)));
@@ -362,6 +368,10 @@ static int64_t set_fill(bs2e_blockset* b, int64_t max_l_1p, const double* Hl, co
     HIP_TRY(ctx, hipEventCreate(&e0));
     HIP_TRY(ctx, hipEventCreate(&e1));
     hipError_t le = hipEventRecord(e0, st);
+    if (le == hipSuccess)
+        le = launch_block_prepare(b->d_Hl, b->d_S, (int)b->n_l, (int)p.n_b, b->rk->d.Ptot, b->rk->d.Qtot,
+                                  (int)p.n_pairs, (int)p.K, b->d_T, b->d_T + 2 * nn * b->n_l,
+                                  b->d_T + 2 * nn * (b->n_l + 1), b->d_T + 2 * nn * (b->n_l + 1) + nf, st);
     if (le == hipSuccess) le = launch_block_fill(b->d_tab, b->d_row0, (int)b->n_sym, b->n_rows, st);
     if (le == hipSuccess) le = hipEventRecord(e1, st);
     if (le == hipSuccess) le = hipEventSynchronize(e1);

@@ -48,36 +48,58 @@ __device__ __forceinline__ int pair_of(const int32_t* c_first, int ks, int a, in
     return c_first[c] + (a - lo);
 }
 
-// sum_k coef[k] R^k[p][q].  When the windows of p and q do not overlap,
-// R^k[p][q] is the rank-one product the fill kernel stored (Ptot_p Qtot_q if
-// p's window lies left of q's, Qtot_p Ptot_q otherwise: the same single
-// multiplication, so the same bits), read from the small factor tables,
-// which stay in L2, instead of the tensor; overlapping pairs read the tensor.
-__device__ __forceinline__ double r12_row(const BlockArgs& A, int p, int q, const double* __restrict__ coef)
+// the support window [lo, hi] of pair (a, c): the intervals where both
+// splines are non-zero, [max(a,c) + 2 - k, min(a,c) + 1] within [1, n_iv]
+__device__ __forceinline__ int window_lo(int a, int c, int ks)
 {
-    const int K = A.K;
-    const double *x, *y;
-    const PairInfo pp = A.pairs[p], pq = A.pairs[q];
+    const int lo = (a > c ? a : c) + 2 - ks;
+    return lo < 1 ? 1 : lo;
+}
+__device__ __forceinline__ int window_hi(int a, int c, int n_iv)
+{
+    const int hi = (a < c ? a : c) + 1;
+    return hi > n_iv ? n_iv : hi;
+}
+
+// sum_k coef[k] R^k[p][q] for p = pair(a1, c1), q = pair(a2, c2).  When the
+// windows of p and q do not overlap, R^k[p][q] is the rank-one product the
+// fill kernel stored (Ptot_p Qtot_q if p's window lies left of q's, Qtot_p
+// Ptot_q otherwise: the same single multiplication, so the same bits), read
+// from the small multipole-major factor tables, which stay in L2 and put a
+// wavefront's consecutive q on consecutive addresses; overlapping pairs read
+// the tensor.  coef is the wavefront's copy of the channel coefficients in LDS.
+__device__ __forceinline__ double r12_row(const BlockArgs& A, int a1, int c1, int a2, int c2,
+                                          const double* __restrict__ coef)
+{
+    const int K = A.K, ks = A.ks;
+    const int p = pair_of(A.c_first, ks, a1, c1), q = pair_of(A.c_first, ks, a2, c2);
+    const bool left = window_hi(a1, c1, A.n_iv) < window_lo(a2, c2, ks);
+    const bool right = window_lo(a1, c1, ks) > window_hi(a2, c2, A.n_iv);
     double t = 0.0;
-    if (pp.v0 + pp.m - 1 < pq.v0) {
-        x = A.Ptot + (size_t)p * K;
-        y = A.Qtot + (size_t)q * K;
-    } else if (pp.v0 > pq.v0 + pq.m - 1) {
-        x = A.Qtot + (size_t)p * K;
-        y = A.Ptot + (size_t)q * K;
-    } else {
+    if (!left && !right) {
         const double* v = A.R + ((size_t)p * A.n_pairs + q) * K;
         for (int k = 0; k < K; ++k) t = __dadd_rn(t, __dmul_rn(v[k], coef[k]));
         return t;
     }
-    for (int k = 0; k < K; ++k) t = __dadd_rn(t, __dmul_rn(__dmul_rn(x[k], y[k]), coef[k]));
+    const double* x = (left ? A.PtotT : A.QtotT) + p;
+    const double* y = (left ? A.QtotT : A.PtotT) + q;
+    const unsigned np = (unsigned)A.n_pairs;   // K * n_pairs < 2^31 (checked on the host)
+    for (int k = 0; k < K; ++k) t = __dadd_rn(t, __dmul_rn(__dmul_rn(x[k * np], y[k * np]), coef[k]));
     return t;
 }
 
+// element (r, c), 1-based, of a complex row-major n_b x n_b matrix
 __device__ __forceinline__ Cplx mat(const double* M, int n_b, int r, int c)
 {
-    const size_t o = 2 * ((size_t)(c - 1) * n_b + (r - 1));
+    const size_t o = 2 * ((size_t)(r - 1) * n_b + (c - 1));
     return Cplx{M[o], M[o + 1]};
+}
+
+__device__ __forceinline__ int64_t uniform64(int64_t x)
+{
+    const int lo = __builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const int hi = __builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
 }
 
 // One wavefront per configuration row of any of n_blocks symmetry blocks:
@@ -88,9 +110,12 @@ template <bool FILL>
 __global__ __launch_bounds__(kCThreads) __attribute__((amdgpu_waves_per_eu(5))) void kc_block(const BlockArgs* __restrict__ tab,
                                                       const int64_t* __restrict__ row0, int n_blocks, int64_t n_rows)
 {
+    __shared__ double sCoef[kCThreads / kWave][2 * kMaxK];   // per wavefront: the channel pair's coefficients
     const int lane = threadIdx.x & (kWave - 1);
-    const int64_t g = ((int64_t)blockIdx.x * kCThreads + threadIdx.x) / kWave;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
+    const int64_t g = (int64_t)blockIdx.x * (kCThreads / kWave) + wave;
     if (g >= n_rows) return;   // whole wave exits together
+    double* sc = sCoef[wave];
     int blo = 0, bhi = n_blocks - 1;
     while (blo < bhi) {
         const int mid = (blo + bhi + 1) >> 1;
@@ -102,11 +127,12 @@ __global__ __launch_bounds__(kCThreads) __attribute__((amdgpu_waves_per_eu(5))) 
     const int i = (int)(g - row0[b]);
     const int ks = A.ks, K = A.K;
     const int64_t* ci = A.cfg + 5 * (size_t)i;
-    const int n_a = (int)ci[0], n_bb = (int)ci[1], l_a = (int)ci[2], l_b = (int)ci[3];
-    const int ch_i = A.cfg_chan[i];
+    const int n_a = __builtin_amdgcn_readfirstlane((int)ci[0]), n_bb = __builtin_amdgcn_readfirstlane((int)ci[1]);
+    const int l_a = __builtin_amdgcn_readfirstlane((int)ci[2]), l_b = __builtin_amdgcn_readfirstlane((int)ci[3]);
+    const int ch_i = __builtin_amdgcn_readfirstlane(A.cfg_chan[i]);
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int64_t nH = 0, nS = 0, rH = 0, rS = 0;
-    const int64_t baseH = FILL ? A.ptrH[i] : 0, baseS = FILL ? A.ptrS[i] : 0;
+    const int64_t baseH = FILL ? uniform64(A.ptrH[i]) : 0, baseS = FILL ? uniform64(A.ptrS[i]) : 0;
 
     for (int ch = 0; ch < A.n_chan; ++ch) {
         const BlockChan cj = A.chan[ch];
@@ -116,8 +142,17 @@ __global__ __launch_bounds__(kCThreads) __attribute__((amdgpu_waves_per_eu(5))) 
         const bool l_eq = (l_a == l_c) && (l_b == l_d);
         const bool l_eq_ex = (l_a == l_d) && (l_b == l_c);
         if (!dir_any && !ex_any && !l_eq && !l_eq_ex) continue;   // no entry can exist in this channel
-        const double* cd = A.coef + ((size_t)ch_i * A.n_chan + ch) * 2 * K;
-        const double* ce = cd + K;
+        if (FILL) {
+            // the wavefront's lanes read the previous channel's copy in program
+            // order before any lane overwrites it (one wavefront, in-order LDS)
+            const double* cg = A.coef + ((size_t)ch_i * A.n_chan + ch) * 2 * K;
+            for (int e = lane; e < 2 * K; e += kWave) sc[e] = cg[e];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        const double* cd = sc;
+        const double* ce = sc + K;
         const BlockSeg* sg = A.seg + (size_t)ch * (A.n_b + 1);
         walk_channel(n_a, n_bb, ks, A.n_b, sg, lane, [&](bool active, int n_c, int n_d, int col, bool in_dir_c,
                                                           bool in_ex_c) {
@@ -137,14 +172,10 @@ __global__ __launch_bounds__(kCThreads) __attribute__((amdgpu_waves_per_eu(5))) 
                         double res = 0.0;
                         // R(a,b,c,d) = R(a,d,c,b): the key [n_a, n_d, n_c, n_b] puts the
                         // wavefront's consecutive n_d on consecutive columns of one row
-                        if (support)
-                            res = __dadd_rn(res, r12_row(A, pair_of(A.c_first, ks, n_a, n_c),
-                                                         pair_of(A.c_first, ks, n_d, n_bb), cd));
+                        if (support) res = __dadd_rn(res, r12_row(A, n_a, n_c, n_d, n_bb, cd));
                         // exchange R(a,b,d,c) = R(b,a,c,d) = R(b,d,c,a): row pair(n_b, n_c),
                         // consecutive n_d on consecutive columns
-                        if (support_ex)
-                            res = __dadd_rn(res, r12_row(A, pair_of(A.c_first, ks, n_bb, n_c),
-                                                         pair_of(A.c_first, ks, n_d, n_a), ce));
+                        if (support_ex) res = __dadd_rn(res, r12_row(A, n_bb, n_c, n_d, n_a, ce));
                         h.re = res;
                     }
                     if (one_el) {
@@ -193,6 +224,40 @@ __global__ __launch_bounds__(kCThreads) __attribute__((amdgpu_waves_per_eu(5))) 
         A.refH[i] = rH;
         A.refS[i] = rS;
     }
+}
+
+// One thread per element: HlT/ST take the transposes of the n_l column-major
+// complex matrices Hl and of S; PtotT/QtotT the transposes of Ptot/Qtot.
+__global__ __launch_bounds__(256) void kc_prepare(const double* __restrict__ Hl, const double* __restrict__ S, int n_l,
+                                                  int n_b, const double* __restrict__ Ptot,
+                                                  const double* __restrict__ Qtot, int n_pairs, int K,
+                                                  double* __restrict__ HlT, double* __restrict__ ST,
+                                                  double* __restrict__ PtotT, double* __restrict__ QtotT)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t nn = (int64_t)n_b * n_b, nm = (int64_t)(n_l + 1) * nn, nf = (int64_t)n_pairs * K;
+    if (e < nm) {
+        const int64_t l = e / nn, rc = e - l * nn;
+        const int64_t r = rc / n_b, c = rc - r * n_b;          // destination (r, c), row-major
+        const double* src = (l < n_l ? Hl + 2 * l * nn : S) + 2 * (c * n_b + r);
+        double* dst = (l < n_l ? HlT + 2 * l * nn : ST) + 2 * rc;
+        dst[0] = src[0];
+        dst[1] = src[1];
+    } else if (e < nm + 2 * nf) {
+        const int64_t f = e - nm, w = f / nf, kq = f - w * nf;
+        const int64_t k = kq / n_pairs, q = kq - k * n_pairs;  // destination [k][q]
+        (w == 0 ? PtotT : QtotT)[kq] = (w == 0 ? Ptot : Qtot)[q * K + k];
+    }
+}
+
+hipError_t launch_block_prepare(const double* Hl, const double* S, int n_l, int n_b, const double* Ptot,
+                                const double* Qtot, int n_pairs, int K, double* HlT, double* ST, double* PtotT,
+                                double* QtotT, hipStream_t s)
+{
+    const int64_t n = (int64_t)(n_l + 1) * n_b * n_b + 2 * (int64_t)n_pairs * K;
+    hipLaunchKernelGGL(kc_prepare, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, Hl, S, n_l, n_b, Ptot, Qtot,
+                       n_pairs, K, HlT, ST, PtotT, QtotT);
+    return hipGetLastError();
 }
 
 hipError_t launch_block_count(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, hipStream_t s)
