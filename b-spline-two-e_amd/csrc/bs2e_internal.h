@@ -191,7 +191,8 @@ hipError_t launch_radial_dipole(const Plan& p, const DevTables& d, double* B, do
 // Count / fill launches over the rows of n_blocks blocks (device table tab,
 // device row offsets row0[n_blocks + 1], n_rows = row0[n_blocks]).
 hipError_t launch_block_count(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, hipStream_t s);
-hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, hipStream_t s);
+hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, int ks, int K,
+                             hipStream_t s);
 // The fill's operand layouts: the caller's column-major one-electron matrices
 // (n_l of them, then S) to row-major, and the window totals [n_pairs][K] to
 // [K][n_pairs].

@@ -372,7 +372,8 @@ static int64_t set_fill(bs2e_blockset* b, int64_t max_l_1p, const double* Hl, co
         le = launch_block_prepare(b->d_Hl, b->d_S, (int)b->n_l, (int)p.n_b, b->rk->d.Ptot, b->rk->d.Qtot,
                                   (int)p.n_pairs, (int)p.K, b->d_T, b->d_T + 2 * nn * b->n_l,
                                   b->d_T + 2 * nn * (b->n_l + 1), b->d_T + 2 * nn * (b->n_l + 1) + nf, st);
-    if (le == hipSuccess) le = launch_block_fill(b->d_tab, b->d_row0, (int)b->n_sym, b->n_rows, st);
+    if (le == hipSuccess)
+        le = launch_block_fill(b->d_tab, b->d_row0, (int)b->n_sym, b->n_rows, (int)p.k_s, (int)p.K, st);
     if (le == hipSuccess) le = hipEventRecord(e1, st);
     if (le == hipSuccess) le = hipEventSynchronize(e1);
     if (le == hipSuccess) le = hipEventElapsedTime(&b->ms_fill, e0, e1);

@@ -88,6 +88,17 @@ __device__ __forceinline__ double r12_row(const BlockArgs& A, int a1, int c1, in
     return t;
 }
 
+// The same sum for a row whose windows never meet (|n_a - n_b| >= k): every
+// product is rank-one with a known side; x and y are the p and q factors in
+// the wavefront's LDS copy (stride L per multipole).
+__device__ __forceinline__ double r12_staged(const double* __restrict__ x, const double* __restrict__ y, int L, int K,
+                                             const double* __restrict__ coef)
+{
+    double t = 0.0;
+    for (int k = 0; k < K; ++k) t = __dadd_rn(t, __dmul_rn(__dmul_rn(x[k * L], y[k * L]), coef[k]));
+    return t;
+}
+
 // element (r, c), 1-based, of a complex row-major n_b x n_b matrix
 __device__ __forceinline__ Cplx mat(const double* M, int n_b, int r, int c)
 {
@@ -102,20 +113,28 @@ __device__ __forceinline__ int64_t uniform64(int64_t x)
     return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
 }
 
+// LDS of one fill wavefront (doubles): the channel pair's 2K coefficients,
+// then the q factors of a row whose windows never meet, [2][K][L] with
+// L = 2k - 1: direct term pairs (d, n_b), exchange term pairs (d, n_a), for
+// the d within k of n_b / n_a.
+__host__ __device__ constexpr int fill_wave_lds(int ks, int K) { return 2 * K + 2 * K * (2 * ks - 1); }
+
 // One wavefront per configuration row of any of n_blocks symmetry blocks:
 // global row g belongs to the last block b with row0[b] <= g and is row
 // g - row0[b] of it; each block's tables, inputs and outputs are in tab[b].
-// Five waves per SIMD keep the fill at 94 VGPRs without spilling.
+// The fill's LDS (fill_wave_lds per wavefront) is dynamic; ks and K are the
+// blocks' common spline order and multipole count.
 template <bool FILL>
-__global__ __launch_bounds__(kCThreads) __attribute__((amdgpu_waves_per_eu(5))) void kc_block(const BlockArgs* __restrict__ tab,
-                                                      const int64_t* __restrict__ row0, int n_blocks, int64_t n_rows)
+__global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restrict__ tab, const int64_t* __restrict__ row0,
+                                                      int n_blocks, int64_t n_rows, int lds_ks, int lds_K)
 {
-    __shared__ double sCoef[kCThreads / kWave][2 * kMaxK];   // per wavefront: the channel pair's coefficients
+    extern __shared__ double lds_fill[];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
     const int64_t g = (int64_t)blockIdx.x * (kCThreads / kWave) + wave;
     if (g >= n_rows) return;   // whole wave exits together
-    double* sc = sCoef[wave];
+    double* sc = lds_fill + (size_t)wave * fill_wave_lds(lds_ks, lds_K);   // coefficients
+    double* sy = sc + 2 * lds_K;                                           // staged q factors
     int blo = 0, bhi = n_blocks - 1;
     while (blo < bhi) {
         const int mid = (blo + bhi + 1) >> 1;
@@ -133,6 +152,33 @@ __global__ __launch_bounds__(kCThreads) __attribute__((amdgpu_waves_per_eu(5))) 
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int64_t nH = 0, nS = 0, rH = 0, rS = 0;
     const int64_t baseH = FILL ? uniform64(A.ptrH[i]) : 0, baseS = FILL ? uniform64(A.ptrS[i]) : 0;
+    // |n_a - n_b| >= k: the direct pairs (n_a, n_c), (n_d, n_b) and the
+    // exchange pairs (n_b, n_c), (n_d, n_a) have disjoint windows, on a side
+    // fixed by the row: n_a < n_b puts every direct p left of its q
+    // (Ptot_p Qtot_q) and every exchange p right of its q (Qtot_p Ptot_q).
+    // The staged q factors serve the p side too: the window totals of (a, c)
+    // and (c, a) are the same sums of the same local moments (one unordered
+    // local pair per interval), so the direct p = (n_a, n_c) reads the
+    // exchange q run at (n_c, n_a), and the exchange p = (n_b, n_c) the
+    // direct q run at (n_c, n_b).
+    const bool staged = FILL && (n_bb - n_a >= ks || n_a - n_bb >= ks);
+    const bool a_low = n_bb > n_a;
+    const int L = 2 * ks - 1;
+    const unsigned np = (unsigned)A.n_pairs;   // K * n_pairs < 2^31 (checked on the host)
+    const int dlo_dir = n_bb - ks + 1 < 1 ? 1 : n_bb - ks + 1, dlo_ex = n_a - ks + 1 < 1 ? 1 : n_a - ks + 1;
+    if (staged) {
+        // the q of the direct term are the pairs of column n_b, those of the
+        // exchange term the pairs of column n_a: runs of consecutive pairs
+        const int q_dir = A.c_first[n_bb], len_dir = A.c_first[n_bb + 1] - q_dir;
+        const int q_ex = A.c_first[n_a], len_ex = A.c_first[n_a + 1] - q_ex;
+        const double* src_dir = a_low ? A.QtotT : A.PtotT;
+        const double* src_ex = a_low ? A.PtotT : A.QtotT;
+        for (int e = lane; e < 2 * K * L; e += kWave) {
+            const int t = e >= K * L, r = e - t * K * L, k = r / L, dl = r - k * L;
+            if (dl < (t ? len_ex : len_dir))
+                sy[e] = (t ? src_ex : src_dir)[(unsigned)k * np + (unsigned)((t ? q_ex : q_dir) + dl)];
+        }
+    }
 
     for (int ch = 0; ch < A.n_chan; ++ch) {
         const BlockChan cj = A.chan[ch];
@@ -172,10 +218,16 @@ __global__ __launch_bounds__(kCThreads) __attribute__((amdgpu_waves_per_eu(5))) 
                         double res = 0.0;
                         // R(a,b,c,d) = R(a,d,c,b): the key [n_a, n_d, n_c, n_b] puts the
                         // wavefront's consecutive n_d on consecutive columns of one row
-                        if (support) res = __dadd_rn(res, r12_row(A, n_a, n_c, n_d, n_bb, cd));
+                        if (support)
+                            res = __dadd_rn(res, staged ? r12_staged(sy + K * L + (n_c - dlo_ex), sy + (n_d - dlo_dir), L,
+                                                                     K, cd)
+                                                        : r12_row(A, n_a, n_c, n_d, n_bb, cd));
                         // exchange R(a,b,d,c) = R(b,a,c,d) = R(b,d,c,a): row pair(n_b, n_c),
                         // consecutive n_d on consecutive columns
-                        if (support_ex) res = __dadd_rn(res, r12_row(A, n_bb, n_c, n_d, n_a, ce));
+                        if (support_ex)
+                            res = __dadd_rn(res, staged ? r12_staged(sy + (n_c - dlo_dir), sy + K * L + (n_d - dlo_ex), L,
+                                                                     K, ce)
+                                                        : r12_row(A, n_bb, n_c, n_d, n_a, ce));
                         h.re = res;
                     }
                     if (one_el) {
@@ -264,15 +316,19 @@ hipError_t launch_block_count(const BlockArgs* tab, const int64_t* row0, int n_b
 {
     const int64_t threads = n_rows * kWave;
     hipLaunchKernelGGL(kc_block<false>, dim3((unsigned)((threads + kCThreads - 1) / kCThreads)), dim3(kCThreads), 0, s,
-                       tab, row0, n_blocks, n_rows);
+                       tab, row0, n_blocks, n_rows, 0, 0);
     return hipGetLastError();
 }
 
-hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, hipStream_t s)
+hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, int ks, int K,
+                             hipStream_t s)
 {
     const int64_t threads = n_rows * kWave;
-    hipLaunchKernelGGL(kc_block<true>, dim3((unsigned)((threads + kCThreads - 1) / kCThreads)), dim3(kCThreads), 0, s,
-                       tab, row0, n_blocks, n_rows);
+    const size_t lds = sizeof(double) * (size_t)(kCThreads / kWave) * (size_t)fill_wave_lds(ks, K);
+    hipError_t e = allow_lds((const void*)kc_block<true>, lds);   // <= 4 x 32 KiB (k <= 16, K <= 64)
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kc_block<true>, dim3((unsigned)((threads + kCThreads - 1) / kCThreads)), dim3(kCThreads), lds,
+                       s, tab, row0, n_blocks, n_rows, ks, K);
     return hipGetLastError();
 }
 

@@ -68,6 +68,30 @@ def test_block_matches_oracle(c1, oracle, L, full):
     compare(H_sp, S_sp, ref)
 
 
+@pytest.mark.parametrize("L", [0, 1, 2])
+def test_block_on_device_tensor(c1, oracle, L):
+    """Fed the device's own tensor, the oracle's block assembly and the
+    device's agree to 1e-14 of the block's largest entry: what remains are
+    the angular factors (two independent 3j/6j evaluations, ~1 ulp) and, for
+    rows with |n1 - n2| < k, the exchange term read as the tensor element
+    (q, p) through R(a,b,d,c) = R(b,a,c,d).  The fill forms rank-one elements
+    from the factor tables, with the p factors of a far-only row taken
+    through the pair symmetry (a, c) <-> (c, a) (pinned bit for bit in
+    test_gpu_rk)."""
+    w, t, n_b, rk, Rref, S, Hl = c1
+    term = bs2e.sym(l=L, pi=bool(L % 2))
+    term.configs = bs2e.count_configs(term, 2, n_b, w.k, n_b, n_b, 2)
+    print(f"block on device tensor c1 L={L} n_config={term.n_config}", flush=True)
+    ref = oracle.construct_block(w.k, n_b, L, w.max_k, True, term.configs, rk.to_host(), Hl, S)
+    blk = bs2e.Block(rk, term, True)
+    H_sp, S_sp = blk.fill(Hl, S)
+    blk.release()
+    assert np.array_equal(H_sp.index_ptr, ref["H"][0]) and np.array_equal(H_sp.indices, ref["H"][1])
+    assert np.array_equal(S_sp.data, ref["S"][2])
+    err = np.abs(H_sp.data - ref["H"][2]).max()
+    assert err <= 1e-14 * np.abs(ref["H"][2]).max(), err
+
+
 def test_block_truncated_configs_and_cap(c1, oracle):
     """Truncated configuration lists (max_n_b, n_all_l, l_2_max) and a complex
     absorbing potential in the one-electron blocks."""
