@@ -23,6 +23,10 @@
 #include "bs2e_internal.h"
 #include "channel_walk.h"
 
+// Every product and sum below is rounded on its own, as the reference's
+// Fortran evaluates them: no fused multiply-adds.
+#pragma clang fp contract(off)
+
 namespace bs2e {
 
 constexpr int kCThreads = 256;
@@ -36,11 +40,10 @@ struct Cplx {
 // double complex arithmetic: (a+bi)(c+di) = (ac-bd) + (ad+bc)i
 __device__ __forceinline__ Cplx cmul(Cplx a, Cplx b)
 {
-    return Cplx{__dsub_rn(__dmul_rn(a.re, b.re), __dmul_rn(a.im, b.im)),
-                __dadd_rn(__dmul_rn(a.re, b.im), __dmul_rn(a.im, b.re))};
+    return Cplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
-__device__ __forceinline__ Cplx cadd(Cplx a, Cplx b) { return Cplx{__dadd_rn(a.re, b.re), __dadd_rn(a.im, b.im)}; }
-__device__ __forceinline__ Cplx cscale(double s, Cplx a) { return Cplx{__dmul_rn(s, a.re), __dmul_rn(s, a.im)}; }
+__device__ __forceinline__ Cplx cadd(Cplx a, Cplx b) { return Cplx{a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ Cplx cscale(double s, Cplx a) { return Cplx{s * a.re, s * a.im}; }
 
 __device__ __forceinline__ int pair_of(const int32_t* c_first, int ks, int a, int c)
 {
@@ -61,6 +64,43 @@ __device__ __forceinline__ int window_hi(int a, int c, int n_iv)
     return hi > n_iv ? n_iv : hi;
 }
 
+// sum_k v[k * sv] coef[k] and sum_k (x[k * sx] y[k * sy]) coef[k], k
+// ascending, each operation rounded; four multipoles' operands are loaded
+// before they are combined, so that their loads overlap.
+__device__ __forceinline__ double kdot2(const double* __restrict__ v, const double* __restrict__ coef, int K)
+{
+    double t = 0.0;
+    int k = 0;
+    for (; k + 4 <= K; k += 4) {
+        const double v0 = v[k], v1 = v[k + 1], v2 = v[k + 2], v3 = v[k + 3];
+        const double c0 = coef[k], c1 = coef[k + 1], c2 = coef[k + 2], c3 = coef[k + 3];
+        t = t + v0 * c0;
+        t = t + v1 * c1;
+        t = t + v2 * c2;
+        t = t + v3 * c3;
+    }
+    for (; k < K; ++k) t = t + v[k] * coef[k];
+    return t;
+}
+__device__ __forceinline__ double kdot3(const double* __restrict__ x, unsigned sx, const double* __restrict__ y,
+                                        unsigned sy, const double* __restrict__ coef, int K)
+{
+    double t = 0.0;
+    int k = 0;
+    for (; k + 4 <= K; k += 4) {
+        const unsigned u = (unsigned)k;
+        const double x0 = x[u * sx], x1 = x[(u + 1) * sx], x2 = x[(u + 2) * sx], x3 = x[(u + 3) * sx];
+        const double y0 = y[u * sy], y1 = y[(u + 1) * sy], y2 = y[(u + 2) * sy], y3 = y[(u + 3) * sy];
+        const double c0 = coef[k], c1 = coef[k + 1], c2 = coef[k + 2], c3 = coef[k + 3];
+        t = t + (x0 * y0) * c0;
+        t = t + (x1 * y1) * c1;
+        t = t + (x2 * y2) * c2;
+        t = t + (x3 * y3) * c3;
+    }
+    for (; k < K; ++k) t = t + (x[(unsigned)k * sx] * y[(unsigned)k * sy]) * coef[k];
+    return t;
+}
+
 // sum_k coef[k] R^k[p][q] for p = pair(a1, c1), q = pair(a2, c2).  When the
 // windows of p and q do not overlap, R^k[p][q] is the rank-one product the
 // fill kernel stored (Ptot_p Qtot_q if p's window lies left of q's, Qtot_p
@@ -75,17 +115,11 @@ __device__ __forceinline__ double r12_row(const BlockArgs& A, int a1, int c1, in
     const int p = pair_of(A.c_first, ks, a1, c1), q = pair_of(A.c_first, ks, a2, c2);
     const bool left = window_hi(a1, c1, A.n_iv) < window_lo(a2, c2, ks);
     const bool right = window_lo(a1, c1, ks) > window_hi(a2, c2, A.n_iv);
-    double t = 0.0;
-    if (!left && !right) {
-        const double* v = A.R + ((size_t)p * A.n_pairs + q) * K;
-        for (int k = 0; k < K; ++k) t = __dadd_rn(t, __dmul_rn(v[k], coef[k]));
-        return t;
-    }
+    if (!left && !right) return kdot2(A.R + ((size_t)p * A.n_pairs + q) * K, coef, K);
     const double* x = (left ? A.PtotT : A.QtotT) + p;
     const double* y = (left ? A.QtotT : A.PtotT) + q;
     const unsigned np = (unsigned)A.n_pairs;   // K * n_pairs < 2^31 (checked on the host)
-    for (int k = 0; k < K; ++k) t = __dadd_rn(t, __dmul_rn(__dmul_rn(x[k * np], y[k * np]), coef[k]));
-    return t;
+    return kdot3(x, np, y, np, coef, K);
 }
 
 // The same sum for a row whose windows never meet (|n_a - n_b| >= k): every
@@ -94,9 +128,7 @@ __device__ __forceinline__ double r12_row(const BlockArgs& A, int a1, int c1, in
 __device__ __forceinline__ double r12_staged(const double* __restrict__ x, const double* __restrict__ y, int L, int K,
                                              const double* __restrict__ coef)
 {
-    double t = 0.0;
-    for (int k = 0; k < K; ++k) t = __dadd_rn(t, __dmul_rn(__dmul_rn(x[k * L], y[k * L]), coef[k]));
-    return t;
+    return kdot3(x, (unsigned)L, y, (unsigned)L, coef, K);
 }
 
 // element (r, c), 1-based, of a complex row-major n_b x n_b matrix
@@ -219,13 +251,13 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
                         // R(a,b,c,d) = R(a,d,c,b): the key [n_a, n_d, n_c, n_b] puts the
                         // wavefront's consecutive n_d on consecutive columns of one row
                         if (support)
-                            res = __dadd_rn(res, staged ? r12_staged(sy + K * L + (n_c - dlo_ex), sy + (n_d - dlo_dir), L,
+                            res = res + (staged ? r12_staged(sy + K * L + (n_c - dlo_ex), sy + (n_d - dlo_dir), L,
                                                                      K, cd)
                                                         : r12_row(A, n_a, n_c, n_d, n_bb, cd));
                         // exchange R(a,b,d,c) = R(b,a,c,d) = R(b,d,c,a): row pair(n_b, n_c),
                         // consecutive n_d on consecutive columns
                         if (support_ex)
-                            res = __dadd_rn(res, staged ? r12_staged(sy + (n_c - dlo_dir), sy + K * L + (n_d - dlo_ex), L,
+                            res = res + (staged ? r12_staged(sy + (n_c - dlo_dir), sy + K * L + (n_d - dlo_ex), L,
                                                                      K, ce)
                                                         : r12_row(A, n_bb, n_c, n_d, n_a, ce));
                         h.re = res;

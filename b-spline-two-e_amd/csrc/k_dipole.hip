@@ -24,6 +24,9 @@
 #include "bspline_device.h"
 #include "channel_walk.h"
 
+// Every product and sum below is rounded on its own: no fused multiply-adds.
+#pragma clang fp contract(off)
+
 namespace bs2e {
 
 constexpr int kDThreads = 256;
@@ -75,11 +78,11 @@ __global__ __launch_bounds__(256) void kr_assemble(int n_b, int n_iv, int ks, in
                 const double bi = B[o * ks + ta], bj = B[o * ks + tc];
                 const double r = rw[2 * o], wq = rw[2 * o + 1];
                 if (gauge == 0) {
-                    a += __dmul_rn(__dmul_rn(__dmul_rn(wq, r), bi), bj);
+                    a += ((wq * r) * bi) * bj;
                 } else {
                     const double mw = -wq;   // (0,-1) w: imaginary part
-                    a += __dmul_rn(__dmul_rn(mw, bi), B1[o * ks + tc]);
-                    bsum += __dmul_rn(__dmul_rn(mw, bi), bj) / r;
+                    a += (mw * bi) * B1[o * ks + tc];
+                    bsum += ((mw * bi) * bj) / r;
                 }
             }
         }
@@ -137,12 +140,11 @@ struct DCplx {
 };
 __device__ __forceinline__ DCplx dmul(DCplx a, DCplx b)
 {
-    return DCplx{__dsub_rn(__dmul_rn(a.re, b.re), __dmul_rn(a.im, b.im)),
-                 __dadd_rn(__dmul_rn(a.re, b.im), __dmul_rn(a.im, b.re))};
+    return DCplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
-__device__ __forceinline__ DCplx dadd(DCplx a, DCplx b) { return DCplx{__dadd_rn(a.re, b.re), __dadd_rn(a.im, b.im)}; }
-__device__ __forceinline__ DCplx dsub(DCplx a, DCplx b) { return DCplx{__dsub_rn(a.re, b.re), __dsub_rn(a.im, b.im)}; }
-__device__ __forceinline__ DCplx dscale(double s, DCplx a) { return DCplx{__dmul_rn(s, a.re), __dmul_rn(s, a.im)}; }
+__device__ __forceinline__ DCplx dadd(DCplx a, DCplx b) { return DCplx{a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ DCplx dsub(DCplx a, DCplx b) { return DCplx{a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ DCplx dscale(double s, DCplx a) { return DCplx{s * a.re, s * a.im}; }
 __device__ __forceinline__ DCplx dmat(const double* M, int n_b, int r, int c)
 {
     const size_t o = 2 * ((size_t)(c - 1) * n_b + (r - 1));

@@ -127,12 +127,13 @@ __global__ __launch_bounds__(256) void ko_rk_int(int n_b, int n_iv, int ks, int 
             const int ta = ib + 1 - iv, tc = jb + 1 - iv;
             double res = 0.0;
             for (int q = 0; q < kgl; ++q) {
+#pragma clang fp contract(off)
                 const size_t o = (size_t)(iv - 1) * kgl + q;
                 const double r = rw[2 * o], wq = rw[2 * o + 1];
                 const double rk = r > r0 ? r0k / ipow_d(r, k + 1) : ipow_d(r, k) / r0k1;
-                res = __dadd_rn(res, __dmul_rn(__dmul_rn(__dmul_rn(wq, rk), B[o * ks + ta]), B[o * ks + tc]));
+                res = res + ((wq * rk) * B[o * ks + ta]) * B[o * ks + tc];   // each product rounded: no FMA
             }
-            acc = __dadd_rn(acc, res);
+            acc = acc + res;
         }
     }
     out[e] = acc;
