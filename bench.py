@@ -78,15 +78,31 @@ def cpu_baseline(w, t, R_gpu, nthreads, block_max_l=None):
 
 
 def solve_h_max(k, m, Z, r_max, n_b):
-    """h_max whose grid (grid_tools.f90:6-55 rule) gives exactly n_b B-splines, or None."""
+    """h_max whose grid (grid_tools.f90:6-55 rule) gives exactly n_b B-splines, or None.
+    generate_grid refuses grids that overrun the reference's knot array (a
+    linear part finer than 2^-m, or a geometric ramp longer than r_max); the
+    search stays inside the h_max range it accepts."""
     import bs2e
-    nb = lambda h: len(bs2e.generate_grid(k, m, Z, h, r_max)) - k - 2   # noqa: E731
+
+    def nb(h):
+        try:
+            return len(bs2e.generate_grid(k, m, Z, h, r_max)) - k - 2
+        except bs2e.Bs2eError:
+            return None
+
     lo, hi = r_max / 4000.0, r_max
-    if nb(lo) < n_b or nb(hi) > n_b:
+    while nb(lo) is None and lo < hi:
+        lo *= 1.25
+    while nb(hi) is None and hi > lo:
+        hi /= 1.25
+    n_lo, n_hi = nb(lo), nb(hi)
+    if n_lo is None or n_hi is None or n_lo < n_b or n_hi > n_b:
         return None
     for _ in range(80):
         mid = 0.5 * (lo + hi)
         v = nb(mid)
+        if v is None:
+            return None
         if v == n_b:
             return mid
         if v > n_b:
@@ -104,7 +120,9 @@ def serving_jobs(n, seed):
     import bs2e
     rng = np.random.default_rng(seed)
     jobs = []
-    while len(jobs) < n:
+    for _ in range(100 * n):
+        if len(jobs) == n:
+            break
         n_b = int(rng.integers(40, 401))
         k = int(rng.choice([6, 7, 8, 9]))
         r_max = float(rng.uniform(50.0, 500.0))
@@ -113,6 +131,8 @@ def serving_jobs(n, seed):
         if h is None:
             continue
         jobs.append((k, bs2e.generate_grid(k, 3, 2, h, r_max), k + 6, 2 * lmax))
+    if len(jobs) < n:
+        raise RuntimeError(f"serving_jobs: only {len(jobs)} of {n} jobs have a grid with the drawn size")
     return jobs
 
 
