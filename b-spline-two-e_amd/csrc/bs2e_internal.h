@@ -175,7 +175,7 @@ struct DipArgs {
     const BlockChan* chan2;     // column channels
     const BlockSeg* seg2;       // [n_chan2][n_b + 1]
     const DipCoef* coef;        // [n_chan1][n_chan2]
-    const double *S, *rad1, *rad2;   // complex n_b x n_b column-major (rad: r_mat, or dr_mat and r_inv_mat)
+    const double *S, *rad1, *rad2;   // complex n_b x n_b row-major (rad: r_mat, or dr_mat and r_inv_mat)
     int64_t* cnt;               // count pass
     const int64_t* ptr;         // fill pass: 0-based row starts
     int64_t* ix;
@@ -184,6 +184,10 @@ struct DipArgs {
 
 hipError_t launch_dip_count(const DipArgs& a, hipStream_t s);
 hipError_t launch_dip_fill(const DipArgs& a, hipStream_t s);
+// the fill's layouts: the caller's column-major S, rad1 and (R2 != NULL) rad2
+// transposed to row-major
+hipError_t launch_dip_prepare(const double* S, const double* R1, const double* R2, int n_b, double* ST, double* R1T,
+                              double* R2T, hipStream_t s);
 // radial dipole matrices: gauge 0 -> M1 = r_mat; 1 -> M1 = dr_mat, M2 = r_inv_mat
 hipError_t launch_radial_dipole(const Plan& p, const DevTables& d, double* B, double* B1, double* rw, int gauge,
                                 double* M1, double* M2, hipStream_t s);

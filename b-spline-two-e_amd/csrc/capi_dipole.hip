@@ -289,9 +289,10 @@ int64_t bs2e_dip_block_fill(bs2e_dip* b, int64_t gauge, const double* S, const d
     for (int64_t i = 0; i < b->n1; ++i) ip[i + 1] = ip[i] + b->cnt[i];
     if (b->nnz == 0) return BS2E_OK;
     const size_t nn = (size_t)b->n_b * b->n_b;
-    double *dS = nullptr, *dR1 = nullptr, *dR2 = nullptr, *dd = nullptr;
+    double *dS = nullptr, *dR1 = nullptr, *dR2 = nullptr, *dd = nullptr, *dT = nullptr;
     int64_t* dix = nullptr;
     auto cleanup = [&]() {
+        dev_free(dT);
         dev_free(dS);
         dev_free(dR1);
         dev_free(dR2);
@@ -303,6 +304,7 @@ int64_t bs2e_dip_block_fill(bs2e_dip* b, int64_t gauge, const double* S, const d
         HIP_TRY(ctx, dev_alloc(&dS, 2 * nn));
         HIP_TRY(ctx, dev_alloc(&dR1, 2 * nn));
         if (gauge == 1) HIP_TRY(ctx, dev_alloc(&dR2, 2 * nn));
+        HIP_TRY(ctx, dev_alloc(&dT, 2 * nn * (gauge == 1 ? 3 : 2)));   // row-major S, rad1 (, rad2)
         HIP_TRY(ctx, dev_alloc(&dd, 2 * (size_t)b->nnz));
         HIP_TRY(ctx, dev_alloc(&dix, (size_t)b->nnz));
         hipStream_t s = ctx->stream;
@@ -312,9 +314,9 @@ int64_t bs2e_dip_block_fill(bs2e_dip* b, int64_t gauge, const double* S, const d
         clear_stale_error();
         DipArgs a = dip_args(b);
         a.gauge = (int)gauge;
-        a.S = dS;
-        a.rad1 = dR1;
-        a.rad2 = gauge == 1 ? dR2 : dR1;
+        a.S = dT;
+        a.rad1 = dT + 2 * nn;
+        a.rad2 = gauge == 1 ? dT + 4 * nn : a.rad1;
         a.ptr = b->d_ptr;
         a.ix = dix;
         a.data = dd;
@@ -322,6 +324,9 @@ int64_t bs2e_dip_block_fill(bs2e_dip* b, int64_t gauge, const double* S, const d
         HIP_TRY(ctx, hipEventCreate(&e0));
         HIP_TRY(ctx, hipEventCreate(&e1));
         hipError_t le = hipEventRecord(e0, s);
+        if (le == hipSuccess)
+            le = launch_dip_prepare(dS, dR1, gauge == 1 ? dR2 : nullptr, (int)b->n_b, dT, dT + 2 * nn,
+                                    gauge == 1 ? dT + 4 * nn : nullptr, s);
         if (le == hipSuccess) le = launch_dip_fill(a, s);
         if (le == hipSuccess) le = hipEventRecord(e1, s);
         if (le == hipSuccess) le = hipEventSynchronize(e1);

@@ -145,9 +145,11 @@ __device__ __forceinline__ DCplx dmul(DCplx a, DCplx b)
 __device__ __forceinline__ DCplx dadd(DCplx a, DCplx b) { return DCplx{a.re + b.re, a.im + b.im}; }
 __device__ __forceinline__ DCplx dsub(DCplx a, DCplx b) { return DCplx{a.re - b.re, a.im - b.im}; }
 __device__ __forceinline__ DCplx dscale(double s, DCplx a) { return DCplx{s * a.re, s * a.im}; }
+// element (r, c), 1-based, of a complex row-major n_b x n_b matrix: a
+// wavefront's consecutive n_d are consecutive columns
 __device__ __forceinline__ DCplx dmat(const double* M, int n_b, int r, int c)
 {
-    const size_t o = 2 * ((size_t)(c - 1) * n_b + (r - 1));
+    const size_t o = 2 * ((size_t)(r - 1) * n_b + (c - 1));
     return DCplx{M[o], M[o + 1]};
 }
 
@@ -166,12 +168,12 @@ template <bool FILL>
 __global__ __launch_bounds__(kDThreads) void kd_block(DipArgs A)
 {
     const int lane = threadIdx.x & (kDWave - 1);
-    const int i = (int)(((size_t)blockIdx.x * kDThreads + threadIdx.x) / kDWave);
+    const int i = (int)blockIdx.x * (kDThreads / kDWave) + __builtin_amdgcn_readfirstlane((int)threadIdx.x / kDWave);
     if (i >= A.n1) return;   // whole wave exits together
     const int ks = A.ks;
     const int64_t* ci = A.cfg1 + 5 * (size_t)i;
-    const int n_a = (int)ci[0], n_bb = (int)ci[1];
-    const int ch_i = A.cfg_chan1[i];
+    const int n_a = __builtin_amdgcn_readfirstlane((int)ci[0]), n_bb = __builtin_amdgcn_readfirstlane((int)ci[1]);
+    const int ch_i = __builtin_amdgcn_readfirstlane(A.cfg_chan1[i]);
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int64_t n = 0;
     const int64_t base = FILL ? A.ptr[i] : 0;
@@ -216,6 +218,27 @@ hipError_t launch_dip_count(const DipArgs& a, hipStream_t s)
     const int64_t threads = (int64_t)a.n1 * kDWave;
     hipLaunchKernelGGL(kd_block<false>, dim3((unsigned)((threads + kDThreads - 1) / kDThreads)), dim3(kDThreads), 0, s,
                        a);
+    return hipGetLastError();
+}
+
+// dst = the transpose of the complex n_b x n_b matrix src (column-major to
+// row-major), one thread per element
+__global__ __launch_bounds__(256) void kd_transpose(const double* __restrict__ src, double* __restrict__ dst, int n_b)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)n_b * n_b) return;
+    const int64_t r = e / n_b, c = e - r * n_b;   // destination (r, c)
+    dst[2 * e] = src[2 * (c * n_b + r)];
+    dst[2 * e + 1] = src[2 * (c * n_b + r) + 1];
+}
+
+hipError_t launch_dip_prepare(const double* S, const double* R1, const double* R2, int n_b, double* ST, double* R1T,
+                              double* R2T, hipStream_t s)
+{
+    const unsigned g = (unsigned)(((int64_t)n_b * n_b + 255) / 256);
+    hipLaunchKernelGGL(kd_transpose, dim3(g), dim3(256), 0, s, S, ST, n_b);
+    hipLaunchKernelGGL(kd_transpose, dim3(g), dim3(256), 0, s, R1, R1T, n_b);
+    if (R2) hipLaunchKernelGGL(kd_transpose, dim3(g), dim3(256), 0, s, R2, R2T, n_b);
     return hipGetLastError();
 }
 
