@@ -34,8 +34,8 @@ HBM_PEAK_GBS = 8000.0
 # each scaled by what the counter reads for a 1 GiB device copy
 # (profiles/rk_build_c2.md), per (workload, fill shape); None for pairs not
 # measured that way.
-FAR_TRAFFIC_BYTES = {("c2", 0): 739.33e6, ("c2", 1): 739.94e6, ("c2", 2): 740.30e6, ("c2", 3): 739.94e6,
-                     ("c2", 4): 739.83e6, ("c2", 5): 739.68e6, ("c2", 6): 739.05e6}
+FAR_TRAFFIC_BYTES = {("c2", 0): 739.28e6, ("c2", 1): 739.59e6, ("c2", 2): 739.99e6, ("c2", 3): 739.74e6,
+                     ("c2", 4): 739.52e6, ("c2", 5): 739.51e6, ("c2", 6): 739.05e6}
 
 
 def cpu_baseline(w, t, R_gpu, nthreads, block_max_l=None):
