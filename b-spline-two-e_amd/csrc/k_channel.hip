@@ -122,13 +122,28 @@ __device__ __forceinline__ double r12_row(const BlockArgs& A, int a1, int c1, in
     return kdot3(x, np, y, np, coef, K);
 }
 
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
 // The same sum for a row whose windows never meet (|n_a - n_b| >= k): every
 // product is rank-one with a known side; x and y are the p and q factors in
-// the wavefront's LDS copy (stride L per multipole).
-__device__ __forceinline__ double r12_staged(const double* __restrict__ x, const double* __restrict__ y, int L, int K,
-                                             const double* __restrict__ coef)
+// the wavefront's LDS copy, K consecutive multipoles from a 16-byte boundary,
+// as is coef, so that one LDS read brings two multipoles.
+__device__ __forceinline__ double r12_staged(const double* __restrict__ x, const double* __restrict__ y,
+                                             const double* __restrict__ coef, int K)
 {
-    return kdot3(x, (unsigned)L, y, (unsigned)L, coef, K);
+    double t = 0.0;
+    int k = 0;
+    for (; k + 4 <= K; k += 4) {
+        const dbl2 xa = *(const dbl2*)(x + k), xb = *(const dbl2*)(x + k + 2);
+        const dbl2 ya = *(const dbl2*)(y + k), yb = *(const dbl2*)(y + k + 2);
+        const dbl2 ca = *(const dbl2*)(coef + k), cb = *(const dbl2*)(coef + k + 2);
+        t = t + (xa.x * ya.x) * ca.x;
+        t = t + (xa.y * ya.y) * ca.y;
+        t = t + (xb.x * yb.x) * cb.x;
+        t = t + (xb.y * yb.y) * cb.y;
+    }
+    for (; k < K; ++k) t = t + (x[k] * y[k]) * coef[k];
+    return t;
 }
 
 // element (r, c), 1-based, of a complex row-major n_b x n_b matrix
@@ -145,11 +160,13 @@ __device__ __forceinline__ int64_t uniform64(int64_t x)
     return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
 }
 
-// LDS of one fill wavefront (doubles): the channel pair's 2K coefficients,
-// then the q factors of a row whose windows never meet, [2][K][L] with
-// L = 2k - 1: direct term pairs (d, n_b), exchange term pairs (d, n_a), for
-// the d within k of n_b / n_a.
-__host__ __device__ constexpr int fill_wave_lds(int ks, int K) { return 2 * K + 2 * K * (2 * ks - 1); }
+// LDS of one fill wavefront (doubles), in rows of KP = K rounded up to even:
+// the channel pair's direct and exchange coefficients (2 rows), then the q
+// factors of a row whose windows never meet, [2][L][KP] with L = 2k - 1:
+// direct term pairs (d, n_b), exchange term pairs (d, n_a), for the d within
+// k of n_b / n_a.
+__host__ __device__ constexpr int fill_kp(int K) { return (K + 1) & ~1; }
+__host__ __device__ constexpr int fill_wave_lds(int ks, int K) { return 2 * fill_kp(K) * (1 + (2 * ks - 1)); }
 
 // One wavefront per configuration row of any of n_blocks symmetry blocks:
 // global row g belongs to the last block b with row0[b] <= g and is row
@@ -160,13 +177,13 @@ template <bool FILL>
 __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restrict__ tab, const int64_t* __restrict__ row0,
                                                       int n_blocks, int64_t n_rows, int lds_ks, int lds_K)
 {
-    extern __shared__ double lds_fill[];
+    extern __shared__ __align__(16) double lds_fill[];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
     const int64_t g = (int64_t)blockIdx.x * (kCThreads / kWave) + wave;
     if (g >= n_rows) return;   // whole wave exits together
-    double* sc = lds_fill + (size_t)wave * fill_wave_lds(lds_ks, lds_K);   // coefficients
-    double* sy = sc + 2 * lds_K;                                           // staged q factors
+    double* sc = lds_fill + (size_t)wave * fill_wave_lds(lds_ks, lds_K);   // coefficients: direct, exchange
+    double* sy = sc + 2 * fill_kp(lds_K);                                  // staged q factors
     int blo = 0, bhi = n_blocks - 1;
     while (blo < bhi) {
         const int mid = (blo + bhi + 1) >> 1;
@@ -195,7 +212,7 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
     // direct q run at (n_c, n_b).
     const bool staged = FILL && (n_bb - n_a >= ks || n_a - n_bb >= ks);
     const bool a_low = n_bb > n_a;
-    const int L = 2 * ks - 1;
+    const int L = 2 * ks - 1, KP = fill_kp(K);
     const unsigned np = (unsigned)A.n_pairs;   // K * n_pairs < 2^31 (checked on the host)
     const int dlo_dir = n_bb - ks + 1 < 1 ? 1 : n_bb - ks + 1, dlo_ex = n_a - ks + 1 < 1 ? 1 : n_a - ks + 1;
     if (staged) {
@@ -205,10 +222,10 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
         const int q_ex = A.c_first[n_a], len_ex = A.c_first[n_a + 1] - q_ex;
         const double* src_dir = a_low ? A.QtotT : A.PtotT;
         const double* src_ex = a_low ? A.PtotT : A.QtotT;
-        for (int e = lane; e < 2 * K * L; e += kWave) {
+        for (int e = lane; e < 2 * K * L; e += kWave) {   // read along the runs, store multipole-fastest
             const int t = e >= K * L, r = e - t * K * L, k = r / L, dl = r - k * L;
             if (dl < (t ? len_ex : len_dir))
-                sy[e] = (t ? src_ex : src_dir)[(unsigned)k * np + (unsigned)((t ? q_ex : q_dir) + dl)];
+                sy[(t * L + dl) * KP + k] = (t ? src_ex : src_dir)[(unsigned)k * np + (unsigned)((t ? q_ex : q_dir) + dl)];
         }
     }
 
@@ -224,13 +241,13 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
             // the wavefront's lanes read the previous channel's copy in program
             // order before any lane overwrites it (one wavefront, in-order LDS)
             const double* cg = A.coef + ((size_t)ch_i * A.n_chan + ch) * 2 * K;
-            for (int e = lane; e < 2 * K; e += kWave) sc[e] = cg[e];
+            for (int e = lane; e < 2 * K; e += kWave) sc[e < K ? e : e - K + KP] = cg[e];
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         const double* cd = sc;
-        const double* ce = sc + K;
+        const double* ce = sc + KP;
         const BlockSeg* sg = A.seg + (size_t)ch * (A.n_b + 1);
         walk_channel(n_a, n_bb, ks, A.n_b, sg, lane, [&](bool active, int n_c, int n_d, int col, bool in_dir_c,
                                                           bool in_ex_c) {
@@ -251,14 +268,14 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
                         // R(a,b,c,d) = R(a,d,c,b): the key [n_a, n_d, n_c, n_b] puts the
                         // wavefront's consecutive n_d on consecutive columns of one row
                         if (support)
-                            res = res + (staged ? r12_staged(sy + K * L + (n_c - dlo_ex), sy + (n_d - dlo_dir), L,
-                                                                     K, cd)
+                            res = res + (staged ? r12_staged(sy + (L + n_c - dlo_ex) * KP, sy + (n_d - dlo_dir) * KP,
+                                                                     cd, K)
                                                         : r12_row(A, n_a, n_c, n_d, n_bb, cd));
                         // exchange R(a,b,d,c) = R(b,a,c,d) = R(b,d,c,a): row pair(n_b, n_c),
                         // consecutive n_d on consecutive columns
                         if (support_ex)
-                            res = res + (staged ? r12_staged(sy + (n_c - dlo_dir), sy + K * L + (n_d - dlo_ex), L,
-                                                                     K, ce)
+                            res = res + (staged ? r12_staged(sy + (n_c - dlo_dir) * KP, sy + (L + n_d - dlo_ex) * KP,
+                                                                     ce, K)
                                                         : r12_row(A, n_bb, n_c, n_d, n_a, ce));
                         h.re = res;
                     }
