@@ -149,10 +149,7 @@ __global__ __launch_bounds__(kThreads) void k3_far(int n_pairs, int K, int far_c
     __shared__ int sLo[ROWS], sHi[ROWS];
     const int64_t rowlen = (int64_t)n_pairs * K;
     const int tid = threadIdx.x;
-    // each XCD takes one contiguous run of row tiles: its stores stream through
-    // one region of the tensor instead of interleaving with the other XCDs'
-    const int bid = (int)xcd_block(blockIdx.x, gridDim.x);
-    const int ry = bid / far_cols, cx = bid - ry * far_cols;
+    const int ry = (int)blockIdx.x / far_cols, cx = (int)blockIdx.x - ry * far_cols;
     const int p0 = ry * ROWS;
     const int nrows = (n_pairs - p0) < ROWS ? (n_pairs - p0) : ROWS;
     for (int e = tid; e < nrows * K; e += kThreads) {

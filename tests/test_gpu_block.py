@@ -92,6 +92,30 @@ def test_block_on_device_tensor(c1, oracle, L):
     assert err <= 1e-14 * np.abs(ref["H"][2]).max(), err
 
 
+@pytest.mark.parametrize("L", [0, 1])
+def test_block_order_16_many_multipoles(ctx, oracle, L):
+    """Order 16 with 34 multipoles (an even count): the fill's per-wavefront
+    LDS (coefficients and the factor runs of 31 pairs) passes 64 KiB per
+    workgroup, and rows with |n1 - n2| >= 16 take the staged path."""
+    k, k_gl, max_k, max_l = 16, 20, 33, 1
+    t = bs2e.generate_grid(k, 3, 2, 0.5, 4.0)
+    n_b = oracle.dims(k, t)[1]
+    print(f"block order 16 K=34 L={L} n_b={n_b}", flush=True)
+    rk = bs2e.RkTensor.build(k, t, k_gl, max_k, ctx=ctx)
+    Rref = oracle.rk_dense(k, n_b, max_k, oracle.slater_off_diag(k, t, k_gl, max_k),
+                           oracle.slater_diag(k, t, k_gl, max_k))
+    S = oracle.setup_S(k, t, k_gl)
+    Hl = [oracle.setup_H1p(k, t, k_gl, 2, l, cap_eta=0.0) for l in range(max_l + 1)]
+    term = bs2e.sym(l=L, pi=bool(L % 2))
+    term.configs = bs2e.count_configs(term, max_l, n_b, k, n_b, n_b, max_l)
+    assert (np.abs(term.configs[:, 0] - term.configs[:, 1]) >= k).any()
+    blk = bs2e.Block(rk, term, True)
+    H_sp, S_sp = blk.fill(Hl, S)
+    blk.release()
+    rk.release()
+    compare(H_sp, S_sp, oracle.construct_block(k, n_b, L, max_k, True, term.configs, Rref, Hl, S))
+
+
 def test_block_truncated_configs_and_cap(c1, oracle):
     """Truncated configuration lists (max_n_b, n_all_l, l_2_max) and a complex
     absorbing potential in the one-electron blocks."""
