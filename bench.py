@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # each scaled by what the counter reads for a 1 GiB device copy
 # (profiles/rk_build_c2.md), per (workload, fill shape); None for pairs not
 # measured that way.
-FAR_TRAFFIC_BYTES = {("c2", 0): 739.28e6, ("c2", 1): 739.59e6, ("c2", 2): 739.99e6, ("c2", 3): 739.74e6,
+FAR_TRAFFIC_BYTES = {("c2", 0): 739.35e6, ("c2", 1): 739.59e6, ("c2", 2): 739.99e6, ("c2", 3): 739.74e6,
                      ("c2", 4): 739.52e6, ("c2", 5): 739.51e6, ("c2", 6): 739.05e6}
 
 
