@@ -137,6 +137,7 @@ struct BlockArgs {
     int64_t* refS;
     const int64_t* ptrH;         // 0-based row starts (fill pass)
     const int64_t* ptrS;
+    int64_t limH, limS;          // entries the output arrays hold (fill pass)
     int64_t* ixH;
     double* dH;                  // complex
     int64_t* ixS;

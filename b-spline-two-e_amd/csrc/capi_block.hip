@@ -202,6 +202,8 @@ static BlockArgs set_args(const bs2e_blockset* b, int64_t s, bool fill)
         a.QtotT = a.PtotT + nf;
         a.ptrH = b->d_ptr + r0;
         a.ptrS = b->d_ptr + N + r0;
+        a.limH = b->totH;
+        a.limS = b->totS;
         a.ixH = b->d_ixH;
         a.dH = b->d_dH;
         a.ixS = b->d_ixS;

@@ -18,13 +18,14 @@ namespace bs2e {
 
 __device__ __forceinline__ int iabs_d(int x) { return x < 0 ? -x : x; }
 
-// sg: the channel's n2 ranges per n1 (index 1..n_b); k <= 16.
-template <class Visit>
-__device__ __forceinline__ void walk_channel(int n_a, int n_bb, int ks, int n_b, const BlockSeg* __restrict__ sg,
-                                             int lane, Visit&& visit)
+// The n_c candidates of row (n_a, n_b): within k of n_a (direct) or of n_b
+// (exchange), ascending, as one or two runs [r1lo, r1lo + n1) and
+// [r2lo, r2lo + n2) inside [1, n_b]; n1 + n2 <= 2 (2k - 1) <= 62 for k <= 16.
+struct CandRanges {
+    int r1lo, n1, r2lo, n2;
+};
+__device__ __forceinline__ CandRanges candidate_ranges(int n_a, int n_bb, int ks, int n_b)
 {
-    constexpr int kWave = 64;
-    // n_c candidates: within k of n_a (direct) or of n_b (exchange), ascending
     int r1lo = n_a - ks + 1, r1hi = n_a + ks - 1, r2lo = n_bb - ks + 1, r2hi = n_bb + ks - 1;
     if (r2lo < r1lo) {
         int t0 = r1lo, t1 = r1hi;
@@ -42,8 +43,18 @@ __device__ __forceinline__ void walk_channel(int n_a, int n_bb, int ks, int n_b,
     r1hi = r1hi > n_b ? n_b : r1hi;
     r2lo = r2lo < 1 ? 1 : r2lo;
     r2hi = r2hi > n_b ? n_b : r2hi;
-    const int n1 = r1hi >= r1lo ? r1hi - r1lo + 1 : 0, n2 = r2hi >= r2lo ? r2hi - r2lo + 1 : 0;
-    const int ncand = n1 + n2;    // <= 2 (2k - 1) <= 62 for k <= 16: one per lane
+    return CandRanges{r1lo, r1hi >= r1lo ? r1hi - r1lo + 1 : 0, r2lo, r2hi >= r2lo ? r2hi - r2lo + 1 : 0};
+}
+
+// sg: the channel's n2 ranges per n1 (index 1..n_b); k <= 16.
+template <class Visit>
+__device__ __forceinline__ void walk_channel(int n_a, int n_bb, int ks, int n_b, const BlockSeg* __restrict__ sg,
+                                             int lane, Visit&& visit)
+{
+    constexpr int kWave = 64;
+    const CandRanges cr = candidate_ranges(n_a, n_bb, ks, n_b);
+    const int r1lo = cr.r1lo, r2lo = cr.r2lo, n1 = cr.n1;
+    const int ncand = cr.n1 + cr.n2;   // one per lane
     // lane l takes the l-th candidate n_c and its n_d runs: direct needs
     // |n_b - n_d| < k, exchange |n_a - n_d| < k, both clipped to the
     // channel's n2 range for that n1 = n_c

@@ -172,8 +172,10 @@ __host__ __device__ constexpr int fill_wave_lds(int ks, int K) { return 2 * fill
 // global row g belongs to the last block b with row0[b] <= g and is row
 // g - row0[b] of it; each block's tables, inputs and outputs are in tab[b].
 // The fill's LDS (fill_wave_lds per wavefront) is dynamic; ks and K are the
-// blocks' common spline order and multipole count.
-template <bool FILL>
+// blocks' common spline order and multipole count.  Entries go to the row's
+// CSR range from the count pass; stores beyond the set's outputs (limH,
+// limS entries) are dropped, so a count that disagreed with the walk could
+// not write outside them.
 __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restrict__ tab, const int64_t* __restrict__ row0,
                                                       int n_blocks, int64_t n_rows, int lds_ks, int lds_K)
 {
@@ -199,8 +201,8 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
     const int l_a = __builtin_amdgcn_readfirstlane((int)ci[2]), l_b = __builtin_amdgcn_readfirstlane((int)ci[3]);
     const int ch_i = __builtin_amdgcn_readfirstlane(A.cfg_chan[i]);
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    int64_t nH = 0, nS = 0, rH = 0, rS = 0;
-    const int64_t baseH = FILL ? uniform64(A.ptrH[i]) : 0, baseS = FILL ? uniform64(A.ptrS[i]) : 0;
+    int64_t nH = 0, nS = 0;
+    const int64_t baseH = uniform64(A.ptrH[i]), baseS = uniform64(A.ptrS[i]);
     // |n_a - n_b| >= k: the direct pairs (n_a, n_c), (n_d, n_b) and the
     // exchange pairs (n_b, n_c), (n_d, n_a) have disjoint windows, on a side
     // fixed by the row: n_a < n_b puts every direct p left of its q
@@ -210,7 +212,7 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
     // local pair per interval), so the direct p = (n_a, n_c) reads the
     // exchange q run at (n_c, n_a), and the exchange p = (n_b, n_c) the
     // direct q run at (n_c, n_b).
-    const bool staged = FILL && (n_bb - n_a >= ks || n_a - n_bb >= ks);
+    const bool staged = n_bb - n_a >= ks || n_a - n_bb >= ks;
     const bool a_low = n_bb > n_a;
     const int L = 2 * ks - 1, KP = fill_kp(K);
     const unsigned np = (unsigned)A.n_pairs;   // K * n_pairs < 2^31 (checked on the host)
@@ -237,7 +239,7 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
         const bool l_eq = (l_a == l_c) && (l_b == l_d);
         const bool l_eq_ex = (l_a == l_d) && (l_b == l_c);
         if (!dir_any && !ex_any && !l_eq && !l_eq_ex) continue;   // no entry can exist in this channel
-        if (FILL) {
+        {
             // the wavefront's lanes read the previous channel's copy in program
             // order before any lane overwrites it (one wavefront, in-order LDS)
             const double* cg = A.coef + ((size_t)ch_i * A.n_chan + ch) * 2 * K;
@@ -260,7 +262,7 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
             const bool one_el = active && ((support && l_eq) || (support_ex && l_eq_ex));
             const unsigned long long mH = __ballot(entry);
             const unsigned long long mS = __ballot(one_el);
-            if (FILL) {
+            {
                 if (entry) {
                     Cplx h{0.0, 0.0};
                     if (r12) {
@@ -299,31 +301,104 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
                         }
                         h = cadd(h, h1);
                         const int64_t ps = baseS + nS + __popcll(mS & lt_mask);
-                        A.ixS[ps] = j + 1;
-                        A.dS[2 * ps] = s1.re;
-                        A.dS[2 * ps + 1] = s1.im;
+                        if (ps < A.limS) {
+                            A.ixS[ps] = j + 1;
+                            A.dS[2 * ps] = s1.re;
+                            A.dS[2 * ps + 1] = s1.im;
+                        }
                     }
                     const int64_t ph = baseH + nH + __popcll(mH & lt_mask);
-                    A.ixH[ph] = j + 1;
-                    A.dH[2 * ph] = h.re;
-                    A.dH[2 * ph + 1] = h.im;
+                    if (ph < A.limH) {
+                        A.ixH[ph] = j + 1;
+                        A.dH[2 * ph] = h.re;
+                        A.dH[2 * ph + 1] = h.im;
+                    }
                 }
-            } else {
-                // the reference's count_nnz predicate (:386-413), for diagnostics
-                const bool rh = active && (support ? dir_any : (support_ex && ex_any));
-                const bool rs = active && (support ? l_eq : (support_ex && l_eq_ex));
-                rH += __popcll(__ballot(rh));
-                rS += __popcll(__ballot(rs));
             }
             nH += __popcll(mH);
             nS += __popcll(mS);
         });
     }
-    if (!FILL && lane == 0) {
-        A.cntH[i] = nH;
-        A.cntS[i] = nS;
-        A.refH[i] = rH;
-        A.refS[i] = rS;
+}
+
+// Row counts of the count pass, in closed form.  Within one column channel,
+// candidate n_c (one per lane, within k of n_a or n_b, with configurations in
+// the channel) owns two runs of n_d inside its n2 range: D, the n_d within k
+// of n_b when |n_c - n_a| < k (direct support), and E, the n_d within k of
+// n_a when |n_c - n_b| < k (exchange support); with full false both start at
+// the first column j >= i.  The walk's predicates are unions of the runs:
+//   H entry    D u E if l_eq or l_eq_ex, else (D if dir_any) u (E if ex_any)
+//   S entry    (D if l_eq) u (E if l_eq_ex)
+// and the reference's count_nnz (hamiltonian.f90:386-413), which tests the
+// exchange term only where the direct support fails:
+//   H          (D if dir_any) + (E \ D if ex_any)
+//   S          (D if l_eq) + (E \ D if l_eq_ex)
+__device__ __forceinline__ int run_len(int lo, int hi) { return hi >= lo ? hi - lo + 1 : 0; }
+
+__global__ __launch_bounds__(kCThreads) void kc_count(const BlockArgs* __restrict__ tab, const int64_t* __restrict__ row0,
+                                                      int n_blocks, int64_t n_rows)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
+    const int64_t g = (int64_t)blockIdx.x * (kCThreads / kWave) + wave;
+    if (g >= n_rows) return;   // whole wave exits together
+    int blo = 0, bhi = n_blocks - 1;
+    while (blo < bhi) {
+        const int mid = (blo + bhi + 1) >> 1;
+        if (row0[mid] <= g) blo = mid;
+        else bhi = mid - 1;
+    }
+    const int b = __builtin_amdgcn_readfirstlane(blo);
+    const BlockArgs A = tab[b];
+    const int i = (int)(g - row0[b]);
+    const int ks = A.ks, n_b = A.n_b;
+    const int64_t* ci = A.cfg + 5 * (size_t)i;
+    const int n_a = __builtin_amdgcn_readfirstlane((int)ci[0]), n_bb = __builtin_amdgcn_readfirstlane((int)ci[1]);
+    const int l_a = __builtin_amdgcn_readfirstlane((int)ci[2]), l_b = __builtin_amdgcn_readfirstlane((int)ci[3]);
+    const int ch_i = __builtin_amdgcn_readfirstlane(A.cfg_chan[i]);
+    const CandRanges cr = candidate_ranges(n_a, n_bb, ks, n_b);
+    const bool cand = lane < cr.n1 + cr.n2;
+    const int n_c = lane < cr.n1 ? cr.r1lo + lane : cr.r2lo + (lane - cr.n1);
+    const bool in_dir_c = cand && iabs_d(n_c - n_a) < ks, in_ex_c = cand && iabs_d(n_c - n_bb) < ks;
+    int h = 0, s = 0, rh = 0, rs = 0;
+    for (int ch = 0; ch < A.n_chan; ++ch) {
+        const BlockChan cj = A.chan[ch];
+        const int fl = A.flags[ch_i * A.n_chan + ch];
+        const bool dir_any = fl & 1, ex_any = (fl >> 1) & 1;
+        const bool l_eq = (l_a == cj.l1) && (l_b == cj.l2);
+        const bool l_eq_ex = (l_a == cj.l2) && (l_b == cj.l1);
+        if (!dir_any && !ex_any && !l_eq && !l_eq_ex) continue;   // no entry can exist in this channel
+        if (!cand) continue;
+        const BlockSeg sg = A.seg[(size_t)ch * (n_b + 1) + n_c];
+        if (sg.hi < sg.lo) continue;
+        int lo = sg.lo;
+        if (!A.full) {   // column j = start + off + (n_d - lo2) >= i
+            const int t = i - cj.start - sg.off + sg.lo;
+            lo = t > lo ? t : lo;
+        }
+        const int dlo = in_dir_c ? (lo > n_bb - ks + 1 ? lo : n_bb - ks + 1) : 1;
+        const int dhi = in_dir_c ? (sg.hi < n_bb + ks - 1 ? sg.hi : n_bb + ks - 1) : 0;
+        const int elo = in_ex_c ? (lo > n_a - ks + 1 ? lo : n_a - ks + 1) : 1;
+        const int ehi = in_ex_c ? (sg.hi < n_a + ks - 1 ? sg.hi : n_a + ks - 1) : 0;
+        const int D = run_len(dlo, dhi), E = run_len(elo, ehi);
+        const int DE = run_len(dlo > elo ? dlo : elo, dhi < ehi ? dhi : ehi);
+        const bool hd = dir_any || l_eq || l_eq_ex, he = ex_any || l_eq || l_eq_ex;
+        h += (hd ? D : 0) + (he ? E : 0) - (hd && he ? DE : 0);
+        s += (l_eq ? D : 0) + (l_eq_ex ? E : 0) - (l_eq && l_eq_ex ? DE : 0);
+        rh += (dir_any ? D : 0) + (ex_any ? E - DE : 0);
+        rs += (l_eq ? D : 0) + (l_eq_ex ? E - DE : 0);
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        h += __shfl_xor(h, off, kWave);
+        s += __shfl_xor(s, off, kWave);
+        rh += __shfl_xor(rh, off, kWave);
+        rs += __shfl_xor(rs, off, kWave);
+    }
+    if (lane == 0) {
+        A.cntH[i] = h;
+        A.cntS[i] = s;
+        A.refH[i] = rh;
+        A.refS[i] = rs;
     }
 }
 
@@ -364,8 +439,8 @@ hipError_t launch_block_prepare(const double* Hl, const double* S, int n_l, int 
 hipError_t launch_block_count(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, hipStream_t s)
 {
     const int64_t threads = n_rows * kWave;
-    hipLaunchKernelGGL(kc_block<false>, dim3((unsigned)((threads + kCThreads - 1) / kCThreads)), dim3(kCThreads), 0, s,
-                       tab, row0, n_blocks, n_rows, 0, 0);
+    hipLaunchKernelGGL(kc_count, dim3((unsigned)((threads + kCThreads - 1) / kCThreads)), dim3(kCThreads), 0, s,
+                       tab, row0, n_blocks, n_rows);
     return hipGetLastError();
 }
 
@@ -374,9 +449,9 @@ hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_bl
 {
     const int64_t threads = n_rows * kWave;
     const size_t lds = sizeof(double) * (size_t)(kCThreads / kWave) * (size_t)fill_wave_lds(ks, K);
-    hipError_t e = allow_lds((const void*)kc_block<true>, lds);   // <= 4 x 32 KiB (k <= 16, K <= 64)
+    hipError_t e = allow_lds((const void*)kc_block, lds);   // <= 4 x 32 KiB (k <= 16, K <= 64)
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kc_block<true>, dim3((unsigned)((threads + kCThreads - 1) / kCThreads)), dim3(kCThreads), lds,
+    hipLaunchKernelGGL(kc_block, dim3((unsigned)((threads + kCThreads - 1) / kCThreads)), dim3(kCThreads), lds,
                        s, tab, row0, n_blocks, n_rows, ks, K);
     return hipGetLastError();
 }
