@@ -179,6 +179,7 @@ struct DipArgs {
     const double *S, *rad1, *rad2;   // complex n_b x n_b row-major (rad: r_mat, or dr_mat and r_inv_mat)
     int64_t* cnt;               // count pass
     const int64_t* ptr;         // fill pass: 0-based row starts
+    int64_t lim;                // fill pass: entries the outputs hold
     int64_t* ix;
     double* data;               // complex
 };

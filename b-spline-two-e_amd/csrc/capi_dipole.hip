@@ -318,6 +318,7 @@ int64_t bs2e_dip_block_fill(bs2e_dip* b, int64_t gauge, const double* S, const d
         a.rad1 = dT + 2 * nn;
         a.rad2 = gauge == 1 ? dT + 4 * nn : a.rad1;
         a.ptr = b->d_ptr;
+        a.lim = b->nnz;
         a.ix = dix;
         a.data = dd;
         hipEvent_t e0 = nullptr, e1 = nullptr;

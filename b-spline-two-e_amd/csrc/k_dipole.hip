@@ -164,7 +164,8 @@ __device__ __forceinline__ DCplx red1p(const DipArgs& A, const DipCoef& c, int t
     return DCplx{-v.re, -v.im};
 }
 
-template <bool FILL>
+// The fill: entries go to the row's range from the count pass; stores beyond
+// the outputs (lim entries) are dropped.
 __global__ __launch_bounds__(kDThreads) void kd_block(DipArgs A)
 {
     const int lane = threadIdx.x & (kDWave - 1);
@@ -176,7 +177,7 @@ __global__ __launch_bounds__(kDThreads) void kd_block(DipArgs A)
     const int ch_i = __builtin_amdgcn_readfirstlane(A.cfg_chan1[i]);
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int64_t n = 0;
-    const int64_t base = FILL ? A.ptr[i] : 0;
+    const int64_t base = A.ptr[i];
     for (int ch = 0; ch < A.n_chan2; ++ch) {
         const DipCoef& c = A.coef[(size_t)ch_i * A.n_chan2 + ch];
         if (!c.nz) continue;
@@ -188,7 +189,7 @@ __global__ __launch_bounds__(kDThreads) void kd_block(DipArgs A)
             const bool support_ex = in_ex_c && iabs_d(n_a - n_d) < ks;
             const bool entry = active && (support || support_ex);
             const unsigned long long m = __ballot(entry);
-            if (FILL && entry) {
+            if (entry) {
                 const DCplx zero{0.0, 0.0};
                 // direct: electron 1 n_a -> n_c, electron 2 n_b -> n_d
                 const DCplx r1 = c.cond[0] ? dmul(dscale(c.pf[0], red1p(A, c, 0, n_a, n_c)), dmat(A.S, A.n_b, n_bb, n_d))
@@ -203,20 +204,55 @@ __global__ __launch_bounds__(kDThreads) void kd_block(DipArgs A)
                 const DCplx red = dadd(r1, r2), redx = dscale(c.sgn_ex, dadd(x1, x2));
                 const DCplx v = dscale(A.ang, dadd(red, redx));
                 const int64_t p = base + n + __popcll(m & lt_mask);
-                A.ix[p] = cj.start + col + 1;
-                A.data[2 * p] = v.re;
-                A.data[2 * p + 1] = v.im;
+                if (p < A.lim) {
+                    A.ix[p] = cj.start + col + 1;
+                    A.data[2 * p] = v.re;
+                    A.data[2 * p + 1] = v.im;
+                }
             }
             n += __popcll(m);
         });
     }
-    if (!FILL && lane == 0) A.cnt[i] = n;
+}
+
+// Row counts of the count pass, in closed form: within one column channel
+// whose channel pair carries entries, candidate n_c (one per lane) owns the
+// direct run D of n_d (within k of n_b, when |n_c - n_a| < k) and the
+// exchange run E (within k of n_a, when |n_c - n_b| < k) inside its n2 range;
+// the entries are D u E.
+__global__ __launch_bounds__(kDThreads) void kd_count(DipArgs A)
+{
+    const int lane = threadIdx.x & (kDWave - 1);
+    const int i = (int)blockIdx.x * (kDThreads / kDWave) + __builtin_amdgcn_readfirstlane((int)threadIdx.x / kDWave);
+    if (i >= A.n1) return;   // whole wave exits together
+    const int ks = A.ks, n_b = A.n_b;
+    const int64_t* ci = A.cfg1 + 5 * (size_t)i;
+    const int n_a = __builtin_amdgcn_readfirstlane((int)ci[0]), n_bb = __builtin_amdgcn_readfirstlane((int)ci[1]);
+    const int ch_i = __builtin_amdgcn_readfirstlane(A.cfg_chan1[i]);
+    const CandRanges cr = candidate_ranges(n_a, n_bb, ks, n_b);
+    const bool cand = lane < cr.n1 + cr.n2;
+    const int n_c = lane < cr.n1 ? cr.r1lo + lane : cr.r2lo + (lane - cr.n1);
+    const bool in_dir_c = cand && iabs_d(n_c - n_a) < ks, in_ex_c = cand && iabs_d(n_c - n_bb) < ks;
+    int n = 0;
+    for (int ch = 0; ch < A.n_chan2; ++ch) {
+        if (!A.coef[(size_t)ch_i * A.n_chan2 + ch].nz || !cand) continue;
+        const BlockSeg sg = A.seg2[(size_t)ch * (n_b + 1) + n_c];
+        if (sg.hi < sg.lo) continue;
+        const int dlo = in_dir_c ? (sg.lo > n_bb - ks + 1 ? sg.lo : n_bb - ks + 1) : 1;
+        const int dhi = in_dir_c ? (sg.hi < n_bb + ks - 1 ? sg.hi : n_bb + ks - 1) : 0;
+        const int elo = in_ex_c ? (sg.lo > n_a - ks + 1 ? sg.lo : n_a - ks + 1) : 1;
+        const int ehi = in_ex_c ? (sg.hi < n_a + ks - 1 ? sg.hi : n_a + ks - 1) : 0;
+        const int ov_lo = dlo > elo ? dlo : elo, ov_hi = dhi < ehi ? dhi : ehi;
+        n += (dhi >= dlo ? dhi - dlo + 1 : 0) + (ehi >= elo ? ehi - elo + 1 : 0) - (ov_hi >= ov_lo ? ov_hi - ov_lo + 1 : 0);
+    }
+    for (int off = kDWave / 2; off > 0; off >>= 1) n += __shfl_xor(n, off, kDWave);
+    if (lane == 0) A.cnt[i] = n;
 }
 
 hipError_t launch_dip_count(const DipArgs& a, hipStream_t s)
 {
     const int64_t threads = (int64_t)a.n1 * kDWave;
-    hipLaunchKernelGGL(kd_block<false>, dim3((unsigned)((threads + kDThreads - 1) / kDThreads)), dim3(kDThreads), 0, s,
+    hipLaunchKernelGGL(kd_count, dim3((unsigned)((threads + kDThreads - 1) / kDThreads)), dim3(kDThreads), 0, s,
                        a);
     return hipGetLastError();
 }
@@ -245,7 +281,7 @@ hipError_t launch_dip_prepare(const double* S, const double* R1, const double* R
 hipError_t launch_dip_fill(const DipArgs& a, hipStream_t s)
 {
     const int64_t threads = (int64_t)a.n1 * kDWave;
-    hipLaunchKernelGGL(kd_block<true>, dim3((unsigned)((threads + kDThreads - 1) / kDThreads)), dim3(kDThreads), 0, s,
+    hipLaunchKernelGGL(kd_block, dim3((unsigned)((threads + kDThreads - 1) / kDThreads)), dim3(kDThreads), 0, s,
                        a);
     return hipGetLastError();
 }
