@@ -111,6 +111,20 @@ struct BlockSeg {             // configurations of one channel with a given n1
 // workgroup index that gives XCD (b mod 8) one contiguous range of the n
 // logical workgroups, so neighbours that gather the same tables share an L2.
 constexpr int kXcds = 8;
+
+// Loads and stores through the global address space: the tables a kernel
+// reaches through a descriptor are global memory, and saying so lets the
+// compiler use global instead of flat instructions (which also wait on LDS).
+template <class T>
+__device__ __forceinline__ T gload(const T* p)
+{
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ void gstore(T* p, T v)
+{
+    *(__attribute__((address_space(1))) T*)p = v;
+}
 __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t n)
 {
     const int64_t x = b % kXcds, j = b / kXcds;

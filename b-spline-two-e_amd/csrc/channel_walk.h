@@ -61,7 +61,9 @@ __device__ __forceinline__ void walk_channel(int n_a, int n_bb, int ks, int n_b,
     int my_c = 0, d1lo = 1, d1hi = 0, d2lo = 1, d2hi = 0, my_soff = 0, my_slo = 0;
     if (lane < ncand) {
         my_c = lane < n1 ? r1lo + lane : r2lo + (lane - n1);
-        const BlockSeg sgc = sg[my_c];
+        typedef int int4v __attribute__((ext_vector_type(4)));
+        const int4v q = gload((const int4v*)(sg + my_c));   // BlockSeg {off, lo, hi, pad}
+        const BlockSeg sgc{q.x, q.y, q.z, q.w};
         if (sgc.hi >= sgc.lo) {
             const bool in_dir_c = iabs_d(my_c - n_a) < ks;
             const bool in_ex_c = iabs_d(my_c - n_bb) < ks;

@@ -29,6 +29,8 @@
 
 namespace bs2e {
 
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
 constexpr int kCThreads = 256;
 constexpr int kWave = 64;
 
@@ -48,7 +50,7 @@ __device__ __forceinline__ Cplx cscale(double s, Cplx a) { return Cplx{s * a.re,
 __device__ __forceinline__ int pair_of(const int32_t* c_first, int ks, int a, int c)
 {
     const int lo = (c - ks + 1) < 1 ? 1 : (c - ks + 1);
-    return c_first[c] + (a - lo);
+    return gload(c_first + c) + (a - lo);
 }
 
 // the support window [lo, hi] of pair (a, c): the intervals where both
@@ -72,14 +74,14 @@ __device__ __forceinline__ double kdot2(const double* __restrict__ v, const doub
     double t = 0.0;
     int k = 0;
     for (; k + 4 <= K; k += 4) {
-        const double v0 = v[k], v1 = v[k + 1], v2 = v[k + 2], v3 = v[k + 3];
+        const double v0 = gload(v + k), v1 = gload(v + k + 1), v2 = gload(v + k + 2), v3 = gload(v + k + 3);
         const double c0 = coef[k], c1 = coef[k + 1], c2 = coef[k + 2], c3 = coef[k + 3];
         t = t + v0 * c0;
         t = t + v1 * c1;
         t = t + v2 * c2;
         t = t + v3 * c3;
     }
-    for (; k < K; ++k) t = t + v[k] * coef[k];
+    for (; k < K; ++k) t = t + gload(v + k) * coef[k];
     return t;
 }
 __device__ __forceinline__ double kdot3(const double* __restrict__ x, unsigned sx, const double* __restrict__ y,
@@ -89,15 +91,17 @@ __device__ __forceinline__ double kdot3(const double* __restrict__ x, unsigned s
     int k = 0;
     for (; k + 4 <= K; k += 4) {
         const unsigned u = (unsigned)k;
-        const double x0 = x[u * sx], x1 = x[(u + 1) * sx], x2 = x[(u + 2) * sx], x3 = x[(u + 3) * sx];
-        const double y0 = y[u * sy], y1 = y[(u + 1) * sy], y2 = y[(u + 2) * sy], y3 = y[(u + 3) * sy];
+        const double x0 = gload(x + u * sx), x1 = gload(x + (u + 1) * sx), x2 = gload(x + (u + 2) * sx),
+                     x3 = gload(x + (u + 3) * sx);
+        const double y0 = gload(y + u * sy), y1 = gload(y + (u + 1) * sy), y2 = gload(y + (u + 2) * sy),
+                     y3 = gload(y + (u + 3) * sy);
         const double c0 = coef[k], c1 = coef[k + 1], c2 = coef[k + 2], c3 = coef[k + 3];
         t = t + (x0 * y0) * c0;
         t = t + (x1 * y1) * c1;
         t = t + (x2 * y2) * c2;
         t = t + (x3 * y3) * c3;
     }
-    for (; k < K; ++k) t = t + (x[(unsigned)k * sx] * y[(unsigned)k * sy]) * coef[k];
+    for (; k < K; ++k) t = t + (gload(x + (unsigned)k * sx) * gload(y + (unsigned)k * sy)) * coef[k];
     return t;
 }
 
@@ -121,8 +125,6 @@ __device__ __forceinline__ double r12_row(const BlockArgs& A, int a1, int c1, in
     const unsigned np = (unsigned)A.n_pairs;   // K * n_pairs < 2^31 (checked on the host)
     return kdot3(x, np, y, np, coef, K);
 }
-
-typedef double dbl2 __attribute__((ext_vector_type(2)));
 
 // The same sum for a row whose windows never meet (|n_a - n_b| >= k): every
 // product is rank-one with a known side; x and y are the p and q factors in
@@ -150,7 +152,8 @@ __device__ __forceinline__ double r12_staged(const double* __restrict__ x, const
 __device__ __forceinline__ Cplx mat(const double* M, int n_b, int r, int c)
 {
     const size_t o = 2 * ((size_t)(r - 1) * n_b + (c - 1));
-    return Cplx{M[o], M[o + 1]};
+    const dbl2 v = gload((const dbl2*)(M + o));
+    return Cplx{v.x, v.y};
 }
 
 __device__ __forceinline__ int64_t uniform64(int64_t x)
@@ -227,7 +230,7 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
         for (int e = lane; e < 2 * K * L; e += kWave) {   // read along the runs, store multipole-fastest
             const int t = e >= K * L, r = e - t * K * L, k = r / L, dl = r - k * L;
             if (dl < (t ? len_ex : len_dir))
-                sy[(t * L + dl) * KP + k] = (t ? src_ex : src_dir)[(unsigned)k * np + (unsigned)((t ? q_ex : q_dir) + dl)];
+                sy[(t * L + dl) * KP + k] = gload((t ? src_ex : src_dir) + ((unsigned)k * np + (unsigned)((t ? q_ex : q_dir) + dl)));
         }
     }
 
@@ -243,7 +246,7 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
             // the wavefront's lanes read the previous channel's copy in program
             // order before any lane overwrites it (one wavefront, in-order LDS)
             const double* cg = A.coef + ((size_t)ch_i * A.n_chan + ch) * 2 * K;
-            for (int e = lane; e < 2 * K; e += kWave) sc[e < K ? e : e - K + KP] = cg[e];
+            for (int e = lane; e < 2 * K; e += kWave) sc[e < K ? e : e - K + KP] = gload(cg + e);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -302,16 +305,14 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
                         h = cadd(h, h1);
                         const int64_t ps = baseS + nS + __popcll(mS & lt_mask);
                         if (ps < A.limS) {
-                            A.ixS[ps] = j + 1;
-                            A.dS[2 * ps] = s1.re;
-                            A.dS[2 * ps + 1] = s1.im;
+                            gstore(A.ixS + ps, (int64_t)(j + 1));
+                            gstore((dbl2*)(A.dS + 2 * ps), dbl2{s1.re, s1.im});
                         }
                     }
                     const int64_t ph = baseH + nH + __popcll(mH & lt_mask);
                     if (ph < A.limH) {
-                        A.ixH[ph] = j + 1;
-                        A.dH[2 * ph] = h.re;
-                        A.dH[2 * ph + 1] = h.im;
+                        gstore(A.ixH + ph, (int64_t)(j + 1));
+                        gstore((dbl2*)(A.dH + 2 * ph), dbl2{h.re, h.im});
                     }
                 }
             }
