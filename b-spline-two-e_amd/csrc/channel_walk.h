@@ -46,10 +46,21 @@ __device__ __forceinline__ CandRanges candidate_ranges(int n_a, int n_bb, int ks
     return CandRanges{r1lo, r1hi >= r1lo ? r1hi - r1lo + 1 : 0, r2lo, r2hi >= r2lo ? r2hi - r2lo + 1 : 0};
 }
 
-// sg: the channel's n2 ranges per n1 (index 1..n_b); k <= 16.
-template <class Visit>
+// Orders a wavefront's LDS writes before its lanes read what other lanes
+// wrote (one wavefront: no workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// sg: the channel's n2 ranges per n1 (index 1..n_b); k <= 16.  own: 64 bytes
+// of the wavefront's LDS for the owner marks (MARKS), else unused: each lane
+// then finds its candidate by a binary search over the run starts.
+template <bool MARKS, class Visit>
 __device__ __forceinline__ void walk_channel(int n_a, int n_bb, int ks, int n_b, const BlockSeg* __restrict__ sg,
-                                             int lane, Visit&& visit)
+                                             int lane, unsigned char* __restrict__ own, Visit&& visit)
 {
     constexpr int kWave = 64;
     const CandRanges cr = candidate_ranges(n_a, n_bb, ks, n_b);
@@ -112,14 +123,37 @@ __device__ __forceinline__ void walk_channel(int n_a, int n_bb, int ks, int n_b,
     }
     const int total = __shfl(incl, kWave - 1, kWave);
     const int excl = incl - len1 - len2;
-    // candidates flattened in (n_c, n_d) order, which is ascending column order
+    // candidates flattened in (n_c, n_d) order, which is ascending column
+    // order.  Lane f of a window [f0, f0 + 64) belongs to the last candidate
+    // whose (non-empty) run starts at or before f.  With MARKS the runs
+    // starting inside the window mark their first position in own[] with
+    // their lane + 1, a ballot of the marks gives each lane the last start at
+    // or below it, and a lane before every start continues the run of the
+    // window's first position (c0, the owner of the previous window's last
+    // lane).
+    const bool starts = ncand > lane && len1 + len2 > 0;
+    int c0 = 0;
     for (int f0 = 0; f0 < total; f0 += kWave) {
         const int f = f0 + lane;
-        int pos = 0;   // last candidate lane whose run starts at or before f
-        for (int step = kWave / 2; step > 0; step >>= 1) {
-            const int cand = pos + step;
-            const int e = __shfl(excl, cand < kWave ? cand : kWave - 1, kWave);
-            if (cand < ncand && e <= f) pos = cand;
+        int pos = 0;
+        if (MARKS) {
+            own[lane] = 0;
+            wave_lds_sync();
+            if (starts && excl >= f0 && excl < f0 + kWave) own[excl - f0] = (unsigned char)(lane + 1);
+            wave_lds_sync();
+            const int mark = own[lane];
+            const unsigned long long marks = __ballot(mark != 0);
+            const unsigned long long le = marks & (lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1));
+            const int last = 63 - __builtin_clzll(le | 1ull);   // highest mark at or below this lane (0 if none)
+            const int owner = __shfl(mark, last, kWave);
+            pos = le ? owner - 1 : c0;
+            c0 = __shfl(pos, kWave - 1, kWave);
+        } else {
+            for (int step = kWave / 2; step > 0; step >>= 1) {
+                const int cand = pos + step;
+                const int e = __shfl(excl, cand < kWave ? cand : kWave - 1, kWave);
+                if (cand < ncand && e <= f) pos = cand;
+            }
         }
         const int n_c = __shfl(my_c, pos, kWave);
         const int o_lo1 = __shfl(d1lo, pos, kWave), o_len1 = __shfl(len1, pos, kWave);

@@ -169,7 +169,8 @@ __device__ __forceinline__ int64_t uniform64(int64_t x)
 // direct term pairs (d, n_b), exchange term pairs (d, n_a), for the d within
 // k of n_b / n_a.
 __host__ __device__ constexpr int fill_kp(int K) { return (K + 1) & ~1; }
-__host__ __device__ constexpr int fill_wave_lds(int ks, int K) { return 2 * fill_kp(K) * (1 + (2 * ks - 1)); }
+// (plus 64 bytes for the channel walk's marks)
+__host__ __device__ constexpr int fill_wave_lds(int ks, int K) { return 2 * fill_kp(K) * (1 + (2 * ks - 1)) + 8; }
 
 // One wavefront per configuration row of any of n_blocks symmetry blocks:
 // global row g belongs to the last block b with row0[b] <= g and is row
@@ -189,6 +190,7 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
     if (g >= n_rows) return;   // whole wave exits together
     double* sc = lds_fill + (size_t)wave * fill_wave_lds(lds_ks, lds_K);   // coefficients: direct, exchange
     double* sy = sc + 2 * fill_kp(lds_K);                                  // staged q factors
+    unsigned char* own = (unsigned char*)(sc + fill_wave_lds(lds_ks, lds_K) - 8);   // walk marks
     int blo = 0, bhi = n_blocks - 1;
     while (blo < bhi) {
         const int mid = (blo + bhi + 1) >> 1;
@@ -254,7 +256,7 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
         const double* cd = sc;
         const double* ce = sc + KP;
         const BlockSeg* sg = A.seg + (size_t)ch * (A.n_b + 1);
-        walk_channel(n_a, n_bb, ks, A.n_b, sg, lane, [&](bool active, int n_c, int n_d, int col, bool in_dir_c,
+        walk_channel<true>(n_a, n_bb, ks, A.n_b, sg, lane, own, [&](bool active, int n_c, int n_d, int col, bool in_dir_c,
                                                           bool in_ex_c) {
             const int j = cj.start + col;
             if (!A.full && j < i) active = false;

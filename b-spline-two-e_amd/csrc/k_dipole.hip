@@ -183,7 +183,8 @@ __global__ __launch_bounds__(kDThreads) void kd_block(DipArgs A)
         if (!c.nz) continue;
         const BlockChan cj = A.chan2[ch];
         const BlockSeg* sg = A.seg2 + (size_t)ch * (A.n_b + 1);
-        walk_channel(n_a, n_bb, ks, A.n_b, sg, lane, [&](bool active, int n_c, int n_d, int col, bool in_dir_c,
+        walk_channel<false>(n_a, n_bb, ks, A.n_b, sg, lane, nullptr,
+                            [&](bool active, int n_c, int n_d, int col, bool in_dir_c,
                                                           bool in_ex_c) {
             const bool support = in_dir_c && iabs_d(n_bb - n_d) < ks;
             const bool support_ex = in_ex_c && iabs_d(n_a - n_d) < ks;
