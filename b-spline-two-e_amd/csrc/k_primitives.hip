@@ -129,7 +129,8 @@ __global__ __launch_bounds__(kPThreads) void kp_outer(int K, int kgl, const doub
 }
 
 // SYM: write Dsym[v][u1][u2][k] = D[v][u1][u2][k] + D[v][u2][u1][k] (what the
-// tensor consumes) instead of D (what the sparse_6d container holds).
+// tensor consumes) for u1 <= u2 only, the symmetric matrix's upper triangle,
+// instead of D (what the sparse_6d container holds).
 template <int KS, bool SYM>
 __global__ __launch_bounds__(kPThreads) void kp_inner(int K, int kgl, int kc, int kchunks,
                                                       const double* __restrict__ t, const double* __restrict__ bp,
@@ -205,6 +206,7 @@ __global__ __launch_bounds__(kPThreads) void kp_inner(int K, int kgl, int kc, in
     // D[v][uo][ui][k0 .. k0+nk) = h * sum_q WB[q][uo] J'[kk][q][ui]
     for (int e = tid; e < U * U; e += kPThreads) {
         const int uo = e / U, ui = e - uo * U;
+        if (SYM && uo > ui) continue;   // the lower triangle mirrors the upper
         double acc[KCMAX] = {0.0, 0.0, 0.0, 0.0};
         for (int q = 0; q < kgl; ++q) {
             const double a = sWB[q * U + uo];
@@ -234,8 +236,9 @@ __global__ __launch_bounds__(kPThreads) void kp_inner(int K, int kgl, int kc, in
 //            k > k0 by repeated multiplication with the ratio;
 //   stage 4: per 16 x 16 tile of (uo, ui) and multipole, on v_mfma_f64_16x16x4:
 //            D = WB^T J' (SYM: Dsym = WB^T J' + J'^T WB, the two products in
-//            separate accumulators), zero-padded to whole tiles and to 16
-//            GL points; two multipoles per pass.
+//            separate accumulators, upper-triangle tiles and elements only),
+//            zero-padded to whole tiles and to 16 GL points; two multipoles
+//            per pass.
 template <int KS, bool SYM>
 __global__ __launch_bounds__(kPThreads) void kp_inner_mf(int K, int kgl, int kc, int kchunks,
                                                          const double* __restrict__ t, const double* __restrict__ bp,
@@ -319,8 +322,21 @@ __global__ __launch_bounds__(kPThreads) void kp_inner_mf(int K, int kgl, int kc,
     // Operands are read from clamped addresses and zeroed outside the U x kgl
     // range (no lane-predicated loads), so the loads of a pass issue back to
     // back ahead of its matrix instructions.
-    for (int tile = wave; tile < TU * TU; tile += kPThreads / 64) {
-        const int ta = tile / TU, tb = tile - ta * TU;
+    constexpr int NT = SYM ? TU * (TU + 1) / 2 : TU * TU;
+    for (int tile = wave; tile < NT; tile += kPThreads / 64) {
+        int ta, tb;
+        if (SYM) {   // upper-triangle tiles, row by row
+            ta = 0;
+            int r = tile;
+            while (r >= TU - ta) {
+                r -= TU - ta;
+                ++ta;
+            }
+            tb = ta + r;
+        } else {
+            ta = tile / TU;
+            tb = tile - ta * TU;
+        }
         const int uo_a = ta * 16 + li, ui_b = tb * 16 + li;   // operand row / column of this lane
         const bool va = uo_a < U, vb = ui_b < U;
         const int ua = va ? uo_a : U - 1, ub = vb ? ui_b : U - 1;
@@ -370,7 +386,7 @@ __global__ __launch_bounds__(kPThreads) void kp_inner_mf(int K, int kgl, int kc,
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int uo = ta * 16 + lk + 4 * r, ui = tb * 16 + li;
-                if (uo < U && ui < U) {
+                if (uo < U && ui < U && (!SYM || uo <= ui)) {
                     double* dst = D + (((size_t)v * U + uo) * U + ui) * K + k0 + kk;
                     dst[0] = h * (SYM ? c0[r] + e0[r] : c0[r]);
                     if (two) dst[1] = h * (SYM ? c1[r] + e1[r] : c1[r]);

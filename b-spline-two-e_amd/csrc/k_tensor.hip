@@ -123,8 +123,9 @@ __global__ __launch_bounds__(kThreads) void k3_near(int n_pairs, int n_b, int K,
         for (int w = wlo; w <= whi; ++w) {
             const int jq = w - v0q, jp = w - lo, vz = w - 1;
             acc += Qw[qb + (size_t)jq * K] * sPPx[jp * K + k] + Pw[qb + (size_t)jq * K] * sQSx[(jp + 1) * K + k];
-            const unsigned uq = (unsigned)sym_local(pq.a - vz, pq.c - vz, ks);
-            dacc += Dsym[(unsigned)vz * U * UK + (unsigned)sUp[jp] * UK + uq * (unsigned)K + (unsigned)k];
+            const unsigned uq = (unsigned)sym_local(pq.a - vz, pq.c - vz, ks), up = (unsigned)sUp[jp];
+            const unsigned ulo = up < uq ? up : uq, uhi = up < uq ? uq : up;   // Dsym is read in its upper triangle
+            dacc += Dsym[(unsigned)vz * U * UK + ulo * UK + uhi * (unsigned)K + (unsigned)k];
         }
         val += acc + dacc;
         const size_t o = (size_t)q * K + k;
