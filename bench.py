@@ -372,6 +372,20 @@ def main():
         rk.synchronize()
         lat.append((time.perf_counter() - a) * 1e3)
     lat = np.array(lat) if lat else np.zeros(1)
+    # the practical store ceiling on this box: a store-only fill of a buffer of
+    # the tensor's size on the same stream
+    with torch.cuda.stream(stream):
+        buf = torch.empty(n_int, dtype=torch.float64, device="cuda")
+        buf.zero_()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(5):
+            buf.zero_()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    store_ceiling = 8.0 * n_int * 5 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del buf
     if args.copy_calibration_mib > 0:
         n = args.copy_calibration_mib * (1 << 20) // 8
         with torch.cuda.stream(stream):
@@ -413,7 +427,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": FAR_TRAFFIC_BYTES.get((w.name, int(tuning[0]))),
                      "algorithmic_bytes": traffic_alg, "elements": far_elems,
-                     "avg_launch_ms": t_far * 1e3},
+                     "avg_launch_ms": t_far * 1e3, "store_ceiling_GBs": store_ceiling,
+                     "frac_of_store_ceiling": achieved / store_ceiling},
     }
     if args.blocks:
         out["blocks"] = blocks_benchmark(rk, w, t)
