@@ -38,11 +38,12 @@ FAR_TRAFFIC_BYTES = {("c2", 0): 739.35e6, ("c2", 1): 739.59e6, ("c2", 2): 739.99
                      ("c2", 4): 739.52e6, ("c2", 5): 739.51e6, ("c2", 6): 739.05e6}
 
 
-def cpu_baseline(w, t, R_gpu, nthreads, block_max_l=None):
+def cpu_baseline(w, t, R_gpu, nthreads, block_max_l=None, gpu_block=None):
     """Oracle (C restatement, OpenMP over k like the reference) on the host:
     primitives + dense tensor assembly for the same basis; with block_max_l,
     also the L = 0 block of bench's `blocks` line (count and fill, one
-    thread: the reference builds each symmetry on one thread)."""
+    thread: the reference builds each symmetry on one thread), compared with
+    the device's block (gpu_block = (H, S) CSR) when given."""
     import numpy as np
 
     from oracle import oracle as O
@@ -70,6 +71,16 @@ def cpu_baseline(w, t, R_gpu, nthreads, block_max_l=None):
                       sample=(f"{w.name} basis, L=0 block, l1,l2 <= {block_max_l} ({len(cf)} configurations, "
                               f"{blk['nnz'][0]} H entries): the oracle's count and fill passes; {db:.2f} s"),
                       seconds=db)
+        if gpu_block is not None:   # the whole block against the oracle's (oracle tensor and one-electron matrices)
+            H, Sb = gpu_block
+            pattern = bool(np.array_equal(H.index_ptr, blk["H"][0]) and np.array_equal(H.indices, blk["H"][1])
+                           and np.array_equal(Sb.index_ptr, blk["S"][0]) and np.array_equal(Sb.indices, blk["S"][1]))
+            blocks["parity_vs_oracle"] = dict(
+                pattern_bit_exact=pattern,
+                H_max_abs_err_over_max_abs=float(np.abs(H.data - blk["H"][2]).max() / np.abs(blk["H"][2]).max())
+                if pattern else None,
+                S_max_abs_err_over_max_abs=float(np.abs(Sb.data - blk["S"][2]).max() / np.abs(blk["S"][2]).max())
+                if pattern else None)
     return dict(value=n_int / dt, unit="integrals/s", cores=int(min(nthreads, K)), kind="port",
                 sample=(f"full {w.name} build (primitives + tensor, {n_int} integrals) by the oracle: "
                         f"OpenMP over the {K} multipoles as the reference's !$omp parallel do, "
@@ -184,7 +195,7 @@ def serving_benchmark(n_jobs, seed=20261202, slots=4, budget_gib=64, load=0.8):
                            "mean": float(lat.mean()), "max": float(lat.max())}}
 
 
-def blocks_benchmark(rk, w, t, max_l=2, L=0):
+def blocks_benchmark(rk, w, t, max_l=2, L=0, keep=None):
     """Channel summation into the CSR H/S block of one symmetry (L, even
     parity, l1, l2 <= max_l) from the built tensor; kernel times from the
     library's HIP events, wall time including the host transfers."""
@@ -201,6 +212,8 @@ def blocks_benchmark(rk, w, t, max_l=2, L=0):
     wall = time.perf_counter() - t0
     ms_count, ms_fill = blk.kernel_ms()
     blk.release()
+    if keep is not None:
+        keep.extend([H_sp, S_sp])
     written = 24.0 * (H_sp.nnz + S_sp.nnz)          # 1-based column index + complex value per entry
     return {"workload": f"{w.name} basis, L={L} block, l1,l2 <= {max_l}", "n_config": term.n_config,
             "nnz_H": H_sp.nnz, "nnz_S": S_sp.nnz, "count_kernel_ms": ms_count, "fill_kernel_ms": ms_fill,
@@ -431,7 +444,8 @@ def main():
                      "frac_of_store_ceiling": achieved / store_ceiling},
     }
     if args.blocks:
-        out["blocks"] = blocks_benchmark(rk, w, t)
+        gpu_block = [] if not args.no_cpu_baseline else None
+        out["blocks"] = blocks_benchmark(rk, w, t, keep=gpu_block)
     if args.dipoles:
         out["dipoles"] = dipoles_benchmark(rk.ctx, w, t)
     if args.blockset:
@@ -440,7 +454,8 @@ def main():
         out["serving"] = serving_benchmark(args.serve_jobs)
     if not args.no_cpu_baseline:
         nthreads = min(16, os.cpu_count() or 1)
-        cb, rel, cblocks = cpu_baseline(w, t, R_gpu, nthreads, block_max_l=2 if args.blocks else None)
+        cb, rel, cblocks = cpu_baseline(w, t, R_gpu, nthreads, block_max_l=2 if args.blocks else None,
+                                        gpu_block=tuple(gpu_block) if args.blocks else None)
         out["cpu_baseline"] = cb
         if cblocks is not None:
             out["blocks"]["cpu_baseline"] = cblocks
