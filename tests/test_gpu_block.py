@@ -116,6 +116,42 @@ def test_block_order_16_many_multipoles(ctx, oracle, L):
     compare(H_sp, S_sp, oracle.construct_block(k, n_b, L, max_k, True, term.configs, Rref, Hl, S))
 
 
+RANDOM_BASES = [   # seed, order, k_GL, max_l, L, full
+    (1, 2, 5, 1, 0, True), (2, 3, 6, 2, 1, False), (3, 4, 7, 3, 2, True), (4, 5, 8, 2, 3, False),
+    (5, 6, 9, 1, 1, True), (6, 9, 12, 2, 0, False), (7, 11, 14, 1, 2, True), (8, 12, 15, 2, 1, False),
+]
+
+
+@pytest.mark.parametrize("seed,k,k_gl,max_l,L,full", RANDOM_BASES)
+def test_block_random_small_bases(ctx, oracle, seed, k, k_gl, max_l, L, full):
+    """Small bases of many orders with drawn box sizes, multipole counts and
+    symmetries: patterns (so the closed-form counts, row by row) bit-exact and
+    values against the oracle, through both parities."""
+    rng = np.random.default_rng(seed)
+    h_max, r_max = float(rng.uniform(0.6, 1.6)), float(rng.uniform(6.0, 14.0))
+    max_k = int(rng.integers(1, 2 * max_l + 2))
+    t = bs2e.generate_grid(k, 3, 2, h_max, r_max)
+    n_b = oracle.dims(k, t)[1]
+    print(f"block random seed={seed} k={k} n_b={n_b} max_k={max_k} L={L} full={full}", flush=True)
+    rk = bs2e.RkTensor.build(k, t, k_gl, max_k, ctx=ctx)
+    Rref = oracle.rk_dense(k, n_b, max_k, oracle.slater_off_diag(k, t, k_gl, max_k),
+                           oracle.slater_diag(k, t, k_gl, max_k))
+    S = oracle.setup_S(k, t, k_gl)
+    Hl = [oracle.setup_H1p(k, t, k_gl, 2, l, cap_eta=0.0) for l in range(max_l + 1)]
+    for pi in (bool(L % 2), not bool(L % 2)):
+        term = bs2e.sym(l=L, pi=pi)
+        term.configs = bs2e.count_configs(term, max_l, n_b, k, n_b, n_b, max_l)
+        if term.n_config == 0:
+            continue
+        ref = oracle.construct_block(k, n_b, L, max_k, full, term.configs, Rref, Hl, S)
+        blk = bs2e.Block(rk, term, full)
+        assert (blk.nnz_H, blk.nnz_S) == ref["nnz"] and (blk.ref_nnz_H, blk.ref_nnz_S) == ref["ref_count"]
+        H_sp, S_sp = blk.fill(Hl, S)
+        blk.release()
+        compare(H_sp, S_sp, ref)
+    rk.release()
+
+
 def test_block_truncated_configs_and_cap(c1, oracle):
     """Truncated configuration lists (max_n_b, n_all_l, l_2_max) and a complex
     absorbing potential in the one-electron blocks."""
