@@ -97,6 +97,38 @@ def test_dip_block_matches_oracle(oracle, ctx, name, s1, s2, q, gauge):
     assert ms[0] > 0 and ms[1] > 0
 
 
+@pytest.mark.parametrize("seed,k,k_gl,gauge", [(11, 3, 6, "l"), (12, 5, 8, "v"), (13, 10, 13, "l"), (14, 12, 15, "v")])
+def test_dip_block_random_small_bases(oracle, ctx, seed, k, k_gl, gauge):
+    """Small bases of other orders with drawn boxes: the closed-form row
+    counts and the patterns bit-exact, values within 1e-12 of the block scale,
+    for a drawn symmetry pair and component."""
+    rng = np.random.default_rng(seed)
+    t = bs2e.generate_grid(k, 3, 2, float(rng.uniform(0.6, 1.6)), float(rng.uniform(6.0, 14.0)))
+    n_b = oracle.dims(k, t)[1]
+    L1 = int(rng.integers(0, 3))
+    pi1 = L1 % 2
+    M1 = int(rng.integers(-L1, L1 + 1))
+    q = int(rng.integers(-1, 2))
+    L2, M2 = L1 + 1, M1 - q           # the 3j symbol (L1 1 L2; -M1 q M2) needs -M1 + q + M2 = 0
+    s1, s2 = (L1, M1, pi1), (L2, M2, 1 - pi1)
+    max_l = 2
+    cf1 = oracle.count_configs(L1, pi1, max_l, n_b, k, n_b, n_b, max_l)
+    cf2 = oracle.count_configs(L2, 1 - pi1, max_l, n_b, k, n_b, n_b, max_l)
+    print("dip random", seed, k, n_b, s1, s2, q, gauge, len(cf1), len(cf2), flush=True)
+    S = oracle.setup_S(k, t, k_gl)
+    rad_o = oracle.setup_radial_dip(k, t, k_gl, gauge)
+    rad = bs2e.radial_dipole("l", r_mat=rad_o) if gauge == "l" else \
+        bs2e.radial_dipole("v", dr_mat=rad_o[0], r_inv_mat=rad_o[1])
+    syms = [bs2e.sym(L1, M1, bool(pi1), cf1), bs2e.sym(L2, M2, not bool(pi1), cf2)]
+    blk = bs2e.DipBlock(syms, q, bs2e.b_spline(k, t), ctx=ctx)
+    got = blk.fill(S, rad)
+    blk.release()
+    ip, ix, d = oracle.construct_dip_block(k, n_b, s1, cf1, s2, cf2, q, 1, gauge, S, rad_o)
+    assert got.shape == (len(cf1), len(cf2)) and got.nnz == len(ix)
+    assert np.array_equal(got.index_ptr, ip) and np.array_equal(got.indices, ix)
+    assert len(ix) > 0 and _close(got.data, d)
+
+
 def test_dip_block_end_to_end_and_empty_cases(oracle, ctx):
     """Radial matrices and overlap from the device feed the device block; the
     reference's empty blocks (no parity change, vanishing 3j, compute false)
