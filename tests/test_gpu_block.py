@@ -152,6 +152,32 @@ def test_block_random_small_bases(ctx, oracle, seed, k, k_gl, max_l, L, full):
     rk.release()
 
 
+@pytest.mark.parametrize("k,max_k", [(2, 0), (3, 1), (4, 0), (4, 3)])
+def test_block_tiny_bases(ctx, oracle, k, max_k):
+    """Bases of 7-9 B-splines (every row and column touches a boundary of
+    the basis) and one or a few multipoles (K = 1 included): patterns and
+    values against the oracle for L = 0 and 1, full and upper triangle."""
+    t = bs2e.generate_grid(k, 1, 2, 1.0, 3.0)
+    n_b = oracle.dims(k, t)[1]
+    k_gl, max_l = k + 3, 1
+    rk = bs2e.RkTensor.build(k, t, k_gl, max_k, ctx=ctx)
+    Rref = oracle.rk_dense(k, n_b, max_k, oracle.slater_off_diag(k, t, k_gl, max_k),
+                           oracle.slater_diag(k, t, k_gl, max_k))
+    S = oracle.setup_S(k, t, k_gl)
+    Hl = [oracle.setup_H1p(k, t, k_gl, 2, l, cap_eta=0.0) for l in range(max_l + 1)]
+    print(f"block tiny k={k} n_b={n_b} max_k={max_k}", flush=True)
+    for L, full in ((0, True), (1, False)):
+        term = bs2e.sym(l=L, pi=bool(L % 2))
+        term.configs = bs2e.count_configs(term, max_l, n_b, k, n_b, n_b, max_l)
+        ref = oracle.construct_block(k, n_b, L, max_k, full, term.configs, Rref, Hl, S)
+        blk = bs2e.Block(rk, term, full)
+        assert (blk.nnz_H, blk.nnz_S) == ref["nnz"] and (blk.ref_nnz_H, blk.ref_nnz_S) == ref["ref_count"]
+        H_sp, S_sp = blk.fill(Hl, S)
+        blk.release()
+        compare(H_sp, S_sp, ref)
+    rk.release()
+
+
 def test_block_truncated_configs_and_cap(c1, oracle):
     """Truncated configuration lists (max_n_b, n_all_l, l_2_max) and a complex
     absorbing potential in the one-electron blocks."""
