@@ -112,7 +112,8 @@ __device__ __forceinline__ double kdot3(const double* __restrict__ x, unsigned s
 // from the small multipole-major factor tables, which stay in L2 and put a
 // wavefront's consecutive q on consecutive addresses; overlapping pairs read
 // the tensor.  coef is the wavefront's copy of the channel coefficients in LDS.
-__device__ __forceinline__ double r12_row(const BlockArgs& A, int a1, int c1, int a2, int c2,
+template <class BA>
+__device__ __forceinline__ double r12_row(const BA& A, int a1, int c1, int a2, int c2,
                                           const double* __restrict__ coef)
 {
     const int K = A.K, ks = A.ks;
@@ -198,7 +199,12 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
         else bhi = mid - 1;
     }
     const int b = __builtin_amdgcn_readfirstlane(blo);
-    const BlockArgs A = tab[b];
+    // The descriptor is read through the constant address space (the host
+    // writes it before the launch; nothing in the launch writes it), so the
+    // compiler may load a field again where it is used instead of keeping all
+    // of them in scalar registers and spilling them to vector lanes.
+    typedef const __attribute__((address_space(4))) BlockArgs CBlockArgs;
+    const CBlockArgs& A = ((const CBlockArgs*)tab)[b];
     const int i = (int)(g - row0[b]);
     const int ks = A.ks, K = A.K;
     const int64_t* ci = A.cfg + 5 * (size_t)i;
