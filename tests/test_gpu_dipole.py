@@ -129,6 +129,35 @@ def test_dip_block_random_small_bases(oracle, ctx, seed, k, k_gl, gauge):
     assert len(ix) > 0 and _close(got.data, d)
 
 
+@pytest.mark.parametrize("k,gauge", [(2, "l"), (3, "v"), (4, "l"), (4, "v")])
+def test_dip_block_tiny_bases(oracle, ctx, k, gauge):
+    """The smallest bases the grid allows (orders 2-4, 7-9 B-splines), where
+    every row and column touches an end of the basis: L = 0 -> 1 for
+    q = -1, 0, 1 and L = 1 -> 2 for q = 0, patterns bit-exact, values within
+    1e-12 of the block scale."""
+    t = bs2e.generate_grid(k, 1, 2, 1.0, 3.0)
+    n_b = oracle.dims(k, t)[1]
+    k_gl, max_l = k + 3, 2
+    S = oracle.setup_S(k, t, k_gl)
+    rad_o = oracle.setup_radial_dip(k, t, k_gl, gauge)
+    rad = bs2e.radial_dipole("l", r_mat=rad_o) if gauge == "l" else \
+        bs2e.radial_dipole("v", dr_mat=rad_o[0], r_inv_mat=rad_o[1])
+    for L1, M1, q in ((0, 0, -1), (0, 0, 0), (0, 0, 1), (1, 1, 0)):
+        pi1, L2, M2 = L1 % 2, L1 + 1, M1 - q
+        s1, s2 = (L1, M1, pi1), (L2, M2, 1 - pi1)
+        cf1 = oracle.count_configs(L1, pi1, max_l, n_b, k, n_b, n_b, max_l)
+        cf2 = oracle.count_configs(L2, 1 - pi1, max_l, n_b, k, n_b, n_b, max_l)
+        print("dip tiny", k, n_b, s1, s2, q, gauge, len(cf1), len(cf2), flush=True)
+        syms = [bs2e.sym(L1, M1, bool(pi1), cf1), bs2e.sym(L2, M2, not bool(pi1), cf2)]
+        blk = bs2e.DipBlock(syms, q, bs2e.b_spline(k, t), ctx=ctx)
+        got = blk.fill(S, rad)
+        blk.release()
+        ip, ix, d = oracle.construct_dip_block(k, n_b, s1, cf1, s2, cf2, q, 1, gauge, S, rad_o)
+        assert got.shape == (len(cf1), len(cf2)) and got.nnz == len(ix)
+        assert np.array_equal(got.index_ptr, ip) and np.array_equal(got.indices, ix)
+        assert len(ix) > 0 and _close(got.data, d)
+
+
 def test_dip_block_end_to_end_and_empty_cases(oracle, ctx):
     """Radial matrices and overlap from the device feed the device block; the
     reference's empty blocks (no parity change, vanishing 3j, compute false)
