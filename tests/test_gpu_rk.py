@@ -289,3 +289,25 @@ def test_order_and_gl_sweep_matches_oracle(k, k_gl, max_k, ctx, oracle):
     rel, zp = max_rel(rk.to_host(), oracle.rk_dense(k, n_b, max_k, off, dg))
     assert rel <= TOL and zp == 0.0, (k, k_gl, rel, zp)
     rk.release()
+
+
+@pytest.mark.parametrize("k,max_k", [(2, 0), (2, 3), (3, 0), (3, 2), (4, 1), (4, 4)])
+def test_smallest_bases_match_oracle(k, max_k, ctx, oracle):
+    """The smallest bases the grid allows (orders 2-4, 7-9 B-splines, a few
+    intervals), with one multipole (odd rows of the rank-one fill's 16-byte
+    stores) and a few: every pair window touches an end of the basis.  The
+    tensor element by element against the oracle, and its pair symmetries
+    bit for bit."""
+    t = bs2e.generate_grid(k, 1, 2, 1.0, 3.0)
+    n_b = oracle.dims(k, t)[1]
+    k_gl = k + 2
+    print(f"smallest k={k} max_k={max_k} n_b={n_b}", flush=True)
+    off = oracle.slater_off_diag(k, t, k_gl, max_k)
+    dg = oracle.slater_diag(k, t, k_gl, max_k)
+    rk = bs2e.RkTensor.build(k, t, k_gl, max_k, ctx=ctx)
+    full = rk.to_host()
+    perm = np.array([rk.pair_index(c, a) for a, c in oracle.pair_table(k, n_b)])
+    rk.release()
+    rel, zp = max_rel(full, oracle.rk_dense(k, n_b, max_k, off, dg))
+    assert rel <= TOL and zp == 0.0, (k, max_k, rel, zp)
+    assert np.array_equal(full, full[perm]) and np.array_equal(full, full[:, perm])
