@@ -2,9 +2,12 @@
 
 Knots come from generate_grid(k, m=3, Z=2, h_max, r_max) (grid_tools.f90:6-55);
 h_max is the middle of the plateau of values that gives exactly the named
-number of B-splines n_b.
+number of B-splines n_b.  A workload with perturb_seed set moves every interior
+distinct breakpoint by a seeded multiplicative factor (see perturb_breakpoints).
 """
 from dataclasses import dataclass
+
+import numpy as np
 
 
 @dataclass(frozen=True)
@@ -20,6 +23,8 @@ class Workload:
     m: int = 3
     Z: int = 2
     description: str = ""
+    perturb_seed: int | None = None   # seed of the breakpoint perturbation (None: the bare grid)
+    perturb_rel: float = 0.01         # largest relative move of a breakpoint
 
 
 WORKLOADS = {
@@ -36,4 +41,42 @@ WORKLOADS = {
                    description="order 9, 400 B-splines, r<=400, GL 15, k<=16 (largest serving job)"),
     "c5": Workload("c5", 9, 600, 1000.0, 1.7802, 14, 24, 12,
                    description="order 9, 600 B-splines, r<=1000, GL 14, lmax 12, k<=24"),
+    "c5p": Workload("c5p", 9, 600, 1000.0, 1.7802, 14, 24, 12, perturb_seed=42,
+                    description="order 9, 600 B-splines, r<=1000, GL 14, lmax 12, k<=24, interior breakpoints "
+                                "moved by a seeded (42) factor in [0.99, 1.01]"),
 }
+
+
+def perturb_breakpoints(t, k, seed, rel=0.01):
+    """Knot vector t with every interior distinct breakpoint b_i (all but the
+    first and the last) replaced by b_i * (1 + u), u drawn uniformly from
+    [-rel, rel) by numpy's default_rng(seed), left to right; a draw that is not
+    strictly above the previous (moved) breakpoint and strictly below the next
+    (unmoved) one is drawn again.  The end knots keep their multiplicity k, so
+    the spline count and the support rule are those of t."""
+    t = np.asarray(t, dtype=np.float64)
+    bp = np.unique(t)
+    if len(bp) < 3:
+        return t.copy()
+    rng = np.random.default_rng(seed)
+    out = bp.copy()
+    for i in range(1, len(bp) - 1):
+        while True:
+            v = bp[i] * (1.0 + rng.uniform(-rel, rel))
+            if out[i - 1] < v < bp[i + 1]:
+                out[i] = v
+                break
+    head = np.full(int(np.count_nonzero(t == bp[0])), out[0])
+    tail = np.full(int(np.count_nonzero(t == bp[-1])), out[-1])
+    return np.concatenate([head, out[1:-1], tail])
+
+
+def knots(w):
+    """Knot vector of workload w (a Workload or its name)."""
+    import bs2e
+    if isinstance(w, str):
+        w = WORKLOADS[w]
+    t = bs2e.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+    if w.perturb_seed is not None:
+        t = perturb_breakpoints(t, w.k, w.perturb_seed, w.perturb_rel)
+    return t

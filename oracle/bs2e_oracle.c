@@ -375,8 +375,14 @@ void or_slater_off_diag(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i64 max
 /* mat_els.f90:230-292 setup_Slater_diag + :441-491 compute_Slater_diag.     */
 /* Loop nest j_b_p, j_b (band), i_b_p, i_b (band), i_r (all four supported). */
 /* ------------------------------------------------------------------------ */
-void or_slater_diag(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i64 max_k, i64 nnz6,
-                    double *rdk, i64 *iv6, i64 *i6, i64 *ip6, i64 *j6, i64 *jp6, i64 faithful, i64 nthreads)
+/* The loop for multipoles k_lo..k_hi and outer columns j_b_p <= jp_hi.  With
+ * pair_mask, only the entries whose pair (i, i_p) or (j, j_p) is marked are
+ * computed (the others keep the caller's zeros; ptr order and index arrays are
+ * unchanged, and each computed entry has the bits of the full loop).  rdk may
+ * be NULL (timing: the values go to a sink); *n_done counts computed entries. */
+static void slater_diag_impl(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i64 k_lo, i64 k_hi, i64 jp_hi,
+                             i64 nnz6, double *rdk, i64 *iv6, i64 *i6, i64 *ip6, i64 *j6, i64 *jp6, i64 faithful,
+                             i64 nthreads, const unsigned char *pair_mask, i64 *n_done, double *sink)
 {
     i64 n, n_b, n_iv;
     if (k_gl > 64 || k_s > 64) return;
@@ -410,13 +416,15 @@ void or_slater_diag(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i64 max_k, 
             }
         }
     }
+    i64 done = 0;
+    double sinksum = 0.0;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads((int)nthreads);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) reduction(+ : done, sinksum)
 #else
     (void)nthreads;
 #endif
-    for (i64 k = 0; k <= max_k; ++k) {
+    for (i64 k = k_lo; k <= k_hi; ++k) {
         i64 ptr = 0;
         double r[64], Bi[64], Bip[64];
         /* memo of r_j**k at the inner points and r**(k+1) at the outer points
@@ -439,7 +447,7 @@ void or_slater_diag(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i64 max_k, 
                 }
             }
         }
-        for (i64 jp = 1; jp <= n_b; ++jp)
+        for (i64 jp = 1; jp <= (jp_hi < n_b ? jp_hi : n_b); ++jp)
             for (i64 j = 1; j <= n_b; ++j) {
                 if (llabs(j - jp) >= k_s) continue;               /* :257 */
                 /* Non-faithful mode skips only (i_p, i_r) values for which the
@@ -464,6 +472,17 @@ void or_slater_diag(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i64 max_k, 
                             int sj = supp(k_s, i_r, j + 1) && supp(k_s, i_r, jp + 1);
                             if (!(si && sj)) continue;
                             ++ptr;
+                            if (k == k_lo && iv6) {                       /* :484-490 */
+                                iv6[ptr - 1] = i_r;
+                                i6[ptr - 1] = i;
+                                ip6[ptr - 1] = ip;
+                                j6[ptr - 1] = j;
+                                jp6[ptr - 1] = jp;
+                            }
+                            if (pair_mask && !pair_mask[or_pair_index(k_s, n_b, i, ip)] &&
+                                !pair_mask[or_pair_index(k_s, n_b, j, jp)])
+                                continue;
+                            ++done;
                             double lim1 = bp[i_r - 1], lim2 = bp[i_r];
                             double scale_i = 0.5 * (lim2 - lim1);         /* :460 */
                             double translate_i = 0.5 * (lim2 + lim1);     /* :461 */
@@ -501,14 +520,10 @@ void or_slater_diag(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i64 max_k, 
                                 double rq1 = faithful ? ipow(r[q], k + 1) : pout[(i_r - 1) * k_gl + q];
                                 acc = acc + w[q] * Bi[q] * Bip[q] * int_j / rq1;
                             }
-                            rdk[k * nnz6 + (ptr - 1)] = scale_i * acc;    /* :483 */
-                            if (k == 0) {                                 /* :484-490 */
-                                iv6[ptr - 1] = i_r;
-                                i6[ptr - 1] = i;
-                                ip6[ptr - 1] = ip;
-                                j6[ptr - 1] = j;
-                                jp6[ptr - 1] = jp;
-                            }
+                            if (rdk)
+                                rdk[k * nnz6 + (ptr - 1)] = scale_i * acc;  /* :483 */
+                            else
+                                sinksum += scale_i * acc;
                         }
                     }
             }
@@ -519,6 +534,39 @@ void or_slater_diag(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i64 max_k, 
     free(bi);
     free(x);
     free(w);
+    if (n_done) *n_done = done;
+    if (sink) *sink = sinksum;
+}
+
+void or_slater_diag(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i64 max_k, i64 nnz6,
+                    double *rdk, i64 *iv6, i64 *i6, i64 *ip6, i64 *j6, i64 *jp6, i64 faithful, i64 nthreads)
+{
+    slater_diag_impl(k_s, n_knots, t, k_gl, 0, max_k, INT64_MAX, nnz6, rdk, iv6, i6, ip6, j6, jp6, faithful,
+                     nthreads, NULL, NULL, NULL);
+}
+
+/* or_slater_diag computing only the entries that touch a marked pair
+ * (pair_mask[0-based pair index] != 0): the others stay zero. */
+void or_slater_diag_masked(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i64 max_k, i64 nnz6,
+                           double *rdk, i64 *iv6, i64 *i6, i64 *ip6, i64 *j6, i64 *jp6, i64 nthreads,
+                           const unsigned char *pair_mask)
+{
+    slater_diag_impl(k_s, n_knots, t, k_gl, 0, max_k, INT64_MAX, nnz6, rdk, iv6, i6, ip6, j6, jp6, 0, nthreads,
+                     pair_mask, NULL, NULL);
+}
+
+/* Timing sample of the reference's diagonal-cell schedule: multipole k_sel
+ * alone (the unit of work of one thread of the reference's !$omp parallel do
+ * over k, mat_els.f90:247-248), outer columns j_b_p <= jp_hi, every B-spline
+ * value evaluated inside the loops as the reference evaluates it (faithful).
+ * Returns the number of entries computed; *sink receives their sum. */
+i64 or_slater_diag_sample(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i64 k_sel, i64 jp_hi, i64 faithful,
+                          double *sink)
+{
+    i64 done = 0;
+    slater_diag_impl(k_s, n_knots, t, k_gl, k_sel, k_sel, jp_hi, 0, NULL, NULL, NULL, NULL, NULL, NULL, faithful, 1,
+                     NULL, &done, sink);
+    return done;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -567,6 +615,187 @@ void or_rk_dense(i64 k_s, i64 n_b, i64 max_k,
     }
     free(val);
     free(pidx);
+}
+
+/* Rows of or_rk_dense: out[r][q][k] = R[rows[r]][q][k] for the 0-based pair
+ * indices rows[0..n_rows) (distinct).  The same accumulation restricted to
+ * the products whose first factor's pair is a requested row and to the
+ * diagonal cells that land in one, in the same order: the same bits. */
+void or_rk_rows(i64 k_s, i64 n_b, i64 max_k,
+                i64 nnz4, const double *rk, const double *rmk, const i64 *iv4, const i64 *i4, const i64 *j4,
+                i64 nnz6, const double *rdk, const i64 *i6, const i64 *ip6, const i64 *j6, const i64 *jp6,
+                i64 n_rows, const i64 *rows, double *out)
+{
+    i64 K = max_k + 1;
+    i64 np = or_count_pairs(k_s, n_b);
+    i64 *rowpos = (i64 *)malloc(sizeof(i64) * (size_t)np);
+    for (i64 p = 0; p < np; ++p) rowpos[p] = -1;
+    for (i64 r = 0; r < n_rows; ++r) rowpos[rows[r]] = r;
+    i64 *pidx = (i64 *)malloc(sizeof(i64) * (size_t)(nnz4 > 0 ? nnz4 : 1));
+    for (i64 e = 0; e < nnz4; ++e) pidx[e] = or_pair_index(k_s, n_b, i4[e], j4[e]);
+    memset(out, 0, sizeof(double) * (size_t)(n_rows * np * K));
+    for (i64 nn = 0; nn < nnz4; ++nn) {
+        i64 r = rowpos[pidx[nn]];
+        if (r < 0) continue;
+        for (i64 mm = 0; mm < nnz4; ++mm) {
+            int lt = iv4[nn] < iv4[mm], gt = iv4[nn] > iv4[mm];
+            if (!lt && !gt) continue;
+            double *dst = out + (r * np + pidx[mm]) * K;
+            for (i64 k = 0; k < K; ++k) {
+                double v = lt ? rk[k * nnz4 + nn] * rmk[k * nnz4 + mm] : rk[k * nnz4 + mm] * rmk[k * nnz4 + nn];
+                dst[k] = dst[k] + v;
+            }
+        }
+    }
+    for (i64 e = 0; e < nnz6; ++e) {
+        i64 p = or_pair_index(k_s, n_b, i6[e], ip6[e]);
+        i64 q = or_pair_index(k_s, n_b, j6[e], jp6[e]);
+        if (rowpos[p] >= 0) {
+            double *d1 = out + (rowpos[p] * np + q) * K;
+            for (i64 k = 0; k < K; ++k) d1[k] = d1[k] + rdk[k * nnz6 + e];
+        }
+        if (rowpos[q] >= 0) {
+            double *d2 = out + (rowpos[q] * np + p) * K;
+            for (i64 k = 0; k < K; ++k) d2[k] = d2[k] + rdk[k * nnz6 + e];
+        }
+    }
+    free(pidx);
+    free(rowpos);
+}
+
+/* ------------------------------------------------------------------------ */
+/* sparse_array_tools.f90:452-555 compute_R_K_map into an Nd_DOK as the      */
+/* reference builds it: one set_val per product (:472-482) and two per       */
+/* diagonal cell (:484-488), each keyed by the 32 bytes of the four int64    */
+/* indices (:519 transfer), looked up in an open-addressing hash map (the    */
+/* reference's stdlib open_hashmap_type; its published scheme: FNV-1 hash of */
+/* the key bytes, linear probing, table doubled as it fills); a present key  */
+/* is found through a freshly allocated copy of its slot data (:521-526      */
+/* get_other_data into class(*), allocatable :: temp), a new key appends its */
+/* K values to the data array (:530-533).  Used to time the reference's      */
+/* serial assembly on a leading fraction of the product loop (n < n_lead)   */
+/* and, with R_out, to check it against or_rk_dense.  Returns the number of  */
+/* set_val calls, or -1 when memory runs out.                                */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    i64 key[4];
+    uint32_t hash;
+    int used;
+    i64 ptr;          /* first value in data */
+} dok_slot;
+
+typedef struct {
+    dok_slot *slots;
+    i64 cap, n, K;
+    double *data;
+    i64 data_cap;
+} dok_map;
+
+static uint32_t fnv1_32(const unsigned char *b, size_t n)
+{
+    uint32_t h = 2166136261u;
+    for (size_t i = 0; i < n; ++i) {
+        h = h * 16777619u;
+        h = h ^ b[i];
+    }
+    return h;
+}
+
+static int dok_grow(dok_map *m)
+{
+    i64 nc = m->cap ? 2 * m->cap : 1024;
+    dok_slot *ns = (dok_slot *)calloc((size_t)nc, sizeof(dok_slot));
+    if (!ns) return -1;
+    for (i64 i = 0; i < m->cap; ++i) {
+        if (!m->slots[i].used) continue;
+        i64 j = (i64)(m->slots[i].hash & (uint32_t)(nc - 1));
+        while (ns[j].used) j = (j + 1) & (nc - 1);
+        ns[j] = m->slots[i];
+    }
+    free(m->slots);
+    m->slots = ns;
+    m->cap = nc;
+    return 0;
+}
+
+static int dok_set_val(dok_map *m, const i64 idx[4], const double *val)
+{
+    if (2 * (m->n + 1) > m->cap && dok_grow(m)) return -1;
+    unsigned char key[32];
+    memcpy(key, idx, 32);
+    uint32_t h = fnv1_32(key, 32);
+    i64 j = (i64)(h & (uint32_t)(m->cap - 1));
+    while (m->slots[j].used) {
+        if (m->slots[j].hash == h && memcmp(m->slots[j].key, key, 32) == 0) {
+            i64 *temp = (i64 *)malloc(sizeof(i64));   /* class(*), allocatable :: temp */
+            if (!temp) return -1;
+            *temp = m->slots[j].ptr;
+            double *d = m->data + *temp;
+            for (i64 k = 0; k < m->K; ++k) d[k] = d[k] + val[k];
+            free(temp);
+            return 0;
+        }
+        j = (j + 1) & (m->cap - 1);
+    }
+    if ((m->n + 1) * m->K > m->data_cap) {
+        i64 nc = m->data_cap ? 2 * m->data_cap : 1024 * m->K;
+        double *nd = (double *)realloc(m->data, sizeof(double) * (size_t)nc);
+        if (!nd) return -1;
+        m->data = nd;
+        m->data_cap = nc;
+    }
+    dok_slot *sl = &m->slots[j];
+    memcpy(sl->key, key, 32);
+    sl->hash = h;
+    sl->used = 1;
+    sl->ptr = m->n * m->K;
+    memcpy(m->data + sl->ptr, val, sizeof(double) * (size_t)m->K);
+    m->n += 1;
+    return 0;
+}
+
+i64 or_rk_dok(i64 k_s, i64 n_b, i64 max_k,
+              i64 nnz4, const double *rk, const double *rmk, const i64 *iv4, const i64 *i4, const i64 *j4,
+              i64 nnz6, const double *rdk, const i64 *i6, const i64 *ip6, const i64 *j6, const i64 *jp6,
+              i64 n_lead, double *R_out)
+{
+    i64 K = max_k + 1;
+    dok_map m = {NULL, 0, 0, K, NULL, 0};
+    double *val = (double *)malloc(sizeof(double) * (size_t)K);
+    i64 calls = 0;
+    int err = 0;
+    if (n_lead > nnz4) n_lead = nnz4;
+    for (i64 nn = 0; nn < n_lead && !err; ++nn)
+        for (i64 mm = 0; mm < nnz4; ++mm) {
+            int lt = iv4[nn] < iv4[mm], gt = iv4[nn] > iv4[mm];
+            if (!lt && !gt) continue;
+            for (i64 k = 0; k < K; ++k)
+                val[k] = lt ? rk[k * nnz4 + nn] * rmk[k * nnz4 + mm] : rk[k * nnz4 + mm] * rmk[k * nnz4 + nn];
+            i64 key[4] = {i4[nn], i4[mm], j4[nn], j4[mm]};
+            if (dok_set_val(&m, key, val)) { err = 1; break; }
+            ++calls;
+        }
+    for (i64 e = 0; e < nnz6 && !err; ++e) {
+        for (i64 k = 0; k < K; ++k) val[k] = rdk[k * nnz6 + e];
+        i64 k1[4] = {i6[e], j6[e], ip6[e], jp6[e]};
+        i64 k2[4] = {j6[e], i6[e], jp6[e], ip6[e]};
+        if (dok_set_val(&m, k1, val) || dok_set_val(&m, k2, val)) { err = 1; break; }
+        calls += 2;
+    }
+    if (!err && R_out) {
+        i64 np = or_count_pairs(k_s, n_b);
+        memset(R_out, 0, sizeof(double) * (size_t)(np * np * K));
+        for (i64 i = 0; i < m.cap; ++i) {
+            if (!m.slots[i].used) continue;
+            const i64 *key = m.slots[i].key;   /* [a, b, c, d] */
+            i64 p = or_pair_index(k_s, n_b, key[0], key[2]), q = or_pair_index(k_s, n_b, key[1], key[3]);
+            memcpy(R_out + (p * np + q) * K, m.data + m.slots[i].ptr, sizeof(double) * (size_t)K);
+        }
+    }
+    free(val);
+    free(m.slots);
+    free(m.data);
+    return err ? -1 : calls;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -802,9 +1031,25 @@ static double r12(i64 K, const double *vals, const double *ang_row)
     return res;
 }
 
-i64 or_construct_block(i64 k_s, i64 n_b, i64 L, i64 max_k, i64 full, i64 n_config, const i64 *cf, const double *R,
-                       const double complex *Hl, const double complex *S, i64 *ip_H, i64 *ix_H, double complex *dH,
-                       i64 *ip_S, i64 *ix_S, double complex *dS, i64 cap_H, i64 cap_S, i64 *nnz, i64 *ref_cnt)
+/* Values of key [a,b,c,d] in a dense banded tensor R[(p*NP + q)*K + k]
+ * (p = pair(a,c), q = pair(b,d)), or, with rmap, in a tensor holding only the
+ * rows p with rmap[p] >= 0 (at row rmap[p]).  A row that is absent sets
+ * *missing and reads row 0. */
+static const double *tensor_row(const double *R, const i64 *rmap, i64 np, i64 K, i64 k_s, i64 n_b, i64 a, i64 b,
+                                i64 c, i64 d, int *missing)
+{
+    i64 p = or_pair_index(k_s, n_b, a, c), q = or_pair_index(k_s, n_b, b, d);
+    if (rmap) {
+        p = rmap[p];
+        if (p < 0) { *missing = 1; p = 0; }
+    }
+    return R + (p * np + q) * K;
+}
+
+static i64 construct_block_impl(i64 k_s, i64 n_b, i64 L, i64 max_k, i64 full, i64 n_config, const i64 *cf,
+                                const double *R, const i64 *rmap, i64 row_lo, i64 row_hi, const double complex *Hl,
+                                const double complex *S, i64 *ip_H, i64 *ix_H, double complex *dH, i64 *ip_S,
+                                i64 *ix_S, double complex *dS, i64 cap_H, i64 cap_S, i64 *nnz, i64 *ref_cnt)
 {
     i64 K = max_k + 1;
     i64 np = or_count_pairs(k_s, n_b);
@@ -826,10 +1071,11 @@ i64 or_construct_block(i64 k_s, i64 n_b, i64 L, i64 max_k, i64 full, i64 n_confi
                     for (i64 k = 0; k < K; ++k)
                         angt[(((la * nl + lb) * nl + lc) * nl + ld) * K + k] = or_ang_k_LS(k, la, lb, lc, ld, L);
 #define ANGROW(a, b, c, d) (angt + ((((a) * nl + (b)) * nl + (c)) * nl + (d)) * K)
-#define GETV(a, b, c, d) (R + ((or_pair_index(k_s, n_b, (a), (c)) * np + or_pair_index(k_s, n_b, (b), (d))) * K))
+    int missing_row = 0;
+#define GETV(a, b, c, d) tensor_row(R, rmap, np, K, k_s, n_b, (a), (b), (c), (d), &missing_row)
 #define HL(l, r, c) Hl[(l) * n_b * n_b + ((c) - 1) * n_b + ((r) - 1)]
 #define SM(r, c) S[((c) - 1) * n_b + ((r) - 1)]
-    for (i64 i = 0; i < n_config; ++i) {
+    for (i64 i = row_lo; i < row_hi; ++i) {
         i64 n_a = cf[5 * i + 0], n_bb = cf[5 * i + 1], l_a = cf[5 * i + 2], l_b = cf[5 * i + 3];
         i64 low = full ? 0 : i;
         for (i64 j = low; j < n_config; ++j) {
@@ -894,16 +1140,40 @@ i64 or_construct_block(i64 k_s, i64 n_b, i64 L, i64 max_k, i64 full, i64 n_confi
             if (one_el) ++ptrS;
             if (entry) ++ptrH;
         }
-        if (fill) { ip_H[i + 1] = ptrH + 1; ip_S[i + 1] = ptrS + 1; }
+        if (fill) { ip_H[i - row_lo + 1] = ptrH + 1; ip_S[i - row_lo + 1] = ptrS + 1; }
     }
 #undef GETV
 #undef HL
 #undef SM
 #undef ANGROW
     free(angt);
+    if (missing_row) return -3;
     if (nnz) { nnz[0] = ptrH; nnz[1] = ptrS; }
     if (ref_cnt) { ref_cnt[0] = cntH; ref_cnt[1] = cntS; }
     return 0;
+}
+
+i64 or_construct_block(i64 k_s, i64 n_b, i64 L, i64 max_k, i64 full, i64 n_config, const i64 *cf, const double *R,
+                       const double complex *Hl, const double complex *S, i64 *ip_H, i64 *ix_H, double complex *dH,
+                       i64 *ip_S, i64 *ix_S, double complex *dS, i64 cap_H, i64 cap_S, i64 *nnz, i64 *ref_cnt)
+{
+    return construct_block_impl(k_s, n_b, L, max_k, full, n_config, cf, R, NULL, 0, n_config, Hl, S, ip_H, ix_H, dH,
+                                ip_S, ix_S, dS, cap_H, cap_S, nnz, ref_cnt);
+}
+
+/* Rows [row_lo, row_hi) of the block of or_construct_block, the same loop
+ * restricted to those rows (index_ptr relative to row_lo, 1-based), with R
+ * holding only some tensor rows: row pair(a,c) of the full tensor is row
+ * rmap[pair(a,c)] of R (-1: absent; the call then returns -3).  The counts
+ * returned in ref_cnt cover the rows only. */
+i64 or_construct_block_rows(i64 k_s, i64 n_b, i64 L, i64 max_k, i64 full, i64 n_config, const i64 *cf,
+                            const double *R, const i64 *rmap, i64 row_lo, i64 row_hi, const double complex *Hl,
+                            const double complex *S, i64 *ip_H, i64 *ix_H, double complex *dH, i64 *ip_S, i64 *ix_S,
+                            double complex *dS, i64 cap_H, i64 cap_S, i64 *nnz, i64 *ref_cnt)
+{
+    if (row_lo < 0 || row_hi > n_config || row_lo > row_hi) return -4;
+    return construct_block_impl(k_s, n_b, L, max_k, full, n_config, cf, R, rmap, row_lo, row_hi, Hl, S, ip_H, ix_H,
+                                dH, ip_S, ix_S, dS, cap_H, cap_S, nnz, ref_cnt);
 }
 
 /* ------------------------------------------------------------------------ */

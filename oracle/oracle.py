@@ -71,6 +71,21 @@ def lib():
         L.or_slater_diag.restype = None
         L.or_slater_diag.argtypes = [i64, i64, P(dbl), i64, i64, i64, P(dbl), P(i64), P(i64), P(i64),
                                      P(i64), P(i64), i64, i64]
+        L.or_slater_diag_masked.restype = None
+        L.or_slater_diag_masked.argtypes = [i64, i64, P(dbl), i64, i64, i64, P(dbl), P(i64), P(i64), P(i64),
+                                            P(i64), P(i64), i64, P(C.c_ubyte)]
+        L.or_slater_diag_sample.restype = i64
+        L.or_slater_diag_sample.argtypes = [i64, i64, P(dbl), i64, i64, i64, i64, P(dbl)]
+        L.or_rk_rows.restype = None
+        L.or_rk_rows.argtypes = [i64, i64, i64, i64, P(dbl), P(dbl), P(i64), P(i64), P(i64), i64, P(dbl),
+                                 P(i64), P(i64), P(i64), P(i64), i64, P(i64), P(dbl)]
+        L.or_rk_dok.restype = i64
+        L.or_rk_dok.argtypes = [i64, i64, i64, i64, P(dbl), P(dbl), P(i64), P(i64), P(i64), i64, P(dbl),
+                                P(i64), P(i64), P(i64), P(i64), i64, P(dbl)]
+        L.or_construct_block_rows.restype = i64
+        L.or_construct_block_rows.argtypes = [i64, i64, i64, i64, i64, i64, P(i64), P(dbl), P(i64), i64, i64,
+                                              P(dbl), P(dbl), P(i64), P(i64), P(dbl), P(i64), P(i64), P(dbl), i64,
+                                              i64, P(i64), P(i64)]
         L.or_rk_dense.restype = None
         L.or_rk_dense.argtypes = [i64, i64, i64, i64, P(dbl), P(dbl), P(i64), P(i64), P(i64), i64, P(dbl),
                                   P(i64), P(i64), P(i64), P(i64), P(dbl), P(C.c_ubyte)]
@@ -173,17 +188,34 @@ def slater_off_diag(k, t, k_gl, max_k, faithful=False, nthreads=0):
     return dict(nnz=nnz, rk=rk.reshape(K, nnz).T, rmk=rmk.reshape(K, nnz).T, iv=iv, i=ii, j=jj)
 
 
-def slater_diag(k, t, k_gl, max_k, faithful=False, nthreads=0):
-    """mat_els.f90:230-292 + :441-491 -> dict(rdk (nnz6, K), iv, i, ip, j, jp)"""
+def slater_diag(k, t, k_gl, max_k, faithful=False, nthreads=0, pair_mask=None):
+    """mat_els.f90:230-292 + :441-491 -> dict(rdk (nnz6, K), iv, i, ip, j, jp).
+    pair_mask (bool per 0-based banded pair): compute only the entries whose
+    pair (i, i') or (j, j') is marked; the others stay zero (index arrays are
+    complete; each computed entry has the bits of the full call)."""
     t = np.ascontiguousarray(t, dtype=np.float64)
     nnz = count_nnz6(k, t)
     K = max_k + 1
     rdk = np.zeros(nnz * K)
     arrs = [np.zeros(nnz, np.int64) for _ in range(5)]
-    lib().or_slater_diag(k, len(t), _ptr(t, dbl), k_gl, max_k, nnz, _ptr(rdk, dbl),
-                         *[_ptr(a, i64) for a in arrs], int(faithful), nthreads)
+    if pair_mask is None:
+        lib().or_slater_diag(k, len(t), _ptr(t, dbl), k_gl, max_k, nnz, _ptr(rdk, dbl),
+                             *[_ptr(a, i64) for a in arrs], int(faithful), nthreads)
+    else:
+        mask = np.ascontiguousarray(pair_mask, dtype=np.uint8)
+        lib().or_slater_diag_masked(k, len(t), _ptr(t, dbl), k_gl, max_k, nnz, _ptr(rdk, dbl),
+                                    *[_ptr(a, i64) for a in arrs], nthreads, _ptr(mask, C.c_ubyte))
     iv, ii, ip, jj, jp = arrs
     return dict(nnz=nnz, rdk=rdk.reshape(K, nnz).T, iv=iv, i=ii, ip=ip, j=jj, jp=jp)
+
+
+def slater_diag_sample(k, t, k_gl, k_sel, jp_hi, faithful=True):
+    """Timing sample of the reference's diagonal-cell loop (mat_els.f90:247-289):
+    multipole k_sel only, outer columns j_b_p <= jp_hi, one thread.  Returns the
+    number of sparse_6d entries computed."""
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    sink = dbl(0.0)
+    return lib().or_slater_diag_sample(k, len(t), _ptr(t, dbl), k_gl, k_sel, jp_hi, int(faithful), C.byref(sink))
 
 
 def rk_dense(k, n_b, max_k, off, diag, want_touched=False):
@@ -202,6 +234,47 @@ def rk_dense(k, n_b, max_k, off, diag, want_touched=False):
                       _ptr(touched, C.c_ubyte) if want_touched else None)
     R = R.reshape(npairs, npairs, K)
     return (R, touched.reshape(npairs, npairs)) if want_touched else R
+
+
+def rk_rows(k, n_b, max_k, off, diag, rows):
+    """Rows `rows` (0-based pair indices) of rk_dense, bit for bit: shape
+    (len(rows), n_pairs, K).  Only the diagonal cells that land in these rows
+    are read, so `diag` may come from slater_diag(pair_mask=...) with the rows
+    marked."""
+    K = max_k + 1
+    npairs = n_pairs(k, n_b)
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    assert len(np.unique(rows)) == len(rows) and rows.min() >= 0 and rows.max() < npairs
+    out = np.zeros(len(rows) * npairs * K)
+    rk = np.ascontiguousarray(off["rk"].T)
+    rmk = np.ascontiguousarray(off["rmk"].T)
+    rdk = np.ascontiguousarray(diag["rdk"].T)
+    lib().or_rk_rows(k, n_b, max_k, off["nnz"], _ptr(rk, dbl), _ptr(rmk, dbl), _ptr(off["iv"], i64),
+                     _ptr(off["i"], i64), _ptr(off["j"], i64), diag["nnz"], _ptr(rdk, dbl),
+                     _ptr(diag["i"], i64), _ptr(diag["ip"], i64), _ptr(diag["j"], i64),
+                     _ptr(diag["jp"], i64), len(rows), _ptr(rows, i64), _ptr(out, dbl))
+    return out.reshape(len(rows), npairs, K)
+
+
+def rk_dok(k, n_b, max_k, off, diag, n_lead=None, want_dense=False):
+    """compute_R_K_map (sparse_array_tools.f90:452-493) into a restated Nd_DOK
+    hash map, the reference's serial schedule, for the products of the first
+    n_lead sparse_4d entries (all when None) and every diagonal cell.  Returns
+    the number of set_val calls (and the dense tensor of the map's contents
+    with want_dense)."""
+    K = max_k + 1
+    npairs = n_pairs(k, n_b)
+    rk = np.ascontiguousarray(off["rk"].T)
+    rmk = np.ascontiguousarray(off["rmk"].T)
+    rdk = np.ascontiguousarray(diag["rdk"].T)
+    R = np.zeros(npairs * npairs * K) if want_dense else None
+    n = lib().or_rk_dok(k, n_b, max_k, off["nnz"], _ptr(rk, dbl), _ptr(rmk, dbl), _ptr(off["iv"], i64),
+                        _ptr(off["i"], i64), _ptr(off["j"], i64), diag["nnz"], _ptr(rdk, dbl),
+                        _ptr(diag["i"], i64), _ptr(diag["ip"], i64), _ptr(diag["j"], i64), _ptr(diag["jp"], i64),
+                        off["nnz"] if n_lead is None else int(n_lead), _ptr(R, dbl) if want_dense else None)
+    if n < 0:
+        raise MemoryError("rk_dok: host allocation failed")
+    return (n, R.reshape(npairs, npairs, K)) if want_dense else n
 
 
 def pair_table(k, n_b):
@@ -307,6 +380,55 @@ def construct_block(k, n_b, L, max_k, full, configs, R, Hl, S):
     assert rc == 0, rc
     return dict(H=(ipH, ixH[:nnz[0]], dH[:nnz[0]]), S=(ipS, ixS[:nnz[1]], dS[:nnz[1]]),
                 nnz=tuple(int(x) for x in nnz), ref_count=tuple(int(x) for x in ref))
+
+
+def construct_block_rows(k, n_b, L, max_k, full, configs, R_rows, rmap, row_lo, row_hi, Hl, S):
+    """Rows [row_lo, row_hi) of construct_block (hamiltonian.f90:106-283), the
+    same loop restricted to them, with a tensor that holds only some rows:
+    full-tensor row p is row rmap[p] of R_rows (rmap[p] = -1: absent).
+    Returns dict(H=(index_ptr relative to row_lo, indices, data), S=(...),
+    nnz=(nH, nS))."""
+    Lb = lib()
+    cf = np.ascontiguousarray(configs, dtype=np.int64)
+    n = cf.shape[0]
+    Rf = np.ascontiguousarray(R_rows, dtype=np.float64)
+    rm = np.ascontiguousarray(rmap, dtype=np.int64)
+    Hf = np.ascontiguousarray(np.stack([np.asfortranarray(h).ravel(order="F") for h in Hl]), dtype=np.complex128)
+    Sf = np.ascontiguousarray(np.asfortranarray(S).ravel(order="F"), dtype=np.complex128)
+    nnz = np.zeros(2, np.int64)
+    ref = np.zeros(2, np.int64)
+    common = [k, n_b, L, max_k, int(full), n, _ptr(cf, i64), _ptr(Rf, dbl), _ptr(rm, i64), row_lo, row_hi,
+              Hf.ctypes.data_as(P(dbl)), Sf.ctypes.data_as(P(dbl))]
+    rc = Lb.or_construct_block_rows(*common, None, None, None, None, None, None, 0, 0, _ptr(nnz, i64),
+                                    _ptr(ref, i64))
+    assert rc == 0, rc
+    m = row_hi - row_lo
+    ipH = np.zeros(m + 1, np.int64)
+    ixH = np.zeros(max(nnz[0], 1), np.int64)
+    dH = np.zeros(max(nnz[0], 1), np.complex128)
+    ipS = np.zeros(m + 1, np.int64)
+    ixS = np.zeros(max(nnz[1], 1), np.int64)
+    dS = np.zeros(max(nnz[1], 1), np.complex128)
+    rc = Lb.or_construct_block_rows(*common, _ptr(ipH, i64), _ptr(ixH, i64), dH.ctypes.data_as(P(dbl)),
+                                    _ptr(ipS, i64), _ptr(ixS, i64), dS.ctypes.data_as(P(dbl)), int(nnz[0]),
+                                    int(nnz[1]), _ptr(nnz, i64), _ptr(ref, i64))
+    if rc == -3:   # the count pass reads no values; the fill pass finds the missing row
+        raise ValueError("construct_block_rows: a tensor row the rows need is absent from R_rows")
+    assert rc == 0, rc
+    return dict(H=(ipH, ixH[:nnz[0]], dH[:nnz[0]]), S=(ipS, ixS[:nnz[1]], dS[:nnz[1]]),
+                nnz=tuple(int(x) for x in nnz))
+
+
+def block_tensor_rows(k, n_b, configs, row_lo, row_hi):
+    """0-based pair indices of the tensor rows that rows [row_lo, row_hi) of a
+    block read: pair(n_a, c) for every c within the band of n_a (direct and
+    exchange values of config (n_a, n_b) are R[pair(n_a, .)][...])."""
+    cf = np.asarray(configs)
+    rows = set()
+    for na in np.unique(cf[row_lo:row_hi, 0]):
+        for c in range(max(1, na - k + 1), min(n_b, na + k - 1) + 1):
+            rows.add(pair_index(k, n_b, int(na), c))
+    return np.array(sorted(rows), dtype=np.int64)
 
 
 # --------------------------------------------------------------------------

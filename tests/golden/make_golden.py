@@ -6,6 +6,10 @@ oracle/_ref/libbs2e_ref.so; this script runs them on fixed inputs and stores
 inputs and outputs, so that the pins hold where /root/reference is absent.
 
     make -C oracle ref && python tests/golden/make_golden.py
+
+It also writes tests/golden/workload_knots.npz: the knot vector of every named
+workload (bs2e.workloads.knots, the seeded breakpoint perturbation included),
+so that the workloads' bases are pinned as data.
 """
 import os
 import sys
@@ -20,6 +24,7 @@ from oracle import oracle as O  # noqa: E402
 from bs2e.workloads import WORKLOADS  # noqa: E402
 
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_fixtures.npz")
+KNOTS_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "workload_knots.npz")
 
 
 def main():
@@ -30,6 +35,10 @@ def main():
     for name in ("tiny", "t0", "c1", "c2", "c3", "c5"):
         w = WORKLOADS[name]
         fx[f"knots_{name}"] = O.ref_generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+    # the h_max values recorded beside the benchmark configs
+    for name, h in (("c1", 2.2476), ("c2", 1.2007), ("c3", 1.123), ("c5", 1.7786)):
+        w = WORKLOADS[name]
+        fx[f"knots_baseline_hmax_{name}"] = O.ref_generate_grid(w.k, w.m, w.Z, h, w.r_max)
     # B-spline values (BVALUE_D through eval_d / d_eval_d) at random points of
     # random intervals, for every local spline, value and 1st/2nd derivative.
     for name in ("tiny", "c1"):
@@ -60,6 +69,10 @@ def main():
         fx[f"cfg_c1_trunc_L{L}"] = O.ref_count_configs(L, L % 2, 3, n_b, w.k, trunc[0], trunc[1], 1)
     np.savez_compressed(OUT, **fx)
     print("wrote", OUT, sum(v.nbytes for v in fx.values()), "bytes")
+    from bs2e.workloads import knots
+    kn = {name: knots(w) for name, w in WORKLOADS.items()}
+    np.savez_compressed(KNOTS_OUT, **kn)
+    print("wrote", KNOTS_OUT, sum(v.nbytes for v in kn.values()), "bytes")
 
 
 if __name__ == "__main__":
