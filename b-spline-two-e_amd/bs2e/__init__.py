@@ -218,6 +218,14 @@ class RkTensor:
     def mirror(self):
         check(capi.lib().bs2e_rk_copy_to_host(self.handle, None, 0), self.ctx.handle)
 
+    def rows(self, rows) -> np.ndarray:
+        """Tensor rows (0-based pair indices), shape (len(rows), n_pairs, K)."""
+        rows = np.asarray(rows, dtype=np.int64)
+        out = np.empty((len(rows), self.n_pairs, self.K))
+        for i, r in enumerate(rows):
+            check(capi.lib().bs2e_rk_copy_rows(self.handle, int(r), 1, ptr(out[i])), self.ctx.handle)
+        return out
+
     def get_val(self, indices) -> np.ndarray:
         idx = np.ascontiguousarray(indices, dtype=np.int64)
         vals = np.zeros(self.K)
@@ -580,6 +588,42 @@ class BlockSet:
         check(capi.lib().bs2e_blockset_get(self.handle, i + 1, ptr(ipH, I), ptr(ixH, I), dH.ctypes.data_as(D),
                                            ptr(ipS, I), ptr(ixS, I), dS.ctypes.data_as(D)), self.ctx.handle)
         return (CSR_matrix((n, n), ipH, ixH[:nH], dH[:nH]), CSR_matrix((n, n), ipS, ixS[:nS], dS[:nS]))
+
+    def row_offsets(self, i, row_lo, n_rows):
+        """index_ptr of H and S for rows row_lo .. row_lo+n_rows-1 of symmetry
+        i, relative to the window (1-based); nothing else is copied."""
+        if not 0 <= i < len(self.terms):
+            raise Bs2eError(capi.BS2E_ERANGE, f"symmetry index {i} outside 0..{len(self.terms) - 1}")
+        I = C.c_int64
+        ipH = np.zeros(n_rows + 1, np.int64)
+        ipS = np.zeros(n_rows + 1, np.int64)
+        check(capi.lib().bs2e_blockset_get_rows(self.handle, i + 1, row_lo, n_rows, ptr(ipH, I), None, None,
+                                                ptr(ipS, I), None, None), self.ctx.handle)
+        return ipH, ipS
+
+    def get_rows(self, i, row_lo, n_rows):
+        """Rows row_lo .. row_lo+n_rows-1 of symmetry i (0-based position in
+        `terms`): (H_sp, S_sp) of shape (n_rows, n_config) with index_ptr
+        relative to the window (1-based)."""
+        if not 0 <= i < len(self.terms):
+            raise Bs2eError(capi.BS2E_ERANGE, f"symmetry index {i} outside 0..{len(self.terms) - 1}")
+        I = C.c_int64
+        D = C.POINTER(C.c_double)
+        ipH = np.zeros(n_rows + 1, np.int64)
+        ipS = np.zeros(n_rows + 1, np.int64)
+        lib = capi.lib()
+        check(lib.bs2e_blockset_get_rows(self.handle, i + 1, row_lo, n_rows, ptr(ipH, I), None, None, ptr(ipS, I),
+                                         None, None), self.ctx.handle)
+        nH, nS = int(ipH[-1] - 1), int(ipS[-1] - 1)
+        ixH = np.zeros(max(nH, 1), np.int64)
+        ixS = np.zeros(max(nS, 1), np.int64)
+        dH = np.zeros(max(nH, 1), np.complex128)
+        dS = np.zeros(max(nS, 1), np.complex128)
+        check(lib.bs2e_blockset_get_rows(self.handle, i + 1, row_lo, n_rows, ptr(ipH, I), ptr(ixH, I),
+                                         dH.ctypes.data_as(D), ptr(ipS, I), ptr(ixS, I), dS.ctypes.data_as(D)),
+              self.ctx.handle)
+        n = self.n_config[i]
+        return (CSR_matrix((n_rows, n), ipH, ixH[:nH], dH[:nH]), CSR_matrix((n_rows, n), ipS, ixS[:nS], dS[:nS]))
 
     def kernel_ms(self):
         """(count launch, fill launch) device ms, HIP events."""

@@ -46,6 +46,7 @@ SIGNATURES = [
     ("bs2e_rk_device_ptr", i64, [vp, P(vp), P(i64)]),
     ("bs2e_rk_pair_index", i64, [vp, i64, i64]),
     ("bs2e_rk_copy_to_host", i64, [vp, P(dbl), i64]),
+    ("bs2e_rk_copy_rows", i64, [vp, i64, i64, P(dbl)]),
     ("bs2e_rk_get_val", i64, [vp, P(i64), P(dbl), i64]),
     ("bs2e_rk_set_profiling", i64, [vp, i64]),
     ("bs2e_rk_stage_times", i64, [vp, P(dbl), i64]),
@@ -68,6 +69,7 @@ SIGNATURES = [
     ("bs2e_blockset_sizes", i64, [vp, P(i64), i64]),
     ("bs2e_blockset_fill", i64, [vp, i64, P(dbl), P(dbl)]),
     ("bs2e_blockset_get", i64, [vp, i64, P(i64), P(i64), P(dbl), P(i64), P(i64), P(dbl)]),
+    ("bs2e_blockset_get_rows", i64, [vp, i64, i64, i64, P(i64), P(i64), P(dbl), P(i64), P(i64), P(dbl)]),
     ("bs2e_blockset_times", i64, [vp, P(dbl), i64]),
     ("bs2e_blockset_release", i64, [vp]),
     ("bs2e_setup_r_k_int", i64, [vp, i64, i64, P(dbl), i64, i64, dbl, P(dbl)]),

@@ -80,3 +80,80 @@ def knots(w):
     if w.perturb_seed is not None:
         t = perturb_breakpoints(t, w.k, w.perturb_seed, w.perturb_rel)
     return t
+
+
+def solve_h_max(k, m, Z, r_max, n_b):
+    """h_max whose grid (grid_tools.f90:6-55 rule) gives exactly n_b B-splines, or None.
+    generate_grid refuses grids that overrun the reference's knot array (a
+    linear part finer than 2^-m, or a geometric ramp longer than r_max); the
+    search stays inside the h_max range it accepts."""
+    import bs2e
+
+    def nb(h):
+        try:
+            return len(bs2e.generate_grid(k, m, Z, h, r_max)) - k - 2
+        except bs2e.Bs2eError:
+            return None
+
+    lo, hi = r_max / 4000.0, r_max
+    while nb(lo) is None and lo < hi:
+        lo *= 1.25
+    while nb(hi) is None and hi > lo:
+        hi /= 1.25
+    n_lo, n_hi = nb(lo), nb(hi)
+    if n_lo is None or n_hi is None or n_lo < n_b or n_hi > n_b:
+        return None
+    for _ in range(80):
+        mid = 0.5 * (lo + hi)
+        v = nb(mid)
+        if v is None:
+            return None
+        if v == n_b:
+            return mid
+        if v > n_b:
+            lo = mid
+        else:
+            hi = mid
+    return None
+
+
+@dataclass(frozen=True)
+class ServingJob:
+    """One basis_setup request of the serving stream."""
+    k: int
+    t: np.ndarray
+    k_GL: int
+    max_k: int
+    lmax: int
+
+
+def serving_jobs(n, seed=7):
+    """The serving stream's jobs: B-spline count uniform in [40, 400], order in
+    {6, 7, 8, 9}, r_max uniform in [50, 500], lmax uniform in [2, 8],
+    k <= 2 lmax, k_GL = order + 6 (input_tools.f90:833 default), m = 3, Z = 2,
+    h_max solved for the drawn count (draws without such a grid are drawn
+    again)."""
+    import bs2e
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(100 * n):
+        if len(jobs) == n:
+            break
+        n_b = int(rng.integers(40, 401))
+        k = int(rng.choice([6, 7, 8, 9]))
+        r_max = float(rng.uniform(50.0, 500.0))
+        lmax = int(rng.integers(2, 9))
+        h = solve_h_max(k, 3, 2, r_max, n_b)
+        if h is None:
+            continue
+        jobs.append(ServingJob(k, bs2e.generate_grid(k, 3, 2, h, r_max), k + 6, 2 * lmax, lmax))
+    if len(jobs) < n:
+        raise RuntimeError(f"serving_jobs: only {len(jobs)} of {n} jobs have a grid with the drawn size")
+    return jobs
+
+
+def largest_serving_job():
+    """The largest job the serving stream can draw: order 9, 400 B-splines,
+    lmax 8 (k <= 16), GL 15 (the c4 workload's basis)."""
+    w = WORKLOADS["c4"]
+    return ServingJob(w.k, knots(w), w.k_GL, w.max_k, w.lmax)

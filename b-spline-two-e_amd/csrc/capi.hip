@@ -836,6 +836,26 @@ int64_t bs2e_rk_copy_to_host(bs2e_rk* rk, double* dst, int64_t n_elems)
     return BS2E_OK;
 }
 
+int64_t bs2e_rk_copy_rows(const bs2e_rk* rk, int64_t row_lo, int64_t n_rows, double* dst)
+{
+    if (!rk || !dst) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_copy_rows: NULL argument");
+    bs2e_ctx* ctx = rk->ctx;
+    const Plan& p = rk->plan;
+    if (row_lo < 0 || n_rows < 0 || row_lo + n_rows > p.n_pairs)
+        return fail(ctx, BS2E_ERANGE, "bs2e_rk_copy_rows: rows outside 0..n_pairs-1");
+    if (!rk->computed) return fail(ctx, BS2E_ESTATE, "bs2e_rk_copy_rows: tensor not computed");
+    const size_t row = (size_t)p.n_pairs * p.K;
+    if (rk->mirror_valid) {
+        std::memcpy(dst, rk->mirror.data() + (size_t)row_lo * row, sizeof(double) * row * (size_t)n_rows);
+        return BS2E_OK;
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipMemcpy(dst, rk->R + (size_t)row_lo * row, sizeof(double) * row * (size_t)n_rows,
+                           hipMemcpyDeviceToHost));
+    return BS2E_OK;
+}
+
 int64_t bs2e_rk_get_val(const bs2e_rk* rk, const int64_t* idx, double* values, int64_t n_values)
 {
     if (!rk || !idx || !values) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_get_val: NULL argument");

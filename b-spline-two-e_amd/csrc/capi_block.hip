@@ -564,6 +564,45 @@ int64_t bs2e_blockset_get(const bs2e_blockset* b, int64_t sym, int64_t* ip_H, in
     return set_get(b, sym - 1, ip_H, ix_H, d_H, ip_S, ix_S, d_S, "bs2e_blockset_get");
 }
 
+int64_t bs2e_blockset_get_rows(const bs2e_blockset* b, int64_t sym, int64_t row_lo, int64_t n_rows, int64_t* ip_H,
+                               int64_t* ix_H, double* d_H, int64_t* ip_S, int64_t* ix_S, double* d_S)
+{
+    if (!b || !ip_H || !ip_S) return fail(nullptr, BS2E_EINVAL, "bs2e_blockset_get_rows: NULL argument");
+    bs2e_ctx* ctx = b->rk->ctx;
+    if (sym < 1 || sym > b->n_sym)
+        return fail(ctx, BS2E_ERANGE, "bs2e_blockset_get_rows: symmetry index outside 1..n_sym");
+    const int64_t s = sym - 1, r0 = b->row0[s], n = b->row0[s + 1] - r0;
+    if (row_lo < 0 || n_rows < 0 || row_lo + n_rows > n)
+        return fail(ctx, BS2E_ERANGE, "bs2e_blockset_get_rows: rows outside the symmetry's configurations");
+    if (!b->filled) return fail(ctx, BS2E_ESTATE, "bs2e_blockset_get_rows: the blocks have not been filled");
+    int64_t aH = b->offH[s], aS = b->offS[s];   // first entry of the window in the concatenated output
+    for (int64_t i = 0; i < row_lo; ++i) {
+        aH += b->cntH[r0 + i];
+        aS += b->cntS[r0 + i];
+    }
+    ip_H[0] = 1;
+    ip_S[0] = 1;
+    for (int64_t i = 0; i < n_rows; ++i) {
+        ip_H[i + 1] = ip_H[i] + b->cntH[r0 + row_lo + i];
+        ip_S[i + 1] = ip_S[i] + b->cntS[r0 + row_lo + i];
+    }
+    if (!ix_H && !d_H && !ix_S && !d_S) return BS2E_OK;   // sizes only
+    if (!ix_H || !d_H || !ix_S || !d_S) return fail(ctx, BS2E_EINVAL, "bs2e_blockset_get_rows: NULL argument");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int64_t nH = ip_H[n_rows] - 1, nS = ip_S[n_rows] - 1;
+    if (nH > 0) {
+        HIP_TRY(ctx, hipMemcpyAsync(ix_H, b->d_ixH + aH, sizeof(int64_t) * nH, hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipMemcpyAsync(d_H, b->d_dH + 2 * aH, sizeof(double) * 2 * nH, hipMemcpyDeviceToHost, st));
+    }
+    if (nS > 0) {
+        HIP_TRY(ctx, hipMemcpyAsync(ix_S, b->d_ixS + aS, sizeof(int64_t) * nS, hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipMemcpyAsync(d_S, b->d_dS + 2 * aS, sizeof(double) * 2 * nS, hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    return BS2E_OK;
+}
+
 int64_t bs2e_blockset_times(const bs2e_blockset* b, double* ms, int64_t n)
 {
     if (!b || !ms) return fail(nullptr, BS2E_EINVAL, "bs2e_blockset_times: NULL argument");

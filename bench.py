@@ -8,17 +8,24 @@ and every work buffer already resident in HBM.  Integrals per step =
 n_pairs^2 * (max_k+1), the key count of the reference's Nd_DOK times its
 value length.
 
-    python bench.py [--steps K] [--warmup W] [--workload c2]
+    python bench.py [--steps K] [--warmup W] [--workload c5p]
 
-Prints one JSON line.  The dominant kernel (the tensor fill) is timed with
-HIP events on the stream it runs on and priced against HBM (8 bytes stored
-per integral, 8 TB/s peak).  cpu_baseline times the oracle (the repository's
-C restatement of the reference path) on the host for the same workload.
+The default workload is the largest single-GPU configuration: order 9, 600
+B-splines, r <= 1000, GL 14, k <= 24 (20.5 GB of FP64 tensor), with its
+interior breakpoints moved by a seeded (42) factor in [0.99, 1.01].
+
+Prints one JSON line.  The dominant kernel (the rank-one tensor fill) is
+timed with HIP events on the stream it runs on and priced against HBM (8
+bytes stored per integral, 8 TB/s peak).  cpu_baseline times the oracle (the
+repository's C restatement of the reference path) on the host, on the
+reference's schedule, over bounded samples of the same workload, and
+extrapolates to the whole build.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -31,120 +38,251 @@ for _p in (ROOT, os.path.join(ROOT, "b-spline-two-e_amd")):
 METRIC = "R^k integrals/sec (mixed vs FP64 GPU) at <=1e-10 rel err; p99 build latency"
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per k3_far launch, FETCH_SIZE + WRITE_SIZE from rocprofv3 --pmc,
-# each scaled by what the counter reads for a 1 GiB device copy
-# (profiles/rk_build_c2.md), per (workload, fill shape); None for pairs not
-# measured that way.
+# each scaled by what the counter reads for a device copy of known size
+# (profiles/rk_build_c2.md, profiles/rk_build_c5p.md), per (workload, fill
+# shape); absent for pairs not measured that way.
 FAR_TRAFFIC_BYTES = {("c2", 0): 739.35e6, ("c2", 1): 739.59e6, ("c2", 2): 739.99e6, ("c2", 3): 739.74e6,
                      ("c2", 4): 739.52e6, ("c2", 5): 739.51e6, ("c2", 6): 739.05e6}
 
 
-def cpu_baseline(w, t, R_gpu, nthreads, block_max_l=None, gpu_block=None):
-    """Oracle (C restatement, OpenMP over k like the reference) on the host:
-    primitives + dense tensor assembly for the same basis; with block_max_l,
-    also the L = 0 block of bench's `blocks` line (count and fill, one
-    thread: the reference builds each symmetry on one thread), compared with
-    the device's block (gpu_block = (H, S) CSR) when given."""
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
+def host_threads():
+    """Threads for the host baseline: the CPUs this process may run on,
+    capped at 16 (the GPU host's share per GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def sample_rows(n_pairs, n, seed=3):
+    """n distinct tensor rows: the first, the last and n-2 drawn ones."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return np.unique(np.concatenate([[0, n_pairs - 1], rng.choice(n_pairs, size=n - 2, replace=False)]))
+
+
+def oracle_rows(w, t, rows, nthreads):
+    """The oracle's rows `rows` of the tensor (its diagonal cells computed only
+    where they land in these rows): bit for bit its full tensor's rows."""
+    import numpy as np
+
+    from oracle import oracle as O
+    n_b = O.dims(w.k, t)[1]
+    mask = np.zeros(O.n_pairs(w.k, n_b), bool)
+    mask[rows] = True
+    off = O.slater_off_diag(w.k, t, w.k_GL, w.max_k, nthreads=nthreads)
+    dg = O.slater_diag(w.k, t, w.k_GL, w.max_k, nthreads=nthreads, pair_mask=mask)
+    return O.rk_rows(w.k, n_b, w.max_k, off, dg, rows)
+
+
+def error_stats(got, ref):
+    """Per multipole k: the largest element-wise relative error and the
+    normwise error max|got - ref| / max|ref|; over all elements the p50 /
+    p99 / max of the element-wise relative error (every R^k value is > 0)."""
+    import numpy as np
+    K = ref.shape[-1]
+    g = got.reshape(-1, K)
+    r = ref.reshape(-1, K)
+    d = np.abs(g - r)
+    rel = d / r
+    return {"per_k_max_rel": [float(x) for x in rel.max(axis=0)],
+            "per_k_normwise": [float(x) for x in d.max(axis=0) / np.abs(r).max(axis=0)],
+            "elementwise_rel": {"p50": float(np.percentile(rel, 50)), "p99": float(np.percentile(rel, 99)),
+                                "max": float(rel.max())},
+            "elements": int(rel.size)}
+
+
+def cpu_baseline_reference_schedule(w, t, nthreads, diag_target_s=4.0, dok_target_s=3.0):
+    """The whole build on the reference's schedule, timed on the host over
+    bounded samples and extrapolated:
+      * off-diagonal moments (mat_els.f90:184-228): in full, every B-spline
+        value evaluated inside the loops, OpenMP over the multipoles k;
+      * diagonal cells (mat_els.f90:230-292): one multipole (one thread's
+        unit of the reference's !$omp parallel do over k) on the leading
+        outer columns j_b_p, B-spline values evaluated inside the loops;
+        extrapolated by entry count to all sparse_6d entries, then to the K
+        multipoles over the threads (ceil(K / threads) rounds);
+      * R^k assembly (sparse_array_tools.f90:452-493): serial set_val calls
+        into a restated Nd_DOK hash map, for the products of a leading
+        fraction of the sparse_4d entries (extrapolated by set_val count) and
+        for every diagonal cell (in full)."""
     import numpy as np
 
     from oracle import oracle as O
     K = w.max_k + 1
     n_b = O.dims(w.k, t)[1]
+    npairs = O.n_pairs(w.k, n_b)
+    n_int = npairs * npairs * K
+    # off-diagonal moments, in full
     t0 = time.perf_counter()
-    off = O.slater_off_diag(w.k, t, w.k_GL, w.max_k, nthreads=nthreads)
-    dg = O.slater_diag(w.k, t, w.k_GL, w.max_k, nthreads=nthreads)
-    R = O.rk_dense(w.k, n_b, w.max_k, off, dg)
+    off = O.slater_off_diag(w.k, t, w.k_GL, w.max_k, faithful=True, nthreads=nthreads)
+    t_off = time.perf_counter() - t0
+    # diagonal cells: a calibration run, then a sample of about diag_target_s
+    nnz6 = O.count_nnz6(w.k, t)
+    k_sel = w.max_k // 2
+    jp = max(w.k, n_b // 200)
+    t0 = time.perf_counter()
+    done = O.slater_diag_sample(w.k, t, w.k_GL, k_sel, jp, True)
     dt = time.perf_counter() - t0
-    n_int = R.size
-    rel = None
-    if R_gpu is not None:   # largest relative error per multipole k (every R^k value is positive)
-        rel = [float(x) for x in (np.abs(R_gpu - R) / R).reshape(-1, K).max(axis=0)]
-    blocks = None
-    if block_max_l is not None:
-        S = O.setup_S(w.k, t, w.k_GL)
-        Hl = [O.setup_H1p(w.k, t, w.k_GL, w.Z, l, cap_order=2, cap_r0=0.8 * w.r_max, cap_eta=0.0)
-              for l in range(block_max_l + 1)]
-        cf = O.count_configs(0, 0, block_max_l, n_b, w.k, n_b, n_b, block_max_l)
-        tb = time.perf_counter()
-        blk = O.construct_block(w.k, n_b, 0, w.max_k, True, cf, R, Hl, S)
-        db = time.perf_counter() - tb
-        blocks = dict(value=blk["nnz"][0] / db, unit="H entries/s", cores=1, kind="port",
-                      sample=(f"{w.name} basis, L=0 block, l1,l2 <= {block_max_l} ({len(cf)} configurations, "
-                              f"{blk['nnz'][0]} H entries): the oracle's count and fill passes; {db:.2f} s"),
-                      seconds=db)
-        if gpu_block is not None:   # the whole block against the oracle's (oracle tensor and one-electron matrices)
-            H, Sb = gpu_block
-            pattern = bool(np.array_equal(H.index_ptr, blk["H"][0]) and np.array_equal(H.indices, blk["H"][1])
-                           and np.array_equal(Sb.index_ptr, blk["S"][0]) and np.array_equal(Sb.indices, blk["S"][1]))
-            blocks["parity_vs_oracle"] = dict(
-                pattern_bit_exact=pattern,
-                H_max_abs_err_over_max_abs=float(np.abs(H.data - blk["H"][2]).max() / np.abs(blk["H"][2]).max())
-                if pattern else None,
-                S_max_abs_err_over_max_abs=float(np.abs(Sb.data - blk["S"][2]).max() / np.abs(blk["S"][2]).max())
-                if pattern else None)
-    return dict(value=n_int / dt, unit="integrals/s", cores=int(min(nthreads, K)), kind="port",
-                sample=(f"full {w.name} build (primitives + tensor, {n_int} integrals) by the oracle: "
-                        f"OpenMP over the {K} multipoles as the reference's !$omp parallel do, "
-                        f"serial assembly; {dt:.2f} s"),
-                seconds=dt), rel, blocks
+    per_entry = dt / max(done, 1)
+    jp = min(n_b, max(jp, int(jp * diag_target_s / max(dt, 1e-3))))
+    t0 = time.perf_counter()
+    done = O.slater_diag_sample(w.k, t, w.k_GL, k_sel, jp, True)
+    t_diag_sample = time.perf_counter() - t0
+    per_entry = t_diag_sample / max(done, 1)
+    t_diag = per_entry * nnz6 * math.ceil(K / nthreads)
+    # assembly: the diagonal cells' set_val calls need only the index arrays
+    dg_idx = O.slater_diag(w.k, t, w.k_GL, w.max_k, nthreads=nthreads, pair_mask=np.zeros(npairs, bool))
+    empty = dict(nnz=0, rdk=np.zeros((0, K)), iv=np.zeros(0, np.int64), i=np.zeros(0, np.int64),
+                 ip=np.zeros(0, np.int64), j=np.zeros(0, np.int64), jp=np.zeros(0, np.int64))
+    iv = off["iv"]
+    total_products = off["nnz"] ** 2 - sum(int(np.count_nonzero(iv == v)) ** 2 for v in np.unique(iv))
+    n_lead = max(8, off["nnz"] // 1000)
+    t0 = time.perf_counter()
+    calls = O.rk_dok(w.k, n_b, w.max_k, off, empty, n_lead=n_lead)
+    dt = time.perf_counter() - t0
+    n_lead = min(off["nnz"], max(n_lead, int(n_lead * dok_target_s / max(dt, 1e-3))))
+    t0 = time.perf_counter()
+    calls = O.rk_dok(w.k, n_b, w.max_k, off, empty, n_lead=n_lead)
+    t_prod_sample = time.perf_counter() - t0
+    t_prod = t_prod_sample * total_products / max(calls, 1)
+    t0 = time.perf_counter()
+    O.rk_dok(w.k, n_b, w.max_k, off, dg_idx, n_lead=0)
+    t_dcells = time.perf_counter() - t0
+    total = t_off + t_diag + t_prod + t_dcells
+    sampled = t_off + t_diag_sample + t_prod_sample + t_dcells
+    return {"value": n_int / total, "unit": "integrals/s", "cores": nthreads, "host_cpus": os.cpu_count(),
+            "kind": "port", "extrapolated": True, "seconds_estimated": total, "seconds_sampled": sampled,
+            "sample": (f"{w.name} build on the reference's schedule by the oracle (B-spline values evaluated "
+                       f"inside the loops, OpenMP over the {K} multipoles on {nthreads} threads, serial hash-map "
+                       f"assembly): off-diagonal moments in full ({t_off:.2f} s); diagonal cells of multipole "
+                       f"{k_sel} for outer columns <= {jp} ({done} of {nnz6} entries, {t_diag_sample:.2f} s, one "
+                       f"thread) extrapolated to {t_diag:.1f} s for all entries and multipoles; the products of "
+                       f"the first {n_lead} of {off['nnz']} sparse_4d entries ({calls} of {total_products} set_val "
+                       f"calls, {t_prod_sample:.2f} s) extrapolated to {t_prod:.1f} s; the {2 * dg_idx['nnz']} "
+                       f"diagonal-cell set_val calls in full ({t_dcells:.2f} s); estimated whole build "
+                       f"{total:.1f} s"),
+            "stages_s": {"off_diagonal_moments": t_off, "diagonal_cells": t_diag, "assembly_products": t_prod,
+                         "assembly_diagonal_cells": t_dcells}}
 
 
-def solve_h_max(k, m, Z, r_max, n_b):
-    """h_max whose grid (grid_tools.f90:6-55 rule) gives exactly n_b B-splines, or None.
-    generate_grid refuses grids that overrun the reference's knot array (a
-    linear part finer than 2^-m, or a geometric ramp longer than r_max); the
-    search stays inside the h_max range it accepts."""
-    import bs2e
+def cpu_block_baseline(w, t, R, gpu_block, block_max_l=2):
+    """The oracle's count and fill of the L = 0 block (l1, l2 <= block_max_l)
+    of the basis, on one thread (the reference builds each symmetry on one
+    thread), and the device's block compared with it."""
+    import numpy as np
 
-    def nb(h):
-        try:
-            return len(bs2e.generate_grid(k, m, Z, h, r_max)) - k - 2
-        except bs2e.Bs2eError:
-            return None
+    from oracle import oracle as O
+    n_b = O.dims(w.k, t)[1]
+    S = O.setup_S(w.k, t, w.k_GL)
+    Hl = [O.setup_H1p(w.k, t, w.k_GL, w.Z, l, cap_order=2, cap_r0=0.8 * w.r_max, cap_eta=0.0)
+          for l in range(block_max_l + 1)]
+    cf = O.count_configs(0, 0, block_max_l, n_b, w.k, n_b, n_b, block_max_l)
+    tb = time.perf_counter()
+    blk = O.construct_block(w.k, n_b, 0, w.max_k, True, cf, R, Hl, S)
+    db = time.perf_counter() - tb
+    out = dict(value=blk["nnz"][0] / db, unit="H entries/s", cores=1, kind="port",
+               sample=(f"{w.name} basis, L=0 block, l1,l2 <= {block_max_l} ({len(cf)} configurations, "
+                       f"{blk['nnz'][0]} H entries): the oracle's count and fill passes; {db:.2f} s"),
+               seconds=db)
+    if gpu_block is not None:   # the whole block against the oracle's (oracle tensor and one-electron matrices)
+        H, Sb = gpu_block
+        pattern = bool(np.array_equal(H.index_ptr, blk["H"][0]) and np.array_equal(H.indices, blk["H"][1])
+                       and np.array_equal(Sb.index_ptr, blk["S"][0]) and np.array_equal(Sb.indices, blk["S"][1]))
+        out["parity_vs_oracle"] = dict(
+            pattern_bit_exact=pattern,
+            H_max_abs_err_over_max_abs=float(np.abs(H.data - blk["H"][2]).max() / np.abs(blk["H"][2]).max())
+            if pattern else None,
+            S_max_abs_err_over_max_abs=float(np.abs(Sb.data - blk["S"][2]).max() / np.abs(blk["S"][2]).max())
+            if pattern else None)
+    return out
 
-    lo, hi = r_max / 4000.0, r_max
-    while nb(lo) is None and lo < hi:
-        lo *= 1.25
-    while nb(hi) is None and hi > lo:
-        hi /= 1.25
-    n_lo, n_hi = nb(lo), nb(hi)
-    if n_lo is None or n_hi is None or n_lo < n_b or n_hi > n_b:
-        return None
-    for _ in range(80):
-        mid = 0.5 * (lo + hi)
-        v = nb(mid)
-        if v is None:
-            return None
-        if v == n_b:
-            return mid
-        if v > n_b:
-            lo = mid
-        else:
-            hi = mid
-    return None
+
+def measure_build(rk, stream, steps, warmup, latency_samples):
+    """Times `steps` builds (after `warmup`): the rate from the host clock
+    around the synchronised loop, the per-stage device timeline from the
+    library's HIP events (a second loop), the replay of a captured hipGraph of
+    the build (a third loop) and the latency of single builds."""
+    import numpy as np
+    import torch
+    n_int = rk.n_pairs * rk.n_pairs * rk.K
+    for _ in range(warmup):
+        rk.compute()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(steps):
+        rk.compute()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    rk.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rk.compute()
+    torch.cuda.synchronize()
+    wall_prof = time.perf_counter() - t0
+    st = rk.stage_times()        # ms since start: windows, far end, diag end, near end, total
+    rk.set_profiling(False)
+    rk.set_graph(True)
+    rk.compute()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rk.compute()
+    torch.cuda.synchronize()
+    wall_graph = time.perf_counter() - t0
+    rk.set_graph(False)
+    lat = []
+    for _ in range(latency_samples):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        rk.compute()
+        rk.synchronize()
+        lat.append((time.perf_counter() - a) * 1e3)
+    lat = np.array(lat) if lat else np.zeros(1)
+    return {"value": n_int * steps / wall, "ms_per_step": wall * 1e3 / steps,
+            "event_ms_per_step": ev0.elapsed_time(ev1) / steps, "profiled_ms_per_step": wall_prof * 1e3 / steps,
+            "graph_ms_per_step": wall_graph * 1e3 / steps,
+            "timeline_ms": {"moments_and_windows_done": float(st[0]), "rank_one_fill_done": float(st[1]),
+                            "diag_cells_done_side_stream": float(st[2]),
+                            "near_ranges_done_side_stream": float(st[3]), "build_done": float(st[4])},
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                           "samples": int(latency_samples)},
+            "stages": st}
+
+
+def store_ceiling(stream, n_elems):
+    """GB/s of a store-only fill of n_elems doubles on the build's stream."""
+    import torch
+    with torch.cuda.stream(stream):
+        buf = torch.empty(n_elems, dtype=torch.float64, device="cuda")
+        buf.zero_()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(5):
+            buf.zero_()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    out = 8.0 * n_elems * 5 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del buf
+    torch.cuda.empty_cache()
+    return out
 
 
 def serving_jobs(n, seed):
-    """Serving stream jobs: B-spline count uniform in [40,400], order in {6,7,8,9},
-    r_max uniform in [50,500], lmax uniform in [2,8], k <= 2 lmax, k_GL = order+6."""
-    import numpy as np
-
-    import bs2e
-    rng = np.random.default_rng(seed)
-    jobs = []
-    for _ in range(100 * n):
-        if len(jobs) == n:
-            break
-        n_b = int(rng.integers(40, 401))
-        k = int(rng.choice([6, 7, 8, 9]))
-        r_max = float(rng.uniform(50.0, 500.0))
-        lmax = int(rng.integers(2, 9))
-        h = solve_h_max(k, 3, 2, r_max, n_b)
-        if h is None:
-            continue
-        jobs.append((k, bs2e.generate_grid(k, 3, 2, h, r_max), k + 6, 2 * lmax))
-    if len(jobs) < n:
-        raise RuntimeError(f"serving_jobs: only {len(jobs)} of {n} jobs have a grid with the drawn size")
-    return jobs
+    """The serving stream's jobs as (order, knots, k_GL, max_k) tuples (bs2e.workloads.serving_jobs)."""
+    from bs2e.workloads import serving_jobs as draw
+    return [(j.k, j.t, j.k_GL, j.max_k) for j in draw(n, seed)]
 
 
 def run_stream(srv, jobs, arrivals):
@@ -296,8 +434,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="c5p")
+    ap.add_argument("--secondary", default="c2",
+                    help="a second, smaller workload timed the same way (its blocks and dipoles too); none skips it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sample-rows", type=int, default=48,
+                    help="tensor rows of the headline workload compared with the oracle's rows")
     ap.add_argument("--latency-samples", type=int, default=50,
                     help="synchronised single builds timed for p50/p99 latency")
     ap.add_argument("--blocks", type=int, default=1, help="also time the L=0 symmetry block (0 skips it)")
@@ -319,86 +461,28 @@ def main():
     import torch
 
     import bs2e
-    from bs2e.workloads import WORKLOADS
+    from bs2e.workloads import WORKLOADS, knots
 
     w = WORKLOADS[args.workload]
+    nthreads = host_threads()
     torch.cuda.set_device(0)
     stream = torch.cuda.Stream(device=0)     # default priority; the build and its events share it
     ctx = bs2e.Context(0)
     ctx.set_stream(stream.cuda_stream)
-    t = bs2e.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+    t = knots(w)
     rk = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx, compute=False)
     n_int = rk.n_pairs * rk.n_pairs * rk.K
-    print(f"# bench {w.name}: n_b={rk.n_b} n_pairs={rk.n_pairs} K={rk.K} integrals/step={n_int}", file=sys.stderr,
-          flush=True)
-
+    log(f"bench {w.name}: n_b={rk.n_b} n_pairs={rk.n_pairs} K={rk.K} integrals/step={n_int}")
     rk.autotune()                 # fill shape and multipole chunk timed on this basis
     tuning = rk.tuning()
-    for _ in range(args.warmup):
-        rk.compute()
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        rk.compute()
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
-    value = n_int * args.steps / wall
-
-    # The same K steps again with the library's HIP events recorded on both
-    # build streams around every kernel stage: per-stage device times averaged
-    # over the steps (and the profiled rate, to show the events cost nothing).
-    rk.set_profiling(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        rk.compute()
-    torch.cuda.synchronize()
-    wall_prof = time.perf_counter() - t0
-    st = rk.stage_times()        # ms since start: windows, far end, diag end, near end, total
-    rk.set_profiling(False)
-    # the same K steps replayed from a captured hipGraph of the build
-    rk.set_graph(True)
-    rk.compute()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        rk.compute()
-    torch.cuda.synchronize()
-    wall_graph = time.perf_counter() - t0
-    rk.set_graph(False)
+    m = measure_build(rk, stream, args.steps, args.warmup, args.latency_samples)
+    log(f"{w.name}: {m['ms_per_step']:.4f} ms/build, {m['value']:.3e} integrals/s")
+    st = m.pop("stages")
     t_far = float(st[1] - st[0]) * 1e-3
     far_elems = n_int - rk.near_elems
     traffic_alg = 8.0 * far_elems            # algorithmic bytes of one k3_far launch
     achieved = traffic_alg / t_far / 1e9
-    # build latency: one build submitted to an idle device, host clock to completion
-    lat = []
-    for _ in range(args.latency_samples):
-        torch.cuda.synchronize()
-        a = time.perf_counter()
-        rk.compute()
-        rk.synchronize()
-        lat.append((time.perf_counter() - a) * 1e3)
-    lat = np.array(lat) if lat else np.zeros(1)
-    # the practical store ceiling on this box: a store-only fill of a buffer of
-    # the tensor's size on the same stream
-    with torch.cuda.stream(stream):
-        buf = torch.empty(n_int, dtype=torch.float64, device="cuda")
-        buf.zero_()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(5):
-            buf.zero_()
-        e1.record(stream)
-    torch.cuda.synchronize()
-    store_ceiling = 8.0 * n_int * 5 / (e0.elapsed_time(e1) * 1e-3) / 1e9
-    del buf
+    ceiling = store_ceiling(stream, n_int)
     if args.copy_calibration_mib > 0:
         n = args.copy_calibration_mib * (1 << 20) // 8
         with torch.cuda.stream(stream):
@@ -408,16 +492,14 @@ def main():
                 dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    R_gpu = rk.to_host() if not args.no_cpu_baseline else None
-
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": m["value"],
         "unit": "integrals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -426,44 +508,71 @@ def main():
         "config": {"workload": f"{w.name}: {w.description}", "n_b": rk.n_b, "n_pairs": rk.n_pairs, "K": rk.K,
                    "k_GL": w.k_GL, "integrals_per_step": n_int, "precision": "plain FP64",
                    "parallelism": "single GPU"},
-        "timeline_ms": {"moments_and_windows_done": float(st[0]), "rank_one_fill_done": float(st[1]),
-                        "diag_cells_done_side_stream": float(st[2]), "near_ranges_done_side_stream": float(st[3]),
-                        "build_done": float(st[4])},
+        "timeline_ms": m["timeline_ms"],
         "tensor_stage_GBs": 8.0 * n_int / (float(st[4] - st[0]) * 1e-3) / 1e9,
-        "event_ms_per_step": ev0.elapsed_time(ev1) / args.steps,
-        "profiled_ms_per_step": wall_prof * 1e3 / args.steps,
-        "graph_ms_per_step": wall_graph * 1e3 / args.steps,
+        "event_ms_per_step": m["event_ms_per_step"],
+        "profiled_ms_per_step": m["profiled_ms_per_step"],
+        "graph_ms_per_step": m["graph_ms_per_step"],
         "tuning": {"far_fill_shape": tuning[0], "diag_multipole_chunk": tuning[1]},
-        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                       "samples": int(args.latency_samples)},
+        "latency_ms": m["latency_ms"],
         "roofline": {"bound": "hbm", "kernel": "k3_far (rank-one tensor fill)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": FAR_TRAFFIC_BYTES.get((w.name, int(tuning[0]))),
                      "algorithmic_bytes": traffic_alg, "elements": far_elems,
-                     "avg_launch_ms": t_far * 1e3, "store_ceiling_GBs": store_ceiling,
-                     "frac_of_store_ceiling": achieved / store_ceiling},
+                     "avg_launch_ms": t_far * 1e3, "store_ceiling_GBs": ceiling,
+                     "frac_of_store_ceiling": achieved / ceiling},
     }
-    if args.blocks:
-        gpu_block = [] if not args.no_cpu_baseline else None
-        out["blocks"] = blocks_benchmark(rk, w, t, keep=gpu_block)
-    if args.dipoles:
-        out["dipoles"] = dipoles_benchmark(rk.ctx, w, t)
+    if not args.no_cpu_baseline:
+        rows = sample_rows(rk.n_pairs, args.sample_rows)
+        got = rk.rows(rows)
+        log(f"{w.name}: {len(rows)} sampled rows against the oracle")
+        ref = oracle_rows(w, t, rows, nthreads)
+        out["error_vs_oracle"] = dict(error_stats(got, ref), rows=len(rows),
+                                      sample=f"{len(rows)} tensor rows (first, last, {len(rows) - 2} drawn, seed 3)")
+        del got, ref
+    rk.release()
+
+    gpu_block = None
+    if args.secondary != "none":
+        w2 = WORKLOADS[args.secondary]
+        t2 = knots(w2)
+        rk2 = bs2e.RkTensor.build(w2.k, t2, w2.k_GL, w2.max_k, ctx=ctx, compute=False)
+        rk2.autotune()
+        m2 = measure_build(rk2, stream, args.steps, args.warmup, 0)
+        m2.pop("stages")
+        m2.pop("latency_ms")
+        n2 = rk2.n_pairs * rk2.n_pairs * rk2.K
+        out["secondary"] = dict(workload=f"{w2.name}: {w2.description}", integrals_per_step=n2, **m2)
+        log(f"{w2.name}: {m2['ms_per_step']:.4f} ms/build")
+        R2 = rk2.to_host() if not args.no_cpu_baseline else None
+        if args.blocks:
+            gpu_block = [] if not args.no_cpu_baseline else None
+            out["blocks"] = blocks_benchmark(rk2, w2, t2, keep=gpu_block)
+        if args.dipoles:
+            out["dipoles"] = dipoles_benchmark(ctx, w2, t2)
+        rk2.release()
+        if R2 is not None:
+            from oracle import oracle as O
+            n_b2 = O.dims(w2.k, t2)[1]
+            Rref = O.rk_dense(w2.k, n_b2, w2.max_k, O.slater_off_diag(w2.k, t2, w2.k_GL, w2.max_k, nthreads=nthreads),
+                              O.slater_diag(w2.k, t2, w2.k_GL, w2.max_k, nthreads=nthreads))
+            out["secondary"]["error_vs_oracle"] = dict(error_stats(R2, Rref), rows="all")
+            if args.blocks:
+                log(f"{w2.name}: L=0 block on the oracle")
+                out["blocks"]["cpu_baseline"] = cpu_block_baseline(w2, t2, Rref, tuple(gpu_block))
+                out["blocks"]["speedup_vs_cpu"] = out["blocks"]["H_entries_per_s"] / out["blocks"]["cpu_baseline"]["value"]
+            del Rref, R2
     if args.blockset:
-        out["blockset"] = blockset_benchmark(rk.ctx)
+        log("c3 block set")
+        out["blockset"] = blockset_benchmark(ctx)
     if args.serve_jobs > 0:
+        log("serving stream")
         out["serving"] = serving_benchmark(args.serve_jobs)
     if not args.no_cpu_baseline:
-        nthreads = min(16, os.cpu_count() or 1)
-        cb, rel, cblocks = cpu_baseline(w, t, R_gpu, nthreads, block_max_l=2 if args.blocks else None,
-                                        gpu_block=tuple(gpu_block) if args.blocks else None)
+        log(f"{w.name}: host baseline on the reference's schedule ({nthreads} threads)")
+        cb = cpu_baseline_reference_schedule(w, t, nthreads)
         out["cpu_baseline"] = cb
-        if cblocks is not None:
-            out["blocks"]["cpu_baseline"] = cblocks
-            out["blocks"]["speedup_vs_cpu"] = out["blocks"]["H_entries_per_s"] / cblocks["value"]
-        out["max_rel_err_vs_oracle"] = max(rel)
-        out["max_rel_err_vs_oracle_per_k"] = rel
-        out["speedup_vs_cpu"] = value / cb["value"]
-    rk.release()
+        out["speedup_vs_cpu"] = out["value"] / cb["value"]
     print(json.dumps(out), flush=True)
 
 

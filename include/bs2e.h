@@ -121,6 +121,10 @@ int64_t bs2e_rk_pair_index(const bs2e_rk *rk, int64_t a, int64_t c);
 /* Copies the tensor to the host: into dst (n_elems doubles) when dst != NULL,
  * and into the handle's host mirror used by bs2e_rk_get_val. */
 int64_t bs2e_rk_copy_to_host(bs2e_rk *rk, double *dst, int64_t n_elems);
+/* Copies tensor rows row_lo .. row_lo+n_rows-1 (0-based pair indices; each
+ * row n_pairs * K doubles) to dst.  (No reference counterpart: a bulk
+ * Nd_DOK%get_val over every key of some electron-1 pairs.) */
+int64_t bs2e_rk_copy_rows(const bs2e_rk *rk, int64_t row_lo, int64_t n_rows, double *dst);
 /* sparse_array_tools.f90:536-555 Nd_DOK%get_val(indices(4), values(0:max_k)):
  * values of key [a,b,c,d].  Reads the host mirror (lock-free); without a
  * mirror it copies the K values from the device.  BS2E_ERANGE for a key
@@ -230,6 +234,15 @@ int64_t bs2e_blockset_fill(bs2e_blockset *set, int64_t max_l_1p, const double *H
  * bs2e_block_fill writes them (sizes from bs2e_blockset_sizes). */
 int64_t bs2e_blockset_get(const bs2e_blockset *set, int64_t sym, int64_t *ip_H, int64_t *ix_H, double *d_H,
                           int64_t *ip_S, int64_t *ix_S, double *d_S);
+/* Rows row_lo .. row_lo+n_rows-1 (0-based, within symmetry sym) of H_sp and
+ * S_sp: ip_H/ip_S (n_rows+1 entries) are 1-based offsets relative to the
+ * window (ip[0] = 1), ix/data hold ip[n_rows]-1 entries each.  With all four
+ * of ix_H, d_H, ix_S, d_S NULL only the offsets are written (to size the
+ * buffers).  (No reference counterpart: lets a caller read a block larger
+ * than its host memory window by window.) */
+int64_t bs2e_blockset_get_rows(const bs2e_blockset *set, int64_t sym, int64_t row_lo, int64_t n_rows,
+                               int64_t *ip_H, int64_t *ix_H, double *d_H, int64_t *ip_S, int64_t *ix_S,
+                               double *d_S);
 /* Device time (ms) of the count launch (ms[0]) and the last fill (ms[1]). */
 int64_t bs2e_blockset_times(const bs2e_blockset *set, double *ms, int64_t n);
 int64_t bs2e_blockset_release(bs2e_blockset *set);

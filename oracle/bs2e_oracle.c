@@ -239,19 +239,28 @@ i64 or_count_nnz_6d(i64 k, i64 n_knots)
 {
     i64 n, n_b, n_iv, nnz = 0;
     or_spline_dims(k, n_knots, &n, &n_b, &n_iv);
+    /* The reference's loops over j_p, j, i_p, i (all n_b) and iv (all n_iv),
+     * visiting only the values its support tests can accept: i within the band
+     * of i_p, and iv in the common support window of the four splines (spline
+     * s = b+1 is supported on iv in [s-k+1, s]).  The same count. */
     for (i64 jp = 1; jp <= n_b; ++jp)
-        for (i64 j = 1; j <= n_b; ++j) {
-            if (llabs(j - jp) >= k) continue;
+        for (i64 j = (jp - k + 1 < 1 ? 1 : jp - k + 1); j <= (jp + k - 1 > n_b ? n_b : jp + k - 1); ++j)
             for (i64 ip = 1; ip <= n_b; ++ip)
-                for (i64 i = 1; i <= n_b; ++i) {
-                    if (llabs(i - ip) >= k) continue;
-                    for (i64 iv = 1; iv <= n_iv; ++iv) {
+                for (i64 i = (ip - k + 1 < 1 ? 1 : ip - k + 1); i <= (ip + k - 1 > n_b ? n_b : ip + k - 1); ++i) {
+                    i64 hi_s = i > ip ? i : ip, lo_s = i < ip ? i : ip;
+                    if (j > hi_s) hi_s = j;
+                    if (jp > hi_s) hi_s = jp;
+                    if (j < lo_s) lo_s = j;
+                    if (jp < lo_s) lo_s = jp;
+                    i64 v0 = hi_s + 2 - k, v1 = lo_s + 1;
+                    if (v0 < 1) v0 = 1;
+                    if (v1 > n_iv) v1 = n_iv;
+                    for (i64 iv = v0; iv <= v1; ++iv) {
                         int si = supp(k, iv, i + 1) && supp(k, iv, ip + 1);
                         int sj = supp(k, iv, j + 1) && supp(k, iv, jp + 1);
                         if (si && sj) ++nnz;
                     }
                 }
-        }
     return nnz;
 }
 
@@ -418,6 +427,13 @@ static void slater_diag_impl(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i6
     }
     i64 done = 0;
     double sinksum = 0.0;
+    /* pair_off[c]: 0-based index of the first banded pair of column c */
+    i64 *pair_off = (i64 *)malloc(sizeof(i64) * (size_t)(n_b + 2));
+    pair_off[1] = 0;
+    for (i64 c = 1; c <= n_b; ++c) {
+        i64 lo = c - k_s + 1 < 1 ? 1 : c - k_s + 1, hi = c + k_s - 1 > n_b ? n_b : c + k_s - 1;
+        pair_off[c + 1] = pair_off[c] + (hi - lo + 1);
+    }
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads((int)nthreads);
 #pragma omp parallel for schedule(static) reduction(+ : done, sinksum)
@@ -465,7 +481,8 @@ static void slater_diag_impl(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i6
                     r_hi = vJ1;
                 }
                 for (i64 ip = ip_lo; ip <= ip_hi; ++ip)
-                    for (i64 i = 1; i <= n_b; ++i) {
+                    for (i64 i = (faithful ? 1 : (ip - k_s + 1 < 1 ? 1 : ip - k_s + 1));
+                         i <= (faithful ? n_b : (ip + k_s - 1 > n_b ? n_b : ip + k_s - 1)); ++i) {
                         if (llabs(i - ip) >= k_s) continue;       /* :273 */
                         for (i64 i_r = r_lo; i_r <= r_hi; ++i_r) {
                             int si = supp(k_s, i_r, i + 1) && supp(k_s, i_r, ip + 1);
@@ -479,8 +496,8 @@ static void slater_diag_impl(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i6
                                 j6[ptr - 1] = j;
                                 jp6[ptr - 1] = jp;
                             }
-                            if (pair_mask && !pair_mask[or_pair_index(k_s, n_b, i, ip)] &&
-                                !pair_mask[or_pair_index(k_s, n_b, j, jp)])
+                            if (pair_mask && !pair_mask[pair_off[ip] + i - (ip - k_s + 1 < 1 ? 1 : ip - k_s + 1)] &&
+                                !pair_mask[pair_off[jp] + j - (jp - k_s + 1 < 1 ? 1 : jp - k_s + 1)])
                                 continue;
                             ++done;
                             double lim1 = bp[i_r - 1], lim2 = bp[i_r];
@@ -530,6 +547,7 @@ static void slater_diag_impl(i64 k_s, i64 n_knots, const double *t, i64 k_gl, i6
         free(pin);
         free(pout);
     }
+    free(pair_off);
     free(bo);
     free(bi);
     free(x);

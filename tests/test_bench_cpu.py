@@ -32,5 +32,8 @@ def test_serving_jobs_have_the_drawn_sizes():
         n_b = len(t) - k - 2
         assert 40 <= n_b <= 400
         assert t[0] == 0.0 and np.all(np.diff(t) >= 0.0)
-    h = bench.solve_h_max(8, 3, 2, 100.0, 150)
+    from bs2e.workloads import largest_serving_job, solve_h_max
+    h = solve_h_max(8, 3, 2, 100.0, 150)
     assert h is not None and len(bs2e.generate_grid(8, 3, 2, h, 100.0)) - 8 - 2 == 150
+    big = largest_serving_job()
+    assert (big.k, len(big.t) - big.k - 2, big.k_GL, big.max_k) == (9, 400, 15, 16)

@@ -301,3 +301,123 @@ def test_blockset_uneven_channels_t0(oracle, ctx):
         ref = oracle.construct_block(w.k, n_b, term.l, 2 * max_l, True, term.configs, Rref, Hl, S)
         compare(H_sp, S_sp, ref)
     rk.release()
+
+
+def _dense(csr, n):
+    import scipy.sparse as sp
+    return sp.csr_matrix((csr.data, csr.indices - 1, csr.index_ptr - 1), shape=(n, n)).toarray()
+
+
+@pytest.mark.parametrize("L,pi", [(0, False), (1, True), (2, False)])
+def test_eigenvalues_device_blocks_match_oracle_blocks(ctx, oracle, L, pi):
+    """He (Z = 2) on the c1 basis (order 7, 60 B-splines, r <= 50, GL 10,
+    l1, l2 <= 2, k <= 4, no absorbing potential): the lowest six eigenvalues
+    of H c = E S c for the block built entirely on the device (device tensor,
+    device one-electron matrices, device channel sum) and for the block built
+    entirely by the oracle (its tensor, its one-electron matrices, its
+    construct_block), by the same dense solver on the CPU, agree within
+    1e-9 Ha.  L = 0 holds 1s^2 1S, L = 1 odd 1s2p 1P, L = 2 even 1s3d 1D
+    (the reference's diag solves these blocks, diagonalization.f90:86-118)."""
+    import scipy.linalg as sl
+    w = WORKLOADS["c1"]
+    t = bs2e.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+    spl = bs2e.b_spline(w.k, t)
+    n_b = spl.n_b
+    term = bs2e.sym(l=L, pi=pi)
+    term.configs = bs2e.count_configs(term, 2, n_b, w.k, n_b, n_b, 2)
+    n = term.n_config
+    print(f"eigenvalue gate c1 L={L} n_config={n}", flush=True)
+    rk = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx)
+    S_d = bs2e.setup_S(spl, w.k_GL, ctx=ctx)
+    H_d = bs2e.setup_H_one_particle(w.Z, (2, 0.8 * w.r_max, 0.0), range(3), spl, w.k_GL, ctx=ctx)
+    H_sp, S_sp = bs2e.construct_block_tensor(list(H_d), S_d, spl, term, w.max_k, rk)
+    rk.release()
+    Rref = oracle.rk_dense(w.k, n_b, w.max_k, oracle.slater_off_diag(w.k, t, w.k_GL, w.max_k),
+                           oracle.slater_diag(w.k, t, w.k_GL, w.max_k))
+    S_o = oracle.setup_S(w.k, t, w.k_GL)
+    H_o = [oracle.setup_H1p(w.k, t, w.k_GL, w.Z, l, cap_order=2, cap_r0=0.8 * w.r_max, cap_eta=0.0)
+           for l in range(3)]
+    ref = oracle.construct_block(w.k, n_b, L, w.max_k, True, term.configs, Rref, H_o, S_o)
+    E = []
+    for Hc, Sc in ((H_sp, S_sp), (bs2e.CSR_matrix((n, n), *ref["H"]), bs2e.CSR_matrix((n, n), *ref["S"]))):
+        Hm, Sm = _dense(Hc, n), _dense(Sc, n)
+        assert np.abs(Hm.imag).max() == 0 and np.abs(Sm.imag).max() == 0
+        E.append(sl.eigh(Hm.real, Sm.real, subset_by_index=[0, 5], eigvals_only=True))
+    print(f"  device {E[0]}\n  oracle {E[1]}  max |dE| {np.abs(E[0] - E[1]).max():.3e}", flush=True)
+    assert np.abs(E[0] - E[1]).max() <= 1e-9
+    lowest = {0: -2.9028, 1: -2.1238, 2: -2.0556}[L]   # spd-limit He levels, Ha
+    assert abs(E[0][0] - lowest) < 2e-3
+
+
+def test_c3_blockset_sampled_rows(ctx, oracle):
+    """The c3 basis (order 8, 300 B-splines, r <= 300, GL 12, k <= 16) with
+    every symmetry L = 0..4 (natural parity) of l1, l2 <= 4 in one count and
+    one fill launch (1.86 M configuration rows, 3.5e9 H entries, every
+    symmetry's channels of uneven size): per symmetry, windows of rows at the
+    start, the middle and the end against the oracle's rows of the same block
+    (its own tensor rows, the same one-electron matrices): patterns and S bit
+    for bit, H within 1e-12 of the window's largest entry; the square part of
+    each window is symmetric (full = true), and the per-row counts sum to the
+    sizes the set reports."""
+    w = WORKLOADS["c3"]
+    t = bs2e.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+    n_b = oracle.dims(w.k, t)[1]
+    max_l, W = 4, 24
+    rk = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx)
+    S = oracle.setup_S(w.k, t, w.k_GL)
+    Hl = [oracle.setup_H1p(w.k, t, w.k_GL, w.Z, l, cap_order=2, cap_r0=0.8 * w.r_max, cap_eta=0.0)
+          for l in range(max_l + 1)]
+    terms = []
+    for L in range(5):
+        term = bs2e.sym(l=L, pi=bool(L % 2))
+        term.configs = bs2e.count_configs(term, max_l, n_b, w.k, n_b, n_b, max_l)
+        terms.append(term)
+    print("c3 blockset rows", [t_.n_config for t_ in terms], flush=True)
+    bset = bs2e.BlockSet(rk, terms, True)
+    bset.fill(Hl, S)
+    windows = []
+    for i, term in enumerate(terms):
+        n = term.n_config
+        for lo in (0, n // 2 - W // 2, n - W):
+            windows.append((i, lo))
+    got = {}
+    for i, lo in windows:
+        got[(i, lo)] = bset.get_rows(i, lo, W)
+        H_sp, S_sp = got[(i, lo)]
+        assert H_sp.index_ptr[0] == 1 and H_sp.nnz == H_sp.index_ptr[-1] - 1
+    # every row's counts: the windows' offsets agree with the set's sizes
+    for i, term in enumerate(terms):
+        ipH, ipS = bset.row_offsets(i, 0, term.n_config)
+        assert (ipH[-1] - 1, ipS[-1] - 1) == tuple(int(x) for x in bset.sizes[i, :2])
+        lo = term.n_config // 2 - W // 2
+        a, b = bset.row_offsets(i, lo, W)
+        assert np.array_equal(a - 1, ipH[lo:lo + W + 1] - ipH[lo]) and np.array_equal(b - 1, ipS[lo:lo + W + 1] - ipS[lo])
+    bset.release()
+    rk.release()
+    # the oracle's rows: tensor rows pair(n_a, .) of every window's configurations
+    need = np.unique(np.concatenate([oracle.block_tensor_rows(w.k, n_b, terms[i].configs, lo, lo + W)
+                                     for i, lo in windows]))
+    npairs = oracle.n_pairs(w.k, n_b)
+    mask = np.zeros(npairs, bool)
+    mask[need] = True
+    Rr = oracle.rk_rows(w.k, n_b, w.max_k, oracle.slater_off_diag(w.k, t, w.k_GL, w.max_k),
+                        oracle.slater_diag(w.k, t, w.k_GL, w.max_k, pair_mask=mask), need)
+    rmap = np.full(npairs, -1, np.int64)
+    rmap[need] = np.arange(len(need))
+    worst = 0.0
+    for i, lo in windows:
+        term = terms[i]
+        ref = oracle.construct_block_rows(w.k, n_b, term.l, w.max_k, True, term.configs, Rr, rmap, lo, lo + W, Hl, S)
+        H_sp, S_sp = got[(i, lo)]
+        compare(H_sp, S_sp, ref)
+        dH = ref["H"][2]
+        worst = max(worst, float(np.abs(H_sp.data - dH).max() / np.abs(dH).max()))
+        # the window's square part (columns lo+1..lo+W) is symmetric
+        Hd = np.zeros((W, W), np.complex128)
+        for r in range(W):
+            a, b = H_sp.index_ptr[r] - 1, H_sp.index_ptr[r + 1] - 1
+            cols, vals = H_sp.indices[a:b], H_sp.data[a:b]
+            sel = (cols > lo) & (cols <= lo + W)
+            Hd[r, cols[sel] - lo - 1] = vals[sel]
+        assert np.abs(Hd - Hd.T).max() <= TOL * np.abs(dH).max()
+    print(f"  {len(windows)} windows of {W} rows: H max err / scale {worst:.3e}", flush=True)

@@ -152,3 +152,55 @@ def test_autotune_keeps_values(ctx, name):
     assert np.array_equal(again.to_host(), ref)
     rk.release()
     again.release()
+
+
+def test_service_serving_stream_jobs_equal_alone(ctx, oracle):
+    """The serving stream's own job distribution (seed 7: B-spline counts in
+    [40, 400], orders 6-9, r_max in [50, 500], k <= 2 lmax <= 16, GL order + 6)
+    plus its largest possible job (order 9, 400 B-splines, k <= 16, 6.2 GB of
+    tensor), all submitted at once to a service with four slots: each job's
+    tensor equals the same basis built alone, bit for bit; and the largest
+    job's tensor, contracted with the hydrogenic Z = 2 orbitals, gives the
+    textbook Slater integrals F^0(1s,1s) = 5Z/8, G^1(1s,2p) = 112Z/2187 and
+    F^2(2p,2p) = 45Z/512 to 1e-9."""
+    import scipy.linalg as sl
+    from bs2e.workloads import largest_serving_job, serving_jobs
+    js = [(j.k, j.t, j.k_GL, j.max_k) for j in serving_jobs(9, 7)]
+    big = largest_serving_job()
+    js.insert(3, (big.k, big.t, big.k_GL, big.max_k))
+    print("serving jobs (order, n_b, k_GL, max_k):", [(k, len(t) - k - 2, g, m) for k, t, g, m in js], flush=True)
+    alone = []
+    for k, t, kgl, mk in js:
+        r = bs2e.RkTensor.build(k, t, kgl, mk, ctx=ctx)
+        alone.append(r.to_host())
+        r.release()
+    srv = bs2e.Server(64 << 30, slots=4)
+    tickets = [srv.submit(*j) for j in js]
+    for i, tk in enumerate(tickets):
+        srv.wait(tk)
+        assert np.array_equal(srv.tensor(tk).to_host(), alone[i]), i
+        srv.retire(tk)
+    assert srv.memory()[1] == 0
+    srv.close()
+    # Slater integrals of the largest job's tensor
+    k, t, kgl, mk = js[3]
+    R = alone[3]
+    n_b = oracle.dims(k, t)[1]
+    pairs = oracle.pair_table(k, n_b)
+    n_pairs, K = R.shape[0], R.shape[2]
+    S = oracle.setup_S(k, t, kgl).real
+    E0, V0 = sl.eigh(oracle.setup_H1p(k, t, kgl, 2, 0, cap_eta=0.0).real, S)
+    E1, V1 = sl.eigh(oracle.setup_H1p(k, t, kgl, 2, 1, cap_eta=0.0).real, S)
+    assert abs(E0[0] + 2.0) < 1e-9 and abs(E1[0] + 0.5) < 1e-9   # 1s, 2p of Z = 2: -Z^2 / (2 n^2)
+    s1, p2 = V0[:, 0], V1[:, 0]
+    flat = R.reshape(n_pairs, n_pairs * K)
+
+    def slater(kk, a, b, c, d):
+        A = a[pairs[:, 0] - 1] * c[pairs[:, 1] - 1]
+        B = b[pairs[:, 0] - 1] * d[pairs[:, 1] - 1]
+        return float((A @ flat).reshape(n_pairs, K)[:, kk] @ B)
+
+    Z = 2
+    for got, exact in ((slater(0, s1, s1, s1, s1), 5 * Z / 8), (slater(1, s1, p2, p2, s1), 112 * Z / 2187),
+                       (slater(2, p2, p2, p2, p2), 45 * Z / 512)):
+        assert abs(got - exact) < 1e-9 * exact, (got, exact)
