@@ -183,13 +183,15 @@ class RkTensor:
         self.max_k = self.K - 1
 
     @classmethod
-    def build(cls, k, t, k_GL, max_k, ctx: Context | None = None, compute=True):
-        """Plans + allocates, then (optionally) enqueues the full device build."""
+    def build(cls, k, t, k_GL, max_k, ctx: Context | None = None, compute=True, precision="fp64"):
+        """Plans + allocates, then (optionally) enqueues the full device build.
+        precision: "fp64" (plain) or "mixed" (the off-diagonal moments on the
+        FP32 matrix cores from exact integer slices, BS2E_PREC_MIXED)."""
         ctx = ctx or default_context()
         t = np.ascontiguousarray(t, dtype=np.float64)
         h = C.c_void_p()
-        check(capi.lib().bs2e_rk_create(ctx.handle, k, len(t), ptr(t), k_GL, max_k, capi.PREC_FP64, C.byref(h)),
-              ctx.handle)
+        prec = {"fp64": capi.PREC_FP64, "mixed": capi.PREC_MIXED}[precision]
+        check(capi.lib().bs2e_rk_create(ctx.handle, k, len(t), ptr(t), k_GL, max_k, prec, C.byref(h)), ctx.handle)
         obj = cls(h, ctx)
         if compute:
             obj.compute()

@@ -26,6 +26,7 @@ BS2E_ENOMEM = -3
 BS2E_ERANGE = -4
 BS2E_ESTATE = -5
 PREC_FP64 = 0
+PREC_MIXED = 1
 
 # (name, restype, argtypes) of every symbol include/bs2e.h declares
 SIGNATURES = [

@@ -231,6 +231,10 @@ hipError_t launch_diag_cells(const Plan& p, const DevTables& d, hipStream_t s, b
 hipError_t launch_symmetrize(const Plan& p, const DevTables& d, hipStream_t s);                    // D -> Dsym
 size_t near_lds_bytes(int64_t ks, int64_t K);
 constexpr int kNearMaxSeg = 64;   // column segments of one near row (<= 2k - 1 <= 39 for k <= 20)
+// mixed precision: Pl, Ql on the FP32 matrix cores from exact integer slices
+hipError_t launch_outer_moments_mixed(const Plan& p, const DevTables& d, hipStream_t s);
+bool mixed_supported(int64_t ks, int64_t kgl, int64_t K);   // order <= 16, k_GL <= 16
+size_t mixed_lds_bytes(int64_t ks, int64_t K);
 hipError_t launch_pair_windows(const Plan& p, const DevTables& d, hipStream_t s);            // Pw/Qw + sums
 hipError_t launch_tensor_far(const Plan& p, const DevTables& d, double* R, hipStream_t s);   // rank-one part
 int far_variant_count();

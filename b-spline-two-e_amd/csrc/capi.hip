@@ -327,7 +327,7 @@ static int64_t alloc_tables(bs2e_ctx* ctx, bs2e_rk* rk, bool with_tensor)
     return BS2E_OK;
 }
 
-static int64_t check_plan_limits(bs2e_ctx* ctx, const Plan& p, bool needs_primitives)
+int64_t bs2e::check_plan_limits(bs2e_ctx* ctx, const Plan& p, bool needs_primitives)
 {
     if (needs_primitives &&
         (p.k_s > 16 || inner_kc(p.k_s, p.k_gl, p.K) == 0 || outer_lds_bytes(p.k_s, p.k_gl, p.K) > kLdsPerWorkgroup))
@@ -387,7 +387,10 @@ static int64_t run_build(bs2e_ctx* ctx, bs2e_rk* rk)
     }
     if (prof) HIP_TRY(ctx, hipEventRecord(ev[0], m));
     if (!rk->from_primitives) {
-        HIP_TRY(ctx, launch_outer_moments(rk->plan, rk->d, m));
+        if (rk->precision == BS2E_PREC_MIXED)
+            HIP_TRY(ctx, launch_outer_moments_mixed(rk->plan, rk->d, m));
+        else
+            HIP_TRY(ctx, launch_outer_moments(rk->plan, rk->d, m));
         HIP_TRY(ctx, launch_pair_windows(rk->plan, rk->d, m));
     }
     if (prof) HIP_TRY(ctx, hipEventRecord(ev[1], m));
@@ -481,14 +484,14 @@ int64_t bs2e_slater_primitives(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const 
 
 // A tensor handle for an existing plan (whose near rows are built): checks
 // the limits, allocates (in the context's arena when it has one), uploads.
-int64_t bs2e::rk_from_plan(bs2e_ctx* ctx, Plan&& plan, bs2e_rk** out)
+int64_t bs2e::rk_from_plan(bs2e_ctx* ctx, Plan&& plan, bs2e_rk** out, int64_t precision)
 {
     *out = nullptr;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     bs2e_rk* rk = new (std::nothrow) bs2e_rk();
     if (!rk) return fail(ctx, BS2E_ENOMEM, "bs2e_rk_create: host allocation failed");
     rk_attach(ctx, rk);
-    rk->precision = BS2E_PREC_FP64;
+    rk->precision = precision;
     rk->plan = std::move(plan);
     int64_t rc = check_plan_limits(ctx, rk->plan, true);
     if (!rc) rc = alloc_tables(ctx, rk, true);
@@ -510,12 +513,15 @@ int64_t bs2e_rk_create(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const double* 
     if (!ctx) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_create: ctx is NULL");
     if (!out) return fail(ctx, BS2E_EINVAL, "bs2e_rk_create: out is NULL");
     *out = nullptr;
-    if (precision != BS2E_PREC_FP64) return fail(ctx, BS2E_EINVAL, "bs2e_rk_create: unknown precision mode");
+    if (precision != BS2E_PREC_FP64 && precision != BS2E_PREC_MIXED)
+        return fail(ctx, BS2E_EINVAL, "bs2e_rk_create: unknown precision mode");
     Plan plan;
     std::string msg = make_plan(k, n_knots, knots, k_gl, max_k, plan);
     if (!msg.empty()) return fail(ctx, BS2E_EINVAL, "bs2e_rk_create: " + msg);
+    if (precision == BS2E_PREC_MIXED && !mixed_supported(plan.k_s, plan.k_gl, plan.K))
+        return fail(ctx, BS2E_EINVAL, "bs2e_rk_create: the mixed-precision moments take k_GL <= 16");
     build_near_rows(plan);
-    return rk_from_plan(ctx, std::move(plan), out);
+    return rk_from_plan(ctx, std::move(plan), out, precision);
 }
 
 int64_t bs2e_rk_from_primitives(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const double* knots, int64_t max_k,

@@ -135,8 +135,12 @@ size_t rk_footprint(const Plan& p);
 std::string channel_tables(const int64_t* configs, int64_t n_config, int64_t nb, std::vector<BlockChan>& chan,
                            std::vector<BlockSeg>& seg, std::vector<int32_t>& cfg_chan, int64_t& max_l);
 
+// The kernels' size limits for a plan (BS2E_EINVAL with a message when one is
+// exceeded); needs_primitives: the plan's primitives are computed on the device.
+int64_t check_plan_limits(bs2e_ctx* ctx, const Plan& p, bool needs_primitives);
+
 // bs2e_rk_create for an existing plan with its near rows built.
-int64_t rk_from_plan(bs2e_ctx* ctx, Plan&& plan, bs2e_rk** out);
+int64_t rk_from_plan(bs2e_ctx* ctx, Plan&& plan, bs2e_rk** out, int64_t precision = BS2E_PREC_FP64);
 
 }  // namespace bs2e
 

@@ -36,6 +36,13 @@ extern "C" {
 #define BS2E_ESTATE (-5)   /* object not in a state that allows the call */
 
 #define BS2E_PREC_FP64 0   /* plain FP64 path */
+/* Mixed precision: the off-diagonal moments (pair densities x radial moments,
+ * mat_els.f90:413-428) on the FP32 matrix cores from exact 10-bit integer
+ * slices of both operands with power-of-two scales, recombined in FP64 (about
+ * 2^-41 of the largest term of a moment's row and column); de Boor values,
+ * diagonal cells, window sums and the tensor's products stay FP64.
+ * Orders <= 16 with k_GL <= 16. */
+#define BS2E_PREC_MIXED 1
 
 typedef struct bs2e_ctx bs2e_ctx; /* one device + one stream */
 typedef struct bs2e_rk bs2e_rk;   /* one basis and its device-resident R^k tensor */
@@ -92,7 +99,8 @@ int64_t bs2e_setup_H_one_particle(bs2e_ctx *ctx, int64_t k, int64_t n_knots, con
 /* sparse_array_tools.f90:452-493 compute_R_K_map(r_d_k, r_k, r_m_k,
  * b_splines, max_k, R) and :495-507 Nd_DOK%init.  Plans the basis and
  * allocates the device tensor R[pair(a,c)][pair(b,d)][k] (FP64) and its
- * work tables; computes nothing yet. */
+ * work tables; computes nothing yet.  precision: BS2E_PREC_FP64 or
+ * BS2E_PREC_MIXED. */
 int64_t bs2e_rk_create(bs2e_ctx *ctx, int64_t k, int64_t n_knots, const double *knots, int64_t k_gl, int64_t max_k,
                        int64_t precision, bs2e_rk **out);
 /* sparse_array_tools.f90:452-493 compute_R_K_map(r_d_k, r_k, r_m_k,
