@@ -303,6 +303,41 @@ class Server:
         t = np.ascontiguousarray(t, dtype=np.float64)
         return self._check(capi.lib().bs2e_server_submit(self.handle, k, len(t), ptr(t), k_GL, max_k))
 
+    def submit_job(self, k, t, k_GL, max_k, precision="fp64", terms=(), full=True, Z=2, max_l_1p=None,
+                   CAP_c=(2, 10.0, 0.0)) -> int:
+        """A job of the two-electron branch: the tensor, the one-electron
+        matrices (hydrogenic Z, CAP_c = (order, r_0, eta)) and the H/S blocks
+        of `terms` (sym objects with their configs).  Returns the ticket."""
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        cfs = [np.ascontiguousarray(tm.configs, dtype=np.int64) for tm in terms]
+        L = np.array([int(tm.l) for tm in terms], np.int64)
+        n_config = np.array([c.shape[0] for c in cfs], np.int64)
+        configs = np.ascontiguousarray(np.concatenate(cfs, axis=0)) if cfs else np.zeros((0, 5), np.int64)
+        if max_l_1p is None:
+            max_l_1p = int(configs[:, 2:4].max()) if len(configs) else 0
+        order, r0, eta = CAP_c
+        eta = complex(eta)
+        I = C.c_int64
+        job = capi.Job(k=k, n_knots=len(t), knots=ptr(t), k_gl=k_GL, max_k=max_k,
+                       precision={"fp64": capi.PREC_FP64, "mixed": capi.PREC_MIXED}[precision],
+                       n_sym=len(cfs), L=ptr(L, I), n_config=ptr(n_config, I), configs=ptr(configs, I),
+                       full=int(bool(full)), Z=int(Z), max_l_1p=int(max_l_1p), cap_order=int(order), cap_r0=float(r0),
+                       cap_eta_re=eta.real, cap_eta_im=eta.imag)
+        return self._check(capi.lib().bs2e_server_submit_job(self.handle, C.byref(job)))
+
+    def blocks(self, ticket, terms):
+        """The finished job's blocks (owned by the service until the job is
+        retired), read as a BlockSet of `terms` (the job's symmetries)."""
+        h = C.c_void_p()
+        self._check(capi.lib().bs2e_server_blocks(self.handle, ticket, C.byref(h)))
+        return BlockSet.wrap(h, terms)
+
+    def cache_info(self):
+        """(entries, device bytes, admissions served from the cache, admissions that computed their tables)"""
+        v = np.zeros(4, np.int64)
+        self._check(capi.lib().bs2e_server_cache(self.handle, ptr(v, C.c_int64), 4))
+        return tuple(int(x) for x in v)
+
     def progress(self) -> int:
         return self._check(capi.lib().bs2e_server_progress(self.handle))
 
@@ -548,6 +583,23 @@ class BlockSet:
     of them (main_basis_setup.f90:104-115 runs construct_block_tensor per
     symmetry)."""
 
+    owned = True
+
+    @classmethod
+    def wrap(cls, handle, terms):
+        """A set the library owns (a service job's blocks): read-only access."""
+        obj = cls.__new__(cls)
+        obj.owned = False
+        obj.tensor = None
+        obj.ctx = _ThreadErrors()
+        obj.terms = list(terms)
+        obj.n_config = [int(np.asarray(t.configs).shape[0]) for t in obj.terms]
+        obj.handle = handle
+        sz = np.zeros(5 * len(obj.terms), np.int64)
+        check(capi.lib().bs2e_blockset_sizes(handle, ptr(sz, C.c_int64), sz.size))
+        obj.sizes = sz.reshape(-1, 5)
+        return obj
+
     def __init__(self, tensor: RkTensor, terms, full=True):
         self.tensor = tensor
         self.ctx = tensor.ctx
@@ -634,9 +686,9 @@ class BlockSet:
         return float(ms[0]), float(ms[1])
 
     def release(self):
-        if self.handle:
+        if self.handle and self.owned:
             capi.lib().bs2e_blockset_release(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:

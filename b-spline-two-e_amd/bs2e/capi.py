@@ -86,6 +86,9 @@ SIGNATURES = [
     ("bs2e_server_last_error", C.c_char_p, [vp]),
     ("bs2e_footprint", i64, [i64, i64, P(dbl), i64, i64, P(i64)]),
     ("bs2e_server_submit", i64, [vp, i64, i64, P(dbl), i64, i64]),
+    ("bs2e_server_submit_job", i64, [vp, vp]),
+    ("bs2e_server_blocks", i64, [vp, i64, P(vp)]),
+    ("bs2e_server_cache", i64, [vp, P(i64), i64]),
     ("bs2e_server_progress", i64, [vp]),
     ("bs2e_server_status", i64, [vp, i64, P(i64), P(dbl)]),
     ("bs2e_server_wait", i64, [vp, i64]),
@@ -100,6 +103,14 @@ class Bs2eError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"bs2e error {code}: {msg}")
         self.code = code
+
+
+class Job(C.Structure):
+    """struct bs2e_job (include/bs2e.h)."""
+    _fields_ = [("k", i64), ("n_knots", i64), ("knots", P(dbl)), ("k_gl", i64), ("max_k", i64),
+                ("precision", i64), ("n_sym", i64), ("L", P(i64)), ("n_config", P(i64)), ("configs", P(i64)),
+                ("full", i64), ("Z", i64), ("max_l_1p", i64), ("cap_order", i64), ("cap_r0", dbl),
+                ("cap_eta_re", dbl), ("cap_eta_im", dbl)]
 
 
 _lib = None

@@ -139,31 +139,35 @@ struct bs2e_blockset {
 // A single symmetry's block is a set of one.
 struct bs2e_block : bs2e_blockset {};
 
+// A set's device memory comes from its tensor's context: the context's arena
+// when it has one (a build service's slot), else hipMalloc.
 static void set_drop_outputs(bs2e_blockset* b)
 {
-    dev_free(b->d_Hl);
-    dev_free(b->d_S);
-    dev_free(b->d_T);
-    dev_free(b->d_ixH);
-    dev_free(b->d_dH);
-    dev_free(b->d_ixS);
-    dev_free(b->d_dS);
+    bs2e_ctx* c = b->rk->ctx;
+    ctx_free(c, b->d_Hl);
+    ctx_free(c, b->d_S);
+    ctx_free(c, b->d_T);
+    ctx_free(c, b->d_ixH);
+    ctx_free(c, b->d_dH);
+    ctx_free(c, b->d_ixS);
+    ctx_free(c, b->d_dS);
     b->filled = false;
 }
 
 static void set_free(bs2e_blockset* b)
 {
+    bs2e_ctx* c = b->rk->ctx;
     set_drop_outputs(b);
-    dev_free(b->d_cfg);
-    dev_free(b->d_cfg_chan);
-    dev_free(b->d_chan);
-    dev_free(b->d_seg);
-    dev_free(b->d_coef);
-    dev_free(b->d_flags);
-    dev_free(b->d_cnt);
-    dev_free(b->d_ptr);
-    dev_free(b->d_tab);
-    dev_free(b->d_row0);
+    ctx_free(c, b->d_cfg);
+    ctx_free(c, b->d_cfg_chan);
+    ctx_free(c, b->d_chan);
+    ctx_free(c, b->d_seg);
+    ctx_free(c, b->d_coef);
+    ctx_free(c, b->d_flags);
+    ctx_free(c, b->d_cnt);
+    ctx_free(c, b->d_ptr);
+    ctx_free(c, b->d_tab);
+    ctx_free(c, b->d_row0);
 }
 
 // descriptor of symmetry s for the count (fill = false) or fill pass
@@ -268,16 +272,16 @@ static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* 
             }
     }
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    HIP_TRY(ctx, dev_alloc(&b->d_cfg, (size_t)5 * N));
-    HIP_TRY(ctx, dev_alloc(&b->d_cfg_chan, (size_t)N));
-    HIP_TRY(ctx, dev_alloc(&b->d_chan, b->chan.size()));
-    HIP_TRY(ctx, dev_alloc(&b->d_seg, b->seg.size()));
-    HIP_TRY(ctx, dev_alloc(&b->d_coef, std::max<size_t>(b->coef.size(), 1)));
-    HIP_TRY(ctx, dev_alloc(&b->d_flags, std::max<size_t>(b->flags.size(), 1)));
-    HIP_TRY(ctx, dev_alloc(&b->d_cnt, (size_t)4 * N));
-    HIP_TRY(ctx, dev_alloc(&b->d_ptr, (size_t)2 * N));
-    HIP_TRY(ctx, dev_alloc(&b->d_tab, (size_t)b->n_sym));
-    HIP_TRY(ctx, dev_alloc(&b->d_row0, (size_t)b->n_sym + 1));
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_cfg, (size_t)5 * N));
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_cfg_chan, (size_t)N));
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_chan, b->chan.size()));
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_seg, b->seg.size()));
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_coef, std::max<size_t>(b->coef.size(), 1)));
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_flags, std::max<size_t>(b->flags.size(), 1)));
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_cnt, (size_t)4 * N));
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_ptr, (size_t)2 * N));
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_tab, (size_t)b->n_sym));
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_row0, (size_t)b->n_sym + 1));
     hipStream_t st = ctx->stream;
     HIP_TRY(ctx, hipMemcpyAsync(b->d_cfg, configs, sizeof(int64_t) * 5 * N, hipMemcpyHostToDevice, st));
     HIP_TRY(ctx, hipMemcpyAsync(b->d_cfg_chan, b->cfg_chan.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
@@ -339,8 +343,100 @@ static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* 
     return BS2E_OK;
 }
 
-// Uploads the one-electron matrices, allocates the outputs and runs the fill
-// launch over every row of the set; results stay on the device.
+// Allocates the outputs and the fill's operand layouts, and enqueues the
+// layouts and the fill launch over every row of the set on the context's
+// stream, from the column-major one-electron matrices dHl (n_l of them) and
+// dS already on the device.  wait: blocks until the fill is done and records
+// its device time.  Results stay on the device.
+int64_t bs2e::set_fill_alloc(bs2e_blockset* b, int64_t n_l, const char* who)
+{
+    bs2e_ctx* ctx = b->rk->ctx;
+    if (n_l - 1 < b->max_l)
+        return fail(ctx, BS2E_EINVAL, std::string(who) + ": one-electron matrices do not cover every l of the blocks");
+    const Plan& p = b->rk->plan;
+    const size_t nn = (size_t)p.n_b * p.n_b;
+    const size_t nf = (size_t)p.n_pairs * p.K;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    ctx_free(ctx, b->d_T);
+    ctx_free(ctx, b->d_ixH);
+    ctx_free(ctx, b->d_dH);
+    ctx_free(ctx, b->d_ixS);
+    ctx_free(ctx, b->d_dS);
+    b->filled = false;
+    b->n_l = n_l;
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_T, 2 * nn * (size_t)(b->n_l + 1) + 2 * nf));
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_ixH, (size_t)std::max<int64_t>(b->totH, 1)));
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_dH, 2 * (size_t)std::max<int64_t>(b->totH, 1)));
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_ixS, (size_t)std::max<int64_t>(b->totS, 1)));
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_dS, 2 * (size_t)std::max<int64_t>(b->totS, 1)));
+    return BS2E_OK;
+}
+
+int64_t bs2e::set_fill_enqueue(bs2e_blockset* b, const double* dHl, const double* dS, bool wait, const char* who)
+{
+    (void)who;
+    bs2e_ctx* ctx = b->rk->ctx;
+    const Plan& p = b->rk->plan;
+    const size_t nn = (size_t)p.n_b * p.n_b;
+    const size_t nf = (size_t)p.n_pairs * p.K;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    for (int64_t s = 0; s < b->n_sym; ++s) b->tab[s] = set_args(b, s, true);
+    HIP_TRY(ctx, hipMemcpyAsync(b->d_tab, b->tab.data(), sizeof(BlockArgs) * b->n_sym, hipMemcpyHostToDevice, st));
+    clear_stale_error();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (wait) {
+        HIP_TRY(ctx, hipEventCreate(&e0));
+        HIP_TRY(ctx, hipEventCreate(&e1));
+    }
+    hipError_t le = wait ? hipEventRecord(e0, st) : hipSuccess;
+    if (le == hipSuccess)
+        le = launch_block_prepare(dHl, dS, (int)b->n_l, (int)p.n_b, b->rk->d.Ptot, b->rk->d.Qtot, (int)p.n_pairs,
+                                  (int)p.K, b->d_T, b->d_T + 2 * nn * b->n_l, b->d_T + 2 * nn * (b->n_l + 1),
+                                  b->d_T + 2 * nn * (b->n_l + 1) + nf, st);
+    if (le == hipSuccess)
+        le = launch_block_fill(b->d_tab, b->d_row0, (int)b->n_sym, b->n_rows, (int)p.k_s, (int)p.K, st);
+    if (wait) {
+        if (le == hipSuccess) le = hipEventRecord(e1, st);
+        if (le == hipSuccess) le = hipEventSynchronize(e1);
+        if (le == hipSuccess) le = hipEventElapsedTime(&b->ms_fill, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    HIP_TRY(ctx, le);
+    b->filled = true;   // stream-ordered: readers of the outputs on this stream see the fill
+    return BS2E_OK;
+}
+
+int64_t bs2e::set_fill_device(bs2e_blockset* b, int64_t n_l, const double* dHl, const double* dS, bool wait,
+                              const char* who)
+{
+    const int64_t rc = set_fill_alloc(b, n_l, who);
+    return rc ? rc : set_fill_enqueue(b, dHl, dS, wait, who);
+}
+
+static size_t arena_round(size_t bytes) { return (std::max<size_t>(bytes, 1) + 255) / 256 * 256; }
+
+size_t bs2e::blockset_fill_bytes(const bs2e_blockset* b, int64_t n_l)
+{
+    const Plan& p = b->rk->plan;
+    const size_t nn = (size_t)p.n_b * p.n_b, nf = (size_t)p.n_pairs * p.K;
+    const size_t H = (size_t)std::max<int64_t>(b->totH, 1), S = (size_t)std::max<int64_t>(b->totS, 1);
+    return arena_round(8 * (2 * nn * (size_t)(n_l + 1) + 2 * nf)) + arena_round(8 * H) + arena_round(16 * H) +
+           arena_round(8 * S) + arena_round(16 * S);
+}
+
+size_t bs2e::blockset_table_bytes(const bs2e_blockset* b)
+{
+    const size_t N = (size_t)b->n_rows;
+    return arena_round(40 * N) + arena_round(4 * N) + arena_round(sizeof(BlockChan) * b->chan.size()) +
+           arena_round(sizeof(BlockSeg) * b->seg.size()) + arena_round(8 * b->coef.size()) +
+           arena_round(4 * b->flags.size()) + arena_round(32 * N) + arena_round(16 * N) +
+           arena_round(sizeof(BlockArgs) * (size_t)b->n_sym) + arena_round(8 * ((size_t)b->n_sym + 1));
+}
+
+// Uploads the caller's one-electron matrices, then fills as set_fill_device
+// and waits.
 static int64_t set_fill(bs2e_blockset* b, int64_t max_l_1p, const double* Hl, const double* S, const char* who)
 {
     bs2e_ctx* ctx = b->rk->ctx;
@@ -351,39 +447,12 @@ static int64_t set_fill(bs2e_blockset* b, int64_t max_l_1p, const double* Hl, co
     const size_t nn = (size_t)p.n_b * p.n_b;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     set_drop_outputs(b);
-    HIP_TRY(ctx, dev_alloc(&b->d_Hl, 2 * nn * (size_t)(max_l_1p + 1)));
-    HIP_TRY(ctx, dev_alloc(&b->d_S, 2 * nn));
-    b->n_l = max_l_1p + 1;
-    const size_t nf = (size_t)p.n_pairs * p.K;
-    HIP_TRY(ctx, dev_alloc(&b->d_T, 2 * nn * (size_t)(b->n_l + 1) + 2 * nf));
-    HIP_TRY(ctx, dev_alloc(&b->d_ixH, (size_t)std::max<int64_t>(b->totH, 1)));
-    HIP_TRY(ctx, dev_alloc(&b->d_dH, 2 * (size_t)std::max<int64_t>(b->totH, 1)));
-    HIP_TRY(ctx, dev_alloc(&b->d_ixS, (size_t)std::max<int64_t>(b->totS, 1)));
-    HIP_TRY(ctx, dev_alloc(&b->d_dS, 2 * (size_t)std::max<int64_t>(b->totS, 1)));
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_Hl, 2 * nn * (size_t)(max_l_1p + 1)));
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_S, 2 * nn));
     hipStream_t st = ctx->stream;
     HIP_TRY(ctx, hipMemcpyAsync(b->d_Hl, Hl, sizeof(double) * 2 * nn * (max_l_1p + 1), hipMemcpyHostToDevice, st));
     HIP_TRY(ctx, hipMemcpyAsync(b->d_S, S, sizeof(double) * 2 * nn, hipMemcpyHostToDevice, st));
-    for (int64_t s = 0; s < b->n_sym; ++s) b->tab[s] = set_args(b, s, true);
-    HIP_TRY(ctx, hipMemcpyAsync(b->d_tab, b->tab.data(), sizeof(BlockArgs) * b->n_sym, hipMemcpyHostToDevice, st));
-    clear_stale_error();
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    HIP_TRY(ctx, hipEventCreate(&e0));
-    HIP_TRY(ctx, hipEventCreate(&e1));
-    hipError_t le = hipEventRecord(e0, st);
-    if (le == hipSuccess)
-        le = launch_block_prepare(b->d_Hl, b->d_S, (int)b->n_l, (int)p.n_b, b->rk->d.Ptot, b->rk->d.Qtot,
-                                  (int)p.n_pairs, (int)p.K, b->d_T, b->d_T + 2 * nn * b->n_l,
-                                  b->d_T + 2 * nn * (b->n_l + 1), b->d_T + 2 * nn * (b->n_l + 1) + nf, st);
-    if (le == hipSuccess)
-        le = launch_block_fill(b->d_tab, b->d_row0, (int)b->n_sym, b->n_rows, (int)p.k_s, (int)p.K, st);
-    if (le == hipSuccess) le = hipEventRecord(e1, st);
-    if (le == hipSuccess) le = hipEventSynchronize(e1);
-    if (le == hipSuccess) le = hipEventElapsedTime(&b->ms_fill, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    HIP_TRY(ctx, le);
-    b->filled = true;
-    return BS2E_OK;
+    return set_fill_device(b, max_l_1p + 1, b->d_Hl, b->d_S, true, who);
 }
 
 // Copies symmetry s's CSR arrays to the host (index_ptr 1-based as the
@@ -506,11 +575,26 @@ int64_t bs2e_block_release(bs2e_block* b)
 int64_t bs2e_blockset_create(bs2e_rk* rk, int64_t n_sym, const int64_t* L, const int64_t* n_config,
                              const int64_t* configs, int64_t full, bs2e_blockset** out)
 {
+    if (!rk || !out) return fail(rk ? rk->ctx : nullptr, BS2E_EINVAL, "bs2e_blockset_create: NULL argument");
+    *out = nullptr;
+    if (!rk->computed)
+        return fail(rk->ctx, BS2E_ESTATE, "bs2e_blockset_create: the R^k tensor has not been computed");
+    return blockset_new(rk, n_sym, L, n_config, configs, full, out);
+}
+
+}  // extern "C"
+
+// bs2e_blockset_create without the computed-tensor check: the count pass
+// reads only the plan's tables, so a build service counts a job's blocks
+// before its tensor is built (the fill, enqueued after the build on the same
+// stream, reads the tensor).
+int64_t bs2e::blockset_new(bs2e_rk* rk, int64_t n_sym, const int64_t* L, const int64_t* n_config,
+                           const int64_t* configs, int64_t full, bs2e_blockset** out)
+{
     if (!rk || !out || !L || !n_config || !configs)
         return fail(rk ? rk->ctx : nullptr, BS2E_EINVAL, "bs2e_blockset_create: NULL argument");
     bs2e_ctx* ctx = rk->ctx;
     *out = nullptr;
-    if (!rk->computed) return fail(ctx, BS2E_ESTATE, "bs2e_blockset_create: the R^k tensor has not been computed");
     if (n_sym < 1 || n_sym > 100000) return fail(ctx, BS2E_EINVAL, "bs2e_blockset_create: need 1 <= n_sym <= 100000");
     int64_t total = 0;
     for (int64_t s = 0; s < n_sym; ++s) {
@@ -538,6 +622,8 @@ int64_t bs2e_blockset_create(bs2e_rk* rk, int64_t n_sym, const int64_t* L, const
     *out = b;
     return BS2E_OK;
 }
+
+extern "C" {
 
 int64_t bs2e_blockset_sizes(const bs2e_blockset* b, int64_t* sizes, int64_t n)
 {

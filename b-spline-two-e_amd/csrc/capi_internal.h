@@ -139,6 +139,24 @@ std::string channel_tables(const int64_t* configs, int64_t n_config, int64_t nb,
 // exceeded); needs_primitives: the plan's primitives are computed on the device.
 int64_t check_plan_limits(bs2e_ctx* ctx, const Plan& p, bool needs_primitives);
 
+// Blocks of several symmetries against a planned tensor (capi_block.hip):
+// bs2e_blockset_create without the computed-tensor check (the count pass
+// needs only the plan), and the fill from one-electron matrices already on the
+// device (column-major, n_l of H then S), enqueued on the tensor context's
+// stream (wait: block until done).
+int64_t blockset_new(bs2e_rk* rk, int64_t n_sym, const int64_t* L, const int64_t* n_config, const int64_t* configs,
+                     int64_t full, bs2e_blockset** out);
+int64_t set_fill_device(bs2e_blockset* b, int64_t n_l, const double* dHl, const double* dS, bool wait,
+                        const char* who);
+// set_fill_device in two steps: the allocations (nothing enqueued), then the
+// layouts and the fill launch.
+int64_t set_fill_alloc(bs2e_blockset* b, int64_t n_l, const char* who);
+int64_t set_fill_enqueue(bs2e_blockset* b, const double* dHl, const double* dS, bool wait, const char* who);
+// Device bytes set_fill_device allocates for n_l one-electron matrices (with
+// the arena's rounding), and the set's own tables (allocated at creation).
+size_t blockset_fill_bytes(const bs2e_blockset* b, int64_t n_l);
+size_t blockset_table_bytes(const bs2e_blockset* b);
+
 // bs2e_rk_create for an existing plan with its near rows built.
 int64_t rk_from_plan(bs2e_ctx* ctx, Plan&& plan, bs2e_rk** out, int64_t precision = BS2E_PREC_FP64);
 

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <deque>
+#include <list>
 #include <map>
 #include <new>
 
@@ -21,14 +22,49 @@ double now_ms()
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
-struct Job {
-    Plan plan;                     // planned at submission, handed to the tensor at admission
+size_t arena_round(size_t bytes) { return (std::max<size_t>(bytes, 1) + 255) / 256 * 256; }
+
+// The one-electron tables of one basis on the device: spline values and
+// second derivatives at the GL points, S and H(l) for l = 0 .. n_l-1
+// (column-major complex, as the fill reads them).  Jobs on the same knots, GL
+// rule, charge and absorbing potential share one entry (a convergence scan
+// resubmits a basis with other multipole counts and symmetries); an entry
+// with more l's serves a job that needs fewer.
+struct OneBody {
+    int64_t k = 0, k_gl = 0, Z = 0, cap_order = 0, n_l = 0;
+    double cap_r0 = 0, eta_re = 0, eta_im = 0;
+    std::vector<double> knots;
+    double *B = nullptr, *B2 = nullptr, *rw = nullptr, *S = nullptr, *H = nullptr;   // the service's arena
+    hipEvent_t ready = nullptr;   // recorded after S and H are written
+    hipStream_t made_on = nullptr;
+    int refs = 0;                 // jobs admitted with it and not retired
+    uint64_t last = 0;            // admission tick of its last use
     size_t bytes = 0;
+};
+
+size_t onebody_bytes(const Plan& p, int64_t n_l)
+{
+    const size_t tab = (size_t)p.n_iv * p.k_gl * p.k_s, nn = (size_t)p.n_b * p.n_b;
+    return 2 * arena_round(8 * tab) + arena_round(16 * (size_t)p.n_iv * p.k_gl) + arena_round(16 * nn) +
+           arena_round(16 * nn * (size_t)n_l);
+}
+
+struct Job {
+    Plan plan;                     // planned at submission, copied into the tensor at admission
+    int64_t precision = BS2E_PREC_FP64;
+    size_t bytes = 0;              // device footprint: tensor + tables (+ blocks, once counted)
+    bool sized = false;            // bytes includes the blocks' exact sizes
     int state = 0;                 // 0 queued, 1 running, 2 done, 3 failed
     int slot = -1;
     double t_submit = 0, t_admit = 0, t_done = 0;
     bs2e_rk* rk = nullptr;
     hipEvent_t done = nullptr;
+    // the symmetry blocks of the job (n_sym = 0: the tensor only)
+    int64_t n_sym = 0, full = 1, Z = 0, max_l_1p = 0, cap_order = 0;
+    double cap_r0 = 0, eta_re = 0, eta_im = 0;
+    std::vector<int64_t> L, n_config, configs;
+    bs2e_blockset* set = nullptr;
+    OneBody* ob = nullptr;
 };
 
 }  // namespace
@@ -41,18 +77,166 @@ struct bs2e_server {
     std::deque<int64_t> queue;
     std::map<int64_t, Job> jobs;
     std::deque<int64_t> finished;    // done since the last bs2e_server_finished
+    std::list<OneBody> cache;        // one-electron tables by basis
+    int64_t cache_hits = 0, cache_misses = 0;
+    uint64_t tick = 0;
     int64_t next = 0;
     bs2e_ctx* err = nullptr;         // error messages of the service
 };
 
-static int64_t job_fail(bs2e_server* s, Job& j, int64_t code)
+static void onebody_free(bs2e_server* s, OneBody& o)
 {
-    j.state = 3;
+    for (double** p : {&o.B, &o.B2, &o.rw, &o.S, &o.H})
+        if (*p) {
+            s->arena.release(*p);
+            *p = nullptr;
+        }
+    if (o.ready) (void)hipEventDestroy(o.ready);
+    o.ready = nullptr;
+}
+
+// Frees the cache entries no job holds, least recently used first, until
+// `need` bytes fit the largest free range; returns whether any were freed.
+static bool cache_evict(bs2e_server* s, size_t need)
+{
+    bool any = false;
+    while (s->arena.largest_free() < need) {
+        auto victim = s->cache.end();
+        for (auto it = s->cache.begin(); it != s->cache.end(); ++it)
+            if (it->refs == 0 && (victim == s->cache.end() || it->last < victim->last)) victim = it;
+        if (victim == s->cache.end()) break;
+        onebody_free(s, *victim);
+        s->cache.erase(victim);
+        any = true;
+    }
+    return any;
+}
+
+static OneBody* cache_find(bs2e_server* s, const Job& j)
+{
+    const Plan& p = j.plan;
+    for (OneBody& o : s->cache)
+        if (o.k == p.k_s && o.k_gl == p.k_gl && o.Z == j.Z && o.cap_order == j.cap_order && o.cap_r0 == j.cap_r0 &&
+            o.eta_re == j.eta_re && o.eta_im == j.eta_im && o.n_l >= j.max_l_1p + 1 && o.knots == p.t)
+            return &o;
+    return nullptr;
+}
+
+// A new entry for job j, computed on slot context c's stream from the job's
+// tensor tables (knots, breakpoints, GL rule), which outlive the kernels: the
+// job is retired only after its done event, recorded behind them.
+static int64_t cache_make(bs2e_server* s, Job& j, bs2e_ctx* c, OneBody** out)
+{
+    const Plan& p = j.plan;
+    s->cache.emplace_back();
+    OneBody& o = s->cache.back();
+    o.k = p.k_s, o.k_gl = p.k_gl, o.Z = j.Z, o.cap_order = j.cap_order, o.n_l = j.max_l_1p + 1;
+    o.cap_r0 = j.cap_r0, o.eta_re = j.eta_re, o.eta_im = j.eta_im;
+    o.knots = p.t;
+    o.bytes = onebody_bytes(p, o.n_l);
+    const size_t tab = (size_t)p.n_iv * p.k_gl * p.k_s, nn = (size_t)p.n_b * p.n_b;
+    auto grab = [&](double** d, size_t n) { return (*d = (double*)s->arena.alloc(8 * n)) != nullptr; };
+    if (!grab(&o.B, tab) || !grab(&o.B2, tab) || !grab(&o.rw, 2 * (size_t)p.n_iv * p.k_gl) || !grab(&o.S, 2 * nn) ||
+        !grab(&o.H, 2 * nn * (size_t)o.n_l)) {
+        onebody_free(s, o);
+        s->cache.pop_back();
+        return BS2E_ENOMEM;
+    }
+    hipError_t e = hipEventCreateWithFlags(&o.ready, hipEventDisableTiming);
+    if (e == hipSuccess)
+        e = launch_one_body(j.rk->plan, j.rk->d, o.B, o.B2, o.rw, 0, (int)o.n_l, (int)j.Z, (int)j.cap_order, j.cap_r0,
+                            j.eta_re, j.eta_im, o.S, o.H, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(o.ready, c->stream);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(c->stream);
+        onebody_free(s, o);
+        s->cache.pop_back();
+        return fail(s->err, BS2E_EDEVICE, std::string("bs2e_server: one-electron tables: ") + hipGetErrorString(e));
+    }
+    o.made_on = c->stream;
+    *out = &o;
+    return BS2E_OK;
+}
+
+// Frees what an admitted (or half-admitted) job holds on the device.
+static void job_release(Job& j)
+{
+    if (j.set) {
+        (void)bs2e_blockset_release(j.set);
+        j.set = nullptr;
+    }
     if (j.rk) {
         (void)bs2e_rk_release(j.rk);
         j.rk = nullptr;
     }
+    if (j.ob) {
+        j.ob->refs -= 1;
+        j.ob = nullptr;
+    }
+}
+
+static int64_t job_fail(bs2e_server* s, Job& j, int64_t code)
+{
+    (void)s;
+    j.state = 3;
+    job_release(j);
     return code;
+}
+
+// Admits job j on slot `slot`: the tensor (and its tables) in the arena, the
+// blocks counted (a short synchronous pass on the slot's stream: the count
+// kernel after the tables' upload), the one-electron tables found in the
+// cache or computed, the block outputs allocated, then the build and the
+// fill enqueued behind each other and the done event recorded.  Returns
+// BS2E_ENOMEM (nothing held, j.bytes updated) when the budget lacks room now.
+static int64_t admit(bs2e_server* s, Job& j, int slot)
+{
+    bs2e_ctx* c = s->slots[slot];
+    Plan keep = j.plan;
+    int64_t rc = rk_from_plan(c, std::move(keep), &j.rk, j.precision);
+    if (rc) {
+        j.rk = nullptr;
+        return rc;
+    }
+    if (j.n_sym > 0) {
+        rc = blockset_new(j.rk, j.n_sym, j.L.data(), j.n_config.data(), j.configs.data(), j.full, &j.set);
+        OneBody* ob = rc ? nullptr : cache_find(s, j);
+        bool made = false;
+        if (!rc && ob) {
+            s->cache_hits += 1;
+        } else if (!rc) {
+            rc = cache_make(s, j, c, &ob);
+            made = rc == BS2E_OK;
+            if (made) s->cache_misses += 1;
+        }
+        if (!rc) {
+            j.ob = ob;
+            ob->refs += 1;
+            ob->last = ++s->tick;
+            rc = set_fill_alloc(j.set, j.max_l_1p + 1, "bs2e_server");
+        }
+        if (j.set) {
+            j.bytes = rk_footprint(j.plan) + blockset_table_bytes(j.set) + blockset_fill_bytes(j.set, j.max_l_1p + 1);
+            j.sized = true;
+        }
+        if (rc) {
+            if (made) (void)hipStreamSynchronize(c->stream);   // the new entry's kernels read the job's tables
+            job_release(j);
+            return rc;
+        }
+    }
+    rc = bs2e_rk_compute(j.rk);
+    if (!rc && j.set) {
+        if (j.ob->made_on != c->stream && hipStreamWaitEvent(c->stream, j.ob->ready, 0) != hipSuccess)
+            rc = fail(s->err, BS2E_EDEVICE, "bs2e_server: hipStreamWaitEvent failed");
+        if (!rc) rc = set_fill_enqueue(j.set, j.ob->H, j.ob->S, false, "bs2e_server");
+    }
+    if (!rc && hipEventRecord(j.done, c->stream) != hipSuccess) rc = BS2E_EDEVICE;
+    if (rc) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamSynchronize(c->side);
+    }
+    return rc;
 }
 
 // Retires finished builds and admits queued jobs (FIFO; a job that does not
@@ -82,26 +266,18 @@ static int64_t progress(bs2e_server* s)
             }
         if (slot < 0) break;
         Job& j = s->jobs[*it];
-        if (j.bytes > s->arena.largest_free()) {
+        if (j.bytes > s->arena.largest_free() && !cache_evict(s, j.bytes)) {
             ++it;
             continue;
         }
-        bs2e_ctx* c = s->slots[slot];
-        Plan keep = j.plan;        // rk_from_plan consumes its plan; keep one for a retry
-        int64_t rc = rk_from_plan(c, std::move(keep), &j.rk);
-        if (rc == BS2E_ENOMEM) {   // fragmentation: keep it queued
-            j.rk = nullptr;
+        int64_t rc = admit(s, j, slot);
+        if (rc == BS2E_ENOMEM && cache_evict(s, j.bytes)) rc = admit(s, j, slot);
+        if (rc == BS2E_ENOMEM && j.bytes <= s->arena.capacity()) {   // room later: keep it queued
             ++it;
             continue;
         }
-        j.plan = Plan();
-        if (rc) {
-            job_fail(s, j, rc);
-            it = s->queue.erase(it);
-            continue;
-        }
-        rc = bs2e_rk_compute(j.rk);
-        if (!rc && hipEventRecord(j.done, c->stream) != hipSuccess) rc = BS2E_EDEVICE;
+        if (rc == BS2E_ENOMEM)
+            fail(s->err, BS2E_ENOMEM, "bs2e_server: the job's blocks and tensor exceed the service's budget");
         if (rc) {
             job_fail(s, j, rc);
             it = s->queue.erase(it);
@@ -169,9 +345,11 @@ int64_t bs2e_server_destroy(bs2e_server* s)
         (void)hipStreamSynchronize(c->side);
     }
     for (auto& kv : s->jobs) {
-        if (kv.second.rk) (void)bs2e_rk_release(kv.second.rk);
+        job_release(kv.second);
         if (kv.second.done) (void)hipEventDestroy(kv.second.done);
     }
+    for (OneBody& o : s->cache) onebody_free(s, o);
+    s->cache.clear();
     for (bs2e_ctx* c : s->slots) bs2e_destroy(c);
     s->arena.destroy();
     bs2e_destroy(s->err);
@@ -192,26 +370,63 @@ int64_t bs2e_footprint(int64_t k, int64_t n_knots, const double* knots, int64_t 
     return BS2E_OK;
 }
 
-int64_t bs2e_server_submit(bs2e_server* s, int64_t k, int64_t n_knots, const double* knots, int64_t k_gl,
-                           int64_t max_k)
+int64_t bs2e_server_submit_job(bs2e_server* s, const bs2e_job* job)
 {
-    if (!s || !knots || n_knots < 1) return fail(s ? s->err : nullptr, BS2E_EINVAL, "bs2e_server_submit: bad argument");
+    if (!s || !job || !job->knots || job->n_knots < 1)
+        return fail(s ? s->err : nullptr, BS2E_EINVAL, "bs2e_server_submit_job: bad argument");
+    if (job->precision != BS2E_PREC_FP64 && job->precision != BS2E_PREC_MIXED)
+        return fail(s->err, BS2E_EINVAL, "bs2e_server_submit_job: unknown precision mode");
     Job j;
     j.t_submit = now_ms();
-    std::string msg = make_plan(k, n_knots, knots, k_gl, max_k, j.plan);
-    if (!msg.empty()) return fail(s->err, BS2E_EINVAL, "bs2e_server_submit: " + msg);
+    std::string msg = make_plan(job->k, job->n_knots, job->knots, job->k_gl, job->max_k, j.plan);
+    if (!msg.empty()) return fail(s->err, BS2E_EINVAL, "bs2e_server_submit_job: " + msg);
+    if (job->precision == BS2E_PREC_MIXED && !mixed_supported(j.plan.k_s, j.plan.k_gl, j.plan.K))
+        return fail(s->err, BS2E_EINVAL, "bs2e_server_submit_job: the mixed-precision moments take k_GL <= 16");
+    j.precision = job->precision;
+    if (job->n_sym < 0) return fail(s->err, BS2E_EINVAL, "bs2e_server_submit_job: n_sym < 0");
+    if (job->n_sym > 0) {
+        if (!job->L || !job->n_config || !job->configs)
+            return fail(s->err, BS2E_EINVAL, "bs2e_server_submit_job: NULL symmetry arrays");
+        if (job->max_l_1p < 0 || job->cap_order < 0 || job->cap_order > 64)
+            return fail(s->err, BS2E_EINVAL, "bs2e_server_submit_job: need max_l_1p >= 0, 0 <= cap_order <= 64");
+        int64_t rows = 0;
+        for (int64_t i = 0; i < job->n_sym; ++i) {
+            if (job->L[i] < 0 || job->n_config[i] < 1)
+                return fail(s->err, BS2E_EINVAL, "bs2e_server_submit_job: need L >= 0, n_config >= 1");
+            rows += job->n_config[i];
+        }
+        for (int64_t r = 0; r < rows; ++r)
+            if (job->configs[5 * r + 2] > job->max_l_1p || job->configs[5 * r + 3] > job->max_l_1p)
+                return fail(s->err, BS2E_EINVAL, "bs2e_server_submit_job: a configuration's l exceeds max_l_1p");
+        j.n_sym = job->n_sym;
+        j.full = job->full ? 1 : 0;
+        j.Z = job->Z, j.max_l_1p = job->max_l_1p, j.cap_order = job->cap_order;
+        j.cap_r0 = job->cap_r0, j.eta_re = job->cap_eta_re, j.eta_im = job->cap_eta_im;
+        j.L.assign(job->L, job->L + job->n_sym);
+        j.n_config.assign(job->n_config, job->n_config + job->n_sym);
+        j.configs.assign(job->configs, job->configs + 5 * rows);
+    }
     build_near_rows(j.plan);
     j.bytes = rk_footprint(j.plan);
     if (j.bytes > s->arena.capacity())
-        return fail(s->err, BS2E_ENOMEM, "bs2e_server_submit: the job's device footprint exceeds the service's budget");
+        return fail(s->err, BS2E_ENOMEM, "bs2e_server_submit_job: the job's device footprint exceeds the budget");
     (void)hipSetDevice(s->device);
     if (hipEventCreateWithFlags(&j.done, hipEventDisableTiming) != hipSuccess)
-        return fail(s->err, BS2E_EDEVICE, "bs2e_server_submit: hipEventCreate failed");
+        return fail(s->err, BS2E_EDEVICE, "bs2e_server_submit_job: hipEventCreate failed");
     const int64_t t = s->next++;
     s->jobs[t] = std::move(j);
     s->queue.push_back(t);
     const int64_t rc = progress(s);
     return rc < 0 ? rc : t;
+}
+
+int64_t bs2e_server_submit(bs2e_server* s, int64_t k, int64_t n_knots, const double* knots, int64_t k_gl,
+                           int64_t max_k)
+{
+    bs2e_job job{};
+    job.k = k, job.n_knots = n_knots, job.knots = knots, job.k_gl = k_gl, job.max_k = max_k;
+    job.precision = BS2E_PREC_FP64;
+    return bs2e_server_submit_job(s, &job);
 }
 
 int64_t bs2e_server_progress(bs2e_server* s)
@@ -273,6 +488,27 @@ int64_t bs2e_server_tensor(bs2e_server* s, int64_t ticket, bs2e_rk** rk)
     return BS2E_OK;
 }
 
+int64_t bs2e_server_blocks(bs2e_server* s, int64_t ticket, bs2e_blockset** set)
+{
+    if (!s || !set) return fail(nullptr, BS2E_EINVAL, "bs2e_server_blocks: NULL");
+    auto it = s->jobs.find(ticket);
+    if (it == s->jobs.end() || it->second.state != 2)
+        return fail(s->err, BS2E_ESTATE, "bs2e_server_blocks: the job is unknown or not finished");
+    if (!it->second.set) return fail(s->err, BS2E_ESTATE, "bs2e_server_blocks: the job has no blocks");
+    *set = it->second.set;
+    return BS2E_OK;
+}
+
+int64_t bs2e_server_cache(const bs2e_server* s, int64_t* info, int64_t n)
+{
+    if (!s || !info) return fail(nullptr, BS2E_EINVAL, "bs2e_server_cache: NULL");
+    size_t bytes = 0;
+    for (const OneBody& o : s->cache) bytes += o.bytes;
+    const int64_t v[4] = {(int64_t)s->cache.size(), (int64_t)bytes, s->cache_hits, s->cache_misses};
+    for (int64_t i = 0; i < n && i < 4; ++i) info[i] = v[i];
+    return BS2E_OK;
+}
+
 int64_t bs2e_server_retire(bs2e_server* s, int64_t ticket)
 {
     if (!s) return fail(nullptr, BS2E_EINVAL, "bs2e_server_retire: NULL");
@@ -281,7 +517,7 @@ int64_t bs2e_server_retire(bs2e_server* s, int64_t ticket)
     Job& j = it->second;
     if (j.state == 0 || j.state == 1)
         return fail(s->err, BS2E_ESTATE, "bs2e_server_retire: the job is still queued or running");
-    if (j.rk) (void)bs2e_rk_release(j.rk);
+    job_release(j);
     if (j.done) (void)hipEventDestroy(j.done);
     s->jobs.erase(it);
     const int64_t rc = progress(s);

@@ -279,14 +279,24 @@ def store_ceiling(stream, n_elems):
     return out
 
 
-def serving_jobs(n, seed):
-    """The serving stream's jobs as (order, knots, k_GL, max_k) tuples (bs2e.workloads.serving_jobs)."""
-    from bs2e.workloads import serving_jobs as draw
-    return [(j.k, j.t, j.k_GL, j.max_k) for j in draw(n, seed)]
+def branch_terms(n_b, k, lmax, max_L=2, l2=1):
+    """The symmetries of one basis_setup request of the serving stream: total
+    L = 0..max_L with natural parity (z_pol), one-electron l <= lmax, the
+    second electron's n2 <= n_b // 4 (an r_2_max truncation) and its l <= l2
+    for n1 > n_b // 2 (r_all_l; orbital_tools.f90:144-193)."""
+    import bs2e
+    terms = []
+    for L in range(max_L + 1):
+        term = bs2e.sym(l=L, pi=bool(L % 2))
+        term.configs = bs2e.count_configs(term, lmax, n_b, k, n_b // 4, n_b // 2, min(l2, lmax))
+        if term.n_config:
+            terms.append(term)
+    return terms
 
 
-def run_stream(srv, jobs, arrivals):
-    """Submits job i at arrivals[i] (s after the start), retires finished jobs;
+def run_stream(srv, jobs, arrivals, submit):
+    """Submits job i at arrivals[i] (s after the start) through submit(srv,
+    job), hands each finished ticket to its job's `done` hook, retires it;
     returns (latency ms per job, wall s)."""
     n = len(jobs)
     lat = [None] * n
@@ -296,41 +306,60 @@ def run_stream(srv, jobs, arrivals):
     while nxt < n or tickets:
         now = time.perf_counter() - t0
         while nxt < n and arrivals[nxt] <= now:
-            tickets[srv.submit(*jobs[nxt])] = nxt
+            tickets[submit(srv, jobs[nxt])] = nxt
             nxt += 1
         for tk, ms in srv.finished():
-            lat[tickets.pop(tk)] = ms
+            i = tickets.pop(tk)
+            lat[i] = ms
             srv.retire(tk)
     return lat, time.perf_counter() - t0
 
 
-def serving_benchmark(n_jobs, seed=20261202, slots=4, budget_gib=64, load=0.8):
-    """Continuous batching of a seeded stream of differing builds through the
-    build service: closed-loop capacity, then Poisson arrivals at `load` x
-    capacity with p50/p99 latency (submission to completion)."""
+def serving_benchmark(n_jobs, seed=20261202, slots=4, budget_gib=160, load=0.8, precision="mixed"):
+    """Continuous batching of a seeded stream of basis_setup requests through
+    the build service: each job is the two-electron branch (R^k tensor in
+    `precision`, one-electron matrices, the H/S blocks of L = 0..2); closed-
+    loop capacity, then Poisson arrivals at `load` x capacity with p50/p99
+    latency from arrival to the job's H/S CSR written."""
     import numpy as np
 
     import bs2e
-    jobs = serving_jobs(n_jobs, 7)
-    integrals = []
-    for k, t, kgl, mk in jobs:
-        _, _, n_pairs = bs2e.count_nnz(k, t)
-        integrals.append(n_pairs * n_pairs * (mk + 1))
+    from bs2e.workloads import serving_jobs
+    jobs = []
+    integrals, rows = [], []
+    for j in serving_jobs(n_jobs, 7):
+        n_b = len(j.t) - j.k - 2
+        terms = branch_terms(n_b, j.k, j.lmax)
+        jobs.append((j, terms))
+        _, _, n_pairs = bs2e.count_nnz(j.k, j.t)
+        integrals.append(n_pairs * n_pairs * (j.max_k + 1))
+        rows.append(sum(tm.n_config for tm in terms))
+
+    def submit(srv, job):
+        j, terms = job
+        return srv.submit_job(j.k, j.t, j.k_GL, j.max_k, precision=precision, terms=terms, full=False, Z=2,
+                              max_l_1p=j.lmax, CAP_c=(2, 0.8 * j.t[-1], 1e-3))
+
     srv = bs2e.Server(budget_gib << 30, slots=slots)
-    run_stream(srv, jobs[:8], [0.0] * 8)                     # warm-up
-    _, wall_cap = run_stream(srv, jobs, [0.0] * n_jobs)       # all at once: capacity
+    run_stream(srv, jobs[:4], [0.0] * 4, submit)                    # warm-up
+    _, wall_cap = run_stream(srv, jobs, [0.0] * n_jobs, submit)      # all at once: capacity
     cap = n_jobs / wall_cap
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / (load * cap), n_jobs))
-    lat, wall = run_stream(srv, jobs, arrivals)
+    lat, wall = run_stream(srv, jobs, arrivals, submit)
+    cache = srv.cache_info()
     srv.close()
     lat = np.array(lat)
-    return {"jobs": n_jobs, "slots": slots, "budget_GiB": budget_gib, "job_seed": 7, "arrival_seed": seed,
-            "integrals_total": int(sum(integrals)),
+    return {"unit_of_work": "the two-electron branch of one basis_setup request: R^k tensor, S and H(l), H/S CSR "
+                            "of L = 0..2 (l <= lmax, n2 <= n_b/4, l2 <= 1 beyond n_b/2, upper triangles)",
+            "precision": precision, "jobs": n_jobs, "slots": slots, "budget_GiB": budget_gib, "job_seed": 7,
+            "arrival_seed": seed, "integrals_total": int(sum(integrals)), "block_rows_total": int(sum(rows)),
             "capacity_jobs_per_s": cap, "capacity_integrals_per_s": sum(integrals) / wall_cap,
             "offered_load": load, "stream_integrals_per_s": sum(integrals) / wall,
             "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                           "mean": float(lat.mean()), "max": float(lat.max())}}
+                           "mean": float(lat.mean()), "max": float(lat.max()),
+                           "from": "submission", "to": "H/S CSR of every symmetry written (device event)"},
+            "one_electron_cache": {"entries": cache[0], "bytes": cache[1], "hits": cache[2], "misses": cache[3]}}
 
 
 def blocks_benchmark(rk, w, t, max_l=2, L=0, keep=None):
@@ -446,7 +475,7 @@ def main():
     ap.add_argument("--blockset", type=int, default=1,
                     help="also time all symmetries L=0..4 (l <= 4) of the c3 basis in one launch (0 skips it)")
     ap.add_argument("--dipoles", type=int, default=1, help="also time the L=0 -> L=1 dipole block (0 skips it)")
-    ap.add_argument("--serve-jobs", type=int, default=256,
+    ap.add_argument("--serve-jobs", type=int, default=64,
                     help="jobs of the serving stream (differing bases through the build service); 0 skips it")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size 3 times on the build stream "

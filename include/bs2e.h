@@ -39,7 +39,7 @@ extern "C" {
 /* Mixed precision: the off-diagonal moments (pair densities x radial moments,
  * mat_els.f90:413-428) on the FP32 matrix cores from exact 10-bit integer
  * slices of both operands with power-of-two scales, recombined in FP64 (about
- * 2^-41 of the largest term of a moment's row and column); de Boor values,
+ * 2^-51 of the largest term of a moment's row and column); de Boor values,
  * diagonal cells, window sums and the tensor's products stay FP64.
  * Orders <= 16 with k_GL <= 16. */
 #define BS2E_PREC_MIXED 1
@@ -310,14 +310,62 @@ const char *bs2e_server_last_error(const bs2e_server *srv);
 /* Device bytes one build of this basis allocates (tables + tensor). */
 int64_t bs2e_footprint(int64_t k, int64_t n_knots, const double *knots, int64_t k_gl, int64_t max_k, int64_t *bytes);
 /* Queues a job and admits what fits; returns its ticket (>= 0) or a
- * negative status (BS2E_ENOMEM when the job alone exceeds the budget). */
+ * negative status (BS2E_ENOMEM when the job alone exceeds the budget).  The
+ * job is the R^k build alone (plain FP64); see bs2e_server_submit_job. */
 int64_t bs2e_server_submit(bs2e_server *srv, int64_t k, int64_t n_knots, const double *knots, int64_t k_gl,
                            int64_t max_k);
+
+/* One request of the service: the two-electron branch of basis_setup
+ * (main_basis_setup.f90:75-115) for one basis: the R^k tensor, the
+ * one-electron matrices S and H(l) (mat_els.f90:47-118; hydrogenic charge Z,
+ * absorbing potential -i eta (r - cap_r0)^cap_order) and the H/S CSR blocks
+ * of n_sym symmetries (hamiltonian.f90:106-283), as bs2e_blockset_create /
+ * _fill describe them.  n_sym = 0: the tensor only.  The arrays are copied at
+ * submission. */
+typedef struct bs2e_job {
+    int64_t k;                  /* B-spline order */
+    int64_t n_knots;
+    const double *knots;
+    int64_t k_gl;               /* GL points per knot interval */
+    int64_t max_k;              /* multipoles 0 .. max_k */
+    int64_t precision;          /* BS2E_PREC_FP64 or BS2E_PREC_MIXED */
+    int64_t n_sym;              /* symmetry blocks */
+    const int64_t *L;           /* [n_sym] total L of each symmetry */
+    const int64_t *n_config;    /* [n_sym] configurations of each symmetry */
+    const int64_t *configs;     /* 5 int64 per configuration (n1, n2, l1, l2, eqv), symmetry after symmetry */
+    int64_t full;               /* 0: upper triangles only */
+    int64_t Z;                  /* nuclear charge of the one-particle Hamiltonian */
+    int64_t max_l_1p;           /* H(l) for l = 0 .. max_l_1p; every configuration's l within it */
+    int64_t cap_order;
+    double cap_r0, cap_eta_re, cap_eta_im;
+} bs2e_job;
+
+/* Queues a job of the two-electron branch and admits what fits: a job is
+ * admitted when a slot is free and its footprint fits (tensor and tables at
+ * first; the blocks' exact sizes once its configurations are counted, a pass
+ * of a fraction of a millisecond at admission).  Its tensor build, the
+ * one-electron matrices and the block fill run one after the other on the
+ * slot's streams, beside the other admitted jobs.  The spline tables and
+ * one-electron matrices of a basis stay on the device in the service's cache
+ * (keyed by knots, GL rule, charge and absorbing potential) and are reused by
+ * later jobs on the same basis; entries no running job holds are evicted,
+ * least recently used first, when a job needs their memory.  Completion (the
+ * status, finished and latency of a job) is the H/S CSR of every symmetry
+ * written. */
+int64_t bs2e_server_submit_job(bs2e_server *srv, const bs2e_job *job);
+/* A finished job's blocks (read them with bs2e_blockset_sizes, _get,
+ * _get_rows); they belong to the service and stay valid until the job is
+ * retired.  BS2E_ESTATE for a job without blocks. */
+int64_t bs2e_server_blocks(bs2e_server *srv, int64_t ticket, bs2e_blockset **set);
+/* info[0..3] = cache entries, their device bytes, admissions that found their
+ * basis in the cache, admissions that computed it. */
+int64_t bs2e_server_cache(const bs2e_server *srv, int64_t *info, int64_t n);
 /* Retires finished builds and admits queued jobs; returns the number of
  * jobs still queued or running. */
 int64_t bs2e_server_progress(bs2e_server *srv);
 /* state: 0 queued, 1 running, 2 done, 3 failed; times[0] = admission and
- * times[1] = completion, ms after submission (-1 when not yet). */
+ * times[1] = completion (the job's last kernel done: the tensor, or the H/S
+ * blocks of a job with blocks), ms after submission (-1 when not yet). */
 int64_t bs2e_server_status(bs2e_server *srv, int64_t ticket, int64_t *state, double *times);
 /* Blocks until the job is done, admitting others meanwhile. */
 int64_t bs2e_server_wait(bs2e_server *srv, int64_t ticket);

@@ -23,8 +23,11 @@ def test_serving_jobs_have_the_drawn_sizes():
     sys.path.insert(0, ROOT)
     import bench
     import bs2e
-    jobs = bench.serving_jobs(64, 20261202)
-    again = bench.serving_jobs(64, 20261202)
+    terms = bench.branch_terms(120, 8, 3)
+    assert [tm.l for tm in terms] == [0, 1, 2] and all(tm.n_config > 0 for tm in terms)
+    from bs2e.workloads import serving_jobs
+    jobs = [(j.k, j.t, j.k_GL, j.max_k) for j in serving_jobs(64, 20261202)]
+    again = [(j.k, j.t, j.k_GL, j.max_k) for j in serving_jobs(64, 20261202)]
     assert len(jobs) == 64
     for (k, t, k_gl, max_k), (k2, t2, _, _) in zip(jobs, again):
         assert k == k2 and np.array_equal(t, t2)            # deterministic for a seed

@@ -204,3 +204,74 @@ def test_service_serving_stream_jobs_equal_alone(ctx, oracle):
     for got, exact in ((slater(0, s1, s1, s1, s1), 5 * Z / 8), (slater(1, s1, p2, p2, s1), 112 * Z / 2187),
                        (slater(2, p2, p2, p2, p2), 45 * Z / 512)):
         assert abs(got - exact) < 1e-9 * exact, (got, exact)
+
+
+def branch_terms(n_b, k, lmax, max_L=2, l2=1):
+    """The symmetries of a basis_setup request: L = 0..max_L with natural
+    parity (z_pol), l1 <= lmax, the second electron's n2 <= n_b // 3 (an
+    r_2_max truncation) and its l <= l2 beyond n1 = n_b // 2 (r_all_l)."""
+    terms = []
+    for L in range(max_L + 1):
+        term = bs2e.sym(l=L, pi=bool(L % 2))
+        term.configs = bs2e.count_configs(term, lmax, n_b, k, n_b // 3, n_b // 2, min(l2, lmax))
+        if term.n_config:
+            terms.append(term)
+    return terms
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_service_whole_branch_equals_alone(ctx, precision):
+    """Jobs of the serving distribution as whole two-electron branches (tensor,
+    one-electron matrices with an absorbing potential, the H/S blocks of
+    L = 0, 1, 2), all submitted at once to a service with three slots: every
+    job's tensor and every symmetry's CSR arrays equal the same job built
+    alone through the single-build calls, bit for bit.  A second job on the
+    first job's knots (fewer multipoles and l's) takes its one-electron tables
+    from the service's cache."""
+    from bs2e.workloads import serving_jobs
+    picked = [j for j in serving_jobs(40, 7) if len(j.t) - j.k - 2 <= 160][:4]
+    assert len(picked) == 4
+    first = picked[0]
+    specs = [(j.k, j.t, j.k_GL, j.max_k, j.lmax) for j in picked]
+    specs.append((first.k, first.t, first.k_GL, max(0, first.max_k - 2), max(1, first.lmax - 1)))
+    cap = (2, 0.0, 1e-3)
+    srv = bs2e.Server(32 << 30, slots=3)
+    tickets, terms_of = [], []
+    for k, t, kgl, mk, lmax in specs:
+        n_b = len(t) - k - 2
+        terms = branch_terms(n_b, k, lmax)
+        terms_of.append(terms)
+        c = (cap[0], 0.8 * t[-1], cap[2])
+        tickets.append(srv.submit_job(k, t, kgl, mk, precision=precision, terms=terms, full=False, Z=2,
+                                      max_l_1p=lmax, CAP_c=c))
+    print("whole-branch jobs (order, n_b, max_k, lmax, rows):",
+          [(k, len(t) - k - 2, mk, lm, sum(tm.n_config for tm in tr)) for (k, t, _, mk, lm), tr in
+           zip(specs, terms_of)], flush=True)
+    got = []
+    for tk, terms in zip(tickets, terms_of):
+        srv.wait(tk)
+        st, t_admit, t_done = srv.status(tk)
+        assert st == bs2e.Server.DONE and 0.0 <= t_admit <= t_done
+        bset = srv.blocks(tk, terms)
+        got.append((srv.tensor(tk).to_host(), [bset.get(i) for i in range(len(terms))]))
+    entries, nbytes, hits, misses = srv.cache_info()
+    assert hits >= 1 and misses >= 1 and entries == misses and nbytes > 0
+    for tk in tickets:
+        srv.retire(tk)
+    assert srv.memory()[1] == nbytes    # only the cache stays
+    srv.close()
+    for (k, t, kgl, mk, lmax), terms, (R, blocks) in zip(specs, terms_of, got):
+        spl = bs2e.b_spline(k, t)
+        rk = bs2e.RkTensor.build(k, t, kgl, mk, ctx=ctx, precision=precision)
+        assert np.array_equal(rk.to_host(), R)
+        S = bs2e.setup_S(spl, kgl, ctx=ctx)
+        Hl = bs2e.setup_H_one_particle(2, (cap[0], 0.8 * t[-1], cap[2]), range(lmax + 1), spl, kgl, ctx=ctx)
+        alone = bs2e.BlockSet(rk, terms, False)
+        alone.fill(list(Hl), S)
+        for i, (H_sp, S_sp) in enumerate(blocks):
+            H1, S1 = alone.get(i)
+            for a, b in ((H_sp, H1), (S_sp, S1)):
+                assert np.array_equal(a.index_ptr, b.index_ptr) and np.array_equal(a.indices, b.indices)
+                assert np.array_equal(a.data, b.data)
+        alone.release()
+        rk.release()
