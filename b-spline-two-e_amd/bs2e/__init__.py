@@ -643,6 +643,21 @@ class BlockSet:
                                            ptr(ipS, I), ptr(ixS, I), dS.ctypes.data_as(D)), self.ctx.handle)
         return (CSR_matrix((n, n), ipH, ixH[:nH], dH[:nH]), CSR_matrix((n, n), ipS, ixS[:nS], dS[:nS]))
 
+    def refill(self):
+        """The fill again from the device copy of the last fill's one-electron matrices (waits)."""
+        check(capi.lib().bs2e_blockset_refill(self.handle), self.ctx.handle)
+
+    def set_graph(self, on=True):
+        """Captures the whole branch (tensor build, count, row offsets, fill) as one hipGraph."""
+        check(capi.lib().bs2e_blockset_set_graph(self.handle, int(bool(on))), self.ctx.handle)
+
+    def replay(self):
+        """One launch of the captured branch (asynchronous; synchronize() waits)."""
+        check(capi.lib().bs2e_blockset_replay(self.handle), self.ctx.handle)
+
+    def synchronize(self):
+        check(capi.lib().bs2e_blockset_synchronize(self.handle), self.ctx.handle)
+
     def row_offsets(self, i, row_lo, n_rows):
         """index_ptr of H and S for rows row_lo .. row_lo+n_rows-1 of symmetry
         i, relative to the window (1-based); nothing else is copied."""

@@ -71,6 +71,10 @@ SIGNATURES = [
     ("bs2e_blockset_fill", i64, [vp, i64, P(dbl), P(dbl)]),
     ("bs2e_blockset_get", i64, [vp, i64, P(i64), P(i64), P(dbl), P(i64), P(i64), P(dbl)]),
     ("bs2e_blockset_get_rows", i64, [vp, i64, i64, i64, P(i64), P(i64), P(dbl), P(i64), P(i64), P(dbl)]),
+    ("bs2e_blockset_refill", i64, [vp]),
+    ("bs2e_blockset_set_graph", i64, [vp, i64]),
+    ("bs2e_blockset_replay", i64, [vp]),
+    ("bs2e_blockset_synchronize", i64, [vp]),
     ("bs2e_blockset_times", i64, [vp, P(dbl), i64]),
     ("bs2e_blockset_release", i64, [vp]),
     ("bs2e_setup_r_k_int", i64, [vp, i64, i64, P(dbl), i64, i64, dbl, P(dbl)]),

@@ -213,6 +213,11 @@ hipError_t launch_radial_dipole(const Plan& p, const DevTables& d, double* B, do
 hipError_t launch_block_count(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, hipStream_t s);
 hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, int ks, int K,
                              hipStream_t s);
+// Exclusive prefix sums of the count pass's per-row counts (cnt: H counts then
+// S counts, n_rows each) into ptr (same layout), on the device; tmp holds
+// row_scan_tmp_elems(n_rows) int64.
+int64_t row_scan_tmp_elems(int64_t n_rows);
+hipError_t launch_row_scan(const int64_t* cnt, int64_t* ptr, int64_t n_rows, int64_t* tmp, hipStream_t s);
 // The fill's operand layouts: the caller's column-major one-electron matrices
 // (n_l of them, then S) to row-major, and the window totals [n_pairs][K] to
 // [K][n_pairs].

@@ -408,6 +408,8 @@ static int64_t run_build(bs2e_ctx* ctx, bs2e_rk* rk)
     return BS2E_OK;
 }
 
+int64_t bs2e::rk_enqueue_build(bs2e_ctx* ctx, bs2e_rk* rk) { return run_build(ctx, rk); }
+
 extern "C" {
 
 int64_t bs2e_slater_primitives(bs2e_ctx* ctx, int64_t k, int64_t n_knots, const double* knots, int64_t k_gl,

@@ -134,15 +134,31 @@ struct bs2e_blockset {
     double* d_T = nullptr;                     // the fill's layouts: Hl, S row-major; Ptot, Qtot [K][n_pairs]
     int64_t *d_ixH = nullptr, *d_ixS = nullptr;
     double *d_dH = nullptr, *d_dS = nullptr;
+    // the whole branch as one captured graph (bs2e_blockset_set_graph)
+    BlockArgs* d_tabg = nullptr;   // count and fill descriptors in one table
+    int64_t* d_scan = nullptr;     // the row-offset scan's tile totals
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
 };
 
 // A single symmetry's block is a set of one.
 struct bs2e_block : bs2e_blockset {};
 
+static void set_drop_graph(bs2e_blockset* b)
+{
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+    ctx_free(b->rk->ctx, b->d_tabg);
+    ctx_free(b->rk->ctx, b->d_scan);
+}
+
 // A set's device memory comes from its tensor's context: the context's arena
 // when it has one (a build service's slot), else hipMalloc.
 static void set_drop_outputs(bs2e_blockset* b)
 {
+    set_drop_graph(b);   // its nodes address these buffers
     bs2e_ctx* c = b->rk->ctx;
     ctx_free(c, b->d_Hl);
     ctx_free(c, b->d_S);
@@ -357,6 +373,7 @@ int64_t bs2e::set_fill_alloc(bs2e_blockset* b, int64_t n_l, const char* who)
     const size_t nn = (size_t)p.n_b * p.n_b;
     const size_t nf = (size_t)p.n_pairs * p.K;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
+    set_drop_graph(b);   // its nodes address the buffers reallocated here
     ctx_free(ctx, b->d_T);
     ctx_free(ctx, b->d_ixH);
     ctx_free(ctx, b->d_dH);
@@ -686,6 +703,90 @@ int64_t bs2e_blockset_get_rows(const bs2e_blockset* b, int64_t sym, int64_t row_
         HIP_TRY(ctx, hipMemcpyAsync(d_S, b->d_dS + 2 * aS, sizeof(double) * 2 * nS, hipMemcpyDeviceToHost, st));
     }
     HIP_TRY(ctx, hipStreamSynchronize(st));
+    return BS2E_OK;
+}
+
+int64_t bs2e_blockset_refill(bs2e_blockset* b)
+{
+    if (!b) return fail(nullptr, BS2E_EINVAL, "bs2e_blockset_refill: NULL argument");
+    if (!b->filled || !b->d_Hl || !b->d_S)
+        return fail(b->rk->ctx, BS2E_ESTATE, "bs2e_blockset_refill: fill the set from host matrices first");
+    return set_fill_enqueue(b, b->d_Hl, b->d_S, true, "bs2e_blockset_refill");
+}
+
+int64_t bs2e_blockset_set_graph(bs2e_blockset* b, int64_t on)
+{
+    if (!b) return fail(nullptr, BS2E_EINVAL, "bs2e_blockset_set_graph: NULL argument");
+    bs2e_rk* rk = b->rk;
+    bs2e_ctx* ctx = rk->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    set_drop_graph(b);
+    if (!on) return BS2E_OK;
+    if (!b->filled || !b->d_Hl || !b->d_S)
+        return fail(ctx, BS2E_ESTATE, "bs2e_blockset_set_graph: fill the set from host matrices first");
+    if (rk->use_graph || rk->profiling || rk->from_primitives)
+        return fail(ctx, BS2E_ESTATE,
+                    "bs2e_blockset_set_graph: the tensor must build from its knots, without its own graph or profiling");
+    const Plan& p = rk->plan;
+    const int64_t N = b->n_rows;
+    std::vector<BlockArgs> tg((size_t)b->n_sym);
+    for (int64_t s = 0; s < b->n_sym; ++s) {
+        const BlockArgs c = set_args(b, s, false);
+        tg[s] = set_args(b, s, true);
+        tg[s].cntH = c.cntH;
+        tg[s].cntS = c.cntS;
+        tg[s].refH = c.refH;
+        tg[s].refS = c.refS;
+    }
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_tabg, (size_t)b->n_sym));
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_scan, (size_t)row_scan_tmp_elems(N)));
+    HIP_TRY(ctx, hipMemcpy(b->d_tabg, tg.data(), sizeof(BlockArgs) * tg.size(), hipMemcpyHostToDevice));
+    const size_t nn = (size_t)p.n_b * p.n_b, nf = (size_t)p.n_pairs * p.K;
+    hipStream_t st = ctx->stream;
+    clear_stale_error();
+    HIP_TRY(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int64_t rc = rk_enqueue_build(ctx, rk);
+    hipError_t e = rc ? hipErrorLaunchFailure : hipSuccess;
+    if (e == hipSuccess) e = launch_block_count(b->d_tabg, b->d_row0, (int)b->n_sym, N, st);
+    if (e == hipSuccess) e = launch_row_scan(b->d_cnt, b->d_ptr, N, b->d_scan, st);
+    if (e == hipSuccess)
+        e = launch_block_prepare(b->d_Hl, b->d_S, (int)b->n_l, (int)p.n_b, rk->d.Ptot, rk->d.Qtot, (int)p.n_pairs,
+                                 (int)p.K, b->d_T, b->d_T + 2 * nn * b->n_l, b->d_T + 2 * nn * (b->n_l + 1),
+                                 b->d_T + 2 * nn * (b->n_l + 1) + nf, st);
+    if (e == hipSuccess) e = launch_block_fill(b->d_tabg, b->d_row0, (int)b->n_sym, N, (int)p.k_s, (int)p.K, st);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &g);
+    if (e == hipSuccess) e = ec;
+    if (e == hipSuccess) {
+        b->graph = g;
+        e = hipGraphInstantiateWithFlags(&b->graph_exec, b->graph, 0);
+    } else if (g) {
+        (void)hipGraphDestroy(g);
+    }
+    if (e != hipSuccess) {
+        set_drop_graph(b);
+        return rc ? rc : fail(ctx, BS2E_EDEVICE, std::string("bs2e_blockset_set_graph: ") + hipGetErrorString(e));
+    }
+    return BS2E_OK;
+}
+
+int64_t bs2e_blockset_replay(bs2e_blockset* b)
+{
+    if (!b) return fail(nullptr, BS2E_EINVAL, "bs2e_blockset_replay: NULL argument");
+    bs2e_ctx* ctx = b->rk->ctx;
+    if (!b->graph_exec) return fail(ctx, BS2E_ESTATE, "bs2e_blockset_replay: no graph (bs2e_blockset_set_graph)");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipGraphLaunch(b->graph_exec, ctx->stream));
+    b->rk->computed = true;
+    b->rk->mirror_valid = false;
+    b->filled = true;
+    return BS2E_OK;
+}
+
+int64_t bs2e_blockset_synchronize(bs2e_blockset* b)
+{
+    if (!b) return fail(nullptr, BS2E_EINVAL, "bs2e_blockset_synchronize: NULL argument");
+    HIP_TRY(b->rk->ctx, hipStreamSynchronize(b->rk->ctx->stream));
     return BS2E_OK;
 }
 

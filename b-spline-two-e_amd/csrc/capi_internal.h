@@ -135,6 +135,10 @@ size_t rk_footprint(const Plan& p);
 std::string channel_tables(const int64_t* configs, int64_t n_config, int64_t nb, std::vector<BlockChan>& chan,
                            std::vector<BlockSeg>& seg, std::vector<int32_t>& cfg_chan, int64_t& max_l);
 
+// Enqueues the two-stream tensor build of rk on ctx's streams (also under
+// stream capture: events and kernels only).
+int64_t rk_enqueue_build(bs2e_ctx* ctx, bs2e_rk* rk);
+
 // The kernels' size limits for a plan (BS2E_EINVAL with a message when one is
 // exceeded); needs_primitives: the plan's primitives are computed on the device.
 int64_t check_plan_limits(bs2e_ctx* ctx, const Plan& p, bool needs_primitives);

@@ -127,28 +127,6 @@ __device__ __forceinline__ double r12_row(const BA& A, int a1, int c1, int a2, i
     return kdot3(x, np, y, np, coef, K);
 }
 
-// The same sum for a row whose windows never meet (|n_a - n_b| >= k): every
-// product is rank-one with a known side; x and y are the p and q factors in
-// the wavefront's LDS copy, K consecutive multipoles from a 16-byte boundary,
-// as is coef, so that one LDS read brings two multipoles.
-__device__ __forceinline__ double r12_staged(const double* __restrict__ x, const double* __restrict__ y,
-                                             const double* __restrict__ coef, int K)
-{
-    double t = 0.0;
-    int k = 0;
-    for (; k + 4 <= K; k += 4) {
-        const dbl2 xa = *(const dbl2*)(x + k), xb = *(const dbl2*)(x + k + 2);
-        const dbl2 ya = *(const dbl2*)(y + k), yb = *(const dbl2*)(y + k + 2);
-        const dbl2 ca = *(const dbl2*)(coef + k), cb = *(const dbl2*)(coef + k + 2);
-        t = t + (xa.x * ya.x) * ca.x;
-        t = t + (xa.y * ya.y) * ca.y;
-        t = t + (xb.x * yb.x) * cb.x;
-        t = t + (xb.y * yb.y) * cb.y;
-    }
-    for (; k < K; ++k) t = t + (x[k] * y[k]) * coef[k];
-    return t;
-}
-
 // element (r, c), 1-based, of a complex row-major n_b x n_b matrix
 __device__ __forceinline__ Cplx mat(const double* M, int n_b, int r, int c)
 {
@@ -164,14 +142,43 @@ __device__ __forceinline__ int64_t uniform64(int64_t x)
     return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
 }
 
+// sum_k coef[k] R^k[p][q] for a row whose windows never meet (|n_a - n_b| >=
+// k): every product is rank-one with a known side; xs holds the p factors
+// with the coefficients folded in (xs[k] = x[k] coef[k], formed once per
+// channel) and y the q factors, both in the wavefront's LDS copy, K
+// consecutive multipoles from a 16-byte boundary, so that one LDS read brings
+// two multipoles: one product and one sum per multipole.  (x coef) y rounds
+// differently from the reference's (x y) coef in the last bits of each term.
+__device__ __forceinline__ double r12_scaled(const double* __restrict__ xs, const double* __restrict__ y, int K)
+{
+    double t = 0.0;
+    int k = 0;
+    for (; k + 4 <= K; k += 4) {
+        const dbl2 xa = *(const dbl2*)(xs + k), xb = *(const dbl2*)(xs + k + 2);
+        const dbl2 ya = *(const dbl2*)(y + k), yb = *(const dbl2*)(y + k + 2);
+        t = t + xa.x * ya.x;
+        t = t + xa.y * ya.y;
+        t = t + xb.x * yb.x;
+        t = t + xb.y * yb.y;
+    }
+    for (; k < K; ++k) t = t + xs[k] * y[k];
+    return t;
+}
+
 // LDS of one fill wavefront (doubles), in rows of KP = K rounded up to even:
 // the channel pair's direct and exchange coefficients (2 rows), then the q
 // factors of a row whose windows never meet, [2][L][KP] with L = 2k - 1:
 // direct term pairs (d, n_b), exchange term pairs (d, n_a), for the d within
-// k of n_b / n_a.
+// k of n_b / n_a; then the same rows times the channel's coefficients,
+// [2][L][KP]: the direct term's p factors (the exchange run times the direct
+// coefficients) and the exchange term's (the direct run times the exchange
+// coefficients).
 __host__ __device__ constexpr int fill_kp(int K) { return (K + 1) & ~1; }
 // (plus 64 bytes for the channel walk's marks)
-__host__ __device__ constexpr int fill_wave_lds(int ks, int K) { return 2 * fill_kp(K) * (1 + (2 * ks - 1)) + 8; }
+__host__ __device__ constexpr int fill_wave_lds(int ks, int K)
+{
+    return 2 * fill_kp(K) * (1 + 2 * (2 * ks - 1)) + 8;
+}
 
 // One wavefront per configuration row of any of n_blocks symmetry blocks:
 // global row g belongs to the last block b with row0[b] <= g and is row
@@ -187,10 +194,11 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
     extern __shared__ __align__(16) double lds_fill[];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
-    const int64_t g = (int64_t)blockIdx.x * (kCThreads / kWave) + wave;
+    const int64_t g = (int64_t)blockIdx.x * (blockDim.x / kWave) + wave;   // 1, 2 or 4 wavefronts per workgroup
     if (g >= n_rows) return;   // whole wave exits together
     double* sc = lds_fill + (size_t)wave * fill_wave_lds(lds_ks, lds_K);   // coefficients: direct, exchange
     double* sy = sc + 2 * fill_kp(lds_K);                                  // staged q factors
+    double* sx = sy + 2 * (2 * lds_ks - 1) * fill_kp(lds_K);               // staged p factors x coefficients
     unsigned char* own = (unsigned char*)(sc + fill_wave_lds(lds_ks, lds_K) - 8);   // walk marks
     int blo = 0, bhi = n_blocks - 1;
     while (blo < bhi) {
@@ -258,6 +266,15 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (staged) {   // p factors times the coefficients, once per channel
+                for (int e = lane; e < 2 * L * K; e += kWave) {
+                    const int t = e >= L * K, r = (e - t * L * K) / K, k = e - t * L * K - r * K;
+                    sx[(t * L + r) * KP + k] = t ? sy[r * KP + k] * sc[KP + k] : sy[(L + r) * KP + k] * sc[k];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
         }
         const double* cd = sc;
         const double* ce = sc + KP;
@@ -281,15 +298,14 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
                         // R(a,b,c,d) = R(a,d,c,b): the key [n_a, n_d, n_c, n_b] puts the
                         // wavefront's consecutive n_d on consecutive columns of one row
                         if (support)
-                            res = res + (staged ? r12_staged(sy + (L + n_c - dlo_ex) * KP, sy + (n_d - dlo_dir) * KP,
-                                                                     cd, K)
-                                                        : r12_row(A, n_a, n_c, n_d, n_bb, cd));
+                            res = res + (staged ? r12_scaled(sx + (n_c - dlo_ex) * KP, sy + (n_d - dlo_dir) * KP, K)
+                                                : r12_row(A, n_a, n_c, n_d, n_bb, cd));
                         // exchange R(a,b,d,c) = R(b,a,c,d) = R(b,d,c,a): row pair(n_b, n_c),
                         // consecutive n_d on consecutive columns
                         if (support_ex)
-                            res = res + (staged ? r12_staged(sy + (n_c - dlo_dir) * KP, sy + (L + n_d - dlo_ex) * KP,
-                                                                     ce, K)
-                                                        : r12_row(A, n_bb, n_c, n_d, n_a, ce));
+                            res = res + (staged ? r12_scaled(sx + (L + n_c - dlo_dir) * KP, sy + (L + n_d - dlo_ex) * KP,
+                                                             K)
+                                                : r12_row(A, n_bb, n_c, n_d, n_a, ce));
                         h.re = res;
                     }
                     if (one_el) {
@@ -453,15 +469,122 @@ hipError_t launch_block_count(const BlockArgs* tab, const int64_t* row0, int n_b
     return hipGetLastError();
 }
 
+// Exclusive prefix sums of the count pass's row counts on the device (the
+// fill's row offsets: ptr[r] = sum of cnt[0 .. r) for H, and at ptr + n for
+// S), so that a captured graph can count and fill without the host: tiles of
+// kScanTile rows are summed and scanned in place (kc_scan_tiles), the tile
+// totals scanned by one workgroup (kc_scan_totals), and each tile's offset
+// added (kc_scan_add).  tmp holds 2 * ceil(n / kScanTile) int64.
+constexpr int kScanTile = 1024;
+
+__device__ __forceinline__ int64_t wave_incl_scan(int64_t v, int lane)
+{
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int64_t y = __shfl_up(v, off, kWave);
+        if (lane >= off) v += y;
+    }
+    return v;
+}
+
+// workgroup-wide exclusive scan of one value per thread (256 threads)
+__device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t* sw, int64_t* total)
+{
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int64_t incl = wave_incl_scan(v, lane);
+    if (lane == kWave - 1) sw[wave] = incl;
+    __syncthreads();
+    int64_t before = 0, all = 0;
+    for (int w = 0; w < kCThreads / kWave; ++w) {
+        before += w < wave ? sw[w] : 0;
+        all += sw[w];
+    }
+    __syncthreads();
+    *total = all;
+    return before + incl - v;
+}
+
+__global__ __launch_bounds__(kCThreads) void kc_scan_tiles(const int64_t* __restrict__ cnt, int64_t* __restrict__ ptr,
+                                                           int64_t n, int64_t* __restrict__ tile_tot)
+{
+    __shared__ int64_t sw[kCThreads / kWave];
+    const int64_t n_tiles = (n + kScanTile - 1) / kScanTile;
+    const int which = (int)(blockIdx.x / n_tiles);        // 0: H, 1: S
+    const int64_t tile = blockIdx.x - which * n_tiles;
+    const int64_t* c = cnt + which * n;
+    int64_t* o = ptr + which * n;
+    const int per = kScanTile / kCThreads;                 // rows per thread
+    const int64_t r0 = tile * kScanTile + (int64_t)threadIdx.x * per;
+    int64_t v[kScanTile / kCThreads];
+    int64_t mine = 0;
+#pragma unroll
+    for (int j = 0; j < per; ++j) {
+        v[j] = r0 + j < n ? c[r0 + j] : 0;
+        mine += v[j];
+    }
+    int64_t total = 0;
+    int64_t run = block_excl_scan(mine, sw, &total);
+#pragma unroll
+    for (int j = 0; j < per; ++j) {
+        if (r0 + j < n) o[r0 + j] = run;
+        run += v[j];
+    }
+    if (threadIdx.x == 0) tile_tot[which * n_tiles + tile] = total;
+}
+
+__global__ __launch_bounds__(kCThreads) void kc_scan_totals(int64_t* __restrict__ tile_tot, int64_t n_tiles)
+{
+    __shared__ int64_t sw[kCThreads / kWave];
+    for (int which = 0; which < 2; ++which) {
+        int64_t* t = tile_tot + which * n_tiles;
+        int64_t carry = 0;
+        for (int64_t base = 0; base < n_tiles; base += kCThreads) {
+            const int64_t e = base + threadIdx.x;
+            const int64_t v = e < n_tiles ? t[e] : 0;
+            int64_t total = 0;
+            const int64_t ex = block_excl_scan(v, sw, &total);
+            if (e < n_tiles) t[e] = carry + ex;
+            carry += total;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kCThreads) void kc_scan_add(int64_t* __restrict__ ptr, int64_t n,
+                                                         const int64_t* __restrict__ tile_tot)
+{
+    const int64_t n_tiles = (n + kScanTile - 1) / kScanTile;
+    const int64_t e = (int64_t)blockIdx.x * kCThreads + threadIdx.x;
+    if (e >= 2 * n) return;
+    const int which = e >= n;
+    const int64_t r = e - which * n;
+    ptr[e] += tile_tot[which * n_tiles + r / kScanTile];
+}
+
+int64_t row_scan_tmp_elems(int64_t n_rows) { return 2 * ((n_rows + kScanTile - 1) / kScanTile); }
+
+hipError_t launch_row_scan(const int64_t* cnt, int64_t* ptr, int64_t n_rows, int64_t* tmp, hipStream_t s)
+{
+    if (n_rows <= 0) return hipSuccess;
+    const int64_t n_tiles = (n_rows + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(kc_scan_tiles, dim3((unsigned)(2 * n_tiles)), dim3(kCThreads), 0, s, cnt, ptr, n_rows, tmp);
+    hipLaunchKernelGGL(kc_scan_totals, dim3(1), dim3(kCThreads), 0, s, tmp, n_tiles);
+    hipLaunchKernelGGL(kc_scan_add, dim3((unsigned)((2 * n_rows + kCThreads - 1) / kCThreads)), dim3(kCThreads), 0, s,
+                       ptr, n_rows, tmp);
+    return hipGetLastError();
+}
+
 hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, int ks, int K,
                              hipStream_t s)
 {
-    const int64_t threads = n_rows * kWave;
-    const size_t lds = sizeof(double) * (size_t)(kCThreads / kWave) * (size_t)fill_wave_lds(ks, K);
-    hipError_t e = allow_lds((const void*)kc_block, lds);   // <= 4 x 32 KiB (k <= 16, K <= 64)
+    // four wavefronts per workgroup where their LDS fits, else two or one
+    // (one wavefront takes at most 65 KiB: k <= 16, K <= 64)
+    const size_t per = sizeof(double) * (size_t)fill_wave_lds(ks, K);
+    int waves = kCThreads / kWave;
+    while (waves > 1 && per * (size_t)waves > kLdsPerWorkgroup) waves >>= 1;
+    const size_t lds = per * (size_t)waves;
+    hipError_t e = allow_lds((const void*)kc_block, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kc_block, dim3((unsigned)((threads + kCThreads - 1) / kCThreads)), dim3(kCThreads), lds,
-                       s, tab, row0, n_blocks, n_rows, ks, K);
+    hipLaunchKernelGGL(kc_block, dim3((unsigned)((n_rows + waves - 1) / waves)), dim3(waves * kWave), lds, s, tab,
+                       row0, n_blocks, n_rows, ks, K);
     return hipGetLastError();
 }
 

@@ -19,7 +19,12 @@
 // multiplication and its applications", Numer. Algorithms 59, 2012).  With
 // NS = 5 and BETA = 10 the moments carry about 2^-51 of the largest term of
 // their row and column (with NS = 4, 2^-41, the element-wise error of the
-// tensor's smallest elements reached 1.4e-10 at the order-8, 200-spline basis).  The B-spline values (de Boor in FP64), the moments'
+// tensor's smallest elements reached 1.4e-10 at the order-8, 200-spline basis).
+// A moment's error relative to its own value grows with the spread of the
+// moment over the interval's GL points (the ratio of the column's scale to
+// the points where the pair density sits): the intervals next to r = 0, where
+// r^k spreads over more than 2^kMxRange, keep FP64 (with k <= 24 on an
+// order-9 grid, the first three intervals).  The B-spline values (de Boor in FP64), the moments'
 // scales and the recombination stay FP64, as do every later stage of the
 // build (window sums, diagonal cells, the tensor's products).
 #include "bs2e_internal.h"
@@ -31,6 +36,7 @@ constexpr int kMxThreads = 256;
 constexpr int kMxSlices = 5;     // NS
 constexpr int kMxBeta = 10;      // bits per slice
 constexpr int kMxQ = 16;         // GL points per contraction (k_GL <= 16, zero-padded)
+constexpr int kMxRange = 11;     // log2 of the largest spread of r^K over an interval's GL points taken on the matrix cores
 
 static constexpr int mx_tiles(int n) { return (n + 15) / 16; }
 
@@ -104,6 +110,32 @@ __global__ __launch_bounds__(kMxThreads) void kp_outer_mx(int K, int kgl, const 
         splines_at<KS>(t, i1, r, sB + q * KS, 1);
     }
     __syncthreads();
+    // Near r = 0 the moments r^k, r^-(k+1) span (r_max / r_min)^K over the
+    // interval's GL points (about 1e54 on the first interval of an order-9
+    // grid with k <= 24): beyond 2^11, one scale per column leaves the small
+    // points too few bits, and the interval's moments are summed in FP64 as
+    // the plain path sums them (the same operations, so the same bits).
+    if ((double)K * log2(sR[kgl - 1] / sR[0]) > (double)kMxRange) {
+        for (int e = tid; e < U * K; e += kMxThreads) {
+            const int u = e / K, k = e - u * K;
+            int ta = 0, base = 0;
+            while (base + (KS - ta) <= u) {
+                base += KS - ta;
+                ++ta;
+            }
+            const int tc = ta + (u - base);
+            double accp = 0.0, accq = 0.0;
+            for (int q = 0; q < kgl; ++q) {
+                const double bb = w[q] * sB[q * KS + ta] * sB[q * KS + tc];
+                const double r = sR[q];
+                accp = accp + bb * ipow_d(r, k);
+                accq = accq + bb * (1.0 / ipow_d(r, k + 1));
+            }
+            Pl[((size_t)v * U + u) * K + k] = h * accp;
+            Ql[((size_t)v * U + u) * K + k] = h * accq;
+        }
+        return;
+    }
     // slices: one thread per pair-density row, per moment column
     for (int e = tid; e < UP + 2 * KP; e += kMxThreads) {
         double col[kMxQ];

@@ -47,6 +47,7 @@ program bs2e_basis_setup
     type(Nd_DOK) :: R_direct
     type(block), allocatable :: H_vec(:)
     complex(dp), allocatable :: S(:,:)
+    complex(dp), allocatable :: eigs_v(:,:)   ! main_basis_setup.f90:88 passes one-particle energies to bas%init
     type(sym) :: term
     type(CSR_matrix) :: H_sp, S_sp
     type(hydrogenic) :: pot
@@ -164,7 +165,10 @@ program bs2e_basis_setup
     if (command_argument_count() >= 16) then
         call get_command_argument(15, arg); read(arg, *) max_L_all
         call get_command_argument(16, res_dir)
-        call bas%init(max_L_all, max_l, splines%n_b, splines%k, splines%n_b, splines%n_b, max_l, .true., .true.)
+        allocate(eigs_v(splines%n_b, 0:max_l))
+        eigs_v = (0.0_dp, 0.0_dp)
+        call bas%init(max_L_all, max_l, splines%n_b, splines%k, splines%n_b, splines%n_b, max_l, .true., .true., &
+                      eigs_v)
         call H_diag%init(bas%n_sym)
         call S_diag%init(bas%n_sym)
         call construct_blocks_tensor(H_vec, S, splines, bas%syms, max_k, R_p, H_diag, S_diag, ifull /= 0)

@@ -1,8 +1,14 @@
 ! bs2e_mat_els.f90 -- Fortran drop-in for the two-electron radial path of
 ! edvinolo/b-spline-two-e, running on the MI355X through the C ABI.
 !
-! The public names and argument lists are the reference's, so that a caller
-! written against them (main_basis_setup.f90:75-118) compiles unchanged:
+! The public names and argument lists are the reference's (trailing arguments
+! the path does not read, such as basis%init's eigs, are accepted), so that a
+! caller written against them (main_basis_setup.f90:75-118) keeps its calls;
+! it changes its `use` lines to this module (which holds all of them, where the
+! reference spreads them over mat_els, bspline_tools, sparse_array_tools,
+! hamiltonian, orbital_tools, grid_tools, potentials and CAP_tools) and
+! declares its tensor class(Nd_DOK) (the reference: class(sparse_Nd) with an
+! Nd_DOK allocated into it, main_basis_setup.f90:27,84):
 !
 !   generate_grid(k, m, Z, h_max, r_max, grid)           grid_tools.f90:6-55
 !   b_spline%init(k, t), %find_max_n_b(x)                bspline_tools.f90:27-39, 364-373
@@ -353,10 +359,14 @@ contains
     ! orbital_tools.f90:263-343 init_basis, two-electron symmetries: (0,0,even),
     ! then per L either (L, 0, parity of L) (z_pol) or every (L, M, parity)
     ! with |M| even for even parity and odd for odd parity.
-    subroutine init_basis(this, max_L, max_l_1p, n_b, k_spline, max_n_b, n_all_l, l_2_max, z_pol, two_el)
+    ! eigs (the one-particle energies the reference passes on to count_configs,
+    ! which computes an energy from them and filters nothing with it,
+    ! orbital_tools.f90:169-182) is accepted and not read.
+    subroutine init_basis(this, max_L, max_l_1p, n_b, k_spline, max_n_b, n_all_l, l_2_max, z_pol, two_el, eigs)
         class(basis), intent(inout) :: this
         integer, intent(in) :: max_L, max_l_1p, n_b, k_spline, max_n_b, n_all_l, l_2_max
         logical, intent(in) :: z_pol, two_el
+        complex(dp), intent(in), optional :: eigs(:,:)
         integer :: l, m, p, ptr
         if (.not. two_el) error stop "basis%init: only the two-electron basis is built here"
         this%max_l_1p = max_l_1p

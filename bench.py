@@ -388,6 +388,57 @@ def blocks_benchmark(rk, w, t, max_l=2, L=0, keep=None):
             "wall_ms_with_transfers": wall * 1e3}
 
 
+def branch_benchmark(ctx, name="c2", max_l=2, max_L=2, steps=50, warmup=5):
+    """The two-electron branch of one basis, repeated: the tensor build and the
+    H/S blocks of L = 0..max_L (natural parity, l1, l2 <= max_l, upper
+    triangles), eager (the build, then the fill from the device copy of the
+    one-electron matrices, waited for) against one launch of the captured
+    graph of the whole branch (waited for); ms per branch."""
+    import torch
+
+    import bs2e
+    from bs2e.workloads import WORKLOADS, knots
+    w = WORKLOADS[name]
+    t = knots(w)
+    spl = bs2e.b_spline(w.k, t)
+    rk = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx)
+    S = bs2e.setup_S(spl, w.k_GL, ctx=ctx)
+    Hl = bs2e.setup_H_one_particle(w.Z, (2, 0.8 * w.r_max, 1e-3), range(max_l + 1), spl, w.k_GL, ctx=ctx)
+    terms = []
+    for L in range(max_L + 1):
+        term = bs2e.sym(l=L, pi=bool(L % 2))
+        term.configs = bs2e.count_configs(term, max_l, spl.n_b, w.k, spl.n_b, spl.n_b, max_l)
+        terms.append(term)
+    bset = bs2e.BlockSet(rk, terms, False)
+    bset.fill(list(Hl), S)
+    for _ in range(warmup):
+        rk.compute()
+        bset.refill()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rk.compute()
+        bset.refill()
+    eager = (time.perf_counter() - t0) * 1e3 / steps
+    bset.set_graph(True)
+    for _ in range(warmup):
+        bset.replay()
+        bset.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bset.replay()
+        bset.synchronize()
+    graph = (time.perf_counter() - t0) * 1e3 / steps
+    nH = int(bset.sizes[:, 0].sum())
+    bset.release()
+    rk.release()
+    return {"workload": f"{w.name} basis: tensor (k <= {w.max_k}) + H/S blocks of L = 0..{max_L}, l1,l2 <= {max_l}, "
+                        f"upper triangles ({sum(t_.n_config for t_ in terms)} rows, {nH} H entries)",
+            "eager_ms": eager, "graph_ms": graph, "steps": steps,
+            "eager": "build enqueued, then layouts + fill launched and waited for",
+            "graph": "one hipGraph launch: build, count, device row offsets, layouts, fill; waited for"}
+
+
 def blockset_benchmark(ctx, name="c3", max_l=4, max_L=4):
     """Channel summation of every symmetry L = 0..max_L (natural parity) of
     the named basis with l1, l2 <= max_l in one count and one fill launch
@@ -475,6 +526,8 @@ def main():
     ap.add_argument("--blockset", type=int, default=1,
                     help="also time all symmetries L=0..4 (l <= 4) of the c3 basis in one launch (0 skips it)")
     ap.add_argument("--dipoles", type=int, default=1, help="also time the L=0 -> L=1 dipole block (0 skips it)")
+    ap.add_argument("--branch", type=int, default=1,
+                    help="also time the c2 branch (tensor + L=0..2 blocks) eager against its captured graph (0 skips)")
     ap.add_argument("--serve-jobs", type=int, default=64,
                     help="jobs of the serving stream (differing bases through the build service); 0 skips it")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -619,6 +672,9 @@ def main():
                 out["blocks"]["cpu_baseline"] = cpu_block_baseline(w2, t2, Rref, tuple(gpu_block))
                 out["blocks"]["speedup_vs_cpu"] = out["blocks"]["H_entries_per_s"] / out["blocks"]["cpu_baseline"]["value"]
             del Rref, tensors
+    if args.branch:
+        log("branch graph")
+        out["branch"] = branch_benchmark(ctx)
     if args.blockset:
         log("c3 block set")
         out["blockset"] = blockset_benchmark(ctx)

@@ -251,6 +251,21 @@ int64_t bs2e_blockset_get(const bs2e_blockset *set, int64_t sym, int64_t *ip_H, 
 int64_t bs2e_blockset_get_rows(const bs2e_blockset *set, int64_t sym, int64_t row_lo, int64_t n_rows,
                                int64_t *ip_H, int64_t *ix_H, double *d_H, int64_t *ip_S, int64_t *ix_S,
                                double *d_S);
+/* The fill again, from the one-electron matrices of the last
+ * bs2e_blockset_fill (kept on the device): the layouts and the fill launch,
+ * waited for.  (No reference counterpart.) */
+int64_t bs2e_blockset_refill(bs2e_blockset *set);
+/* on != 0: captures the whole branch of the set into one hipGraph: its
+ * tensor's two-stream build, the count pass, the device prefix sums of the
+ * row counts, the fill's layouts and the fill over every symmetry and channel
+ * (one-electron matrices of the last bs2e_blockset_fill); 0 drops it.  A new
+ * fill drops it too.  bs2e_blockset_replay then recomputes tensor and blocks
+ * with one launch on the context's stream (asynchronous:
+ * bs2e_blockset_synchronize waits).  (No reference counterpart: the reference
+ * builds each symmetry by its own call, main_basis_setup.f90:104-115.) */
+int64_t bs2e_blockset_set_graph(bs2e_blockset *set, int64_t on);
+int64_t bs2e_blockset_replay(bs2e_blockset *set);
+int64_t bs2e_blockset_synchronize(bs2e_blockset *set);
 /* Device time (ms) of the count launch (ms[0]) and the last fill (ms[1]). */
 int64_t bs2e_blockset_times(const bs2e_blockset *set, double *ms, int64_t n);
 int64_t bs2e_blockset_release(bs2e_blockset *set);

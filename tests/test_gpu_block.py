@@ -421,3 +421,55 @@ def test_c3_blockset_sampled_rows(ctx, oracle):
             Hd[r, cols[sel] - lo - 1] = vals[sel]
         assert np.abs(Hd - Hd.T).max() <= TOL * np.abs(dH).max()
     print(f"  {len(windows)} windows of {W} rows: H max err / scale {worst:.3e}", flush=True)
+
+
+def test_branch_graph_replays_match_eager(ctx, oracle):
+    """The whole branch of a set (tensor build, count, device row offsets,
+    layouts, fill over L = 0..2) captured as one hipGraph: every replay, with
+    eager builds and synchronisations between them, gives the eager tensor and
+    every symmetry's CSR arrays bit for bit; refill() from the device copy of
+    the one-electron matrices equals the first fill; a new fill drops the
+    graph."""
+    w = WORKLOADS["c1"]
+    t = bs2e.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+    n_b = oracle.dims(w.k, t)[1]
+    S = oracle.setup_S(w.k, t, w.k_GL)
+    Hl = [oracle.setup_H1p(w.k, t, w.k_GL, 2, l, cap_eta=1e-3) for l in range(3)]
+    rk = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx)
+    terms = []
+    for L in range(3):
+        term = bs2e.sym(l=L, pi=bool(L % 2))
+        term.configs = bs2e.count_configs(term, 2, n_b, w.k, n_b, n_b, 2)
+        terms.append(term)
+    bset = bs2e.BlockSet(rk, terms, False)
+    bset.fill(Hl, S)
+    R0 = rk.to_host()
+    ref = [bset.get(i) for i in range(len(terms))]
+
+    def same():
+        for i, (H0, S0) in enumerate(ref):
+            H1, S1 = bset.get(i)
+            for a, b in ((H0, H1), (S0, S1)):
+                assert np.array_equal(a.index_ptr, b.index_ptr) and np.array_equal(a.indices, b.indices)
+                assert np.array_equal(a.data, b.data)
+
+    bset.refill()
+    same()
+    bset.set_graph(True)
+    other = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx, compute=False)
+    for rep in range(3):
+        print(f"branch graph replay {rep}", flush=True)
+        bset.replay()
+        bset.synchronize()
+        assert np.array_equal(rk.to_host(), R0), rep
+        same()
+        other.compute()               # eager work between replays
+        other.synchronize()
+    bset.fill(Hl, S)                  # a new fill drops the graph
+    same()
+    with pytest.raises(capi.Bs2eError) as e:
+        bset.replay()
+    assert e.value.code == capi.BS2E_ESTATE
+    other.release()
+    bset.release()
+    rk.release()
