@@ -127,6 +127,28 @@ __device__ __forceinline__ double r12_row(const BA& A, int a1, int c1, int a2, i
     return kdot3(x, np, y, np, coef, K);
 }
 
+// The same sum for a row whose windows never meet (|n_a - n_b| >= k): every
+// product is rank-one with a known side; x and y are the p and q factors in
+// the wavefront's LDS copy, K consecutive multipoles from a 16-byte boundary,
+// as is coef, so that one LDS read brings two multipoles.
+__device__ __forceinline__ double r12_staged(const double* __restrict__ x, const double* __restrict__ y,
+                                             const double* __restrict__ coef, int K)
+{
+    double t = 0.0;
+    int k = 0;
+    for (; k + 4 <= K; k += 4) {
+        const dbl2 xa = *(const dbl2*)(x + k), xb = *(const dbl2*)(x + k + 2);
+        const dbl2 ya = *(const dbl2*)(y + k), yb = *(const dbl2*)(y + k + 2);
+        const dbl2 ca = *(const dbl2*)(coef + k), cb = *(const dbl2*)(coef + k + 2);
+        t = t + (xa.x * ya.x) * ca.x;
+        t = t + (xa.y * ya.y) * ca.y;
+        t = t + (xb.x * yb.x) * cb.x;
+        t = t + (xb.y * yb.y) * cb.y;
+    }
+    for (; k < K; ++k) t = t + (x[k] * y[k]) * coef[k];
+    return t;
+}
+
 // element (r, c), 1-based, of a complex row-major n_b x n_b matrix
 __device__ __forceinline__ Cplx mat(const double* M, int n_b, int r, int c)
 {
@@ -142,43 +164,14 @@ __device__ __forceinline__ int64_t uniform64(int64_t x)
     return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
 }
 
-// sum_k coef[k] R^k[p][q] for a row whose windows never meet (|n_a - n_b| >=
-// k): every product is rank-one with a known side; xs holds the p factors
-// with the coefficients folded in (xs[k] = x[k] coef[k], formed once per
-// channel) and y the q factors, both in the wavefront's LDS copy, K
-// consecutive multipoles from a 16-byte boundary, so that one LDS read brings
-// two multipoles: one product and one sum per multipole.  (x coef) y rounds
-// differently from the reference's (x y) coef in the last bits of each term.
-__device__ __forceinline__ double r12_scaled(const double* __restrict__ xs, const double* __restrict__ y, int K)
-{
-    double t = 0.0;
-    int k = 0;
-    for (; k + 4 <= K; k += 4) {
-        const dbl2 xa = *(const dbl2*)(xs + k), xb = *(const dbl2*)(xs + k + 2);
-        const dbl2 ya = *(const dbl2*)(y + k), yb = *(const dbl2*)(y + k + 2);
-        t = t + xa.x * ya.x;
-        t = t + xa.y * ya.y;
-        t = t + xb.x * yb.x;
-        t = t + xb.y * yb.y;
-    }
-    for (; k < K; ++k) t = t + xs[k] * y[k];
-    return t;
-}
-
 // LDS of one fill wavefront (doubles), in rows of KP = K rounded up to even:
 // the channel pair's direct and exchange coefficients (2 rows), then the q
 // factors of a row whose windows never meet, [2][L][KP] with L = 2k - 1:
 // direct term pairs (d, n_b), exchange term pairs (d, n_a), for the d within
-// k of n_b / n_a; then the same rows times the channel's coefficients,
-// [2][L][KP]: the direct term's p factors (the exchange run times the direct
-// coefficients) and the exchange term's (the direct run times the exchange
-// coefficients).
+// k of n_b / n_a.
 __host__ __device__ constexpr int fill_kp(int K) { return (K + 1) & ~1; }
 // (plus 64 bytes for the channel walk's marks)
-__host__ __device__ constexpr int fill_wave_lds(int ks, int K)
-{
-    return 2 * fill_kp(K) * (1 + 2 * (2 * ks - 1)) + 8;
-}
+__host__ __device__ constexpr int fill_wave_lds(int ks, int K) { return 2 * fill_kp(K) * (1 + (2 * ks - 1)) + 8; }
 
 // One wavefront per configuration row of any of n_blocks symmetry blocks:
 // global row g belongs to the last block b with row0[b] <= g and is row
@@ -194,11 +187,10 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
     extern __shared__ __align__(16) double lds_fill[];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
-    const int64_t g = (int64_t)blockIdx.x * (blockDim.x / kWave) + wave;   // 1, 2 or 4 wavefronts per workgroup
+    const int64_t g = (int64_t)blockIdx.x * (kCThreads / kWave) + wave;
     if (g >= n_rows) return;   // whole wave exits together
     double* sc = lds_fill + (size_t)wave * fill_wave_lds(lds_ks, lds_K);   // coefficients: direct, exchange
     double* sy = sc + 2 * fill_kp(lds_K);                                  // staged q factors
-    double* sx = sy + 2 * (2 * lds_ks - 1) * fill_kp(lds_K);               // staged p factors x coefficients
     unsigned char* own = (unsigned char*)(sc + fill_wave_lds(lds_ks, lds_K) - 8);   // walk marks
     int blo = 0, bhi = n_blocks - 1;
     while (blo < bhi) {
@@ -266,15 +258,6 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (staged) {   // p factors times the coefficients, once per channel
-                for (int e = lane; e < 2 * L * K; e += kWave) {
-                    const int t = e >= L * K, r = (e - t * L * K) / K, k = e - t * L * K - r * K;
-                    sx[(t * L + r) * KP + k] = t ? sy[r * KP + k] * sc[KP + k] : sy[(L + r) * KP + k] * sc[k];
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            }
         }
         const double* cd = sc;
         const double* ce = sc + KP;
@@ -298,14 +281,15 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
                         // R(a,b,c,d) = R(a,d,c,b): the key [n_a, n_d, n_c, n_b] puts the
                         // wavefront's consecutive n_d on consecutive columns of one row
                         if (support)
-                            res = res + (staged ? r12_scaled(sx + (n_c - dlo_ex) * KP, sy + (n_d - dlo_dir) * KP, K)
-                                                : r12_row(A, n_a, n_c, n_d, n_bb, cd));
+                            res = res + (staged ? r12_staged(sy + (L + n_c - dlo_ex) * KP, sy + (n_d - dlo_dir) * KP,
+                                                                     cd, K)
+                                                        : r12_row(A, n_a, n_c, n_d, n_bb, cd));
                         // exchange R(a,b,d,c) = R(b,a,c,d) = R(b,d,c,a): row pair(n_b, n_c),
                         // consecutive n_d on consecutive columns
                         if (support_ex)
-                            res = res + (staged ? r12_scaled(sx + (L + n_c - dlo_dir) * KP, sy + (L + n_d - dlo_ex) * KP,
-                                                             K)
-                                                : r12_row(A, n_bb, n_c, n_d, n_a, ce));
+                            res = res + (staged ? r12_staged(sy + (n_c - dlo_dir) * KP, sy + (L + n_d - dlo_ex) * KP,
+                                                                     ce, K)
+                                                        : r12_row(A, n_bb, n_c, n_d, n_a, ce));
                         h.re = res;
                     }
                     if (one_el) {
@@ -575,16 +559,12 @@ hipError_t launch_row_scan(const int64_t* cnt, int64_t* ptr, int64_t n_rows, int
 hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, int ks, int K,
                              hipStream_t s)
 {
-    // four wavefronts per workgroup where their LDS fits, else two or one
-    // (one wavefront takes at most 65 KiB: k <= 16, K <= 64)
-    const size_t per = sizeof(double) * (size_t)fill_wave_lds(ks, K);
-    int waves = kCThreads / kWave;
-    while (waves > 1 && per * (size_t)waves > kLdsPerWorkgroup) waves >>= 1;
-    const size_t lds = per * (size_t)waves;
-    hipError_t e = allow_lds((const void*)kc_block, lds);
+    const int64_t threads = n_rows * kWave;
+    const size_t lds = sizeof(double) * (size_t)(kCThreads / kWave) * (size_t)fill_wave_lds(ks, K);
+    hipError_t e = allow_lds((const void*)kc_block, lds);   // <= 4 x 32 KiB (k <= 16, K <= 64)
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kc_block, dim3((unsigned)((n_rows + waves - 1) / waves)), dim3(waves * kWave), lds, s, tab,
-                       row0, n_blocks, n_rows, ks, K);
+    hipLaunchKernelGGL(kc_block, dim3((unsigned)((threads + kCThreads - 1) / kCThreads)), dim3(kCThreads), lds,
+                       s, tab, row0, n_blocks, n_rows, ks, K);
     return hipGetLastError();
 }
 

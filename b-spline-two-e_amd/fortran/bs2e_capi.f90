@@ -9,7 +9,8 @@ module bs2e_capi
     private
 
     integer(c_int64_t), parameter, public :: BS2E_OK = 0, BS2E_EINVAL = -1, BS2E_EDEVICE = -2, &
-        BS2E_ENOMEM = -3, BS2E_ERANGE = -4, BS2E_ESTATE = -5, BS2E_PREC_FP64 = 0
+        BS2E_ENOMEM = -3, BS2E_ERANGE = -4, BS2E_ESTATE = -5, BS2E_PREC_FP64 = 0, &
+        BS2E_PREC_MIXED = 1
 
     public :: bs2e_create, bs2e_destroy, bs2e_last_error, bs2e_generate_grid, bs2e_count_nnz
     public :: bs2e_slater_primitives, bs2e_rk_create, bs2e_rk_from_primitives, bs2e_rk_compute

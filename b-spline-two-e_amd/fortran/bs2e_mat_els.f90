@@ -51,7 +51,7 @@ module bs2e_mat_els
     public :: count_configs, construct_block_tensor, ang_k_LS, bs2e_use_device, bs2e_shutdown
     public :: hydrogenic, CAP, setup_S, setup_H_one_particle, basis, block_diag_CS
     public :: block_CS, radial_dipole, setup_radial_dip, construct_dip_block_tensor, rkh_multipoles
-    public :: construct_blocks_tensor
+    public :: construct_blocks_tensor, BS2E_PREC_FP64, BS2E_PREC_MIXED
 
     type(c_ptr), save :: g_ctx = c_null_ptr
     integer, save :: g_device = 0
@@ -800,13 +800,19 @@ contains
     end subroutine compute_R_k_map
 
     ! The whole build on the device from the knots (no host containers).
-    subroutine compute_R_k_direct(b_splines, max_k, k_GL, R)
+    ! precision: BS2E_PREC_FP64 (default) or BS2E_PREC_MIXED (the off-diagonal
+    ! moments on the FP32 matrix cores from exact integer slices).
+    subroutine compute_R_k_direct(b_splines, max_k, k_GL, R, precision)
         type(b_spline), intent(in) :: b_splines
         integer, intent(in) :: max_k, k_GL
         class(Nd_DOK), intent(inout) :: R
+        integer, intent(in), optional :: precision
+        integer(c_int64_t) :: prec
+        prec = BS2E_PREC_FP64
+        if (present(precision)) prec = int(precision, c_int64_t)
         call R%release()
         call check(bs2e_rk_create(ctx(), int(b_splines%k, c_int64_t), int(size(b_splines%knots), c_int64_t), &
-                   b_splines%knots, int(k_GL, c_int64_t), int(max_k, c_int64_t), BS2E_PREC_FP64, R%handle), &
+                   b_splines%knots, int(k_GL, c_int64_t), int(max_k, c_int64_t), prec, R%handle), &
                    "compute_R_k_direct")
         call finish_tensor(R, max_k)
     end subroutine compute_R_k_direct

@@ -42,7 +42,10 @@ HBM_PEAK_GBS = 8000.0
 # (profiles/rk_build_c2.md, profiles/rk_build_c5p.md), per (workload, fill
 # shape); absent for pairs not measured that way.
 FAR_TRAFFIC_BYTES = {("c2", 0): 739.35e6, ("c2", 1): 739.59e6, ("c2", 2): 739.99e6, ("c2", 3): 739.74e6,
-                     ("c2", 4): 739.52e6, ("c2", 5): 739.51e6, ("c2", 6): 739.05e6}
+                     ("c2", 4): 739.52e6, ("c2", 5): 739.51e6, ("c2", 6): 739.05e6,
+                     ("c5p", 0): 35204.48e6, ("c5p", 1): 20266.54e6, ("c5p", 2): 20273.27e6,
+                     ("c5p", 3): 20267.69e6, ("c5p", 4): 20262.93e6, ("c5p", 5): 20269.25e6,
+                     ("c5p", 6): 20260.59e6}
 
 
 def log(msg):
