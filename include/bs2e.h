@@ -163,6 +163,22 @@ int64_t bs2e_rk_autotune(bs2e_rk *rk, int64_t force);
  * diagonal-cell workgroup. */
 int64_t bs2e_rk_tuning(const bs2e_rk *rk, int64_t *out, int64_t n);
 int64_t bs2e_rk_release(bs2e_rk *rk);
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The whole build of bs2e_rk_create + bs2e_rk_compute (same
+ * kernels, same values, bit for bit) into the caller's device buffer R of
+ * n_elems = n_pairs^2 * (max_k+1) doubles on the calling thread's current
+ * device, enqueued on hip_stream (NULL: the null stream), with the work tables in one static device array of the
+ * library (128 MiB per GPU, held until the process ends; no hipMalloc,
+ * hipFree or allocating library call).  A basis whose tables exceed it (for
+ * instance order 9 with 600 B-splines and k <= 24) is refused with
+ * BS2E_EINVAL.  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (other streams, other threads) race; calls on one stream are
+ * ordered.  The knot-derived tables are uploaded with a wait on hip_stream
+ * before the kernels are enqueued; the call returns without waiting for
+ * them. */
+int64_t bs2e_rk_build_static(int64_t k, int64_t n_knots, const double *knots, int64_t k_gl, int64_t max_k,
+                             int64_t precision, double *R, int64_t n_elems, void *hip_stream);
 
 /* ---- angular algebra --------------------------------------------------- */
 /* wigner_tools.f90:126-138 ang_k_LS(k, confs, L) with confs(1)%l = (l1a,l1b),
