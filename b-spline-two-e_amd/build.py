@@ -33,6 +33,10 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
+# Per-file additions.  The separated-row block fill keeps its FP64 matrix-core
+# accumulators in ordinary vector registers (otherwise the compiler moves
+# them between accumulation and vector registers around every step).
+FILE_FLAGS = {"k_channel.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def _run(cmd):
@@ -49,7 +53,8 @@ def build_hip(verbose: bool = False) -> str:
     objs = [os.path.join(OBJ, os.path.basename(s)[:-4] + ".o") for s in srcs]
     jobs = min(len(srcs), int(os.environ.get("MAX_JOBS", "8")))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = [ex.submit(_run, [HIPCC, *HIP_FLAGS, "-I", CSRC, "-I", os.path.join(ROOT, "include"), "-c", s, "-o", o])
+        futs = [ex.submit(_run, [HIPCC, *HIP_FLAGS, *FILE_FLAGS.get(os.path.basename(s), []), "-I", CSRC, "-I",
+                                 os.path.join(ROOT, "include"), "-c", s, "-o", o])
                 for s, o in zip(srcs, objs)]
         for f in futs:
             out = f.result()

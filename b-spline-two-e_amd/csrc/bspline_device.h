@@ -14,6 +14,19 @@
 
 namespace bs2e {
 
+// Local pair u of an interval's k(k+1)/2 unordered spline pairs (row-major
+// upper triangle) -> its splines ta <= tc.
+__device__ __forceinline__ void local_pair(int u, int ks, int& ta, int& tc)
+{
+    int lo = 0, base = 0;
+    while (base + (ks - lo) <= u) {
+        base += ks - lo;
+        ++lo;
+    }
+    ta = lo;
+    tc = lo + (u - base);
+}
+
 __device__ __forceinline__ double ipow_d(double x, int m)
 {
     // x**m for integer m >= 0 by square-and-multiply (how Fortran lowers an

@@ -61,6 +61,7 @@ std::string bs2e::channel_tables(const int64_t* configs, int64_t n_config, int64
 {
     chan.clear();
     seg.clear();
+    if (nb >= (1 << 15)) return "the block fill takes fewer than 2^15 B-splines";   // n2 ranges packed in 16 bits
     cfg_chan.assign((size_t)n_config, 0);
     max_l = 0;
     int64_t prev_n1 = -1, prev_n2 = -1;
@@ -139,6 +140,11 @@ struct bs2e_blockset {
     int64_t* d_scan = nullptr;     // the row-offset scan's tile totals
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // the fill's rows: the separated ones (|n1 - n2| >= 2k - 1, the
+    // matrix-core fill) first, then the others, each ascending
+    std::vector<int32_t> rowlist;
+    int64_t n_sep = 0;
+    int32_t* d_rowlist = nullptr;
 };
 
 // A single symmetry's block is a set of one.
@@ -184,6 +190,14 @@ static void set_free(bs2e_blockset* b)
     ctx_free(c, b->d_ptr);
     ctx_free(c, b->d_tab);
     ctx_free(c, b->d_row0);
+    ctx_free(c, b->d_rowlist);
+}
+
+static hipError_t set_fill_launch(bs2e_blockset* b, const BlockArgs* tab, hipStream_t st)
+{
+    const Plan& p = b->rk->plan;
+    return launch_block_fill(tab, b->d_row0, (int)b->n_sym, b->d_rowlist, b->n_sep, b->n_rows, (int)p.k_s, (int)p.K,
+                             st);
 }
 
 // descriptor of symmetry s for the count (fill = false) or fill pass
@@ -287,7 +301,18 @@ static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* 
                 b->flags[f0 + ci * nch + cj] = fl;
             }
     }
+    if (N >= ((int64_t)1 << 31))
+        return fail(ctx, BS2E_EINVAL, std::string(who) + ": a set holds fewer than 2^31 configurations");
+    b->rowlist.clear();
+    b->rowlist.reserve((size_t)N);
+    for (int pass = 0; pass < 2; ++pass) {   // separated rows, then the others
+        for (int64_t g = 0; g < N; ++g)
+            if ((std::llabs(configs[5 * g] - configs[5 * g + 1]) >= 2 * p.k_s - 1) == (pass == 0))
+                b->rowlist.push_back((int32_t)g);
+        if (pass == 0) b->n_sep = (int64_t)b->rowlist.size();
+    }
     HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_rowlist, (size_t)N));
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_cfg, (size_t)5 * N));
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_cfg_chan, (size_t)N));
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_chan, b->chan.size()));
@@ -301,6 +326,7 @@ static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* 
     hipStream_t st = ctx->stream;
     HIP_TRY(ctx, hipMemcpyAsync(b->d_cfg, configs, sizeof(int64_t) * 5 * N, hipMemcpyHostToDevice, st));
     HIP_TRY(ctx, hipMemcpyAsync(b->d_cfg_chan, b->cfg_chan.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(b->d_rowlist, b->rowlist.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
     HIP_TRY(ctx, hipMemcpyAsync(b->d_chan, b->chan.data(), sizeof(BlockChan) * b->chan.size(), hipMemcpyHostToDevice,
                                 st));
     HIP_TRY(ctx, hipMemcpyAsync(b->d_seg, b->seg.data(), sizeof(BlockSeg) * b->seg.size(), hipMemcpyHostToDevice, st));
@@ -412,7 +438,7 @@ int64_t bs2e::set_fill_enqueue(bs2e_blockset* b, const double* dHl, const double
                                   (int)p.K, b->d_T, b->d_T + 2 * nn * b->n_l, b->d_T + 2 * nn * (b->n_l + 1),
                                   b->d_T + 2 * nn * (b->n_l + 1) + nf, st);
     if (le == hipSuccess)
-        le = launch_block_fill(b->d_tab, b->d_row0, (int)b->n_sym, b->n_rows, (int)p.k_s, (int)p.K, st);
+        le = set_fill_launch(b, b->d_tab, st);
     if (wait) {
         if (le == hipSuccess) le = hipEventRecord(e1, st);
         if (le == hipSuccess) le = hipEventSynchronize(e1);
@@ -446,8 +472,9 @@ size_t bs2e::blockset_fill_bytes(const bs2e_blockset* b, int64_t n_l)
 size_t bs2e::blockset_table_bytes(const bs2e_blockset* b)
 {
     const size_t N = (size_t)b->n_rows;
-    return arena_round(40 * N) + arena_round(4 * N) + arena_round(sizeof(BlockChan) * b->chan.size()) +
-           arena_round(sizeof(BlockSeg) * b->seg.size()) + arena_round(8 * b->coef.size()) +
+    return arena_round(40 * N) + arena_round(4 * N) + arena_round(4 * N) +
+           arena_round(sizeof(BlockChan) * b->chan.size()) + arena_round(sizeof(BlockSeg) * b->seg.size()) +
+           arena_round(8 * b->coef.size()) +
            arena_round(4 * b->flags.size()) + arena_round(32 * N) + arena_round(16 * N) +
            arena_round(sizeof(BlockArgs) * (size_t)b->n_sym) + arena_round(8 * ((size_t)b->n_sym + 1));
 }
@@ -753,7 +780,7 @@ int64_t bs2e_blockset_set_graph(bs2e_blockset* b, int64_t on)
         e = launch_block_prepare(b->d_Hl, b->d_S, (int)b->n_l, (int)p.n_b, rk->d.Ptot, rk->d.Qtot, (int)p.n_pairs,
                                  (int)p.K, b->d_T, b->d_T + 2 * nn * b->n_l, b->d_T + 2 * nn * (b->n_l + 1),
                                  b->d_T + 2 * nn * (b->n_l + 1) + nf, st);
-    if (e == hipSuccess) e = launch_block_fill(b->d_tabg, b->d_row0, (int)b->n_sym, N, (int)p.k_s, (int)p.K, st);
+    if (e == hipSuccess) e = set_fill_launch(b, b->d_tabg, st);
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(st, &g);
     if (e == hipSuccess) e = ec;

@@ -9,13 +9,17 @@
 // exist where the radial pairs share support:
 //   support    = |n_a-n_c| < k  and |n_b-n_d| < k
 //   support_ex = |n_a-n_d| < k  and |n_b-n_c| < k,
-// so one wavefront per row walks, channel by channel, only the columns with
+// so one wavefront per row visits, channel by channel, only the columns with
 // n_c within k of n_a or n_b, and for each such n_c the n_d within k of n_b
-// (direct) or n_a (exchange), in ascending column order; lanes evaluate the
+// (direct) or n_a (exchange), in ascending column order, under the
 // reference's predicate
 //   H entry <=> (support or support_ex) and (r12_allowed or l_eq or l_eq_ex)
-//   S entry <=> (support and l_eq) or (support_ex and l_eq_ex)
-// and ballots give each entry its CSR position.  The angular factors are per
+//   S entry <=> (support and l_eq) or (support_ex and l_eq_ex).
+// Rows with |n_a - n_b| >= 2k - 1 (most of a large block) take the
+// matrix-core path (fill_row_separated): per channel the direct and exchange
+// sums are two small matrix products, placed in CSR by a prefix sum over the
+// candidates n_c.  The other rows walk the candidates 64 at a time and
+// ballots give each entry its CSR position.  The angular factors are per
 // (row channel, column channel) coefficient vectors:
 //   cdir[k] = ang_k_LS(k, confs, L)               if |ang| >= 5e-16 else 0
 //   cex[k]  = (-1)^(l_c+l_d+L) ang_k_LS(k, confs_ex, L)  (same threshold)
@@ -157,6 +161,33 @@ __device__ __forceinline__ Cplx mat(const double* M, int n_b, int r, int c)
     return Cplx{v.x, v.y};
 }
 
+// The one-electron parts of H and S at entry (n_a, n_b | n_c, n_d)
+// (mat_els.f90:664-678 S_mat_neq, :697-715 H_1p_neq): the direct products
+// where l_eq holds, the exchange products with the sign (-1)^(l_c+l_d+L)
+// (odd_sign) where l_eq_ex holds, each operation rounded on its own.
+template <class BA>
+__device__ __forceinline__ void one_electron(const BA& A, int n_a, int n_bb, int n_c, int n_d, int l_a, int l_b,
+                                             bool l_eq, bool l_eq_ex, bool odd_sign, Cplx& h1, Cplx& s1)
+{
+    const size_t nn = (size_t)A.n_b * A.n_b;
+    const double* Ha = A.Hl + 2 * nn * l_a;
+    const double* Hb = A.Hl + 2 * nn * l_b;
+    const double sgn = odd_sign ? -1.0 : 1.0;
+    h1 = Cplx{0.0, 0.0};
+    s1 = Cplx{0.0, 0.0};
+    if (l_eq) {
+        h1 = cadd(cadd(h1, cmul(mat(Ha, A.n_b, n_a, n_c), mat(A.S, A.n_b, n_bb, n_d))),
+                  cmul(mat(Hb, A.n_b, n_bb, n_d), mat(A.S, A.n_b, n_a, n_c)));
+        s1 = cadd(s1, cmul(mat(A.S, A.n_b, n_a, n_c), mat(A.S, A.n_b, n_bb, n_d)));
+    }
+    if (l_eq_ex) {
+        const Cplx x = cadd(cmul(mat(Ha, A.n_b, n_a, n_d), mat(A.S, A.n_b, n_bb, n_c)),
+                            cmul(mat(Hb, A.n_b, n_bb, n_c), mat(A.S, A.n_b, n_a, n_d)));
+        h1 = cadd(h1, cscale(sgn, x));
+        s1 = cadd(s1, cmul(cscale(sgn, mat(A.S, A.n_b, n_a, n_d)), mat(A.S, A.n_b, n_bb, n_c)));
+    }
+}
+
 __device__ __forceinline__ int64_t uniform64(int64_t x)
 {
     const int lo = __builtin_amdgcn_readfirstlane((int)(uint32_t)x);
@@ -173,44 +204,23 @@ __host__ __device__ constexpr int fill_kp(int K) { return (K + 1) & ~1; }
 // (plus 64 bytes for the channel walk's marks)
 __host__ __device__ constexpr int fill_wave_lds(int ks, int K) { return 2 * fill_kp(K) * (1 + (2 * ks - 1)) + 8; }
 
-// One wavefront per configuration row of any of n_blocks symmetry blocks:
-// global row g belongs to the last block b with row0[b] <= g and is row
-// g - row0[b] of it; each block's tables, inputs and outputs are in tab[b].
-// The fill's LDS (fill_wave_lds per wavefront) is dynamic; ks and K are the
-// blocks' common spline order and multipole count.  Entries go to the row's
-// CSR range from the count pass; stores beyond the set's outputs (limH,
-// limS entries) are dropped, so a count that disagreed with the walk could
-// not write outside them.
-__global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restrict__ tab, const int64_t* __restrict__ row0,
-                                                      int n_blocks, int64_t n_rows, int lds_ks, int lds_K)
+// A row whose electrons sit at least 2k - 1 splines apart: its direct
+// candidates n_c (within k of n_a) and exchange candidates (within k of n_b)
+// are disjoint, and every entry has exactly one of the two supports.
+__device__ __forceinline__ bool separated_row(int n_a, int n_bb, int ks) { return iabs_d(n_bb - n_a) >= 2 * ks - 1; }
+
+typedef const __attribute__((address_space(4))) BlockArgs CBlockArgs;
+
+// The fill of one configuration row i of block A by one wavefront, in
+// general: a channel walk with ballot-placed entries.  wl is the wavefront's
+// LDS (fill_wave_lds(lds_ks, lds_K) words).
+__device__ __forceinline__ void fill_row_general(const CBlockArgs& A, int i, int n_a, int n_bb, int l_a, int l_b,
+                                                 int ch_i, int lane, double* wl, int lds_ks, int lds_K)
 {
-    extern __shared__ __align__(16) double lds_fill[];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
-    const int64_t g = (int64_t)blockIdx.x * (kCThreads / kWave) + wave;
-    if (g >= n_rows) return;   // whole wave exits together
-    double* sc = lds_fill + (size_t)wave * fill_wave_lds(lds_ks, lds_K);   // coefficients: direct, exchange
+    double* sc = wl;                                                       // coefficients: direct, exchange
     double* sy = sc + 2 * fill_kp(lds_K);                                  // staged q factors
     unsigned char* own = (unsigned char*)(sc + fill_wave_lds(lds_ks, lds_K) - 8);   // walk marks
-    int blo = 0, bhi = n_blocks - 1;
-    while (blo < bhi) {
-        const int mid = (blo + bhi + 1) >> 1;
-        if (row0[mid] <= g) blo = mid;
-        else bhi = mid - 1;
-    }
-    const int b = __builtin_amdgcn_readfirstlane(blo);
-    // The descriptor is read through the constant address space (the host
-    // writes it before the launch; nothing in the launch writes it), so the
-    // compiler may load a field again where it is used instead of keeping all
-    // of them in scalar registers and spilling them to vector lanes.
-    typedef const __attribute__((address_space(4))) BlockArgs CBlockArgs;
-    const CBlockArgs& A = ((const CBlockArgs*)tab)[b];
-    const int i = (int)(g - row0[b]);
     const int ks = A.ks, K = A.K;
-    const int64_t* ci = A.cfg + 5 * (size_t)i;
-    const int n_a = __builtin_amdgcn_readfirstlane((int)ci[0]), n_bb = __builtin_amdgcn_readfirstlane((int)ci[1]);
-    const int l_a = __builtin_amdgcn_readfirstlane((int)ci[2]), l_b = __builtin_amdgcn_readfirstlane((int)ci[3]);
-    const int ch_i = __builtin_amdgcn_readfirstlane(A.cfg_chan[i]);
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int64_t nH = 0, nS = 0;
     const int64_t baseH = uniform64(A.ptrH[i]), baseS = uniform64(A.ptrS[i]);
@@ -242,10 +252,15 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
         }
     }
 
+    // the channel table and flags through the scalar cache: a vector load
+    // here would wait for the previous channel's stores
+    typedef const __attribute__((address_space(4))) BlockChan CChan;
+    typedef const __attribute__((address_space(4))) int32_t CInt;
     for (int ch = 0; ch < A.n_chan; ++ch) {
-        const BlockChan cj = A.chan[ch];
+        const CChan& cjr = ((CChan*)A.chan)[ch];
+        const BlockChan cj{cjr.l1, cjr.l2, cjr.start, 0};
         const int l_c = cj.l1, l_d = cj.l2;
-        const int fl = A.flags[ch_i * A.n_chan + ch];
+        const int fl = ((CInt*)A.flags)[ch_i * A.n_chan + ch];
         const bool dir_any = fl & 1, ex_any = (fl >> 1) & 1;
         const bool l_eq = (l_a == l_c) && (l_b == l_d);
         const bool l_eq_ex = (l_a == l_d) && (l_b == l_c);
@@ -293,23 +308,9 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
                         h.re = res;
                     }
                     if (one_el) {
-                        const size_t nn = (size_t)A.n_b * A.n_b;
-                        const double* Ha = A.Hl + 2 * nn * l_a;
-                        const double* Hb = A.Hl + 2 * nn * l_b;
-                        Cplx h1{0.0, 0.0}, s1{0.0, 0.0};
-                        const double sgn = ((l_c + l_d + A.L) & 1) ? -1.0 : 1.0;
-                        if (l_eq) {
-                            h1 = cadd(cadd(h1, cmul(mat(Ha, A.n_b, n_a, n_c), mat(A.S, A.n_b, n_bb, n_d))),
-                                      cmul(mat(Hb, A.n_b, n_bb, n_d), mat(A.S, A.n_b, n_a, n_c)));
-                            s1 = cadd(s1, cmul(mat(A.S, A.n_b, n_a, n_c), mat(A.S, A.n_b, n_bb, n_d)));
-                        }
-                        if (l_eq_ex) {
-                            const Cplx x = cadd(cmul(mat(Ha, A.n_b, n_a, n_d), mat(A.S, A.n_b, n_bb, n_c)),
-                                                cmul(mat(Hb, A.n_b, n_bb, n_c), mat(A.S, A.n_b, n_a, n_d)));
-                            h1 = cadd(h1, cscale(sgn, x));
-                            s1 = cadd(s1, cmul(cscale(sgn, mat(A.S, A.n_b, n_a, n_d)),
-                                               mat(A.S, A.n_b, n_bb, n_c)));
-                        }
+                        Cplx h1, s1;
+                        one_electron(A, n_a, n_bb, n_c, n_d, l_a, l_b, l_eq, l_eq_ex, ((l_c + l_d + A.L) & 1) != 0,
+                                     h1, s1);
                         h = cadd(h, h1);
                         const int64_t ps = baseS + nS + __popcll(mS & lt_mask);
                         if (ps < A.limS) {
@@ -328,6 +329,343 @@ __global__ __launch_bounds__(kCThreads) void kc_block(const BlockArgs* __restric
             nS += __popcll(mS);
         });
     }
+}
+
+// The fill of the separated rows (|n_a - n_b| >= 2k - 1) on the FP64 matrix
+// cores.  In such a row every R^k product is rank one with a side fixed by
+// the row, and all its factors come from two runs of window totals: run 1,
+// the pairs (d, n_b) for d within k of n_b, and run 2, the pairs (d, n_a) for
+// d within k of n_a (the totals of (a, c) and (c, a) are the same sums).  Per
+// column channel the direct term of entry (n_c, n_d), n_c in run 2's range
+// and n_d in run 1's, is
+//   sum_k run2_k(n_c) cdir_k run1_k(n_d),
+// and the exchange term, n_c in run 1's range and n_d in run 2's,
+//   sum_k run1_k(n_c) cex_k run2_k(n_d):
+// each a (2k-1) x (2k-1) block of one matrix product with inner dimension K,
+// taken in 16 x 16 tiles on v_mfma_f64_16x16x4 (NT x NT tiles, 2k - 1 <= 16
+// NT) from the runs in LDS, multipole-major and zero-padded to whole tiles
+// and to a multiple of four multipoles.  Each candidate n_c's entries are one
+// run of n_d (its term's window, clipped to the channel's n2 range and, for
+// upper triangles, to j >= i), and a prefix sum over the candidates (lanes,
+// in column order) places every run in the row's CSR range, so an entry's
+// position follows from its tile coordinates.  The products are summed in
+// the matrix cores' order with fused multiply-adds, and the coefficient is
+// applied to the p factor: the terms differ from the reference's (x y) c in
+// the last bits.  The one-electron parts are the general fill's, bit for bit.
+// Inclusive prefix sum over the wavefront's 64 lanes: within each row of 16
+// lanes by DPP row shifts (a lane shifted in from outside the row adds 0),
+// then the totals of the rows before, read from lanes 15, 31 and 47.
+__device__ __forceinline__ int wave_incl_sum(int v, int lane)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);   // row_shr:8
+    const int t0 = __builtin_amdgcn_readlane(v, 15), t1 = __builtin_amdgcn_readlane(v, 31);
+    const int t2 = __builtin_amdgcn_readlane(v, 47);
+    const int row = lane >> 4;
+    return v + (row > 0 ? t0 : 0) + (row > 1 ? t1 : 0) + (row > 2 ? t2 : 0);
+}
+
+__host__ __device__ constexpr int sep_kp(int K) { return 4 * ((K + 3) / 4); }
+constexpr int kSepChunk = 4;   // channels whose n2 ranges and coefficients are staged at a time
+constexpr int kSepSteps = 4;   // 4-multipole steps unrolled together
+// LDS of one wavefront (8-byte words, 16-byte aligned parts): the two runs
+// [2][KP][16 NT]; per staged channel the candidates' n2 ranges
+// [kSepChunk][64] {off, lo | hi << 16} and the coefficients [kSepChunk][2][KP]
+// (direct, exchange; zero past K); the current channel's candidate runs [64]
+// {rlo * 64 + len, H offset, S offset, column base} (16 bytes each)
+__host__ __device__ constexpr int sep_wave_lds(int K, int nt)
+{
+    return 2 * sep_kp(K) * 16 * nt + kSepChunk * 64 + kSepChunk * 2 * sep_kp(K) + 2 * 64;
+}
+
+template <int NT>
+__device__ __forceinline__ void fill_row_separated(const CBlockArgs& A, int i, int n_a, int n_bb, int l_a, int l_b,
+                                                   int ch_i, int lane, double* wl)
+{
+    typedef double d4_t __attribute__((ext_vector_type(4)));
+    typedef int int4v __attribute__((ext_vector_type(4)));
+    typedef int int2v __attribute__((ext_vector_type(2)));
+    constexpr int LD = 16 * NT;   // doubles per multipole row of a run
+    const int ks = A.ks, K = A.K, N = A.n_b;
+    const int KP = sep_kp(K);
+    double* run1 = wl;                                                  // [KP][LD]
+    double* run2 = run1 + KP * LD;                                      // [KP][LD]
+    int2v* sseg = (int2v*)(run2 + KP * LD) + lane;                      // [kSepChunk][64], this lane's column
+    double* scoef = run2 + KP * LD + kSepChunk * kWave;                 // [kSepChunk][2][KP]
+    int4v* sinfo = (int4v*)(scoef + kSepChunk * 2 * KP);                // [64]
+    const bool a_low = n_bb > n_a;
+    const int d1lo = n_bb - ks + 1 < 1 ? 1 : n_bb - ks + 1, d1hi = n_bb + ks - 1 > N ? N : n_bb + ks - 1;
+    const int d2lo = n_a - ks + 1 < 1 ? 1 : n_a - ks + 1, d2hi = n_a + ks - 1 > N ? N : n_a + ks - 1;
+    const int len1 = d1hi - d1lo + 1, len2 = d2hi - d2lo + 1;
+    {
+        // n_a < n_b: a direct p = (n_a, n_c) lies left of its q = (n_d, n_b)
+        // (Ptot_p Qtot_q) and an exchange p = (n_b, n_c) right of its q =
+        // (n_d, n_a) (Qtot_p Ptot_q): run 1 holds Qtot, run 2 Ptot; the
+        // other way round for n_a > n_b
+        const double* src1 = (a_low ? A.QtotT : A.PtotT) + gload(A.c_first + n_bb);
+        const double* src2 = (a_low ? A.PtotT : A.QtotT) + gload(A.c_first + n_a);
+        const unsigned np = (unsigned)A.n_pairs;   // K * n_pairs < 2^31 (checked on the host)
+        // consecutive pairs on consecutive lanes; four loads in flight per
+        // lane before their LDS writes (2 KP LD is a multiple of 64)
+        for (int e0 = 0; e0 < 2 * KP * LD; e0 += 4 * kWave) {
+            double v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int e = e0 + j * kWave + lane;
+                const int r = e >= KP * LD, f = e - r * KP * LD, kk = f / LD, o = f - kk * LD;
+                v[j] = 0.0;
+                if (e < 2 * KP * LD && kk < K && o < (r ? len2 : len1))
+                    v[j] = gload((r ? src2 : src1) + ((unsigned)kk * np + (unsigned)o));
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (e0 + j * kWave < 2 * KP * LD) run1[e0 + j * kWave + lane] = v[j];   // run2 follows run1
+        }
+    }
+    const int li = lane & 15, lk = lane >> 4;
+    // candidates n_c in column order, one per lane: the lower range first
+    const int lo_start = a_low ? d2lo : d1lo, lo_len = a_low ? len2 : len1;
+    const int hi_start = a_low ? d1lo : d2lo, hi_len = a_low ? len1 : len2;
+    const bool cand = lane < lo_len + hi_len;
+    const int my_c = lane < lo_len ? lo_start + lane : hi_start + (lane - lo_len);
+    const bool my_dir = a_low ? lane < lo_len : lane >= lo_len;   // within k of n_a: the direct term
+    const int wlo = my_dir ? d1lo : d2lo, whi = my_dir ? d1hi : d2hi;
+    const int dir_base = a_low ? 0 : lo_len, ex_base = a_low ? lo_len : 0;   // first candidate lane of each term
+    const int64_t baseH = uniform64(A.ptrH[i]), baseS = uniform64(A.ptrS[i]);
+    int64_t nH = 0, nS = 0;
+    // Vector loads complete in order with the stores issued before them, so
+    // a vector load after a channel's stores would wait for those: the
+    // candidates' n2 ranges and the coefficients of kSepChunk channels are
+    // staged in LDS before their stores, and the channel table and flags come
+    // through the scalar cache.
+    const int4v* segc = (const int4v*)A.seg + my_c;   // BlockSeg {off, lo, hi, pad} of (channel, n_c)
+    typedef const __attribute__((address_space(4))) BlockChan CChan;
+    typedef const __attribute__((address_space(4))) int32_t CInt;
+    for (int ch = 0; ch < A.n_chan; ++ch) {
+        const int slot = ch % kSepChunk;
+        if (slot == 0) {   // all loads in flight before the first LDS write
+            int4v qq[kSepChunk];
+#pragma unroll
+            for (int c = 0; c < kSepChunk; ++c) {   // (past the last channel: the last one again)
+                const int cc = ch + c < A.n_chan ? ch + c : A.n_chan - 1;
+                qq[c] = cand ? gload(segc + (size_t)cc * (N + 1)) : int4v{0, 1, 0, 0};
+            }
+#pragma unroll
+            for (int c = 0; c < kSepChunk; ++c) sseg[c * kWave] = int2v{qq[c].x, (qq[c].y & 0xffff) | (qq[c].z << 16)};
+            // [c][t][kk] from the coefficient table's [ch_i][ch + c][t][k]
+            // (2 KP <= 128: two elements per lane and channel)
+            const double* cgc = A.coef + ((size_t)ch_i * A.n_chan + ch) * 2 * K;
+#pragma unroll 1
+            for (int c0 = 0; c0 < kSepChunk; c0 += 2) {   // two channels' loads in flight at a time
+                double v[2][2];
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        const int e = j * kWave + lane, t = e >= KP, kk = e - t * KP;
+                        v[c][j] = 0.0;
+                        if (e < 2 * KP && ch + c0 + c < A.n_chan && kk < K)
+                            v[c][j] = gload(cgc + (size_t)(2 * (c0 + c) + t) * K + kk);
+                    }
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        if (j * kWave + lane < 2 * KP) scoef[(c0 + c) * 2 * KP + j * kWave + lane] = v[c][j];
+            }
+            wave_lds_sync();   // the runs and the coefficients are read across lanes
+        }
+        const int2v qs = sseg[slot * kWave];
+        const int qoff = qs.x, qlo = qs.y & 0xffff, qhi = qs.y >> 16;   // n_b < 2^15 (checked on the host)
+        const CChan& cjr = ((CChan*)A.chan)[ch];
+        const int c_l1 = cjr.l1, c_l2 = cjr.l2, c_start = cjr.start;
+        const int fl = ((CInt*)A.flags)[ch_i * A.n_chan + ch];
+        const bool dir_any = fl & 1, ex_any = (fl >> 1) & 1;
+        const bool l_eq = (l_a == c_l1) && (l_b == c_l2);
+        const bool l_eq_ex = (l_a == c_l2) && (l_b == c_l1);
+        const bool hd = dir_any || l_eq || l_eq_ex, he = ex_any || l_eq || l_eq_ex;
+        if (!hd && !he) continue;   // no entry can exist in this channel
+        // this lane's candidate: its run of n_d [rlo, rlo + len) and the
+        // column of (n_c, n_d), colb + n_d
+        int rlo = 1, len = 0, colb = 0;
+        if (cand) {
+            int lo = wlo > qlo ? wlo : qlo;
+            if (!A.full) {   // column j = start + off + (n_d - lo2) >= i
+                const int t = i - c_start - qoff + qlo;
+                lo = t > lo ? t : lo;
+            }
+            const int hi = whi < qhi ? whi : qhi;
+            if (hi >= lo) {
+                rlo = lo;
+                len = hi - lo + 1;
+            }
+            colb = c_start + qoff - qlo;
+        }
+        // H and S counts in one prefix sum (a row's entries in one channel
+        // number fewer than 2^16: at most 62 runs of at most 31)
+        const int cHS = ((my_dir ? hd : he) ? len : 0) | ((my_dir ? l_eq : l_eq_ex) ? len << 16 : 0);
+        const int inc = wave_incl_sum(cHS, lane);
+        const int tot = __builtin_amdgcn_readlane(inc, kWave - 1);
+        const int totH = tot & 0xffff, totS = tot >> 16;
+        if (totH == 0) continue;
+        // the candidates' runs for the tile lanes (the previous channel's
+        // reads of this array precede these writes: one wavefront, in-order LDS)
+        const int exc = inc - cHS;
+        sinfo[lane] = int4v{rlo * 64 + len, exc & 0xffff, exc >> 16, colb};
+        wave_lds_sync();
+        const bool odd = ((c_l1 + c_l2 + A.L) & 1) != 0;
+        // both terms' tiles in one pass over the multipoles: the direct term's
+        // operands are run 2 x cdir (rows) and run 1 (columns), the exchange
+        // term's run 1 x cex and run 2, so one LDS read of each run serves both
+        d4_t accd[NT][NT], acce[NT][NT];
+#pragma unroll
+        for (int tm = 0; tm < NT; ++tm)
+#pragma unroll
+            for (int tn = 0; tn < NT; ++tn) accd[tm][tn] = acce[tm][tn] = d4_t{0.0, 0.0, 0.0, 0.0};
+        const bool r12d = dir_any, r12e = ex_any;
+        if (r12d || r12e) {   // (a term without r12 gets a tile it does not read)
+            const double* cdir = scoef + slot * 2 * KP;
+            const double* cex = cdir + KP;
+            // one step: multipoles kk0 .. kk0 + 3, this lane's kk0 + lk
+            auto step = [&](int kk) {
+                double v1[NT], v2[NT];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    v1[t] = run1[kk * LD + 16 * t + li];
+                    v2[t] = run2[kk * LD + 16 * t + li];
+                }
+                const double cd = cdir[kk], ce = cex[kk];
+#pragma unroll
+                for (int tm = 0; tm < NT; ++tm)
+#pragma unroll
+                    for (int tn = 0; tn < NT; ++tn) {
+                        accd[tm][tn] = __builtin_amdgcn_mfma_f64_16x16x4f64(v2[tm] * cd, v1[tn], accd[tm][tn], 0, 0, 0);
+                        acce[tm][tn] = __builtin_amdgcn_mfma_f64_16x16x4f64(v1[tm] * ce, v2[tn], acce[tm][tn], 0, 0, 0);
+                    }
+            };
+            int kk0 = 0;
+            for (; kk0 + 4 * kSepSteps <= KP; kk0 += 4 * kSepSteps) {   // whole groups of steps, unrolled
+#pragma unroll
+                for (int j = 0; j < kSepSteps; ++j) step(kk0 + 4 * j + lk);
+            }
+            for (; kk0 < KP; kk0 += 4) step(kk0 + lk);
+        }
+        // result row lk + 4 r (candidate n_c), column li (n_d) of each tile.
+        // The channel's outputs start at CSR positions hbase, sbase (uniform
+        // pointers); an entry's position within them is a 32-bit offset.
+        const int64_t hbase = baseH + nH, sbase = baseS + nS;
+        const int64_t hroom = A.limH - hbase, sroom = A.limS - sbase;   // entries the outputs hold from there
+        const int hlim = hroom < 0 ? 0 : hroom > 0x7fffffff ? 0x7fffffff : (int)hroom;
+        const int slim = sroom < 0 ? 0 : sroom > 0x7fffffff ? 0x7fffffff : (int)sroom;
+        int64_t* const ixh = A.ixH + hbase;
+        dbl2* const dh = (dbl2*)A.dH + hbase;
+        int64_t* const ixs = A.ixS + sbase;
+        dbl2* const ds = (dbl2*)A.dS + sbase;
+#pragma unroll 1
+        for (int term = 0; term < 2; ++term) {
+            const bool dir = term == 0;
+            if (!(dir ? hd : he)) continue;
+            const bool r12 = dir ? r12d : r12e, one_el = dir ? l_eq : l_eq_ex;
+            const int rows = dir ? len2 : len1, row_lo = dir ? d2lo : d1lo, col_lo = dir ? d1lo : d2lo;
+            const int cbase = dir ? dir_base : ex_base;
+            int4v infos[NT][4];   // the runs of this lane's tile rows, read before any store
+#pragma unroll
+            for (int tm = 0; tm < NT; ++tm)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = 16 * tm + lk + 4 * r;
+                    infos[tm][r] = sinfo[cbase + m < kWave ? cbase + m : kWave - 1];
+                }
+#pragma unroll
+            for (int tm = 0; tm < NT; ++tm)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = 16 * tm + lk + 4 * r;
+                    const int4v info = infos[tm][r];
+                    const int s_lo = info.x >> 6, s_len = info.x & 63;
+                    const int n_c = row_lo + m;
+#pragma unroll
+                    for (int tn = 0; tn < NT; ++tn) {
+                        const int n_d = col_lo + 16 * tn + li;
+                        const int o = n_d - s_lo;
+                        if (m < rows && o >= 0 && o < s_len) {
+                            double res = 0.0;
+                            if (r12) res = res + (dir ? accd[tm][tn][r] : acce[tm][tn][r]);
+                            Cplx h{res, 0.0};
+                            const int64_t col = info.w + n_d + 1;
+                            if (one_el) {
+                                Cplx h1, s1;
+                                one_electron(A, n_a, n_bb, n_c, n_d, l_a, l_b, l_eq, l_eq_ex, odd, h1, s1);
+                                h = cadd(h, h1);
+                                const unsigned ps = (unsigned)(info.z + o);
+                                if ((int)ps < slim) {
+                                    gstore(ixs + ps, col);
+                                    gstore(ds + ps, dbl2{s1.re, s1.im});
+                                }
+                            }
+                            const unsigned ph = (unsigned)(info.y + o);
+                            if ((int)ph < hlim) {
+                                gstore(ixh + ph, col);
+                                gstore(dh + ph, dbl2{h.re, h.im});
+                            }
+                        }
+                    }
+                }
+        }
+        nH += totH;
+        nS += totS;
+    }
+}
+
+// One wavefront per configuration row of any of n_blocks symmetry blocks:
+// wavefront w takes global row rows[w] (w < n), which belongs to the last
+// block b with row0[b] <= row and is row row - row0[b] of it; each block's
+// tables, inputs and outputs are in tab[b].  SEP: the rows are separated ones
+// (NT tiles per window side, 2k - 1 <= 16 NT); otherwise they are walked.
+// The LDS (sep_wave_lds or fill_wave_lds words per wavefront) is dynamic; ks
+// and K are the blocks' common spline order and multipole count.  Entries go
+// to the row's CSR range from the count pass; stores beyond the set's
+// outputs (limH, limS entries) are dropped, so a count that disagreed with
+// the fill could not write outside them.
+// (the separated-row launch with one tile per side is held to 128 vector
+// registers, four wavefronts per SIMD, as many as its LDS allows)
+template <bool SEP, int NT>
+__global__ __launch_bounds__(kCThreads, SEP && NT == 1 ? 4 : 1) void kc_block(const BlockArgs* __restrict__ tab,
+                                                                             const int64_t* __restrict__ row0,
+                                                                             int n_blocks,
+                                                                             const int32_t* __restrict__ rows,
+                                                                             int64_t n, int lds_ks, int lds_K)
+{
+    extern __shared__ __align__(16) double lds_fill[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
+    const int64_t w = (int64_t)blockIdx.x * (kCThreads / kWave) + wave;
+    if (w >= n) return;   // whole wave exits together
+    const int64_t g = rows[w];
+    int blo = 0, bhi = n_blocks - 1;
+    while (blo < bhi) {
+        const int mid = (blo + bhi + 1) >> 1;
+        if (row0[mid] <= g) blo = mid;
+        else bhi = mid - 1;
+    }
+    const int b = __builtin_amdgcn_readfirstlane(blo);
+    // The descriptor is read through the constant address space (the host
+    // writes it before the launch; nothing in the launch writes it), so the
+    // compiler may load a field again where it is used instead of keeping all
+    // of them in scalar registers and spilling them to vector lanes.
+    const CBlockArgs& A = ((const CBlockArgs*)tab)[b];
+    const int i = (int)(g - row0[b]);
+    const int64_t* ci = A.cfg + 5 * (size_t)i;
+    const int n_a = __builtin_amdgcn_readfirstlane((int)ci[0]), n_bb = __builtin_amdgcn_readfirstlane((int)ci[1]);
+    const int l_a = __builtin_amdgcn_readfirstlane((int)ci[2]), l_b = __builtin_amdgcn_readfirstlane((int)ci[3]);
+    const int ch_i = __builtin_amdgcn_readfirstlane(A.cfg_chan[i]);
+    if (SEP)
+        fill_row_separated<NT>(A, i, n_a, n_bb, l_a, l_b, ch_i, lane, lds_fill + (size_t)wave * sep_wave_lds(lds_K, NT));
+    else
+        fill_row_general(A, i, n_a, n_bb, l_a, l_b, ch_i, lane, lds_fill + (size_t)wave * fill_wave_lds(lds_ks, lds_K),
+                         lds_ks, lds_K);
 }
 
 // Row counts of the count pass, in closed form.  Within one column channel,
@@ -556,16 +894,37 @@ hipError_t launch_row_scan(const int64_t* cnt, int64_t* ptr, int64_t n_rows, int
     return hipGetLastError();
 }
 
-hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, int ks, int K,
-                             hipStream_t s)
+hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, const int32_t* rows,
+                             int64_t n_sep, int64_t n_rows, int ks, int K, hipStream_t s)
 {
-    const int64_t threads = n_rows * kWave;
-    const size_t lds = sizeof(double) * (size_t)(kCThreads / kWave) * (size_t)fill_wave_lds(ks, K);
-    hipError_t e = allow_lds((const void*)kc_block, lds);   // <= 4 x 32 KiB (k <= 16, K <= 64)
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kc_block, dim3((unsigned)((threads + kCThreads - 1) / kCThreads)), dim3(kCThreads), lds,
-                       s, tab, row0, n_blocks, n_rows, ks, K);
-    return hipGetLastError();
+    const int nt = 2 * ks - 1 <= 16 ? 1 : 2;
+    const int64_t n_gen = n_rows - n_sep;
+    hipError_t e = hipSuccess;
+    if (n_sep > 0) {
+        const unsigned grid = (unsigned)((n_sep * kWave + kCThreads - 1) / kCThreads);
+        const void* fn = nt == 1 ? (const void*)kc_block<true, 1> : (const void*)kc_block<true, 2>;
+        const size_t lds = sizeof(double) * (size_t)(kCThreads / kWave) * (size_t)sep_wave_lds(K, nt);
+        e = allow_lds(fn, lds);   // <= 4 x 39 KiB (k <= 16, K <= 64)
+        if (e != hipSuccess) return e;
+        if (nt == 1)
+            hipLaunchKernelGGL((kc_block<true, 1>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks, rows,
+                               n_sep, ks, K);
+        else
+            hipLaunchKernelGGL((kc_block<true, 2>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks, rows,
+                               n_sep, ks, K);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (n_gen > 0) {
+        const unsigned grid = (unsigned)((n_gen * kWave + kCThreads - 1) / kCThreads);
+        const size_t lds = sizeof(double) * (size_t)(kCThreads / kWave) * (size_t)fill_wave_lds(ks, K);
+        e = allow_lds((const void*)kc_block<false, 1>, lds);   // <= 4 x 32 KiB (k <= 16, K <= 64)
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((kc_block<false, 1>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks,
+                           rows + n_sep, n_gen, ks, K);
+        e = hipGetLastError();
+    }
+    return e;
 }
 
 }  // namespace bs2e

@@ -53,28 +53,6 @@ bool mixed_supported(int64_t ks, int64_t kgl, int64_t K)
     return ks <= 16 && kgl <= kMxQ && K <= kMaxK && mixed_lds_bytes(ks, K) <= kLdsPerWorkgroup;
 }
 
-// Slices of the n values x[0 .. n) (all >= 0, zero-padded to kMxQ) with the
-// scale of their largest: dst[s * sstride + q * qstride] = d_{s+1} of value q;
-// returns e.
-__device__ __forceinline__ int mx_split(const double* x, int n, float* dst, int sstride, int qstride)
-{
-    double m = 0.0;
-    for (int q = 0; q < n; ++q) m = fmax(m, x[q]);
-    int e = 0;
-    if (m > 0.0) (void)frexp(m, &e);   // m < 2^e
-    for (int q = 0; q < kMxQ; ++q) {
-        double y = q < n ? ldexp(x[q], -e) : 0.0;   // in [0, 1)
-#pragma unroll
-        for (int s = 0; s < kMxSlices; ++s) {
-            y = ldexp(y, kMxBeta);
-            const double d = rint(y);
-            y -= d;   // exact: the integer part and the remainder of a double
-            dst[s * sstride + q * qstride] = (float)d;
-        }
-    }
-    return e;
-}
-
 // One workgroup per interval.  LDS: spline values and points (FP64), then the
 // slices of the pair densities sA[s][q][UP], of the moments r^k sP[s][q][KP]
 // and r^-(k+1) sQ[s][q][KP] (FP32), and the row / column exponents.
@@ -116,56 +94,70 @@ __global__ __launch_bounds__(kMxThreads) void kp_outer_mx(int K, int kgl, const 
     // points too few bits, and the interval's moments are summed in FP64 as
     // the plain path sums them (the same operations, so the same bits).
     if ((double)K * log2(sR[kgl - 1] / sR[0]) > (double)kMxRange) {
+        double* sPow = (double*)sA;          // [kgl][K]  r^k        (the slice area is free here)
+        double* sInv = sPow + kgl * K;       // [kgl][K]  1 / r^(k+1)
+        for (int e = tid; e < kgl * K; e += kMxThreads) {
+            const int q = e / K, k = e - q * K;
+            const double r = sR[q];
+            sPow[e] = ipow_d(r, k);
+            sInv[e] = 1.0 / ipow_d(r, k + 1);
+        }
+        __syncthreads();
         for (int e = tid; e < U * K; e += kMxThreads) {
             const int u = e / K, k = e - u * K;
-            int ta = 0, base = 0;
-            while (base + (KS - ta) <= u) {
-                base += KS - ta;
-                ++ta;
-            }
-            const int tc = ta + (u - base);
+            int ta, tc;
+            local_pair(u, KS, ta, tc);
             double accp = 0.0, accq = 0.0;
             for (int q = 0; q < kgl; ++q) {
                 const double bb = w[q] * sB[q * KS + ta] * sB[q * KS + tc];
-                const double r = sR[q];
-                accp = accp + bb * ipow_d(r, k);
-                accq = accq + bb * (1.0 / ipow_d(r, k + 1));
+                accp = accp + bb * sPow[q * K + k];
+                accq = accq + bb * sInv[q * K + k];
             }
             Pl[((size_t)v * U + u) * K + k] = h * accp;
             Ql[((size_t)v * U + u) * K + k] = h * accq;
         }
         return;
     }
-    // slices: one thread per pair-density row, per moment column
-    for (int e = tid; e < UP + 2 * KP; e += kMxThreads) {
-        double col[kMxQ];
-        if (e < UP) {
-            const int u = e;
-            if (u < U) {
-                int ta = 0, base = 0;   // local pair u = (ta, tc), row-major upper triangle
-                while (base + (KS - ta) <= u) {
-                    base += KS - ta;
-                    ++ta;
-                }
-                const int tc = ta + (u - base);
-                for (int q = 0; q < kgl; ++q) col[q] = w[q] * sB[q * KS + ta] * sB[q * KS + tc];
-                eA[u] = mx_split(col, kgl, sA + u, kMxQ * UP, UP);
-            } else {
-                eA[u] = 0;
-                mx_split(col, 0, sA + u, kMxQ * UP, UP);
+    // slices: sixteen threads per pair-density row or moment column, one per
+    // GL point (zero beyond k_GL); the column's scale is the exponent of its
+    // largest value, found across the sixteen lanes
+    for (int e = tid; e < kMxQ * (UP + 2 * KP); e += kMxThreads) {
+        const int col = e >> 4, q = e & (kMxQ - 1);
+        double val = 0.0;
+        float* dst;
+        int* ex;
+        int stride;
+        if (col < UP) {
+            const int u = col;
+            if (u < U && q < kgl) {
+                int ta, tc;
+                local_pair(u, KS, ta, tc);
+                val = w[q] * sB[q * KS + ta] * sB[q * KS + tc];
             }
+            dst = sA + q * UP + u;
+            ex = eA + u;
+            stride = kMxQ * UP;
         } else {
-            const bool inv = e >= UP + KP;
-            const int k = e - UP - (inv ? KP : 0);
-            float* dst = (inv ? sQ : sP) + k;
-            int* ex = inv ? eQ : eP;
-            if (k < K) {
-                for (int q = 0; q < kgl; ++q) col[q] = inv ? 1.0 / ipow_d(sR[q], k + 1) : ipow_d(sR[q], k);
-                ex[k] = mx_split(col, kgl, dst, kMxQ * KP, KP);
-            } else {
-                ex[k] = 0;
-                mx_split(col, 0, dst, kMxQ * KP, KP);
-            }
+            const bool inv = col >= UP + KP;
+            const int k = col - UP - (inv ? KP : 0);
+            if (k < K && q < kgl) val = inv ? 1.0 / ipow_d(sR[q], k + 1) : ipow_d(sR[q], k);
+            dst = (inv ? sQ : sP) + q * KP + k;
+            ex = (inv ? eQ : eP) + k;
+            stride = kMxQ * KP;
+        }
+        double m = val;
+#pragma unroll
+        for (int o = 1; o < kMxQ; o <<= 1) m = fmax(m, __shfl_xor(m, o, kMxQ));
+        int ee = 0;
+        if (m > 0.0) (void)frexp(m, &ee);   // m < 2^ee
+        if (q == 0) *ex = ee;
+        double y = ldexp(val, -ee);          // in [0, 1)
+#pragma unroll
+        for (int sl = 0; sl < kMxSlices; ++sl) {
+            y = ldexp(y, kMxBeta);
+            const double d = rint(y);
+            y -= d;   // exact: the integer part and the remainder of a double
+            dst[sl * stride] = (float)d;
         }
     }
     __syncthreads();

@@ -29,18 +29,6 @@
 
 namespace bs2e {
 
-__device__ __forceinline__ void local_pair(int u, int ks, int& ta, int& tc)
-{
-    // inverse of sym_local: row-major upper triangle
-    int lo = 0, base = 0;
-    while (base + (ks - lo) <= u) {
-        base += ks - lo;
-        ++lo;
-    }
-    ta = lo;
-    tc = lo + (u - base);
-}
-
 constexpr int kPThreads = 256;
 
 hipError_t allow_lds(const void* fn, size_t bytes)

@@ -116,6 +116,35 @@ def test_block_order_16_many_multipoles(ctx, oracle, L):
     compare(H_sp, S_sp, oracle.construct_block(k, n_b, L, max_k, True, term.configs, Rref, Hl, S))
 
 
+@pytest.mark.parametrize("k,k_gl,max_k,L,full", [(9, 12, 2, 0, True), (12, 14, 3, 1, False), (7, 10, 2, 1, True)])
+def test_block_separated_rows(ctx, oracle, k, k_gl, max_k, L, full):
+    """Rows whose electrons sit at least 2k - 1 splines apart take the
+    matrix-core fill: one 16 x 16 tile per window side for order 7, two for
+    orders 9 and 12 (2k - 1 > 16).  A basis long enough to hold such rows;
+    patterns bit-exact and values against the oracle, with the rows of both
+    kinds (separated and not) present."""
+    t = bs2e.generate_grid(k, 3, 2, 0.8, 40.0)
+    n_b = oracle.dims(k, t)[1]
+    max_l = 1
+    print(f"block separated rows k={k} n_b={n_b} max_k={max_k} L={L} full={full}", flush=True)
+    rk = bs2e.RkTensor.build(k, t, k_gl, max_k, ctx=ctx)
+    Rref = oracle.rk_dense(k, n_b, max_k, oracle.slater_off_diag(k, t, k_gl, max_k),
+                           oracle.slater_diag(k, t, k_gl, max_k))
+    S = oracle.setup_S(k, t, k_gl)
+    Hl = [oracle.setup_H1p(k, t, k_gl, 2, l, cap_eta=0.0) for l in range(max_l + 1)]
+    term = bs2e.sym(l=L, pi=bool(L % 2))
+    term.configs = bs2e.count_configs(term, max_l, n_b, k, n_b, n_b, max_l)
+    gap = np.abs(term.configs[:, 0] - term.configs[:, 1])
+    assert (gap >= 2 * k - 1).sum() > 100 and (gap < 2 * k - 1).any()
+    ref = oracle.construct_block(k, n_b, L, max_k, full, term.configs, Rref, Hl, S)
+    blk = bs2e.Block(rk, term, full)
+    assert (blk.nnz_H, blk.nnz_S) == ref["nnz"] and (blk.ref_nnz_H, blk.ref_nnz_S) == ref["ref_count"]
+    H_sp, S_sp = blk.fill(Hl, S)
+    blk.release()
+    rk.release()
+    compare(H_sp, S_sp, ref)
+
+
 RANDOM_BASES = [   # seed, order, k_GL, max_l, L, full
     (1, 2, 5, 1, 0, True), (2, 3, 6, 2, 1, False), (3, 4, 7, 3, 2, True), (4, 5, 8, 2, 3, False),
     (5, 6, 9, 1, 1, True), (6, 9, 12, 2, 0, False), (7, 11, 14, 1, 2, True), (8, 12, 15, 2, 1, False),
