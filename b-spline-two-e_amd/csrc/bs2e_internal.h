@@ -211,8 +211,8 @@ hipError_t launch_radial_dipole(const Plan& p, const DevTables& d, double* B, do
 // Count / fill launches over the rows of n_blocks blocks (device table tab,
 // device row offsets row0[n_blocks + 1], n_rows = row0[n_blocks]).
 hipError_t launch_block_count(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, hipStream_t s);
-// The fill takes the rows in the order of rows[n_rows]: the n_sep separated
-// rows (|n1 - n2| >= 2k - 1) first, then the others.
+// The fill takes the rows in the order of rows[n_rows]: the n_sep staged
+// rows (|n1 - n2| >= k, on the matrix cores) first, then the others.
 hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, const int32_t* rows,
                              int64_t n_sep, int64_t n_rows, int ks, int K, hipStream_t s);
 // Exclusive prefix sums of the count pass's per-row counts (cnt: H counts then

@@ -140,8 +140,8 @@ struct bs2e_blockset {
     int64_t* d_scan = nullptr;     // the row-offset scan's tile totals
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    // the fill's rows: the separated ones (|n1 - n2| >= 2k - 1, the
-    // matrix-core fill) first, then the others, each ascending
+    // the fill's rows: the staged ones (|n1 - n2| >= k, the matrix-core
+    // fill) first, then the others, each ascending
     std::vector<int32_t> rowlist;
     int64_t n_sep = 0;
     int32_t* d_rowlist = nullptr;
@@ -305,9 +305,9 @@ static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* 
         return fail(ctx, BS2E_EINVAL, std::string(who) + ": a set holds fewer than 2^31 configurations");
     b->rowlist.clear();
     b->rowlist.reserve((size_t)N);
-    for (int pass = 0; pass < 2; ++pass) {   // separated rows, then the others
+    for (int pass = 0; pass < 2; ++pass) {   // staged rows, then the others
         for (int64_t g = 0; g < N; ++g)
-            if ((std::llabs(configs[5 * g] - configs[5 * g + 1]) >= 2 * p.k_s - 1) == (pass == 0))
+            if ((std::llabs(configs[5 * g] - configs[5 * g + 1]) >= p.k_s) == (pass == 0))
                 b->rowlist.push_back((int32_t)g);
         if (pass == 0) b->n_sep = (int64_t)b->rowlist.size();
     }

@@ -40,6 +40,7 @@ struct OneBody {
     int refs = 0;                 // jobs admitted with it and not retired
     uint64_t last = 0;            // admission tick of its last use
     size_t bytes = 0;
+    const void* maker = nullptr;  // the job it was computed for, until a job is admitted with it
 };
 
 size_t onebody_bytes(const Plan& p, int64_t n_l)
@@ -97,13 +98,14 @@ static void onebody_free(bs2e_server* s, OneBody& o)
 
 // Frees the cache entries no job holds, least recently used first, until
 // `need` bytes fit the largest free range; returns whether any were freed.
-static bool cache_evict(bs2e_server* s, size_t need)
+// `keep` (the entry of the job being admitted, if any) is spared.
+static bool cache_evict(bs2e_server* s, size_t need, const OneBody* keep = nullptr)
 {
     bool any = false;
     while (s->arena.largest_free() < need) {
         auto victim = s->cache.end();
         for (auto it = s->cache.begin(); it != s->cache.end(); ++it)
-            if (it->refs == 0 && (victim == s->cache.end() || it->last < victim->last)) victim = it;
+            if (it->refs == 0 && &*it != keep && (victim == s->cache.end() || it->last < victim->last)) victim = it;
         if (victim == s->cache.end()) break;
         onebody_free(s, *victim);
         s->cache.erase(victim);
@@ -154,6 +156,7 @@ static int64_t cache_make(bs2e_server* s, Job& j, bs2e_ctx* c, OneBody** out)
         return fail(s->err, BS2E_EDEVICE, std::string("bs2e_server: one-electron tables: ") + hipGetErrorString(e));
     }
     o.made_on = c->stream;
+    o.maker = &j;
     *out = &o;
     return BS2E_OK;
 }
@@ -202,12 +205,9 @@ static int64_t admit(bs2e_server* s, Job& j, int slot)
         rc = blockset_new(j.rk, j.n_sym, j.L.data(), j.n_config.data(), j.configs.data(), j.full, &j.set);
         OneBody* ob = rc ? nullptr : cache_find(s, j);
         bool made = false;
-        if (!rc && ob) {
-            s->cache_hits += 1;
-        } else if (!rc) {
+        if (!rc && !ob) {
             rc = cache_make(s, j, c, &ob);
             made = rc == BS2E_OK;
-            if (made) s->cache_misses += 1;
         }
         if (!rc) {
             j.ob = ob;
@@ -235,6 +235,10 @@ static int64_t admit(bs2e_server* s, Job& j, int slot)
     if (rc) {
         (void)hipStreamSynchronize(c->stream);
         (void)hipStreamSynchronize(c->side);
+    } else if (j.ob) {   // counted once per admitted job: its basis was computed for it, or cached
+        if (j.ob->maker == &j) s->cache_misses += 1;
+        else s->cache_hits += 1;
+        j.ob->maker = nullptr;
     }
     return rc;
 }
@@ -266,12 +270,12 @@ static int64_t progress(bs2e_server* s)
             }
         if (slot < 0) break;
         Job& j = s->jobs[*it];
-        if (j.bytes > s->arena.largest_free() && !cache_evict(s, j.bytes)) {
+        if (j.bytes > s->arena.largest_free() && !cache_evict(s, j.bytes, cache_find(s, j))) {
             ++it;
             continue;
         }
         int64_t rc = admit(s, j, slot);
-        if (rc == BS2E_ENOMEM && cache_evict(s, j.bytes)) rc = admit(s, j, slot);
+        if (rc == BS2E_ENOMEM && cache_evict(s, j.bytes, cache_find(s, j))) rc = admit(s, j, slot);
         if (rc == BS2E_ENOMEM && j.bytes <= s->arena.capacity()) {   // room later: keep it queued
             ++it;
             continue;

@@ -15,11 +15,11 @@
 // reference's predicate
 //   H entry <=> (support or support_ex) and (r12_allowed or l_eq or l_eq_ex)
 //   S entry <=> (support and l_eq) or (support_ex and l_eq_ex).
-// Rows with |n_a - n_b| >= 2k - 1 (most of a large block) take the
-// matrix-core path (fill_row_separated): per channel the direct and exchange
-// sums are two small matrix products, placed in CSR by a prefix sum over the
-// candidates n_c.  The other rows walk the candidates 64 at a time and
-// ballots give each entry its CSR position.  The angular factors are per
+// Rows with |n_a - n_b| >= k (most of a large block) take the matrix-core
+// path (fill_row_staged): per channel the direct and exchange sums are small
+// matrix products, placed in CSR by a prefix sum over the candidates n_c.
+// The other rows walk the candidates 64 at a time and ballots give each
+// entry its CSR position.  The angular factors are per
 // (row channel, column channel) coefficient vectors:
 //   cdir[k] = ang_k_LS(k, confs, L)               if |ang| >= 5e-16 else 0
 //   cex[k]  = (-1)^(l_c+l_d+L) ang_k_LS(k, confs_ex, L)  (same threshold)
@@ -204,10 +204,6 @@ __host__ __device__ constexpr int fill_kp(int K) { return (K + 1) & ~1; }
 // (plus 64 bytes for the channel walk's marks)
 __host__ __device__ constexpr int fill_wave_lds(int ks, int K) { return 2 * fill_kp(K) * (1 + (2 * ks - 1)) + 8; }
 
-// A row whose electrons sit at least 2k - 1 splines apart: its direct
-// candidates n_c (within k of n_a) and exchange candidates (within k of n_b)
-// are disjoint, and every entry has exactly one of the two supports.
-__device__ __forceinline__ bool separated_row(int n_a, int n_bb, int ks) { return iabs_d(n_bb - n_a) >= 2 * ks - 1; }
 
 typedef const __attribute__((address_space(4))) BlockArgs CBlockArgs;
 
@@ -331,27 +327,6 @@ __device__ __forceinline__ void fill_row_general(const CBlockArgs& A, int i, int
     }
 }
 
-// The fill of the separated rows (|n_a - n_b| >= 2k - 1) on the FP64 matrix
-// cores.  In such a row every R^k product is rank one with a side fixed by
-// the row, and all its factors come from two runs of window totals: run 1,
-// the pairs (d, n_b) for d within k of n_b, and run 2, the pairs (d, n_a) for
-// d within k of n_a (the totals of (a, c) and (c, a) are the same sums).  Per
-// column channel the direct term of entry (n_c, n_d), n_c in run 2's range
-// and n_d in run 1's, is
-//   sum_k run2_k(n_c) cdir_k run1_k(n_d),
-// and the exchange term, n_c in run 1's range and n_d in run 2's,
-//   sum_k run1_k(n_c) cex_k run2_k(n_d):
-// each a (2k-1) x (2k-1) block of one matrix product with inner dimension K,
-// taken in 16 x 16 tiles on v_mfma_f64_16x16x4 (NT x NT tiles, 2k - 1 <= 16
-// NT) from the runs in LDS, multipole-major and zero-padded to whole tiles
-// and to a multiple of four multipoles.  Each candidate n_c's entries are one
-// run of n_d (its term's window, clipped to the channel's n2 range and, for
-// upper triangles, to j >= i), and a prefix sum over the candidates (lanes,
-// in column order) places every run in the row's CSR range, so an entry's
-// position follows from its tile coordinates.  The products are summed in
-// the matrix cores' order with fused multiply-adds, and the coefficient is
-// applied to the p factor: the terms differ from the reference's (x y) c in
-// the last bits.  The one-electron parts are the general fill's, bit for bit.
 // Inclusive prefix sum over the wavefront's 64 lanes: within each row of 16
 // lanes by DPP row shifts (a lane shifted in from outside the row adds 0),
 // then the totals of the rows before, read from lanes 15, 31 and 47.
@@ -367,6 +342,32 @@ __device__ __forceinline__ int wave_incl_sum(int v, int lane)
     return v + (row > 0 ? t0 : 0) + (row > 1 ? t1 : 0) + (row > 2 ? t2 : 0);
 }
 
+// The fill of the staged rows (|n_a - n_b| >= k) on the FP64 matrix cores.
+// In such a row every R^k product is rank one with a side fixed by the row,
+// and all its factors come from two runs of window totals: run 1, the pairs
+// (d, n_b) for d in D1 (within k of n_b), and run 2, the pairs (d, n_a) for
+// d in D2 (within k of n_a); the totals of (a, c) and (c, a) are the same
+// sums.  Per column channel the direct term of entry (n_c, n_d) (support:
+// n_c in D2, n_d in D1) and the exchange term (support_ex: n_c in D1, n_d in
+// D2) are
+//   D[n_c][n_d] = sum_k run2_k(n_c) cdir_k run1_k(n_d),
+//   E[n_c][n_d] = sum_k run1_k(n_c) cex_k run2_k(n_d),
+// each a block of one matrix product with inner dimension K, taken in 16 x 16
+// tiles on v_mfma_f64_16x16x4 (NT x NT tiles per block, 2k - 1 <= 16 NT)
+// from the runs in LDS, multipole-major and zero-padded to whole tiles and to
+// a multiple of four multipoles.  When D1 and D2 overlap (k <= |n_a - n_b| <
+// 2k - 1) an entry can have both supports; a third block, E at the direct
+// block's coordinates (the runs read at shifted offsets), gives such an entry
+// both terms in one lane.  A candidate n_c's H entries are one run of n_d
+// (the union of its direct and exchange windows, as far as the channel's
+// flags allow, clipped to the channel's n2 range and, for upper triangles, to
+// j >= i), its S entries another, and a prefix sum over the candidates
+// (lanes, in column order) places the runs in the row's CSR range, so an
+// entry's position follows from its tile coordinates.  The products are
+// summed in the matrix cores' order with fused multiply-adds, and the
+// coefficient is applied to the p factor: the terms differ from the
+// reference's (x y) c in the last bits.  The one-electron parts are the
+// walked fill's, bit for bit.
 __host__ __device__ constexpr int sep_kp(int K) { return 4 * ((K + 3) / 4); }
 constexpr int kSepChunk = 4;   // channels whose n2 ranges and coefficients are staged at a time
 constexpr int kSepSteps = 4;   // 4-multipole steps unrolled together
@@ -374,15 +375,16 @@ constexpr int kSepSteps = 4;   // 4-multipole steps unrolled together
 // [2][KP][16 NT]; per staged channel the candidates' n2 ranges
 // [kSepChunk][64] {off, lo | hi << 16} and the coefficients [kSepChunk][2][KP]
 // (direct, exchange; zero past K); the current channel's candidate runs [64]
-// {rlo * 64 + len, H offset, S offset, column base} (16 bytes each)
+// {H lo * 64 + H len, H offset | S offset << 16, S lo * 64 + S len, column
+// base} (16 bytes each)
 __host__ __device__ constexpr int sep_wave_lds(int K, int nt)
 {
     return 2 * sep_kp(K) * 16 * nt + kSepChunk * 64 + kSepChunk * 2 * sep_kp(K) + 2 * 64;
 }
 
 template <int NT>
-__device__ __forceinline__ void fill_row_separated(const CBlockArgs& A, int i, int n_a, int n_bb, int l_a, int l_b,
-                                                   int ch_i, int lane, double* wl)
+__device__ __forceinline__ void fill_row_staged(const CBlockArgs& A, int i, int n_a, int n_bb, int l_a, int l_b,
+                                                int ch_i, int lane, double* wl)
 {
     typedef double d4_t __attribute__((ext_vector_type(4)));
     typedef int int4v __attribute__((ext_vector_type(4)));
@@ -425,14 +427,22 @@ __device__ __forceinline__ void fill_row_separated(const CBlockArgs& A, int i, i
         }
     }
     const int li = lane & 15, lk = lane >> 4;
-    // candidates n_c in column order, one per lane: the lower range first
-    const int lo_start = a_low ? d2lo : d1lo, lo_len = a_low ? len2 : len1;
-    const int hi_start = a_low ? d1lo : d2lo, hi_len = a_low ? len1 : len2;
-    const bool cand = lane < lo_len + hi_len;
-    const int my_c = lane < lo_len ? lo_start + lane : hi_start + (lane - lo_len);
-    const bool my_dir = a_low ? lane < lo_len : lane >= lo_len;   // within k of n_a: the direct term
-    const int wlo = my_dir ? d1lo : d2lo, whi = my_dir ? d1hi : d2hi;
-    const int dir_base = a_low ? 0 : lo_len, ex_base = a_low ? lo_len : 0;   // first candidate lane of each term
+    // candidates n_c in column order, one per lane: the lower range (D2 when
+    // n_a < n_b), then the upper, as one run when they meet
+    const int lo_lo = a_low ? d2lo : d1lo, lo_hi = a_low ? d2hi : d1hi;
+    const int up_lo = a_low ? d1lo : d2lo, up_hi = a_low ? d1hi : d2hi;
+    const bool merged = up_lo <= lo_hi + 1;
+    const int lo_len = merged ? (up_hi > lo_hi ? up_hi : lo_hi) - lo_lo + 1 : lo_hi - lo_lo + 1;
+    const int n_cand = merged ? lo_len : lo_len + (up_hi - up_lo + 1);
+    const bool cand = lane < n_cand;
+    const int my_c = lane < lo_len ? lo_lo + lane : up_lo + (lane - lo_len);
+    const bool in2 = cand && iabs_d(my_c - n_a) < ks, in1 = cand && iabs_d(my_c - n_bb) < ks;
+    // the first candidate lane of the direct block's rows (D2) and of the
+    // exchange block's rows (D1)
+    const int dir_base = a_low ? 0 : merged ? d2lo - d1lo : lo_len;
+    const int ex_base = a_low ? (merged ? d1lo - d2lo : lo_len) : 0;
+    const bool both = (d1lo > d2lo ? d1lo : d2lo) <= (d1hi < d2hi ? d1hi : d2hi);   // D1 and D2 overlap
+    const int shift = d2lo - d1lo;   // the third block reads run 1 at m + shift and run 2 at n - shift
     const int64_t baseH = uniform64(A.ptrH[i]), baseS = uniform64(A.ptrS[i]);
     int64_t nH = 0, nS = 0;
     // Vector loads complete in order with the stores issued before them, so
@@ -487,45 +497,53 @@ __device__ __forceinline__ void fill_row_separated(const CBlockArgs& A, int i, i
         const bool l_eq_ex = (l_a == c_l2) && (l_b == c_l1);
         const bool hd = dir_any || l_eq || l_eq_ex, he = ex_any || l_eq || l_eq_ex;
         if (!hd && !he) continue;   // no entry can exist in this channel
-        // this lane's candidate: its run of n_d [rlo, rlo + len) and the
-        // column of (n_c, n_d), colb + n_d
-        int rlo = 1, len = 0, colb = 0;
+        // this lane's candidate: its direct window D1 (if n_c is in D2) and
+        // exchange window D2 (if n_c is in D1), clipped; the H run is the
+        // union of those the channel's flags admit, the S run likewise (two
+        // windows of one candidate overlap, so a union is one run); the
+        // column of (n_c, n_d) is colb + n_d
+        int hlo = 1, hlen = 0, slo = 1, slen = 0, colb = 0;
         if (cand) {
-            int lo = wlo > qlo ? wlo : qlo;
+            int clo = qlo;
             if (!A.full) {   // column j = start + off + (n_d - lo2) >= i
                 const int t = i - c_start - qoff + qlo;
-                lo = t > lo ? t : lo;
+                clo = t > clo ? t : clo;
             }
-            const int hi = whi < qhi ? whi : qhi;
-            if (hi >= lo) {
-                rlo = lo;
-                len = hi - lo + 1;
-            }
+            const int dlo = d1lo > clo ? d1lo : clo, dhi = d1hi < qhi ? d1hi : qhi;
+            const int elo = d2lo > clo ? d2lo : clo, ehi = d2hi < qhi ? d2hi : qhi;
+            const bool dok = in2 && dhi >= dlo, eok = in1 && ehi >= elo;
+            int a0 = 1 << 20, a1 = -1, b0 = 1 << 20, b1 = -1;
+            if (dok && hd) a0 = dlo, a1 = dhi;
+            if (eok && he) a0 = elo < a0 ? elo : a0, a1 = ehi > a1 ? ehi : a1;
+            if (dok && l_eq) b0 = dlo, b1 = dhi;
+            if (eok && l_eq_ex) b0 = elo < b0 ? elo : b0, b1 = ehi > b1 ? ehi : b1;
+            if (a1 >= a0) hlo = a0, hlen = a1 - a0 + 1;
+            if (b1 >= b0) slo = b0, slen = b1 - b0 + 1;
             colb = c_start + qoff - qlo;
         }
         // H and S counts in one prefix sum (a row's entries in one channel
-        // number fewer than 2^16: at most 62 runs of at most 31)
-        const int cHS = ((my_dir ? hd : he) ? len : 0) | ((my_dir ? l_eq : l_eq_ex) ? len << 16 : 0);
+        // number fewer than 2^16: at most 62 runs of at most 61)
+        const int cHS = hlen | (slen << 16);
         const int inc = wave_incl_sum(cHS, lane);
         const int tot = __builtin_amdgcn_readlane(inc, kWave - 1);
         const int totH = tot & 0xffff, totS = tot >> 16;
         if (totH == 0) continue;
         // the candidates' runs for the tile lanes (the previous channel's
         // reads of this array precede these writes: one wavefront, in-order LDS)
-        const int exc = inc - cHS;
-        sinfo[lane] = int4v{rlo * 64 + len, exc & 0xffff, exc >> 16, colb};
+        sinfo[lane] = int4v{hlo * 64 + hlen, inc - cHS, slo * 64 + slen, colb};
         wave_lds_sync();
         const bool odd = ((c_l1 + c_l2 + A.L) & 1) != 0;
-        // both terms' tiles in one pass over the multipoles: the direct term's
+        // the blocks in one pass over the multipoles: the direct block's
         // operands are run 2 x cdir (rows) and run 1 (columns), the exchange
-        // term's run 1 x cex and run 2, so one LDS read of each run serves both
-        d4_t accd[NT][NT], acce[NT][NT];
+        // block's run 1 x cex and run 2, so one LDS read of each run serves
+        // both; the third block (overlapping windows only) reads the runs at
+        // shifted offsets
+        d4_t accd[NT][NT], acce[NT][NT], accx[NT][NT];
 #pragma unroll
         for (int tm = 0; tm < NT; ++tm)
 #pragma unroll
-            for (int tn = 0; tn < NT; ++tn) accd[tm][tn] = acce[tm][tn] = d4_t{0.0, 0.0, 0.0, 0.0};
-        const bool r12d = dir_any, r12e = ex_any;
-        if (r12d || r12e) {   // (a term without r12 gets a tile it does not read)
+            for (int tn = 0; tn < NT; ++tn) accd[tm][tn] = acce[tm][tn] = accx[tm][tn] = d4_t{0.0, 0.0, 0.0, 0.0};
+        if (dir_any || ex_any) {   // (a block without r12 is computed and not read)
             const double* cdir = scoef + slot * 2 * KP;
             const double* cex = cdir + KP;
             // one step: multipoles kk0 .. kk0 + 3, this lane's kk0 + lk
@@ -545,12 +563,29 @@ __device__ __forceinline__ void fill_row_separated(const CBlockArgs& A, int i, i
                         acce[tm][tn] = __builtin_amdgcn_mfma_f64_16x16x4f64(v1[tm] * ce, v2[tn], acce[tm][tn], 0, 0, 0);
                     }
             };
+            auto step_x = [&](int kk) {   // E at direct coordinates: run 1 at n_c - d1lo, run 2 at n_d - d2lo
+                double x1[NT], x2[NT];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const int a = 16 * t + li + shift, c = 16 * t + li - shift;
+                    x1[t] = a >= 0 && a < LD ? run1[kk * LD + a] : 0.0;
+                    x2[t] = c >= 0 && c < LD ? run2[kk * LD + c] : 0.0;
+                }
+                const double ce = cex[kk];
+#pragma unroll
+                for (int tm = 0; tm < NT; ++tm)
+#pragma unroll
+                    for (int tn = 0; tn < NT; ++tn)
+                        accx[tm][tn] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1[tm] * ce, x2[tn], accx[tm][tn], 0, 0, 0);
+            };
             int kk0 = 0;
             for (; kk0 + 4 * kSepSteps <= KP; kk0 += 4 * kSepSteps) {   // whole groups of steps, unrolled
 #pragma unroll
                 for (int j = 0; j < kSepSteps; ++j) step(kk0 + 4 * j + lk);
             }
             for (; kk0 < KP; kk0 += 4) step(kk0 + lk);
+            if (both)
+                for (int k0 = 0; k0 < KP; k0 += 4) step_x(k0 + lk);
         }
         // result row lk + 4 r (candidate n_c), column li (n_d) of each tile.
         // The channel's outputs start at CSR positions hbase, sbase (uniform
@@ -563,12 +598,15 @@ __device__ __forceinline__ void fill_row_separated(const CBlockArgs& A, int i, i
         dbl2* const dh = (dbl2*)A.dH + hbase;
         int64_t* const ixs = A.ixS + sbase;
         dbl2* const ds = (dbl2*)A.dS + sbase;
+        // the direct block emits every entry with the direct support (adding
+        // the exchange term where it also has that support), the exchange
+        // block those with the exchange support alone
 #pragma unroll 1
         for (int term = 0; term < 2; ++term) {
             const bool dir = term == 0;
-            if (!(dir ? hd : he)) continue;
-            const bool r12 = dir ? r12d : r12e, one_el = dir ? l_eq : l_eq_ex;
-            const int rows = dir ? len2 : len1, row_lo = dir ? d2lo : d1lo, col_lo = dir ? d1lo : d2lo;
+            if (dir ? !(hd || (he && both)) : !he) continue;   // no entry of this block's kind
+            const int rows = dir ? len2 : len1, cols = dir ? len1 : len2;
+            const int row_lo = dir ? d2lo : d1lo, col_lo = dir ? d1lo : d2lo;
             const int cbase = dir ? dir_base : ex_base;
             int4v infos[NT][4];   // the runs of this lane's tile rows, read before any store
 #pragma unroll
@@ -584,28 +622,38 @@ __device__ __forceinline__ void fill_row_separated(const CBlockArgs& A, int i, i
                 for (int r = 0; r < 4; ++r) {
                     const int m = 16 * tm + lk + 4 * r;
                     const int4v info = infos[tm][r];
-                    const int s_lo = info.x >> 6, s_len = info.x & 63;
+                    const int h_lo = info.x >> 6, h_len = info.x & 63, s_lo = info.z >> 6;
                     const int n_c = row_lo + m;
 #pragma unroll
                     for (int tn = 0; tn < NT; ++tn) {
                         const int n_d = col_lo + 16 * tn + li;
-                        const int o = n_d - s_lo;
-                        if (m < rows && o >= 0 && o < s_len) {
+                        const int o = n_d - h_lo;
+                        // the other support of this entry
+                        const bool other = dir ? iabs_d(n_c - n_bb) < ks && iabs_d(n_d - n_a) < ks
+                                               : iabs_d(n_c - n_a) < ks && iabs_d(n_d - n_bb) < ks;
+                        // (a candidate's H run can pass this block's columns
+                        // where its other window continues)
+                        if (m < rows && 16 * tn + li < cols && o >= 0 && o < h_len && (dir || !other)) {
+                            const bool sup = dir, sex = dir ? other : true;
+                            const bool r12 = (dir_any && sup) || (ex_any && sex);
                             double res = 0.0;
-                            if (r12) res = res + (dir ? accd[tm][tn][r] : acce[tm][tn][r]);
+                            if (r12) {
+                                res = res + (dir ? accd[tm][tn][r] : acce[tm][tn][r]);
+                                if (dir && other) res = res + accx[tm][tn][r];
+                            }
                             Cplx h{res, 0.0};
                             const int64_t col = info.w + n_d + 1;
-                            if (one_el) {
+                            if ((sup && l_eq) || (sex && l_eq_ex)) {
                                 Cplx h1, s1;
                                 one_electron(A, n_a, n_bb, n_c, n_d, l_a, l_b, l_eq, l_eq_ex, odd, h1, s1);
                                 h = cadd(h, h1);
-                                const unsigned ps = (unsigned)(info.z + o);
+                                const unsigned ps = (unsigned)((info.y >> 16) + (n_d - s_lo));
                                 if ((int)ps < slim) {
                                     gstore(ixs + ps, col);
                                     gstore(ds + ps, dbl2{s1.re, s1.im});
                                 }
                             }
-                            const unsigned ph = (unsigned)(info.y + o);
+                            const unsigned ph = (unsigned)((info.y & 0xffff) + o);
                             if ((int)ph < hlim) {
                                 gstore(ixh + ph, col);
                                 gstore(dh + ph, dbl2{h.re, h.im});
@@ -622,14 +670,15 @@ __device__ __forceinline__ void fill_row_separated(const CBlockArgs& A, int i, i
 // One wavefront per configuration row of any of n_blocks symmetry blocks:
 // wavefront w takes global row rows[w] (w < n), which belongs to the last
 // block b with row0[b] <= row and is row row - row0[b] of it; each block's
-// tables, inputs and outputs are in tab[b].  SEP: the rows are separated ones
-// (NT tiles per window side, 2k - 1 <= 16 NT); otherwise they are walked.
+// tables, inputs and outputs are in tab[b].  SEP: the rows are staged ones
+// (|n_a - n_b| >= k; NT tiles per window side, 2k - 1 <= 16 NT), taken on the
+// matrix cores; otherwise they are walked.
 // The LDS (sep_wave_lds or fill_wave_lds words per wavefront) is dynamic; ks
 // and K are the blocks' common spline order and multipole count.  Entries go
 // to the row's CSR range from the count pass; stores beyond the set's
 // outputs (limH, limS entries) are dropped, so a count that disagreed with
 // the fill could not write outside them.
-// (the separated-row launch with one tile per side is held to 128 vector
+// (the matrix-core launch with one tile per side is held to 128 vector
 // registers, four wavefronts per SIMD, as many as its LDS allows)
 template <bool SEP, int NT>
 __global__ __launch_bounds__(kCThreads, SEP && NT == 1 ? 4 : 1) void kc_block(const BlockArgs* __restrict__ tab,
@@ -662,7 +711,7 @@ __global__ __launch_bounds__(kCThreads, SEP && NT == 1 ? 4 : 1) void kc_block(co
     const int l_a = __builtin_amdgcn_readfirstlane((int)ci[2]), l_b = __builtin_amdgcn_readfirstlane((int)ci[3]);
     const int ch_i = __builtin_amdgcn_readfirstlane(A.cfg_chan[i]);
     if (SEP)
-        fill_row_separated<NT>(A, i, n_a, n_bb, l_a, l_b, ch_i, lane, lds_fill + (size_t)wave * sep_wave_lds(lds_K, NT));
+        fill_row_staged<NT>(A, i, n_a, n_bb, l_a, l_b, ch_i, lane, lds_fill + (size_t)wave * sep_wave_lds(lds_K, NT));
     else
         fill_row_general(A, i, n_a, n_bb, l_a, l_b, ch_i, lane, lds_fill + (size_t)wave * fill_wave_lds(lds_ks, lds_K),
                          lds_ks, lds_K);
