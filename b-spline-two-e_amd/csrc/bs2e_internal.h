@@ -214,7 +214,8 @@ hipError_t launch_block_count(const BlockArgs* tab, const int64_t* row0, int n_b
 // The fill takes the rows in the order of rows[n_rows]: the n_sep staged
 // rows (|n1 - n2| >= k, on the matrix cores) first, then the others.
 hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, const int32_t* rows,
-                             int64_t n_sep, int64_t n_rows, int ks, int K, hipStream_t s);
+                             int64_t n_sep, int64_t n_rows, int ks, int K, hipStream_t s, hipStream_t side,
+                             const hipEvent_t* ev);
 // Exclusive prefix sums of the count pass's per-row counts (cnt: H counts then
 // S counts, n_rows each) into ptr (same layout), on the device; tmp holds
 // row_scan_tmp_elems(n_rows) int64.

@@ -145,6 +145,7 @@ struct bs2e_blockset {
     std::vector<int32_t> rowlist;
     int64_t n_sep = 0;
     int32_t* d_rowlist = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};   // the fill's fork to and join from the second stream
 };
 
 // A single symmetry's block is a set of one.
@@ -191,13 +192,23 @@ static void set_free(bs2e_blockset* b)
     ctx_free(c, b->d_tab);
     ctx_free(c, b->d_row0);
     ctx_free(c, b->d_rowlist);
+    for (hipEvent_t& e : b->ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
 }
 
 static hipError_t set_fill_launch(bs2e_blockset* b, const BlockArgs* tab, hipStream_t st)
 {
     const Plan& p = b->rk->plan;
+    for (hipEvent_t& e : b->ev)
+        if (!e) {
+            const hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+            if (r != hipSuccess) return r;
+        }
+    hipStream_t side = b->rk->ctx->side ? b->rk->ctx->side : st;
     return launch_block_fill(tab, b->d_row0, (int)b->n_sym, b->d_rowlist, b->n_sep, b->n_rows, (int)p.k_s, (int)p.K,
-                             st);
+                             st, side, b->ev);
 }
 
 // descriptor of symmetry s for the count (fill = false) or fill pass
