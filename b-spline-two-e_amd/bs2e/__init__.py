@@ -577,6 +577,23 @@ def _one_electron_arrays(H, S, n_b):
     return Hl, Sf
 
 
+def row_windows(cnt_H, cnt_S, max_bytes):
+    """Consecutive windows (row_lo, n_rows) covering rows 0 .. len(cnt_H)-1,
+    each as long as it stays within max_bytes of CSR entries (24 bytes per H
+    or S entry: column index and complex value); a row that alone needs more
+    is a window of its own."""
+    per = 24 * (np.asarray(cnt_H, np.int64) + np.asarray(cnt_S, np.int64))
+    out, lo, acc = [], 0, 0
+    for r, b in enumerate(per.tolist()):
+        if r > lo and acc + b > max_bytes:
+            out.append((lo, r - lo))
+            lo, acc = r, 0
+        acc += b
+    if len(per) > lo:
+        out.append((lo, len(per) - lo))
+    return out
+
+
 class BlockSet:
     """The H/S blocks of several symmetries against one device tensor, counted
     in one launch and filled in one launch over the configuration rows of all
@@ -643,8 +660,64 @@ class BlockSet:
                                            ptr(ipS, I), ptr(ixS, I), dS.ctypes.data_as(D)), self.ctx.handle)
         return (CSR_matrix((n, n), ipH, ixH[:nH], dH[:nH]), CSR_matrix((n, n), ipS, ixS[:nS], dS[:nS]))
 
+    @property
+    def row0(self):
+        """First global row of each symmetry (and the total row count last)."""
+        return np.concatenate([[0], np.cumsum(self.n_config)]).astype(np.int64)
+
+    def row_counts(self, row_lo=0, n_rows=None):
+        """H and S entries of global rows row_lo .. row_lo+n_rows-1 (the count pass)."""
+        if n_rows is None:
+            n_rows = int(self.row0[-1]) - row_lo
+        I = C.c_int64
+        cH = np.zeros(max(n_rows, 1), np.int64)
+        cS = np.zeros(max(n_rows, 1), np.int64)
+        check(capi.lib().bs2e_blockset_row_counts(self.handle, row_lo, n_rows, ptr(cH, I), ptr(cS, I)),
+              self.ctx.handle)
+        return cH[:n_rows], cS[:n_rows]
+
+    def windows(self, max_bytes):
+        """Consecutive windows (row_lo, n_rows) of global rows covering the
+        set, each holding at most max_bytes of CSR entries (row_windows)."""
+        return row_windows(*self.row_counts(), max_bytes)
+
+    def reserve(self, nnz_H, nnz_S):
+        """Output buffers with room for nnz_H and nnz_S entries (allocated once for a stream of windows)."""
+        check(capi.lib().bs2e_blockset_reserve(self.handle, int(nnz_H), int(nnz_S)), self.ctx.handle)
+
+    def stream(self, max_bytes, H=None, S=None):
+        """Fills the set window by window (windows(max_bytes)), the outputs
+        reserved once for the largest window, yielding (row_lo, n_rows) after
+        each window's fill; H and S as fill_rows (the first window's fill
+        takes them)."""
+        wins = self.windows(max_bytes)
+        cH, cS = self.row_counts()
+        pH = np.concatenate([[0], np.cumsum(cH)])
+        pS = np.concatenate([[0], np.cumsum(cS)])
+        self.reserve(max((int(pH[a + n] - pH[a]) for a, n in wins), default=0),
+                     max((int(pS[a + n] - pS[a]) for a, n in wins), default=0))
+        for k, (lo, n) in enumerate(wins):
+            if k == 0 and H is not None:
+                self.fill_rows(lo, n, H, S)
+            else:
+                self.fill_rows(lo, n)
+            yield lo, n
+
+    def fill_rows(self, row_lo, n_rows, H=None, S=None):
+        """Fills global rows row_lo .. row_lo+n_rows-1 only (the outputs hold
+        that window); H and S None reuse the last fill's one-electron matrices."""
+        D = C.POINTER(C.c_double)
+        if H is None and S is None:
+            check(capi.lib().bs2e_blockset_fill_rows(self.handle, 0, None, None, int(row_lo), int(n_rows)),
+                  self.ctx.handle)
+            return
+        Hl, Sf = _one_electron_arrays(H, S, self.tensor.n_b)
+        check(capi.lib().bs2e_blockset_fill_rows(self.handle, len(H) - 1, Hl.ctypes.data_as(D),
+                                                 Sf.ctypes.data_as(D), int(row_lo), int(n_rows)), self.ctx.handle)
+
     def refill(self):
-        """The fill again from the device copy of the last fill's one-electron matrices (waits)."""
+        """The fill again (all rows, or the last window) from the device copy of the last fill's one-electron
+        matrices (waits)."""
         check(capi.lib().bs2e_blockset_refill(self.handle), self.ctx.handle)
 
     def set_graph(self, on=True):

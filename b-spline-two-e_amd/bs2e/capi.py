@@ -72,6 +72,9 @@ SIGNATURES = [
     ("bs2e_blockset_fill", i64, [vp, i64, P(dbl), P(dbl)]),
     ("bs2e_blockset_get", i64, [vp, i64, P(i64), P(i64), P(dbl), P(i64), P(i64), P(dbl)]),
     ("bs2e_blockset_get_rows", i64, [vp, i64, i64, i64, P(i64), P(i64), P(dbl), P(i64), P(i64), P(dbl)]),
+    ("bs2e_blockset_fill_rows", i64, [vp, i64, P(dbl), P(dbl), i64, i64]),
+    ("bs2e_blockset_row_counts", i64, [vp, i64, i64, P(i64), P(i64)]),
+    ("bs2e_blockset_reserve", i64, [vp, i64, i64]),
     ("bs2e_blockset_refill", i64, [vp]),
     ("bs2e_blockset_set_graph", i64, [vp, i64]),
     ("bs2e_blockset_replay", i64, [vp]),

@@ -149,8 +149,9 @@ struct BlockArgs {
     int64_t* cntS;
     int64_t* refH;               // per row: the reference's count_nnz predicate
     int64_t* refS;
-    const int64_t* ptrH;         // 0-based row starts (fill pass)
+    const int64_t* ptrH;         // 0-based row starts in the whole set's entries (fill pass)
     const int64_t* ptrS;
+    int64_t winH, winS;          // the set's first entry held by the outputs: row i starts at ptr[i] - win
     int64_t limH, limS;          // entries the output arrays hold (fill pass)
     int64_t* ixH;
     double* dH;                  // complex
@@ -211,11 +212,12 @@ hipError_t launch_radial_dipole(const Plan& p, const DevTables& d, double* B, do
 // Count / fill launches over the rows of n_blocks blocks (device table tab,
 // device row offsets row0[n_blocks + 1], n_rows = row0[n_blocks]).
 hipError_t launch_block_count(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, hipStream_t s);
-// The fill takes the rows in the order of rows[n_rows]: the n_sep staged
-// rows (|n1 - n2| >= k, on the matrix cores) first, then the others.
-hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, const int32_t* rows,
-                             int64_t n_sep, int64_t n_rows, int ks, int K, hipStream_t s, hipStream_t side,
-                             const hipEvent_t* ev);
+// The fill takes the n_sep staged rows (|n1 - n2| >= k, on the matrix cores)
+// listed in rows_sep, then the n_gen others listed in rows_gen (global row
+// numbers).
+hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, const int32_t* rows_sep,
+                             int64_t n_sep, const int32_t* rows_gen, int64_t n_gen, int ks, int K, hipStream_t s,
+                             hipStream_t side, const hipEvent_t* ev);
 // Exclusive prefix sums of the count pass's per-row counts (cnt: H counts then
 // S counts, n_rows each) into ptr (same layout), on the device; tmp holds
 // row_scan_tmp_elems(n_rows) int64.

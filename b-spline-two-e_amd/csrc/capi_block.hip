@@ -103,7 +103,10 @@ std::string bs2e::channel_tables(const int64_t* configs, int64_t n_config, int64
 // [row0[s], row0[s+1])); one count launch and one fill launch cover them
 // all, each wavefront locating its block in the descriptor table.  The fill
 // keeps the CSR arrays of every symmetry on the device, concatenated in
-// symmetry order; bs2e_blockset_get copies one symmetry's arrays out.
+// symmetry order; bs2e_blockset_get copies one symmetry's arrays out.  A
+// window fill (bs2e_blockset_fill_rows) holds only the entries of a range of
+// global rows, so that a set larger than the device's memory streams through
+// it window by window.
 struct bs2e_blockset {
     bs2e_rk* rk = nullptr;
     int64_t n_sym = 0, full = 1, max_l = 0, n_rows = 0, n_l = 0;   // n_l: one-electron matrices of the last fill
@@ -117,7 +120,12 @@ struct bs2e_blockset {
     std::vector<int64_t> cntH, cntS;               // per global row
     std::vector<int64_t> nnzH, nnzS, refH, refS;   // per symmetry
     std::vector<int64_t> offH, offS;               // first entry of each symmetry in the concatenated output
+    std::vector<int64_t> prefH, prefS;             // n_rows + 1: first entry of each global row in the whole set
     int64_t totH = 0, totS = 0;
+    // the rows the outputs hold: global rows [win_lo, win_lo + win_n), whose
+    // entries are [winH, winH + nwinH) and [winS, winS + nwinS) of the whole
+    // set; capH, capS: entries the output buffers have room for
+    int64_t win_lo = 0, win_n = 0, winH = 0, winS = 0, nwinH = 0, nwinS = 0, capH = 0, capS = 0;
     std::vector<BlockArgs> tab;                    // host copy of the descriptor table
     bool filled = false;
     // device
@@ -174,6 +182,7 @@ static void set_drop_outputs(bs2e_blockset* b)
     ctx_free(c, b->d_dH);
     ctx_free(c, b->d_ixS);
     ctx_free(c, b->d_dS);
+    b->capH = b->capS = 0;
     b->filled = false;
 }
 
@@ -198,6 +207,9 @@ static void set_free(bs2e_blockset* b)
     }
 }
 
+// The fill launches over the rows of the window: the staged rows and the
+// others of the window are each one contiguous part of the row list (both
+// parts ascend).
 static hipError_t set_fill_launch(bs2e_blockset* b, const BlockArgs* tab, hipStream_t st)
 {
     const Plan& p = b->rk->plan;
@@ -207,9 +219,26 @@ static hipError_t set_fill_launch(bs2e_blockset* b, const BlockArgs* tab, hipStr
             if (r != hipSuccess) return r;
         }
     hipStream_t side = b->rk->ctx->side ? b->rk->ctx->side : st;
-    return launch_block_fill(tab, b->d_row0, (int)b->n_sym, b->d_rowlist, b->n_sep, b->n_rows, (int)p.k_s, (int)p.K,
-                             st, side, b->ev);
+    const int32_t lo = (int32_t)b->win_lo, hi = (int32_t)(b->win_lo + b->win_n);
+    const auto l0 = b->rowlist.begin(), l1 = l0 + b->n_sep, l2 = b->rowlist.end();
+    const int64_t s0 = std::lower_bound(l0, l1, lo) - l0, s1 = std::lower_bound(l0, l1, hi) - l0;
+    const int64_t g0 = std::lower_bound(l1, l2, lo) - l0, g1 = std::lower_bound(l1, l2, hi) - l0;
+    return launch_block_fill(tab, b->d_row0, (int)b->n_sym, b->d_rowlist + s0, s1 - s0, b->d_rowlist + g0, g1 - g0,
+                             (int)p.k_s, (int)p.K, st, side, b->ev);
 }
+
+// The outputs hold global rows [lo, lo + n).
+static void set_window(bs2e_blockset* b, int64_t lo, int64_t n)
+{
+    b->win_lo = lo;
+    b->win_n = n;
+    b->winH = b->prefH[lo];
+    b->winS = b->prefS[lo];
+    b->nwinH = b->prefH[lo + n] - b->winH;
+    b->nwinS = b->prefS[lo + n] - b->winS;
+}
+
+static bool window_is_whole(const bs2e_blockset* b) { return b->win_lo == 0 && b->win_n == b->n_rows; }
 
 // descriptor of symmetry s for the count (fill = false) or fill pass
 static BlockArgs set_args(const bs2e_blockset* b, int64_t s, bool fill)
@@ -247,8 +276,10 @@ static BlockArgs set_args(const bs2e_blockset* b, int64_t s, bool fill)
         a.QtotT = a.PtotT + nf;
         a.ptrH = b->d_ptr + r0;
         a.ptrS = b->d_ptr + N + r0;
-        a.limH = b->totH;
-        a.limS = b->totS;
+        a.winH = b->winH;
+        a.winS = b->winS;
+        a.limH = b->nwinH;
+        a.limS = b->nwinS;
         a.ixH = b->d_ixH;
         a.dH = b->d_dH;
         a.ixS = b->d_ixS;
@@ -370,6 +401,8 @@ static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* 
     b->cntH.assign(cnt.begin(), cnt.begin() + N);
     b->cntS.assign(cnt.begin() + N, cnt.begin() + 2 * N);
     std::vector<int64_t> ptr((size_t)2 * N);
+    b->prefH.assign((size_t)N + 1, 0);
+    b->prefS.assign((size_t)N + 1, 0);
     b->nnzH.assign(b->n_sym, 0);
     b->nnzS.assign(b->n_sym, 0);
     b->refH.assign(b->n_sym, 0);
@@ -381,8 +414,8 @@ static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* 
         b->offH[s] = b->totH;
         b->offS[s] = b->totS;
         for (int64_t g = b->row0[s]; g < b->row0[s + 1]; ++g) {
-            ptr[g] = b->totH;
-            ptr[N + g] = b->totS;
+            ptr[g] = b->prefH[g] = b->totH;
+            ptr[N + g] = b->prefS[g] = b->totS;
             b->totH += b->cntH[g];
             b->totS += b->cntS[g];
             b->refH[s] += cnt[2 * N + g];
@@ -391,6 +424,8 @@ static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* 
         b->nnzH[s] = b->totH - b->offH[s];
         b->nnzS[s] = b->totS - b->offS[s];
     }
+    b->prefH[N] = b->totH;
+    b->prefS[N] = b->totS;
     HIP_TRY(ctx, hipMemcpyAsync(b->d_ptr, ptr.data(), sizeof(int64_t) * 2 * N, hipMemcpyHostToDevice, st));
     HIP_TRY(ctx, hipStreamSynchronize(st));
     return BS2E_OK;
@@ -423,6 +458,9 @@ int64_t bs2e::set_fill_alloc(bs2e_blockset* b, int64_t n_l, const char* who)
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_dH, 2 * (size_t)std::max<int64_t>(b->totH, 1)));
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_ixS, (size_t)std::max<int64_t>(b->totS, 1)));
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_dS, 2 * (size_t)std::max<int64_t>(b->totS, 1)));
+    b->capH = b->totH;
+    b->capS = b->totS;
+    set_window(b, 0, b->n_rows);
     return BS2E_OK;
 }
 
@@ -519,16 +557,19 @@ static int64_t set_get(const bs2e_blockset* b, int64_t s, int64_t* ip_H, int64_t
     if (!ip_H || !ix_H || !d_H || !ip_S || !ix_S || !d_S)
         return fail(ctx, BS2E_EINVAL, std::string(who) + ": NULL argument");
     if (!b->filled) return fail(ctx, BS2E_ESTATE, std::string(who) + ": the blocks have not been filled");
+    if (b->row0[s] < b->win_lo || b->row0[s + 1] > b->win_lo + b->win_n)
+        return fail(ctx, BS2E_ESTATE, std::string(who) + ": the filled row window does not hold the whole symmetry");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const int64_t nH = b->nnzH[s], nS = b->nnzS[s];
+    const int64_t aH = b->offH[s] - b->winH, aS = b->offS[s] - b->winS;   // within the outputs
     if (nH > 0) {
-        HIP_TRY(ctx, hipMemcpyAsync(ix_H, b->d_ixH + b->offH[s], sizeof(int64_t) * nH, hipMemcpyDeviceToHost, st));
-        HIP_TRY(ctx, hipMemcpyAsync(d_H, b->d_dH + 2 * b->offH[s], sizeof(double) * 2 * nH, hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipMemcpyAsync(ix_H, b->d_ixH + aH, sizeof(int64_t) * nH, hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipMemcpyAsync(d_H, b->d_dH + 2 * aH, sizeof(double) * 2 * nH, hipMemcpyDeviceToHost, st));
     }
     if (nS > 0) {
-        HIP_TRY(ctx, hipMemcpyAsync(ix_S, b->d_ixS + b->offS[s], sizeof(int64_t) * nS, hipMemcpyDeviceToHost, st));
-        HIP_TRY(ctx, hipMemcpyAsync(d_S, b->d_dS + 2 * b->offS[s], sizeof(double) * 2 * nS, hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipMemcpyAsync(ix_S, b->d_ixS + aS, sizeof(int64_t) * nS, hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipMemcpyAsync(d_S, b->d_dS + 2 * aS, sizeof(double) * 2 * nS, hipMemcpyDeviceToHost, st));
     }
     HIP_TRY(ctx, hipStreamSynchronize(st));
     ip_H[0] = 1;
@@ -715,12 +756,6 @@ int64_t bs2e_blockset_get_rows(const bs2e_blockset* b, int64_t sym, int64_t row_
     const int64_t s = sym - 1, r0 = b->row0[s], n = b->row0[s + 1] - r0;
     if (row_lo < 0 || n_rows < 0 || row_lo + n_rows > n)
         return fail(ctx, BS2E_ERANGE, "bs2e_blockset_get_rows: rows outside the symmetry's configurations");
-    if (!b->filled) return fail(ctx, BS2E_ESTATE, "bs2e_blockset_get_rows: the blocks have not been filled");
-    int64_t aH = b->offH[s], aS = b->offS[s];   // first entry of the window in the concatenated output
-    for (int64_t i = 0; i < row_lo; ++i) {
-        aH += b->cntH[r0 + i];
-        aS += b->cntS[r0 + i];
-    }
     ip_H[0] = 1;
     ip_S[0] = 1;
     for (int64_t i = 0; i < n_rows; ++i) {
@@ -729,6 +764,11 @@ int64_t bs2e_blockset_get_rows(const bs2e_blockset* b, int64_t sym, int64_t row_
     }
     if (!ix_H && !d_H && !ix_S && !d_S) return BS2E_OK;   // sizes only
     if (!ix_H || !d_H || !ix_S || !d_S) return fail(ctx, BS2E_EINVAL, "bs2e_blockset_get_rows: NULL argument");
+    if (!b->filled) return fail(ctx, BS2E_ESTATE, "bs2e_blockset_get_rows: the blocks have not been filled");
+    const int64_t g = r0 + row_lo;   // first global row asked for
+    if (g < b->win_lo || g + n_rows > b->win_lo + b->win_n)
+        return fail(ctx, BS2E_ESTATE, "bs2e_blockset_get_rows: rows outside the filled row window");
+    const int64_t aH = b->prefH[g] - b->winH, aS = b->prefS[g] - b->winS;   // first entry within the outputs
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const int64_t nH = ip_H[n_rows] - 1, nS = ip_S[n_rows] - 1;
@@ -742,6 +782,92 @@ int64_t bs2e_blockset_get_rows(const bs2e_blockset* b, int64_t sym, int64_t row_
     }
     HIP_TRY(ctx, hipStreamSynchronize(st));
     return BS2E_OK;
+}
+
+int64_t bs2e_blockset_row_counts(const bs2e_blockset* b, int64_t row_lo, int64_t n_rows, int64_t* cnt_H,
+                                 int64_t* cnt_S)
+{
+    if (!b || !cnt_H || !cnt_S) return fail(b ? b->rk->ctx : nullptr, BS2E_EINVAL, "bs2e_blockset_row_counts: NULL argument");
+    if (row_lo < 0 || n_rows < 0 || row_lo + n_rows > b->n_rows)
+        return fail(b->rk->ctx, BS2E_ERANGE, "bs2e_blockset_row_counts: rows outside the set");
+    std::copy(b->cntH.begin() + row_lo, b->cntH.begin() + row_lo + n_rows, cnt_H);
+    std::copy(b->cntS.begin() + row_lo, b->cntS.begin() + row_lo + n_rows, cnt_S);
+    return BS2E_OK;
+}
+
+// Output buffers with room for at least nH and nS entries: reallocated (their
+// contents dropped) only when smaller.
+static int64_t set_reserve(bs2e_blockset* b, int64_t nH, int64_t nS)
+{
+    bs2e_ctx* ctx = b->rk->ctx;
+    if (nH > b->capH || !b->d_ixH) {
+        set_drop_graph(b);
+        b->filled = false;
+        ctx_free(ctx, b->d_ixH);
+        ctx_free(ctx, b->d_dH);
+        b->capH = 0;
+        HIP_TRY(ctx, ctx_alloc(ctx, &b->d_ixH, (size_t)std::max<int64_t>(nH, 1)));
+        HIP_TRY(ctx, ctx_alloc(ctx, &b->d_dH, 2 * (size_t)std::max<int64_t>(nH, 1)));
+        b->capH = std::max<int64_t>(nH, 1);
+    }
+    if (nS > b->capS || !b->d_ixS) {
+        set_drop_graph(b);
+        b->filled = false;
+        ctx_free(ctx, b->d_ixS);
+        ctx_free(ctx, b->d_dS);
+        b->capS = 0;
+        HIP_TRY(ctx, ctx_alloc(ctx, &b->d_ixS, (size_t)std::max<int64_t>(nS, 1)));
+        HIP_TRY(ctx, ctx_alloc(ctx, &b->d_dS, 2 * (size_t)std::max<int64_t>(nS, 1)));
+        b->capS = std::max<int64_t>(nS, 1);
+    }
+    return BS2E_OK;
+}
+
+int64_t bs2e_blockset_reserve(bs2e_blockset* b, int64_t nnz_H, int64_t nnz_S)
+{
+    if (!b) return fail(nullptr, BS2E_EINVAL, "bs2e_blockset_reserve: NULL argument");
+    if (nnz_H < 0 || nnz_S < 0) return fail(b->rk->ctx, BS2E_EINVAL, "bs2e_blockset_reserve: negative size");
+    HIP_TRY(b->rk->ctx, hipSetDevice(b->rk->ctx->device));
+    return set_reserve(b, nnz_H, nnz_S);
+}
+
+int64_t bs2e_blockset_fill_rows(bs2e_blockset* b, int64_t max_l_1p, const double* Hl, const double* S, int64_t row_lo,
+                                int64_t n_rows)
+{
+    const char* who = "bs2e_blockset_fill_rows";
+    if (!b) return fail(nullptr, BS2E_EINVAL, std::string(who) + ": NULL argument");
+    bs2e_ctx* ctx = b->rk->ctx;
+    if (row_lo < 0 || n_rows < 1 || row_lo + n_rows > b->n_rows)
+        return fail(ctx, BS2E_ERANGE, std::string(who) + ": rows outside the set");
+    if ((Hl == nullptr) != (S == nullptr))
+        return fail(ctx, BS2E_EINVAL, std::string(who) + ": give both Hl and S, or neither");
+    if (!Hl && (!b->d_Hl || !b->d_S))
+        return fail(ctx, BS2E_ESTATE, std::string(who) + ": no one-electron matrices on the device yet");
+    if (Hl && max_l_1p < b->max_l)
+        return fail(ctx, BS2E_EINVAL, std::string(who) + ": one-electron matrices do not cover every l of the blocks");
+    const Plan& p = b->rk->plan;
+    const size_t nn = (size_t)p.n_b * p.n_b, nf = (size_t)p.n_pairs * p.K;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    set_drop_graph(b);   // its fill addresses the whole set's outputs
+    b->filled = false;
+    hipStream_t st = ctx->stream;
+    if (Hl) {
+        ctx_free(ctx, b->d_Hl);
+        ctx_free(ctx, b->d_S);
+        ctx_free(ctx, b->d_T);
+        b->n_l = max_l_1p + 1;
+        HIP_TRY(ctx, ctx_alloc(ctx, &b->d_Hl, 2 * nn * (size_t)b->n_l));
+        HIP_TRY(ctx, ctx_alloc(ctx, &b->d_S, 2 * nn));
+        HIP_TRY(ctx, hipMemcpyAsync(b->d_Hl, Hl, sizeof(double) * 2 * nn * b->n_l, hipMemcpyHostToDevice, st));
+        HIP_TRY(ctx, hipMemcpyAsync(b->d_S, S, sizeof(double) * 2 * nn, hipMemcpyHostToDevice, st));
+    }
+    if (!b->d_T) HIP_TRY(ctx, ctx_alloc(ctx, &b->d_T, 2 * nn * (size_t)(b->n_l + 1) + 2 * nf));
+    // the outputs grow to the window (kept when they already hold it)
+    const int64_t rc = set_reserve(b, b->prefH[row_lo + n_rows] - b->prefH[row_lo],
+                                   b->prefS[row_lo + n_rows] - b->prefS[row_lo]);
+    if (rc) return rc;
+    set_window(b, row_lo, n_rows);
+    return set_fill_enqueue(b, b->d_Hl, b->d_S, true, who);
 }
 
 int64_t bs2e_blockset_refill(bs2e_blockset* b)
@@ -762,6 +888,8 @@ int64_t bs2e_blockset_set_graph(bs2e_blockset* b, int64_t on)
     if (!on) return BS2E_OK;
     if (!b->filled || !b->d_Hl || !b->d_S)
         return fail(ctx, BS2E_ESTATE, "bs2e_blockset_set_graph: fill the set from host matrices first");
+    if (!window_is_whole(b))
+        return fail(ctx, BS2E_ESTATE, "bs2e_blockset_set_graph: the last fill was a row window; fill the whole set");
     if (rk->use_graph || rk->profiling || rk->from_primitives)
         return fail(ctx, BS2E_ESTATE,
                     "bs2e_blockset_set_graph: the tensor must build from its knots, without its own graph or profiling");

@@ -219,7 +219,7 @@ __device__ __forceinline__ void fill_row_general(const CBlockArgs& A, int i, int
     const int ks = A.ks, K = A.K;
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int64_t nH = 0, nS = 0;
-    const int64_t baseH = uniform64(A.ptrH[i]), baseS = uniform64(A.ptrS[i]);
+    const int64_t baseH = uniform64(A.ptrH[i]) - A.winH, baseS = uniform64(A.ptrS[i]) - A.winS;
     // |n_a - n_b| >= k: the direct pairs (n_a, n_c), (n_d, n_b) and the
     // exchange pairs (n_b, n_c), (n_d, n_a) have disjoint windows, on a side
     // fixed by the row: n_a < n_b puts every direct p left of its q
@@ -443,7 +443,7 @@ __device__ __forceinline__ void fill_row_staged(const CBlockArgs& A, int i, int 
     const int ex_base = a_low ? (merged ? d1lo - d2lo : lo_len) : 0;
     const bool both = (d1lo > d2lo ? d1lo : d2lo) <= (d1hi < d2hi ? d1hi : d2hi);   // D1 and D2 overlap
     const int shift = d2lo - d1lo;   // the third block reads run 1 at m + shift and run 2 at n - shift
-    const int64_t baseH = uniform64(A.ptrH[i]), baseS = uniform64(A.ptrS[i]);
+    const int64_t baseH = uniform64(A.ptrH[i]) - A.winH, baseS = uniform64(A.ptrS[i]) - A.winS;
     int64_t nH = 0, nS = 0;
     // Vector loads complete in order with the stores issued before them, so
     // a vector load after a channel's stores would wait for those: the
@@ -675,9 +675,10 @@ __device__ __forceinline__ void fill_row_staged(const CBlockArgs& A, int i, int 
 // matrix cores; otherwise they are walked.
 // The LDS (sep_wave_lds or fill_wave_lds words per wavefront) is dynamic; ks
 // and K are the blocks' common spline order and multipole count.  Entries go
-// to the row's CSR range from the count pass; stores beyond the set's
-// outputs (limH, limS entries) are dropped, so a count that disagreed with
-// the fill could not write outside them.
+// to the row's CSR range from the count pass, taken relative to the first
+// entry the outputs hold (winH, winS: the host launches only the rows of
+// that window); stores beyond the outputs (limH, limS entries) are dropped,
+// so a count that disagreed with the fill could not write past them.
 // (the matrix-core launch with one tile per side is held to 128 vector
 // registers, four wavefronts per SIMD, as many as its LDS allows)
 template <bool SEP, int NT>
@@ -949,12 +950,11 @@ hipError_t launch_row_scan(const int64_t* cnt, int64_t* ptr, int64_t n_rows, int
 // Measured on the c3 set (L = 0..4, l <= 4): 23.4 ms so, 27.9 ms with both
 // launches on one stream (23.5 ms there with every launch serialised by the
 // runtime, 24.7 ms under the kernel-trace profiler).
-hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, const int32_t* rows,
-                             int64_t n_sep, int64_t n_rows, int ks, int K, hipStream_t s, hipStream_t side,
-                             const hipEvent_t* ev)
+hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, const int32_t* rows_sep,
+                             int64_t n_sep, const int32_t* rows_gen, int64_t n_gen, int ks, int K, hipStream_t s,
+                             hipStream_t side, const hipEvent_t* ev)
 {
     const int nt = 2 * ks - 1 <= 16 ? 1 : 2;
-    const int64_t n_gen = n_rows - n_sep;
     hipError_t e = hipSuccess;
     if (n_sep > 0) {
         const unsigned grid = (unsigned)((n_sep * kWave + kCThreads - 1) / kCThreads);
@@ -963,11 +963,11 @@ hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_bl
         e = allow_lds(fn, lds);   // <= 4 x 39 KiB (k <= 16, K <= 64)
         if (e != hipSuccess) return e;
         if (nt == 1)
-            hipLaunchKernelGGL((kc_block<true, 1>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks, rows,
-                               n_sep, ks, K);
+            hipLaunchKernelGGL((kc_block<true, 1>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks,
+                               rows_sep, n_sep, ks, K);
         else
-            hipLaunchKernelGGL((kc_block<true, 2>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks, rows,
-                               n_sep, ks, K);
+            hipLaunchKernelGGL((kc_block<true, 2>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks,
+                               rows_sep, n_sep, ks, K);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -983,7 +983,7 @@ hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_bl
         e = allow_lds((const void*)kc_block<false, 1>, lds);   // <= 4 x 32 KiB (k <= 16, K <= 64)
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL((kc_block<false, 1>), dim3(grid), dim3(kCThreads), lds, side, tab, row0, n_blocks,
-                           rows + n_sep, n_gen, ks, K);
+                           rows_gen, n_gen, ks, K);
         e = hipGetLastError();
         if (e == hipSuccess && fork) {
             e = hipEventRecord(ev[1], side);

@@ -20,6 +20,7 @@ module bs2e_capi
     public :: bs2e_setup_radial_dip, bs2e_dip_block_create, bs2e_dip_block_sizes, bs2e_dip_block_fill
     public :: bs2e_dip_block_release, bs2e_setup_r_k_int
     public :: bs2e_blockset_create, bs2e_blockset_sizes, bs2e_blockset_fill, bs2e_blockset_get, bs2e_blockset_release
+    public :: bs2e_blockset_fill_rows, bs2e_blockset_row_counts, bs2e_blockset_get_rows, bs2e_blockset_reserve
 
     interface
         function bs2e_create(ctx, device) bind(C, name="bs2e_create") result(rc)
@@ -233,6 +234,41 @@ module bs2e_capi
             import :: c_ptr, c_int64_t
             type(c_ptr), value :: set
             integer(c_int64_t), value :: sym
+            type(c_ptr), value :: ipH, ixH, dH, ipS, ixS, dS
+            integer(c_int64_t) :: rc
+        end function
+
+        ! A window of global rows (0-based); Hl and S as c_loc of the arrays, or
+        ! c_null_ptr for both to reuse the matrices of the set's last fill.
+        function bs2e_blockset_fill_rows(set, max_l_1p, Hl, S, row_lo, n_rows) &
+                bind(C, name="bs2e_blockset_fill_rows") result(rc)
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: set, Hl, S
+            integer(c_int64_t), value :: max_l_1p, row_lo, n_rows
+            integer(c_int64_t) :: rc
+        end function
+
+        function bs2e_blockset_reserve(set, nnz_H, nnz_S) bind(C, name="bs2e_blockset_reserve") result(rc)
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: set
+            integer(c_int64_t), value :: nnz_H, nnz_S
+            integer(c_int64_t) :: rc
+        end function
+
+        function bs2e_blockset_row_counts(set, row_lo, n_rows, cnt_H, cnt_S) &
+                bind(C, name="bs2e_blockset_row_counts") result(rc)
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: set
+            integer(c_int64_t), value :: row_lo, n_rows
+            integer(c_int64_t), intent(out) :: cnt_H(*), cnt_S(*)
+            integer(c_int64_t) :: rc
+        end function
+
+        function bs2e_blockset_get_rows(set, sym, row_lo, n_rows, ipH, ixH, dH, ipS, ixS, dS) &
+                bind(C, name="bs2e_blockset_get_rows") result(rc)
+            import :: c_ptr, c_int64_t
+            type(c_ptr), value :: set
+            integer(c_int64_t), value :: sym, row_lo, n_rows
             type(c_ptr), value :: ipH, ixH, dH, ipS, ixS, dS
             integer(c_int64_t) :: rc
         end function

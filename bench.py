@@ -493,6 +493,60 @@ def blockset_benchmark(ctx, name="c3", max_l=4, max_L=4, refills=3):
     return out
 
 
+def blockset_stream_benchmark(ctx, name="c3", max_l=8, max_L=4, window_gib=64):
+    """The named basis's channel batch at its stated size: every symmetry
+    L = 0..max_L (natural parity) with l1, l2 <= max_l, all channels, counted
+    in one launch and filled window by window (bs2e_blockset_fill_rows) into
+    outputs of at most window_gib GiB, since the whole set exceeds the
+    device's memory; each window's fill is timed by the library's HIP events
+    (layouts and the two fill launches) and the window's entries stay on the
+    device until the next window overwrites them (a consumer would write them
+    out in between)."""
+    import bs2e
+    from bs2e.workloads import WORKLOADS
+    w = WORKLOADS[name]
+    t = bs2e.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+    spl = bs2e.b_spline(w.k, t)
+    rk = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx)
+    S = bs2e.setup_S(spl, w.k_GL, ctx=ctx)
+    Hl = bs2e.setup_H_one_particle(w.Z, (2, 0.8 * w.r_max, 0.0), range(max_l + 1), spl, w.k_GL, ctx=ctx)
+    terms = []
+    for L in range(max_L + 1):
+        term = bs2e.sym(l=L, pi=bool(L % 2))
+        term.configs = bs2e.count_configs(term, max_l, spl.n_b, w.k, spl.n_b, spl.n_b, max_l)
+        terms.append(term)
+    out = {"workload": f"{w.name} basis (order {w.k}, {spl.n_b} B-splines, k <= {w.max_k}), L = 0..{max_L}, "
+                       f"l1,l2 <= {max_l} (the stated channel batch), all channels, streamed in row windows of "
+                       f"<= {window_gib} GiB",
+           "rows": int(sum(t_.n_config for t_ in terms))}
+    try:
+        t0 = time.perf_counter()
+        bset = bs2e.BlockSet(rk, terms)
+        t_count = time.perf_counter() - t0
+        ms = []
+        t0 = time.perf_counter()
+        for _ in bset.stream(window_gib << 30, list(Hl), S):   # outputs reserved once for the largest window
+            ms.append(bset.kernel_ms()[1])
+        wall = time.perf_counter() - t0
+    except bs2e.Bs2eError as e:
+        rk.release()
+        out["error"] = str(e)
+        return out
+    nH, nS = int(bset.sizes[:, 0].sum()), int(bset.sizes[:, 1].sum())
+    count_ms = bset.kernel_ms()[0]
+    bset.release()
+    rk.release()
+    fill = float(sum(ms))
+    out.update({"channels_per_symmetry": [int(len({(int(a), int(b)) for a, b in t_.configs[:, 2:4]})) for t_ in terms],
+                "nnz_H": nH, "nnz_S": nS, "csr_GB": 24.0 * (nH + nS) / 1e9, "windows": len(ms),
+                "count_kernel_ms": count_ms, "fill_kernel_ms": fill, "window_fill_ms": ms,
+                "H_entries_per_s": nH / (fill * 1e-3), "fill_GBs": 24.0 * (nH + nS) / (fill * 1e-3) / 1e9,
+                "wall_ms_create": t_count * 1e3, "wall_ms_windows": wall * 1e3,
+                "wall_ms_windows_is": "host clock over the reservation of the outputs and every window's fill, "
+                                      "each waited for"})
+    return out
+
+
 def dipoles_benchmark(ctx, w, t, max_l=2, L1=0, gauge="l"):
     """Dipole block between symmetries (L1, even) and (L1+1, odd), q = 0
     (dipole.f90:8-47): radial dipole matrices, pattern count and fill on the
@@ -537,6 +591,8 @@ def main():
     ap.add_argument("--blocks", type=int, default=1, help="also time the L=0 symmetry block (0 skips it)")
     ap.add_argument("--blockset", type=int, default=1,
                     help="also time all symmetries L=0..4 (l <= 4) of the c3 basis in one launch (0 skips it)")
+    ap.add_argument("--blockset-stream", type=int, default=1,
+                    help="also stream the stated c3 channel batch (l <= 8, L=0..4) through row windows (0 skips it)")
     ap.add_argument("--dipoles", type=int, default=1, help="also time the L=0 -> L=1 dipole block (0 skips it)")
     ap.add_argument("--branch", type=int, default=1,
                     help="also time the c2 branch (tensor + L=0..2 blocks) eager against its captured graph (0 skips)")
@@ -690,6 +746,9 @@ def main():
     if args.blockset:
         log("c3 block set")
         out["blockset"] = blockset_benchmark(ctx)
+    if args.blockset_stream:
+        log("c3 stated channel batch in row windows")
+        out["blockset_stream"] = blockset_stream_benchmark(ctx)
     if args.serve_jobs > 0:
         log("serving stream")
         out["serving"] = serving_benchmark(args.serve_jobs)

@@ -267,15 +267,42 @@ int64_t bs2e_blockset_get(const bs2e_blockset *set, int64_t sym, int64_t *ip_H, 
 int64_t bs2e_blockset_get_rows(const bs2e_blockset *set, int64_t sym, int64_t row_lo, int64_t n_rows,
                                int64_t *ip_H, int64_t *ix_H, double *d_H, int64_t *ip_S, int64_t *ix_S,
                                double *d_S);
+/* hamiltonian.f90:106-283 construct_block_tensor over a window of rows:
+ * global rows row_lo .. row_lo+n_rows-1 (0-based over the set's rows, the
+ * symmetries laid end to end in bs2e_blockset_create's order) filled on the
+ * device into outputs that hold that window only (grown when a window needs
+ * more room, kept otherwise).  Hl and S as bs2e_blockset_fill, or both NULL
+ * to reuse the one-electron matrices of the set's last fill (kept on the
+ * device).  The window replaces the earlier fill: bs2e_blockset_get_rows
+ * then reads rows inside it, bs2e_blockset_get a symmetry it holds whole,
+ * bs2e_blockset_refill fills it again.  The entries are those of the whole
+ * set's fill, bit for bit.  (The reference holds each symmetry's whole block,
+ * main_basis_setup.f90:104-115; a window lets a set larger than the GPU's
+ * memory stream through it.) */
+int64_t bs2e_blockset_fill_rows(bs2e_blockset *set, int64_t max_l_1p, const double *Hl, const double *S,
+                                int64_t row_lo, int64_t n_rows);
+/* Output buffers of the set with room for at least nnz_H H and nnz_S S
+ * entries (reallocated, dropping what they held, only when smaller), so that
+ * a stream of windows allocates once, for its largest window: a later
+ * reallocation costs about as much as the driver's clearing of the freed
+ * memory.  (No reference counterpart.) */
+int64_t bs2e_blockset_reserve(bs2e_blockset *set, int64_t nnz_H, int64_t nnz_S);
+/* The count pass's entries per row for global rows row_lo ..
+ * row_lo+n_rows-1 (as bs2e_blockset_fill_rows numbers them): cnt_H[r],
+ * cnt_S[r], what a window of those rows holds.  (No reference counterpart.) */
+int64_t bs2e_blockset_row_counts(const bs2e_blockset *set, int64_t row_lo, int64_t n_rows, int64_t *cnt_H,
+                                 int64_t *cnt_S);
 /* The fill again, from the one-electron matrices of the last
- * bs2e_blockset_fill (kept on the device): the layouts and the fill launch,
- * waited for.  (No reference counterpart.) */
+ * bs2e_blockset_fill (kept on the device): the layouts and the fill launch
+ * over the rows of the last fill (all of them, or the last window), waited
+ * for.  (No reference counterpart.) */
 int64_t bs2e_blockset_refill(bs2e_blockset *set);
 /* on != 0: captures the whole branch of the set into one hipGraph: its
  * tensor's two-stream build, the count pass, the device prefix sums of the
  * row counts, the fill's layouts and the fill over every symmetry and channel
- * (one-electron matrices of the last bs2e_blockset_fill); 0 drops it.  A new
- * fill drops it too.  bs2e_blockset_replay then recomputes tensor and blocks
+ * (one-electron matrices of the last bs2e_blockset_fill, which must have
+ * filled the whole set, not a window: BS2E_ESTATE otherwise); 0 drops it.  A
+ * new fill or window drops it too.  bs2e_blockset_replay then recomputes tensor and blocks
  * with one launch on the context's stream (asynchronous:
  * bs2e_blockset_synchronize waits).  (No reference counterpart: the reference
  * builds each symmetry by its own call, main_basis_setup.f90:104-115.) */

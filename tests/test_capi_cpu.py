@@ -224,3 +224,24 @@ def test_read_namelist_formats(tmp_path):
         assert (v["k"], v["m"], v["z"]) == (8, 3, 2)
         assert v["h_max"] == 1.5 and v["r_max"] == 15.0 and v["cap_eta"] == 1e-3 + 0j
         assert v["gauge"] == "v" and v["z_pol"] is True and v["full"] is False and v["two_el"] is True
+
+
+def test_row_windows_plan():
+    """Windows cover every row once, in order, each within the byte budget
+    unless a single row alone exceeds it."""
+    from bs2e import row_windows
+    rng = np.random.default_rng(2)
+    cH = rng.integers(0, 5000, 10_000)
+    cS = rng.integers(0, 800, 10_000)
+    per = 24 * (cH + cS)
+    for budget in (1, 24 * 3000, 10 << 20, int(per.sum()), int(per.sum()) * 2):
+        wins = row_windows(cH, cS, budget)
+        lo = 0
+        for a, n in wins:
+            assert a == lo and n >= 1
+            assert n == 1 or per[a:a + n].sum() <= budget
+            if a + n < len(per):   # a window stops only where the next row would not fit
+                assert per[a:a + n + 1].sum() > budget
+            lo = a + n
+        assert lo == len(per)
+    assert row_windows([], [], 100) == []

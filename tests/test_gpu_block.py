@@ -452,6 +452,147 @@ def test_c3_blockset_sampled_rows(ctx, oracle):
     print(f"  {len(windows)} windows of {W} rows: H max err / scale {worst:.3e}", flush=True)
 
 
+def _global_window_rows(bset, lo, n):
+    """(H_sp, S_sp) pieces of global rows lo .. lo+n-1, one per symmetry the
+    window touches: [(symmetry, first row within it, H_sp, S_sp)]."""
+    r0 = bset.row0
+    out = []
+    for i in range(len(bset.terms)):
+        a, b = max(lo, int(r0[i])), min(lo + n, int(r0[i + 1]))
+        if a < b:
+            out.append((i, a - int(r0[i])) + tuple(bset.get_rows(i, a - int(r0[i]), b - a)))
+    return out
+
+
+@pytest.mark.parametrize("full", [True, False])
+def test_row_windows_equal_whole_fill(c1, full):
+    """Windows of global rows (one row, windows across symmetry boundaries,
+    the whole set, drawn sizes) filled one at a time into outputs that hold
+    only the window give the whole set's fill bit for bit; refill() fills the
+    window again; reading outside the window, or a symmetry it does not hold,
+    is refused, as is a graph of a window."""
+    w, t, n_b, rk, Rref, S, Hl = c1
+    terms = []
+    for L in range(3):
+        term = bs2e.sym(l=L, pi=bool(L % 2))
+        term.configs = bs2e.count_configs(term, 2, n_b, w.k, n_b, n_b, 2)
+        terms.append(term)
+    bset = bs2e.BlockSet(rk, terms, full)
+    N = int(bset.row0[-1])
+    with pytest.raises(bs2e.Bs2eError):   # no one-electron matrices on the device yet
+        bset.fill_rows(0, 10)
+    bset.fill(Hl, S)
+    whole = [bset.get(i) for i in range(len(terms))]
+    cH, cS = bset.row_counts()
+    assert int(cH.sum()) == int(bset.sizes[:, 0].sum()) and int(cS.sum()) == int(bset.sizes[:, 1].sum())
+    rng = np.random.default_rng(11)
+    r0 = bset.row0
+    wins = [(0, 1), (N - 1, 1), (int(r0[1]) - 5, 10), (int(r0[2]) - 100, 300), (0, N)]
+    wins += [(int(a), int(rng.integers(1, N - a + 1))) for a in rng.integers(0, N, 4)]
+    for k, (lo, n) in enumerate(wins):
+        print(f"window fill c1 full={full} rows {lo}..{lo + n - 1}", flush=True)
+        if k == 1:
+            bset.fill_rows(lo, n, Hl, S)   # with host matrices
+        else:
+            bset.fill_rows(lo, n)          # the last fill's device copy
+        for rep in range(2):
+            for i, a, H_sp, S_sp in _global_window_rows(bset, lo, n):
+                Hw, Sw = whole[i]
+                m = H_sp.index_ptr.size - 1
+                pa, pb = Hw.index_ptr[a] - 1, Hw.index_ptr[a + m] - 1
+                qa, qb = Sw.index_ptr[a] - 1, Sw.index_ptr[a + m] - 1
+                assert np.array_equal(H_sp.index_ptr - 1, Hw.index_ptr[a:a + m + 1] - 1 - pa)
+                assert np.array_equal(S_sp.index_ptr - 1, Sw.index_ptr[a:a + m + 1] - 1 - qa)
+                assert np.array_equal(H_sp.indices, Hw.indices[pa:pb])
+                assert np.array_equal(H_sp.data, Hw.data[pa:pb])
+                assert np.array_equal(S_sp.indices, Sw.indices[qa:qb])
+                assert np.array_equal(S_sp.data, Sw.data[qa:qb])
+            bset.refill()
+    lo, n = int(r0[1]) - 5, 10   # a window across the boundary of symmetries 0 and 1
+    bset.fill_rows(lo, n)
+    with pytest.raises(bs2e.Bs2eError):
+        bset.get_rows(0, 0, 3)              # rows outside the window
+    with pytest.raises(bs2e.Bs2eError):
+        bset.get(1)                         # a symmetry the window holds only in part
+    with pytest.raises(bs2e.Bs2eError):
+        bset.set_graph(True)                # graphs cover the whole set
+    with pytest.raises(bs2e.Bs2eError):
+        bset.fill_rows(N - 2, 3)            # rows outside the set
+    bset.fill(Hl, S)                        # the whole set again
+    for i in range(len(terms)):
+        H_sp, S_sp = bset.get(i)
+        assert np.array_equal(H_sp.data, whole[i][0].data) and np.array_equal(S_sp.indices, whole[i][1].indices)
+    bset.release()
+
+
+def test_c3_stated_size_streamed_in_windows(ctx, oracle):
+    """The c3 channel batch at its stated size: the c3 basis with lmax 8 and
+    every symmetry L = 0..4 (natural parity), all (l1, l2) channels: 4.47 M
+    configuration rows whose CSR blocks (about 5e11 bytes) exceed the device's
+    memory.  The set is counted in one launch and streamed through windows of
+    at most 24 GB of entries; from every window two drawn runs of 24 rows are
+    read back, and rows at the start, middle and end of every symmetry are
+    filled as windows of their own.  All of them against the oracle's rows of
+    the same blocks: patterns and S bit for bit, H within 1e-12 of the run's
+    largest entry; the windows' counts sum to the set's sizes."""
+    w = WORKLOADS["c3"]
+    t = bs2e.generate_grid(w.k, w.m, w.Z, w.h_max, w.r_max)
+    n_b = oracle.dims(w.k, t)[1]
+    max_l, W = 8, 24
+    rk = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx)
+    S = oracle.setup_S(w.k, t, w.k_GL)
+    Hl = [oracle.setup_H1p(w.k, t, w.k_GL, w.Z, l, cap_order=2, cap_r0=0.8 * w.r_max, cap_eta=0.0)
+          for l in range(max_l + 1)]
+    terms = []
+    for L in range(5):
+        term = bs2e.sym(l=L, pi=bool(L % 2))
+        term.configs = bs2e.count_configs(term, max_l, n_b, w.k, n_b, n_b, max_l)
+        terms.append(term)
+    bset = bs2e.BlockSet(rk, terms, True)
+    total = 24 * int(bset.sizes[:, :2].sum())
+    wins = bset.windows(24 << 30)
+    print(f"c3 stated size: rows {[t_.n_config for t_ in terms]}, {total / 1e9:.1f} GB of CSR, "
+          f"{len(wins)} windows", flush=True)
+    assert total > 288e9   # larger than the device
+    assert sum(n for _, n in wins) == int(bset.row0[-1])
+    rng = np.random.default_rng(5)
+    got = {}
+    r0 = bset.row0
+    for k, (lo, n) in enumerate(bset.stream(24 << 30, Hl, S)):
+        print(f"  window {k}: rows {lo}..{lo + n - 1}", flush=True)
+        assert (lo, n) == wins[k]
+        for g in rng.integers(lo, lo + n - W, 2):
+            i = int(np.searchsorted(r0, g, side="right") - 1)
+            a = int(g - r0[i])
+            if a + W <= terms[i].n_config:
+                got[(i, a)] = bset.get_rows(i, a, W)
+    for i, term in enumerate(terms):
+        n = term.n_config
+        for a in (0, n // 2 - W // 2, n - W):
+            bset.fill_rows(int(r0[i]) + a, W)
+            got[(i, a)] = bset.get_rows(i, a, W)
+    bset.release()
+    rk.release()
+    print(f"  {len(got)} runs of {W} rows read back; oracle rows", flush=True)
+    need = np.unique(np.concatenate([oracle.block_tensor_rows(w.k, n_b, terms[i].configs, a, a + W)
+                                     for i, a in got]))
+    npairs = oracle.n_pairs(w.k, n_b)
+    mask = np.zeros(npairs, bool)
+    mask[need] = True
+    Rr = oracle.rk_rows(w.k, n_b, w.max_k, oracle.slater_off_diag(w.k, t, w.k_GL, w.max_k),
+                        oracle.slater_diag(w.k, t, w.k_GL, w.max_k, pair_mask=mask), need)
+    rmap = np.full(npairs, -1, np.int64)
+    rmap[need] = np.arange(len(need))
+    worst = 0.0
+    for (i, a), (H_sp, S_sp) in got.items():
+        term = terms[i]
+        ref = oracle.construct_block_rows(w.k, n_b, term.l, w.max_k, True, term.configs, Rr, rmap, a, a + W, Hl, S)
+        compare(H_sp, S_sp, ref)
+        dH = ref["H"][2]
+        worst = max(worst, float(np.abs(H_sp.data - dH).max() / np.abs(dH).max()))
+    print(f"  H max err / scale {worst:.3e}", flush=True)
+
+
 def test_branch_graph_replays_match_eager(ctx, oracle):
     """The whole branch of a set (tensor build, count, device row offsets,
     layouts, fill over L = 0..2) captured as one hipGraph: every replay, with
