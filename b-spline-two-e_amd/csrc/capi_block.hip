@@ -153,7 +153,6 @@ struct bs2e_blockset {
     std::vector<int32_t> rowlist;
     int64_t n_sep = 0;
     int32_t* d_rowlist = nullptr;
-    hipEvent_t ev[2] = {nullptr, nullptr};   // the fill's fork to and join from the second stream
 };
 
 // A single symmetry's block is a set of one.
@@ -201,10 +200,6 @@ static void set_free(bs2e_blockset* b)
     ctx_free(c, b->d_tab);
     ctx_free(c, b->d_row0);
     ctx_free(c, b->d_rowlist);
-    for (hipEvent_t& e : b->ev) {
-        if (e) (void)hipEventDestroy(e);
-        e = nullptr;
-    }
 }
 
 // The fill launches over the rows of the window: the staged rows and the
@@ -213,18 +208,12 @@ static void set_free(bs2e_blockset* b)
 static hipError_t set_fill_launch(bs2e_blockset* b, const BlockArgs* tab, hipStream_t st)
 {
     const Plan& p = b->rk->plan;
-    for (hipEvent_t& e : b->ev)
-        if (!e) {
-            const hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableTiming);
-            if (r != hipSuccess) return r;
-        }
-    hipStream_t side = b->rk->ctx->side ? b->rk->ctx->side : st;
     const int32_t lo = (int32_t)b->win_lo, hi = (int32_t)(b->win_lo + b->win_n);
     const auto l0 = b->rowlist.begin(), l1 = l0 + b->n_sep, l2 = b->rowlist.end();
     const int64_t s0 = std::lower_bound(l0, l1, lo) - l0, s1 = std::lower_bound(l0, l1, hi) - l0;
     const int64_t g0 = std::lower_bound(l1, l2, lo) - l0, g1 = std::lower_bound(l1, l2, hi) - l0;
     return launch_block_fill(tab, b->d_row0, (int)b->n_sym, b->d_rowlist + s0, s1 - s0, b->d_rowlist + g0, g1 - g0,
-                             (int)p.k_s, (int)p.K, st, side, b->ev);
+                             (int)p.k_s, (int)p.K, st);
 }
 
 // The outputs hold global rows [lo, lo + n).

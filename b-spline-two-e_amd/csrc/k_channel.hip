@@ -667,12 +667,234 @@ __device__ __forceinline__ void fill_row_staged(const CBlockArgs& A, int i, int 
     }
 }
 
+// The fill of a row with |n_a - n_b| < k in tiles of cells.  The candidates
+// n_c (within k of n_a or n_b) form one range U = [u0, u0 + m) (the two
+// windows D2 = [n_a - k + 1, n_a + k - 1] and D1 = [n_b - k + 1, n_b + k - 1]
+// overlap), and so do the n_d of every entry, so the row's entries lie in the
+// grid U x U: cell (n_c, n_d) has the direct support when n_c is in D2 and
+// n_d in D1, the exchange support when n_c is in D1 and n_d in D2.  The grid
+// is taken 64 cells at a time, one per lane; each lane loads its cell's
+// direct and exchange multipole vectors once (the tensor's element where the
+// pairs' windows overlap, else the product of the window totals, which is the
+// tensor's element bit for bit) into registers and keeps them while the
+// column channels of a chunk are summed over, so that each tensor element is
+// read once per chunk of channels rather than once per channel, and the
+// loads of a tile are all in flight together.  Per channel, a candidate's H
+// entries are one run of n_d (the union of its direct and exchange windows
+// the channel's flags admit, clipped to the channel's n2 range and, for
+// upper triangles, to j >= i), its S entries another; a prefix sum over the
+// candidates, done for the chunk's channels before the tiles, gives each run
+// its CSR offset.  The sums are the walked fill's (each product and sum
+// rounded, multipoles ascending), so the entries are bit for bit the same.
+// KM >= K multipoles are held per term; m <= 64 (k <= 16).
+constexpr int kTChunk = 4;   // column channels per chunk
+__host__ __device__ constexpr int tiled_kp(int km) { return (km + 1) & ~1; }
+// LDS of one wavefront (8-byte words): per chunk channel the candidates'
+// runs [kTChunk][64] {H lo | H len << 16, H offset | S offset << 16,
+// S lo | S len << 16, column base} (16 bytes each), the coefficients
+// [kTChunk][2][KP] (direct, exchange), and the channel's first H and S
+// entries and flags [kTChunk][3]
+__host__ __device__ constexpr int tiled_wave_lds(int km)
+{
+    return kTChunk * 64 * 2 + kTChunk * 2 * tiled_kp(km) + kTChunk * 3;
+}
+
+template <int KM>
+__device__ __forceinline__ void fill_row_tiled(const CBlockArgs& A, int i, int n_a, int n_bb, int l_a, int l_b,
+                                               int ch_i, int lane, double* wl)
+{
+    typedef int int4v __attribute__((ext_vector_type(4)));
+    constexpr int KP = tiled_kp(KM);
+    const int ks = A.ks, K = A.K, N = A.n_b;
+    int4v* sinfo = (int4v*)wl;                                   // [kTChunk][64]
+    double* scoef = wl + kTChunk * 64 * 2;                       // [kTChunk][2][KP]
+    int64_t* schan = (int64_t*)(scoef + kTChunk * 2 * KP);       // [kTChunk][3]: H base, S base, flags
+    const int lo_ = (n_a < n_bb ? n_a : n_bb) - ks + 1, hi_ = (n_a > n_bb ? n_a : n_bb) + ks - 1;
+    const int u0 = lo_ < 1 ? 1 : lo_, u1 = hi_ > N ? N : hi_;
+    const int m = u1 - u0 + 1;
+    const int d1lo = n_bb - ks + 1 < 1 ? 1 : n_bb - ks + 1, d1hi = n_bb + ks - 1 > N ? N : n_bb + ks - 1;
+    const int d2lo = n_a - ks + 1 < 1 ? 1 : n_a - ks + 1, d2hi = n_a + ks - 1 > N ? N : n_a + ks - 1;
+    const int64_t baseH = uniform64(A.ptrH[i]) - A.winH, baseS = uniform64(A.ptrS[i]) - A.winS;
+    const unsigned np = (unsigned)A.n_pairs;   // K * n_pairs < 2^31 (checked on the host)
+    // this lane's candidate (the chunk passes) and its supports
+    const int my_c = u0 + lane;
+    const bool cand = lane < m;
+    const bool c_in2 = cand && my_c >= d2lo && my_c <= d2hi, c_in1 = cand && my_c >= d1lo && my_c <= d1hi;
+    const int4v* segc = (const int4v*)A.seg + (cand ? my_c : 0);   // BlockSeg {off, lo, hi, pad} of (channel, n_c)
+    const int n_cells = m * m;
+    int64_t nH = 0, nS = 0;
+    typedef const __attribute__((address_space(4))) BlockChan CChan;
+    typedef const __attribute__((address_space(4))) int32_t CInt;
+    for (int c0 = 0; c0 < A.n_chan; c0 += kTChunk) {
+        const int nc = A.n_chan - c0 < kTChunk ? A.n_chan - c0 : kTChunk;
+        // the chunk's runs and coefficients, every load before any LDS write
+        int4v qq[kTChunk];
+#pragma unroll
+        for (int c = 0; c < kTChunk; ++c)
+            qq[c] = cand && c < nc ? gload(segc + (size_t)(c0 + c) * (N + 1)) : int4v{0, 1, 0, 0};
+        double cv[kTChunk][2];
+        const double* cgc = A.coef + ((size_t)ch_i * A.n_chan + c0) * 2 * K;
+#pragma unroll
+        for (int c = 0; c < kTChunk; ++c)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int e = j * kWave + lane, t = e >= KP, kk = e - t * KP;
+                cv[c][j] = (c < nc && e < 2 * KP && kk < K) ? gload(cgc + (size_t)(2 * c + t) * K + kk) : 0.0;
+            }
+        bool any_r12 = false;
+#pragma unroll
+        for (int c = 0; c < kTChunk; ++c) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                if (j * kWave + lane < 2 * KP) scoef[c * 2 * KP + j * kWave + lane] = cv[c][j];
+            if (c >= nc) continue;
+            const int ch = c0 + c;
+            const CChan& cjr = ((CChan*)A.chan)[ch];
+            const int c_l1 = cjr.l1, c_l2 = cjr.l2, c_start = cjr.start;
+            const int fl = ((CInt*)A.flags)[ch_i * A.n_chan + ch];
+            const bool dir_any = fl & 1, ex_any = (fl >> 1) & 1;
+            const bool l_eq = (l_a == c_l1) && (l_b == c_l2);
+            const bool l_eq_ex = (l_a == c_l2) && (l_b == c_l1);
+            const bool hd = dir_any || l_eq || l_eq_ex, he = ex_any || l_eq || l_eq_ex;
+            any_r12 |= (fl & 3) != 0;
+            if (lane == 0) {   // the channel's first entries (this lane's candidate is not needed)
+                schan[3 * c] = baseH + nH;
+                schan[3 * c + 1] = baseS + nS;
+                schan[3 * c + 2] = fl | (l_eq ? 4 : 0) | (l_eq_ex ? 8 : 0) | ((((c_l1 + c_l2 + A.L) & 1) != 0) ? 16 : 0);
+            }
+            const int qoff = qq[c].x, qlo = qq[c].y, qhi = qq[c].z;
+            int hlo = 1, hlen = 0, slo = 1, slen = 0, colb = 0;
+            if (cand && (hd || he)) {
+                int clo = qlo;
+                if (!A.full) {   // column j = start + off + (n_d - lo2) >= i
+                    const int t = i - c_start - qoff + qlo;
+                    clo = t > clo ? t : clo;
+                }
+                const int dlo = d1lo > clo ? d1lo : clo, dhi = d1hi < qhi ? d1hi : qhi;
+                const int elo = d2lo > clo ? d2lo : clo, ehi = d2hi < qhi ? d2hi : qhi;
+                const bool dok = c_in2 && dhi >= dlo, eok = c_in1 && ehi >= elo;
+                int a0 = 1 << 20, a1 = -1, b0 = 1 << 20, b1 = -1;
+                if (dok && hd) a0 = dlo, a1 = dhi;
+                if (eok && he) a0 = elo < a0 ? elo : a0, a1 = ehi > a1 ? ehi : a1;
+                if (dok && l_eq) b0 = dlo, b1 = dhi;
+                if (eok && l_eq_ex) b0 = elo < b0 ? elo : b0, b1 = ehi > b1 ? ehi : b1;
+                if (a1 >= a0) hlo = a0, hlen = a1 - a0 + 1;
+                if (b1 >= b0) slo = b0, slen = b1 - b0 + 1;
+                colb = c_start + qoff - qlo;
+            }
+            // H and S counts in one prefix sum (at most 64 runs of at most
+            // 64 entries: below 2^16 each)
+            const int cHS = hlen | (slen << 16);
+            const int inc = wave_incl_sum(cHS, lane);
+            const int tot = __builtin_amdgcn_readlane(inc, kWave - 1);
+            const int ex = inc - cHS;
+            sinfo[c * kWave + lane] = int4v{hlo | (hlen << 16), ex, slo | (slen << 16), colb};
+            nH += tot & 0xffff;
+            nS += tot >> 16;
+        }
+        wave_lds_sync();   // the runs, coefficients and channel entries are read across lanes
+        for (int e0 = 0; e0 < n_cells; e0 += kWave) {
+            const int e = e0 + lane;
+            const int n_c = u0 + e / m, n_d = u0 + e % m;
+            const bool live = e < n_cells;
+            const bool sd = live && n_c >= d2lo && n_c <= d2hi && n_d >= d1lo && n_d <= d1hi;
+            const bool se = live && n_c >= d1lo && n_c <= d1hi && n_d >= d2lo && n_d <= d2hi;
+            // the cell's direct vector R^k[pair(n_a, n_c)][pair(n_d, n_b)] and
+            // exchange vector R^k[pair(n_b, n_c)][pair(n_d, n_a)]
+            double rd[KM], re[KM];
+            // v[k] = R^k[pair(a1, n_c)][pair(n_d, a2)], k < K (zero beyond)
+            auto load_vec = [&](double(&v)[KM], bool on, int a1, int a2) {
+#pragma unroll
+                for (int k = 0; k < KM; ++k) v[k] = 0.0;
+                if (!on) return;
+                const int p = pair_of(A.c_first, ks, a1, n_c), q = pair_of(A.c_first, ks, n_d, a2);
+                const bool left = window_hi(a1, n_c, A.n_iv) < window_lo(n_d, a2, ks);
+                const bool right = window_lo(a1, n_c, ks) > window_hi(n_d, a2, A.n_iv);
+                if (!left && !right) {
+                    const double* r = A.R + ((size_t)p * A.n_pairs + q) * K;
+#pragma unroll
+                    for (int k = 0; k < KM; ++k)
+                        if (k < K) v[k] = gload(r + k);
+                } else {
+                    const double* x = (left ? A.PtotT : A.QtotT) + p;
+                    const double* y = (left ? A.QtotT : A.PtotT) + q;
+#pragma unroll
+                    for (int k = 0; k < KM; ++k)
+                        if (k < K) v[k] = gload(x + (unsigned)k * np) * gload(y + (unsigned)k * np);
+                }
+            };
+            load_vec(rd, any_r12 && sd, n_a, n_bb);
+            load_vec(re, any_r12 && se, n_bb, n_a);
+            // the chunk's two-electron sums first (the vectors are dead after
+            // them), then the one-electron parts and the stores
+            double res[kTChunk];
+#pragma unroll
+            for (int c = 0; c < kTChunk; ++c) {
+                res[c] = 0.0;
+                if (c >= nc) continue;
+                const int fl = (int)schan[3 * c + 2];
+                const bool r12 = ((fl & 1) && sd) || (((fl >> 1) & 1) && se);
+                if (!r12) continue;
+                const double* cd = scoef + c * 2 * KP;
+                const double* ce = cd + KP;
+                double r = 0.0;
+                if (sd) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int k = 0; k < KM; ++k)
+                        if (k < K) t = t + rd[k] * cd[k];
+                    r = r + t;
+                }
+                if (se) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int k = 0; k < KM; ++k)
+                        if (k < K) t = t + re[k] * ce[k];
+                    r = r + t;
+                }
+                res[c] = r;
+            }
+#pragma unroll
+            for (int c = 0; c < kTChunk; ++c) {
+                if (c >= nc) break;
+                const int4v info = sinfo[c * kWave + (live ? n_c - u0 : 0)];
+                const int h_lo = info.x & 0xffff, h_len = info.x >> 16;
+                const int s_lo = info.z & 0xffff, s_len = info.z >> 16;
+                const int oh = n_d - h_lo, os = n_d - s_lo;
+                if (!live || oh < 0 || oh >= h_len) continue;   // no H entry here (and then no S entry)
+                const int fl = (int)schan[3 * c + 2];
+                // the sum is the entry's real part where r12 holds, zero otherwise
+                Cplx h{res[c], 0.0};
+                const int64_t col = (int64_t)info.w + n_d + 1;
+                if (os >= 0 && os < s_len) {
+                    Cplx h1, s1;
+                    one_electron(A, n_a, n_bb, n_c, n_d, l_a, l_b, (fl >> 2) & 1, (fl >> 3) & 1, (fl >> 4) & 1, h1,
+                                 s1);
+                    h = cadd(h, h1);
+                    const int64_t ps = schan[3 * c + 1] + (info.y >> 16) + os;
+                    if (ps < A.limS) {
+                        gstore(A.ixS + ps, col);
+                        gstore((dbl2*)(A.dS + 2 * ps), dbl2{s1.re, s1.im});
+                    }
+                }
+                const int64_t ph = schan[3 * c] + (info.y & 0xffff) + oh;
+                if (ph < A.limH) {
+                    gstore(A.ixH + ph, col);
+                    gstore((dbl2*)(A.dH + 2 * ph), dbl2{h.re, h.im});
+                }
+            }
+        }
+        wave_lds_sync();   // the next chunk overwrites the runs and coefficients
+    }
+}
+
 // One wavefront per configuration row of any of n_blocks symmetry blocks:
 // wavefront w takes global row rows[w] (w < n), which belongs to the last
 // block b with row0[b] <= row and is row row - row0[b] of it; each block's
 // tables, inputs and outputs are in tab[b].  SEP: the rows are staged ones
 // (|n_a - n_b| >= k; NT tiles per window side, 2k - 1 <= 16 NT), taken on the
-// matrix cores; otherwise they are walked.
+// matrix cores; otherwise NT > 1: tiles of cells holding NT >= K multipoles
+// (3k - 2 <= 64), NT = 1: the channel walk.
 // The LDS (sep_wave_lds or fill_wave_lds words per wavefront) is dynamic; ks
 // and K are the blocks' common spline order and multipole count.  Entries go
 // to the row's CSR range from the count pass, taken relative to the first
@@ -682,7 +904,7 @@ __device__ __forceinline__ void fill_row_staged(const CBlockArgs& A, int i, int 
 // (the matrix-core launch with one tile per side is held to 128 vector
 // registers, four wavefronts per SIMD, as many as its LDS allows)
 template <bool SEP, int NT>
-__global__ __launch_bounds__(kCThreads, SEP && NT == 1 ? 4 : 1) void kc_block(const BlockArgs* __restrict__ tab,
+__global__ __launch_bounds__(kCThreads, (SEP && NT == 1) || (!SEP && NT > 1) ? 4 : 1) void kc_block(const BlockArgs* __restrict__ tab,
                                                                              const int64_t* __restrict__ row0,
                                                                              int n_blocks,
                                                                              const int32_t* __restrict__ rows,
@@ -711,8 +933,10 @@ __global__ __launch_bounds__(kCThreads, SEP && NT == 1 ? 4 : 1) void kc_block(co
     const int n_a = __builtin_amdgcn_readfirstlane((int)ci[0]), n_bb = __builtin_amdgcn_readfirstlane((int)ci[1]);
     const int l_a = __builtin_amdgcn_readfirstlane((int)ci[2]), l_b = __builtin_amdgcn_readfirstlane((int)ci[3]);
     const int ch_i = __builtin_amdgcn_readfirstlane(A.cfg_chan[i]);
-    if (SEP)
+    if constexpr (SEP)
         fill_row_staged<NT>(A, i, n_a, n_bb, l_a, l_b, ch_i, lane, lds_fill + (size_t)wave * sep_wave_lds(lds_K, NT));
+    else if constexpr (NT > 1)
+        fill_row_tiled<NT>(A, i, n_a, n_bb, l_a, l_b, ch_i, lane, lds_fill + (size_t)wave * tiled_wave_lds(NT));
     else
         fill_row_general(A, i, n_a, n_bb, l_a, l_b, ch_i, lane, lds_fill + (size_t)wave * fill_wave_lds(lds_ks, lds_K),
                          lds_ks, lds_K);
@@ -944,18 +1168,38 @@ hipError_t launch_row_scan(const int64_t* cnt, int64_t* ptr, int64_t n_rows, int
     return hipGetLastError();
 }
 
-// The walked rows run on `side` once the matrix-core launch has finished
-// (ev[0] recorded behind it on s and waited for on side; ev[1] recorded
-// behind the walked launch on side and waited for on s); side may be s.
-// Measured on the c3 set (L = 0..4, l <= 4): 23.4 ms so, 27.9 ms with both
-// launches on one stream (23.5 ms there with every launch serialised by the
-// runtime, 24.7 ms under the kernel-trace profiler).
+// Both launches on s, the walked rows first.  (Measured on the c3 set: the
+// same fill time as with the walked rows on a second stream beside the
+// matrix-core launch, or after it; the two kernels share the memory system.)
 hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, const int32_t* rows_sep,
-                             int64_t n_sep, const int32_t* rows_gen, int64_t n_gen, int ks, int K, hipStream_t s,
-                             hipStream_t side, const hipEvent_t* ev)
+                             int64_t n_sep, const int32_t* rows_gen, int64_t n_gen, int ks, int K, hipStream_t s)
 {
     const int nt = 2 * ks - 1 <= 16 ? 1 : 2;
     hipError_t e = hipSuccess;
+    if (n_gen > 0) {
+        const unsigned grid = (unsigned)((n_gen * kWave + kCThreads - 1) / kCThreads);
+        // tiles of cells where the multipoles fit the registers, else the walk
+        // (KM = 18 holds 111 vector registers, four wavefronts per SIMD; more
+        // multipoles would spill)
+        const int km = 3 * ks - 2 > kWave ? 0 : K <= 12 ? 12 : K <= 18 ? 18 : 0;
+        const void* fn = km == 12 ? (const void*)kc_block<false, 12>
+                         : km == 18 ? (const void*)kc_block<false, 18> : (const void*)kc_block<false, 1>;
+        const size_t lds = sizeof(double) * (size_t)(kCThreads / kWave) *
+                           (size_t)(km ? tiled_wave_lds(km) : fill_wave_lds(ks, K));
+        e = allow_lds(fn, lds);   // <= 4 x 32 KiB (k <= 16, K <= 64)
+        if (e != hipSuccess) return e;
+        if (km == 12)
+            hipLaunchKernelGGL((kc_block<false, 12>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks,
+                               rows_gen, n_gen, ks, K);
+        else if (km == 18)
+            hipLaunchKernelGGL((kc_block<false, 18>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks,
+                               rows_gen, n_gen, ks, K);
+        else
+            hipLaunchKernelGGL((kc_block<false, 1>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks,
+                               rows_gen, n_gen, ks, K);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     if (n_sep > 0) {
         const unsigned grid = (unsigned)((n_sep * kWave + kCThreads - 1) / kCThreads);
         const void* fn = nt == 1 ? (const void*)kc_block<true, 1> : (const void*)kc_block<true, 2>;
@@ -969,26 +1213,6 @@ hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_bl
             hipLaunchKernelGGL((kc_block<true, 2>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks,
                                rows_sep, n_sep, ks, K);
         e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    if (n_gen > 0) {
-        const bool fork = side != s;
-        if (fork) {
-            e = hipEventRecord(ev[0], s);
-            if (e == hipSuccess) e = hipStreamWaitEvent(side, ev[0], 0);
-            if (e != hipSuccess) return e;
-        }
-        const unsigned grid = (unsigned)((n_gen * kWave + kCThreads - 1) / kCThreads);
-        const size_t lds = sizeof(double) * (size_t)(kCThreads / kWave) * (size_t)fill_wave_lds(ks, K);
-        e = allow_lds((const void*)kc_block<false, 1>, lds);   // <= 4 x 32 KiB (k <= 16, K <= 64)
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((kc_block<false, 1>), dim3(grid), dim3(kCThreads), lds, side, tab, row0, n_blocks,
-                           rows_gen, n_gen, ks, K);
-        e = hipGetLastError();
-        if (e == hipSuccess && fork) {
-            e = hipEventRecord(ev[1], side);
-            if (e == hipSuccess) e = hipStreamWaitEvent(s, ev[1], 0);
-        }
     }
     return e;
 }
