@@ -131,6 +131,17 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t n)
     const int64_t per = n / kXcds, extra = n % kXcds;
     return x * per + (x < extra ? x : extra) + j;
 }
+// The same in chunks: the j-th workgroup an XCD receives takes logical block
+// (j / C) 8C + x C + j % C, so the XCDs work through adjacent bands of C
+// logical blocks each (a window of 8C blocks moving through the grid); the
+// last, partial window is split as xcd_block splits a grid.  A bijection of
+// [0, n).
+__device__ __forceinline__ int64_t xcd_chunk_block(int64_t b, int64_t n, int64_t C)
+{
+    const int64_t R = n / (kXcds * C), x = b % kXcds, j = b / kXcds;
+    if (j < R * C) return (j / C) * kXcds * C + x * C + (j % C);
+    return R * kXcds * C + xcd_block(b - R * kXcds * C, n - R * kXcds * C);
+}
 
 struct BlockArgs {
     int n_config, n_chan, full, L, ks, n_b, n_pairs, K, n_iv;
