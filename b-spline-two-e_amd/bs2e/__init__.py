@@ -46,6 +46,13 @@ class Context:
         self.handle = h
         self.device = device
 
+    def store_ceiling(self, dev_ptr: int, n_bytes: int, reps: int = 5) -> float:
+        """GB/s of the library's plain store kernel over n_bytes of device memory at dev_ptr (bs2e_store_ceiling)."""
+        out = C.c_double()
+        check(capi.lib().bs2e_store_ceiling(self.handle, C.c_void_p(dev_ptr), int(n_bytes), int(reps), C.byref(out)),
+              self.handle)
+        return out.value
+
     def set_stream(self, stream_ptr: int | None):
         check(capi.lib().bs2e_set_stream(self.handle, C.c_void_p(stream_ptr) if stream_ptr else None), self.handle)
 
@@ -255,6 +262,10 @@ class RkTensor:
         out = np.zeros(2, np.int64)
         check(capi.lib().bs2e_rk_tuning(self.handle, ptr(out, C.c_int64), 2), self.ctx.handle)
         return int(out[0]), int(out[1])
+
+    def set_tuning(self, shape, kc):
+        """Pins the rank-one fill shape and the diagonal-cell multipole chunk (bs2e_rk_set_tuning)."""
+        check(capi.lib().bs2e_rk_set_tuning(self.handle, int(shape), int(kc)), self.ctx.handle)
 
     def set_graph(self, on=True):
         """Capture the build into a hipGraph at the next compute and replay it afterwards."""

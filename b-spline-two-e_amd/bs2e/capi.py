@@ -66,6 +66,8 @@ SIGNATURES = [
     ("bs2e_setup_H_one_particle", i64, [vp, i64, i64, P(dbl), i64, i64, i64, i64, i64, dbl, dbl, dbl, P(dbl)]),
     ("bs2e_rk_autotune", i64, [vp, i64]),
     ("bs2e_rk_tuning", i64, [vp, P(i64), i64]),
+    ("bs2e_rk_set_tuning", i64, [vp, i64, i64]),
+    ("bs2e_store_ceiling", i64, [vp, vp, i64, i64, P(dbl)]),
     ("bs2e_rk_build_static", i64, [i64, i64, P(dbl), i64, i64, i64, vp, i64, vp]),
     ("bs2e_blockset_create", i64, [vp, i64, P(i64), P(i64), P(i64), i64, P(vp)]),
     ("bs2e_blockset_sizes", i64, [vp, P(i64), i64]),

@@ -258,6 +258,9 @@ size_t mixed_lds_bytes(int64_t ks, int64_t K);
 hipError_t launch_pair_windows(const Plan& p, const DevTables& d, hipStream_t s);            // Pw/Qw + sums
 hipError_t launch_tensor_far(const Plan& p, const DevTables& d, double* R, hipStream_t s);   // rank-one part
 int far_variant_count();
+// Zeros over n16 16-byte elements at dst, one store per thread (the store
+// ceiling the rank-one fill is measured against).
+hipError_t launch_store_fill(void* dst, int64_t n16, bool nt, hipStream_t s);
 bool far_variant_valid(const Plan& p, int v);      // 16-byte variants need rows of even length
 int far_variant_default(const Plan& p);
 hipError_t launch_tensor_near(const Plan& p, const DevTables& d, double* R, hipStream_t s);  // overlapping pairs

@@ -776,6 +776,52 @@ int64_t bs2e_rk_tuning(const bs2e_rk* rk, int64_t* out, int64_t n)
     return BS2E_OK;
 }
 
+int64_t bs2e_store_ceiling(bs2e_ctx* ctx, void* dst, int64_t n_bytes, int64_t reps, double* gbs)
+{
+    if (!ctx || !dst || !gbs) return fail(ctx, BS2E_EINVAL, "bs2e_store_ceiling: NULL argument");
+    if (n_bytes < 16 || n_bytes % 16 != 0 || (uintptr_t)dst % 16 != 0 || n_bytes / 16 >= ((int64_t)1 << 38))
+        return fail(ctx, BS2E_EINVAL, "bs2e_store_ceiling: need a 16-byte aligned buffer of a positive multiple of 16 bytes");
+    if (reps < 1 || reps > 1000) return fail(ctx, BS2E_EINVAL, "bs2e_store_ceiling: need 1 <= reps <= 1000");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(ctx, hipEventCreate(&a));
+    HIP_TRY(ctx, hipEventCreate(&b));
+    double best = 0.0;
+    hipError_t e = hipSuccess;
+    for (int nt = 0; nt < 2 && e == hipSuccess; ++nt) {
+        e = launch_store_fill(dst, n_bytes / 16, nt != 0, st);   // warm-up
+        if (e == hipSuccess) e = hipEventRecord(a, st);
+        for (int64_t r = 0; r < reps && e == hipSuccess; ++r) e = launch_store_fill(dst, n_bytes / 16, nt != 0, st);
+        if (e == hipSuccess) e = hipEventRecord(b, st);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+        if (e == hipSuccess && ms > 0.f) best = std::max(best, (double)n_bytes * reps / (ms * 1e-3) / 1e9);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    HIP_TRY(ctx, e);
+    *gbs = best;
+    return BS2E_OK;
+}
+
+int64_t bs2e_rk_set_tuning(bs2e_rk* rk, int64_t shape, int64_t kc)
+{
+    if (!rk) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_set_tuning: rk is NULL");
+    bs2e_ctx* ctx = rk->ctx;
+    if (rk->use_graph) return fail(ctx, BS2E_ESTATE, "bs2e_rk_set_tuning: switch the graph off first");
+    Plan& p = rk->plan;
+    if (shape < 0 || shape >= far_variant_count() || !far_variant_valid(p, (int)shape))
+        return fail(ctx, BS2E_EINVAL, "bs2e_rk_set_tuning: no such fill shape for this basis");
+    const int kmax = inner_kc(p.k_s, p.k_gl, p.K);
+    if (kc < 1 || kc > std::max(kmax, 1))
+        return fail(ctx, BS2E_EINVAL, "bs2e_rk_set_tuning: multipole chunk outside 1..the largest that fits");
+    p.far_variant = (int)shape;
+    p.inner_kc = (int)kc;
+    return BS2E_OK;
+}
+
 int64_t bs2e_rk_set_graph(bs2e_rk* rk, int64_t on)
 {
     if (!rk) return fail(nullptr, BS2E_EINVAL, "bs2e_rk_set_graph: rk is NULL");

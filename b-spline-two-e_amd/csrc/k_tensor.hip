@@ -141,7 +141,8 @@ __global__ __launch_bounds__(kThreads) void k3_near(int n_pairs, int n_b, int K,
 typedef double double2_t __attribute__((ext_vector_type(2)));
 
 template <int V, int ROWS, int ITERS>
-__global__ __launch_bounds__(kThreads) void k3_far(int n_pairs, int K, int far_cols, const PairInfo* __restrict__ pairs,
+__global__ __launch_bounds__(kThreads) void k3_far(int n_pairs, int K, int far_cols, int x_cols,
+                                                   const PairInfo* __restrict__ pairs,
                                                    const double* __restrict__ Ptot,
                                                    const double* __restrict__ Qtot, double* __restrict__ R)
 {
@@ -150,7 +151,16 @@ __global__ __launch_bounds__(kThreads) void k3_far(int n_pairs, int K, int far_c
     __shared__ int sLo[ROWS], sHi[ROWS];
     const int64_t rowlen = (int64_t)n_pairs * K;
     const int tid = threadIdx.x;
-    const int ry = (int)blockIdx.x / far_cols, cx = (int)blockIdx.x - ry * far_cols;
+    // XCD x owns the x-th eighth of the column chunks (x_cols of them) and
+    // walks it row group by row group: its L2 keeps that slice of the column
+    // factors while the stores stream past, and the XCDs' stores in flight
+    // spread over eight regions of a row.  Measured in one process against
+    // the row-major order and against chunks x, x + 8, ... per XCD: c5p
+    // 3.45-3.48 ms per build against 3.71-3.76 (k3_far<2,2,2>), c2, c3, c4
+    // within 2 %.
+    const int x = (int)(blockIdx.x % kXcds), j = (int)(blockIdx.x / kXcds);
+    const int ry = j / x_cols, cx = x * x_cols + (j - ry * x_cols);
+    if (cx >= far_cols) return;   // (the last XCD's slice can be short; uniform over the workgroup)
     const int p0 = ry * ROWS;
     const int nrows = (n_pairs - p0) < ROWS ? (n_pairs - p0) : ROWS;
     for (int e = tid; e < nrows * K; e += kThreads) {
@@ -201,6 +211,29 @@ __global__ __launch_bounds__(kThreads) void k3_far(int n_pairs, int K, int far_c
             }
         }
     }
+}
+
+// The store ceiling the rank-one fill is measured against: zeros over n16
+// 16-byte elements, one store per thread (NT: with the non-temporal hint),
+// the write pattern that reached the highest rate among those timed (more
+// stores per thread, grid-stride loops and 64 contiguous bytes per thread
+// were slower).
+template <bool NT>
+__global__ __launch_bounds__(kThreads) void k_store_fill(double2_t* __restrict__ dst, int64_t n16)
+{
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n16) return;
+    const double2_t z = {0.0, 0.0};
+    if (NT) __builtin_nontemporal_store(z, dst + i);
+    else dst[i] = z;
+}
+
+hipError_t launch_store_fill(void* dst, int64_t n16, bool nt, hipStream_t s)
+{
+    const unsigned grid = (unsigned)((n16 + kThreads - 1) / kThreads);
+    if (nt) hipLaunchKernelGGL((k_store_fill<true>), dim3(grid), dim3(kThreads), 0, s, (double2_t*)dst, n16);
+    else hipLaunchKernelGGL((k_store_fill<false>), dim3(grid), dim3(kThreads), 0, s, (double2_t*)dst, n16);
+    return hipGetLastError();
 }
 
 // Dsym[v][u1][u2][k] = D[v][u1][u2][k] + D[v][u2][u1][k], in place (one
@@ -285,8 +318,9 @@ static void launch_far_shape(const Plan& p, const DevTables& d, double* R, hipSt
     const int64_t per = (int64_t)V * kThreads * ITERS;
     const int far_cols = (int)((rowlen + per - 1) / per);
     const int far_rows = (int)((p.n_pairs + ROWS - 1) / ROWS);
-    hipLaunchKernelGGL((k3_far<V, ROWS, ITERS>), dim3((unsigned)(far_cols * far_rows)), dim3(kThreads), 0, s,
-                       (int)p.n_pairs, (int)p.K, far_cols, d.pairs, d.Ptot, d.Qtot, R);
+    const int x_cols = (far_cols + kXcds - 1) / kXcds;   // column chunks per XCD
+    hipLaunchKernelGGL((k3_far<V, ROWS, ITERS>), dim3((unsigned)(kXcds * x_cols * far_rows)), dim3(kThreads), 0, s,
+                       (int)p.n_pairs, (int)p.K, far_cols, x_cols, d.pairs, d.Ptot, d.Qtot, R);
 }
 
 hipError_t launch_tensor_far(const Plan& p, const DevTables& d, double* R, hipStream_t s)

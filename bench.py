@@ -41,11 +41,11 @@ HBM_PEAK_GBS = 8000.0
 # each scaled by what the counter reads for a device copy of known size
 # (profiles/rk_build_c2.md, profiles/rk_build_c5p.md), per (workload, fill
 # shape); absent for pairs not measured that way.
-FAR_TRAFFIC_BYTES = {("c2", 0): 739.35e6, ("c2", 1): 739.59e6, ("c2", 2): 739.99e6, ("c2", 3): 739.74e6,
-                     ("c2", 4): 739.52e6, ("c2", 5): 739.51e6, ("c2", 6): 739.05e6,
-                     ("c5p", 0): 35204.48e6, ("c5p", 1): 20266.54e6, ("c5p", 2): 20273.27e6,
-                     ("c5p", 3): 20267.69e6, ("c5p", 4): 20262.93e6, ("c5p", 5): 20269.25e6,
-                     ("c5p", 6): 20260.59e6}
+FAR_TRAFFIC_BYTES = {("c2", 0): 739.26e6, ("c2", 1): 739.82e6, ("c2", 2): 740.34e6, ("c2", 3): 739.93e6,
+                     ("c2", 4): 739.86e6, ("c2", 5): 739.71e6, ("c2", 6): 739.04e6,
+                     ("c5p", 0): 20260.12e6, ("c5p", 1): 20263.29e6, ("c5p", 2): 20273.51e6,
+                     ("c5p", 3): 20267.81e6, ("c5p", 4): 20262.35e6, ("c5p", 5): 20271.02e6,
+                     ("c5p", 6): 20259.39e6}
 
 
 def log(msg):
@@ -263,11 +263,16 @@ def measure_build(rk, stream, steps, warmup, latency_samples):
             "stages": st}
 
 
-def store_ceiling(stream, n_elems):
-    """GB/s of a store-only fill of n_elems doubles on the build's stream."""
+def store_ceiling(ctx, stream, n_elems):
+    """(library, torch) GB/s of a store-only fill of a buffer of n_elems
+    doubles on the build's stream: the library's plain store kernel (one
+    16-byte store per thread, the faster of with and without the
+    non-temporal hint; bs2e_store_ceiling) and torch's own fill (zero_)."""
     import torch
     with torch.cuda.stream(stream):
-        buf = torch.empty(n_elems, dtype=torch.float64, device="cuda")
+        buf = torch.empty(n_elems + 2, dtype=torch.float64, device="cuda")
+        off = (-buf.data_ptr() % 16) // 8          # a 16-byte aligned start
+        lib = ctx.store_ceiling(buf.data_ptr() + 8 * off, 8 * (n_elems // 2) * 2, 5)
         buf.zero_()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
@@ -276,10 +281,10 @@ def store_ceiling(stream, n_elems):
             buf.zero_()
         e1.record(stream)
     torch.cuda.synchronize()
-    out = 8.0 * n_elems * 5 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    tfill = 8.0 * (n_elems + 2) * 5 / (e0.elapsed_time(e1) * 1e-3) / 1e9
     del buf
     torch.cuda.empty_cache()
-    return out
+    return lib, tfill
 
 
 def branch_terms(n_b, k, lmax, max_L=2, l2=1):
@@ -598,6 +603,8 @@ def main():
                     help="also time the c2 branch (tensor + L=0..2 blocks) eager against its captured graph (0 skips)")
     ap.add_argument("--serve-jobs", type=int, default=64,
                     help="jobs of the serving stream (differing bases through the build service); 0 skips it")
+    ap.add_argument("--far-shape", type=int, default=-1,
+                    help="pin the rank-one fill shape (bs2e_rk_set_tuning) instead of autotuning (-1: autotune)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size 3 times on the build stream "
                          "(reference kernel for HBM counter scaling under rocprofv3 --pmc)")
@@ -626,7 +633,11 @@ def main():
         rk = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx, compute=False, precision=precision)
         n_int = rk.n_pairs * rk.n_pairs * rk.K
         log(f"bench {w.name} {precision}: n_b={rk.n_b} n_pairs={rk.n_pairs} K={rk.K} integrals/step={n_int}")
-        rk.autotune()                 # fill shape and multipole chunk timed on this basis
+        if args.far_shape >= 0:
+            rk.autotune()
+            rk.set_tuning(args.far_shape, rk.tuning()[1])   # the pinned shape, the tuned multipole chunk
+        else:
+            rk.autotune()                 # fill shape and multipole chunk timed on this basis
         tuning = rk.tuning()
         m = measure_build(rk, stream, args.steps, args.warmup, args.latency_samples)
         log(f"{w.name} {precision}: {m['ms_per_step']:.4f} ms/build, {m['value']:.3e} integrals/s")
@@ -648,7 +659,7 @@ def main():
                                         sample=f"{len(rows)} tensor rows (first, last, {len(rows) - 2} drawn, seed 3)")
         rk.release()
         runs[precision] = m
-    ceiling = store_ceiling(stream, n_int)
+    ceiling, torch_fill = store_ceiling(ctx, stream, n_int)
     if args.copy_calibration_mib > 0:
         n = args.copy_calibration_mib * (1 << 20) // 8
         with torch.cuda.stream(stream):
@@ -692,7 +703,10 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": far["achieved"] / HBM_PEAK_GBS,
                      "traffic": far["traffic"], "algorithmic_bytes": far["traffic_alg"], "elements": far["far_elems"],
                      "avg_launch_ms": far["t_far"] * 1e3, "store_ceiling_GBs": ceiling,
-                     "frac_of_store_ceiling": far["achieved"] / ceiling},
+                     "frac_of_store_ceiling": far["achieved"] / ceiling,
+                     "store_ceiling_is": "the library's plain store kernel (one 16-byte store per thread, the faster of "
+                                         "with and without the non-temporal hint) over a tensor-sized buffer",
+                     "torch_fill_GBs": torch_fill},
         "mixed_over_fp64": mx["value"] / fp["value"],
         "fp64": dict(fp, roofline_frac=fp_far["achieved"] / HBM_PEAK_GBS),
     }

@@ -162,6 +162,19 @@ int64_t bs2e_rk_autotune(bs2e_rk *rk, int64_t force);
 /* out[0] = rank-one fill shape in use, out[1] = multipoles per
  * diagonal-cell workgroup. */
 int64_t bs2e_rk_tuning(const bs2e_rk *rk, int64_t *out, int64_t n);
+/* Measurement: writes zeros over n_bytes of device memory at dst (16-byte
+ * aligned, a positive multiple of 16 bytes) on the context's stream with the
+ * library's plain store kernel (one 16-byte store per thread), without and
+ * then with the non-temporal hint, each reps times after a warm-up pass, and
+ * reports the faster rate in GB/s (HIP events): the store ceiling the tensor's
+ * rank-one fill is measured against.  (No reference counterpart.) */
+int64_t bs2e_store_ceiling(bs2e_ctx *ctx, void *dst, int64_t n_bytes, int64_t reps, double *gbs);
+/* Pins the rank-one fill shape (0 .. the shapes the autotuner times; those
+ * with two elements per store need an even n_pairs * K) and the multipoles
+ * per diagonal-cell workgroup (1 .. the largest that fits) for this handle's
+ * next builds; BS2E_EINVAL outside those.  All choices compute the same
+ * values.  (No reference counterpart.) */
+int64_t bs2e_rk_set_tuning(bs2e_rk *rk, int64_t shape, int64_t kc);
 int64_t bs2e_rk_release(bs2e_rk *rk);
 /* No reference counterpart: for callers that cannot allocate device memory
  * (code under stream capture, tight serving loops, harnesses that forbid

@@ -131,18 +131,40 @@ def test_service_waits_for_memory(ctx):
 
 @pytest.mark.parametrize("name", ["c1", "c2"])
 def test_autotune_keeps_values(ctx, name):
-    """Every fill shape and multipole chunk the autotuner picks from computes
-    the same tensor, bit for bit; a second basis of the same shape reuses the
-    choice."""
+    """Every rank-one fill shape and every diagonal-cell multipole chunk,
+    pinned one at a time, computes the same tensor, bit for bit; the
+    autotuner picks one of them, and a second basis of the same shape reuses
+    the choice."""
     w, t = knots_of(name)
     ref_rk = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx)
     ref = ref_rk.to_host()
     ref_rk.release()
     rk = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx, compute=False)
+    shapes, chunks = [], []
+    for shape in range(16):
+        try:
+            rk.set_tuning(shape, 1)
+        except bs2e.Bs2eError:
+            continue
+        rk.compute()
+        assert np.array_equal(rk.to_host(), ref), shape
+        shapes.append(shape)
+    for kc in range(1, 17):
+        try:
+            rk.set_tuning(shapes[0], kc)
+        except bs2e.Bs2eError:
+            continue
+        rk.compute()
+        assert np.array_equal(rk.to_host(), ref), kc
+        chunks.append(kc)
+    print(f"{name}: fill shapes {shapes}, multipole chunks {chunks} give the same tensor", flush=True)
+    assert len(shapes) >= 3 and chunks and chunks == list(range(1, len(chunks) + 1))
+    with pytest.raises(bs2e.Bs2eError):
+        rk.set_tuning(shapes[0], chunks[-1] + 1)
     rk.autotune(force=True)
     shape, kc = rk.tuning()
-    print(f"{name}: fill shape {shape}, multipole chunk {kc}", flush=True)
-    assert 0 <= shape < 7 and 1 <= kc <= 16
+    print(f"{name}: autotuned fill shape {shape}, multipole chunk {kc}", flush=True)
+    assert shape in shapes and kc in chunks
     rk.compute()
     assert np.array_equal(rk.to_host(), ref)
     again = bs2e.RkTensor.build(w.k, t, w.k_GL, w.max_k, ctx=ctx, compute=False)
