@@ -58,6 +58,7 @@ static void ctx_free_now(bs2e_ctx* ctx)
             (void)hipStreamSynchronize(s);
             (void)hipStreamDestroy(s);
         }
+    if (ctx->pin) (void)hipHostFree(ctx->pin);
     delete ctx;
 }
 

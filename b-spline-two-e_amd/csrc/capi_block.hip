@@ -117,10 +117,10 @@ struct bs2e_blockset {
     std::vector<BlockSeg> seg;
     std::vector<int32_t> cfg_chan, flags;
     std::vector<double> coef;
-    std::vector<int64_t> cntH, cntS;               // per global row
     std::vector<int64_t> nnzH, nnzS, refH, refS;   // per symmetry
     std::vector<int64_t> offH, offS;               // first entry of each symmetry in the concatenated output
     std::vector<int64_t> prefH, prefS;             // n_rows + 1: first entry of each global row in the whole set
+                                                   // (row g holds prefH[g + 1] - prefH[g] H entries)
     int64_t totH = 0, totS = 0;
     // the rows the outputs hold: global rows [win_lo, win_lo + win_n), whose
     // entries are [winH, winH + nwinH) and [winS, winS + nwinS) of the whole
@@ -373,25 +373,42 @@ static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* 
     for (int64_t s = 0; s < b->n_sym; ++s) b->tab[s] = set_args(b, s, false);
     HIP_TRY(ctx, hipMemcpyAsync(b->d_tab, b->tab.data(), sizeof(BlockArgs) * b->n_sym, hipMemcpyHostToDevice, st));
     clear_stale_error();
+    // the count pass, the rows' CSR offsets by a prefix sum on the device, and
+    // the four per-row counts back to the host through the context's pinned
+    // staging (the host keeps the offsets for the row windows and sizes)
+    int64_t* d_tmp = nullptr;
+    HIP_TRY(ctx, ctx_alloc(ctx, &d_tmp, (size_t)row_scan_tmp_elems(N)));
+    std::lock_guard<std::mutex> pin_lock(ctx->pin_mu);
+    const size_t need = sizeof(int64_t) * 4 * (size_t)N;
+    if (ctx->pin_bytes < need) {
+        if (ctx->pin) (void)hipHostFree(ctx->pin);
+        ctx->pin = nullptr;
+        ctx->pin_bytes = 0;
+        const hipError_t pe = hipHostMalloc(&ctx->pin, need, hipHostMallocDefault);
+        if (pe != hipSuccess) {
+            ctx->pin = nullptr;
+            ctx_free(ctx, d_tmp);
+            HIP_TRY(ctx, pe);
+        }
+        ctx->pin_bytes = need;
+    }
+    const int64_t* cnt = (const int64_t*)ctx->pin;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    HIP_TRY(ctx, hipEventCreate(&e0));
-    HIP_TRY(ctx, hipEventCreate(&e1));
-    std::vector<int64_t> cnt((size_t)4 * N);
-    hipError_t le = hipEventRecord(e0, st);
+    hipError_t le = hipEventCreate(&e0);
+    if (le == hipSuccess) le = hipEventCreate(&e1);
+    if (le == hipSuccess) le = hipEventRecord(e0, st);
     if (le == hipSuccess) le = launch_block_count(b->d_tab, b->d_row0, (int)b->n_sym, N, st);
     if (le == hipSuccess) le = hipEventRecord(e1, st);
-    if (le == hipSuccess)
-        le = hipMemcpyAsync(cnt.data(), b->d_cnt, sizeof(int64_t) * 4 * N, hipMemcpyDeviceToHost, st);
+    if (le == hipSuccess) le = launch_row_scan(b->d_cnt, b->d_ptr, N, d_tmp, st);
+    if (le == hipSuccess) le = hipMemcpyAsync(ctx->pin, b->d_cnt, need, hipMemcpyDeviceToHost, st);
     if (le == hipSuccess) le = hipStreamSynchronize(st);
     if (le == hipSuccess) le = hipEventElapsedTime(&b->ms_count, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    ctx_free(ctx, d_tmp);
     HIP_TRY(ctx, le);
-    b->cntH.assign(cnt.begin(), cnt.begin() + N);
-    b->cntS.assign(cnt.begin() + N, cnt.begin() + 2 * N);
-    std::vector<int64_t> ptr((size_t)2 * N);
-    b->prefH.assign((size_t)N + 1, 0);
-    b->prefS.assign((size_t)N + 1, 0);
+    b->prefH.resize((size_t)N + 1);
+    b->prefS.resize((size_t)N + 1);
     b->nnzH.assign(b->n_sym, 0);
     b->nnzS.assign(b->n_sym, 0);
     b->refH.assign(b->n_sym, 0);
@@ -399,24 +416,27 @@ static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* 
     b->offH.assign(b->n_sym, 0);
     b->offS.assign(b->n_sym, 0);
     b->totH = b->totS = 0;
+    int64_t* ph = b->prefH.data();
+    int64_t* ps = b->prefS.data();
     for (int64_t s = 0; s < b->n_sym; ++s) {
         b->offH[s] = b->totH;
         b->offS[s] = b->totS;
+        int64_t rh = 0, rs = 0;
         for (int64_t g = b->row0[s]; g < b->row0[s + 1]; ++g) {
-            ptr[g] = b->prefH[g] = b->totH;
-            ptr[N + g] = b->prefS[g] = b->totS;
-            b->totH += b->cntH[g];
-            b->totS += b->cntS[g];
-            b->refH[s] += cnt[2 * N + g];
-            b->refS[s] += cnt[3 * N + g];
+            ph[g] = b->totH;
+            ps[g] = b->totS;
+            b->totH += cnt[g];
+            b->totS += cnt[N + g];
+            rh += cnt[2 * N + g];
+            rs += cnt[3 * N + g];
         }
+        b->refH[s] = rh;
+        b->refS[s] = rs;
         b->nnzH[s] = b->totH - b->offH[s];
         b->nnzS[s] = b->totS - b->offS[s];
     }
-    b->prefH[N] = b->totH;
-    b->prefS[N] = b->totS;
-    HIP_TRY(ctx, hipMemcpyAsync(b->d_ptr, ptr.data(), sizeof(int64_t) * 2 * N, hipMemcpyHostToDevice, st));
-    HIP_TRY(ctx, hipStreamSynchronize(st));
+    ph[N] = b->totH;
+    ps[N] = b->totS;
     return BS2E_OK;
 }
 
@@ -565,8 +585,8 @@ static int64_t set_get(const bs2e_blockset* b, int64_t s, int64_t* ip_H, int64_t
     ip_S[0] = 1;
     const int64_t r0 = b->row0[s], n = b->row0[s + 1] - r0;
     for (int64_t i = 0; i < n; ++i) {
-        ip_H[i + 1] = ip_H[i] + b->cntH[r0 + i];
-        ip_S[i + 1] = ip_S[i] + b->cntS[r0 + i];
+        ip_H[i + 1] = 1 + b->prefH[r0 + i + 1] - b->prefH[r0];
+        ip_S[i + 1] = 1 + b->prefS[r0 + i + 1] - b->prefS[r0];
     }
     return BS2E_OK;
 }
@@ -747,9 +767,10 @@ int64_t bs2e_blockset_get_rows(const bs2e_blockset* b, int64_t sym, int64_t row_
         return fail(ctx, BS2E_ERANGE, "bs2e_blockset_get_rows: rows outside the symmetry's configurations");
     ip_H[0] = 1;
     ip_S[0] = 1;
+    const int64_t g0 = r0 + row_lo;
     for (int64_t i = 0; i < n_rows; ++i) {
-        ip_H[i + 1] = ip_H[i] + b->cntH[r0 + row_lo + i];
-        ip_S[i + 1] = ip_S[i] + b->cntS[r0 + row_lo + i];
+        ip_H[i + 1] = 1 + b->prefH[g0 + i + 1] - b->prefH[g0];
+        ip_S[i + 1] = 1 + b->prefS[g0 + i + 1] - b->prefS[g0];
     }
     if (!ix_H && !d_H && !ix_S && !d_S) return BS2E_OK;   // sizes only
     if (!ix_H || !d_H || !ix_S || !d_S) return fail(ctx, BS2E_EINVAL, "bs2e_blockset_get_rows: NULL argument");
@@ -779,8 +800,10 @@ int64_t bs2e_blockset_row_counts(const bs2e_blockset* b, int64_t row_lo, int64_t
     if (!b || !cnt_H || !cnt_S) return fail(b ? b->rk->ctx : nullptr, BS2E_EINVAL, "bs2e_blockset_row_counts: NULL argument");
     if (row_lo < 0 || n_rows < 0 || row_lo + n_rows > b->n_rows)
         return fail(b->rk->ctx, BS2E_ERANGE, "bs2e_blockset_row_counts: rows outside the set");
-    std::copy(b->cntH.begin() + row_lo, b->cntH.begin() + row_lo + n_rows, cnt_H);
-    std::copy(b->cntS.begin() + row_lo, b->cntS.begin() + row_lo + n_rows, cnt_S);
+    for (int64_t r = 0; r < n_rows; ++r) {
+        cnt_H[r] = b->prefH[row_lo + r + 1] - b->prefH[row_lo + r];
+        cnt_S[r] = b->prefS[row_lo + r + 1] - b->prefS[row_lo + r];
+    }
     return BS2E_OK;
 }
 

@@ -44,6 +44,11 @@ struct bs2e_ctx {
     hipStream_t stream = nullptr;   // the build's stream (own, or the caller's)
     hipStream_t side = nullptr;     // diagonal cells + near ranges, concurrent with the rank-one fill
     std::string err;
+    // pinned host staging for device-to-host copies of per-row tables (grown
+    // on demand, freed with the context; pin_mu held while it is in use)
+    void* pin = nullptr;
+    size_t pin_bytes = 0;
+    std::mutex pin_mu;
 };
 
 struct bs2e_rk {
