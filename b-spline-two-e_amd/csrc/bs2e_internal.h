@@ -145,7 +145,7 @@ __device__ __forceinline__ int64_t xcd_chunk_block(int64_t b, int64_t n, int64_t
 
 struct BlockArgs {
     int n_config, n_chan, full, L, ks, n_b, n_pairs, K, n_iv;
-    const int64_t* cfg;          // 5 per config: n1, n2, l1, l2, eqv
+    const int32_t* cfg;          // 2 per config: n1 | n2 << 16, l1 | l2 << 16
     const int32_t* cfg_chan;     // channel of each config
     const BlockChan* chan;       // per channel
     const BlockSeg* seg;         // [n_chan][n_b + 1]: per n1

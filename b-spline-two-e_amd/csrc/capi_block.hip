@@ -129,7 +129,7 @@ struct bs2e_blockset {
     std::vector<BlockArgs> tab;                    // host copy of the descriptor table
     bool filled = false;
     // device
-    int64_t* d_cfg = nullptr;
+    int32_t* d_cfg = nullptr;   // 2 per configuration: n1 | n2 << 16, l1 | l2 << 16
     int32_t* d_cfg_chan = nullptr;
     BlockChan* d_chan = nullptr;
     BlockSeg* d_seg = nullptr;
@@ -244,7 +244,7 @@ static BlockArgs set_args(const bs2e_blockset* b, int64_t s, bool fill)
     a.n_pairs = (int)p.n_pairs;
     a.K = (int)p.K;
     a.n_iv = (int)p.n_iv;
-    a.cfg = b->d_cfg + 5 * r0;
+    a.cfg = b->d_cfg + 2 * r0;
     a.cfg_chan = b->d_cfg_chan + r0;
     a.chan = b->d_chan + b->chan_off[s];
     a.seg = b->d_seg + b->seg_off[s];
@@ -334,17 +334,25 @@ static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* 
     }
     if (N >= ((int64_t)1 << 31))
         return fail(ctx, BS2E_EINVAL, std::string(who) + ": a set holds fewer than 2^31 configurations");
-    b->rowlist.clear();
-    b->rowlist.reserve((size_t)N);
-    for (int pass = 0; pass < 2; ++pass) {   // staged rows, then the others
-        for (int64_t g = 0; g < N; ++g)
-            if ((std::llabs(configs[5 * g] - configs[5 * g + 1]) >= p.k_s) == (pass == 0))
-                b->rowlist.push_back((int32_t)g);
-        if (pass == 0) b->n_sep = (int64_t)b->rowlist.size();
+    // the configurations as the kernels read them (n1 | n2 << 16, l1 | l2 << 16:
+    // channel_tables checked n < 2^15 and l < 64), and the row list: the
+    // staged rows (|n1 - n2| >= k), then the others, each ascending
+    std::vector<int32_t> pk((size_t)2 * N);
+    b->n_sep = 0;
+    for (int64_t g = 0; g < N; ++g) {
+        const int64_t* c = configs + 5 * g;
+        pk[2 * g] = (int32_t)(c[0] | (c[1] << 16));
+        pk[2 * g + 1] = (int32_t)(c[2] | (c[3] << 16));
+        b->n_sep += std::llabs(c[0] - c[1]) >= p.k_s;
+    }
+    b->rowlist.resize((size_t)N);
+    for (int64_t g = 0, a = 0, o = b->n_sep; g < N; ++g) {
+        const int32_t w = pk[2 * g], n1 = w & 0xffff, n2 = w >> 16;
+        b->rowlist[(size_t)(std::abs(n1 - n2) >= p.k_s ? a++ : o++)] = (int32_t)g;
     }
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_rowlist, (size_t)N));
-    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_cfg, (size_t)5 * N));
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_cfg, (size_t)2 * N));
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_cfg_chan, (size_t)N));
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_chan, b->chan.size()));
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_seg, b->seg.size()));
@@ -355,7 +363,7 @@ static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* 
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_tab, (size_t)b->n_sym));
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_row0, (size_t)b->n_sym + 1));
     hipStream_t st = ctx->stream;
-    HIP_TRY(ctx, hipMemcpyAsync(b->d_cfg, configs, sizeof(int64_t) * 5 * N, hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(b->d_cfg, pk.data(), sizeof(int32_t) * 2 * N, hipMemcpyHostToDevice, st));
     HIP_TRY(ctx, hipMemcpyAsync(b->d_cfg_chan, b->cfg_chan.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
     HIP_TRY(ctx, hipMemcpyAsync(b->d_rowlist, b->rowlist.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
     HIP_TRY(ctx, hipMemcpyAsync(b->d_chan, b->chan.data(), sizeof(BlockChan) * b->chan.size(), hipMemcpyHostToDevice,
@@ -530,7 +538,7 @@ size_t bs2e::blockset_fill_bytes(const bs2e_blockset* b, int64_t n_l)
 size_t bs2e::blockset_table_bytes(const bs2e_blockset* b)
 {
     const size_t N = (size_t)b->n_rows;
-    return arena_round(40 * N) + arena_round(4 * N) + arena_round(4 * N) +
+    return arena_round(8 * N) + arena_round(4 * N) + arena_round(4 * N) +
            arena_round(sizeof(BlockChan) * b->chan.size()) + arena_round(sizeof(BlockSeg) * b->seg.size()) +
            arena_round(8 * b->coef.size()) +
            arena_round(4 * b->flags.size()) + arena_round(32 * N) + arena_round(16 * N) +

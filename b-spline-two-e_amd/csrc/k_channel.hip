@@ -937,9 +937,9 @@ __global__ __launch_bounds__(kCThreads, (SEP && NT == 1) || (!SEP && NT > 1) ? 4
     // of them in scalar registers and spilling them to vector lanes.
     const CBlockArgs& A = ((const CBlockArgs*)tab)[b];
     const int i = (int)(g - row0[b]);
-    const int64_t* ci = A.cfg + 5 * (size_t)i;
-    const int n_a = __builtin_amdgcn_readfirstlane((int)ci[0]), n_bb = __builtin_amdgcn_readfirstlane((int)ci[1]);
-    const int l_a = __builtin_amdgcn_readfirstlane((int)ci[2]), l_b = __builtin_amdgcn_readfirstlane((int)ci[3]);
+    const int32_t* ci = A.cfg + 2 * (size_t)i;
+    const int cn = __builtin_amdgcn_readfirstlane(ci[0]), cl = __builtin_amdgcn_readfirstlane(ci[1]);
+    const int n_a = cn & 0xffff, n_bb = cn >> 16, l_a = cl & 0xffff, l_b = cl >> 16;
     const int ch_i = __builtin_amdgcn_readfirstlane(A.cfg_chan[i]);
     if constexpr (SEP)
         fill_row_staged<NT>(A, i, n_a, n_bb, l_a, l_b, ch_i, lane, lds_fill + (size_t)wave * sep_wave_lds(lds_K, NT));
@@ -981,9 +981,9 @@ __global__ __launch_bounds__(kCThreads) void kc_count(const BlockArgs* __restric
     const BlockArgs A = tab[b];
     const int i = (int)(g - row0[b]);
     const int ks = A.ks, n_b = A.n_b;
-    const int64_t* ci = A.cfg + 5 * (size_t)i;
-    const int n_a = __builtin_amdgcn_readfirstlane((int)ci[0]), n_bb = __builtin_amdgcn_readfirstlane((int)ci[1]);
-    const int l_a = __builtin_amdgcn_readfirstlane((int)ci[2]), l_b = __builtin_amdgcn_readfirstlane((int)ci[3]);
+    const int32_t* ci = A.cfg + 2 * (size_t)i;
+    const int cn = __builtin_amdgcn_readfirstlane(ci[0]), cl = __builtin_amdgcn_readfirstlane(ci[1]);
+    const int n_a = cn & 0xffff, n_bb = cn >> 16, l_a = cl & 0xffff, l_b = cl >> 16;
     const int ch_i = __builtin_amdgcn_readfirstlane(A.cfg_chan[i]);
     const CandRanges cr = candidate_ranges(n_a, n_bb, ks, n_b);
     const bool cand = lane < cr.n1 + cr.n2;
