@@ -153,6 +153,12 @@ struct bs2e_blockset {
     std::vector<int32_t> rowlist;
     int64_t n_sep = 0;
     int32_t* d_rowlist = nullptr;
+    // a count pass enqueued without waiting (the build service's admission):
+    // its prefix sums' scratch, timing events and the event behind the
+    // counts' copy to the context's pinned staging; set_count_finish reads them
+    int64_t* d_count_tmp = nullptr;
+    hipEvent_t count_ev[3] = {nullptr, nullptr, nullptr};
+    bool count_pending = false;
 };
 
 // A single symmetry's block is a set of one.
@@ -200,6 +206,12 @@ static void set_free(bs2e_blockset* b)
     ctx_free(c, b->d_tab);
     ctx_free(c, b->d_row0);
     ctx_free(c, b->d_rowlist);
+    if (b->count_pending) (void)hipStreamSynchronize(c->stream);   // the pass still uses its scratch
+    ctx_free(c, b->d_count_tmp);
+    for (hipEvent_t& e : b->count_ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
 }
 
 // The fill launches over the rows of the window: the staged rows and the
@@ -279,7 +291,7 @@ static BlockArgs set_args(const bs2e_blockset* b, int64_t s, bool fill)
 
 // Host tables, device upload and the count launch of a set whose rk, n_sym,
 // full, L and row0 are given.  Returns BS2E_OK or the error code (set on ctx).
-static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* who)
+static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* who, bool async)
 {
     bs2e_ctx* ctx = b->rk->ctx;
     const Plan& p = b->rk->plan;
@@ -383,38 +395,46 @@ static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* 
     clear_stale_error();
     // the count pass, the rows' CSR offsets by a prefix sum on the device, and
     // the four per-row counts back to the host through the context's pinned
-    // staging (the host keeps the offsets for the row windows and sizes)
-    int64_t* d_tmp = nullptr;
-    HIP_TRY(ctx, ctx_alloc(ctx, &d_tmp, (size_t)row_scan_tmp_elems(N)));
-    std::lock_guard<std::mutex> pin_lock(ctx->pin_mu);
+    // staging (the host keeps the offsets for the row windows and sizes).
+    // async: enqueued behind an event and left running (the build service
+    // owns its contexts and reads the counts with set_count_finish).
+    HIP_TRY(ctx, ctx_alloc(ctx, &b->d_count_tmp, (size_t)row_scan_tmp_elems(N)));
+    std::unique_lock<std::mutex> pin_lock(ctx->pin_mu, std::defer_lock);
+    if (!async) pin_lock.lock();
     const size_t need = sizeof(int64_t) * 4 * (size_t)N;
     if (ctx->pin_bytes < need) {
         if (ctx->pin) (void)hipHostFree(ctx->pin);
         ctx->pin = nullptr;
         ctx->pin_bytes = 0;
-        const hipError_t pe = hipHostMalloc(&ctx->pin, need, hipHostMallocDefault);
-        if (pe != hipSuccess) {
-            ctx->pin = nullptr;
-            ctx_free(ctx, d_tmp);
-            HIP_TRY(ctx, pe);
-        }
+        HIP_TRY(ctx, hipHostMalloc(&ctx->pin, need, hipHostMallocDefault));
         ctx->pin_bytes = need;
     }
-    const int64_t* cnt = (const int64_t*)ctx->pin;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipError_t le = hipEventCreate(&e0);
-    if (le == hipSuccess) le = hipEventCreate(&e1);
-    if (le == hipSuccess) le = hipEventRecord(e0, st);
+    for (hipEvent_t& e : b->count_ev)
+        if (!e) HIP_TRY(ctx, hipEventCreate(&e));
+    hipError_t le = hipEventRecord(b->count_ev[0], st);
     if (le == hipSuccess) le = launch_block_count(b->d_tab, b->d_row0, (int)b->n_sym, N, st);
-    if (le == hipSuccess) le = hipEventRecord(e1, st);
-    if (le == hipSuccess) le = launch_row_scan(b->d_cnt, b->d_ptr, N, d_tmp, st);
+    if (le == hipSuccess) le = hipEventRecord(b->count_ev[1], st);
+    if (le == hipSuccess) le = launch_row_scan(b->d_cnt, b->d_ptr, N, b->d_count_tmp, st);
     if (le == hipSuccess) le = hipMemcpyAsync(ctx->pin, b->d_cnt, need, hipMemcpyDeviceToHost, st);
-    if (le == hipSuccess) le = hipStreamSynchronize(st);
-    if (le == hipSuccess) le = hipEventElapsedTime(&b->ms_count, e0, e1);
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    ctx_free(ctx, d_tmp);
+    if (le == hipSuccess) le = hipEventRecord(b->count_ev[2], st);
     HIP_TRY(ctx, le);
+    b->count_pending = true;
+    return async ? BS2E_OK : set_count_finish(b);
+}
+
+// Waits for the count pass (set_create) and takes the rows' offsets, the
+// symmetries' sizes and the reference's counts from the counts it copied to
+// the context's pinned staging.
+int64_t bs2e::set_count_finish(bs2e_blockset* b)
+{
+    bs2e_ctx* ctx = b->rk->ctx;
+    if (!b->count_pending) return BS2E_OK;
+    HIP_TRY(ctx, hipEventSynchronize(b->count_ev[2]));
+    b->count_pending = false;
+    HIP_TRY(ctx, hipEventElapsedTime(&b->ms_count, b->count_ev[0], b->count_ev[1]));
+    ctx_free(ctx, b->d_count_tmp);
+    const int64_t N = b->n_rows;
+    const int64_t* cnt = (const int64_t*)ctx->pin;
     b->prefH.resize((size_t)N + 1);
     b->prefS.resize((size_t)N + 1);
     b->nnzH.assign(b->n_sym, 0);
@@ -637,7 +657,7 @@ int64_t bs2e_block_create(bs2e_rk* rk, int64_t L, int64_t n_config, const int64_
     b->L = {L};
     b->row0 = {0, n_config};
     b->n_rows = n_config;
-    const int64_t rc = set_create(b, configs, "bs2e_block_create");
+    const int64_t rc = set_create(b, configs, "bs2e_block_create", false);
     if (rc) {
         set_free(b);
         delete b;
@@ -702,7 +722,7 @@ int64_t bs2e_blockset_create(bs2e_rk* rk, int64_t n_sym, const int64_t* L, const
 // before its tensor is built (the fill, enqueued after the build on the same
 // stream, reads the tensor).
 int64_t bs2e::blockset_new(bs2e_rk* rk, int64_t n_sym, const int64_t* L, const int64_t* n_config,
-                           const int64_t* configs, int64_t full, bs2e_blockset** out)
+                           const int64_t* configs, int64_t full, bs2e_blockset** out, bool async)
 {
     if (!rk || !out || !L || !n_config || !configs)
         return fail(rk ? rk->ctx : nullptr, BS2E_EINVAL, "bs2e_blockset_create: NULL argument");
@@ -725,7 +745,7 @@ int64_t bs2e::blockset_new(bs2e_rk* rk, int64_t n_sym, const int64_t* L, const i
     b->row0.assign((size_t)n_sym + 1, 0);
     for (int64_t s = 0; s < n_sym; ++s) b->row0[s + 1] = b->row0[s] + n_config[s];
     b->n_rows = total;
-    const int64_t rc = set_create(b, configs, "bs2e_blockset_create");
+    const int64_t rc = set_create(b, configs, "bs2e_blockset_create", async);
     if (rc) {
         set_free(b);
         delete b;
@@ -735,6 +755,8 @@ int64_t bs2e::blockset_new(bs2e_rk* rk, int64_t n_sym, const int64_t* L, const i
     *out = b;
     return BS2E_OK;
 }
+
+hipEvent_t bs2e::set_counted_event(const bs2e_blockset* b) { return b->count_ev[2]; }
 
 extern "C" {
 

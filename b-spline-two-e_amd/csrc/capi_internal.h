@@ -154,7 +154,11 @@ int64_t check_plan_limits(bs2e_ctx* ctx, const Plan& p, bool needs_primitives);
 // device (column-major, n_l of H then S), enqueued on the tensor context's
 // stream (wait: block until done).
 int64_t blockset_new(bs2e_rk* rk, int64_t n_sym, const int64_t* L, const int64_t* n_config, const int64_t* configs,
-                     int64_t full, bs2e_blockset** out);
+                     int64_t full, bs2e_blockset** out, bool async = false);
+// With async, blockset_new leaves the count pass running; set_count_finish
+// waits for it and takes the sizes (set_counted_event: recorded behind it).
+int64_t set_count_finish(bs2e_blockset* b);
+hipEvent_t set_counted_event(const bs2e_blockset* b);
 int64_t set_fill_device(bs2e_blockset* b, int64_t n_l, const double* dHl, const double* dS, bool wait,
                         const char* who);
 // set_fill_device in two steps: the allocations (nothing enqueued), then the

@@ -56,6 +56,7 @@ struct Job {
     size_t bytes = 0;              // device footprint: tensor + tables (+ blocks, once counted)
     bool sized = false;            // bytes includes the blocks' exact sizes
     int state = 0;                 // 0 queued, 1 running, 2 done, 3 failed
+    int phase = 0;                 // while running: 1 blocks being counted, 2 fill enqueued (done event recorded)
     int slot = -1;
     double t_submit = 0, t_admit = 0, t_done = 0;
     bs2e_rk* rk = nullptr;
@@ -186,12 +187,12 @@ static int64_t job_fail(bs2e_server* s, Job& j, int64_t code)
     return code;
 }
 
-// Admits job j on slot `slot`: the tensor (and its tables) in the arena, the
-// blocks counted (a short synchronous pass on the slot's stream: the count
-// kernel after the tables' upload), the one-electron tables found in the
-// cache or computed, the block outputs allocated, then the build and the
-// fill enqueued behind each other and the done event recorded.  Returns
-// BS2E_ENOMEM (nothing held, j.bytes updated) when the budget lacks room now.
+// Admits job j on slot `slot`, phase one: the tensor (and its tables) in the
+// arena and its build enqueued, the blocks' tables uploaded and their count
+// pass enqueued behind it without waiting (the counts come back to the slot
+// context's pinned staging while the build runs).  A job without blocks is
+// done with its build.  Returns BS2E_ENOMEM (nothing held) when the budget
+// lacks room now.
 static int64_t admit(bs2e_server* s, Job& j, int slot)
 {
     bs2e_ctx* c = s->slots[slot];
@@ -202,49 +203,77 @@ static int64_t admit(bs2e_server* s, Job& j, int slot)
         return rc;
     }
     if (j.n_sym > 0) {
-        rc = blockset_new(j.rk, j.n_sym, j.L.data(), j.n_config.data(), j.configs.data(), j.full, &j.set);
-        OneBody* ob = rc ? nullptr : cache_find(s, j);
-        bool made = false;
-        if (!rc && !ob) {
-            rc = cache_make(s, j, c, &ob);
-            made = rc == BS2E_OK;
-        }
-        if (!rc) {
-            j.ob = ob;
-            ob->refs += 1;
-            ob->last = ++s->tick;
-            rc = set_fill_alloc(j.set, j.max_l_1p + 1, "bs2e_server");
-        }
-        if (j.set) {
-            j.bytes = rk_footprint(j.plan) + blockset_table_bytes(j.set) + blockset_fill_bytes(j.set, j.max_l_1p + 1);
-            j.sized = true;
-        }
+        rc = blockset_new(j.rk, j.n_sym, j.L.data(), j.n_config.data(), j.configs.data(), j.full, &j.set, true);
         if (rc) {
-            if (made) (void)hipStreamSynchronize(c->stream);   // the new entry's kernels read the job's tables
             job_release(j);
             return rc;
         }
     }
     rc = bs2e_rk_compute(j.rk);
-    if (!rc && j.set) {
-        if (j.ob->made_on != c->stream && hipStreamWaitEvent(c->stream, j.ob->ready, 0) != hipSuccess)
-            rc = fail(s->err, BS2E_EDEVICE, "bs2e_server: hipStreamWaitEvent failed");
-        if (!rc) rc = set_fill_enqueue(j.set, j.ob->H, j.ob->S, false, "bs2e_server");
-    }
-    if (!rc && hipEventRecord(j.done, c->stream) != hipSuccess) rc = BS2E_EDEVICE;
+    if (!rc && !j.set && hipEventRecord(j.done, c->stream) != hipSuccess) rc = BS2E_EDEVICE;
     if (rc) {
         (void)hipStreamSynchronize(c->stream);
         (void)hipStreamSynchronize(c->side);
-    } else if (j.ob) {   // counted once per admitted job: its basis was computed for it, or cached
-        if (j.ob->maker == &j) s->cache_misses += 1;
-        else s->cache_hits += 1;
-        j.ob->maker = nullptr;
+        job_release(j);
+        return rc;
     }
-    return rc;
+    j.phase = j.set ? 1 : 2;
+    return BS2E_OK;
 }
 
-// Retires finished builds and admits queued jobs (FIFO; a job that does not
-// fit yet is passed over by smaller ones behind it).
+// Phase two, once the counts are back: the blocks' exact sizes, the
+// one-electron tables found in the cache or computed, the block outputs
+// allocated (least recently used cache entries evicted if they are in the
+// way), then the fill enqueued behind the build and the done event recorded.
+// Returns BS2E_ENOMEM, with nothing held and j.bytes exact, when the outputs
+// do not fit now.
+static int64_t admit_fill(bs2e_server* s, Job& j)
+{
+    bs2e_ctx* c = s->slots[j.slot];
+    int64_t rc = set_count_finish(j.set);
+    OneBody* ob = rc ? nullptr : cache_find(s, j);
+    if (!rc && !ob) {
+        rc = cache_make(s, j, c, &ob);
+        if (rc == BS2E_ENOMEM && cache_evict(s, onebody_bytes(j.plan, j.max_l_1p + 1))) rc = cache_make(s, j, c, &ob);
+    }
+    if (!rc) {
+        j.ob = ob;
+        ob->refs += 1;
+        ob->last = ++s->tick;
+        rc = set_fill_alloc(j.set, j.max_l_1p + 1, "bs2e_server");
+        if (rc == BS2E_ENOMEM && cache_evict(s, blockset_fill_bytes(j.set, j.max_l_1p + 1), ob))
+            rc = set_fill_alloc(j.set, j.max_l_1p + 1, "bs2e_server");
+    }
+    if (!rc) {
+        if (j.ob->made_on != c->stream && hipStreamWaitEvent(c->stream, j.ob->ready, 0) != hipSuccess)
+            rc = fail(s->err, BS2E_EDEVICE, "bs2e_server: hipStreamWaitEvent failed");
+        if (!rc) rc = set_fill_enqueue(j.set, j.ob->H, j.ob->S, false, "bs2e_server");
+        if (!rc && hipEventRecord(j.done, c->stream) != hipSuccess) rc = BS2E_EDEVICE;
+    }
+    if (j.set) {
+        j.bytes = rk_footprint(j.plan) + blockset_table_bytes(j.set) + blockset_fill_bytes(j.set, j.max_l_1p + 1);
+        j.sized = true;
+    }
+    if (rc) {
+        (void)hipStreamSynchronize(c->stream);   // the build and a new entry's kernels read the job's tables
+        (void)hipStreamSynchronize(c->side);
+        job_release(j);
+        return rc;
+    }
+    if (j.ob->maker == &j) s->cache_misses += 1;   // counted once per admitted job
+    else s->cache_hits += 1;
+    j.ob->maker = nullptr;
+    j.phase = 2;
+    return BS2E_OK;
+}
+
+// The event that marks a running job's next step: its counts (phase one) or
+// its end.
+static hipEvent_t pending_event(const Job& j) { return j.phase == 1 ? set_counted_event(j.set) : j.done; }
+
+// Takes running jobs on (counted blocks to their fill, finished builds out of
+// their slots) and admits queued jobs (FIFO; a job that does not fit yet is
+// passed over by smaller ones behind it).
 static int64_t progress(bs2e_server* s)
 {
     (void)hipSetDevice(s->device);
@@ -252,11 +281,33 @@ static int64_t progress(bs2e_server* s)
         const int64_t t = s->slot_job[i];
         if (t < 0) continue;
         Job& j = s->jobs[t];
+        if (j.phase == 1) {
+            const hipError_t e = hipEventQuery(set_counted_event(j.set));
+            if (e == hipErrorNotReady) continue;
+            if (e != hipSuccess)
+                return fail(s->err, BS2E_EDEVICE, std::string("bs2e_server: count failed: ") + hipGetErrorString(e));
+            const int64_t rc = admit_fill(s, j);
+            if (rc) {
+                s->slot_job[i] = -1;
+                j.slot = -1;
+                j.phase = 0;
+                if (rc == BS2E_ENOMEM && j.bytes <= s->arena.capacity()) {   // room later: first in line again
+                    j.state = 0;
+                    s->queue.push_front(t);
+                } else {
+                    if (rc == BS2E_ENOMEM)
+                        fail(s->err, BS2E_ENOMEM, "bs2e_server: the job's blocks and tensor exceed the service's budget");
+                    job_fail(s, j, rc);
+                }
+                continue;
+            }
+        }
         const hipError_t e = hipEventQuery(j.done);
         if (e == hipErrorNotReady) continue;
         if (e != hipSuccess)
             return fail(s->err, BS2E_EDEVICE, std::string("bs2e_server: build failed: ") + hipGetErrorString(e));
         j.state = 2;
+        j.phase = 0;
         j.t_done = now_ms();
         s->slot_job[i] = -1;
         s->finished.push_back(t);
@@ -466,11 +517,11 @@ int64_t bs2e_server_wait(bs2e_server* s, int64_t ticket)
         if (j.state == 3) return fail(s->err, BS2E_EDEVICE, "bs2e_server_wait: the job failed to start");
         hipEvent_t e = nullptr;
         if (j.state == 1) {
-            e = j.done;
+            e = pending_event(j);
         } else {
             for (int64_t t : s->slot_job)
                 if (t >= 0) {
-                    e = s->jobs[t].done;
+                    e = pending_event(s->jobs[t]);
                     break;
                 }
             if (!e)
