@@ -37,11 +37,16 @@ typedef double dbl2 __attribute__((ext_vector_type(2)));
 
 constexpr int kCThreads = 256;
 constexpr int kWave = 64;
-// Fill workgroups per XCD band.  Measured on the c3 set (L = 0..4, l <= 4)
-// in four processes, each against its own placement of the outputs: bands of
-// 512 fill in 21.4-24.8 ms, within 0.4 ms of the best of 32 to 16384 in each;
-// one eighth of the grid per XCD 24.0-24.9 ms, no remapping 22.3-26.8 ms.
+// Fill workgroups per XCD band, for launches of at least kFillBandRows rows.
+// Measured on the c3 set (L = 0..4, l <= 4; 1.77 M matrix-core rows) in four
+// processes, each against its own placement of the outputs: bands of 512
+// fill in 21.4-24.8 ms, within 0.4 ms of the best of 32 to 16384 in each; one
+// eighth of the grid per XCD 24.0-24.9 ms, no remapping 22.3-26.8 ms.  On
+// smaller sets the bands cost time (the 64 jobs of the serving distribution,
+// 26 K-485 K rows each: 174.8 ms of fills against 155.8 ms in launch order),
+// so they take the launch order.
 constexpr int kFillChunk = 512;
+constexpr int64_t kFillBandRows = (int64_t)1 << 20;
 
 struct Cplx {
     double re, im;
@@ -918,10 +923,12 @@ __global__ __launch_bounds__(kCThreads, (SEP && NT == 1) || (!SEP && NT > 1) ? 4
     extern __shared__ __align__(16) double lds_fill[];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
-    // the XCDs take adjacent bands of kFillChunk workgroups (4 kFillChunk
-    // rows), so that each L2 and its address translation see one band of the
-    // outputs at a time while the bands in flight stay close together
-    const int64_t w = xcd_chunk_block(blockIdx.x, gridDim.x, kFillChunk) * (kCThreads / kWave) + wave;
+    // in a large launch the XCDs take adjacent bands of kFillChunk workgroups
+    // (4 kFillChunk rows), so that each L2 and its address translation see
+    // one band of the outputs at a time while the bands in flight stay close
+    // together
+    const int64_t lb = n >= kFillBandRows ? xcd_chunk_block(blockIdx.x, gridDim.x, kFillChunk) : (int64_t)blockIdx.x;
+    const int64_t w = lb * (kCThreads / kWave) + wave;
     if (w >= n) return;   // whole wave exits together
     const int64_t g = rows[w];
     int blo = 0, bhi = n_blocks - 1;

@@ -323,12 +323,14 @@ def run_stream(srv, jobs, arrivals, submit):
     return lat, time.perf_counter() - t0
 
 
-def serving_benchmark(n_jobs, seed=20261202, slots=4, budget_gib=160, load=0.8, precision="mixed"):
+def serving_benchmark(n_jobs, seed=20261202, slots=4, budget_gib=160, load=0.8, precision="mixed", fixed_rate=60.0):
     """Continuous batching of a seeded stream of basis_setup requests through
     the build service: each job is the two-electron branch (R^k tensor in
     `precision`, one-electron matrices, the H/S blocks of L = 0..2); closed-
     loop capacity, then Poisson arrivals at `load` x capacity with p50/p99
-    latency from arrival to the job's H/S CSR written."""
+    latency from arrival to the job's H/S CSR written, and the same arrivals
+    rescaled to a fixed rate of `fixed_rate` jobs/s (comparable between
+    builds of different capacity)."""
     import numpy as np
 
     import bs2e
@@ -355,9 +357,11 @@ def serving_benchmark(n_jobs, seed=20261202, slots=4, budget_gib=160, load=0.8, 
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / (load * cap), n_jobs))
     lat, wall = run_stream(srv, jobs, arrivals, submit)
+    lat_fixed, _ = run_stream(srv, jobs, arrivals * (load * cap / fixed_rate), submit)
     cache = srv.cache_info()
     srv.close()
     lat = np.array(lat)
+    lat_fixed = np.array(lat_fixed)
     return {"unit_of_work": "the two-electron branch of one basis_setup request: R^k tensor, S and H(l), H/S CSR "
                             "of L = 0..2 (l <= lmax, n2 <= n_b/4, l2 <= 1 beyond n_b/2, upper triangles)",
             "precision": precision, "jobs": n_jobs, "slots": slots, "budget_GiB": budget_gib, "job_seed": 7,
@@ -367,6 +371,8 @@ def serving_benchmark(n_jobs, seed=20261202, slots=4, budget_gib=160, load=0.8, 
             "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                            "mean": float(lat.mean()), "max": float(lat.max()),
                            "from": "submission", "to": "H/S CSR of every symmetry written (device event)"},
+            "latency_ms_fixed_rate": {"jobs_per_s": fixed_rate, "p50": float(np.percentile(lat_fixed, 50)),
+                                      "p99": float(np.percentile(lat_fixed, 99)), "mean": float(lat_fixed.mean())},
             "one_electron_cache": {"entries": cache[0], "bytes": cache[1], "hits": cache[2], "misses": cache[3]}}
 
 
