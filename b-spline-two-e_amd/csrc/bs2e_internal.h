@@ -224,10 +224,12 @@ hipError_t launch_radial_dipole(const Plan& p, const DevTables& d, double* B, do
 // device row offsets row0[n_blocks + 1], n_rows = row0[n_blocks]).
 hipError_t launch_block_count(const BlockArgs* tab, const int64_t* row0, int n_blocks, int64_t n_rows, hipStream_t s);
 // The fill takes the n_gen rows with |n1 - n2| < k listed in rows_gen, then
-// the n_sep staged rows (|n1 - n2| >= k, on the matrix cores) listed in
+// the staged rows (on the matrix cores): the n_far with |n1 - n2| >= 2k - 1
+// listed in rows_far and the n_sep with k <= |n1 - n2| < 2k - 1 listed in
 // rows_sep (global row numbers).
-hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, const int32_t* rows_sep,
-                             int64_t n_sep, const int32_t* rows_gen, int64_t n_gen, int ks, int K, hipStream_t s);
+hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, const int32_t* rows_far,
+                             int64_t n_far, const int32_t* rows_sep, int64_t n_sep, const int32_t* rows_gen,
+                             int64_t n_gen, int ks, int K, hipStream_t s);
 // Exclusive prefix sums of the count pass's per-row counts (cnt: H counts then
 // S counts, n_rows each) into ptr (same layout), on the device; tmp holds
 // row_scan_tmp_elems(n_rows) int64.

@@ -148,10 +148,12 @@ struct bs2e_blockset {
     int64_t* d_scan = nullptr;     // the row-offset scan's tile totals
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    // the fill's rows: the staged ones (|n1 - n2| >= k, the matrix-core
-    // fill) first, then the others, each ascending
+    // the fill's rows in three ascending parts: the staged rows (the
+    // matrix-core fill) whose windows cannot overlap (|n1 - n2| >= 2k - 1)
+    // up to n_far, the other staged rows (k <= |n1 - n2| < 2k - 1) up to
+    // n_sep, then the rest
     std::vector<int32_t> rowlist;
-    int64_t n_sep = 0;
+    int64_t n_far = 0, n_sep = 0;
     int32_t* d_rowlist = nullptr;
     // a count pass enqueued without waiting (the build service's admission):
     // its prefix sums' scratch, timing events and the event behind the
@@ -214,18 +216,18 @@ static void set_free(bs2e_blockset* b)
     }
 }
 
-// The fill launches over the rows of the window: the staged rows and the
-// others of the window are each one contiguous part of the row list (both
-// parts ascend).
+// The fill launches over the rows of the window: the window's rows of each
+// part of the row list are one contiguous run of it (the parts ascend).
 static hipError_t set_fill_launch(bs2e_blockset* b, const BlockArgs* tab, hipStream_t st)
 {
     const Plan& p = b->rk->plan;
     const int32_t lo = (int32_t)b->win_lo, hi = (int32_t)(b->win_lo + b->win_n);
-    const auto l0 = b->rowlist.begin(), l1 = l0 + b->n_sep, l2 = b->rowlist.end();
-    const int64_t s0 = std::lower_bound(l0, l1, lo) - l0, s1 = std::lower_bound(l0, l1, hi) - l0;
-    const int64_t g0 = std::lower_bound(l1, l2, lo) - l0, g1 = std::lower_bound(l1, l2, hi) - l0;
-    return launch_block_fill(tab, b->d_row0, (int)b->n_sym, b->d_rowlist + s0, s1 - s0, b->d_rowlist + g0, g1 - g0,
-                             (int)p.k_s, (int)p.K, st);
+    const auto l0 = b->rowlist.begin(), lf = l0 + b->n_far, ls = l0 + b->n_sep, le = b->rowlist.end();
+    const int64_t f0 = std::lower_bound(l0, lf, lo) - l0, f1 = std::lower_bound(l0, lf, hi) - l0;
+    const int64_t s0 = std::lower_bound(lf, ls, lo) - l0, s1 = std::lower_bound(lf, ls, hi) - l0;
+    const int64_t g0 = std::lower_bound(ls, le, lo) - l0, g1 = std::lower_bound(ls, le, hi) - l0;
+    return launch_block_fill(tab, b->d_row0, (int)b->n_sym, b->d_rowlist + f0, f1 - f0, b->d_rowlist + s0, s1 - s0,
+                             b->d_rowlist + g0, g1 - g0, (int)p.k_s, (int)p.K, st);
 }
 
 // The outputs hold global rows [lo, lo + n).
@@ -347,20 +349,23 @@ static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* 
     if (N >= ((int64_t)1 << 31))
         return fail(ctx, BS2E_EINVAL, std::string(who) + ": a set holds fewer than 2^31 configurations");
     // the configurations as the kernels read them (n1 | n2 << 16, l1 | l2 << 16:
-    // channel_tables checked n < 2^15 and l < 64), and the row list: the
-    // staged rows (|n1 - n2| >= k), then the others, each ascending
+    // channel_tables checked n < 2^15 and l < 64), and the row list in its
+    // three parts
     std::vector<int32_t> pk((size_t)2 * N);
-    b->n_sep = 0;
+    b->n_far = b->n_sep = 0;
+    const int64_t kf = 2 * p.k_s - 1;
     for (int64_t g = 0; g < N; ++g) {
         const int64_t* c = configs + 5 * g;
         pk[2 * g] = (int32_t)(c[0] | (c[1] << 16));
         pk[2 * g + 1] = (int32_t)(c[2] | (c[3] << 16));
-        b->n_sep += std::llabs(c[0] - c[1]) >= p.k_s;
+        const int64_t gap = std::llabs(c[0] - c[1]);
+        b->n_far += gap >= kf;
+        b->n_sep += gap >= p.k_s;
     }
     b->rowlist.resize((size_t)N);
-    for (int64_t g = 0, a = 0, o = b->n_sep; g < N; ++g) {
-        const int32_t w = pk[2 * g], n1 = w & 0xffff, n2 = w >> 16;
-        b->rowlist[(size_t)(std::abs(n1 - n2) >= p.k_s ? a++ : o++)] = (int32_t)g;
+    for (int64_t g = 0, a = 0, m = b->n_far, o = b->n_sep; g < N; ++g) {
+        const int32_t w = pk[2 * g], gap = std::abs((w & 0xffff) - (w >> 16));
+        b->rowlist[(size_t)(gap >= kf ? a++ : gap >= p.k_s ? m++ : o++)] = (int32_t)g;
     }
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_rowlist, (size_t)N));

@@ -392,7 +392,10 @@ __host__ __device__ constexpr int sep_wave_lds(int K, int nt)
     return 2 * sep_kp(K) * 16 * nt + kSepChunk * 64 + kSepChunk * 2 * sep_kp(K) + 2 * 64;
 }
 
-template <int NT>
+// X: the row's two windows may overlap (k <= |n_a - n_b| < 2k - 1), which
+// takes the third block; rows further apart run without it (fewer registers,
+// more wavefronts per SIMD at orders >= 9).
+template <int NT, bool X>
 __device__ __forceinline__ void fill_row_staged(const CBlockArgs& A, int i, int n_a, int n_bb, int l_a, int l_b,
                                                 int ch_i, int lane, double* wl)
 {
@@ -451,7 +454,7 @@ __device__ __forceinline__ void fill_row_staged(const CBlockArgs& A, int i, int 
     // exchange block's rows (D1)
     const int dir_base = a_low ? 0 : merged ? d2lo - d1lo : lo_len;
     const int ex_base = a_low ? (merged ? d1lo - d2lo : lo_len) : 0;
-    const bool both = (d1lo > d2lo ? d1lo : d2lo) <= (d1hi < d2hi ? d1hi : d2hi);   // D1 and D2 overlap
+    const bool both = X && (d1lo > d2lo ? d1lo : d2lo) <= (d1hi < d2hi ? d1hi : d2hi);   // D1 and D2 overlap
     const int shift = d2lo - d1lo;   // the third block reads run 1 at m + shift and run 2 at n - shift
     const int64_t baseH = uniform64(A.ptrH[i]) - A.winH, baseS = uniform64(A.ptrS[i]) - A.winS;
     int64_t nH = 0, nS = 0;
@@ -913,8 +916,8 @@ __device__ __forceinline__ void fill_row_tiled(const CBlockArgs& A, int i, int n
 // so a count that disagreed with the fill could not write past them.
 // (the matrix-core launch with one tile per side is held to 128 vector
 // registers, four wavefronts per SIMD, as many as its LDS allows)
-template <bool SEP, int NT>
-__global__ __launch_bounds__(kCThreads, (SEP && NT == 1) || (!SEP && NT > 1) ? 4 : 1) void kc_block(const BlockArgs* __restrict__ tab,
+template <bool SEP, int NT, bool X = true>
+__global__ __launch_bounds__(kCThreads, (SEP && NT == 1) || (!SEP && NT > 1) ? 4 : (SEP && !X) ? 3 : 1) void kc_block(const BlockArgs* __restrict__ tab,
                                                                              const int64_t* __restrict__ row0,
                                                                              int n_blocks,
                                                                              const int32_t* __restrict__ rows,
@@ -949,7 +952,7 @@ __global__ __launch_bounds__(kCThreads, (SEP && NT == 1) || (!SEP && NT > 1) ? 4
     const int n_a = cn & 0xffff, n_bb = cn >> 16, l_a = cl & 0xffff, l_b = cl >> 16;
     const int ch_i = __builtin_amdgcn_readfirstlane(A.cfg_chan[i]);
     if constexpr (SEP)
-        fill_row_staged<NT>(A, i, n_a, n_bb, l_a, l_b, ch_i, lane, lds_fill + (size_t)wave * sep_wave_lds(lds_K, NT));
+        fill_row_staged<NT, X>(A, i, n_a, n_bb, l_a, l_b, ch_i, lane, lds_fill + (size_t)wave * sep_wave_lds(lds_K, NT));
     else if constexpr (NT > 1)
         fill_row_tiled<NT>(A, i, n_a, n_bb, l_a, l_b, ch_i, lane, lds_fill + (size_t)wave * tiled_wave_lds(NT));
     else
@@ -1186,8 +1189,9 @@ hipError_t launch_row_scan(const int64_t* cnt, int64_t* ptr, int64_t n_rows, int
 // Both launches on s, the walked rows first.  (Measured on the c3 set: the
 // same fill time as with the walked rows on a second stream beside the
 // matrix-core launch, or after it; the two kernels share the memory system.)
-hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, const int32_t* rows_sep,
-                             int64_t n_sep, const int32_t* rows_gen, int64_t n_gen, int ks, int K, hipStream_t s)
+hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_blocks, const int32_t* rows_far,
+                             int64_t n_far, const int32_t* rows_sep, int64_t n_sep, const int32_t* rows_gen,
+                             int64_t n_gen, int ks, int K, hipStream_t s)
 {
     const int nt = 2 * ks - 1 <= 16 ? 1 : 2;
     hipError_t e = hipSuccess;
@@ -1215,18 +1219,30 @@ hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_bl
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    if (n_sep > 0) {
-        const unsigned grid = (unsigned)((n_sep * kWave + kCThreads - 1) / kCThreads);
-        const void* fn = nt == 1 ? (const void*)kc_block<true, 1> : (const void*)kc_block<true, 2>;
+    // the staged rows: those whose windows cannot overlap (|n1 - n2| >=
+    // 2k - 1, rows_far) without the third block, then the others (rows_sep)
+    for (int part = 0; part < 2 && e == hipSuccess; ++part) {
+        const int64_t n = part == 0 ? n_far : n_sep;
+        if (n <= 0) continue;
+        const int32_t* rws = part == 0 ? rows_far : rows_sep;
+        const unsigned grid = (unsigned)((n * kWave + kCThreads - 1) / kCThreads);
+        const void* fn = part == 0 ? (nt == 1 ? (const void*)kc_block<true, 1, false> : (const void*)kc_block<true, 2, false>)
+                                   : (nt == 1 ? (const void*)kc_block<true, 1, true> : (const void*)kc_block<true, 2, true>);
         const size_t lds = sizeof(double) * (size_t)(kCThreads / kWave) * (size_t)sep_wave_lds(K, nt);
         e = allow_lds(fn, lds);   // <= 4 x 39 KiB (k <= 16, K <= 64)
         if (e != hipSuccess) return e;
-        if (nt == 1)
-            hipLaunchKernelGGL((kc_block<true, 1>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks,
-                               rows_sep, n_sep, ks, K);
+        if (part == 0 && nt == 1)
+            hipLaunchKernelGGL((kc_block<true, 1, false>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks,
+                               rws, n, ks, K);
+        else if (part == 0)
+            hipLaunchKernelGGL((kc_block<true, 2, false>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks,
+                               rws, n, ks, K);
+        else if (nt == 1)
+            hipLaunchKernelGGL((kc_block<true, 1, true>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks,
+                               rws, n, ks, K);
         else
-            hipLaunchKernelGGL((kc_block<true, 2>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks,
-                               rows_sep, n_sep, ks, K);
+            hipLaunchKernelGGL((kc_block<true, 2, true>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks,
+                               rws, n, ks, K);
         e = hipGetLastError();
     }
     return e;
