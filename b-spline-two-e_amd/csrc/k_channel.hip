@@ -701,6 +701,11 @@ __device__ __forceinline__ void fill_row_staged(const CBlockArgs& A, int i, int 
 // rounded, multipoles ascending), so the entries are bit for bit the same.
 // KM >= K multipoles are held per term; m <= 64 (k <= 16).
 constexpr int kTChunk = 4;   // column channels per chunk
+// ones for the second factor of a tensor entry read in a tile (the product
+// with 1.0 is the entry, so every lane takes the same loads and products)
+__device__ const double kOnes[64] = {
+    1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
+    1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
 __host__ __device__ constexpr int tiled_kp(int km) { return (km + 1) & ~1; }
 // LDS of one wavefront (8-byte words): per chunk channel the candidates'
 // runs [kTChunk][64] {H lo | H len << 16, H offset | S offset << 16,
@@ -735,6 +740,14 @@ __device__ __forceinline__ void fill_row_tiled(const CBlockArgs& A, int i, int n
     const bool c_in2 = cand && my_c >= d2lo && my_c <= d2hi, c_in1 = cand && my_c >= d1lo && my_c <= d1hi;
     const int4v* segc = (const int4v*)A.seg + (cand ? my_c : 0);   // BlockSeg {off, lo, hi, pad} of (channel, n_c)
     const int n_cells = m * m;
+    // first(n) - lo(n) of the row's two splines and of this lane's candidate
+    // (pair indices below); the tables' addresses in scalar registers
+    const int fa = __builtin_amdgcn_readfirstlane(gload(A.c_first + n_a)) - (n_a - ks + 1 < 1 ? 1 : n_a - ks + 1);
+    const int fb = __builtin_amdgcn_readfirstlane(gload(A.c_first + n_bb)) - (n_bb - ks + 1 < 1 ? 1 : n_bb - ks + 1);
+    const int f_my = cand ? gload(A.c_first + my_c) - (my_c - ks + 1 < 1 ? 1 : my_c - ks + 1) : 0;
+    const double* const tR = (const double*)uniform64((int64_t)A.R);
+    const double* const tP = (const double*)uniform64((int64_t)A.PtotT);
+    const double* const tQ = (const double*)uniform64((int64_t)A.QtotT);
     int64_t nH = 0, nS = 0;
     typedef const __attribute__((address_space(4))) BlockChan CChan;
     typedef const __attribute__((address_space(4))) int32_t CInt;
@@ -815,31 +828,45 @@ __device__ __forceinline__ void fill_row_tiled(const CBlockArgs& A, int i, int n
             // the cell's direct vector R^k[pair(n_a, n_c)][pair(n_d, n_b)] and
             // exchange vector R^k[pair(n_b, n_c)][pair(n_d, n_a)]
             double rd[KM], re[KM];
-            // v[k] = R^k[pair(a1, n_c)][pair(n_d, a2)], k < K (zero beyond)
-            auto load_vec = [&](double(&v)[KM], bool on, int a1, int a2) {
+            // v[k] = R^k[pair(a1, n_c)][pair(n_d, a2)], k < K.  Straight-line
+            // loads: every lane issues the same KM pairs (an index past K - 1
+            // reads entry K - 1 again, a lane that is off reads the tensor's
+            // first entries) and all of them are in flight before the first
+            // product.  A pair of overlapping windows reads its row of the
+            // tensor and ones (the product is the entry itself), a rank-one
+            // pair its two factor columns.  Past K - 1 the values are finite
+            // and meet coefficients of +0; an off lane's vector is not summed
+            // (its cell has no two-electron part in any channel of the chunk).
+            // pair(a1, n_c) = first(n_c) + a1 - lo(n_c), pair(n_d, a2) =
+            // first(a2) + n_d - lo(a2), lo(c) = max(c - k + 1, 1)
+            const int cc = live ? n_c : u0, dd = live ? n_d : u0;
+            const int fc = __shfl(f_my, cc - u0, kWave);
+            auto load_vec = [&](double(&v)[KM], bool on, int a1, int a2, int fq) {
+                const int p = fc + a1, q = fq + dd;
+                const bool left = window_hi(a1, cc, A.n_iv) < window_lo(dd, a2, ks);
+                const bool right = window_lo(a1, cc, ks) > window_hi(dd, a2, A.n_iv);
+                const bool nr = !left && !right;
+                const double* x = !on ? tR : nr ? tR + ((size_t)p * A.n_pairs + q) * K : (left ? tP : tQ) + p;
+                const double* y = !on || nr ? kOnes : (left ? tQ : tP) + q;
+                const unsigned st = !on || nr ? 1u : np;
+                double a[KM], b[KM];
 #pragma unroll
-                for (int k = 0; k < KM; ++k) v[k] = 0.0;
-                if (!on) return;
-                const int p = pair_of(A.c_first, ks, a1, n_c), q = pair_of(A.c_first, ks, n_d, a2);
-                const bool left = window_hi(a1, n_c, A.n_iv) < window_lo(n_d, a2, ks);
-                const bool right = window_lo(a1, n_c, ks) > window_hi(n_d, a2, A.n_iv);
-                if (!left && !right) {
-                    const double* r = A.R + ((size_t)p * A.n_pairs + q) * K;
-#pragma unroll
-                    for (int k = 0; k < KM; ++k)
-                        if (k < K) v[k] = gload(r + k);
-                } else {
-                    const double* x = (left ? A.PtotT : A.QtotT) + p;
-                    const double* y = (left ? A.QtotT : A.PtotT) + q;
-#pragma unroll
-                    for (int k = 0; k < KM; ++k)
-                        if (k < K) v[k] = gload(x + (unsigned)k * np) * gload(y + (unsigned)k * np);
+                for (int k = 0; k < KM; ++k) {
+                    const unsigned kk = (unsigned)(k < K ? k : K - 1) * st;
+                    a[k] = gload(x + kk);
+                    b[k] = gload(y + kk);
                 }
+#pragma unroll
+                for (int k = 0; k < KM; ++k) v[k] = a[k] * b[k];
             };
-            load_vec(rd, any_r12 && sd, n_a, n_bb);
-            load_vec(re, any_r12 && se, n_bb, n_a);
+            load_vec(rd, any_r12 && sd, n_a, n_bb, fb);
+            __builtin_amdgcn_sched_barrier(0);   // the direct vector's loads issued before the exchange vector's
+            load_vec(re, any_r12 && se, n_bb, n_a, fa);
             // the chunk's two-electron sums first (the vectors are dead after
-            // them), then the one-electron parts and the stores
+            // them), then the one-electron parts and the stores; the sums run
+            // over all KM terms without a branch (past K - 1 the coefficients
+            // are +0: the products are zeros, which leave a sum begun at +0
+            // as it is)
             double res[kTChunk];
 #pragma unroll
             for (int c = 0; c < kTChunk; ++c) {
@@ -854,15 +881,13 @@ __device__ __forceinline__ void fill_row_tiled(const CBlockArgs& A, int i, int n
                 if (sd) {
                     double t = 0.0;
 #pragma unroll
-                    for (int k = 0; k < KM; ++k)
-                        if (k < K) t = t + rd[k] * cd[k];
+                    for (int k = 0; k < KM; ++k) t = t + rd[k] * cd[k];
                     r = r + t;
                 }
                 if (se) {
                     double t = 0.0;
 #pragma unroll
-                    for (int k = 0; k < KM; ++k)
-                        if (k < K) t = t + re[k] * ce[k];
+                    for (int k = 0; k < KM; ++k) t = t + re[k] * ce[k];
                     r = r + t;
                 }
                 res[c] = r;
@@ -915,9 +940,11 @@ __device__ __forceinline__ void fill_row_tiled(const CBlockArgs& A, int i, int n
 // that window); stores beyond the outputs (limH, limS entries) are dropped,
 // so a count that disagreed with the fill could not write past them.
 // (the matrix-core launch with one tile per side is held to 128 vector
-// registers, four wavefronts per SIMD, as many as its LDS allows)
+// registers, four wavefronts per SIMD, as many as its LDS allows; so are the
+// tiles of at most 12 multipoles, while those of 18 hold 150 registers, three
+// wavefronts per SIMD: at 128 they spill and run 6 % longer)
 template <bool SEP, int NT, bool X = true>
-__global__ __launch_bounds__(kCThreads, (SEP && NT == 1) || (!SEP && NT > 1) ? 4 : (SEP && !X) ? 3 : 1) void kc_block(const BlockArgs* __restrict__ tab,
+__global__ __launch_bounds__(kCThreads, (SEP && NT == 1) || (!SEP && NT > 1 && NT <= 12) ? 4 : (SEP && !X) || (!SEP && NT > 12) ? 3 : 1) void kc_block(const BlockArgs* __restrict__ tab,
                                                                              const int64_t* __restrict__ row0,
                                                                              int n_blocks,
                                                                              const int32_t* __restrict__ rows,
