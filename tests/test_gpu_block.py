@@ -145,6 +145,36 @@ def test_block_separated_rows(ctx, oracle, k, k_gl, max_k, L, full):
     compare(H_sp, S_sp, ref)
 
 
+@pytest.mark.parametrize("k,max_k", [(5, 11), (6, 12), (8, 15), (7, 17)])
+def test_block_tiled_rows_multipole_counts(ctx, oracle, k, max_k):
+    """The rows with |n1 - n2| < k are filled in tiles that hold 12 or 18
+    multipoles per cell: K = max_k + 1 = 12 and 18 fill them exactly, 13 and
+    16 leave 5 and 2 slots past the last multipole (their loads repeat the
+    last one and meet zero coefficients).  Patterns bit-exact and values
+    against the oracle, both triangles and both parities."""
+    t = bs2e.generate_grid(k, 3, 2, 1.0, 14.0)
+    n_b = oracle.dims(k, t)[1]
+    max_l = 2
+    print(f"block tiled rows k={k} n_b={n_b} max_k={max_k}", flush=True)
+    rk = bs2e.RkTensor.build(k, t, k + 4, max_k, ctx=ctx)
+    Rref = oracle.rk_dense(k, n_b, max_k, oracle.slater_off_diag(k, t, k + 4, max_k),
+                           oracle.slater_diag(k, t, k + 4, max_k))
+    S = oracle.setup_S(k, t, k + 4)
+    Hl = [oracle.setup_H1p(k, t, k + 4, 2, l, cap_eta=0.0) for l in range(max_l + 1)]
+    for L, pi, full in ((1, True, True), (2, False, False), (1, False, True)):
+        term = bs2e.sym(l=L, pi=pi)
+        term.configs = bs2e.count_configs(term, max_l, n_b, k, n_b, n_b, max_l)
+        if term.n_config == 0:
+            continue
+        assert (np.abs(term.configs[:, 0] - term.configs[:, 1]) < k).any()
+        ref = oracle.construct_block(k, n_b, L, max_k, full, term.configs, Rref, Hl, S)
+        blk = bs2e.Block(rk, term, full)
+        H_sp, S_sp = blk.fill(Hl, S)
+        blk.release()
+        compare(H_sp, S_sp, ref)
+    rk.release()
+
+
 RANDOM_BASES = [   # seed, order, k_GL, max_l, L, full
     (1, 2, 5, 1, 0, True), (2, 3, 6, 2, 1, False), (3, 4, 7, 3, 2, True), (4, 5, 8, 2, 3, False),
     (5, 6, 9, 1, 1, True), (6, 9, 12, 2, 0, False), (7, 11, 14, 1, 2, True), (8, 12, 15, 2, 1, False),
