@@ -320,20 +320,21 @@ class Server:
         matrices (hydrogenic Z, CAP_c = (order, r_0, eta)) and the H/S blocks
         of `terms` (sym objects with their configs).  Returns the ticket."""
         t = np.ascontiguousarray(t, dtype=np.float64)
+        # each symmetry's configurations by pointer (the service copies them)
         cfs = [np.ascontiguousarray(tm.configs, dtype=np.int64) for tm in terms]
         L = np.array([int(tm.l) for tm in terms], np.int64)
         n_config = np.array([c.shape[0] for c in cfs], np.int64)
-        configs = np.ascontiguousarray(np.concatenate(cfs, axis=0)) if cfs else np.zeros((0, 5), np.int64)
         if max_l_1p is None:
-            max_l_1p = int(configs[:, 2:4].max()) if len(configs) else 0
+            max_l_1p = max((int(c[:, 2:4].max()) for c in cfs if len(c)), default=0)
         order, r0, eta = CAP_c
         eta = complex(eta)
         I = C.c_int64
+        per_sym = (C.POINTER(I) * max(len(cfs), 1))(*[ptr(c, I) for c in cfs])
         job = capi.Job(k=k, n_knots=len(t), knots=ptr(t), k_gl=k_GL, max_k=max_k,
                        precision={"fp64": capi.PREC_FP64, "mixed": capi.PREC_MIXED}[precision],
-                       n_sym=len(cfs), L=ptr(L, I), n_config=ptr(n_config, I), configs=ptr(configs, I),
+                       n_sym=len(cfs), L=ptr(L, I), n_config=ptr(n_config, I), configs=None,
                        full=int(bool(full)), Z=int(Z), max_l_1p=int(max_l_1p), cap_order=int(order), cap_r0=float(r0),
-                       cap_eta_re=eta.real, cap_eta_im=eta.imag)
+                       cap_eta_re=eta.real, cap_eta_im=eta.imag, sym_configs=per_sym)
         return self._check(capi.lib().bs2e_server_submit_job(self.handle, C.byref(job)))
 
     def blocks(self, ticket, terms):
@@ -367,8 +368,13 @@ class Server:
         self._check(capi.lib().bs2e_server_tensor(self.handle, ticket, C.byref(h)))
         return RkTensor(h, _ThreadErrors(), owned=False)
 
+    def last_error(self) -> str:
+        msg = capi.lib().bs2e_server_last_error(self.handle)
+        return msg.decode() if msg else ""
+
     def finished(self, cap=64):
-        """Progresses the service; [(ticket, latency ms)] of the jobs done since the previous call."""
+        """Progresses the service; [(ticket, latency ms)] of the jobs done since the previous call (latency -1: the
+        job failed, last_error() says why)."""
         tk = np.zeros(cap, np.int64)
         lat = np.zeros(cap)
         n = self._check(capi.lib().bs2e_server_finished(self.handle, ptr(tk, C.c_int64), ptr(lat), cap))

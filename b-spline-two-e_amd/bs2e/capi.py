@@ -120,7 +120,7 @@ class Job(C.Structure):
     _fields_ = [("k", i64), ("n_knots", i64), ("knots", P(dbl)), ("k_gl", i64), ("max_k", i64),
                 ("precision", i64), ("n_sym", i64), ("L", P(i64)), ("n_config", P(i64)), ("configs", P(i64)),
                 ("full", i64), ("Z", i64), ("max_l_1p", i64), ("cap_order", i64), ("cap_r0", dbl),
-                ("cap_eta_re", dbl), ("cap_eta_im", dbl)]
+                ("cap_eta_re", dbl), ("cap_eta_im", dbl), ("sym_configs", P(P(i64)))]
 
 
 _lib = None

@@ -115,7 +115,7 @@ struct bs2e_blockset {
     std::vector<int64_t> chan_off, seg_off, coef_off, flag_off, n_chan;
     std::vector<BlockChan> chan;
     std::vector<BlockSeg> seg;
-    std::vector<int32_t> cfg_chan, flags;
+    std::vector<int32_t> flags;
     std::vector<double> coef;
     std::vector<int64_t> nnzH, nnzS, refH, refS;   // per symmetry
     std::vector<int64_t> offH, offS;               // first entry of each symmetry in the concatenated output
@@ -293,80 +293,113 @@ static BlockArgs set_args(const bs2e_blockset* b, int64_t s, bool fill)
 
 // Host tables, device upload and the count launch of a set whose rk, n_sym,
 // full, L and row0 are given.  Returns BS2E_OK or the error code (set on ctx).
-static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* who, bool async)
+void bs2e::set_host_tables(int64_t nb, int64_t K, int64_t k_s, int64_t n_sym, const int64_t* L, const int64_t* row0,
+                           const int64_t* configs, SetHost& h)
 {
-    bs2e_ctx* ctx = b->rk->ctx;
-    const Plan& p = b->rk->plan;
-    const int64_t nb = p.n_b, K = p.K, N = b->n_rows;
-    b->chan_off.assign(b->n_sym, 0);
-    b->seg_off.assign(b->n_sym, 0);
-    b->coef_off.assign(b->n_sym, 0);
-    b->flag_off.assign(b->n_sym, 0);
-    b->n_chan.assign(b->n_sym, 0);
-    b->cfg_chan.assign((size_t)N, 0);
-    for (int64_t s = 0; s < b->n_sym; ++s) {
+    const int64_t N = row0[n_sym];
+    h = SetHost();
+    if (N >= ((int64_t)1 << 31)) {
+        h.err = "a set holds fewer than 2^31 configurations";
+        return;
+    }
+    h.chan_off.assign(n_sym, 0);
+    h.seg_off.assign(n_sym, 0);
+    h.coef_off.assign(n_sym, 0);
+    h.flag_off.assign(n_sym, 0);
+    h.n_chan.assign(n_sym, 0);
+    h.cfg_chan.assign((size_t)N, 0);
+    for (int64_t s = 0; s < n_sym; ++s) {
         std::vector<BlockChan> ch;
         std::vector<BlockSeg> sg;
         std::vector<int32_t> cc;
         int64_t ml = 0;
-        const int64_t r0 = b->row0[s], n = b->row0[s + 1] - r0;
+        const int64_t r0 = row0[s], n = row0[s + 1] - r0;
         std::string bad = channel_tables(configs + 5 * r0, n, nb, ch, sg, cc, ml);
-        if (!bad.empty())
-            return fail(ctx, BS2E_EINVAL, std::string(who) + ": symmetry " + std::to_string(s + 1) + ": " + bad +
-                                              " (count_configs order required)");
-        b->max_l = std::max(b->max_l, ml);
-        b->chan_off[s] = (int64_t)b->chan.size();
-        b->seg_off[s] = (int64_t)b->seg.size();
-        b->coef_off[s] = (int64_t)b->coef.size();
-        b->flag_off[s] = (int64_t)b->flags.size();
-        b->n_chan[s] = (int64_t)ch.size();
-        std::copy(cc.begin(), cc.end(), b->cfg_chan.begin() + r0);
-        b->chan.insert(b->chan.end(), ch.begin(), ch.end());
-        b->seg.insert(b->seg.end(), sg.begin(), sg.end());
+        if (!bad.empty()) {
+            h.err = "symmetry " + std::to_string(s + 1) + ": " + bad + " (count_configs order required)";
+            return;
+        }
+        h.max_l = std::max(h.max_l, ml);
+        h.chan_off[s] = (int64_t)h.chan.size();
+        h.seg_off[s] = (int64_t)h.seg.size();
+        h.coef_off[s] = (int64_t)h.coef.size();
+        h.flag_off[s] = (int64_t)h.flags.size();
+        h.n_chan[s] = (int64_t)ch.size();
+        std::copy(cc.begin(), cc.end(), h.cfg_chan.begin() + r0);
+        h.chan.insert(h.chan.end(), ch.begin(), ch.end());
+        h.seg.insert(h.seg.end(), sg.begin(), sg.end());
         // angular coefficients per (row channel, column channel): wigner_tools.f90:126-138
-        const int64_t nch = (int64_t)ch.size(), L = b->L[s];
-        const size_t c0 = b->coef.size(), f0 = b->flags.size();
-        b->coef.resize(c0 + (size_t)(nch * nch * 2 * K), 0.0);
-        b->flags.resize(f0 + (size_t)(nch * nch), 0);
+        const int64_t nch = (int64_t)ch.size(), Ls = L[s];
+        const size_t c0 = h.coef.size(), f0 = h.flags.size();
+        h.coef.resize(c0 + (size_t)(nch * nch * 2 * K), 0.0);
+        h.flags.resize(f0 + (size_t)(nch * nch), 0);
         for (int64_t ci = 0; ci < nch; ++ci)
             for (int64_t cj = 0; cj < nch; ++cj) {
                 const int64_t la = ch[ci].l1, lb = ch[ci].l2, lc = ch[cj].l1, ld = ch[cj].l2;
-                const double sgn = ((lc + ld + L) & 1) ? -1.0 : 1.0;
-                double* cd = b->coef.data() + c0 + (ci * nch + cj) * 2 * K;
+                const double sgn = ((lc + ld + Ls) & 1) ? -1.0 : 1.0;
+                double* cd = h.coef.data() + c0 + (ci * nch + cj) * 2 * K;
                 double* ce = cd + K;
                 int32_t fl = 0;
                 for (int64_t k = 0; k < K; ++k) {
-                    const double ang = ang_k_LS(k, la, lb, lc, ld, L);
-                    const double ang_ex = ang_k_LS(k, la, lb, ld, lc, L);
+                    const double ang = ang_k_LS(k, la, lb, lc, ld, Ls);
+                    const double ang_ex = ang_k_LS(k, la, lb, ld, lc, Ls);
                     if (std::fabs(ang) >= 5e-16) cd[k] = ang;              // r_12_tens, mat_els.f90:566-570
                     if (std::fabs(ang_ex) >= 5e-16) ce[k] = sgn * ang_ex;
                     if (std::fabs(ang) > 5e-15) fl |= 1;                    // hamiltonian.f90:171-178
                     if (std::fabs(ang_ex) > 5e-15) fl |= 2;
                 }
-                b->flags[f0 + ci * nch + cj] = fl;
+                h.flags[f0 + ci * nch + cj] = fl;
             }
     }
-    if (N >= ((int64_t)1 << 31))
-        return fail(ctx, BS2E_EINVAL, std::string(who) + ": a set holds fewer than 2^31 configurations");
     // the configurations as the kernels read them (n1 | n2 << 16, l1 | l2 << 16:
     // channel_tables checked n < 2^15 and l < 64), and the row list in its
     // three parts
-    std::vector<int32_t> pk((size_t)2 * N);
-    b->n_far = b->n_sep = 0;
-    const int64_t kf = 2 * p.k_s - 1;
+    h.pk.resize((size_t)2 * N);
+    const int64_t kf = 2 * k_s - 1;
     for (int64_t g = 0; g < N; ++g) {
         const int64_t* c = configs + 5 * g;
-        pk[2 * g] = (int32_t)(c[0] | (c[1] << 16));
-        pk[2 * g + 1] = (int32_t)(c[2] | (c[3] << 16));
+        h.pk[2 * g] = (int32_t)(c[0] | (c[1] << 16));
+        h.pk[2 * g + 1] = (int32_t)(c[2] | (c[3] << 16));
         const int64_t gap = std::llabs(c[0] - c[1]);
-        b->n_far += gap >= kf;
-        b->n_sep += gap >= p.k_s;
+        h.n_far += gap >= kf;
+        h.n_sep += gap >= k_s;
     }
-    b->rowlist.resize((size_t)N);
-    for (int64_t g = 0, a = 0, m = b->n_far, o = b->n_sep; g < N; ++g) {
-        const int32_t w = pk[2 * g], gap = std::abs((w & 0xffff) - (w >> 16));
-        b->rowlist[(size_t)(gap >= kf ? a++ : gap >= p.k_s ? m++ : o++)] = (int32_t)g;
+    h.rowlist.resize((size_t)N);
+    for (int64_t g = 0, a = 0, m = h.n_far, o = h.n_sep; g < N; ++g) {
+        const int32_t w = h.pk[2 * g], gap = std::abs((w & 0xffff) - (w >> 16));
+        h.rowlist[(size_t)(gap >= kf ? a++ : gap >= k_s ? m++ : o++)] = (int32_t)g;
     }
+}
+
+static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* who, bool async,
+                          const SetHost* prepared = nullptr)
+{
+    bs2e_ctx* ctx = b->rk->ctx;
+    const Plan& p = b->rk->plan;
+    const int64_t N = b->n_rows;
+    // prepared tables are copied where the set keeps them and read in place
+    // for the uploads, so that they serve again if this set is released
+    // before its fill (the build service admits such a job anew)
+    SetHost own;
+    if (!prepared) set_host_tables(p.n_b, p.K, p.k_s, b->n_sym, b->L.data(), b->row0.data(), configs, own);
+    const SetHost& h = prepared ? *prepared : own;
+    if (!h.err.empty()) return fail(ctx, BS2E_EINVAL, std::string(who) + ": " + h.err);
+    b->max_l = h.max_l;
+    b->n_far = h.n_far;
+    b->n_sep = h.n_sep;
+    if (prepared) {
+        b->chan_off = h.chan_off, b->seg_off = h.seg_off, b->coef_off = h.coef_off, b->flag_off = h.flag_off;
+        b->n_chan = h.n_chan, b->chan = h.chan, b->seg = h.seg, b->coef = h.coef, b->flags = h.flags;
+        b->rowlist = h.rowlist;
+    } else {
+        b->chan_off = std::move(own.chan_off), b->seg_off = std::move(own.seg_off);
+        b->coef_off = std::move(own.coef_off), b->flag_off = std::move(own.flag_off);
+        b->n_chan = std::move(own.n_chan), b->chan = std::move(own.chan), b->seg = std::move(own.seg);
+        b->coef = std::move(own.coef), b->flags = std::move(own.flags);
+        b->rowlist = std::move(own.rowlist);
+    }
+    const std::vector<int32_t>& pk = h.pk;
+    const std::vector<int32_t>& cfg_chan = h.cfg_chan;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_rowlist, (size_t)N));
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_cfg, (size_t)2 * N));
@@ -380,9 +413,28 @@ static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* 
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_tab, (size_t)b->n_sym));
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_row0, (size_t)b->n_sym + 1));
     hipStream_t st = ctx->stream;
-    HIP_TRY(ctx, hipMemcpyAsync(b->d_cfg, pk.data(), sizeof(int32_t) * 2 * N, hipMemcpyHostToDevice, st));
-    HIP_TRY(ctx, hipMemcpyAsync(b->d_cfg_chan, b->cfg_chan.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
-    HIP_TRY(ctx, hipMemcpyAsync(b->d_rowlist, b->rowlist.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
+    // The context's pinned staging carries the per-row tables up (a copy from
+    // pageable memory can hold the host until the stream reaches it) and the
+    // counts back down; stream order keeps the two uses apart.  It grows by
+    // doubling (pinning memory takes milliseconds).
+    std::unique_lock<std::mutex> pin_lock(ctx->pin_mu, std::defer_lock);
+    if (!async) pin_lock.lock();
+    const size_t need = sizeof(int64_t) * 4 * (size_t)N;   // >= the 16 N bytes of the uploads
+    if (ctx->pin_bytes < need) {
+        const size_t grow = std::max(need, std::max<size_t>(2 * ctx->pin_bytes, (size_t)16 << 20));
+        if (ctx->pin) (void)hipHostFree(ctx->pin);
+        ctx->pin = nullptr;
+        ctx->pin_bytes = 0;
+        HIP_TRY(ctx, hipHostMalloc(&ctx->pin, grow, hipHostMallocDefault));
+        ctx->pin_bytes = grow;
+    }
+    int32_t* stage = (int32_t*)ctx->pin;
+    std::memcpy(stage, pk.data(), sizeof(int32_t) * 2 * N);
+    std::memcpy(stage + 2 * N, cfg_chan.data(), sizeof(int32_t) * N);
+    std::memcpy(stage + 3 * N, b->rowlist.data(), sizeof(int32_t) * N);
+    HIP_TRY(ctx, hipMemcpyAsync(b->d_cfg, stage, sizeof(int32_t) * 2 * N, hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(b->d_cfg_chan, stage + 2 * N, sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(b->d_rowlist, stage + 3 * N, sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
     HIP_TRY(ctx, hipMemcpyAsync(b->d_chan, b->chan.data(), sizeof(BlockChan) * b->chan.size(), hipMemcpyHostToDevice,
                                 st));
     HIP_TRY(ctx, hipMemcpyAsync(b->d_seg, b->seg.data(), sizeof(BlockSeg) * b->seg.size(), hipMemcpyHostToDevice, st));
@@ -404,16 +456,6 @@ static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* 
     // async: enqueued behind an event and left running (the build service
     // owns its contexts and reads the counts with set_count_finish).
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_count_tmp, (size_t)row_scan_tmp_elems(N)));
-    std::unique_lock<std::mutex> pin_lock(ctx->pin_mu, std::defer_lock);
-    if (!async) pin_lock.lock();
-    const size_t need = sizeof(int64_t) * 4 * (size_t)N;
-    if (ctx->pin_bytes < need) {
-        if (ctx->pin) (void)hipHostFree(ctx->pin);
-        ctx->pin = nullptr;
-        ctx->pin_bytes = 0;
-        HIP_TRY(ctx, hipHostMalloc(&ctx->pin, need, hipHostMallocDefault));
-        ctx->pin_bytes = need;
-    }
     for (hipEvent_t& e : b->count_ev)
         if (!e) HIP_TRY(ctx, hipEventCreate(&e));
     hipError_t le = hipEventRecord(b->count_ev[0], st);
@@ -727,9 +769,10 @@ int64_t bs2e_blockset_create(bs2e_rk* rk, int64_t n_sym, const int64_t* L, const
 // before its tensor is built (the fill, enqueued after the build on the same
 // stream, reads the tensor).
 int64_t bs2e::blockset_new(bs2e_rk* rk, int64_t n_sym, const int64_t* L, const int64_t* n_config,
-                           const int64_t* configs, int64_t full, bs2e_blockset** out, bool async)
+                           const int64_t* configs, int64_t full, bs2e_blockset** out, bool async,
+                           const SetHost* prepared)
 {
-    if (!rk || !out || !L || !n_config || !configs)
+    if (!rk || !out || !L || !n_config || (!configs && !prepared))
         return fail(rk ? rk->ctx : nullptr, BS2E_EINVAL, "bs2e_blockset_create: NULL argument");
     bs2e_ctx* ctx = rk->ctx;
     *out = nullptr;
@@ -750,7 +793,7 @@ int64_t bs2e::blockset_new(bs2e_rk* rk, int64_t n_sym, const int64_t* L, const i
     b->row0.assign((size_t)n_sym + 1, 0);
     for (int64_t s = 0; s < n_sym; ++s) b->row0[s + 1] = b->row0[s] + n_config[s];
     b->n_rows = total;
-    const int64_t rc = set_create(b, configs, "bs2e_blockset_create", async);
+    const int64_t rc = set_create(b, configs, "bs2e_blockset_create", async, prepared);
     if (rc) {
         set_free(b);
         delete b;

@@ -44,8 +44,9 @@ struct bs2e_ctx {
     hipStream_t stream = nullptr;   // the build's stream (own, or the caller's)
     hipStream_t side = nullptr;     // diagonal cells + near ranges, concurrent with the rank-one fill
     std::string err;
-    // pinned host staging for device-to-host copies of per-row tables (grown
-    // on demand, freed with the context; pin_mu held while it is in use)
+    // pinned host staging for per-row tables: a block set's tables on their
+    // way up and its counts on their way back (grown by doubling, freed with
+    // the context; pin_mu held while it is in use)
     void* pin = nullptr;
     size_t pin_bytes = 0;
     std::mutex pin_mu;
@@ -148,13 +149,35 @@ int64_t rk_enqueue_build(bs2e_ctx* ctx, bs2e_rk* rk);
 // exceeded); needs_primitives: the plan's primitives are computed on the device.
 int64_t check_plan_limits(bs2e_ctx* ctx, const Plan& p, bool needs_primitives);
 
+// The host tables of a block set, built without the device: per symmetry the
+// channel runs and the angular coefficients and flags of every pair of
+// channels, the configurations packed as the kernels read them and the row
+// list in its three parts.  The build service builds them on a worker thread
+// while the job waits for a slot.
+struct SetHost {
+    std::string err;   // empty, or why the configurations were refused
+    int64_t max_l = 0, n_far = 0, n_sep = 0;
+    std::vector<int64_t> chan_off, seg_off, coef_off, flag_off, n_chan;
+    std::vector<BlockChan> chan;
+    std::vector<BlockSeg> seg;
+    std::vector<int32_t> cfg_chan, flags, pk, rowlist;
+    std::vector<double> coef;
+};
+// Fills h (or h.err) for the n_sym symmetries whose configurations are the
+// rows [row0[s], row0[s + 1]) of configs, against a basis of n_b splines of
+// order k_s and K multipoles.
+void set_host_tables(int64_t n_b, int64_t K, int64_t k_s, int64_t n_sym, const int64_t* L, const int64_t* row0,
+                     const int64_t* configs, SetHost& h);
+
 // Blocks of several symmetries against a planned tensor (capi_block.hip):
 // bs2e_blockset_create without the computed-tensor check (the count pass
 // needs only the plan), and the fill from one-electron matrices already on the
 // device (column-major, n_l of H then S), enqueued on the tensor context's
-// stream (wait: block until done).
+// stream (wait: block until done).  prepared: the set's host tables, built
+// by set_host_tables from the same configurations (read, not changed; configs
+// is then not read).
 int64_t blockset_new(bs2e_rk* rk, int64_t n_sym, const int64_t* L, const int64_t* n_config, const int64_t* configs,
-                     int64_t full, bs2e_blockset** out, bool async = false);
+                     int64_t full, bs2e_blockset** out, bool async = false, const SetHost* prepared = nullptr);
 // With async, blockset_new leaves the count pass running; set_count_finish
 // waits for it and takes the sizes (set_counted_event: recorded behind it).
 int64_t set_count_finish(bs2e_blockset* b);

@@ -5,10 +5,15 @@
 // which builds one basis per process (main_basis_setup.f90:75-86).
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <deque>
+#include <functional>
 #include <list>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <new>
+#include <thread>
 
 #include "capi_internal.h"
 
@@ -50,6 +55,90 @@ size_t onebody_bytes(const Plan& p, int64_t n_l)
            arena_round(16 * nn * (size_t)n_l);
 }
 
+// A job's block-set host tables, built on a worker thread from the job's
+// configurations (copied at submission) while the job waits for a slot.
+struct Prep {
+    std::vector<int64_t> configs;   // freed once the tables are built
+    std::vector<int64_t> L, row0;
+    int64_t n_b = 0, K = 0, k_s = 0;
+    SetHost h;
+    std::mutex mu;
+    std::condition_variable cv;
+    bool done = false;
+
+    void build()
+    {
+        try {
+            set_host_tables(n_b, K, k_s, (int64_t)L.size(), L.data(), row0.data(), configs.data(), h);
+        } catch (...) {   // (std::bad_alloc)
+            h = SetHost();
+            h.err = "host allocation failed";
+        }
+        std::vector<int64_t>().swap(configs);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            done = true;
+        }
+        cv.notify_all();
+    }
+    void wait()
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [this] { return done; });
+    }
+};
+
+// Host threads that build the jobs' tables, started with the first job; a
+// task they cannot take runs on the caller's thread.
+class Workers {
+public:
+    void post(std::function<void()> f)
+    {
+        try {
+            std::lock_guard<std::mutex> lk(mu_);
+            if (th_.empty()) {
+                const unsigned n = std::max(1u, std::min(4u, std::thread::hardware_concurrency()));
+                for (unsigned i = 0; i < n; ++i) th_.emplace_back([this] { run(); });
+            }
+            q_.push_back(std::move(f));
+        } catch (...) {   // no thread or no queue entry: run it here
+            if (f) f();
+            return;
+        }
+        cv_.notify_one();
+    }
+    ~Workers()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (std::thread& t : th_) t.join();
+    }
+
+private:
+    void run()
+    {
+        for (;;) {
+            std::function<void()> f;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
+                if (q_.empty()) return;   // stopping, nothing left
+                f = std::move(q_.front());
+                q_.pop_front();
+            }
+            f();
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<std::function<void()>> q_;
+    std::vector<std::thread> th_;
+    bool stop_ = false;
+};
+
 struct Job {
     Plan plan;                     // planned at submission, copied into the tensor at admission
     int64_t precision = BS2E_PREC_FP64;
@@ -64,7 +153,8 @@ struct Job {
     // the symmetry blocks of the job (n_sym = 0: the tensor only)
     int64_t n_sym = 0, full = 1, Z = 0, max_l_1p = 0, cap_order = 0;
     double cap_r0 = 0, eta_re = 0, eta_im = 0;
-    std::vector<int64_t> L, n_config, configs;
+    std::vector<int64_t> L, n_config;
+    std::shared_ptr<Prep> prep;    // the blocks' host tables (being) built
     bs2e_blockset* set = nullptr;
     OneBody* ob = nullptr;
 };
@@ -84,6 +174,7 @@ struct bs2e_server {
     uint64_t tick = 0;
     int64_t next = 0;
     bs2e_ctx* err = nullptr;         // error messages of the service
+    Workers workers;                 // (last: its threads end first)
 };
 
 static void onebody_free(bs2e_server* s, OneBody& o)
@@ -203,7 +294,8 @@ static int64_t admit(bs2e_server* s, Job& j, int slot)
         return rc;
     }
     if (j.n_sym > 0) {
-        rc = blockset_new(j.rk, j.n_sym, j.L.data(), j.n_config.data(), j.configs.data(), j.full, &j.set, true);
+        j.prep->wait();
+        rc = blockset_new(j.rk, j.n_sym, j.L.data(), j.n_config.data(), nullptr, j.full, &j.set, true, &j.prep->h);
         if (rc) {
             job_release(j);
             return rc;
@@ -297,7 +389,10 @@ static int64_t progress(bs2e_server* s)
                 } else {
                     if (rc == BS2E_ENOMEM)
                         fail(s->err, BS2E_ENOMEM, "bs2e_server: the job's blocks and tensor exceed the service's budget");
+                    else
+                        fail(s->err, rc, "bs2e_server: job " + std::to_string(t) + ": " + bs2e_last_error(s->slots[i]));
                     job_fail(s, j, rc);
+                    s->finished.push_back(t);   // handed out as failed
                 }
                 continue;
             }
@@ -333,8 +428,11 @@ static int64_t progress(bs2e_server* s)
         }
         if (rc == BS2E_ENOMEM)
             fail(s->err, BS2E_ENOMEM, "bs2e_server: the job's blocks and tensor exceed the service's budget");
+        else if (rc)
+            fail(s->err, rc, "bs2e_server: job " + std::to_string(*it) + ": " + bs2e_last_error(s->slots[slot]));
         if (rc) {
             job_fail(s, j, rc);
+            s->finished.push_back(*it);   // handed out as failed
             it = s->queue.erase(it);
             continue;
         }
@@ -440,26 +538,50 @@ int64_t bs2e_server_submit_job(bs2e_server* s, const bs2e_job* job)
     j.precision = job->precision;
     if (job->n_sym < 0) return fail(s->err, BS2E_EINVAL, "bs2e_server_submit_job: n_sym < 0");
     if (job->n_sym > 0) {
-        if (!job->L || !job->n_config || !job->configs)
+        // (sym_configs is read only without configs: callers built against
+        // the structure without that member always pass configs)
+        const int64_t* const* per_sym = job->configs ? nullptr : job->sym_configs;
+        if (!job->L || !job->n_config || (!job->configs && !per_sym))
             return fail(s->err, BS2E_EINVAL, "bs2e_server_submit_job: NULL symmetry arrays");
         if (job->max_l_1p < 0 || job->cap_order < 0 || job->cap_order > 64)
             return fail(s->err, BS2E_EINVAL, "bs2e_server_submit_job: need max_l_1p >= 0, 0 <= cap_order <= 64");
         int64_t rows = 0;
         for (int64_t i = 0; i < job->n_sym; ++i) {
-            if (job->L[i] < 0 || job->n_config[i] < 1)
+            if (job->L[i] < 0 || job->n_config[i] < 1 || (per_sym && !per_sym[i]))
                 return fail(s->err, BS2E_EINVAL, "bs2e_server_submit_job: need L >= 0, n_config >= 1");
             rows += job->n_config[i];
         }
-        for (int64_t r = 0; r < rows; ++r)
-            if (job->configs[5 * r + 2] > job->max_l_1p || job->configs[5 * r + 3] > job->max_l_1p)
-                return fail(s->err, BS2E_EINVAL, "bs2e_server_submit_job: a configuration's l exceeds max_l_1p");
+        std::shared_ptr<Prep> pr;
+        try {
+            pr = std::make_shared<Prep>();
+            pr->configs.resize((size_t)(5 * rows));
+            pr->L.assign(job->L, job->L + job->n_sym);
+            pr->row0.assign((size_t)job->n_sym + 1, 0);
+        } catch (...) {
+            return fail(s->err, BS2E_ENOMEM, "bs2e_server_submit_job: host allocation failed");
+        }
+        for (int64_t i = 0; i < job->n_sym; ++i) pr->row0[i + 1] = pr->row0[i] + job->n_config[i];
+        // the copy, symmetry by symmetry, with the l bound checked on the way
+        int64_t* dst = pr->configs.data();
+        for (int64_t i = 0; i < job->n_sym; ++i) {
+            const int64_t n5 = 5 * job->n_config[i];
+            const int64_t* src = per_sym ? per_sym[i] : job->configs + 5 * pr->row0[i];
+            for (int64_t e = 0; e < n5; e += 5) {
+                if (src[e + 2] > job->max_l_1p || src[e + 3] > job->max_l_1p)
+                    return fail(s->err, BS2E_EINVAL, "bs2e_server_submit_job: a configuration's l exceeds max_l_1p");
+                dst[e] = src[e], dst[e + 1] = src[e + 1], dst[e + 2] = src[e + 2], dst[e + 3] = src[e + 3];
+                dst[e + 4] = src[e + 4];
+            }
+            dst += n5;
+        }
+        pr->n_b = j.plan.n_b, pr->K = j.plan.K, pr->k_s = j.plan.k_s;
         j.n_sym = job->n_sym;
         j.full = job->full ? 1 : 0;
         j.Z = job->Z, j.max_l_1p = job->max_l_1p, j.cap_order = job->cap_order;
         j.cap_r0 = job->cap_r0, j.eta_re = job->cap_eta_re, j.eta_im = job->cap_eta_im;
         j.L.assign(job->L, job->L + job->n_sym);
         j.n_config.assign(job->n_config, job->n_config + job->n_sym);
-        j.configs.assign(job->configs, job->configs + 5 * rows);
+        j.prep = std::move(pr);
     }
     build_near_rows(j.plan);
     j.bytes = rk_footprint(j.plan);
@@ -469,6 +591,10 @@ int64_t bs2e_server_submit_job(bs2e_server* s, const bs2e_job* job)
     if (hipEventCreateWithFlags(&j.done, hipEventDisableTiming) != hipSuccess)
         return fail(s->err, BS2E_EDEVICE, "bs2e_server_submit_job: hipEventCreate failed");
     const int64_t t = s->next++;
+    if (j.prep) {   // the blocks' host tables built on a worker thread meanwhile
+        std::shared_ptr<Prep> pr = j.prep;
+        s->workers.post([pr] { pr->build(); });
+    }
     s->jobs[t] = std::move(j);
     s->queue.push_back(t);
     const int64_t rc = progress(s);
@@ -591,7 +717,7 @@ int64_t bs2e_server_finished(bs2e_server* s, int64_t* tickets, double* latency_m
         auto it = s->jobs.find(t);
         if (it == s->jobs.end()) continue;   // retired already
         tickets[n] = t;
-        if (latency_ms) latency_ms[n] = it->second.t_done - it->second.t_submit;
+        if (latency_ms) latency_ms[n] = it->second.state == 3 ? -1.0 : it->second.t_done - it->second.t_submit;
         ++n;
     }
     return n;

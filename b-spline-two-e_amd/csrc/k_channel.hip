@@ -1225,8 +1225,8 @@ hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_bl
     if (n_gen > 0) {
         const unsigned grid = (unsigned)((n_gen * kWave + kCThreads - 1) / kCThreads);
         // tiles of cells where the multipoles fit the registers, else the walk
-        // (KM = 18 holds 111 vector registers, four wavefronts per SIMD; more
-        // multipoles would spill)
+        // (KM = 18 holds 150 vector registers, three wavefronts per SIMD;
+        // more multipoles would spill)
         const int km = 3 * ks - 2 > kWave ? 0 : K <= 12 ? 12 : K <= 18 ? 18 : 0;
         const void* fn = km == 12 ? (const void*)kc_block<false, 12>
                          : km == 18 ? (const void*)kc_block<false, 18> : (const void*)kc_block<false, 1>;

@@ -318,6 +318,8 @@ def run_stream(srv, jobs, arrivals, submit):
             nxt += 1
         for tk, ms in srv.finished():
             i = tickets.pop(tk)
+            if ms < 0:
+                raise RuntimeError(f"serving job {i} failed: {srv.last_error()}")
             lat[i] = ms
             srv.retire(tk)
     return lat, time.perf_counter() - t0

@@ -392,7 +392,9 @@ int64_t bs2e_server_submit(bs2e_server *srv, int64_t k, int64_t n_knots, const d
  * absorbing potential -i eta (r - cap_r0)^cap_order) and the H/S CSR blocks
  * of n_sym symmetries (hamiltonian.f90:106-283), as bs2e_blockset_create /
  * _fill describe them.  n_sym = 0: the tensor only.  The arrays are copied at
- * submission. */
+ * submission; the blocks' host tables (channel runs, angular coefficients,
+ * row lists) are then built on the service's worker threads while the job
+ * waits for a slot. */
 typedef struct bs2e_job {
     int64_t k;                  /* B-spline order */
     int64_t n_knots;
@@ -409,6 +411,10 @@ typedef struct bs2e_job {
     int64_t max_l_1p;           /* H(l) for l = 0 .. max_l_1p; every configuration's l within it */
     int64_t cap_order;
     double cap_r0, cap_eta_re, cap_eta_im;
+    /* With configs NULL: [n_sym] pointers, symmetry s's n_config[s]
+     * configurations each (no concatenated copy on the caller's side);
+     * not read when configs is given. */
+    const int64_t *const *sym_configs;
 } bs2e_job;
 
 /* Queues a job of the two-electron branch and admits what fits: a job is
@@ -446,7 +452,9 @@ int64_t bs2e_server_wait(bs2e_server *srv, int64_t ticket);
 int64_t bs2e_server_tensor(bs2e_server *srv, int64_t ticket, bs2e_rk **rk);
 /* Progresses the service and hands out the jobs that finished since the
  * previous call (at most cap): their tickets and their latency (ms from
- * submission to completion).  Returns the count. */
+ * submission to completion; -1 for a job that failed, whose reason
+ * bs2e_server_last_error gives and whose status is state 3).  Returns the
+ * count. */
 int64_t bs2e_server_finished(bs2e_server *srv, int64_t *tickets, double *latency_ms, int64_t cap);
 /* Returns a finished job's memory to the budget. */
 int64_t bs2e_server_retire(bs2e_server *srv, int64_t ticket);

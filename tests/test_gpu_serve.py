@@ -297,3 +297,91 @@ def test_service_whole_branch_equals_alone(ctx, precision):
                 assert np.array_equal(a.data, b.data)
         alone.release()
         rk.release()
+
+
+def _job_blocks(srv, tk, terms):
+    bset = srv.blocks(tk, terms)
+    return [bset.get(i) for i in range(len(terms))]
+
+
+def _same_blocks(a, b):
+    for (ha, sa), (hb, sb) in zip(a, b):
+        for x, y in ((ha, hb), (sa, sb)):
+            assert np.array_equal(x.index_ptr, y.index_ptr) and np.array_equal(x.indices, y.indices)
+            assert np.array_equal(x.data, y.data)
+
+
+def test_service_requeued_job_admitted_again(ctx):
+    """A job whose tensor and tables fit but whose blocks' outputs do not (the
+    first job holds them) goes back to the head of the queue once counted;
+    admitted again after the first job is retired, it rebuilds its set from
+    the host tables prepared at submission and gives the first job's H/S bit
+    for bit (both are the same request)."""
+    w, t = knots_of("c1")
+    n_b = len(t) - w.k - 2
+    terms = branch_terms(n_b, w.k, 2)
+    cap = (2, 0.8 * t[-1], 1e-3)
+
+    def submit(s):
+        return s.submit_job(w.k, t, w.k_GL, w.max_k, precision="fp64", terms=terms, full=False, Z=2, max_l_1p=2,
+                            CAP_c=cap)
+
+    probe = bs2e.Server(16 << 30, slots=1)
+    tk = submit(probe)
+    probe.wait(tk)
+    used = probe.memory()[1]               # tensor, tables, outputs and the one-electron cache entry
+    probe.retire(tk)
+    cache = probe.memory()[1]
+    probe.close()
+    fp = bs2e.footprint(w.k, t, w.k_GL, w.max_k)
+    rest = used - cache - fp               # the job's block tables and outputs
+    assert rest > 0
+    srv = bs2e.Server(used + fp + rest // 2, slots=2)
+    a = submit(srv)
+    srv.wait(a)
+    b = submit(srv)                        # admitted, counted, then refused its outputs: queued again
+    seen = [srv.status(b)[0]]
+    for _ in range(100000):
+        srv.progress()
+        st = srv.status(b)[0]
+        if st != seen[-1]:
+            seen.append(st)
+        if st == bs2e.Server.QUEUED:
+            break
+    assert seen == [bs2e.Server.RUNNING, bs2e.Server.QUEUED], seen
+    assert srv.memory()[1] == used         # the requeued job holds nothing
+    first = _job_blocks(srv, a, terms)
+    assert [x for x, _ in srv.finished()] == [a]
+    srv.retire(a)
+    srv.wait(b)
+    assert srv.status(b)[0] == bs2e.Server.DONE
+    _same_blocks(_job_blocks(srv, b, terms), first)
+    srv.retire(b)
+    srv.close()
+
+
+def test_service_hands_out_failed_job(ctx):
+    """A job whose configurations break count_configs order fails at
+    admission: finished() hands it out with latency -1, its status is
+    FAILED and the service's last error says why; a good job after it runs."""
+    w, t = knots_of("c1")
+    n_b = len(t) - w.k - 2
+    terms = branch_terms(n_b, w.k, 2)
+    bad = bs2e.sym(l=terms[0].l, pi=terms[0].pi)
+    bad.configs = terms[0].configs[::-1].copy()   # n1 descending within each channel
+    srv = bs2e.Server(16 << 30, slots=1)
+    tk = srv.submit_job(w.k, t, w.k_GL, w.max_k, terms=[bad], full=False, Z=2, max_l_1p=2)
+    done = []
+    for _ in range(20000):
+        done += srv.finished()
+        if done:
+            break
+    assert done == [(tk, -1.0)]
+    assert srv.status(tk)[0] == bs2e.Server.FAILED
+    assert "count_configs order" in srv.last_error()
+    good = srv.submit_job(w.k, t, w.k_GL, w.max_k, terms=terms, full=False, Z=2, max_l_1p=2)
+    srv.wait(good)
+    assert srv.status(good)[0] == bs2e.Server.DONE
+    srv.retire(tk)
+    srv.retire(good)
+    srv.close()
