@@ -1001,6 +1001,7 @@ __global__ __launch_bounds__(kCThreads, (SEP && NT == 1) || (!SEP && NT > 1 && N
 //   S          (D if l_eq) + (E \ D if l_eq_ex)
 __device__ __forceinline__ int run_len(int lo, int hi) { return hi >= lo ? hi - lo + 1 : 0; }
 
+constexpr int kCountBatch = 8;   // channels whose n2 ranges one count step loads together
 __global__ __launch_bounds__(kCThreads) void kc_count(const BlockArgs* __restrict__ tab, const int64_t* __restrict__ row0,
                                                       int n_blocks, int64_t n_rows)
 {
@@ -1015,9 +1016,13 @@ __global__ __launch_bounds__(kCThreads) void kc_count(const BlockArgs* __restric
         else bhi = mid - 1;
     }
     const int b = __builtin_amdgcn_readfirstlane(blo);
-    const BlockArgs A = tab[b];
+    // the descriptor, channel table and flags through the scalar cache (the
+    // host writes them before the launch)
+    typedef const __attribute__((address_space(4))) BlockChan CChan;
+    typedef const __attribute__((address_space(4))) int32_t CInt;
+    const CBlockArgs& A = ((const CBlockArgs*)tab)[b];
     const int i = (int)(g - row0[b]);
-    const int ks = A.ks, n_b = A.n_b;
+    const int ks = A.ks, n_b = A.n_b, n_chan = A.n_chan;
     const int32_t* ci = A.cfg + 2 * (size_t)i;
     const int cn = __builtin_amdgcn_readfirstlane(ci[0]), cl = __builtin_amdgcn_readfirstlane(ci[1]);
     const int n_a = cn & 0xffff, n_bb = cn >> 16, l_a = cl & 0xffff, l_b = cl >> 16;
@@ -1026,33 +1031,48 @@ __global__ __launch_bounds__(kCThreads) void kc_count(const BlockArgs* __restric
     const bool cand = lane < cr.n1 + cr.n2;
     const int n_c = lane < cr.n1 ? cr.r1lo + lane : cr.r2lo + (lane - cr.n1);
     const bool in_dir_c = cand && iabs_d(n_c - n_a) < ks, in_ex_c = cand && iabs_d(n_c - n_bb) < ks;
+    typedef int int4v __attribute__((ext_vector_type(4)));
+    const int4v* segc = (const int4v*)A.seg + (cand ? n_c : 1);   // BlockSeg {off, lo, hi, pad}; (a valid column
+                                                                   // without a candidate)
     int h = 0, s = 0, rh = 0, rs = 0;
-    for (int ch = 0; ch < A.n_chan; ++ch) {
-        const BlockChan cj = A.chan[ch];
-        const int fl = A.flags[ch_i * A.n_chan + ch];
-        const bool dir_any = fl & 1, ex_any = (fl >> 1) & 1;
-        const bool l_eq = (l_a == cj.l1) && (l_b == cj.l2);
-        const bool l_eq_ex = (l_a == cj.l2) && (l_b == cj.l1);
-        if (!dir_any && !ex_any && !l_eq && !l_eq_ex) continue;   // no entry can exist in this channel
-        if (!cand) continue;
-        const BlockSeg sg = A.seg[(size_t)ch * (n_b + 1) + n_c];
-        if (sg.hi < sg.lo) continue;
-        int lo = sg.lo;
-        if (!A.full) {   // column j = start + off + (n_d - lo2) >= i
-            const int t = i - cj.start - sg.off + sg.lo;
-            lo = t > lo ? t : lo;
+    // kCountBatch channels at a time, their n2 ranges loaded together (past
+    // the last channel: the last one again, not counted)
+    for (int ch0 = 0; ch0 < n_chan; ch0 += kCountBatch) {
+        int4v sgv[kCountBatch];
+#pragma unroll
+        for (int j = 0; j < kCountBatch; ++j) {
+            const int ch = ch0 + j < n_chan ? ch0 + j : n_chan - 1;
+            sgv[j] = gload(segc + (size_t)ch * (n_b + 1));
         }
-        const int dlo = in_dir_c ? (lo > n_bb - ks + 1 ? lo : n_bb - ks + 1) : 1;
-        const int dhi = in_dir_c ? (sg.hi < n_bb + ks - 1 ? sg.hi : n_bb + ks - 1) : 0;
-        const int elo = in_ex_c ? (lo > n_a - ks + 1 ? lo : n_a - ks + 1) : 1;
-        const int ehi = in_ex_c ? (sg.hi < n_a + ks - 1 ? sg.hi : n_a + ks - 1) : 0;
-        const int D = run_len(dlo, dhi), E = run_len(elo, ehi);
-        const int DE = run_len(dlo > elo ? dlo : elo, dhi < ehi ? dhi : ehi);
-        const bool hd = dir_any || l_eq || l_eq_ex, he = ex_any || l_eq || l_eq_ex;
-        h += (hd ? D : 0) + (he ? E : 0) - (hd && he ? DE : 0);
-        s += (l_eq ? D : 0) + (l_eq_ex ? E : 0) - (l_eq && l_eq_ex ? DE : 0);
-        rh += (dir_any ? D : 0) + (ex_any ? E - DE : 0);
-        rs += (l_eq ? D : 0) + (l_eq_ex ? E - DE : 0);
+#pragma unroll
+        for (int j = 0; j < kCountBatch; ++j) {
+            const int ch = ch0 + j;
+            if (ch >= n_chan) break;
+            const CChan& cj = ((CChan*)A.chan)[ch];
+            const int fl = ((CInt*)A.flags)[ch_i * n_chan + ch];
+            const bool dir_any = fl & 1, ex_any = (fl >> 1) & 1;
+            const bool l_eq = (l_a == cj.l1) && (l_b == cj.l2);
+            const bool l_eq_ex = (l_a == cj.l2) && (l_b == cj.l1);
+            if (!dir_any && !ex_any && !l_eq && !l_eq_ex) continue;   // no entry can exist in this channel
+            const int sg_off = sgv[j].x, sg_lo = sgv[j].y, sg_hi = sgv[j].z;
+            if (!cand || sg_hi < sg_lo) continue;
+            int lo = sg_lo;
+            if (!A.full) {   // column j = start + off + (n_d - lo2) >= i
+                const int t = i - cj.start - sg_off + sg_lo;
+                lo = t > lo ? t : lo;
+            }
+            const int dlo = in_dir_c ? (lo > n_bb - ks + 1 ? lo : n_bb - ks + 1) : 1;
+            const int dhi = in_dir_c ? (sg_hi < n_bb + ks - 1 ? sg_hi : n_bb + ks - 1) : 0;
+            const int elo = in_ex_c ? (lo > n_a - ks + 1 ? lo : n_a - ks + 1) : 1;
+            const int ehi = in_ex_c ? (sg_hi < n_a + ks - 1 ? sg_hi : n_a + ks - 1) : 0;
+            const int D = run_len(dlo, dhi), E = run_len(elo, ehi);
+            const int DE = run_len(dlo > elo ? dlo : elo, dhi < ehi ? dhi : ehi);
+            const bool hd = dir_any || l_eq || l_eq_ex, he = ex_any || l_eq || l_eq_ex;
+            h += (hd ? D : 0) + (he ? E : 0) - (hd && he ? DE : 0);
+            s += (l_eq ? D : 0) + (l_eq_ex ? E : 0) - (l_eq && l_eq_ex ? DE : 0);
+            rh += (dir_any ? D : 0) + (ex_any ? E - DE : 0);
+            rs += (l_eq ? D : 0) + (l_eq_ex ? E - DE : 0);
+        }
     }
     for (int off = kWave / 2; off > 0; off >>= 1) {
         h += __shfl_xor(h, off, kWave);
