@@ -59,6 +59,8 @@ static void ctx_free_now(bs2e_ctx* ctx)
             (void)hipStreamDestroy(s);
         }
     if (ctx->pin) (void)hipHostFree(ctx->pin);
+    if (ctx->up) (void)hipHostFree(ctx->up);
+    if (ctx->up_ev) (void)hipEventDestroy(ctx->up_ev);
     delete ctx;
 }
 
@@ -324,7 +326,31 @@ static int64_t alloc_tables(bs2e_ctx* ctx, bs2e_rk* rk, bool with_tensor)
         rk->R_elems = (size_t)p.n_pairs * p.n_pairs * p.K;
         HIP_TRY(ctx, ctx_alloc(ctx, &rk->R, rk->R_elems));
     }
-    HIP_TRY(ctx, hipMemcpyAsync(rk->small, h, L.total, hipMemcpyHostToDevice, ctx->stream));
+    // up through the context's pinned staging (a copy from pageable memory can
+    // hold the host until the stream reaches it), reused once the previous
+    // upload from it has run; it grows by doubling.  On a stream being
+    // captured the copy reads the tensor's own host image (a graph node keeps
+    // its source).
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIP_TRY(ctx, hipStreamIsCapturing(ctx->stream, &cap));
+    if (cap != hipStreamCaptureStatusNone) {
+        HIP_TRY(ctx, hipMemcpyAsync(rk->small, h, L.total, hipMemcpyHostToDevice, ctx->stream));
+        return BS2E_OK;
+    }
+    std::lock_guard<std::mutex> up_lock(ctx->up_mu);
+    if (ctx->up_ev) HIP_TRY(ctx, hipEventSynchronize(ctx->up_ev));
+    else HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->up_ev, hipEventDisableTiming));
+    if (ctx->up_bytes < L.total) {
+        const size_t grow = std::max(L.total, std::max<size_t>(2 * ctx->up_bytes, (size_t)1 << 20));
+        if (ctx->up) (void)hipHostFree(ctx->up);
+        ctx->up = nullptr;
+        ctx->up_bytes = 0;
+        HIP_TRY(ctx, hipHostMalloc(&ctx->up, grow, hipHostMallocDefault));
+        ctx->up_bytes = grow;
+    }
+    std::memcpy(ctx->up, h, L.total);
+    HIP_TRY(ctx, hipMemcpyAsync(rk->small, ctx->up, L.total, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->up_ev, ctx->stream));
     return BS2E_OK;
 }
 

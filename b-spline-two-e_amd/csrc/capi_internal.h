@@ -50,6 +50,12 @@ struct bs2e_ctx {
     void* pin = nullptr;
     size_t pin_bytes = 0;
     std::mutex pin_mu;
+    // pinned host staging for a tensor's small tables on their way up, reused
+    // once up_ev (recorded behind the last upload from it) has passed
+    void* up = nullptr;
+    size_t up_bytes = 0;
+    hipEvent_t up_ev = nullptr;
+    std::mutex up_mu;
 };
 
 struct bs2e_rk {

@@ -325,14 +325,16 @@ def run_stream(srv, jobs, arrivals, submit):
     return lat, time.perf_counter() - t0
 
 
-def serving_benchmark(n_jobs, seed=20261202, slots=4, budget_gib=160, load=0.8, precision="mixed", fixed_rate=60.0):
+def serving_benchmark(n_jobs, seed=20261202, slots=4, budget_gib=160, load=0.8, precision="mixed", fixed_rate=60.0,
+                      pause_s=0.0):
     """Continuous batching of a seeded stream of basis_setup requests through
     the build service: each job is the two-electron branch (R^k tensor in
     `precision`, one-electron matrices, the H/S blocks of L = 0..2); closed-
     loop capacity, then Poisson arrivals at `load` x capacity with p50/p99
     latency from arrival to the job's H/S CSR written, and the same arrivals
     rescaled to a fixed rate of `fixed_rate` jobs/s (comparable between
-    builds of different capacity)."""
+    builds of different capacity).  pause_s: an idle pause before each pass,
+    so that a kernel trace shows the passes apart."""
     import numpy as np
 
     import bs2e
@@ -354,11 +356,14 @@ def serving_benchmark(n_jobs, seed=20261202, slots=4, budget_gib=160, load=0.8, 
 
     srv = bs2e.Server(budget_gib << 30, slots=slots)
     run_stream(srv, jobs[:4], [0.0] * 4, submit)                    # warm-up
+    time.sleep(pause_s)
     _, wall_cap = run_stream(srv, jobs, [0.0] * n_jobs, submit)      # all at once: capacity
     cap = n_jobs / wall_cap
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / (load * cap), n_jobs))
+    time.sleep(pause_s)
     lat, wall = run_stream(srv, jobs, arrivals, submit)
+    time.sleep(pause_s)
     lat_fixed, _ = run_stream(srv, jobs, arrivals * (load * cap / fixed_rate), submit)
     cache = srv.cache_info()
     srv.close()
@@ -611,6 +616,8 @@ def main():
                     help="also time the c2 branch (tensor + L=0..2 blocks) eager against its captured graph (0 skips)")
     ap.add_argument("--serve-jobs", type=int, default=64,
                     help="jobs of the serving stream (differing bases through the build service); 0 skips it")
+    ap.add_argument("--serve-pause-ms", type=float, default=0.0,
+                    help="idle pause before each serving pass (kernel traces show the passes apart)")
     ap.add_argument("--far-shape", type=int, default=-1,
                     help="pin the rank-one fill shape (bs2e_rk_set_tuning) instead of autotuning (-1: autotune)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -773,7 +780,7 @@ def main():
         out["blockset_stream"] = blockset_stream_benchmark(ctx)
     if args.serve_jobs > 0:
         log("serving stream")
-        out["serving"] = serving_benchmark(args.serve_jobs)
+        out["serving"] = serving_benchmark(args.serve_jobs, pause_s=args.serve_pause_ms * 1e-3)
     if not args.no_cpu_baseline:
         log(f"{w.name}: host baseline on the reference's schedule ({nthreads} threads)")
         cb = cpu_baseline_reference_schedule(w, t, nthreads)
