@@ -413,13 +413,31 @@ static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* 
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_tab, (size_t)b->n_sym));
     HIP_TRY(ctx, ctx_alloc(ctx, &b->d_row0, (size_t)b->n_sym + 1));
     hipStream_t st = ctx->stream;
-    // The context's pinned staging carries the per-row tables up (a copy from
+    // The context's pinned staging carries the set's tables up (a copy from
     // pageable memory can hold the host until the stream reaches it) and the
     // counts back down; stream order keeps the two uses apart.  It grows by
     // doubling (pinning memory takes milliseconds).
+    b->tab.resize((size_t)b->n_sym);
+    for (int64_t s = 0; s < b->n_sym; ++s) b->tab[s] = set_args(b, s, false);
+    struct Up {
+        void* dst;
+        const void* src;
+        size_t bytes;
+    };
+    const Up ups[] = {{b->d_cfg, pk.data(), sizeof(int32_t) * 2 * N},
+                      {b->d_cfg_chan, cfg_chan.data(), sizeof(int32_t) * N},
+                      {b->d_rowlist, b->rowlist.data(), sizeof(int32_t) * N},
+                      {b->d_chan, b->chan.data(), sizeof(BlockChan) * b->chan.size()},
+                      {b->d_seg, b->seg.data(), sizeof(BlockSeg) * b->seg.size()},
+                      {b->d_coef, b->coef.data(), sizeof(double) * b->coef.size()},
+                      {b->d_flags, b->flags.data(), sizeof(int32_t) * b->flags.size()},
+                      {b->d_row0, b->row0.data(), sizeof(int64_t) * (b->n_sym + 1)},
+                      {b->d_tab, b->tab.data(), sizeof(BlockArgs) * b->n_sym}};
+    size_t up_bytes = 0;
+    for (const Up& u : ups) up_bytes += (u.bytes + 255) / 256 * 256;
     std::unique_lock<std::mutex> pin_lock(ctx->pin_mu, std::defer_lock);
     if (!async) pin_lock.lock();
-    const size_t need = sizeof(int64_t) * 4 * (size_t)N;   // >= the 16 N bytes of the uploads
+    const size_t need = std::max(sizeof(int64_t) * 4 * (size_t)N, up_bytes);
     if (ctx->pin_bytes < need) {
         const size_t grow = std::max(need, std::max<size_t>(2 * ctx->pin_bytes, (size_t)16 << 20));
         if (ctx->pin) (void)hipHostFree(ctx->pin);
@@ -428,27 +446,14 @@ static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* 
         HIP_TRY(ctx, hipHostMalloc(&ctx->pin, grow, hipHostMallocDefault));
         ctx->pin_bytes = grow;
     }
-    int32_t* stage = (int32_t*)ctx->pin;
-    std::memcpy(stage, pk.data(), sizeof(int32_t) * 2 * N);
-    std::memcpy(stage + 2 * N, cfg_chan.data(), sizeof(int32_t) * N);
-    std::memcpy(stage + 3 * N, b->rowlist.data(), sizeof(int32_t) * N);
-    HIP_TRY(ctx, hipMemcpyAsync(b->d_cfg, stage, sizeof(int32_t) * 2 * N, hipMemcpyHostToDevice, st));
-    HIP_TRY(ctx, hipMemcpyAsync(b->d_cfg_chan, stage + 2 * N, sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
-    HIP_TRY(ctx, hipMemcpyAsync(b->d_rowlist, stage + 3 * N, sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
-    HIP_TRY(ctx, hipMemcpyAsync(b->d_chan, b->chan.data(), sizeof(BlockChan) * b->chan.size(), hipMemcpyHostToDevice,
-                                st));
-    HIP_TRY(ctx, hipMemcpyAsync(b->d_seg, b->seg.data(), sizeof(BlockSeg) * b->seg.size(), hipMemcpyHostToDevice, st));
-    if (!b->coef.empty())
-        HIP_TRY(ctx, hipMemcpyAsync(b->d_coef, b->coef.data(), sizeof(double) * b->coef.size(), hipMemcpyHostToDevice,
-                                    st));
-    if (!b->flags.empty())
-        HIP_TRY(ctx, hipMemcpyAsync(b->d_flags, b->flags.data(), sizeof(int32_t) * b->flags.size(),
-                                    hipMemcpyHostToDevice, st));
-    HIP_TRY(ctx, hipMemcpyAsync(b->d_row0, b->row0.data(), sizeof(int64_t) * (b->n_sym + 1), hipMemcpyHostToDevice,
-                                st));
-    b->tab.resize((size_t)b->n_sym);
-    for (int64_t s = 0; s < b->n_sym; ++s) b->tab[s] = set_args(b, s, false);
-    HIP_TRY(ctx, hipMemcpyAsync(b->d_tab, b->tab.data(), sizeof(BlockArgs) * b->n_sym, hipMemcpyHostToDevice, st));
+    size_t off = 0;
+    for (const Up& u : ups) {
+        if (u.bytes == 0) continue;
+        char* stage = (char*)ctx->pin + off;
+        std::memcpy(stage, u.src, u.bytes);
+        HIP_TRY(ctx, hipMemcpyAsync(u.dst, stage, u.bytes, hipMemcpyHostToDevice, st));
+        off += (u.bytes + 255) / 256 * 256;
+    }
     clear_stale_error();
     // the count pass, the rows' CSR offsets by a prefix sum on the device, and
     // the four per-row counts back to the host through the context's pinned
@@ -462,7 +467,8 @@ static int64_t set_create(bs2e_blockset* b, const int64_t* configs, const char* 
     if (le == hipSuccess) le = launch_block_count(b->d_tab, b->d_row0, (int)b->n_sym, N, st);
     if (le == hipSuccess) le = hipEventRecord(b->count_ev[1], st);
     if (le == hipSuccess) le = launch_row_scan(b->d_cnt, b->d_ptr, N, b->d_count_tmp, st);
-    if (le == hipSuccess) le = hipMemcpyAsync(ctx->pin, b->d_cnt, need, hipMemcpyDeviceToHost, st);
+    if (le == hipSuccess)
+        le = hipMemcpyAsync(ctx->pin, b->d_cnt, sizeof(int64_t) * 4 * (size_t)N, hipMemcpyDeviceToHost, st);
     if (le == hipSuccess) le = hipEventRecord(b->count_ev[2], st);
     HIP_TRY(ctx, le);
     b->count_pending = true;
