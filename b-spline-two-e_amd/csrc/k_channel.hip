@@ -387,9 +387,16 @@ constexpr int kSepSteps = 4;   // 4-multipole steps unrolled together
 // (direct, exchange; zero past K); the current channel's candidate runs [64]
 // {H lo * 64 + H len, H offset | S offset << 16, S lo * 64 + S len, column
 // base} (16 bytes each)
+// The staged fill's tiling: NT matrix-core tiles per window side (2k - 1 <=
+// 16 NT), or kSepEdge for order 9 (2k - 1 = 17): one tile per side on the
+// matrix cores and the 17th row and column as dot products, one entry per
+// lane (two tiles per side would spend three quarters of their products
+// outside the window); its runs hold 17 doubles per multipole.
+constexpr int kSepEdge = 3;
+__host__ __device__ constexpr int sep_ld(int nt) { return nt == kSepEdge ? 17 : 16 * nt; }
 __host__ __device__ constexpr int sep_wave_lds(int K, int nt)
 {
-    return 2 * sep_kp(K) * 16 * nt + kSepChunk * 64 + kSepChunk * 2 * sep_kp(K) + 2 * 64;
+    return 2 * sep_kp(K) * sep_ld(nt) + kSepChunk * 64 + kSepChunk * 2 * sep_kp(K) + 2 * 64;
 }
 
 // X: the row's two windows may overlap (k <= |n_a - n_b| < 2k - 1), which
@@ -402,7 +409,9 @@ __device__ __forceinline__ void fill_row_staged(const CBlockArgs& A, int i, int 
     typedef double d4_t __attribute__((ext_vector_type(4)));
     typedef int int4v __attribute__((ext_vector_type(4)));
     typedef int int2v __attribute__((ext_vector_type(2)));
-    constexpr int LD = 16 * NT;   // doubles per multipole row of a run
+    constexpr bool EDGE = NT == kSepEdge;   // one tile per side plus the 17th row and column
+    constexpr int NM = EDGE ? 1 : NT;        // matrix-core tiles per side
+    constexpr int LD = sep_ld(NT);   // doubles per multipole row of a run
     const int ks = A.ks, K = A.K, N = A.n_b;
     const int KP = sep_kp(K);
     double* run1 = wl;                                                  // [KP][LD]
@@ -423,7 +432,7 @@ __device__ __forceinline__ void fill_row_staged(const CBlockArgs& A, int i, int 
         const double* src2 = (a_low ? A.PtotT : A.QtotT) + gload(A.c_first + n_a);
         const unsigned np = (unsigned)A.n_pairs;   // K * n_pairs < 2^31 (checked on the host)
         // consecutive pairs on consecutive lanes; four loads in flight per
-        // lane before their LDS writes (2 KP LD is a multiple of 64)
+        // lane before their LDS writes
         for (int e0 = 0; e0 < 2 * KP * LD; e0 += 4 * kWave) {
             double v[4];
 #pragma unroll
@@ -436,7 +445,7 @@ __device__ __forceinline__ void fill_row_staged(const CBlockArgs& A, int i, int 
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-                if (e0 + j * kWave < 2 * KP * LD) run1[e0 + j * kWave + lane] = v[j];   // run2 follows run1
+                if (e0 + j * kWave + lane < 2 * KP * LD) run1[e0 + j * kWave + lane] = v[j];   // run2 follows run1
         }
     }
     const int li = lane & 15, lk = lane >> 4;
@@ -551,44 +560,44 @@ __device__ __forceinline__ void fill_row_staged(const CBlockArgs& A, int i, int 
         // block's run 1 x cex and run 2, so one LDS read of each run serves
         // both; the third block (overlapping windows only) reads the runs at
         // shifted offsets
-        d4_t accd[NT][NT], acce[NT][NT], accx[NT][NT];
+        d4_t accd[NM][NM], acce[NM][NM], accx[NM][NM];
 #pragma unroll
-        for (int tm = 0; tm < NT; ++tm)
+        for (int tm = 0; tm < NM; ++tm)
 #pragma unroll
-            for (int tn = 0; tn < NT; ++tn) accd[tm][tn] = acce[tm][tn] = accx[tm][tn] = d4_t{0.0, 0.0, 0.0, 0.0};
+            for (int tn = 0; tn < NM; ++tn) accd[tm][tn] = acce[tm][tn] = accx[tm][tn] = d4_t{0.0, 0.0, 0.0, 0.0};
         if (dir_any || ex_any) {   // (a block without r12 is computed and not read)
             const double* cdir = scoef + slot * 2 * KP;
             const double* cex = cdir + KP;
             // one step: multipoles kk0 .. kk0 + 3, this lane's kk0 + lk
             auto step = [&](int kk) {
-                double v1[NT], v2[NT];
+                double v1[NM], v2[NM];
 #pragma unroll
-                for (int t = 0; t < NT; ++t) {
+                for (int t = 0; t < NM; ++t) {
                     v1[t] = run1[kk * LD + 16 * t + li];
                     v2[t] = run2[kk * LD + 16 * t + li];
                 }
                 const double cd = cdir[kk], ce = cex[kk];
 #pragma unroll
-                for (int tm = 0; tm < NT; ++tm)
+                for (int tm = 0; tm < NM; ++tm)
 #pragma unroll
-                    for (int tn = 0; tn < NT; ++tn) {
+                    for (int tn = 0; tn < NM; ++tn) {
                         accd[tm][tn] = __builtin_amdgcn_mfma_f64_16x16x4f64(v2[tm] * cd, v1[tn], accd[tm][tn], 0, 0, 0);
                         acce[tm][tn] = __builtin_amdgcn_mfma_f64_16x16x4f64(v1[tm] * ce, v2[tn], acce[tm][tn], 0, 0, 0);
                     }
             };
             auto step_x = [&](int kk) {   // E at direct coordinates: run 1 at n_c - d1lo, run 2 at n_d - d2lo
-                double x1[NT], x2[NT];
+                double x1[NM], x2[NM];
 #pragma unroll
-                for (int t = 0; t < NT; ++t) {
+                for (int t = 0; t < NM; ++t) {
                     const int a = 16 * t + li + shift, c = 16 * t + li - shift;
                     x1[t] = a >= 0 && a < LD ? run1[kk * LD + a] : 0.0;
                     x2[t] = c >= 0 && c < LD ? run2[kk * LD + c] : 0.0;
                 }
                 const double ce = cex[kk];
 #pragma unroll
-                for (int tm = 0; tm < NT; ++tm)
+                for (int tm = 0; tm < NM; ++tm)
 #pragma unroll
-                    for (int tn = 0; tn < NT; ++tn)
+                    for (int tn = 0; tn < NM; ++tn)
                         accx[tm][tn] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1[tm] * ce, x2[tn], accx[tm][tn], 0, 0, 0);
             };
             int kk0 = 0;
@@ -621,16 +630,16 @@ __device__ __forceinline__ void fill_row_staged(const CBlockArgs& A, int i, int 
             const int rows = dir ? len2 : len1, cols = dir ? len1 : len2;
             const int row_lo = dir ? d2lo : d1lo, col_lo = dir ? d1lo : d2lo;
             const int cbase = dir ? dir_base : ex_base;
-            int4v infos[NT][4];   // the runs of this lane's tile rows, read before any store
+            int4v infos[NM][4];   // the runs of this lane's tile rows, read before any store
 #pragma unroll
-            for (int tm = 0; tm < NT; ++tm)
+            for (int tm = 0; tm < NM; ++tm)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int m = 16 * tm + lk + 4 * r;
                     infos[tm][r] = sinfo[cbase + m < kWave ? cbase + m : kWave - 1];
                 }
 #pragma unroll
-            for (int tm = 0; tm < NT; ++tm)
+            for (int tm = 0; tm < NM; ++tm)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int m = 16 * tm + lk + 4 * r;
@@ -638,7 +647,7 @@ __device__ __forceinline__ void fill_row_staged(const CBlockArgs& A, int i, int 
                     const int h_lo = info.x >> 6, h_len = info.x & 63, s_lo = info.z >> 6;
                     const int n_c = row_lo + m;
 #pragma unroll
-                    for (int tn = 0; tn < NT; ++tn) {
+                    for (int tn = 0; tn < NM; ++tn) {
                         const int n_d = col_lo + 16 * tn + li;
                         const int o = n_d - h_lo;
                         // the other support of this entry
@@ -674,6 +683,62 @@ __device__ __forceinline__ void fill_row_staged(const CBlockArgs& A, int i, int 
                         }
                     }
                 }
+            if constexpr (EDGE) {
+                // the block's 17th row (m = 16, n = 0..16) and column (m =
+                // 0..15, n = 16), one entry per lane: the block's sum (and the
+                // direct block's exchange term at its coordinates) as fused
+                // multiply-adds over the multipoles in order, then the entry as
+                // in the tiles
+                const int m = lane < 17 ? 16 : lane - 17, nn = lane < 17 ? lane : 16;
+                if (lane < 33 && m < rows && nn < cols) {
+                    const int4v info = sinfo[cbase + m < kWave ? cbase + m : kWave - 1];
+                    const int h_lo = info.x >> 6, h_len = info.x & 63, s_lo = info.z >> 6;
+                    const int n_c = row_lo + m, n_d = col_lo + nn;
+                    const int o = n_d - h_lo;
+                    const bool other = dir ? iabs_d(n_c - n_bb) < ks && iabs_d(n_d - n_a) < ks
+                                           : iabs_d(n_c - n_a) < ks && iabs_d(n_d - n_bb) < ks;
+                    if (o >= 0 && o < h_len && (dir || !other)) {
+                        const bool sup = dir, sex = dir ? other : true;
+                        const bool r12 = (dir_any && sup) || (ex_any && sex);
+                        double res = 0.0;
+                        if (r12) {
+                            const double* cdir = scoef + slot * 2 * KP;
+                            const double* cex = cdir + KP;
+                            const bool third = X && dir && other && both;
+                            const int xa = m + shift, xc = nn - shift;
+                            double v = 0.0, vx = 0.0;
+                            for (int k = 0; k < KP; ++k) {   // (runs and coefficients zero past K)
+                                if (dir) v = fma(run2[k * LD + m] * cdir[k], run1[k * LD + nn], v);
+                                else v = fma(run1[k * LD + m] * cex[k], run2[k * LD + nn], v);
+                                if (third) {
+                                    const double x1 = xa >= 0 && xa < LD ? run1[k * LD + xa] : 0.0;
+                                    const double x2 = xc >= 0 && xc < LD ? run2[k * LD + xc] : 0.0;
+                                    vx = fma(x1 * cex[k], x2, vx);
+                                }
+                            }
+                            res = res + v;
+                            if (dir && other) res = res + vx;
+                        }
+                        Cplx h{res, 0.0};
+                        const int64_t col = info.w + n_d + 1;
+                        if ((sup && l_eq) || (sex && l_eq_ex)) {
+                            Cplx h1, s1;
+                            one_electron(A, n_a, n_bb, n_c, n_d, l_a, l_b, l_eq, l_eq_ex, odd, h1, s1);
+                            h = cadd(h, h1);
+                            const unsigned ps = (unsigned)((info.y >> 16) + (n_d - s_lo));
+                            if ((int)ps < slim) {
+                                gstore(ixs + ps, col);
+                                gstore(ds + ps, dbl2{s1.re, s1.im});
+                            }
+                        }
+                        const unsigned ph = (unsigned)((info.y & 0xffff) + o);
+                        if ((int)ph < hlim) {
+                            gstore(ixh + ph, col);
+                            gstore(dh + ph, dbl2{h.re, h.im});
+                        }
+                    }
+                }
+            }
         }
         nH += totH;
         nS += totS;
@@ -944,7 +1009,7 @@ __device__ __forceinline__ void fill_row_tiled(const CBlockArgs& A, int i, int n
 // tiles of at most 12 multipoles, while those of 18 hold 150 registers, three
 // wavefronts per SIMD: at 128 they spill and run 6 % longer)
 template <bool SEP, int NT, bool X = true>
-__global__ __launch_bounds__(kCThreads, (SEP && NT == 1) || (!SEP && NT > 1 && NT <= 12) ? 4 : (SEP && !X) || (!SEP && NT > 12) ? 3 : 1) void kc_block(const BlockArgs* __restrict__ tab,
+__global__ __launch_bounds__(kCThreads, (SEP && (NT == 1 || NT == kSepEdge)) || (!SEP && NT > 1 && NT <= 12) ? 4 : (SEP && !X) || (!SEP && NT > 12) ? 3 : 1) void kc_block(const BlockArgs* __restrict__ tab,
                                                                              const int64_t* __restrict__ row0,
                                                                              int n_blocks,
                                                                              const int32_t* __restrict__ rows,
@@ -1240,7 +1305,9 @@ hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_bl
                              int64_t n_far, const int32_t* rows_sep, int64_t n_sep, const int32_t* rows_gen,
                              int64_t n_gen, int ks, int K, hipStream_t s)
 {
-    const int nt = 2 * ks - 1 <= 16 ? 1 : 2;
+    // staged tiles per window side: one (k <= 8), one and the 17th row and
+    // column (k = 9), two (k >= 10)
+    const int nt = 2 * ks - 1 <= 16 ? 1 : 2 * ks - 1 == 17 ? kSepEdge : 2;
     hipError_t e = hipSuccess;
     if (n_gen > 0) {
         const unsigned grid = (unsigned)((n_gen * kWave + kCThreads - 1) / kCThreads);
@@ -1268,30 +1335,22 @@ hipError_t launch_block_fill(const BlockArgs* tab, const int64_t* row0, int n_bl
     }
     // the staged rows: those whose windows cannot overlap (|n1 - n2| >=
     // 2k - 1, rows_far) without the third block, then the others (rows_sep)
-    for (int part = 0; part < 2 && e == hipSuccess; ++part) {
-        const int64_t n = part == 0 ? n_far : n_sep;
-        if (n <= 0) continue;
-        const int32_t* rws = part == 0 ? rows_far : rows_sep;
+    auto staged = [&](auto kern, const int32_t* rws, int64_t n) {
         const unsigned grid = (unsigned)((n * kWave + kCThreads - 1) / kCThreads);
-        const void* fn = part == 0 ? (nt == 1 ? (const void*)kc_block<true, 1, false> : (const void*)kc_block<true, 2, false>)
-                                   : (nt == 1 ? (const void*)kc_block<true, 1, true> : (const void*)kc_block<true, 2, true>);
         const size_t lds = sizeof(double) * (size_t)(kCThreads / kWave) * (size_t)sep_wave_lds(K, nt);
-        e = allow_lds(fn, lds);   // <= 4 x 39 KiB (k <= 16, K <= 64)
-        if (e != hipSuccess) return e;
-        if (part == 0 && nt == 1)
-            hipLaunchKernelGGL((kc_block<true, 1, false>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks,
-                               rws, n, ks, K);
-        else if (part == 0)
-            hipLaunchKernelGGL((kc_block<true, 2, false>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks,
-                               rws, n, ks, K);
-        else if (nt == 1)
-            hipLaunchKernelGGL((kc_block<true, 1, true>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks,
-                               rws, n, ks, K);
-        else
-            hipLaunchKernelGGL((kc_block<true, 2, true>), dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks,
-                               rws, n, ks, K);
-        e = hipGetLastError();
-    }
+        hipError_t le = allow_lds((const void*)kern, lds);   // <= 4 x 39 KiB (k <= 16, K <= 64)
+        if (le != hipSuccess) return le;
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kCThreads), lds, s, tab, row0, n_blocks, rws, n, ks, K);
+        return hipGetLastError();
+    };
+    if (n_far > 0 && e == hipSuccess)
+        e = nt == 1 ? staged(kc_block<true, 1, false>, rows_far, n_far)
+            : nt == kSepEdge ? staged(kc_block<true, kSepEdge, false>, rows_far, n_far)
+                             : staged(kc_block<true, 2, false>, rows_far, n_far);
+    if (n_sep > 0 && e == hipSuccess)
+        e = nt == 1 ? staged(kc_block<true, 1, true>, rows_sep, n_sep)
+            : nt == kSepEdge ? staged(kc_block<true, kSepEdge, true>, rows_sep, n_sep)
+                             : staged(kc_block<true, 2, true>, rows_sep, n_sep);
     return e;
 }
 

@@ -119,8 +119,9 @@ def test_block_order_16_many_multipoles(ctx, oracle, L):
 @pytest.mark.parametrize("k,k_gl,max_k,L,full", [(9, 12, 2, 0, True), (12, 14, 3, 1, False), (7, 10, 2, 1, True)])
 def test_block_separated_rows(ctx, oracle, k, k_gl, max_k, L, full):
     """Rows whose electrons sit at least 2k - 1 splines apart take the
-    matrix-core fill: one 16 x 16 tile per window side for order 7, two for
-    orders 9 and 12 (2k - 1 > 16).  A basis long enough to hold such rows;
+    matrix-core fill: one 16 x 16 tile per window side for order 7, one and
+    the 17th row and column as dot products for order 9, two for order 12
+    (2k - 1 > 17).  A basis long enough to hold such rows;
     patterns bit-exact and values against the oracle, with the rows of both
     kinds (separated and not) present."""
     t = bs2e.generate_grid(k, 3, 2, 0.8, 40.0)
