@@ -1,6 +1,7 @@
 // host_setup.hip -- host-side set-up of one basis: knot grid, Gauss-Legendre
 // rule, banded pair table and the sparse_6d ordering.  This is O(n_b k^2)
 // bookkeeping done once per basis; every integral is computed on the GPU.
+#include <array>
 #include <cmath>
 #include <cstdlib>
 
@@ -176,14 +177,15 @@ void build_diag_entries(Plan& p)
 // ---------------------------------------------------------------------------
 static long double fact(int64_t n)
 {
-    static long double tab[401];
-    static bool init = false;
-    if (!init) {
-        tab[0] = 1.0L;
-        for (int i = 1; i <= 400; ++i) tab[i] = tab[i - 1] * (long double)i;
-        init = true;
-    }
-    return (n < 0 || n > 400) ? NAN : tab[n];
+    // built once by the first call (static initialization is thread-safe: the
+    // build service's worker threads compute angular coefficients at once)
+    static const std::array<long double, 401> tab = [] {
+        std::array<long double, 401> t{};
+        t[0] = 1.0L;
+        for (int i = 1; i <= 400; ++i) t[i] = t[i - 1] * (long double)i;
+        return t;
+    }();
+    return (n < 0 || n > 400) ? NAN : tab[(size_t)n];
 }
 
 static bool triangle(int64_t a, int64_t b, int64_t c)
