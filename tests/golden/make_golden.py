@@ -38,7 +38,7 @@ def main():
     # the h_max values recorded beside the benchmark configs
     for name, h in (("c1", 2.2476), ("c2", 1.2007), ("c3", 1.123), ("c5", 1.7786)):
         w = WORKLOADS[name]
-        fx[f"knots_baseline_hmax_{name}"] = O.ref_generate_grid(w.k, w.m, w.Z, h, w.r_max)
+        fx[f"knots_recorded_hmax_{name}"] = O.ref_generate_grid(w.k, w.m, w.Z, h, w.r_max)
     # B-spline values (BVALUE_D through eval_d / d_eval_d) at random points of
     # random intervals, for every local spline, value and 1st/2nd derivative.
     for name in ("tiny", "c1"):

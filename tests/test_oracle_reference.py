@@ -47,7 +47,7 @@ def test_count_configs_bit_exact(oracle, L):
     assert np.array_equal(oracle.count_configs(L, L % 2, 3, n_b, 7, n2, nall, 1), FX[f"cfg_c1_trunc_L{L}"])
 
 
-def test_c1_block_size_matches_survey(oracle):
+def test_c1_block_size_is_5311(oracle):
     assert FX["cfg_c1_L0"].shape[0] == 5311
 
 

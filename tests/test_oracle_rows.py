@@ -214,7 +214,7 @@ def test_c5p_perturbation_rule():
     assert not np.array_equal(perturb_breakpoints(t0, w.k, 43), t)
 
 
-def test_baseline_h_max_values_give_one_more_spline():
+def test_recorded_h_max_values_give_one_more_spline():
     """The h_max values recorded beside the configs (C1 2.2476, C2 1.2007,
     C3 1.123, C5 1.7786) give n_b + 1 B-splines under the reference's grid
     rule (the compiled reference agrees, ref_fixtures.npz); the workloads use
@@ -223,5 +223,5 @@ def test_baseline_h_max_values_give_one_more_spline():
     for name, h in (("c1", 2.2476), ("c2", 1.2007), ("c3", 1.123), ("c5", 1.7786)):
         w = WORKLOADS[name]
         t = bs2e.generate_grid(w.k, w.m, w.Z, h, w.r_max)
-        assert np.array_equal(t, fx[f"knots_baseline_hmax_{name}"])
+        assert np.array_equal(t, fx[f"knots_recorded_hmax_{name}"])
         assert bs2e.b_spline(w.k, t).n_b == w.n_b + 1
